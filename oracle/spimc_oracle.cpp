@@ -1,0 +1,1459 @@
+// spimc_oracle.cpp -- CPU restatement of simpimc's action-evaluation and
+// sampling hot path.
+//
+// TEST INFRASTRUCTURE.  This file is the checker, never the product: only the
+// tests, the smoke check and the CPU-baseline leg of the benchmark load it.
+// The product library must not include, link or call anything here.
+//
+// Every routine restates the reference routine cited beside it (paths relative
+// to the simpimc source tree) with the same loop order, minimum-image
+// convention, window/level conventions, Metropolis draw order and (single-
+// thread) summation order.  Departures, all result-preserving:
+//   * Per-bead rho_k caches (bead_class.h:20-21,125-133) are not stored; the OLD
+//     phase factor of a bead is recomputed by the same CalcC recurrence from the
+//     same position, which yields the same bits as the cached copy.
+//   * Splines: einspline's NATURAL cubic B-splines are replaced by the natural
+//     cubic spline through the same knots (the interpolant is unique), evaluated
+//     in second-derivative form.  Outside the knot range the boundary interval's
+//     cubic is extended (einspline's out-of-range behaviour is parity unpinned).
+//   * The scaffold RNG is unavailable; draws come either from a replay tape
+//     (post-transform values, reference call order) or from this file's own
+//     xoshiro256** stream.
+//   * One species, homologous pair action, max_level = 0 (the configurations
+//     this engine targets).
+#include "../include/spimc.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using cplx = std::complex<double>;
+
+thread_local std::string g_err;
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------------------
+// Natural cubic spline (stands in for einspline create_*_1d_d with
+// BCtype_d{NATURAL, NATURAL}; call sites free_spline_class.h:65-80,
+// bare_pair_action_class.h:47-67, david_pair_action_class.h:260-293).
+// ---------------------------------------------------------------------------
+struct NatSpline {
+    std::vector<double> x, y, m2;  // knots, values, second derivatives
+    bool uniform = false;
+    double x0 = 0., inv_dx = 0.;
+
+    void build(const double* xs, const double* ys, uint32_t n, bool is_uniform) {
+        x.assign(xs, xs + n);
+        y.assign(ys, ys + n);
+        m2.assign(n, 0.);
+        uniform = is_uniform;
+        x0 = xs[0];
+        inv_dx = (n > 1) ? (n - 1) / (xs[n - 1] - xs[0]) : 0.;
+        if (n < 3) return;
+        // Thomas algorithm on the interior second derivatives, m2[0]=m2[n-1]=0
+        std::vector<double> c(n, 0.), d(n, 0.);
+        for (uint32_t i = 1; i + 1 < n; ++i) {
+            double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+            double a = h0, b = 2. * (h0 + h1), cc = h1;
+            double rhs = 6. * ((y[i + 1] - y[i]) / h1 - (y[i] - y[i - 1]) / h0);
+            double denom = b - a * c[i - 1];
+            c[i] = cc / denom;
+            d[i] = (rhs - a * d[i - 1]) / denom;
+        }
+        for (uint32_t i = n - 2; i >= 1; --i) {
+            m2[i] = d[i] - c[i] * m2[i + 1];
+            if (i == 1) break;
+        }
+    }
+
+    // einspline general_grid_reverse_map: largest i with x[i] <= t (binary search);
+    // uniform grids index directly.  Clamped to the last interval.
+    uint32_t locate(double t) const {
+        const uint32_t n = (uint32_t)x.size();
+        long i;
+        if (uniform) {
+            i = (long)std::floor((t - x0) * inv_dx);
+        } else if (t <= x[0]) {
+            i = 0;
+        } else if (t >= x[n - 1]) {
+            i = n - 1;
+        } else {
+            long lo = 0, hi = n - 1;
+            while (hi - lo >= 2) {
+                long mid = (hi + lo) >> 1;
+                if (x[mid] > t)
+                    hi = mid;
+                else
+                    lo = mid;
+            }
+            i = lo;
+        }
+        if (i < 0) i = 0;
+        if (i > (long)n - 2) i = (long)n - 2;
+        return (uint32_t)i;
+    }
+
+    double eval(double t) const {
+        uint32_t i = locate(t);
+        double h = x[i + 1] - x[i];
+        double a = (x[i + 1] - t) / h, b = (t - x[i]) / h;
+        return a * y[i] + b * y[i + 1] + ((a * a * a - a) * m2[i] + (b * b * b - b) * m2[i + 1]) * (h * h) / 6.;
+    }
+
+    double deriv(double t) const {
+        uint32_t i = locate(t);
+        double h = x[i + 1] - x[i];
+        double a = (x[i + 1] - t) / h, b = (t - x[i]) / h;
+        return (y[i + 1] - y[i]) / h - (3. * a * a - 1.) / 6. * h * m2[i] + (3. * b * b - 1.) / 6. * h * m2[i + 1];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// FreeSpline (actions/free_spline_class.h:25-122)
+// ---------------------------------------------------------------------------
+struct FreeSpline {
+    double i4lt = 0., i4ltt = 0.;
+    NatSpline image, dimage;
+    bool has_dtau = false;
+
+    void build(double L, int n_images, double lambda, double tau, bool use_tau_derivative) {
+        // free_spline_class.h:27-63
+        i4lt = 1. / (4. * lambda * tau);
+        i4ltt = 1. / (4. * lambda * tau * tau);
+        double t_l = (L == 0.) ? 1000. : L;
+        const uint32_t num = 10000;
+        double start = -t_l / 2., end = t_l / 2.;
+        double dr = (end - start) / (num - 1);
+        std::vector<double> xs(num), ia(num, 0.), dia(num, 0.);
+        for (uint32_t i = 0; i < num; ++i) {
+            double r = start + i * dr;
+            xs[i] = r;
+            double r_2 = r * r;
+            double r_2_i4lt = r_2 * i4lt;
+            for (int image_i = 1; image_i <= n_images; ++image_i) {
+                double r_p = r + image_i * t_l;
+                double r_p_2_i4lt = r_p * r_p * i4lt;
+                double d_p = r_2_i4lt - r_p_2_i4lt;
+                double e_p = std::exp(d_p);
+                double r_m = r - image_i * t_l;
+                double r_m_2_i4lt = r_m * r_m * i4lt;
+                double d_m = r_2_i4lt - r_m_2_i4lt;
+                double e_m = std::exp(d_m);
+                ia[i] += e_p + e_m;
+                if (use_tau_derivative) dia[i] += (d_p * e_p + d_m * e_m) / tau;
+            }
+            if (use_tau_derivative) dia[i] = dia[i] / (1. + ia[i]);
+            ia[i] = -std::log1p(ia[i]);
+        }
+        image.build(xs.data(), ia.data(), num, true);
+        has_dtau = use_tau_derivative;
+        if (use_tau_derivative) dimage.build(xs.data(), dia.data(), num, true);
+    }
+
+    // GetLogRhoFree (free_spline_class.h:76-84)
+    double log_rho_free(const double* r) const {
+        double tot = 0.;
+        for (int d = 0; d < 3; ++d) tot -= image.eval(r[d]);
+        return tot - (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) * i4lt;
+    }
+    // GetDLogRhoFreeDTau (free_spline_class.h:92-100)
+    double dlog_rho_free_dtau(const double* r) const {
+        double tot = 0.;
+        for (int d = 0; d < 3; ++d) tot -= dimage.eval(r[d]);
+        return tot - (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) * i4ltt;
+    }
+    // gradient of -log rho_free wrt r (primitive part + image spline slope)
+    void grad_neg_log(const double* r, double* g) const {
+        for (int d = 0; d < 3; ++d) g[d] = image.deriv(r[d]) + 2. * r[d] * i4lt;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// xoshiro256** for free-running chains (the reference's scaffold RNG is not
+// available; transforms below are this engine's own definitions).
+// ---------------------------------------------------------------------------
+struct Xoshiro {
+    uint64_t s[4];
+    static uint64_t splitmix(uint64_t& z) {
+        z += 0x9e3779b97f4a7c15ull;
+        uint64_t x = z;
+        x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+        x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+        return x ^ (x >> 31);
+    }
+    void seed(uint64_t v) {
+        for (auto& w : s) w = splitmix(v);
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {
+        uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double unif() { return ((next() >> 11) + 1) * (1.0 / 9007199254740992.0); }  // (0,1]
+    uint32_t index(uint32_t n) { return (uint32_t)(((next() >> 32) * (uint64_t)n) >> 32); }
+    double normal() {
+        double u1 = unif(), u2 = unif();
+        return std::sqrt(-2. * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// One replica: Path + Species + Kinetic + PairAction + Bisect + DisplaceParticle
+// + Energy / StructureFactor / PairCorrelation.
+// ---------------------------------------------------------------------------
+struct Oracle {
+    // Path (path_class.h:25-69)
+    uint32_t N = 0, M = 0;
+    double L = 0., iL = 0., vol = 1., beta = 0., tau = 0., lambda = 0.;
+    int mode = SPIMC_MODE_NEW;
+    std::vector<double> r, rc;  // [M][N][3] NEW and OLD positions (Bead::r_, r_c_)
+
+    // KSpace (k_space_class.h:33-94)
+    uint32_t nk = 0;
+    std::vector<int> kn;  // [nk][3]
+    std::vector<double> kmag;
+    int max_index = 0;
+    double box = 0.;
+    std::vector<cplx> rho, rhoc;  // [M][nk] Species::rho_k_, rho_k_c_
+    bool need_update_rho_k = true;
+
+    // Kinetic (kinetic_class.h:16-32)
+    int kin_n_images = 0;
+    double kin_i4lt = 0., kin_dbeta_const = 0.;
+    std::vector<FreeSpline> kin_splines;
+
+    // PairAction (pair_action_class.h:205-238)
+    int pair_type = SPIMC_PAIR_NONE;
+    bool use_long_range = false;
+    uint32_t max_level = 0;
+    // Bare
+    bool is_coulomb = false;
+    NatSpline v_r, v_long_r;
+    double r_v_min = 0., r_v_max = 0., r_v_long_min = 0., r_v_long_max = 0.;
+    // David
+    uint32_t n_order = 0, n_val = 0;
+    std::vector<NatSpline> u_kj, du_kj;  // n_val+1 components each
+    double g_start = 0., g_end = 0.;
+    // long range
+    std::vector<double> v_long_k, u_long_k, du_long_k;
+    double v_long_k_0 = 0., v_long_r_0 = 0., du_long_k_0 = 0., du_long_r_0 = 0.;
+
+    // Bisect (bisect_class.h:167-190) and DisplaceParticle (displace_particle_class.h:93)
+    uint32_t n_level = 0, n_bisect_beads = 0;
+    int bis_n_images = 0;
+    std::vector<FreeSpline> bis_splines;
+    double step_size = 0.;
+    uint64_t counters[4] = {0, 0, 0, 0};
+    uint64_t bead_updates = 0;
+
+    // draws
+    Xoshiro rng;
+    std::vector<double> tape;
+    uint64_t cursor = 0;
+    bool use_tape = false;
+
+    // proposal bookkeeping for the interface entry points
+    bool have_proposal = false;
+    uint32_t prop_p = 0, prop_first = 0, prop_n = 0;
+
+    // observables
+    int energy_estimator = SPIMC_ENERGY_THERMAL;
+    uint32_t virial_window = 0;
+    double sk_k_cut = 0.;
+    double gr_r_min = 0., gr_r_max = 0., gr_dr = 0., gr_d_ir = 0.;
+    uint32_t gr_n_r = 100;
+    double e_acc[2] = {0., 0.};
+    std::vector<double> sk_acc, gr_acc;
+    uint64_t n_measure[3] = {0, 0, 0};
+
+    // ---- helpers ------------------------------------------------------------
+    uint32_t bead_loop(uint32_t b) const { return b >= M ? b - M : b; }  // species_class.h:101
+    double* R(uint32_t p, uint32_t b) { return (mode ? r.data() : rc.data()) + 3 * ((size_t)bead_loop(b) * N + p); }
+    double* Rm(int m, uint32_t p, uint32_t b) { return (m ? r.data() : rc.data()) + 3 * ((size_t)bead_loop(b) * N + p); }
+    std::vector<cplx>& rho_mode() { return mode ? rho : rhoc; }  // species_class.h:428
+
+    // Path::PutInBox (path_class.h:128-131)
+    void put_in_box(double* v) const {
+        for (int d = 0; d < 3; ++d) v[d] -= std::nearbyint(v[d] * iL) * L;
+    }
+    // Path::Dr (path_class.h:108-112): r0 - r1 put in the box
+    void dr(const double* r0, const double* r1, double* out) const {
+        for (int d = 0; d < 3; ++d) out[d] = r0[d] - r1[d];
+        put_in_box(out);
+    }
+    static double mag(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+    // Path::DrDrpDrrp (path_class.h:137-150)
+    void dr_drp_drrp(uint32_t b_i, uint32_t b_j, uint32_t p0, uint32_t p1, double& r_mag, double& rp_mag,
+                     double& rrp_mag) {
+        double rv[3], rpv[3], rrp[3];
+        const double *a0 = R(p0, b_i), *a1 = R(p1, b_i), *c0 = R(p0, b_j), *c1 = R(p1, b_j);
+        for (int d = 0; d < 3; ++d) {
+            rv[d] = a1[d] - a0[d];
+            rpv[d] = c1[d] - c0[d];
+        }
+        for (int d = 0; d < 3; ++d) {
+            rv[d] -= std::nearbyint(rv[d] * iL) * L;
+            rpv[d] += std::nearbyint((rv[d] - rpv[d]) * iL) * L;
+        }
+        for (int d = 0; d < 3; ++d) rrp[d] = rv[d] - rpv[d];
+        for (int d = 0; d < 3; ++d) rrp[d] -= std::nearbyint(rrp[d] * iL) * L;
+        r_mag = mag(rv);
+        rp_mag = mag(rpv);
+        rrp_mag = mag(rrp);
+    }
+
+    // KSpace::CalcC + Bead::CalcRhoK (k_space_class.h:83-94, bead_class.h:125-133)
+    void calc_c(const double* pos, std::vector<cplx>& c) const {
+        const int w = 2 * max_index + 1;
+        c.assign(3 * w, cplx(0., 0.));
+        for (int d = 0; d < 3; ++d) {
+            double phi = pos[d] * box;
+            cplx t(std::cos(phi), std::sin(phi));
+            cplx* cd = &c[d * w];
+            cd[max_index] = 1.;
+            for (int k = 1; k <= max_index; ++k) {
+                cd[max_index + k] = t * cd[max_index + k - 1];
+                cd[max_index - k] = std::conj(cd[max_index + k]);
+            }
+        }
+    }
+    void calc_rho_k(const double* pos, std::vector<cplx>& out) const {
+        std::vector<cplx> c;
+        calc_c(pos, c);
+        const int w = 2 * max_index + 1;
+        out.resize(nk);
+        for (uint32_t k = 0; k < nk; ++k) {
+            cplx f = 1.;
+            for (int d = 0; d < 3; ++d) f *= c[d * w + max_index + kn[3 * k + d]];
+            out[k] = f;
+        }
+    }
+
+    // Species::InitRhoK (species_class.h:380-403), from the NEW positions
+    void init_rho_k() {
+        rho.assign((size_t)M * nk, cplx(0., 0.));
+        std::vector<cplx> bk;
+        for (uint32_t b = 0; b < M; ++b)
+            for (uint32_t p = 0; p < N; ++p) {
+                calc_rho_k(Rm(1, p, b), bk);
+                for (uint32_t k = 0; k < nk; ++k) rho[(size_t)b * nk + k] += bk[k];
+            }
+        rhoc = rho;
+    }
+
+    // Species::UpdateRhoK (species_class.h:406-425)
+    void update_rho_k(uint32_t b0, uint32_t b1, const std::vector<uint32_t>& parts, uint32_t level) {
+        uint32_t skip = 1u << level;
+        for (uint32_t b = b0; b < b1; b += skip)
+            for (uint32_t k = 0; k < nk; ++k) rho[(size_t)bead_loop(b) * nk + k] = rhoc[(size_t)bead_loop(b) * nk + k];
+        std::vector<cplx> nw, od;
+        for (uint32_t p : parts)
+            for (uint32_t b = b0; b < b1; b += skip) {
+                calc_rho_k(Rm(1, p, b), nw);
+                calc_rho_k(Rm(0, p, b), od);  // the bead's cached OLD copy
+                cplx* rb = &rho[(size_t)bead_loop(b) * nk];
+                for (uint32_t k = 0; k < nk; ++k) rb[k] += nw[k] - od[k];
+            }
+        need_update_rho_k = false;
+    }
+
+    static double cmag2(const cplx& a, const cplx& b) { return a.real() * b.real() + a.imag() * b.imag(); }
+
+    // ---- pair action --------------------------------------------------------
+    static void set_limits(double r_min, double r_max, double& r0, double& r1) {  // pair_action_class.h:32-42
+        if (r0 > r_max)
+            r0 = r_max;
+        else if (r0 < r_min)
+            r0 = r_min;
+        if (r1 > r_max)
+            r1 = r_max;
+        else if (r1 < r_min)
+            r1 = r_min;
+    }
+
+    // BarePairAction::CalcV (bare_pair_action_class.h:99-123)
+    double bare_calc_v(double r0, double r1) const {
+        set_limits(r_v_min, r_v_max, r0, r1);
+        double v = 0.;
+        if (is_coulomb) {
+            v += (0.5 / r0) + (0.5 / r1);
+        } else {
+            v += 0.5 * v_r.eval(r0);
+            v += 0.5 * v_r.eval(r1);
+        }
+        if (use_long_range) {
+            set_limits(r_v_long_min, r_v_long_max, r0, r1);
+            v -= 0.5 * v_long_r.eval(r0);
+            v -= 0.5 * v_long_r.eval(r1);
+        }
+        return v;
+    }
+
+    // DavidPairAction::CalcV (david_pair_action_class.h:42-57)
+    double david_calc_v(double r0, double r1) const {
+        set_limits(g_start, g_end, r0, r1);
+        return 0.5 * (u_kj[0].eval(r0) + u_kj[0].eval(r1));
+    }
+
+    // DavidPairAction::CalcU / CalcdUdBeta (david_pair_action_class.h:63-102,126-168)
+    double david_calc_u(double r0, double r1, double s, const std::vector<NatSpline>& tab, bool add_v) const {
+        double q = 0.5 * (r0 + r1);
+        double z = r0 - r1;
+        double r_min = g_start, r_max = g_end;
+        set_limits(r_min, r_max, r0, r1);
+        double u = 0.5 * (tab[1].eval(r0) + tab[1].eval(r1));
+        if (add_v) u += 0.5 * (tab[0].eval(r0) + tab[0].eval(r1));
+        if (s > 0.0 && q < r_max) {
+            double z_2 = z * z, s_2 = s * s, i_s_2 = 1. / s_2, s_2_k = s_2;
+            for (uint32_t k = 1; k <= n_order; ++k) {
+                double z_2_j = 1., cur_s = s_2_k;
+                for (uint32_t j = 0; j <= k; ++j) {
+                    double cof = tab[k * (k + 1) / 2 + (j + 1)].eval(q);
+                    u += cof * z_2_j * cur_s;
+                    z_2_j *= z_2;
+                    cur_s *= i_s_2;
+                }
+                s_2_k *= s_2;
+            }
+        }
+        return u;
+    }
+
+    // CalcU of the selected family at `level` (max_level = 0, so level = 0 here)
+    double calc_u(double r0, double r1, double s, uint32_t level) const {
+        if (pair_type == SPIMC_PAIR_BARE) {
+            uint32_t skip = 1u >> level;  // bare_pair_action_class.h:147 (sic)
+            double level_tau = skip * tau;
+            return level_tau * bare_calc_v(r0, r1);
+        }
+        return david_calc_u(r0, r1, s, u_kj, false);
+    }
+    double calc_du_dbeta(double r0, double r1, double s) const {
+        if (pair_type == SPIMC_PAIR_BARE) return bare_calc_v(r0, r1);  // bare_pair_action_class.h:175-177
+        // david_pair_action_class.h:126-168: du = 0.5(du_kj[1](r)+du_kj[1](r')) + V + off-diagonal
+        double q = 0.5 * (r0 + r1), z = r0 - r1;
+        double r_max = g_end;
+        double a = r0, b = r1;
+        set_limits(g_start, g_end, a, b);
+        double v = 0.5 * (du_kj[0].eval(a) + du_kj[0].eval(b));
+        double du = 0.5 * (du_kj[1].eval(a) + du_kj[1].eval(b));
+        du += v;
+        if (s > 0.0 && q < r_max) {
+            double z_2 = z * z, s_2 = s * s, i_s_2 = 1. / s_2, s_2_k = s_2;
+            for (uint32_t k = 1; k <= n_order; ++k) {
+                double z_2_j = 1., cur_s = s_2_k;
+                for (uint32_t j = 0; j <= k; ++j) {
+                    double cof = du_kj[k * (k + 1) / 2 + j + 1].eval(q);
+                    du += cof * z_2_j * cur_s;
+                    z_2_j *= z_2;
+                    cur_s *= i_s_2;
+                }
+                s_2_k *= s_2;
+            }
+        }
+        return du;
+    }
+    double calc_v(double r0, double r1) const {
+        return pair_type == SPIMC_PAIR_BARE ? bare_calc_v(r0, r1) : david_calc_v(r0, r1);
+    }
+
+    // CalcULong (bare_pair_action_class.h:153-172, david_pair_action_class.h:105-123)
+    double calc_u_long(uint32_t b0, uint32_t b1, uint32_t level) {
+        std::vector<cplx>& rk = rho_mode();
+        uint32_t skip = 1u << level;
+        double tot = 0.;
+        const std::vector<double>& coef = (pair_type == SPIMC_PAIR_BARE) ? v_long_k : u_long_k;
+        const uint32_t nb = (b1 > b0) ? (b1 - b0 + skip - 1) / skip : 0;
+        const long nkl = nk;
+#pragma omp parallel for collapse(2) reduction(+ : tot)
+        for (long k = 0; k < nkl; ++k)
+            for (uint32_t t = 0; t < nb; ++t) {
+                uint32_t b = bead_loop(b0 + t * skip);
+                const cplx& a = rk[(size_t)b * nk + k];
+                tot += coef[k] * cmag2(a, a);
+            }
+        if (pair_type == SPIMC_PAIR_BARE) return skip * tau * tot;
+        return tot;
+    }
+    // CalcVLong / CalcdUdBetaLong (bare_pair_action_class.h:126-143,180-182;
+    // david_pair_action_class.h:42-60,171-188)
+    double calc_whole_long(bool dbeta) {
+        std::vector<cplx>& rk = rho_mode();
+        const std::vector<double>& coef = (pair_type == SPIMC_PAIR_DAVID && dbeta) ? du_long_k : v_long_k;
+        double tot = 0.;
+        const long nkl = nk;
+#pragma omp parallel for collapse(2) reduction(+ : tot)
+        for (long k = 0; k < nkl; ++k)
+            for (uint32_t b = 0; b < M; ++b) {
+                const cplx& a = rk[(size_t)b * nk + k];
+                tot += coef[k] * cmag2(a, a);
+            }
+        if (pair_type == SPIMC_PAIR_DAVID && dbeta) return tot + du_long_k_0 + du_long_r_0;
+        return tot + v_long_k_0 + v_long_r_0;
+    }
+
+    // PairAction::GetAction (pair_action_class.h:267-302), homologous, one species
+    double pair_get_action(uint32_t b0, uint32_t b1, const std::vector<uint32_t>& parts, uint32_t level) {
+        if (level > max_level || pair_type == SPIMC_PAIR_NONE) return 0.;
+        if (N == 1 || lambda == 0.) return 0.;  // is_constant (pair_action_class.h:225)
+        // GenerateParticlePairs (pair_action_class.h:62-114)
+        std::vector<std::pair<uint32_t, uint32_t>> pairs;
+        std::vector<uint32_t> others;
+        for (uint32_t p = 0; p < N; ++p)
+            if (std::find(parts.begin(), parts.end(), p) == parts.end()) others.push_back(p);
+        for (uint32_t p : parts)
+            for (uint32_t q : others) pairs.emplace_back(p, q);
+        for (size_t i = 0; i + 1 < parts.size(); ++i)
+            for (size_t j = i + 1; j < parts.size(); ++j) pairs.emplace_back(parts[i], parts[j]);
+        if (pairs.empty()) return 0.;
+        uint32_t skip = 1u << level;
+        double tot = 0.;
+        for (uint32_t b_i = b0; b_i < b1; b_i += skip) {
+            uint32_t b_j = b_i + skip;
+            for (auto& pq : pairs) {
+                double rm, rpm, rrpm;
+                dr_drp_drrp(b_i, b_j, pq.first, pq.second, rm, rpm, rrpm);
+                tot += calc_u(rm, rpm, rrpm, level);
+            }
+        }
+        if (use_long_range) {
+            if (need_update_rho_k && mode == SPIMC_MODE_NEW) update_rho_k(b0, b1, parts, level);
+            tot += calc_u_long(b0, b1, level);
+        }
+        return tot;
+    }
+
+    std::vector<std::pair<uint32_t, uint32_t>> all_pairs() const {  // pair_action_class.h:45-59
+        std::vector<std::pair<uint32_t, uint32_t>> pairs;
+        for (uint32_t p = 0; p + 1 < N; ++p)
+            for (uint32_t q = p + 1; q < N; ++q) pairs.emplace_back(p, q);
+        return pairs;
+    }
+
+    // PairAction::DActionDBeta (pair_action_class.h:241-264)
+    double pair_d_action_d_beta() {
+        if (pair_type == SPIMC_PAIR_NONE) return 0.;
+        auto pairs = all_pairs();
+        double tot = 0.;
+        const long np = (long)pairs.size();
+#pragma omp parallel for collapse(2) reduction(+ : tot)
+        for (long i = 0; i < np; ++i)
+            for (uint32_t b = 0; b < M; ++b) {
+                double rm, rpm, rrpm;
+                dr_drp_drrp(b, b + 1, pairs[i].first, pairs[i].second, rm, rpm, rrpm);
+                tot += calc_du_dbeta(rm, rpm, rrpm);
+            }
+        if (use_long_range) tot += calc_whole_long(true);
+        return tot;
+    }
+
+    // PairAction::Potential (pair_action_class.h:369-395)
+    double pair_potential() {
+        if (pair_type == SPIMC_PAIR_NONE) return 0.;
+        auto pairs = all_pairs();
+        double tot = 0.;
+        const long np = (long)pairs.size();
+#pragma omp parallel for collapse(2) reduction(+ : tot)
+        for (long i = 0; i < np; ++i)
+            for (uint32_t b = 0; b < M; ++b) {
+                double d0[3], d1[3];
+                dr(R(pairs[i].first, b), R(pairs[i].second, b), d0);
+                dr(R(pairs[i].first, b + 1), R(pairs[i].second, b + 1), d1);
+                tot += calc_v(mag(d0), mag(d1));
+            }
+        if (use_long_range) tot += calc_whole_long(false);
+        return tot;
+    }
+
+    // PairAction::CalcGradientU (pair_action_class.h:135-155)
+    void calc_gradient_u(uint32_t b_i, uint32_t b_j, uint32_t p_i, uint32_t p_j, double* tot) {
+        double* b = R(p_i, b_i);
+        double r0[3] = {b[0], b[1], b[2]};
+        const double eps = 1.e-4;
+        for (int d = 0; d < 3; ++d) {
+            double rm, rpm, rrpm;
+            b[d] = r0[d] + eps;
+            dr_drp_drrp(b_i, b_j, p_i, p_j, rm, rpm, rrpm);
+            double f1 = calc_u(rm, rpm, rrpm, 0);
+            b[d] = r0[d] - eps;
+            dr_drp_drrp(b_i, b_j, p_i, p_j, rm, rpm, rrpm);
+            double f2 = calc_u(rm, rpm, rrpm, 0);
+            tot[d] = (f1 - f2) / (2. * eps);
+            b[d] = r0[d];
+        }
+    }
+
+    // PairAction::GetActionGradient(b, b+1, {p}, 0) (pair_action_class.h:305-337)
+    void pair_gradient_window(uint32_t b0, uint32_t b1, uint32_t p, double* out) {
+        out[0] = out[1] = out[2] = 0.;
+        if (pair_type == SPIMC_PAIR_NONE || N == 1) return;
+        for (uint32_t b_i = b0; b_i < b1; ++b_i) {
+            uint32_t b_j = b_i + 1;
+            uint32_t b_k = b_i - 1 + M;
+            for (uint32_t q = 0; q < N; ++q) {
+                if (q == p) continue;
+                double g[3];
+                calc_gradient_u(b_i, b_j, p, q, g);
+                for (int d = 0; d < 3; ++d) out[d] += g[d];
+                calc_gradient_u(b_i, b_k, p, q, g);
+                for (int d = 0; d < 3; ++d) out[d] += g[d];
+            }
+        }
+        // long range: CalcGradientULong of the base class is zero (pair_action_class.h:158-165)
+    }
+
+    // ---- kinetic -----------------------------------------------------------
+    // Kinetic::GetAction (kinetic_class.h:105-122)
+    double kinetic_get_action(uint32_t b0, uint32_t b1, const std::vector<uint32_t>& parts, uint32_t level) {
+        uint32_t skip = 1u << level;
+        double tot = 0.;
+        for (uint32_t p : parts) {
+            uint32_t a = bead_loop(b0);
+            uint32_t end = (b0 + (b1 - b0)) % M;
+            uint32_t guard = 0;
+            while (a != end && guard++ <= M) {
+                uint32_t nx = (a + skip) % M;
+                double d[3];
+                dr(R(p, a), R(p, nx), d);
+                tot -= kin_splines[skip - 1].log_rho_free(d);
+                a = nx;
+            }
+        }
+        return tot;
+    }
+    // Kinetic::DActionDBeta (kinetic_class.h:35-45)
+    double kinetic_d_action_d_beta() {
+        double tot = kin_dbeta_const;
+#pragma omp parallel for collapse(2) reduction(+ : tot)
+        for (uint32_t b = 0; b < M; ++b)
+            for (uint32_t p = 0; p < N; ++p) {
+                double d[3];
+                dr(R(p, b), R(p, b + 1), d);
+                tot += kin_splines[0].dlog_rho_free_dtau(d);
+            }
+        return tot;
+    }
+    // Kinetic::VirialEnergy (kinetic_class.h:48-102)
+    double kinetic_virial_energy(uint32_t W) {
+        double tot = kin_dbeta_const / W;
+        for (uint32_t p = 0; p < N; ++p) {
+            uint32_t b_0 = 0, b_1 = 1 % M, b_2 = 2 % M;
+            double dr_i_1[3], dr_i_L[3], t[3];
+            dr(R(p, b_1), R(p, b_0), dr_i_1);
+            for (int d = 0; d < 3; ++d) dr_i_L[d] = dr_i_1[d];
+            for (uint32_t skip = 1; skip < W; ++skip) {
+                dr(R(p, b_2), R(p, b_1), t);
+                for (int d = 0; d < 3; ++d) dr_i_L[d] += t[d];
+                b_1 = b_2;
+                b_2 = (b_2 + 1) % M;
+            }
+            tot -= (dr_i_L[0] * dr_i_1[0] + dr_i_L[1] * dr_i_1[1] + dr_i_L[2] * dr_i_1[2]) * kin_i4lt / (W * tau);
+            for (uint32_t b_i = 1; b_i < M; ++b_i) {
+                for (int d = 0; d < 3; ++d) dr_i_L[d] -= dr_i_1[d];
+                dr(R(p, b_2), R(p, b_1), t);
+                for (int d = 0; d < 3; ++d) dr_i_L[d] += t[d];
+                b_0 = (b_0 + 1) % M;
+                dr(R(p, (b_0 + 1) % M), R(p, b_0), dr_i_1);
+                b_1 = b_2;
+                b_2 = (b_2 + 1) % M;
+                tot -= (dr_i_L[0] * dr_i_1[0] + dr_i_L[1] * dr_i_1[1] + dr_i_L[2] * dr_i_1[2]) * kin_i4lt / (W * tau);
+            }
+        }
+        return tot;
+    }
+
+    // ---- whole-path action totals, for finite-difference gradient checks ----
+    double total_kinetic() {
+        double tot = 0.;
+        for (uint32_t b = 0; b < M; ++b)
+            for (uint32_t p = 0; p < N; ++p) {
+                double d[3];
+                dr(R(p, b), R(p, b + 1), d);
+                tot -= kin_splines[0].log_rho_free(d);
+            }
+        return tot;
+    }
+    double total_pair_short() {
+        double tot = 0.;
+        if (pair_type == SPIMC_PAIR_NONE) return 0.;
+        for (auto& pq : all_pairs())
+            for (uint32_t b = 0; b < M; ++b) {
+                double rm, rpm, rrpm;
+                dr_drp_drrp(b, b + 1, pq.first, pq.second, rm, rpm, rrpm);
+                tot += calc_u(rm, rpm, rrpm, 0);
+            }
+        return tot;
+    }
+    double slice_long(uint32_t b, const cplx* rb) const {
+        const std::vector<double>& coef = (pair_type == SPIMC_PAIR_BARE) ? v_long_k : u_long_k;
+        double tot = 0.;
+        for (uint32_t k = 0; k < nk; ++k) tot += coef[k] * cmag2(rb[k], rb[k]);
+        return pair_type == SPIMC_PAIR_BARE ? tau * tot : tot;
+    }
+
+    // Central-difference gradient of the whole-path action terms for every bead,
+    // using local re-evaluation of the terms a bead enters (test reference for
+    // the adjoint; it restates no reference routine).
+    void gradient_fd(uint32_t terms, double h, double* out) {
+        int saved = mode;
+        mode = SPIMC_MODE_NEW;
+        std::vector<cplx> ph0, ph1, rb(nk);
+        for (uint32_t b = 0; b < M; ++b)
+            for (uint32_t p = 0; p < N; ++p) {
+                double* pos = R(p, b);
+                double g[3] = {0., 0., 0.};
+                for (int d = 0; d < 3; ++d) {
+                    double x0 = pos[d], f[2];
+                    for (int sgn = 0; sgn < 2; ++sgn) {
+                        pos[d] = x0 + (sgn ? -h : h);
+                        double s = 0.;
+                        uint32_t bm = (b + M - 1) % M;
+                        if (terms & SPIMC_GRAD_KINETIC) {
+                            double dd[3];
+                            dr(R(p, b), R(p, b + 1), dd);
+                            s -= kin_splines[0].log_rho_free(dd);
+                            dr(R(p, bm), R(p, b), dd);
+                            s -= kin_splines[0].log_rho_free(dd);
+                        }
+                        if ((terms & SPIMC_GRAD_PAIR_SHORT) && pair_type != SPIMC_PAIR_NONE) {
+                            for (uint32_t q = 0; q < N; ++q) {
+                                if (q == p) continue;
+                                uint32_t lo = std::min(p, q), hi = std::max(p, q);
+                                double rm, rpm, rrpm;
+                                dr_drp_drrp(b, b + 1, lo, hi, rm, rpm, rrpm);
+                                s += calc_u(rm, rpm, rrpm, 0);
+                                dr_drp_drrp(bm, bm + 1, lo, hi, rm, rpm, rrpm);
+                                s += calc_u(rm, rpm, rrpm, 0);
+                            }
+                        }
+                        if ((terms & SPIMC_GRAD_PAIR_LONG) && use_long_range && pair_type != SPIMC_PAIR_NONE) {
+                            pos[d] = x0;
+                            calc_rho_k(pos, ph0);
+                            pos[d] = x0 + (sgn ? -h : h);
+                            calc_rho_k(pos, ph1);
+                            for (uint32_t k = 0; k < nk; ++k) rb[k] = rho[(size_t)b * nk + k] + (ph1[k] - ph0[k]);
+                            s += slice_long(b, rb.data());
+                        }
+                        f[sgn] = s;
+                    }
+                    pos[d] = x0;
+                    g[d] = (f[0] - f[1]) / (2. * h);
+                }
+                double* o = out + 3 * ((size_t)b * N + p);
+                o[0] = g[0];
+                o[1] = g[1];
+                o[2] = g[2];
+            }
+        mode = saved;
+    }
+
+    // ---- draws (post-transform values, reference call order) ---------------
+    bool recording = false;
+    std::vector<double> rec;
+    double from_tape() {
+        if (cursor >= tape.size()) {
+            g_err = "replay tape exhausted";
+            return 0.5;
+        }
+        return tape[cursor++];
+    }
+    double keep(double v) {
+        if (recording) rec.push_back(v);
+        return v;
+    }
+    double draw() { return keep(use_tape ? from_tape() : rng.unif()); }
+    uint32_t draw_index(uint32_t n) { return (uint32_t)keep(use_tape ? from_tape() : (double)rng.index(n)); }
+    double draw_normal() { return keep(use_tape ? from_tape() : rng.normal()); }
+    double draw_step() { return keep(use_tape ? from_tape() : step_size * (rng.unif() - 0.5)); }
+
+    // ---- moves -------------------------------------------------------------
+    void store_r_all_beads_of(uint32_t p, uint32_t first, uint32_t n) {
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t b = (first + i) % M;
+            std::memcpy(Rm(0, p, b), Rm(1, p, b), 3 * sizeof(double));
+        }
+    }
+    void restore_r_all_beads_of(uint32_t p, uint32_t first, uint32_t n) {
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t b = (first + i) % M;
+            std::memcpy(Rm(1, p, b), Rm(0, p, b), 3 * sizeof(double));
+        }
+    }
+    void store_rho(uint32_t b0, uint32_t b1) {  // Species::StoreRhoK (species_class.h:431)
+        for (uint32_t b = b0; b < b1; ++b)
+            std::memcpy(&rhoc[(size_t)bead_loop(b) * nk], &rho[(size_t)bead_loop(b) * nk], nk * sizeof(cplx));
+    }
+    void restore_rho(uint32_t b0, uint32_t b1) {  // Species::RestoreRhoK (species_class.h:434)
+        for (uint32_t b = b0; b < b1; ++b)
+            std::memcpy(&rho[(size_t)bead_loop(b) * nk], &rhoc[(size_t)bead_loop(b) * nk], nk * sizeof(cplx));
+    }
+
+    // Bisect::Attempt (bisect_class.h:39-124)
+    bool bisect_attempt(uint32_t& p_out, uint32_t& b0_out) {
+        uint32_t p = draw_index(N);
+        uint32_t bead0 = draw_index(M);
+        uint32_t bead1 = bead0 + n_bisect_beads;
+        p_out = p;
+        b0_out = bead0;
+        std::vector<uint32_t> parts{p};
+        double prev_action_change = 0.;
+        for (int level = (int)n_level - 1; level >= 0; --level) {
+            uint32_t skip = 1u << level;
+            double level_tau = tau * skip;
+            double sigma = std::sqrt(lambda * level_tau);
+            double old_lsp = 0., new_lsp = 0.;
+            for (uint32_t a = bead0; a != bead1; a += 2 * skip) {
+                uint32_t bb = a + skip, cc = a + 2 * skip;
+                double rbar[3], t[3], delta_old[3], delta_new[3];
+                // old sampling, OLD mode: RBar(c, a) = a + 0.5 Dr(c, a)
+                mode = SPIMC_MODE_OLD;
+                dr(R(p, cc), R(p, a), t);
+                for (int d = 0; d < 3; ++d) rbar[d] = R(p, a)[d] + 0.5 * t[d];
+                dr(R(p, bb), rbar, delta_old);
+                old_lsp += bis_splines[level].log_rho_free(delta_old);
+                // new sampling, NEW mode
+                mode = SPIMC_MODE_NEW;
+                for (int d = 0; d < 3; ++d) delta_new[d] = sigma * draw_normal();
+                put_in_box(delta_new);
+                dr(R(p, cc), R(p, a), t);
+                for (int d = 0; d < 3; ++d) rbar[d] = R(p, a)[d] + 0.5 * t[d];
+                double* nb = R(p, bb);
+                for (int d = 0; d < 3; ++d) nb[d] = rbar[d] + delta_new[d];
+                new_lsp += bis_splines[level].log_rho_free(delta_new);
+            }
+            double old_action = 0., new_action = 0.;
+            // action list order: Kinetic, pair action
+            mode = SPIMC_MODE_OLD;
+            old_action += kinetic_get_action(bead0, bead1, parts, level);
+            mode = SPIMC_MODE_NEW;
+            new_action += kinetic_get_action(bead0, bead1, parts, level);
+            mode = SPIMC_MODE_OLD;
+            old_action += pair_get_action(bead0, bead1, parts, level);
+            mode = SPIMC_MODE_NEW;
+            new_action += pair_get_action(bead0, bead1, parts, level);
+            double log_sample_ratio = -new_lsp + old_lsp;
+            double current_action_change = new_action - old_action;
+            double log_accept = log_sample_ratio - current_action_change + prev_action_change;
+            if (log_accept < std::log(draw())) return false;
+            prev_action_change = current_action_change;
+        }
+        return true;
+    }
+    void bisect_accept(uint32_t p, uint32_t bead0) {  // bisect_class.h:24-36
+        store_r_all_beads_of(p, bead0, n_bisect_beads + 1);
+        store_rho(bead0, bead0 + n_bisect_beads);
+        need_update_rho_k = true;
+    }
+    void bisect_reject(uint32_t p, uint32_t bead0) {  // bisect_class.h:127-139
+        restore_r_all_beads_of(p, bead0, n_bisect_beads + 1);
+        restore_rho(bead0, bead0 + n_bisect_beads);
+        need_update_rho_k = true;
+    }
+
+    // DisplaceParticle::Attempt (displace_particle_class.h:28-72)
+    bool displace_attempt(uint32_t& p_out) {
+        uint32_t p = draw_index(N);
+        p_out = p;
+        std::vector<uint32_t> parts{p};
+        mode = SPIMC_MODE_NEW;
+        double d[3];
+        for (int k = 0; k < 3; ++k) d[k] = draw_step();
+        for (uint32_t b = 0; b < M; ++b) {
+            double* x = R(p, b);
+            for (int k = 0; k < 3; ++k) x[k] = x[k] + d[k];
+        }
+        double old_action = 0., new_action = 0.;
+        mode = SPIMC_MODE_OLD;
+        old_action += kinetic_get_action(0, M, parts, 0);
+        mode = SPIMC_MODE_NEW;
+        new_action += kinetic_get_action(0, M, parts, 0);
+        mode = SPIMC_MODE_OLD;
+        old_action += pair_get_action(0, M, parts, 0);
+        mode = SPIMC_MODE_NEW;
+        new_action += pair_get_action(0, M, parts, 0);
+        double log_accept = old_action - new_action;
+        return !(log_accept < std::log(draw()));
+    }
+    void displace_accept(uint32_t p) {  // displace_particle_class.h:15-26
+        store_r_all_beads_of(p, 0, M);
+        store_rho(0, M);
+        need_update_rho_k = true;
+    }
+    void displace_reject(uint32_t p) {  // displace_particle_class.h:75-86
+        restore_r_all_beads_of(p, 0, M);
+        restore_rho(0, M);
+        need_update_rho_k = true;
+    }
+
+    // ---- observables -------------------------------------------------------
+    // Energy::ThermalEstimator / VirialEstimator (energy_class.h:22-116)
+    void energy_accumulate() {
+        mode = SPIMC_MODE_NEW;
+        if (energy_estimator == SPIMC_ENERGY_THERMAL) {
+            e_acc[0] += kinetic_d_action_d_beta();
+            e_acc[1] += pair_d_action_d_beta();
+        } else {
+            uint32_t W = virial_window ? virial_window : M;
+            double kinetic_energy = kinetic_virial_energy(W);
+            double centroid = 0.;
+            std::vector<double> g;
+            if (energy_estimator == SPIMC_ENERGY_VIRIAL_ADJOINT) {
+                g.resize((size_t)3 * M * N);
+                gradient_fd(SPIMC_GRAD_PAIR_SHORT | SPIMC_GRAD_PAIR_LONG, 1.e-5, g.data());
+            }
+            for (uint32_t p = 0; p < N; ++p)
+                for (uint32_t b = 0; b < M; ++b) {
+                    double d[3], grad[3];
+                    dr(R(p, b), R(p, 0), d);
+                    if (energy_estimator == SPIMC_ENERGY_VIRIAL_COMPAT) {
+                        pair_gradient_window(b, b + 1, p, grad);
+                    } else {
+                        const double* gg = &g[3 * ((size_t)b * N + p)];
+                        grad[0] = gg[0];
+                        grad[1] = gg[1];
+                        grad[2] = gg[2];
+                    }
+                    centroid += grad[0] * d[0] + grad[1] * d[1] + grad[2] * d[2];
+                }
+            kinetic_energy += centroid / (2. * tau);
+            e_acc[0] += kinetic_energy;
+            e_acc[1] += pair_d_action_d_beta();
+        }
+        n_measure[0] += 1;
+    }
+    // StructureFactor::Accumulate (structure_factor_class.h:15-32)
+    void sk_accumulate() {
+        mode = SPIMC_MODE_NEW;
+        for (uint32_t k = 0; k < nk; ++k)
+            if (kmag[k] < sk_k_cut)
+                for (uint32_t b = 0; b < M; ++b) {
+                    const cplx& a = rho[(size_t)b * nk + k];
+                    sk_acc[k] += cmag2(a, a);
+                }
+        n_measure[1] += 1;
+    }
+    // PairCorrelation::Accumulate (pair_correlation_class.h:15-28), LinearGrid::ReverseMap
+    // (observable_class.h:56-62)
+    void gr_accumulate() {
+        mode = SPIMC_MODE_NEW;
+        auto pairs = all_pairs();
+        for (uint32_t b = 0; b < M; ++b)
+            for (auto& pq : pairs) {
+                double d[3];
+                dr(R(pq.first, b), R(pq.second, b), d);
+                // (uint32_t)nearbyint(x): negative x wraps past n_r and is dropped
+                double f = std::nearbyint((mag(d) - gr_r_min) * gr_d_ir - 0.5);
+                if (f >= 0. && f < (double)gr_n_r) {
+                    uint32_t i = (uint32_t)f;
+                    gr_acc[i] = gr_acc[i] + 1.;
+                }
+            }
+        n_measure[2] += 1;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* spimc_oracle_last_error(void) { return g_err.c_str(); }
+
+void spimc_oracle_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int spimc_oracle_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+// KSpace::Setup / Include (k_space_class.h:17-80).  Enumerates every integer
+// triple in [-max_index, max_index]^3, max_index = ceil(1.1 k_cut / (2 pi / L)),
+// keeps the strict-cutoff half space.  Order: n0, then n1, then n2 ascending.
+int spimc_oracle_kspace_build(double L, double k_cut, int32_t* k_n, uint32_t cap, uint32_t* n_k) {
+    if (!(L > 0.) || !(k_cut >= 0.) || !n_k) return fail(SPIMC_ERR_ARG, "bad kspace arguments");
+    double box = 2. * M_PI / L;
+    int mx = (int)(uint32_t)std::ceil(1.1 * k_cut / box);
+    uint32_t cnt = 0;
+    for (int a = -mx; a <= mx; ++a)
+        for (int b = -mx; b <= mx; ++b)
+            for (int c = -mx; c <= mx; ++c) {
+                double k[3] = {a * box, b * box, c * box};
+                double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+                bool inc = false;
+                if (k2 < k_cut * k_cut && k2 != 0.) {
+                    if (k[0] > 0.)
+                        inc = true;
+                    else if (k[0] == 0. && k[1] > 0.)
+                        inc = true;
+                    else if (k[0] == 0. && k[1] == 0. && k[2] > 0.)
+                        inc = true;
+                }
+                if (!inc) continue;
+                if (k_n && cnt < cap) {
+                    k_n[3 * cnt] = a;
+                    k_n[3 * cnt + 1] = b;
+                    k_n[3 * cnt + 2] = c;
+                }
+                ++cnt;
+            }
+    *n_k = cnt;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_create(const spimc_system_desc* s, uint64_t seed, void** out) {
+    if (!s || !out) return fail(SPIMC_ERR_ARG, "null argument");
+    if (s->n_d != 3) return fail(SPIMC_ERR_UNSUPPORTED, "n_d must be 3");
+    if (s->n_bead < 2 || s->n_part < 1) return fail(SPIMC_ERR_ARG, "bad sizes");
+    if (s->pair_max_level != 0) return fail(SPIMC_ERR_UNSUPPORTED, "max_level must be 0");
+    Oracle* o = new Oracle();
+    o->N = s->n_part;
+    o->M = s->n_bead;
+    o->L = s->L;
+    o->iL = s->L > 0. ? 1. / s->L : 0.;
+    o->vol = s->L > 0. ? s->L * s->L * s->L : 1.;
+    o->beta = s->beta;
+    o->tau = s->beta / (1. * s->n_bead);  // path_class.h:43
+    o->lambda = s->lambda;
+    o->r.assign((size_t)3 * o->M * o->N, 0.);
+    o->rc = o->r;
+    // k-space
+    o->nk = s->n_k;
+    o->kn.assign(s->k_n, s->k_n + 3 * (size_t)s->n_k);
+    o->box = s->L > 0. ? 2. * M_PI / s->L : 0.;
+    o->max_index = 0;
+    o->kmag.resize(o->nk);
+    for (uint32_t k = 0; k < o->nk; ++k) {
+        double k2 = 0.;
+        for (int d = 0; d < 3; ++d) {
+            o->max_index = std::max(o->max_index, std::abs(o->kn[3 * k + d]));
+            double kv = o->kn[3 * k + d] * o->box;
+            k2 += kv * kv;
+        }
+        o->kmag[k] = std::sqrt(k2);
+    }
+    // Kinetic (kinetic_class.h:16-32)
+    o->kin_n_images = s->kinetic_n_images;
+    o->kin_i4lt = 1. / (4. * o->lambda * o->tau);
+    o->kin_dbeta_const = o->N * o->M * 3 / (2. * o->tau);
+    uint32_t n_spline = o->M / 2 + (o->M % 2) + 1;
+    o->kin_splines.resize(n_spline);
+    o->kin_splines[0].build(o->L, o->kin_n_images, o->lambda, o->tau, true);
+    for (uint32_t i = 1; i < n_spline; ++i)
+        o->kin_splines[i].build(o->L, o->kin_n_images, o->lambda, o->tau * (i + 1), false);
+    // Bisect (bisect_class.h:158-190)
+    o->n_level = s->bisect_n_level;
+    o->n_bisect_beads = 1u << o->n_level;
+    o->bis_n_images = s->bisect_n_images;
+    uint32_t n_bs = (uint32_t)std::floor(std::log2((double)o->M));
+    if (o->n_level < 1 || o->n_level > n_bs) {
+        delete o;
+        return fail(SPIMC_ERR_ARG, "bisect n_level must be in [1, floor(log2 n_bead)]");
+    }
+    o->bis_splines.resize(n_bs);
+    for (uint32_t i = 0; i < n_bs; ++i)
+        o->bis_splines[i].build(o->L, o->bis_n_images, o->lambda, 0.5 * o->tau * (1u << i), false);
+    o->step_size = s->displace_step;
+    // pair action
+    o->pair_type = s->pair_type;
+    o->use_long_range = s->pair_use_long_range != 0;
+    o->max_level = s->pair_max_level;
+    double nn = (double)o->N;
+    if (o->pair_type == SPIMC_PAIR_BARE) {
+        // BarePairAction ctor (bare_pair_action_class.h:22-96)
+        o->is_coulomb = s->bare_is_coulomb != 0;
+        o->v_r.build(s->bare_r, s->bare_v_r, s->bare_n_r, false);  // create_general_grid
+        o->r_v_min = s->bare_r[0];
+        o->r_v_max = s->bare_r[s->bare_n_r - 1];
+        if (o->use_long_range) {
+            o->v_long_r.build(s->bare_r_long, s->bare_v_long_r, s->bare_n_r_long, false);
+            o->r_v_long_min = s->bare_r_long[0];
+            o->r_v_long_max = s->bare_r_long[s->bare_n_r_long - 1];
+            o->v_long_k.assign(s->pair_v_long_k, s->pair_v_long_k + o->nk);
+            o->v_long_k_0 = s->pair_v_long_k_0 * 0.5 * nn * nn * o->M;
+            o->v_long_r_0 = s->pair_v_long_r_0 * (-0.5) * nn * o->M;
+        }
+    } else if (o->pair_type == SPIMC_PAIR_DAVID) {
+        // DavidPairAction ctor (david_pair_action_class.h:192-360)
+        o->n_order = s->david_n_order;
+        o->n_val = 1;
+        for (uint32_t i = 1; i <= o->n_order; ++i) o->n_val += 1 + i;
+        uint32_t ng = s->david_n_grid;
+        o->g_start = s->david_grid[0];
+        o->g_end = s->david_grid[ng - 1];
+        o->u_kj.resize(o->n_val + 1);
+        o->du_kj.resize(o->n_val + 1);
+        for (uint32_t c = 0; c <= o->n_val; ++c) {
+            o->u_kj[c].build(s->david_grid, s->david_u_kj + (size_t)c * ng, ng, false);
+            o->du_kj[c].build(s->david_grid, s->david_du_kj + (size_t)c * ng, ng, false);
+        }
+        if (o->use_long_range) {
+            o->v_long_k.assign(s->pair_v_long_k, s->pair_v_long_k + o->nk);
+            o->u_long_k.resize(o->nk);
+            o->du_long_k.resize(o->nk);
+            for (uint32_t k = 0; k < o->nk; ++k) {
+                o->u_long_k[k] = o->v_long_k[k] * o->tau;
+                o->du_long_k[k] = o->v_long_k[k];
+            }
+            double vk0 = s->pair_v_long_k_0, vr0 = s->pair_v_long_r_0;
+            o->du_long_k_0 = vk0 * 0.5 * nn * nn * o->M;
+            o->du_long_r_0 = vr0 * (-0.5) * nn * o->M;
+            o->v_long_k_0 = vk0 * 0.5 * nn * nn * o->M;
+            o->v_long_r_0 = vr0 * (-0.5) * nn * o->M;
+        }
+    } else if (o->pair_type != SPIMC_PAIR_NONE) {
+        delete o;
+        return fail(SPIMC_ERR_ARG, "unknown pair type");
+    }
+    o->rng.seed(seed);
+    o->virial_window = o->M;
+    o->sk_k_cut = 1e300;
+    o->gr_n_r = 100;
+    o->gr_r_min = 0.;
+    o->gr_r_max = o->L / 2.;
+    o->gr_dr = (o->gr_r_max - o->gr_r_min) / (o->gr_n_r - 1.);
+    o->gr_d_ir = 1. / o->gr_dr;
+    o->sk_acc.assign(o->nk, 0.);
+    o->gr_acc.assign(o->gr_n_r, 0.);
+    o->init_rho_k();
+    *out = o;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_destroy(void* h) {
+    delete (Oracle*)h;
+    return SPIMC_OK;
+}
+
+// positions r[d][b][p]
+int spimc_oracle_path_upload(void* h, const double* r) {
+    Oracle* o = (Oracle*)h;
+    for (uint32_t d = 0; d < 3; ++d)
+        for (uint32_t b = 0; b < o->M; ++b)
+            for (uint32_t p = 0; p < o->N; ++p) o->r[3 * ((size_t)b * o->N + p) + d] = r[((size_t)d * o->M + b) * o->N + p];
+    o->rc = o->r;
+    o->init_rho_k();
+    o->need_update_rho_k = true;
+    o->have_proposal = false;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_path_download(void* h, int mode, double* r) {
+    Oracle* o = (Oracle*)h;
+    const std::vector<double>& src = mode ? o->r : o->rc;
+    for (uint32_t d = 0; d < 3; ++d)
+        for (uint32_t b = 0; b < o->M; ++b)
+            for (uint32_t p = 0; p < o->N; ++p) r[((size_t)d * o->M + b) * o->N + p] = src[3 * ((size_t)b * o->N + p) + d];
+    return SPIMC_OK;
+}
+
+int spimc_oracle_rho_k_download(void* h, int mode, double* out) {
+    Oracle* o = (Oracle*)h;
+    const std::vector<cplx>& src = mode ? o->rho : o->rhoc;
+    for (size_t i = 0; i < src.size(); ++i) {
+        out[2 * i] = src[i].real();
+        out[2 * i + 1] = src[i].imag();
+    }
+    return SPIMC_OK;
+}
+
+int spimc_oracle_rho_k_rebuild(void* h) {
+    Oracle* o = (Oracle*)h;
+    o->r = o->rc;
+    o->init_rho_k();
+    return SPIMC_OK;
+}
+
+int spimc_oracle_set_proposal(void* h, uint32_t p, uint32_t b_first, uint32_t n, const double* r_new) {
+    Oracle* o = (Oracle*)h;
+    if (p >= o->N || n > o->M) return fail(SPIMC_ERR_ARG, "bad proposal");
+    for (uint32_t i = 0; i < n; ++i) {
+        double* x = o->Rm(1, p, (b_first + i) % o->M);
+        for (int d = 0; d < 3; ++d) x[d] = r_new[3 * i + d];
+    }
+    o->have_proposal = true;
+    o->prop_p = p;
+    o->prop_first = b_first % o->M;
+    o->prop_n = n;
+    o->need_update_rho_k = true;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_accept_proposal(void* h) {
+    Oracle* o = (Oracle*)h;
+    if (!o->have_proposal) return fail(SPIMC_ERR_STATE, "no proposal");
+    o->store_r_all_beads_of(o->prop_p, o->prop_first, o->prop_n);
+    o->store_rho(0, o->M);
+    o->need_update_rho_k = true;
+    o->have_proposal = false;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_reject_proposal(void* h) {
+    Oracle* o = (Oracle*)h;
+    if (!o->have_proposal) return fail(SPIMC_ERR_STATE, "no proposal");
+    o->restore_r_all_beads_of(o->prop_p, o->prop_first, o->prop_n);
+    o->restore_rho(0, o->M);
+    o->need_update_rho_k = true;
+    o->have_proposal = false;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_get_action(void* h, int action, int mode, uint32_t b0, uint32_t b1, const uint32_t* parts,
+                            uint32_t n_parts, uint32_t level, double* out) {
+    Oracle* o = (Oracle*)h;
+    std::vector<uint32_t> pv(parts, parts + n_parts);
+    o->mode = mode;
+    if (action == SPIMC_ACTION_KINETIC)
+        *out = o->kinetic_get_action(b0, b1, pv, level);
+    else
+        *out = o->pair_get_action(b0, b1, pv, level);
+    o->mode = SPIMC_MODE_NEW;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_d_action_d_beta(void* h, int action, double* out) {
+    Oracle* o = (Oracle*)h;
+    o->mode = SPIMC_MODE_NEW;
+    *out = action == SPIMC_ACTION_KINETIC ? o->kinetic_d_action_d_beta() : o->pair_d_action_d_beta();
+    return SPIMC_OK;
+}
+
+int spimc_oracle_potential(void* h, int action, double* out) {
+    Oracle* o = (Oracle*)h;
+    o->mode = SPIMC_MODE_NEW;
+    *out = action == SPIMC_ACTION_KINETIC ? 0. : o->pair_potential();
+    return SPIMC_OK;
+}
+
+int spimc_oracle_kinetic_virial_energy(void* h, uint32_t window, double* out) {
+    Oracle* o = (Oracle*)h;
+    o->mode = SPIMC_MODE_NEW;
+    *out = o->kinetic_virial_energy(window ? window : o->M);
+    return SPIMC_OK;
+}
+
+// compat=1: reference FD gradient per bead (pair_action_class.h:305-337);
+// compat=0: central difference (step h) of the selected whole-path terms.
+int spimc_oracle_action_gradient(void* h, uint32_t terms, int compat, double fd_h, double* out) {
+    Oracle* o = (Oracle*)h;
+    o->mode = SPIMC_MODE_NEW;
+    if (compat) {
+        for (uint32_t b = 0; b < o->M; ++b)
+            for (uint32_t p = 0; p < o->N; ++p) o->pair_gradient_window(b, b + 1, p, out + 3 * ((size_t)b * o->N + p));
+        return SPIMC_OK;
+    }
+    o->gradient_fd(terms, fd_h, out);
+    return SPIMC_OK;
+}
+
+// whole-path totals: out[0] kinetic, out[1] pair short range, out[2] long range (per-slice sum, no constants)
+int spimc_oracle_action_totals(void* h, double* out) {
+    Oracle* o = (Oracle*)h;
+    o->mode = SPIMC_MODE_NEW;
+    out[0] = o->total_kinetic();
+    out[1] = o->total_pair_short();
+    double l = 0.;
+    if (o->use_long_range && o->pair_type != SPIMC_PAIR_NONE)
+        for (uint32_t b = 0; b < o->M; ++b) l += o->slice_long(b, &o->rho[(size_t)b * o->nk]);
+    out[2] = l;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_tape_set(void* h, const double* draws, uint64_t n) {
+    Oracle* o = (Oracle*)h;
+    if (!draws) {
+        o->use_tape = false;
+        o->tape.clear();
+    } else {
+        o->use_tape = true;
+        o->tape.assign(draws, draws + n);
+    }
+    o->cursor = 0;
+    return SPIMC_OK;
+}
+
+// Records every consumed draw (post-transform) while enabled; read with spimc_oracle_recorded.
+int spimc_oracle_record(void* h, int enable) {
+    Oracle* o = (Oracle*)h;
+    o->recording = enable != 0;
+    if (enable) o->rec.clear();
+    return SPIMC_OK;
+}
+
+int spimc_oracle_recorded(void* h, double* out, uint64_t cap, uint64_t* n) {
+    Oracle* o = (Oracle*)h;
+    *n = o->rec.size();
+    if (out) std::copy(o->rec.begin(), o->rec.begin() + std::min<uint64_t>(cap, o->rec.size()), out);
+    return SPIMC_OK;
+}
+
+int spimc_oracle_tape_cursor(void* h, uint64_t* c) {
+    *c = ((Oracle*)h)->cursor;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_bisect_stage(void* h, uint32_t n_attempts, uint8_t* accepted) {
+    Oracle* o = (Oracle*)h;
+    g_err.clear();
+    for (uint32_t i = 0; i < n_attempts; ++i) {
+        uint32_t p, b0;
+        o->counters[0]++;
+        bool ok = o->bisect_attempt(p, b0);
+        if (ok) {
+            o->counters[1]++;
+            o->bisect_accept(p, b0);
+        } else {
+            o->bisect_reject(p, b0);
+        }
+        o->mode = SPIMC_MODE_NEW;
+        o->bead_updates += o->n_bisect_beads - 1;
+        if (accepted) accepted[i] = ok ? 1 : 0;
+    }
+    return g_err.empty() ? SPIMC_OK : SPIMC_ERR_STATE;
+}
+
+int spimc_oracle_displace_stage(void* h, uint32_t n_attempts, uint8_t* accepted) {
+    Oracle* o = (Oracle*)h;
+    g_err.clear();
+    for (uint32_t i = 0; i < n_attempts; ++i) {
+        uint32_t p;
+        o->counters[2]++;
+        bool ok = o->displace_attempt(p);
+        if (ok) {
+            o->counters[3]++;
+            o->displace_accept(p);
+        } else {
+            o->displace_reject(p);
+        }
+        o->mode = SPIMC_MODE_NEW;
+        o->bead_updates += o->M;
+        if (accepted) accepted[i] = ok ? 1 : 0;
+    }
+    return g_err.empty() ? SPIMC_OK : SPIMC_ERR_STATE;
+}
+
+int spimc_oracle_move_counters(void* h, uint64_t* out) {
+    Oracle* o = (Oracle*)h;
+    for (int i = 0; i < 4; ++i) out[i] = o->counters[i];
+    return SPIMC_OK;
+}
+
+int spimc_oracle_bead_updates(void* h, uint64_t* out) {
+    *out = ((Oracle*)h)->bead_updates;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_obs_configure(void* h, int energy_estimator, uint32_t virial_window, double sk_k_cut, double gr_r_min,
+                               double gr_r_max, uint32_t gr_n_r) {
+    Oracle* o = (Oracle*)h;
+    if (gr_n_r < 2) return fail(SPIMC_ERR_ARG, "n_r must be >= 2");
+    o->energy_estimator = energy_estimator;
+    o->virial_window = virial_window ? virial_window : o->M;
+    o->sk_k_cut = sk_k_cut;
+    o->gr_r_min = gr_r_min;
+    o->gr_r_max = gr_r_max > 0. ? gr_r_max : o->L / 2.;
+    o->gr_n_r = gr_n_r;
+    // LinearGrid::CreateGrid (observable_class.h:29-38)
+    o->gr_dr = (o->gr_r_max - o->gr_r_min) / (gr_n_r - 1.);
+    o->gr_d_ir = 1. / o->gr_dr;
+    o->gr_acc.assign(gr_n_r, 0.);
+    o->sk_acc.assign(o->nk, 0.);
+    o->e_acc[0] = o->e_acc[1] = 0.;
+    for (auto& m : o->n_measure) m = 0;
+    return SPIMC_OK;
+}
+
+int spimc_oracle_obs_accumulate(void* h, uint32_t which) {
+    Oracle* o = (Oracle*)h;
+    if (which & SPIMC_OBS_ENERGY) o->energy_accumulate();
+    if (which & SPIMC_OBS_SK) o->sk_accumulate();
+    if (which & SPIMC_OBS_GR) o->gr_accumulate();
+    return SPIMC_OK;
+}
+
+int spimc_oracle_obs_read(void* h, uint32_t which, double* out, uint64_t* n_measure) {
+    Oracle* o = (Oracle*)h;
+    if (which == SPIMC_OBS_ENERGY) {
+        out[0] = o->e_acc[0];
+        out[1] = o->e_acc[1];
+        *n_measure = o->n_measure[0];
+    } else if (which == SPIMC_OBS_SK) {
+        std::copy(o->sk_acc.begin(), o->sk_acc.end(), out);
+        *n_measure = o->n_measure[1];
+    } else if (which == SPIMC_OBS_GR) {
+        std::copy(o->gr_acc.begin(), o->gr_acc.end(), out);
+        *n_measure = o->n_measure[2];
+    } else {
+        return fail(SPIMC_ERR_ARG, "read one observable at a time");
+    }
+    return SPIMC_OK;
+}
+
+// One sweep = N*M/2^n_level bisect attempts, N displace attempts, then estimators.
+int spimc_oracle_sweep(void* h, uint32_t n_sweeps, uint32_t flags) {
+    Oracle* o = (Oracle*)h;
+    uint32_t n_bis = o->N * o->M / o->n_bisect_beads;
+    for (uint32_t s = 0; s < n_sweeps; ++s) {
+        int rc = spimc_oracle_bisect_stage(h, n_bis, nullptr);
+        if (rc) return rc;
+        rc = spimc_oracle_displace_stage(h, o->N, nullptr);
+        if (rc) return rc;
+        uint32_t which = 0;
+        if (flags & SPIMC_SWEEP_ENERGY) which |= SPIMC_OBS_ENERGY;
+        if (flags & SPIMC_SWEEP_SK) which |= SPIMC_OBS_SK;
+        if (flags & SPIMC_SWEEP_GR) which |= SPIMC_OBS_GR;
+        spimc_oracle_obs_accumulate(h, which);
+    }
+    return SPIMC_OK;
+}
+
+// Natural spline evaluation helper for golden-vector tests of the spline itself.
+int spimc_oracle_spline_eval(const double* x, const double* y, uint32_t n, const double* t, uint32_t nt, double* out,
+                             double* dout) {
+    NatSpline s;
+    s.build(x, y, n, false);
+    for (uint32_t i = 0; i < nt; ++i) {
+        out[i] = s.eval(t[i]);
+        if (dout) dout[i] = s.deriv(t[i]);
+    }
+    return SPIMC_OK;
+}
+
+// FreeSpline::GetLogRhoFree helper for kinetic tests.
+int spimc_oracle_free_spline_log_rho(double L, int n_images, double lambda, double tau, const double* r, uint32_t n,
+                                     double* out) {
+    FreeSpline f;
+    f.build(L, n_images, lambda, tau, false);
+    for (uint32_t i = 0; i < n; ++i) out[i] = f.log_rho_free(r + 3 * i);
+    return SPIMC_OK;
+}
+
+}  // extern "C"

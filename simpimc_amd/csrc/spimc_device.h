@@ -1,0 +1,450 @@
+// spimc_device.h -- device data layout and per-element math of the engine.
+//
+// Layout in HBM (one replica):
+//   pos   : FP64 [3][M][N]      committed bead positions, slice-major SoA
+//   rho   : double2 [2][M][nk]  two buffers of per-slice rho_k; sel[b] names the
+//                               committed one, the other receives proposals
+//   sel   : uint8 [M]
+//   tables: splines in per-interval polynomial form, components interleaved
+//           so that one interval of all components is contiguous.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spimc {
+
+constexpr int kMaxLevel = 6;      // bisection levels supported (window of 64 links)
+constexpr int kWave = 64;         // CDNA wavefront
+constexpr int kMaxKinSplines = 8; // kinetic / bisect image splines at tau * 2^l
+
+// Natural cubic spline: s(t) = c0 + dt (c1 + dt (c2 + dt c3)), dt = t - x[i].
+struct Spl {
+    const double* x;  // [n] knots
+    const double* c;  // [n-1][ncomp][4]
+    int n;
+    int ncomp;
+    int uniform;
+    int pad;
+    double x0;
+    double inv_h;
+};
+
+// Per-attempt proposal of one replica.
+struct Prop {
+    int status;   // 0 idle, 1 pending level-0 decision, 2 rejected at a coarse level
+    int move;     // 0 bisect, 1 displace, 2 interface
+    int p;        // moved particle
+    int b0;       // first slice of the link window
+    int n_links;  // links (b, b+1) in the window
+    int first;    // first moved slice
+    int count;    // moved slices
+    int stage_i;  // attempt index within the current stage (for the accepted flags)
+    double lsr;   // log sample ratio at level 0
+    double k_old; // kinetic action at level 0, OLD / NEW
+    double k_new;
+    double prev;  // action change of the previous (coarser) level
+    double u0;    // level-0 uniform
+    double pad;
+    long long tape_next;  // tape cursor after this attempt
+    long long pad2;
+};
+
+struct Counters {
+    unsigned long long bis_att, bis_acc, dis_att, dis_acc;
+    unsigned long long seq;           // attempts issued (device RNG counter)
+    unsigned long long bead_updates;
+    unsigned int remaining;           // attempts left in the running stage
+    unsigned int stage_i;             // attempts done in the running stage
+    unsigned long long error;         // replay tape overrun or invalid draw
+};
+
+struct Rep {
+    // sizes and flags
+    int N, M, nk, max_idx;
+    int n_level, pair_type, use_long_range, is_coulomb;
+    int n_order, n_val, kin_images, bis_images;
+    int n_links_max, n_chunks, index, pad;
+    // scalars
+    double L, iL, tau, lambda;
+    double step, box;
+    double kin_i4lt, kin_i4ltt, kin_dbeta_const;
+    double long_scale;  // per-slice long-range action = long_scale * sum_k coef_s |rho|^2
+    double const_v, const_du;
+    double r_v_min, r_v_max, r_l_min, r_l_max, g_min, g_max;
+    uint64_t key;       // device RNG key
+    // bead storage
+    double* pos;
+    double2* rho;
+    unsigned char* sel;
+    // k-space
+    const int* kidx;       // packed (n0+128) | (n1+128) << 8 | (n2+128) << 16
+    const double* kmag;    // |k|
+    const double* coef_s;  // per-slice action coefficient per k
+    const double* coef_b;  // per-slice beta-derivative / potential coefficient per k
+    // pair tables
+    Spl v_r, v_long_r;     // Bare
+    Spl u, du;             // David, ncomp = n_val + 1
+    // kinetic and sampling image splines (n_images > 0 only)
+    Spl kin[kMaxKinSplines];
+    Spl kin_dtau;
+    Spl bis[kMaxKinSplines];
+    // per-attempt state
+    Prop* prop;
+    double* prop_new;      // [M][3] NEW positions of the moved slices
+    double2* pair_part;    // [n_links_max] (old, new)
+    double2* rho_part;     // [M][n_chunks] (old, new)
+    double2* kin_part;     // interface kinetic (old, new)
+    Counters* cnt;
+    unsigned char* accepted; // [accepted_cap]
+    unsigned int accepted_cap;
+    unsigned int pad3;
+    // replay tape
+    const double* tape;
+    long long tape_len;
+    long long* tape_cursor;
+    // estimators
+    double* e_acc;          // [2]
+    double* sk_acc;         // [nk]
+    unsigned long long* gr_cnt;  // [gr_n_r]
+    double* scratch;        // [M * 8 + nk] reduction partials
+    double* grad;           // [M][N][3] dS/dr of the last adjoint
+    double* grad_est;       // [M][N][3] estimator scratch gradient
+    unsigned long long* n_measure;  // [3]
+};
+
+// ---------------------------------------------------------------------------
+// splines
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int spl_locate(const Spl& s, double t) {
+    int i;
+    if (s.uniform) {
+        double f = floor((t - s.x0) * s.inv_h);
+        i = (f < 0.) ? 0 : (f > (double)(s.n - 2) ? s.n - 2 : (int)f);
+    } else {
+        const double* x = s.x;
+        if (t <= x[0]) {
+            i = 0;
+        } else if (t >= x[s.n - 1]) {
+            i = s.n - 2;
+        } else {
+            int lo = 0, hi = s.n - 1;
+            while (hi - lo >= 2) {
+                int mid = (hi + lo) >> 1;
+                if (x[mid] > t)
+                    hi = mid;
+                else
+                    lo = mid;
+            }
+            i = lo;
+        }
+    }
+    return i;
+}
+
+__device__ __forceinline__ double spl_val(const Spl& s, int i, int comp, double dt) {
+    const double* c = s.c + ((size_t)i * s.ncomp + comp) * 4;
+    return c[0] + dt * (c[1] + dt * (c[2] + dt * c[3]));
+}
+
+__device__ __forceinline__ double spl_der(const Spl& s, int i, int comp, double dt) {
+    const double* c = s.c + ((size_t)i * s.ncomp + comp) * 4;
+    return c[1] + dt * (2. * c[2] + dt * (3. * c[3]));
+}
+
+__device__ __forceinline__ double spl_eval1(const Spl& s, double t) {
+    int i = spl_locate(s, t);
+    return spl_val(s, i, 0, t - s.x[i]);
+}
+
+// ---------------------------------------------------------------------------
+// minimum image (path_class.h:108-150); contraction off so that positions and
+// distances round exactly like the host restatement.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double minimg(double d, double L, double iL) {
+#pragma clang fp contract(off)
+    return d - rint(d * iL) * L;
+}
+
+struct LinkGeom {
+    double rv[3], rpv[3], sv[3];
+    double r, rp, s;
+};
+
+// Path::DrDrpDrrp: r = rq(b) - rp(b) imaged; r' = rq(b') - rp(b') shifted by the
+// image nearest r; s = minimum image of r - r'.
+__device__ __forceinline__ void link_geom(const double* a0, const double* c0, const double* a1, const double* c1,
+                                          double L, double iL, LinkGeom& g) {
+#pragma clang fp contract(off)
+    for (int d = 0; d < 3; ++d) {
+        double rv = c0[d] - a0[d];
+        double rpv = c1[d] - a1[d];
+        rv -= rint(rv * iL) * L;
+        rpv += rint((rv - rpv) * iL) * L;
+        double sv = rv - rpv;
+        sv -= rint(sv * iL) * L;
+        g.rv[d] = rv;
+        g.rpv[d] = rpv;
+        g.sv[d] = sv;
+    }
+    g.r = sqrt(g.rv[0] * g.rv[0] + g.rv[1] * g.rv[1] + g.rv[2] * g.rv[2]);
+    g.rp = sqrt(g.rpv[0] * g.rpv[0] + g.rpv[1] * g.rpv[1] + g.rpv[2] * g.rpv[2]);
+    g.s = sqrt(g.sv[0] * g.sv[0] + g.sv[1] * g.sv[1] + g.sv[2] * g.sv[2]);
+}
+
+__device__ __forceinline__ void clamp2(double lo, double hi, double& a, double& b) {
+    a = (a > hi) ? hi : ((a < lo) ? lo : a);
+    b = (b > hi) ? hi : ((b < lo) ? lo : b);
+}
+
+// BarePairAction::CalcV (bare_pair_action_class.h:99-123)
+__device__ __forceinline__ double bare_v(const Rep& R, double r0, double r1) {
+    clamp2(R.r_v_min, R.r_v_max, r0, r1);
+    double v = 0.;
+    if (R.is_coulomb) {
+        v += (0.5 / r0) + (0.5 / r1);
+    } else {
+        v += 0.5 * spl_eval1(R.v_r, r0);
+        v += 0.5 * spl_eval1(R.v_r, r1);
+    }
+    if (R.use_long_range) {
+        clamp2(R.r_l_min, R.r_l_max, r0, r1);
+        v -= 0.5 * spl_eval1(R.v_long_r, r0);
+        v -= 0.5 * spl_eval1(R.v_long_r, r1);
+    }
+    return v;
+}
+
+// DavidPairAction off-diagonal sum: sum_{k=1..n} sum_{j=0..k} c_kj(q) z^2j s^2(k-j)
+// (david_pair_action_class.h:80-99), components 2.. of the table at interval iq.
+__device__ __forceinline__ double david_offdiag(const Spl& T, int n_order, int iq, double dq, double z, double s) {
+    double z_2 = z * z, s_2 = s * s, i_s_2 = 1. / s_2, s_2_k = s_2;
+    double u = 0.;
+    for (int k = 1; k <= n_order; ++k) {
+        double z_2_j = 1., cur_s = s_2_k;
+        for (int j = 0; j <= k; ++j) {
+            u += spl_val(T, iq, k * (k + 1) / 2 + j + 1, dq) * z_2_j * cur_s;
+            z_2_j *= z_2;
+            cur_s *= i_s_2;
+        }
+        s_2_k *= s_2;
+    }
+    return u;
+}
+
+// DavidPairAction::CalcU (david_pair_action_class.h:63-102), table T (u or du)
+// add_v: CalcdUdBeta adds the potential (component 0).
+__device__ __forceinline__ double david_u(const Rep& R, const Spl& T, double r0, double r1, double s, bool add_v) {
+    double q = 0.5 * (r0 + r1);
+    double z = r0 - r1;
+    clamp2(R.g_min, R.g_max, r0, r1);
+    int i0 = spl_locate(T, r0), i1 = spl_locate(T, r1);
+    double d0 = r0 - T.x[i0], d1 = r1 - T.x[i1];
+    double u = 0.5 * (spl_val(T, i0, 1, d0) + spl_val(T, i1, 1, d1));
+    if (add_v) u += 0.5 * (spl_val(T, i0, 0, d0) + spl_val(T, i1, 0, d1));
+    if (s > 0.0 && q < R.g_max) {
+        int iq = spl_locate(T, q);
+        u += david_offdiag(T, R.n_order, iq, q - T.x[iq], z, s);
+    }
+    return u;
+}
+
+// CalcU of the pair family at level 0 (bare_pair_action_class.h:146-150)
+__device__ __forceinline__ double pair_u(const Rep& R, double r0, double r1, double s) {
+    if (R.pair_type == 1) return R.tau * bare_v(R, r0, r1);
+    return david_u(R, R.u, r0, r1, s, false);
+}
+
+// CalcdUdBeta (bare_pair_action_class.h:175-177, david_pair_action_class.h:126-168)
+__device__ __forceinline__ double pair_du_dbeta(const Rep& R, double r0, double r1, double s) {
+    if (R.pair_type == 1) return bare_v(R, r0, r1);
+    return david_u(R, R.du, r0, r1, s, true);
+}
+
+// CalcV (bare_pair_action_class.h:99-123, david_pair_action_class.h:42-57)
+__device__ __forceinline__ double pair_v(const Rep& R, double r0, double r1) {
+    if (R.pair_type == 1) return bare_v(R, r0, r1);
+    clamp2(R.g_min, R.g_max, r0, r1);
+    int i0 = spl_locate(R.u, r0), i1 = spl_locate(R.u, r1);
+    return 0.5 * (spl_val(R.u, i0, 0, r0 - R.u.x[i0]) + spl_val(R.u, i1, 0, r1 - R.u.x[i1]));
+}
+
+// Partial derivatives of the level-0 link action U(r, r', s) of pair_u.
+__device__ __forceinline__ void pair_u_grad(const Rep& R, double r0, double r1, double s, double& ur, double& urp,
+                                            double& us) {
+    ur = urp = us = 0.;
+    if (R.pair_type == 1) {
+        double a = r0, b = r1;
+        bool ina = (a >= R.r_v_min && a <= R.r_v_max), inb = (b >= R.r_v_min && b <= R.r_v_max);
+        clamp2(R.r_v_min, R.r_v_max, a, b);
+        double ga = 0., gb = 0.;
+        if (R.is_coulomb) {
+            ga = -0.5 / (a * a);
+            gb = -0.5 / (b * b);
+        } else {
+            int ia = spl_locate(R.v_r, a), ib = spl_locate(R.v_r, b);
+            ga = 0.5 * spl_der(R.v_r, ia, 0, a - R.v_r.x[ia]);
+            gb = 0.5 * spl_der(R.v_r, ib, 0, b - R.v_r.x[ib]);
+        }
+        ga = ina ? ga : 0.;
+        gb = inb ? gb : 0.;
+        if (R.use_long_range) {
+            bool la = ina && (a >= R.r_l_min && a <= R.r_l_max), lb = inb && (b >= R.r_l_min && b <= R.r_l_max);
+            clamp2(R.r_l_min, R.r_l_max, a, b);
+            int ia = spl_locate(R.v_long_r, a), ib = spl_locate(R.v_long_r, b);
+            double la_d = 0.5 * spl_der(R.v_long_r, ia, 0, a - R.v_long_r.x[ia]);
+            double lb_d = 0.5 * spl_der(R.v_long_r, ib, 0, b - R.v_long_r.x[ib]);
+            ga -= la ? la_d : 0.;
+            gb -= lb ? lb_d : 0.;
+        }
+        ur = R.tau * ga;
+        urp = R.tau * gb;
+        return;
+    }
+    const Spl& T = R.u;
+    double q = 0.5 * (r0 + r1), z = r0 - r1;
+    double a = r0, b = r1;
+    bool ina = (a >= R.g_min && a <= R.g_max), inb = (b >= R.g_min && b <= R.g_max);
+    clamp2(R.g_min, R.g_max, a, b);
+    int ia = spl_locate(T, a), ib = spl_locate(T, b);
+    ur = ina ? 0.5 * spl_der(T, ia, 1, a - T.x[ia]) : 0.;
+    urp = inb ? 0.5 * spl_der(T, ib, 1, b - T.x[ib]) : 0.;
+    if (s > 0.0 && q < R.g_max) {
+        int iq = spl_locate(T, q);
+        double dq = q - T.x[iq];
+        for (int k = 1; k <= R.n_order; ++k)
+            for (int j = 0; j <= k; ++j) {
+                int comp = k * (k + 1) / 2 + j + 1;
+                double c = spl_val(T, iq, comp, dq), cd = spl_der(T, iq, comp, dq);
+                int e = k - j;  // power of s^2
+                // zp = z^2j, zm = z^(2j-1); sp = s^2e, sm = s^(2e-1)
+                double zp = 1., zm = 0., sp = 1., sm = 0.;
+                for (int t = 0; t < 2 * j; ++t) {
+                    zm = zp;
+                    zp *= z;
+                }
+                for (int t = 0; t < 2 * e; ++t) {
+                    sm = sp;
+                    sp *= s;
+                }
+                double dqterm = 0.5 * cd * zp * sp;         // q = (r + r') / 2
+                double dz = c * (2. * j) * zm * sp;          // z = r - r'
+                ur += dqterm + dz;
+                urp += dqterm - dz;
+                us += c * zp * (2. * e) * sm;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kinetic link action -log rho_free (free_spline_class.h:76-84) and its slope
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double free_neg_log(const Spl* img, int images, double i4lt, const double* d) {
+    double tot = 0.;
+    if (images > 0)
+        for (int k = 0; k < 3; ++k) tot -= spl_eval1(*img, d[k]);
+    return -(tot - (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * i4lt);
+}
+
+// ---------------------------------------------------------------------------
+// phase factors e^{i m phi} by the recurrence of KSpace::CalcC (k_space_class.h:83-94)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// table[0..mx] = e^{i m phi}, m = 0..mx
+__device__ __forceinline__ void phase_chain(double phi, int mx, double2* table) {
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    double2 t = make_double2(cs, sn);
+    double2 c = make_double2(1., 0.);
+    table[0] = c;
+    for (int m = 1; m <= mx; ++m) {
+        c = cmul(t, c);
+        table[m] = c;
+    }
+}
+
+__device__ __forceinline__ double2 phase_at(const double2* table, int m) {
+    double2 v = table[m < 0 ? -m : m];
+    if (m < 0) v.y = -v.y;
+    return v;
+}
+
+// e^{i k.r} = c0(n0) c1(n1) c2(n2), tables laid out [3][stride]
+__device__ __forceinline__ double2 phase_k(const double2* tab, int stride, int packed) {
+    int n0 = (packed & 255) - 128, n1 = ((packed >> 8) & 255) - 128, n2 = ((packed >> 16) & 255) - 128;
+    double2 f = phase_at(tab, n0);
+    f = cmul(f, phase_at(tab + stride, n1));
+    f = cmul(f, phase_at(tab + 2 * stride, n2));
+    return f;
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11) for the device RNG
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox(uint4 c, uint2 k) {
+    for (int i = 0; i < 10; ++i) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {  // (0, 1]
+    uint64_t v = (((uint64_t)a << 32) | b) >> 11;
+    return (double)(v + 1) * (1.0 / 9007199254740992.0);
+}
+
+struct Rng {
+    uint2 key;
+    uint32_t tag;
+    uint64_t seq;
+    __device__ uint4 call(uint32_t i) const {
+        return philox(make_uint4(i, tag, (uint32_t)seq, (uint32_t)(seq >> 32)), key);
+    }
+    __device__ uint32_t index(uint32_t i, int word, uint32_t n) const {
+        uint4 w = call(i);
+        uint32_t x = word == 0 ? w.x : (word == 1 ? w.y : (word == 2 ? w.z : w.w));
+        return (uint32_t)(((uint64_t)x * n) >> 32);
+    }
+    __device__ double unif(uint32_t i, int half) const {
+        uint4 w = call(i);
+        return half ? u53(w.z, w.w) : u53(w.x, w.y);
+    }
+    __device__ double normal(uint32_t i, int which) const {
+        uint4 w = call(i);
+        double u1 = u53(w.x, w.y), u2 = u53(w.z, w.w);
+        double r = sqrt(-2. * log(u1));
+        double sn, cs;
+        sincospi(2. * u2, &sn, &cs);
+        return which ? r * sn : r * cs;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NT / kWave; ++i) t += sh[i];
+    return t;  // valid on thread 0
+}
+
+}  // namespace spimc

@@ -1,0 +1,1293 @@
+// spimc_engine.hip -- context management and C ABI (include/spimc.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spimc.h"
+#include "spimc_kernels.h"
+
+using namespace spimc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(SPIMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kRhoKpt = 2;                       // k-vectors per thread in k_rho
+constexpr int kRhoChunk = kRhoBlock * kRhoKpt;   // k-vectors per k_rho workgroup
+constexpr int kBuildKpt = 2;
+constexpr int kRoundsPerGraph = 16;
+constexpr int kAdjMaxIdx = 40;
+
+uint64_t splitmix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// Natural cubic spline through (x_i, y_i) in per-interval polynomial form
+// c[i][comp][0..3] (stride ncomp*4), s(t) = c0 + dt (c1 + dt (c2 + dt c3)).
+void spline_coefs(const double* x, const double* y, int n, int ncomp, int comp, double* c) {
+    std::vector<double> m(n, 0.);
+    if (n >= 3) {
+        std::vector<double> cp(n, 0.), dp(n, 0.);
+        for (int i = 1; i < n - 1; ++i) {
+            const double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+            const double rhs = 6. * ((y[i + 1] - y[i]) / h1 - (y[i] - y[i - 1]) / h0);
+            const double den = 2. * (h0 + h1) - h0 * cp[i - 1];
+            cp[i] = h1 / den;
+            dp[i] = (rhs - h0 * dp[i - 1]) / den;
+        }
+        for (int i = n - 2; i >= 1; --i) m[i] = dp[i] - cp[i] * m[i + 1];
+    }
+    for (int i = 0; i < n - 1; ++i) {
+        const double h = x[i + 1] - x[i];
+        double* ci = c + ((size_t)i * ncomp + comp) * 4;
+        ci[0] = y[i];
+        ci[1] = (y[i + 1] - y[i]) / h - h * (2. * m[i] + m[i + 1]) / 6.;
+        ci[2] = 0.5 * m[i];
+        ci[3] = (m[i + 1] - m[i]) / (6. * h);
+    }
+}
+
+bool grid_is_uniform(const double* x, int n) {
+    if (n < 3) return true;
+    const double h = (x[n - 1] - x[0]) / (n - 1);
+    for (int i = 0; i < n; ++i)
+        if (std::fabs(x[i] - (x[0] + i * h)) > 1e-9 * std::fabs(h)) return false;
+    return true;
+}
+
+// Host-side spline staged for upload.
+struct SplHost {
+    std::vector<double> x, c;
+    int n = 0, ncomp = 0;
+    bool uniform = false;
+};
+
+SplHost make_spline(const double* x, int n, int ncomp, const std::vector<const double*>& ys) {
+    SplHost s;
+    s.n = n;
+    s.ncomp = ncomp;
+    s.x.assign(x, x + n);
+    s.c.assign((size_t)(n - 1) * ncomp * 4, 0.);
+    for (int k = 0; k < ncomp; ++k) spline_coefs(x, ys[k], n, ncomp, k, s.c.data());
+    s.uniform = grid_is_uniform(x, n);
+    return s;
+}
+
+// FreeSpline tables (free_spline_class.h:25-63): image action or its tau derivative
+// on 10000 uniform points over [-L/2, L/2].
+SplHost make_free_spline(double L, int n_images, double lambda, double tau, bool dtau) {
+    const int num = 10000;
+    const double t_l = (L == 0.) ? 1000. : L;
+    const double start = -t_l / 2., end = t_l / 2., dr = (end - start) / (num - 1);
+    const double i4lt = 1. / (4. * lambda * tau);
+    std::vector<double> xs(num), ia(num, 0.), dia(num, 0.);
+    for (int i = 0; i < num; ++i) {
+        const double r = start + i * dr;
+        xs[i] = r;
+        const double r2l = r * r * i4lt;
+        for (int im = 1; im <= n_images; ++im) {
+            const double rp = r + im * t_l, rm = r - im * t_l;
+            const double dp = r2l - rp * rp * i4lt, dm = r2l - rm * rm * i4lt;
+            const double ep = std::exp(dp), em = std::exp(dm);
+            ia[i] += ep + em;
+            dia[i] += (dp * ep + dm * em) / tau;
+        }
+        dia[i] = dia[i] / (1. + ia[i]);
+        ia[i] = -std::log1p(ia[i]);
+    }
+    return make_spline(xs.data(), num, 1, {dtau ? dia.data() : ia.data()});
+}
+
+// Bump allocator over one device arena (sizing pass with base == nullptr).
+struct Carver {
+    char* base = nullptr;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = base ? (T*)(base + off) : nullptr;
+        off += n * sizeof(T);
+        return p;
+    }
+};
+
+struct RepHost {
+    spimc_system_desc d{};
+    std::vector<int32_t> k_n;
+    std::vector<int> kidx;
+    std::vector<double> kmag, coef_s, coef_b;
+    SplHost v_r, v_long_r, u, du;
+    std::vector<SplHost> kin, bis;
+    SplHost kin_dtau;
+    Rep dev{};
+    char* arena = nullptr;
+    unsigned char* accepted = nullptr;
+    double* tape = nullptr;
+    unsigned long long* gr_cnt = nullptr;
+    uint32_t accepted_cap = 0;
+    int n_chunks = 1;
+};
+
+struct TimedLaunch {
+    int kernel;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct spimc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<RepHost> reps;
+    Rep* d_reps = nullptr;
+    // work lists
+    uint2 *d_pair_bis = nullptr, *d_rho_bis = nullptr, *d_pair_dis = nullptr, *d_rho_dis = nullptr;
+    int n_pair_bis = 0, n_rho_bis = 0, n_pair_dis = 0, n_rho_dis = 0;
+    uint2* d_items_tmp = nullptr;
+    size_t items_tmp_cap = 0;
+    unsigned int* d_counts = nullptr;
+    double* d_tmp = nullptr;  // interface partials + result
+    size_t tmp_cap = 0;
+    bool any_long = false;
+    int max_M = 0, max_N = 0, max_nk = 0;
+    // graphs
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // interface proposal
+    bool prop_set = false, rho_updated = false;
+    int prop_rep = 0, prop_p = 0, prop_first = 0, prop_count = 0;
+    // observables
+    int energy_estimator = SPIMC_ENERGY_THERMAL;
+    uint32_t virial_window = 0;
+    double sk_k_cut = 1e300, gr_r_min = 0., gr_r_max = 0.;
+    uint32_t gr_n_r = 100;
+    double* d_gr_rmax = nullptr;
+    bool adjoint_fresh = false;
+    // timing
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> pool;
+    double t_ms[SPIMC_KERNEL_COUNT] = {0};
+    uint64_t t_n[SPIMC_KERNEL_COUNT] = {0};
+    double t_bytes[SPIMC_KERNEL_COUNT] = {0};
+    unsigned long long* d_work = nullptr;
+    unsigned long long work_seen[WORK_COUNT] = {0, 0};
+};
+
+namespace {
+
+hipEvent_t take_event(spimc_ctx* c) {
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch helper: brackets the launch with events when timing is on.
+template <class F>
+int launch(spimc_ctx* c, int kernel, F&& f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c->timing) {
+        a = take_event(c);
+        b = take_event(c);
+        if (a) (void)hipEventRecord(a, c->stream);
+    }
+    f();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SPIMC_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    if (c->timing && a && b) {
+        (void)hipEventRecord(b, c->stream);
+        c->pending.push_back({kernel, a, b});
+    }
+    return SPIMC_OK;
+}
+
+int sync_and_resolve(spimc_ctx* c) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& t : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            c->t_ms[t.kernel] += ms;
+            c->t_n[t.kernel] += 1;
+        }
+        c->pool.push_back(t.a);
+        c->pool.push_back(t.b);
+    }
+    c->pending.clear();
+    if (c->timing && c->d_work) {
+        unsigned long long w[WORK_COUNT];
+        HIPCHK(hipMemcpy(w, c->d_work, sizeof(w), hipMemcpyDeviceToHost));
+        c->t_bytes[SPIMC_KERNEL_RHO] += 32.0 * (double)(w[WORK_RHO_SLICEK] - c->work_seen[WORK_RHO_SLICEK]);
+        c->t_bytes[SPIMC_KERNEL_PAIR] += (double)(w[WORK_PAIR_LINKS] - c->work_seen[WORK_PAIR_LINKS]);
+        for (int i = 0; i < WORK_COUNT; ++i) c->work_seen[i] = w[i];
+    }
+    return SPIMC_OK;
+}
+
+int check_rep(spimc_ctx* c, uint32_t r) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    if (r >= c->reps.size()) return fail(SPIMC_ERR_ARG, "replica index out of range");
+    return SPIMC_OK;
+}
+
+int validate(const spimc_system_desc& s, uint32_t r) {
+    char buf[160];
+    auto bad = [&](int code, const char* what) {
+        snprintf(buf, sizeof buf, "replica %u: %s", r, what);
+        return fail(code, buf);
+    };
+    if (s.n_d != 3) return bad(SPIMC_ERR_UNSUPPORTED, "n_d must be 3");
+    if (s.n_bead < 2 || s.n_part < 1) return bad(SPIMC_ERR_ARG, "n_bead >= 2 and n_part >= 1 required");
+    if (!(s.L > 0.)) return bad(SPIMC_ERR_UNSUPPORTED, "periodic box with L > 0 required");
+    if (!(s.beta > 0.) || !(s.lambda > 0.)) return bad(SPIMC_ERR_ARG, "beta and lambda must be positive");
+    int max_lvl = (int)std::floor(std::log2((double)s.n_bead));
+    if (s.bisect_n_level < 1 || (int)s.bisect_n_level > std::min(max_lvl, kMaxLevel))
+        return bad(SPIMC_ERR_UNSUPPORTED, "bisect n_level must be in [1, min(6, floor(log2 n_bead))]");
+    if (s.pair_max_level != 0) return bad(SPIMC_ERR_UNSUPPORTED, "pair action max_level must be 0");
+    if (s.kinetic_n_images < 0 || s.bisect_n_images < 0) return bad(SPIMC_ERR_ARG, "n_images must be >= 0");
+    if (s.pair_type < SPIMC_PAIR_NONE || s.pair_type > SPIMC_PAIR_DAVID) return bad(SPIMC_ERR_ARG, "unknown pair type");
+    if (s.pair_type == SPIMC_PAIR_BARE) {
+        if (!s.bare_r || !s.bare_v_r || s.bare_n_r < 3) return bad(SPIMC_ERR_ARG, "bare v/diag table missing");
+        if (s.pair_use_long_range && (!s.bare_r_long || !s.bare_v_long_r || s.bare_n_r_long < 3))
+            return bad(SPIMC_ERR_ARG, "bare long-range table missing");
+    }
+    if (s.pair_type == SPIMC_PAIR_DAVID) {
+        if (!s.david_grid || !s.david_u_kj || !s.david_du_kj || s.david_n_grid < 3)
+            return bad(SPIMC_ERR_ARG, "david table missing");
+        if (s.david_n_order > 3) return bad(SPIMC_ERR_UNSUPPORTED, "david n_order must be <= 3");
+    }
+    if (s.pair_type != SPIMC_PAIR_NONE && s.pair_use_long_range) {
+        if (s.n_k == 0 || !s.k_n || !s.pair_v_long_k) return bad(SPIMC_ERR_ARG, "k-vectors and v_long_k required");
+        for (uint32_t k = 0; k < 3 * s.n_k; ++k)
+            if (std::abs(s.k_n[k]) > kMaxIdx) return bad(SPIMC_ERR_UNSUPPORTED, "|n| of a k-vector must be <= 63");
+        if ((s.n_k + kRhoChunk - 1) / kRhoChunk > 255) return bad(SPIMC_ERR_UNSUPPORTED, "too many k-vectors");
+    }
+    return SPIMC_OK;
+}
+
+Spl carve_spline(Carver& cv, const SplHost& h) {
+    Spl s{};
+    s.n = h.n;
+    s.ncomp = h.ncomp;
+    s.uniform = h.uniform ? 1 : 0;
+    if (h.n > 1) {
+        s.x0 = h.x[0];
+        s.inv_h = (h.n - 1) / (h.x[h.n - 1] - h.x[0]);
+    }
+    s.x = cv.take<double>(h.x.size());
+    s.c = cv.take<double>(h.c.size());
+    return s;
+}
+
+int upload_spline(const Spl& s, const SplHost& h) {
+    if (!h.n) return SPIMC_OK;
+    HIPCHK(hipMemcpy((void*)s.x, h.x.data(), h.x.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void*)s.c, h.c.data(), h.c.size() * sizeof(double), hipMemcpyHostToDevice));
+    return SPIMC_OK;
+}
+
+// Builds the host tables of one replica and lays out its arena.
+int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t seed) {
+    RepHost& h = c->reps[r];
+    h.d = s;
+    const int N = s.n_part, M = s.n_bead;
+    const bool lr = s.pair_type != SPIMC_PAIR_NONE && s.pair_use_long_range;
+    const int nk = lr ? (int)s.n_k : 0;
+    const double tau = s.beta / (1. * M), L = s.L, vol = L * L * L, box = 2. * M_PI / L;
+    // k-space
+    h.k_n.assign(s.k_n ? s.k_n : nullptr, s.k_n ? s.k_n + 3 * (size_t)nk : nullptr);
+    h.kidx.resize(nk);
+    h.kmag.resize(nk);
+    h.coef_s.resize(nk);
+    h.coef_b.resize(nk);
+    int mx = 0;
+    for (int k = 0; k < nk; ++k) {
+        int n0 = h.k_n[3 * k], n1 = h.k_n[3 * k + 1], n2 = h.k_n[3 * k + 2];
+        mx = std::max(mx, std::max(std::abs(n0), std::max(std::abs(n1), std::abs(n2))));
+        h.kidx[k] = (n0 + 128) | ((n1 + 128) << 8) | ((n2 + 128) << 16);
+        double kv[3] = {n0 * box, n1 * box, n2 * box};
+        h.kmag[k] = std::sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+        // Bare: CalcULong multiplies the k sum by level_tau (bare_pair_action_class.h:170);
+        // David: u_long_k = v_long_k * tau (david_pair_action_class.h:519).
+        h.coef_s[k] = (s.pair_type == SPIMC_PAIR_DAVID) ? s.pair_v_long_k[k] * tau : s.pair_v_long_k[k];
+        h.coef_b[k] = s.pair_v_long_k[k];
+    }
+    // pair tables
+    int n_val = 0;
+    if (s.pair_type == SPIMC_PAIR_BARE) {
+        h.v_r = make_spline(s.bare_r, s.bare_n_r, 1, {s.bare_v_r});
+        if (lr) h.v_long_r = make_spline(s.bare_r_long, s.bare_n_r_long, 1, {s.bare_v_long_r});
+    } else if (s.pair_type == SPIMC_PAIR_DAVID) {
+        n_val = 1;
+        for (uint32_t i = 1; i <= s.david_n_order; ++i) n_val += 1 + i;
+        std::vector<const double*> yu, ydu;
+        for (int k = 0; k <= n_val; ++k) {
+            yu.push_back(s.david_u_kj + (size_t)k * s.david_n_grid);
+            ydu.push_back(s.david_du_kj + (size_t)k * s.david_n_grid);
+        }
+        h.u = make_spline(s.david_grid, s.david_n_grid, n_val + 1, yu);
+        h.du = make_spline(s.david_grid, s.david_n_grid, n_val + 1, ydu);
+    }
+    // kinetic and sampling image splines
+    h.kin.clear();
+    h.bis.clear();
+    if (s.kinetic_n_images > 0) {
+        for (int l = 0; l < kMaxKinSplines && (1 << l) <= M / 2 + 1; ++l)
+            h.kin.push_back(make_free_spline(L, s.kinetic_n_images, s.lambda, tau * (double)(1 << l), false));
+        h.kin_dtau = make_free_spline(L, s.kinetic_n_images, s.lambda, tau, true);
+    }
+    if (s.bisect_n_images > 0)
+        for (int l = 0; l < (int)s.bisect_n_level; ++l)
+            h.bis.push_back(make_free_spline(L, s.bisect_n_images, s.lambda, 0.5 * tau * (double)(1 << l), false));
+    h.n_chunks = std::max(1, (nk + kRhoChunk - 1) / kRhoChunk);
+
+    // lay out the arena (sizing pass, allocation, carving pass)
+    Rep& R = h.dev;
+    for (int pass = 0; pass < 2; ++pass) {
+        Carver cv;
+        if (pass == 1) cv.base = h.arena;
+        R.pos = cv.take<double>((size_t)3 * M * N);
+        R.rho = cv.take<double2>((size_t)2 * M * std::max(nk, 1));
+        R.sel = cv.take<unsigned char>(M);
+        R.kidx = cv.take<int>(std::max(nk, 1));
+        R.kmag = cv.take<double>(std::max(nk, 1));
+        R.coef_s = cv.take<double>(std::max(nk, 1));
+        R.coef_b = cv.take<double>(std::max(nk, 1));
+        if (h.v_r.n) R.v_r = carve_spline(cv, h.v_r);
+        if (h.v_long_r.n) R.v_long_r = carve_spline(cv, h.v_long_r);
+        if (h.u.n) R.u = carve_spline(cv, h.u);
+        if (h.du.n) R.du = carve_spline(cv, h.du);
+        for (size_t l = 0; l < h.kin.size(); ++l) R.kin[l] = carve_spline(cv, h.kin[l]);
+        if (h.kin_dtau.n) R.kin_dtau = carve_spline(cv, h.kin_dtau);
+        for (size_t l = 0; l < h.bis.size(); ++l) R.bis[l] = carve_spline(cv, h.bis[l]);
+        R.prop = cv.take<Prop>(1);
+        R.prop_new = cv.take<double>((size_t)3 * M);
+        R.pair_part = cv.take<double2>(M);
+        R.rho_part = cv.take<double2>((size_t)M * h.n_chunks);
+        R.kin_part = cv.take<double2>(1);
+        R.cnt = cv.take<Counters>(1);
+        R.tape_cursor = cv.take<long long>(1);
+        R.e_acc = cv.take<double>(2);
+        R.sk_acc = cv.take<double>(std::max(nk, 1));
+        R.scratch = cv.take<double>(scratch_size(M, std::max(nk, 1)));
+        R.grad = cv.take<double>((size_t)3 * M * N);
+        R.grad_est = cv.take<double>((size_t)3 * M * N);
+        R.n_measure = cv.take<unsigned long long>(3);
+        if (pass == 0) {
+            hipError_t e = hipMalloc((void**)&h.arena, cv.off + 256);
+            if (e != hipSuccess) return fail(SPIMC_ERR_NOMEM, "device arena allocation failed");
+            HIPCHK(hipMemset(h.arena, 0, cv.off + 256));
+        }
+    }
+    // scalars
+    R.N = N;
+    R.M = M;
+    R.nk = nk;
+    R.max_idx = mx;
+    R.n_level = s.bisect_n_level;
+    R.pair_type = s.pair_type;
+    R.use_long_range = lr ? 1 : 0;
+    R.is_coulomb = s.bare_is_coulomb ? 1 : 0;
+    R.n_order = s.david_n_order;
+    R.n_val = n_val;
+    R.kin_images = s.kinetic_n_images;
+    R.bis_images = s.bisect_n_images;
+    R.n_links_max = M;
+    R.n_chunks = h.n_chunks;
+    R.index = r;
+    R.L = L;
+    R.iL = 1. / L;
+    R.tau = tau;
+    R.lambda = s.lambda;
+    R.step = s.displace_step;
+    R.box = box;
+    R.kin_i4lt = 1. / (4. * s.lambda * tau);
+    R.kin_i4ltt = 1. / (4. * s.lambda * tau * tau);
+    R.kin_dbeta_const = N * M * 3 / (2. * tau);
+    R.long_scale = (s.pair_type == SPIMC_PAIR_BARE) ? tau : 1.;
+    const double nn = (double)N;
+    // homologous constants (bare_pair_action_class.h:89-95, david_pair_action_class.h:533-543)
+    R.const_v = lr ? s.pair_v_long_k_0 * 0.5 * nn * nn * M + s.pair_v_long_r_0 * (-0.5) * nn * M : 0.;
+    R.const_du = R.const_v;
+    if (s.pair_type == SPIMC_PAIR_BARE) {
+        R.r_v_min = s.bare_r[0];
+        R.r_v_max = s.bare_r[s.bare_n_r - 1];
+        if (lr) {
+            R.r_l_min = s.bare_r_long[0];
+            R.r_l_max = s.bare_r_long[s.bare_n_r_long - 1];
+        }
+    } else if (s.pair_type == SPIMC_PAIR_DAVID) {
+        R.g_min = s.david_grid[0];
+        R.g_max = s.david_grid[s.david_n_grid - 1];
+    }
+    R.key = splitmix64(seed ^ (0x9E3779B97F4A7C15ull * (r + 1)));
+    R.tape = nullptr;
+    R.tape_len = 0;
+    R.accepted = nullptr;
+    R.accepted_cap = 0;
+    R.gr_cnt = nullptr;
+    // uploads
+    if (nk) {
+        HIPCHK(hipMemcpy((void*)R.kidx, h.kidx.data(), nk * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void*)R.kmag, h.kmag.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void*)R.coef_s, h.coef_s.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void*)R.coef_b, h.coef_b.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+    }
+    int rc;
+    if ((rc = upload_spline(R.v_r, h.v_r)) || (rc = upload_spline(R.v_long_r, h.v_long_r)) ||
+        (rc = upload_spline(R.u, h.u)) || (rc = upload_spline(R.du, h.du)) || (rc = upload_spline(R.kin_dtau, h.kin_dtau)))
+        return rc;
+    for (size_t l = 0; l < h.kin.size(); ++l)
+        if ((rc = upload_spline(R.kin[l], h.kin[l]))) return rc;
+    for (size_t l = 0; l < h.bis.size(); ++l)
+        if ((rc = upload_spline(R.bis[l], h.bis[l]))) return rc;
+    return SPIMC_OK;
+}
+
+int upload_reps(spimc_ctx* c) {
+    std::vector<Rep> v;
+    for (auto& h : c->reps) v.push_back(h.dev);
+    HIPCHK(hipMemcpy(c->d_reps, v.data(), v.size() * sizeof(Rep), hipMemcpyHostToDevice));
+    return SPIMC_OK;
+}
+
+int upload_items(const std::vector<uint2>& v, uint2** d, int* n) {
+    *n = (int)v.size();
+    if (v.empty()) return SPIMC_OK;
+    HIPCHK(hipMalloc((void**)d, v.size() * sizeof(uint2)));
+    HIPCHK(hipMemcpy(*d, v.data(), v.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    return SPIMC_OK;
+}
+
+int build_work_lists(spimc_ctx* c) {
+    std::vector<uint2> pb, rb, pd, rd;
+    for (uint32_t r = 0; r < c->reps.size(); ++r) {
+        const Rep& R = c->reps[r].dev;
+        const int nb = 1 << R.n_level;
+        for (int j = 0; j < nb; ++j) pb.push_back(make_uint2(r, j));
+        for (int j = 0; j < R.M; ++j) pd.push_back(make_uint2(r, j));
+        if (R.use_long_range) {
+            for (int m = 0; m < nb - 1; ++m)
+                for (int ch = 0; ch < R.n_chunks; ++ch) rb.push_back(make_uint2(r, (m << 8) | ch));
+            for (int m = 0; m < R.M; ++m)
+                for (int ch = 0; ch < R.n_chunks; ++ch) rd.push_back(make_uint2(r, (m << 8) | ch));
+        }
+    }
+    int rc;
+    if ((rc = upload_items(pb, &c->d_pair_bis, &c->n_pair_bis)) || (rc = upload_items(rb, &c->d_rho_bis, &c->n_rho_bis)) ||
+        (rc = upload_items(pd, &c->d_pair_dis, &c->n_pair_dis)) || (rc = upload_items(rd, &c->d_rho_dis, &c->n_rho_dis)))
+        return rc;
+    return SPIMC_OK;
+}
+
+int ensure_tmp(spimc_ctx* c, size_t n_doubles, size_t n_items) {
+    if (n_doubles > c->tmp_cap) {
+        if (c->d_tmp) (void)hipFree(c->d_tmp);
+        c->d_tmp = nullptr;
+        HIPCHK(hipMalloc((void**)&c->d_tmp, n_doubles * sizeof(double)));
+        c->tmp_cap = n_doubles;
+    }
+    if (n_items > c->items_tmp_cap) {
+        if (c->d_items_tmp) (void)hipFree(c->d_items_tmp);
+        c->d_items_tmp = nullptr;
+        HIPCHK(hipMalloc((void**)&c->d_items_tmp, n_items * sizeof(uint2)));
+        c->items_tmp_cap = n_items;
+    }
+    return SPIMC_OK;
+}
+
+// One round of a move stage for every replica.
+int launch_round(spimc_ctx* c, int move) {
+    const int R = (int)c->reps.size();
+    unsigned long long* work = c->timing ? c->d_work : nullptr;
+    Rep* reps = c->d_reps;
+    hipStream_t st = c->stream;
+    int rc;
+    if (move == 0) {
+        rc = launch(c, SPIMC_KERNEL_PROPOSE, [&] { k_propose_bisect<<<R, 64, 0, st>>>(reps, R); });
+    } else {
+        rc = launch(c, SPIMC_KERNEL_PROPOSE, [&] { k_propose_displace<<<R, 64, 0, st>>>(reps, R); });
+    }
+    if (rc) return rc;
+    const uint2* pi = move == 0 ? c->d_pair_bis : c->d_pair_dis;
+    const int np = move == 0 ? c->n_pair_bis : c->n_pair_dis;
+    const uint2* ri = move == 0 ? c->d_rho_bis : c->d_rho_dis;
+    const int nr = move == 0 ? c->n_rho_bis : c->n_rho_dis;
+    if (np) {
+        const int g = (np + 3) / 4;
+        rc = launch(c, SPIMC_KERNEL_PAIR, [&] { k_pair<<<g, kPairBlock, 0, st>>>(reps, pi, np, 0, work); });
+        if (rc) return rc;
+    }
+    if (nr) {
+        rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_rho<kRhoKpt><<<nr, kRhoBlock, 0, st>>>(reps, ri, nr, 0, work); });
+        if (rc) return rc;
+    }
+    return launch(c, SPIMC_KERNEL_DECIDE, [&] { k_decide<<<R, 64, 0, st>>>(reps, R); });
+}
+
+int ensure_accepted(spimc_ctx* c, uint32_t need) {
+    bool changed = false;
+    for (auto& h : c->reps)
+        if (h.accepted_cap < need) {
+            if (h.accepted) (void)hipFree(h.accepted);
+            h.accepted = nullptr;
+            uint32_t cap = std::max<uint32_t>(need, 1024);
+            HIPCHK(hipMalloc((void**)&h.accepted, cap));
+            h.accepted_cap = cap;
+            h.dev.accepted = h.accepted;
+            h.dev.accepted_cap = cap;
+            changed = true;
+        }
+    if (changed) {
+        int rc = upload_reps(c);
+        if (rc) return rc;
+        // captured graphs hold no pointers from the Rep table, so they stay valid
+    }
+    return SPIMC_OK;
+}
+
+int capture_round_graph(spimc_ctx* c, int move) {
+    if (c->graph[move]) return SPIMC_OK;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    bool saved = c->timing;
+    c->timing = false;
+    int rc = SPIMC_OK;
+    for (int i = 0; i < kRoundsPerGraph && rc == SPIMC_OK; ++i) rc = launch_round(c, move);
+    c->timing = saved;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SPIMC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    HIPCHK(hipGraphInstantiate(&c->graph[move], g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    return SPIMC_OK;
+}
+
+int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform, uint8_t* accepted, bool use_graph) {
+    if (c->prop_set) return fail(SPIMC_ERR_STATE, "an interface proposal is pending; accept or reject it first");
+    const int R = (int)c->reps.size();
+    std::vector<unsigned int> cnt(R);
+    uint32_t mx = 0;
+    for (int r = 0; r < R; ++r) {
+        cnt[r] = counts ? counts[r] : n_uniform;
+        mx = std::max(mx, cnt[r]);
+    }
+    if (mx == 0) return SPIMC_OK;
+    int rc = ensure_accepted(c, mx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_counts, cnt.data(), R * sizeof(unsigned int), hipMemcpyHostToDevice, c->stream));
+    rc = launch(c, SPIMC_KERNEL_DECIDE, [&] { k_stage_begin<<<(R + 63) / 64, 64, 0, c->stream>>>(c->d_reps, R, c->d_counts); });
+    if (rc) return rc;
+    if (use_graph && !c->timing) {
+        rc = capture_round_graph(c, move);
+        if (rc) return rc;
+        for (uint32_t t = 0; t < mx; t += kRoundsPerGraph) HIPCHK(hipGraphLaunch(c->graph[move], c->stream));
+    } else {
+        for (uint32_t t = 0; t < mx; ++t)
+            if ((rc = launch_round(c, move))) return rc;
+    }
+    if ((rc = sync_and_resolve(c))) return rc;
+    c->adjoint_fresh = false;
+    // replay errors
+    for (int r = 0; r < R; ++r) {
+        Counters k;
+        HIPCHK(hipMemcpy(&k, c->reps[r].dev.cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+        if (k.error) {
+            unsigned long long z = 0;
+            HIPCHK(hipMemcpy(&c->reps[r].dev.cnt->error, &z, sizeof z, hipMemcpyHostToDevice));
+            char b[128];
+            snprintf(b, sizeof b, "replica %d: replay tape exhausted or invalid draw", r);
+            return fail(SPIMC_ERR_STATE, b);
+        }
+    }
+    if (accepted)
+        for (int r = 0; r < R; ++r)
+            HIPCHK(hipMemcpy(accepted + (size_t)r * mx, c->reps[r].accepted, cnt[r], hipMemcpyDeviceToHost));
+    return SPIMC_OK;
+}
+
+// Rebuild rho_k of one replica from its committed positions (buffer sel[b] = 0).
+int rho_rebuild(spimc_ctx* c, uint32_t r) {
+    const Rep& R = c->reps[r].dev;
+    if (!R.use_long_range) return SPIMC_OK;
+    HIPCHK(hipMemsetAsync(R.sel, 0, R.M, c->stream));
+    const int chunks = (R.nk + 256 * kBuildKpt - 1) / (256 * kBuildKpt);
+    dim3 g(R.M, chunks);
+    return launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_rho_build<kBuildKpt><<<g, 256, 0, c->stream>>>(c->d_reps, (int)r); });
+}
+
+// ---- estimator drivers (launch over all replicas or one) -------------------
+// `base` offsets the replica table so that gridDim.y counts replicas from it.
+int est_kin_dbeta(spimc_ctx* c, Rep* base, int nrep) {
+    dim3 g(c->max_M, nrep);
+    return launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_kin_dbeta<<<g, 256, 0, c->stream>>>(base, kSlotKin); });
+}
+int est_pair_whole(spimc_ctx* c, Rep* base, int nrep, int which) {
+    dim3 g(c->max_M, nrep);
+    int rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_pair_whole<<<g, 256, 0, c->stream>>>(base, which, kSlotPair); });
+    if (rc || !c->any_long) return rc;
+    dim3 g2((c->max_nk + 255) / 256, nrep);
+    return launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_long_whole<<<g2, 256, 0, c->stream>>>(base); });
+}
+int est_finalize(spimc_ctx* c, Rep* base, int nrep, int what, int acc) {
+    return launch(c, SPIMC_KERNEL_ESTIMATORS,
+                  [&] { k_finalize<<<(nrep + 63) / 64, 64, 0, c->stream>>>(base, nrep, what, acc); });
+}
+int est_gradient(spimc_ctx* c, Rep* base, int nrep, uint32_t terms, int compat, int out_est) {
+    dim3 g((c->max_N + 63) / 64, c->max_M, nrep);
+    if (compat) return launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_grad_compat<<<g, 64, 0, c->stream>>>(base, out_est); });
+    int rc = launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_adjoint_local<<<g, 64, 0, c->stream>>>(base, terms, out_est); });
+    if (rc || !(terms & SPIMC_GRAD_PAIR_LONG) || !c->any_long) return rc;
+    int mx = 0;
+    for (auto& h : c->reps) mx = std::max(mx, h.dev.max_idx);
+    if (mx > kAdjMaxIdx) return fail(SPIMC_ERR_UNSUPPORTED, "adjoint long range supports |n| <= 40");
+    const size_t shmem = (size_t)64 * 3 * (mx + 1) * sizeof(double2) + kAdjTile * (sizeof(double2) + sizeof(int) + sizeof(double));
+    return launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_adjoint_long<<<g, 64, shmem, c->stream>>>(base, out_est); });
+}
+
+int energy_accumulate(spimc_ctx* c, Rep* base, int nrep) {
+    int rc;
+    if (c->energy_estimator == SPIMC_ENERGY_THERMAL) {
+        if ((rc = est_kin_dbeta(c, base, nrep)) || (rc = est_finalize(c, base, nrep, 0, 0)) ||
+            (rc = est_pair_whole(c, base, nrep, 0)) || (rc = est_finalize(c, base, nrep, 1, 1)))
+            return rc;
+        return SPIMC_OK;
+    }
+    const int W = (int)c->virial_window;
+    rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_kin_virial<<<nrep, 256, 0, c->stream>>>(base, W, kSlotVirial); });
+    if (rc) return rc;
+    int from_sweep = 0;
+    if (c->energy_estimator == SPIMC_ENERGY_VIRIAL_COMPAT) {
+        rc = est_gradient(c, base, nrep, SPIMC_GRAD_PAIR_SHORT, 1, 1);
+    } else if (c->adjoint_fresh) {
+        from_sweep = 1;  // dS/dr of the sweep holds the kinetic term too; the centroid kernel removes it
+    } else {
+        rc = est_gradient(c, base, nrep, SPIMC_GRAD_PAIR_SHORT | SPIMC_GRAD_PAIR_LONG, 0, 1);
+    }
+    if (rc) return rc;
+    dim3 g(c->max_M, nrep);
+    rc = launch(c, SPIMC_KERNEL_ESTIMATORS,
+                [&] { k_centroid<<<g, 256, 0, c->stream>>>(base, kSlotCentroid, from_sweep); });
+    if (rc) return rc;
+    if ((rc = est_finalize(c, base, nrep, 2, 0)) || (rc = est_pair_whole(c, base, nrep, 0)) ||
+        (rc = est_finalize(c, base, nrep, 1, 1)))
+        return rc;
+    return SPIMC_OK;
+}
+
+int obs_accumulate_all(spimc_ctx* c, uint32_t which) {
+    const int R = (int)c->reps.size();
+    int rc;
+    if (which & SPIMC_OBS_ENERGY)
+        if ((rc = energy_accumulate(c, c->d_reps, R))) return rc;
+    if ((which & SPIMC_OBS_SK) && c->any_long) {
+        dim3 g((c->max_nk + 255) / 256, R);
+        const double kc = c->sk_k_cut;
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_sk<<<g, 256, 0, c->stream>>>(c->d_reps, kc); });
+        if (rc) return rc;
+    }
+    if (which & SPIMC_OBS_GR) {
+        dim3 g(c->max_M, R);
+        const double rmin = c->gr_r_min;
+        const int nr = (int)c->gr_n_r;
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS,
+                    [&] { k_gr<<<g, 256, 0, c->stream>>>(c->d_reps, rmin, 0., nr, c->d_gr_rmax); });
+        if (rc) return rc;
+    }
+    return SPIMC_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int spimc_abi_version(void) { return SPIMC_ABI_VERSION; }
+const char* spimc_last_error(void) { return g_err.c_str(); }
+
+int spimc_kspace_build(double L, double k_cut, int32_t* k_n, uint32_t cap, uint32_t* n_k) {
+    // KSpace::Setup / Include (k_space_class.h:17-80): floating-point test on k = n * 2pi/L
+    if (!(L > 0.) || !(k_cut >= 0.) || !n_k) return fail(SPIMC_ERR_ARG, "bad k-space arguments");
+    const double box = 2. * M_PI / L;
+    const int mx = (int)(uint32_t)std::ceil(1.1 * k_cut / box);
+    uint32_t cnt = 0;
+    for (int a = -mx; a <= mx; ++a)
+        for (int b = -mx; b <= mx; ++b)
+            for (int z = -mx; z <= mx; ++z) {
+                const double kx = a * box, ky = b * box, kz = z * box;
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                if (!(k2 < k_cut * k_cut) || k2 == 0.) continue;
+                const bool half = (kx > 0.) || (kx == 0. && ky > 0.) || (kx == 0. && ky == 0. && kz > 0.);
+                if (!half) continue;
+                if (k_n && cnt < cap) {
+                    k_n[3 * cnt] = a;
+                    k_n[3 * cnt + 1] = b;
+                    k_n[3 * cnt + 2] = z;
+                }
+                ++cnt;
+            }
+    *n_k = cnt;
+    return SPIMC_OK;
+}
+
+int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64_t seed, spimc_ctx** out) {
+    if (!descs || !out || n_replicas == 0) return fail(SPIMC_ERR_ARG, "null descriptors or zero replicas");
+    *out = nullptr;
+    for (uint32_t r = 0; r < n_replicas; ++r) {
+        int rc = validate(descs[r], r);
+        if (rc) return rc;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SPIMC_ERR_NODEVICE, "no HIP device");
+    spimc_ctx* c = new spimc_ctx();
+    auto bail = [&](int rc) {
+        spimc_ctx_destroy(c);
+        return rc;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(SPIMC_ERR_HIP, "hipGetDevice"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(SPIMC_ERR_HIP, "stream creation failed"));
+    c->reps.resize(n_replicas);
+    for (uint32_t r = 0; r < n_replicas; ++r) {
+        int rc = build_replica(c, r, descs[r], seed);
+        if (rc) return bail(rc);
+        const Rep& R = c->reps[r].dev;
+        c->any_long = c->any_long || R.use_long_range;
+        c->max_M = std::max(c->max_M, R.M);
+        c->max_N = std::max(c->max_N, R.N);
+        c->max_nk = std::max(c->max_nk, R.nk);
+    }
+    if (hipMalloc((void**)&c->d_reps, n_replicas * sizeof(Rep)) != hipSuccess) return bail(fail(SPIMC_ERR_NOMEM, "rep table"));
+    if (hipMalloc((void**)&c->d_counts, n_replicas * sizeof(unsigned int)) != hipSuccess)
+        return bail(fail(SPIMC_ERR_NOMEM, "counts"));
+    if (hipMalloc((void**)&c->d_work, WORK_COUNT * sizeof(unsigned long long)) != hipSuccess)
+        return bail(fail(SPIMC_ERR_NOMEM, "work counters"));
+    if (hipMemset(c->d_work, 0, WORK_COUNT * sizeof(unsigned long long)) != hipSuccess)
+        return bail(fail(SPIMC_ERR_HIP, "memset"));
+    if (hipMalloc((void**)&c->d_gr_rmax, n_replicas * sizeof(double)) != hipSuccess)
+        return bail(fail(SPIMC_ERR_NOMEM, "g(r) table"));
+    int rc = upload_reps(c);
+    if (rc) return bail(rc);
+    if ((rc = build_work_lists(c))) return bail(rc);
+    if ((rc = spimc_obs_configure(c, SPIMC_ENERGY_THERMAL, 0, 1e300, 0., 0., 100))) return bail(rc);
+    *out = c;
+    return SPIMC_OK;
+}
+
+int spimc_ctx_destroy(spimc_ctx* c) {
+    if (!c) return SPIMC_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& g : c->graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    for (auto& h : c->reps) {
+        if (h.arena) (void)hipFree(h.arena);
+        if (h.accepted) (void)hipFree(h.accepted);
+        if (h.tape) (void)hipFree(h.tape);
+        if (h.gr_cnt) (void)hipFree(h.gr_cnt);
+    }
+    for (void* p : {(void*)c->d_reps, (void*)c->d_counts, (void*)c->d_work, (void*)c->d_gr_rmax, (void*)c->d_tmp,
+                    (void*)c->d_items_tmp, (void*)c->d_pair_bis, (void*)c->d_rho_bis, (void*)c->d_pair_dis,
+                    (void*)c->d_rho_dis})
+        if (p) (void)hipFree(p);
+    for (auto& t : c->pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (auto& e : c->pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SPIMC_OK;
+}
+
+int spimc_ctx_num_replicas(const spimc_ctx* c, uint32_t* out) {
+    if (!c || !out) return fail(SPIMC_ERR_ARG, "null argument");
+    *out = (uint32_t)c->reps.size();
+    return SPIMC_OK;
+}
+
+int spimc_path_upload(spimc_ctx* c, uint32_t r, const double* pos) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!pos) return fail(SPIMC_ERR_ARG, "null positions");
+    const Rep& R = c->reps[r].dev;
+    HIPCHK(hipMemcpyAsync(R.pos, pos, (size_t)3 * R.M * R.N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((rc = rho_rebuild(c, r))) return rc;
+    c->adjoint_fresh = false;
+    return sync_and_resolve(c);
+}
+
+int spimc_path_download(spimc_ctx* c, uint32_t r, double* pos) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!pos) return fail(SPIMC_ERR_ARG, "null positions");
+    const Rep& R = c->reps[r].dev;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(pos, R.pos, (size_t)3 * R.M * R.N * sizeof(double), hipMemcpyDeviceToHost));
+    return SPIMC_OK;
+}
+
+int spimc_rho_k_download(spimc_ctx* c, uint32_t r, double* rho) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!rho) return fail(SPIMC_ERR_ARG, "null output");
+    const Rep& R = c->reps[r].dev;
+    if (!R.use_long_range) return fail(SPIMC_ERR_STATE, "replica has no long-range action");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<unsigned char> sel(R.M);
+    HIPCHK(hipMemcpy(sel.data(), R.sel, R.M, hipMemcpyDeviceToHost));
+    for (int b = 0; b < R.M; ++b)
+        HIPCHK(hipMemcpy(rho + (size_t)2 * b * R.nk, R.rho + ((size_t)sel[b] * R.M + b) * R.nk,
+                         (size_t)R.nk * sizeof(double2), hipMemcpyDeviceToHost));
+    return SPIMC_OK;
+}
+
+int spimc_rho_k_rebuild(spimc_ctx* c, uint32_t r) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if ((rc = rho_rebuild(c, r))) return rc;
+    return sync_and_resolve(c);
+}
+
+// ---- Action interface --------------------------------------------------------
+
+int spimc_set_proposal(spimc_ctx* c, uint32_t r, uint32_t particle, uint32_t b_first, uint32_t n_beads, const double* r_new) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    const Rep& R = c->reps[r].dev;
+    if (particle >= (uint32_t)R.N || n_beads == 0 || n_beads > (uint32_t)R.M || !r_new)
+        return fail(SPIMC_ERR_ARG, "bad proposal");
+    if (c->prop_set && c->prop_rep != (int)r) return fail(SPIMC_ERR_STATE, "a proposal on another replica is pending");
+    HIPCHK(hipMemcpy(R.prop_new, r_new, (size_t)3 * n_beads * sizeof(double), hipMemcpyHostToDevice));
+    c->prop_set = true;
+    c->rho_updated = false;
+    c->prop_rep = r;
+    c->prop_p = particle;
+    c->prop_first = b_first % R.M;
+    c->prop_count = n_beads;
+    return SPIMC_OK;
+}
+
+static int write_iface_prop(spimc_ctx* c, uint32_t r, int p, int b0, int n_links) {
+    Prop pr{};
+    pr.status = 0;
+    pr.move = 2;
+    pr.p = p;
+    pr.b0 = b0;
+    pr.n_links = n_links;
+    const bool mine = c->prop_set && c->prop_rep == (int)r && c->prop_p == p;
+    pr.first = mine ? c->prop_first : 0;
+    pr.count = mine ? c->prop_count : 0;
+    HIPCHK(hipMemcpy(c->reps[r].dev.prop, &pr, sizeof(Prop), hipMemcpyHostToDevice));
+    return SPIMC_OK;
+}
+
+static int finish_proposal(spimc_ctx* c, uint32_t r, int accept) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!c->prop_set || c->prop_rep != (int)r) return fail(SPIMC_ERR_STATE, "no proposal pending on this replica");
+    if ((rc = write_iface_prop(c, r, c->prop_p, c->prop_first, 0))) return rc;
+    const int upd = c->rho_updated ? 1 : 0;
+    rc = launch(c, SPIMC_KERNEL_DECIDE, [&] { k_commit<<<1, 256, 0, c->stream>>>(c->d_reps, (int)r, accept, upd); });
+    if (rc) return rc;
+    c->prop_set = false;
+    c->rho_updated = false;
+    c->adjoint_fresh = false;
+    return sync_and_resolve(c);
+}
+
+int spimc_accept_proposal(spimc_ctx* c, uint32_t r) { return finish_proposal(c, r, 1); }
+int spimc_reject_proposal(spimc_ctx* c, uint32_t r) { return finish_proposal(c, r, 0); }
+
+int spimc_get_action(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0, uint32_t b1, const uint32_t* parts,
+                     uint32_t n_parts, uint32_t level, double* out) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!out || (n_parts && !parts)) return fail(SPIMC_ERR_ARG, "null argument");
+    if (n_parts > 1) return fail(SPIMC_ERR_UNSUPPORTED, "at most one moved particle per query");
+    if (b1 < b0) return fail(SPIMC_ERR_ARG, "b1 < b0");
+    const Rep& R = c->reps[r].dev;
+    if (n_parts == 1 && parts[0] >= (uint32_t)R.N) return fail(SPIMC_ERR_ARG, "particle index out of range");
+    if (b0 >= (uint32_t)R.M || b1 - b0 > (uint32_t)R.M) return fail(SPIMC_ERR_ARG, "window out of range");
+    *out = 0.;
+    if (n_parts == 0) return SPIMC_OK;
+    const int p = (int)parts[0];
+    const uint32_t skip = 1u << level;
+    if ((rc = ensure_tmp(c, (size_t)R.M + 8, (size_t)R.M * std::max(R.n_chunks, 1)))) return rc;
+    double* d_out = c->d_tmp + R.M;
+    if (action == SPIMC_ACTION_KINETIC) {
+        if (level >= kMaxKinSplines || (R.kin_images > 0 && level >= c->reps[r].kin.size()))
+            return fail(SPIMC_ERR_UNSUPPORTED, "kinetic level out of range");
+        const uint32_t span = (b1 - b0) % (uint32_t)R.M;
+        if (span % skip) return fail(SPIMC_ERR_ARG, "window not a multiple of the level step");
+        const int nl = (int)(span / skip);
+        if ((rc = write_iface_prop(c, r, p, (int)b0, 0))) return rc;
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] {
+            k_kinetic_window<<<1, 64, 0, c->stream>>>(c->d_reps, (int)r, mode, p, (int)b0, nl, (int)level, d_out);
+        });
+    } else {
+        // PairAction::GetAction: 0 above max_level (= 0), and for is_constant
+        if (level > 0 || R.pair_type == SPIMC_PAIR_NONE || R.N == 1) return SPIMC_OK;
+        const int nl = (int)(b1 - b0);
+        if ((rc = write_iface_prop(c, r, p, (int)b0, nl))) return rc;
+        std::vector<uint2> items;
+        for (int j = 0; j < nl; ++j) items.push_back(make_uint2(r, j));
+        if (!items.empty()) {
+            HIPCHK(hipMemcpy(c->d_items_tmp, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
+            const int g = (nl + 3) / 4;
+            rc = launch(c, SPIMC_KERNEL_PAIR, [&] { k_pair<<<g, kPairBlock, 0, c->stream>>>(c->d_reps, c->d_items_tmp, nl, 1, nullptr); });
+            if (rc) return rc;
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        int n_long = 0;
+        if (R.use_long_range) {
+            const bool mine = c->prop_set && c->prop_rep == (int)r && c->prop_p == p;
+            if (mode == SPIMC_MODE_NEW && mine && !c->rho_updated) {
+                std::vector<uint2> ri;
+                for (int m = 0; m < c->prop_count; ++m)
+                    for (int ch = 0; ch < R.n_chunks; ++ch) ri.push_back(make_uint2(r, (m << 8) | ch));
+                if ((rc = ensure_tmp(c, (size_t)R.M + 8, ri.size()))) return rc;
+                d_out = c->d_tmp + R.M;
+                HIPCHK(hipMemcpy(c->d_items_tmp, ri.data(), ri.size() * sizeof(uint2), hipMemcpyHostToDevice));
+                const int nri = (int)ri.size();
+                rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_rho<kRhoKpt><<<nri, kRhoBlock, 0, c->stream>>>(c->d_reps, c->d_items_tmp, nri, 1, nullptr); });
+                if (rc) return rc;
+                c->rho_updated = true;
+            }
+            n_long = (int)((b1 - b0 + skip - 1) / skip);
+            const int upd = (mine && c->rho_updated) ? 1 : 0;
+            if (n_long > 0) {
+                rc = launch(c, SPIMC_KERNEL_RHO, [&] {
+                    k_long_window<<<n_long, 256, 0, c->stream>>>(c->d_reps, (int)r, mode, (int)b0, upd, c->d_tmp);
+                });
+                if (rc) return rc;
+            }
+        }
+        rc = launch(c, SPIMC_KERNEL_PAIR, [&] {
+            k_action_sum<<<1, 64, 0, c->stream>>>(c->d_reps, (int)r, mode, nl, n_long, c->d_tmp, d_out);
+        });
+    }
+    if (rc) return rc;
+    if ((rc = sync_and_resolve(c))) return rc;
+    HIPCHK(hipMemcpy(out, d_out, sizeof(double), hipMemcpyDeviceToHost));
+    return SPIMC_OK;
+}
+
+int spimc_d_action_d_beta(spimc_ctx* c, uint32_t r, int action, double* out) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!out) return fail(SPIMC_ERR_ARG, "null output");
+    Rep* base = c->d_reps + r;
+    const Rep& R = c->reps[r].dev;
+    if (action == SPIMC_ACTION_KINETIC) {
+        if ((rc = est_kin_dbeta(c, base, 1)) || (rc = est_finalize(c, base, 1, 0, 0))) return rc;
+    } else {
+        if ((rc = est_pair_whole(c, base, 1, 0)) || (rc = est_finalize(c, base, 1, 1, 0))) return rc;
+    }
+    if ((rc = sync_and_resolve(c))) return rc;
+    double res[2];
+    HIPCHK(hipMemcpy(res, R.scratch + scratch_res(R.M, std::max(R.nk, 1)), sizeof res, hipMemcpyDeviceToHost));
+    *out = action == SPIMC_ACTION_KINETIC ? res[0] : (R.pair_type == SPIMC_PAIR_NONE ? 0. : res[1]);
+    return SPIMC_OK;
+}
+
+int spimc_potential(spimc_ctx* c, uint32_t r, int action, double* out) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!out) return fail(SPIMC_ERR_ARG, "null output");
+    *out = 0.;
+    const Rep& R = c->reps[r].dev;
+    if (action == SPIMC_ACTION_KINETIC || R.pair_type == SPIMC_PAIR_NONE) return SPIMC_OK;
+    Rep* base = c->d_reps + r;
+    if ((rc = est_pair_whole(c, base, 1, 1)) || (rc = est_finalize(c, base, 1, 1, 0))) return rc;
+    if ((rc = sync_and_resolve(c))) return rc;
+    double res[2];
+    HIPCHK(hipMemcpy(res, R.scratch + scratch_res(R.M, std::max(R.nk, 1)), sizeof res, hipMemcpyDeviceToHost));
+    *out = res[1];
+    return SPIMC_OK;
+}
+
+int spimc_kinetic_virial_energy(spimc_ctx* c, uint32_t r, uint32_t window, double* out) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!out) return fail(SPIMC_ERR_ARG, "null output");
+    const Rep& R = c->reps[r].dev;
+    const int W = window ? (int)window : 0;
+    rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_kin_virial<<<1, 256, 0, c->stream>>>(c->d_reps + r, W, kSlotVirial); });
+    if (rc || (rc = sync_and_resolve(c))) return rc;
+    HIPCHK(hipMemcpy(out, R.scratch + kSlotVirial * R.M, sizeof(double), hipMemcpyDeviceToHost));
+    return SPIMC_OK;
+}
+
+int spimc_action_gradient(spimc_ctx* c, uint32_t r, uint32_t terms, int compat, double* out) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!out) return fail(SPIMC_ERR_ARG, "null output");
+    if (compat && terms != SPIMC_GRAD_PAIR_SHORT)
+        return fail(SPIMC_ERR_ARG, "compat gradient covers the short-range pair action only");
+    if (terms & ~SPIMC_GRAD_ALL) return fail(SPIMC_ERR_ARG, "unknown gradient terms");
+    const Rep& R = c->reps[r].dev;
+    Rep* base = c->d_reps + r;
+    dim3 g((R.N + 63) / 64, R.M, 1);
+    if (compat) {
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_grad_compat<<<g, 64, 0, c->stream>>>(base, 0); });
+    } else {
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_adjoint_local<<<g, 64, 0, c->stream>>>(base, terms, 0); });
+        if (!rc && (terms & SPIMC_GRAD_PAIR_LONG) && R.use_long_range) {
+            if (R.max_idx > kAdjMaxIdx) return fail(SPIMC_ERR_UNSUPPORTED, "adjoint long range supports |n| <= 40");
+            const size_t shmem = (size_t)64 * 3 * (R.max_idx + 1) * sizeof(double2) +
+                                 kAdjTile * (sizeof(double2) + sizeof(int) + sizeof(double));
+            rc = launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_adjoint_long<<<g, 64, shmem, c->stream>>>(base, 0); });
+        }
+    }
+    if (rc || (rc = sync_and_resolve(c))) return rc;
+    HIPCHK(hipMemcpy(out, R.grad, (size_t)3 * R.M * R.N * sizeof(double), hipMemcpyDeviceToHost));
+    if (!compat && terms == SPIMC_GRAD_ALL) c->adjoint_fresh = true;
+    return SPIMC_OK;
+}
+
+// ---- Move interface ----------------------------------------------------------
+
+int spimc_tape_set(spimc_ctx* c, uint32_t r, const double* draws, uint64_t n) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    RepHost& h = c->reps[r];
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h.tape) (void)hipFree(h.tape);
+    h.tape = nullptr;
+    h.dev.tape = nullptr;
+    h.dev.tape_len = 0;
+    if (draws && n) {
+        HIPCHK(hipMalloc((void**)&h.tape, n * sizeof(double)));
+        HIPCHK(hipMemcpy(h.tape, draws, n * sizeof(double), hipMemcpyHostToDevice));
+        h.dev.tape = h.tape;
+        h.dev.tape_len = (long long)n;
+    }
+    long long z = 0;
+    HIPCHK(hipMemcpy(h.dev.tape_cursor, &z, sizeof z, hipMemcpyHostToDevice));
+    // graphs captured earlier read the Rep table through its device pointer, so
+    // they see the new tape without re-capture
+    return upload_reps(c);
+}
+
+int spimc_tape_cursor(spimc_ctx* c, uint32_t r, uint64_t* cursor) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    long long v = 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&v, c->reps[r].dev.tape_cursor, sizeof v, hipMemcpyDeviceToHost));
+    *cursor = (uint64_t)v;
+    return SPIMC_OK;
+}
+
+int spimc_bisect_stage(spimc_ctx* c, const uint32_t* n_attempts, uint32_t n_uniform, uint8_t* accepted) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    return run_stage(c, 0, n_attempts, n_uniform, accepted, false);
+}
+
+int spimc_displace_stage(spimc_ctx* c, const uint32_t* n_attempts, uint32_t n_uniform, uint8_t* accepted) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    return run_stage(c, 1, n_attempts, n_uniform, accepted, false);
+}
+
+int spimc_move_counters(spimc_ctx* c, uint32_t r, uint64_t* out) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    Counters k;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&k, c->reps[r].dev.cnt, sizeof k, hipMemcpyDeviceToHost));
+    out[0] = k.bis_att;
+    out[1] = k.bis_acc;
+    out[2] = k.dis_att;
+    out[3] = k.dis_acc;
+    return SPIMC_OK;
+}
+
+// ---- Observable interface ------------------------------------------------------
+
+int spimc_obs_configure(spimc_ctx* c, int energy_estimator, uint32_t virial_window, double sk_k_cut, double gr_r_min,
+                        double gr_r_max, uint32_t gr_n_r) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    if (energy_estimator < SPIMC_ENERGY_THERMAL || energy_estimator > SPIMC_ENERGY_VIRIAL_ADJOINT)
+        return fail(SPIMC_ERR_ARG, "unknown energy estimator");
+    if (gr_n_r < 2 || gr_n_r > (uint32_t)kGrBins) return fail(SPIMC_ERR_ARG, "n_r must be in [2, 4096]");
+    for (auto& h : c->reps)
+        if (virial_window > (uint32_t)h.dev.M) return fail(SPIMC_ERR_ARG, "virial window larger than n_bead");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->energy_estimator = energy_estimator;
+    c->virial_window = virial_window;
+    c->sk_k_cut = sk_k_cut;
+    c->gr_r_min = gr_r_min;
+    c->gr_r_max = gr_r_max;
+    c->gr_n_r = gr_n_r;
+    std::vector<double> rmax;
+    for (auto& h : c->reps) {
+        rmax.push_back(gr_r_max > 0. ? gr_r_max : h.dev.L / 2.);
+        if (h.gr_cnt) (void)hipFree(h.gr_cnt);
+        h.gr_cnt = nullptr;
+        HIPCHK(hipMalloc((void**)&h.gr_cnt, gr_n_r * sizeof(unsigned long long)));
+        h.dev.gr_cnt = h.gr_cnt;
+    }
+    HIPCHK(hipMemcpy(c->d_gr_rmax, rmax.data(), rmax.size() * sizeof(double), hipMemcpyHostToDevice));
+    int rc = upload_reps(c);
+    if (rc) return rc;
+    return spimc_obs_reset(c, SPIMC_OBS_ENERGY | SPIMC_OBS_SK | SPIMC_OBS_GR);
+}
+
+int spimc_obs_reset(spimc_ctx* c, uint32_t which) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& h : c->reps) {
+        const Rep& R = h.dev;
+        if (which & SPIMC_OBS_ENERGY) {
+            HIPCHK(hipMemset(R.e_acc, 0, 2 * sizeof(double)));
+            HIPCHK(hipMemset(R.n_measure, 0, sizeof(unsigned long long)));
+        }
+        if (which & SPIMC_OBS_SK) {
+            HIPCHK(hipMemset(R.sk_acc, 0, std::max(R.nk, 1) * sizeof(double)));
+            HIPCHK(hipMemset(R.n_measure + 1, 0, sizeof(unsigned long long)));
+        }
+        if (which & SPIMC_OBS_GR) {
+            HIPCHK(hipMemset(R.gr_cnt, 0, c->gr_n_r * sizeof(unsigned long long)));
+            HIPCHK(hipMemset(R.n_measure + 2, 0, sizeof(unsigned long long)));
+        }
+    }
+    return SPIMC_OK;
+}
+
+int spimc_obs_accumulate(spimc_ctx* c, uint32_t which) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    if (c->prop_set) return fail(SPIMC_ERR_STATE, "an interface proposal is pending");
+    int rc = obs_accumulate_all(c, which);
+    if (rc) return rc;
+    return sync_and_resolve(c);
+}
+
+int spimc_obs_read(spimc_ctx* c, uint32_t r, uint32_t which, double* out, uint64_t* n_measure) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!out || !n_measure) return fail(SPIMC_ERR_ARG, "null output");
+    const Rep& R = c->reps[r].dev;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    unsigned long long nm[3];
+    HIPCHK(hipMemcpy(nm, R.n_measure, sizeof nm, hipMemcpyDeviceToHost));
+    if (which == SPIMC_OBS_ENERGY) {
+        HIPCHK(hipMemcpy(out, R.e_acc, 2 * sizeof(double), hipMemcpyDeviceToHost));
+        *n_measure = nm[0];
+    } else if (which == SPIMC_OBS_SK) {
+        if (R.nk) HIPCHK(hipMemcpy(out, R.sk_acc, R.nk * sizeof(double), hipMemcpyDeviceToHost));
+        *n_measure = nm[1];
+    } else if (which == SPIMC_OBS_GR) {
+        std::vector<unsigned long long> v(c->gr_n_r);
+        HIPCHK(hipMemcpy(v.data(), R.gr_cnt, v.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < v.size(); ++i) out[i] = (double)v[i];
+        *n_measure = nm[2];
+    } else {
+        return fail(SPIMC_ERR_ARG, "read one observable at a time");
+    }
+    return SPIMC_OK;
+}
+
+// ---- batched sweep ------------------------------------------------------------
+
+int spimc_sweep(spimc_ctx* c, uint32_t n_sweeps, uint32_t flags) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    const int R = (int)c->reps.size();
+    std::vector<uint32_t> nb(R), nd(R);
+    for (int r = 0; r < R; ++r) {
+        const Rep& x = c->reps[r].dev;
+        nb[r] = (uint32_t)(x.N * x.M) >> x.n_level;
+        nd[r] = (uint32_t)x.N;
+    }
+    const bool graph = (flags & SPIMC_SWEEP_GRAPH) != 0;
+    for (uint32_t s = 0; s < n_sweeps; ++s) {
+        int rc = run_stage(c, 0, nb.data(), 0, nullptr, graph);
+        if (rc) return rc;
+        if ((rc = run_stage(c, 1, nd.data(), 0, nullptr, graph))) return rc;
+        if (flags & SPIMC_SWEEP_ADJOINT) {
+            if ((rc = est_gradient(c, c->d_reps, R, SPIMC_GRAD_ALL, 0, 0))) return rc;
+            c->adjoint_fresh = true;
+        }
+        uint32_t which = 0;
+        if (flags & SPIMC_SWEEP_ENERGY) which |= SPIMC_OBS_ENERGY;
+        if (flags & SPIMC_SWEEP_SK) which |= SPIMC_OBS_SK;
+        if (flags & SPIMC_SWEEP_GR) which |= SPIMC_OBS_GR;
+        if (which && (rc = obs_accumulate_all(c, which))) return rc;
+        if ((rc = sync_and_resolve(c))) return rc;
+        c->adjoint_fresh = (flags & SPIMC_SWEEP_ADJOINT) != 0;
+    }
+    return SPIMC_OK;
+}
+
+int spimc_bead_updates(spimc_ctx* c, uint64_t* out) {
+    if (!c || !out) return fail(SPIMC_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    uint64_t t = 0;
+    for (auto& h : c->reps) {
+        Counters k;
+        HIPCHK(hipMemcpy(&k, h.dev.cnt, sizeof k, hipMemcpyDeviceToHost));
+        t += k.bead_updates;
+    }
+    *out = t;
+    return SPIMC_OK;
+}
+
+// ---- measurement ---------------------------------------------------------------
+
+int spimc_timing_enable(spimc_ctx* c, int enable) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    int rc = sync_and_resolve(c);
+    if (rc) return rc;
+    c->timing = enable != 0;
+    HIPCHK(hipMemcpy(c->work_seen, c->d_work, sizeof(c->work_seen), hipMemcpyDeviceToHost));
+    return SPIMC_OK;
+}
+
+int spimc_kernel_timing(spimc_ctx* c, int kernel, double* total_ms, uint64_t* launches, double* bytes) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    if (kernel < 0 || kernel >= SPIMC_KERNEL_COUNT) return fail(SPIMC_ERR_ARG, "unknown kernel class");
+    int rc = sync_and_resolve(c);
+    if (rc) return rc;
+    if (total_ms) *total_ms = c->t_ms[kernel];
+    if (launches) *launches = c->t_n[kernel];
+    if (bytes) *bytes = c->t_bytes[kernel];
+    return SPIMC_OK;
+}
+
+int spimc_timing_reset(spimc_ctx* c) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    int rc = sync_and_resolve(c);
+    if (rc) return rc;
+    for (int i = 0; i < SPIMC_KERNEL_COUNT; ++i) {
+        c->t_ms[i] = 0.;
+        c->t_n[i] = 0;
+        c->t_bytes[i] = 0.;
+    }
+    return SPIMC_OK;
+}
+
+}  // extern "C"

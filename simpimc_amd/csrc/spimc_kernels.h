@@ -1,0 +1,1029 @@
+// spimc_kernels.h -- HIP kernels of the engine (included once, by spimc_engine.hip).
+//
+// Move stages run in rounds: every replica with attempts left performs one
+// attempt per round, through four launches
+//   k_propose_*  one wavefront per replica: draws, bisection sampling of the
+//                coarse levels with their kinetic Metropolis decisions, and the
+//                level-0 NEW beads
+//   k_pair       one wavefront per (replica, link): short-range pair action of
+//                the moved particle against all others, OLD and NEW
+//   k_rho        one workgroup per (replica, moved slice, k chunk): incremental
+//                rho_k update into the spare buffer fused with the long-range
+//                action sums, OLD and NEW
+//   k_decide     one wavefront per replica: level-0 Metropolis decision, commit
+// A single chain (the plain method) is the same launches with one replica.
+#pragma once
+
+#include "spimc_device.h"
+
+namespace spimc {
+
+constexpr int kPairBlock = 256;       // 4 wavefronts, one link each
+constexpr int kRhoBlock = 256;
+constexpr int kMaxIdx = 63;           // largest |n_d| of a k-vector
+constexpr int kGrBins = 4096;
+
+enum { WORK_PAIR_LINKS = 0, WORK_RHO_SLICEK = 1, WORK_COUNT = 2 };
+
+// Per-replica estimator scratch: slots [slot][M] for slots 0..3, then the
+// long-range k-chunk partials, then two results.
+constexpr int kSlotKin = 0, kSlotPair = 1, kSlotCentroid = 2, kSlotVirial = 3;
+__host__ __device__ inline size_t scratch_long(int M) { return (size_t)4 * M; }
+__host__ __device__ inline size_t scratch_res(int M, int nk) { return (size_t)4 * M + (nk + 255) / 256 + 1; }
+__host__ __device__ inline size_t scratch_size(int M, int nk) { return scratch_res(M, nk) + 4; }
+
+// Blocks are dispatched round-robin over the 8 XCDs; this remap hands each XCD
+// a contiguous range of logical blocks, so the items of one replica (sorted
+// together) share an L2.
+__device__ __forceinline__ int xcd_remap(int hw, int G) {
+    const int X = 8;
+    int q = G / X, rem = G % X;
+    int xcd = hw % X, local = hw / X;
+    return (xcd < rem) ? xcd * (q + 1) + local : rem * (q + 1) + (xcd - rem) * q + local;
+}
+
+__device__ __forceinline__ double pos_at(const Rep& R, int d, int s, int p) {
+    return R.pos[((size_t)d * R.M + s) * R.N + p];
+}
+
+// NEW-mode position of the proposal's particle at slice s
+__device__ __forceinline__ double pos_new(const Rep& R, const Prop& pr, int d, int s) {
+    int m = s - pr.first;
+    if (m < 0) m += R.M;
+    return (m < pr.count) ? R.prop_new[3 * m + d] : pos_at(R, d, s, pr.p);
+}
+
+__device__ __forceinline__ double tape_get(const Rep& R, long long i, int* bad) {
+    if (i < 0 || i >= R.tape_len) {
+        *bad = 1;
+        return 0.5;
+    }
+    return R.tape[i];
+}
+
+// ---------------------------------------------------------------------------
+// K3: rho_k of every slice (Species::InitRhoK, species_class.h:380-403);
+// particle sums in particle order per k.
+// ---------------------------------------------------------------------------
+constexpr int kBuildTile = 16;
+template <int KPT>
+__global__ __launch_bounds__(256) void k_rho_build(const Rep* __restrict__ reps, int r) {
+    const Rep& R = reps[r];
+    const int b = blockIdx.x, chunk = blockIdx.y;
+    const int mx = R.max_idx, stride = mx + 1;
+    __shared__ double2 tab[kBuildTile * 3 * (kMaxIdx + 1)];
+    double2 acc[KPT];
+    int kk[KPT];
+    for (int i = 0; i < KPT; ++i) {
+        acc[i] = make_double2(0., 0.);
+        int k = chunk * 256 * KPT + i * 256 + threadIdx.x;
+        kk[i] = (k < R.nk) ? R.kidx[k] : -1;
+    }
+    for (int p0 = 0; p0 < R.N; p0 += kBuildTile) {
+        __syncthreads();
+        if (threadIdx.x < kBuildTile * 3) {
+            int pp = threadIdx.x / 3, d = threadIdx.x % 3;
+            if (p0 + pp < R.N) phase_chain(pos_at(R, d, b, p0 + pp) * R.box, mx, &tab[(pp * 3 + d) * stride]);
+        }
+        __syncthreads();
+        const int np = min(kBuildTile, R.N - p0);
+        for (int pp = 0; pp < np; ++pp)
+            for (int i = 0; i < KPT; ++i)
+                if (kk[i] >= 0) {
+                    double2 e = phase_k(&tab[pp * 3 * stride], stride, kk[i]);
+                    acc[i].x += e.x;
+                    acc[i].y += e.y;
+                }
+    }
+    double2* out = R.rho + ((size_t)R.sel[b] * R.M + b) * R.nk;
+    for (int i = 0; i < KPT; ++i) {
+        int k = chunk * 256 * KPT + i * 256 + threadIdx.x;
+        if (k < R.nk) out[k] = acc[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K4: Bisect::Attempt down to level 0 (bisect_class.h:39-124), one wavefront
+// per replica.  Coarse levels carry only the kinetic action (the pair action
+// returns 0 above max_level = 0, pair_action_class.h:269).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ reps, int n_reps) {
+    const int r = blockIdx.x;
+    if (r >= n_reps) return;
+    const Rep& R = reps[r];
+    const int lane = threadIdx.x;
+    __shared__ double sh_old[(1 << kMaxLevel) + 1][3];
+    __shared__ double sh_new[(1 << kMaxLevel) + 1][3];
+    __shared__ int s_info[4];
+    __shared__ long long s_cur;
+    __shared__ unsigned long long s_seq;
+    Prop* pr = R.prop;
+    if (lane == 0) {
+        Counters* c = R.cnt;
+        unsigned int rem = c->remaining;
+        s_info[0] = rem > 0;
+        if (rem > 0) {
+            c->remaining = rem - 1;
+            s_seq = c->seq;
+            c->seq = s_seq + 1;
+        }
+    }
+    __syncthreads();
+    if (!s_info[0]) {
+        if (lane == 0) pr->status = 0;
+        return;
+    }
+    const int n = R.n_level, nb = 1 << n, N = R.N, M = R.M;
+    Rng rng{make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32)), 1u, s_seq};
+    const bool tape = R.tape != nullptr;
+    int bad = 0;
+    if (lane == 0) {
+        int p, b0;
+        long long cur = 0;
+        if (tape) {
+            cur = *R.tape_cursor;
+            p = (int)tape_get(R, cur, &bad);
+            b0 = (int)tape_get(R, cur + 1, &bad);
+            cur += 2;
+            if (p < 0 || p >= N || b0 < 0 || b0 >= M) {
+                bad = 1;
+                p = 0;
+                b0 = 0;
+            }
+        } else {
+            p = (int)rng.index(0, 0, N);
+            b0 = (int)rng.index(0, 1, M);
+        }
+        s_info[1] = p;
+        s_info[2] = b0;
+        s_info[3] = bad;
+        s_cur = cur;
+    }
+    __syncthreads();
+    const int p = s_info[1], b0 = s_info[2];
+    for (int j = lane; j <= nb; j += kWave) {
+        int s = (b0 + j) % M;
+        for (int d = 0; d < 3; ++d) {
+            double x = pos_at(R, d, s, p);
+            sh_old[j][d] = x;
+            sh_new[j][d] = x;
+        }
+    }
+    __syncthreads();
+    long long cur = s_cur;
+    double prev = 0.;
+    int g_base = 0;  // normals drawn so far (device RNG indexing)
+    for (int level = n - 1; level >= 0; --level) {
+        const int skip = 1 << level;
+        const int nmid = nb >> (level + 1);
+        const double level_tau = R.tau * skip;
+        const double sigma = sqrt(R.lambda * level_tau);
+        const double i4lt_b = 1. / (4. * R.lambda * (0.5 * R.tau * (double)skip));
+        double lspo = 0., lspn = 0.;
+        if (lane < nmid) {
+#pragma clang fp contract(off)
+            const int a = 2 * lane * skip, b = a + skip, c = a + 2 * skip;
+            double t[3], rbar[3], dold[3], dnew[3];
+            for (int d = 0; d < 3; ++d) {
+                t[d] = minimg(sh_old[c][d] - sh_old[a][d], R.L, R.iL);
+                rbar[d] = sh_old[a][d] + 0.5 * t[d];
+                dold[d] = minimg(sh_old[b][d] - rbar[d], R.L, R.iL);
+            }
+            for (int d = 0; d < 3; ++d) {
+                double z;
+                if (tape)
+                    z = tape_get(R, cur + 3 * lane + d, &bad);
+                else {
+                    int g = g_base + 3 * lane + d;
+                    z = rng.normal(1 + n + g / 2, g & 1);
+                }
+                dnew[d] = sigma * z;
+                dnew[d] = minimg(dnew[d], R.L, R.iL);
+                t[d] = minimg(sh_new[c][d] - sh_new[a][d], R.L, R.iL);
+                rbar[d] = sh_new[a][d] + 0.5 * t[d];
+                sh_new[b][d] = rbar[d] + dnew[d];
+            }
+            lspo = -free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dold);
+            lspn = -free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dnew);
+        }
+        __syncthreads();
+        const int nlinks = nb >> level;
+        const double i4lt_k = 1. / (4. * R.lambda * level_tau);
+        double ko = 0., kn = 0.;
+        for (int i = lane; i < nlinks; i += kWave) {
+            const int a = i * skip, b = a + skip;
+            double dd[3];
+            for (int d = 0; d < 3; ++d) dd[d] = minimg(sh_old[a][d] - sh_old[b][d], R.L, R.iL);
+            ko += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+            for (int d = 0; d < 3; ++d) dd[d] = minimg(sh_new[a][d] - sh_new[b][d], R.L, R.iL);
+            kn += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+        }
+        lspo = wave_sum(lspo);
+        lspn = wave_sum(lspn);
+        ko = wave_sum(ko);
+        kn = wave_sum(kn);
+        const double lsr = -lspn + lspo;
+        cur += 3 * nmid;
+        g_base += 3 * nmid;
+        double u;
+        if (tape)
+            u = tape_get(R, cur, &bad);
+        else
+            u = rng.unif(1 + level, 0);
+        cur += 1;
+        if (level > 0) {
+            const double old_action = 0. + ko, new_action = 0. + kn;
+            const double change = new_action - old_action;
+            const double la = lsr - change + prev;
+            if (la < log(u)) {
+                if (lane == 0) {
+                    pr->status = 2;
+                    pr->move = 0;
+                    pr->p = p;
+                    pr->b0 = b0;
+                    pr->count = 0;
+                    pr->tape_next = cur;
+                }
+                return;
+            }
+            prev = change;
+        } else {
+            const int any_bad = __any(bad);
+            if (lane == 0) {
+                pr->status = (any_bad || s_info[3]) ? 3 : 1;
+                pr->move = 0;
+                pr->p = p;
+                pr->b0 = b0;
+                pr->n_links = nb;
+                pr->first = (b0 + 1) % M;
+                pr->count = nb - 1;
+                pr->lsr = lsr;
+                pr->k_old = ko;
+                pr->k_new = kn;
+                pr->prev = prev;
+                pr->u0 = u;
+                pr->tape_next = cur;
+            }
+            for (int j = lane; j < nb - 1; j += kWave)
+                for (int d = 0; d < 3; ++d) R.prop_new[3 * j + d] = sh_new[j + 1][d];
+        }
+    }
+}
+
+// DisplaceParticle::Attempt (displace_particle_class.h:28-72): NEW beads only;
+// the kinetic action of a rigid shift is 0 (the bead walk is empty).
+__global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__ reps, int n_reps) {
+    const int r = blockIdx.x;
+    if (r >= n_reps) return;
+    const Rep& R = reps[r];
+    const int lane = threadIdx.x;
+    __shared__ int s_act, s_p;
+    __shared__ double s_dr[3];
+    Prop* pr = R.prop;
+    if (lane == 0) {
+        Counters* c = R.cnt;
+        unsigned int rem = c->remaining;
+        s_act = rem > 0;
+        if (rem > 0) {
+            c->remaining = rem - 1;
+            unsigned long long seq = c->seq;
+            c->seq = seq + 1;
+            int bad = 0, p;
+            double u0;
+            long long cur = 0;
+            if (R.tape) {
+                cur = *R.tape_cursor;
+                p = (int)tape_get(R, cur, &bad);
+                for (int d = 0; d < 3; ++d) s_dr[d] = tape_get(R, cur + 1 + d, &bad);
+                u0 = tape_get(R, cur + 4, &bad);
+                cur += 5;
+                if (p < 0 || p >= R.N) {
+                    bad = 1;
+                    p = 0;
+                }
+            } else {
+                Rng rng{make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32)), 2u, seq};
+                p = (int)rng.index(0, 0, R.N);
+                s_dr[0] = R.step * (rng.unif(1, 0) - 0.5);
+                s_dr[1] = R.step * (rng.unif(1, 1) - 0.5);
+                s_dr[2] = R.step * (rng.unif(2, 0) - 0.5);
+                u0 = rng.unif(2, 1);
+            }
+            s_p = p;
+            pr->status = bad ? 3 : 1;
+            pr->move = 1;
+            pr->p = p;
+            pr->b0 = 0;
+            pr->n_links = R.M;
+            pr->first = 0;
+            pr->count = R.M;
+            pr->lsr = 0.;
+            pr->k_old = 0.;
+            pr->k_new = 0.;
+            pr->prev = 0.;
+            pr->u0 = u0;
+            pr->tape_next = cur;
+        } else {
+            pr->status = 0;
+        }
+    }
+    __syncthreads();
+    if (!s_act) return;
+    const int p = s_p;
+    for (int s = lane; s < R.M; s += kWave)
+        for (int d = 0; d < 3; ++d) R.prop_new[3 * s + d] = pos_at(R, d, s, p) + s_dr[d];
+}
+
+// ---------------------------------------------------------------------------
+// K1: short-range pair action of the moved particle on the window links,
+// OLD and NEW (pair_action_class.h:279-290), one wavefront per (replica, link).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ reps, const uint2* __restrict__ items,
+                                                     int n_items, int interface_mode,
+                                                     unsigned long long* __restrict__ work) {
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int item = blk * (kPairBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (item >= n_items) return;
+    const uint2 it = items[item];
+    const Rep& R = reps[it.x];
+    const Prop pr = *R.prop;
+    if (!interface_mode && pr.status != 1) return;
+    const int j = (int)it.y;
+    if (j >= pr.n_links) return;
+    const int M = R.M, N = R.N, p = pr.p;
+    const int s0 = (pr.b0 + j) % M, s1 = (s0 + 1) % M;
+    double po0[3], po1[3], pn0[3], pn1[3];
+    for (int d = 0; d < 3; ++d) {
+        po0[d] = pos_at(R, d, s0, p);
+        po1[d] = pos_at(R, d, s1, p);
+        pn0[d] = pos_new(R, pr, d, s0);
+        pn1[d] = pos_new(R, pr, d, s1);
+    }
+    double uo = 0., un = 0.;
+    for (int q = lane; q < N; q += kWave) {
+        if (q == p) continue;
+        double q0[3], q1[3];
+        for (int d = 0; d < 3; ++d) {
+            q0[d] = pos_at(R, d, s0, q);
+            q1[d] = pos_at(R, d, s1, q);
+        }
+        LinkGeom g;
+        link_geom(po0, q0, po1, q1, R.L, R.iL, g);
+        uo += pair_u(R, g.r, g.rp, g.s);
+        link_geom(pn0, q0, pn1, q1, R.L, R.iL, g);
+        un += pair_u(R, g.r, g.rp, g.s);
+    }
+    uo = wave_sum(uo);
+    un = wave_sum(un);
+    if (lane == 0) {
+        R.pair_part[j] = make_double2(uo, un);
+        if (work) atomicAdd(&work[WORK_PAIR_LINKS], (unsigned long long)(2 * (N - 1)));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2: incremental rho_k of the moved slices (Species::UpdateRhoK,
+// species_class.h:406-425) written to the spare buffer, fused with the
+// long-range action sums of CalcULong (bare_pair_action_class.h:153-172,
+// david_pair_action_class.h:105-123) for OLD and NEW.
+// ---------------------------------------------------------------------------
+template <int KPT>
+__global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps, const uint2* __restrict__ items,
+                                                   int n_items, int interface_mode,
+                                                   unsigned long long* __restrict__ work) {
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    if (blk >= n_items) return;
+    const uint2 it = items[blk];
+    const Rep& R = reps[it.x];
+    const Prop pr = *R.prop;
+    if (!interface_mode && pr.status != 1) return;
+    const int m = (int)(it.y >> 8), chunk = (int)(it.y & 255);
+    if (m >= pr.count) return;
+    const int M = R.M, nk = R.nk, mx = R.max_idx, stride = mx + 1;
+    const int s = (pr.first + m) % M;
+    __shared__ double2 tab[2 * 3 * (kMaxIdx + 1)];
+    __shared__ double red[2][kRhoBlock / kWave];
+    if (threadIdx.x < 6) {
+        const int o = threadIdx.x / 3, d = threadIdx.x % 3;
+        const double x = o ? R.prop_new[3 * m + d] : pos_at(R, d, s, pr.p);
+        phase_chain(x * R.box, mx, &tab[(o * 3 + d) * stride]);
+    }
+    __syncthreads();
+    const int cur = R.sel[s];
+    const double2* __restrict__ rc = R.rho + ((size_t)cur * M + s) * nk;
+    double2* __restrict__ rn = R.rho + ((size_t)(1 - cur) * M + s) * nk;
+    double ao = 0., an = 0.;
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const int k = chunk * kRhoBlock * KPT + i * kRhoBlock + threadIdx.x;
+        if (k < nk) {
+            const int kp = R.kidx[k];
+            const double2 en = phase_k(&tab[3 * stride], stride, kp);
+            const double2 eo = phase_k(tab, stride, kp);
+            const double2 rv = rc[k];
+            double2 nv;
+            nv.x = rv.x + (en.x - eo.x);
+            nv.y = rv.y + (en.y - eo.y);
+            rn[k] = nv;
+            const double cf = R.coef_s[k];
+            ao += cf * (rv.x * rv.x + rv.y * rv.y);
+            an += cf * (nv.x * nv.x + nv.y * nv.y);
+        }
+    }
+    ao = wave_sum(ao);
+    an = wave_sum(an);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) {
+        red[0][w] = ao;
+        red[1][w] = an;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double to = 0., tn = 0.;
+        for (int i = 0; i < kRhoBlock / kWave; ++i) {
+            to += red[0][i];
+            tn += red[1][i];
+        }
+        R.rho_part[m * R.n_chunks + chunk] = make_double2(to, tn);
+    }
+    if (work && threadIdx.x == 0)
+        atomicAdd(&work[WORK_RHO_SLICEK], (unsigned long long)min(KPT * kRhoBlock, nk - chunk * kRhoBlock * KPT));
+}
+
+// ---------------------------------------------------------------------------
+// K5: level-0 Metropolis decision and commit (bisect_class.h:100-123,24-36,127-139;
+// displace_particle_class.h:68-86), one wavefront per replica.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int n_reps) {
+    const int r = blockIdx.x;
+    if (r >= n_reps) return;
+    const Rep& R = reps[r];
+    const int lane = threadIdx.x;
+    Prop* prp = R.prop;
+    const Prop pr = *prp;
+    if (pr.status == 0) return;
+    double po = 0., pn = 0., lo = 0., ln = 0.;
+    if (pr.status == 1) {
+        for (int j = lane; j < pr.n_links; j += kWave) {
+            double2 v = R.pair_part[j];
+            po += v.x;
+            pn += v.y;
+        }
+        if (R.use_long_range) {
+            const int np = pr.count * R.n_chunks;
+            for (int j = lane; j < np; j += kWave) {
+                double2 v = R.rho_part[j];
+                lo += v.x;
+                ln += v.y;
+            }
+        }
+        po = wave_sum(po);
+        pn = wave_sum(pn);
+        lo = wave_sum(lo);
+        ln = wave_sum(ln);
+    }
+    __shared__ int s_acc;
+    if (lane == 0) {
+        int acc = 0;
+        if (pr.status == 1) {
+            const double pair_old = po + R.long_scale * lo, pair_new = pn + R.long_scale * ln;
+            const double old_action = (0. + pr.k_old) + pair_old;
+            const double new_action = (0. + pr.k_new) + pair_new;
+            const double change = new_action - old_action;
+            const double la = pr.lsr - change + pr.prev;
+            acc = !(la < log(pr.u0));
+        }
+        s_acc = acc;
+        Counters* c = R.cnt;
+        if (pr.move == 0) {
+            c->bis_att += 1;
+            c->bis_acc += acc;
+            c->bead_updates += (1ull << R.n_level) - 1;
+        } else {
+            c->dis_att += 1;
+            c->dis_acc += acc;
+            c->bead_updates += R.M;
+        }
+        unsigned int si = c->stage_i;
+        if (si < R.accepted_cap) R.accepted[si] = (unsigned char)acc;
+        c->stage_i = si + 1;
+        if (R.tape) *R.tape_cursor = pr.tape_next;
+        if (pr.status == 3) c->error = 1;  // replay tape overrun / bad draw
+        prp->status = 0;
+    }
+    __syncthreads();
+    if (s_acc) {
+        for (int m = lane; m < pr.count; m += kWave) {
+            const int s = (pr.first + m) % R.M;
+            for (int d = 0; d < 3; ++d) R.pos[((size_t)d * R.M + s) * R.N + pr.p] = R.prop_new[3 * m + d];
+            if (R.use_long_range) R.sel[s] ^= 1;
+        }
+    }
+}
+
+// Stage set-up on the device (kept out of hipMemset so captured graphs stay plain kernels)
+__global__ void k_stage_begin(const Rep* __restrict__ reps, int n_reps, const unsigned int* __restrict__ counts) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_reps) return;
+    reps[r].cnt->remaining = counts[r];
+    reps[r].cnt->stage_i = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Interface kernels (Action::GetAction in a chosen mode)
+// ---------------------------------------------------------------------------
+// Kinetic::GetAction (kinetic_class.h:105-122) for the proposal's particle p.
+__global__ __launch_bounds__(64) void k_kinetic_window(const Rep* __restrict__ reps, int r, int mode, int p, int b0,
+                                                        int n_links, int level, double* __restrict__ out) {
+    const Rep& R = reps[r];
+    const Prop pr = *R.prop;
+    const int skip = 1 << level, M = R.M;
+    const double i4lt = 1. / (4. * R.lambda * (R.tau * (double)skip));
+    double tot = 0.;
+    for (int i = threadIdx.x; i < n_links; i += kWave) {
+        const int a = (b0 + i * skip) % M, b = (a + skip) % M;
+        double dd[3];
+        for (int d = 0; d < 3; ++d) {
+            double xa = (mode && pr.p == p) ? pos_new(R, pr, d, a) : pos_at(R, d, a, p);
+            double xb = (mode && pr.p == p) ? pos_new(R, pr, d, b) : pos_at(R, d, b, p);
+            dd[d] = minimg(xa - xb, R.L, R.iL);
+        }
+        tot += free_neg_log(&R.kin[level], R.kin_images, i4lt, dd);
+    }
+    tot = wave_sum(tot);
+    if (threadIdx.x == 0) *out = tot;
+}
+
+// Long-range sum of CalcULong over window slices in a mode; NEW mode reads the
+// spare buffer for slices the proposal moved (after k_rho in interface mode).
+__global__ __launch_bounds__(256) void k_long_window(const Rep* __restrict__ reps, int r, int mode, int b0,
+                                                     int rho_updated, double* __restrict__ part) {
+    const Rep& R = reps[r];
+    const Prop pr = *R.prop;
+    const int w = blockIdx.x, M = R.M;
+    const int s = (b0 + w) % M;
+    int m = s - pr.first;
+    if (m < 0) m += M;
+    const bool moved = mode && rho_updated && (m < pr.count);
+    const int buf = moved ? 1 - R.sel[s] : R.sel[s];
+    const double2* rb = R.rho + ((size_t)buf * M + s) * R.nk;
+    double t = 0.;
+    for (int k = threadIdx.x; k < R.nk; k += 256) {
+        double2 v = rb[k];
+        t += R.coef_s[k] * (v.x * v.x + v.y * v.y);
+    }
+    __shared__ double sh[4];
+    t = block_sum<256>(t, sh);
+    if (threadIdx.x == 0) part[w] = t;
+}
+
+// Sums a pair action query: links (mode column) + long-range window partials.
+__global__ void k_action_sum(const Rep* __restrict__ reps, int r, int mode, int n_links, int n_long,
+                             const double* __restrict__ long_part, double* __restrict__ out) {
+    const Rep& R = reps[r];
+    double s = 0., l = 0.;
+    for (int j = threadIdx.x; j < n_links; j += kWave) s += mode ? R.pair_part[j].y : R.pair_part[j].x;
+    for (int j = threadIdx.x; j < n_long; j += kWave) l += long_part[j];
+    s = wave_sum(s);
+    l = wave_sum(l);
+    if (threadIdx.x == 0) *out = s + R.long_scale * l;
+}
+
+// Move::Accept / Reject of an interface proposal.
+__global__ void k_commit(const Rep* __restrict__ reps, int r, int accept, int rho_updated) {
+    const Rep& R = reps[r];
+    const Prop pr = *R.prop;
+    if (accept)
+        for (int m = threadIdx.x; m < pr.count; m += blockDim.x) {
+            const int s = (pr.first + m) % R.M;
+            for (int d = 0; d < 3; ++d) R.pos[((size_t)d * R.M + s) * R.N + pr.p] = R.prop_new[3 * m + d];
+            if (rho_updated && R.use_long_range) R.sel[s] ^= 1;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// K6: whole-path estimators (thermal energy, potential)
+// ---------------------------------------------------------------------------
+// Kinetic::DActionDBeta summand per slice (kinetic_class.h:35-45)
+__global__ __launch_bounds__(256) void k_kin_dbeta(const Rep* __restrict__ reps, int slot) {
+    const Rep& R = reps[blockIdx.y];
+    const int b = blockIdx.x;
+    if (b >= R.M) return;
+    const int b1 = (b + 1) % R.M;
+    double t = 0.;
+    for (int p = threadIdx.x; p < R.N; p += 256) {
+        double dd[3];
+        for (int d = 0; d < 3; ++d) dd[d] = minimg(pos_at(R, d, b, p) - pos_at(R, d, b1, p), R.L, R.iL);
+        double img = 0.;
+        if (R.kin_images > 0)
+            for (int d = 0; d < 3; ++d) img -= spl_eval1(R.kin_dtau, dd[d]);
+        t += img - (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]) * R.kin_i4ltt;
+    }
+    __shared__ double sh[4];
+    t = block_sum<256>(t, sh);
+    if (threadIdx.x == 0) R.scratch[slot * R.M + b] = t;
+}
+
+// Pair sums over all pairs on link (b, b+1): which 0 = CalcdUdBeta
+// (pair_action_class.h:241-264), 1 = CalcV on plain minimum images
+// (pair_action_class.h:369-395).  Pairs (i, i+d mod N), d = 1..N/2.
+__global__ __launch_bounds__(256) void k_pair_whole(const Rep* __restrict__ reps, int which, int slot) {
+    const Rep& R = reps[blockIdx.y];
+    const int b = blockIdx.x;
+    if (b >= R.M || R.pair_type == 0) {
+        if (b < R.M && threadIdx.x == 0) R.scratch[slot * R.M + b] = 0.;
+        return;
+    }
+    const int N = R.N, M = R.M, b1 = (b + 1) % M;
+    double t = 0.;
+    const int half = N / 2;
+    for (int i = threadIdx.x; i < N; i += 256) {
+        double a0[3], a1[3];
+        for (int d = 0; d < 3; ++d) {
+            a0[d] = pos_at(R, d, b, i);
+            a1[d] = pos_at(R, d, b1, i);
+        }
+        for (int dd = 1; dd <= half; ++dd) {
+            if ((N % 2 == 0) && dd == half && i >= half) continue;
+            const int j = (i + dd) % N;
+            double c0[3], c1[3];
+            for (int d = 0; d < 3; ++d) {
+                c0[d] = pos_at(R, d, b, j);
+                c1[d] = pos_at(R, d, b1, j);
+            }
+            // pair (lo, hi): r = r_hi - r_lo; magnitudes do not depend on the orientation
+            LinkGeom g;
+            if (i < j)
+                link_geom(a0, c0, a1, c1, R.L, R.iL, g);
+            else
+                link_geom(c0, a0, c1, a1, R.L, R.iL, g);
+            if (which == 0) {
+                t += pair_du_dbeta(R, g.r, g.rp, g.s);
+            } else {
+                double v0[3], v1[3];
+                for (int d = 0; d < 3; ++d) {
+                    v0[d] = minimg(a0[d] - c0[d], R.L, R.iL);
+                    v1[d] = minimg(a1[d] - c1[d], R.L, R.iL);
+                }
+                t += pair_v(R, sqrt(v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2]),
+                            sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]));
+            }
+        }
+    }
+    __shared__ double sh[4];
+    t = block_sum<256>(t, sh);
+    if (threadIdx.x == 0) R.scratch[slot * R.M + b] = t;
+}
+
+// sum_k coef_b(k) sum_b |rho(b,k)|^2 (CalcVLong / CalcdUdBetaLong sums), per k chunk
+__global__ __launch_bounds__(256) void k_long_whole(const Rep* __restrict__ reps) {
+    const Rep& R = reps[blockIdx.y];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x * 256 >= R.nk) return;
+    double t = 0.;
+    if (k < R.nk && R.use_long_range) {
+        for (int b = 0; b < R.M; ++b) {
+            const double2 v = R.rho[((size_t)R.sel[b] * R.M + b) * R.nk + k];
+            t += v.x * v.x + v.y * v.y;
+        }
+        t *= R.coef_b[k];
+    }
+    __shared__ double sh[4];
+    t = block_sum<256>(t, sh);
+    if (threadIdx.x == 0) R.scratch[scratch_long(R.M) + blockIdx.x] = t;
+}
+
+// Kinetic::VirialEnergy (kinetic_class.h:48-102), thread per particle
+__global__ __launch_bounds__(256) void k_kin_virial(const Rep* __restrict__ reps, int W_in, int slot) {
+    const Rep& R = reps[blockIdx.x];
+    const int M = R.M;
+    const int W = W_in > 0 ? W_in : M;
+    const double wt = W * R.tau;
+    double tot = 0.;
+    for (int p = threadIdx.x; p < R.N; p += 256) {
+        int b_0 = 0, b_1 = 1 % M, b_2 = 2 % M;
+        double d1[3], dL[3], t[3];
+        for (int d = 0; d < 3; ++d) d1[d] = minimg(pos_at(R, d, b_1, p) - pos_at(R, d, b_0, p), R.L, R.iL);
+        for (int d = 0; d < 3; ++d) dL[d] = d1[d];
+        for (int sk = 1; sk < W; ++sk) {
+            for (int d = 0; d < 3; ++d) t[d] = minimg(pos_at(R, d, b_2, p) - pos_at(R, d, b_1, p), R.L, R.iL);
+            for (int d = 0; d < 3; ++d) dL[d] += t[d];
+            b_1 = b_2;
+            b_2 = (b_2 + 1) % M;
+        }
+        tot -= (dL[0] * d1[0] + dL[1] * d1[1] + dL[2] * d1[2]) * R.kin_i4lt / wt;
+        for (int bi = 1; bi < M; ++bi) {
+            for (int d = 0; d < 3; ++d) dL[d] -= d1[d];
+            for (int d = 0; d < 3; ++d) t[d] = minimg(pos_at(R, d, b_2, p) - pos_at(R, d, b_1, p), R.L, R.iL);
+            for (int d = 0; d < 3; ++d) dL[d] += t[d];
+            b_0 = (b_0 + 1) % M;
+            const int bn = (b_0 + 1) % M;
+            for (int d = 0; d < 3; ++d) d1[d] = minimg(pos_at(R, d, bn, p) - pos_at(R, d, b_0, p), R.L, R.iL);
+            b_1 = b_2;
+            b_2 = (b_2 + 1) % M;
+            tot -= (dL[0] * d1[0] + dL[1] * d1[1] + dL[2] * d1[2]) * R.kin_i4lt / wt;
+        }
+    }
+    __shared__ double sh[4];
+    tot = block_sum<256>(tot, sh);
+    if (threadIdx.x == 0) R.scratch[slot * M] = R.kin_dbeta_const / W + tot;
+}
+
+// ---------------------------------------------------------------------------
+// K7: gradients.  Thread per bead (b, p); output grad[b][p][3].
+// ---------------------------------------------------------------------------
+// Reference-compatible PairAction::GetActionGradient(b, b+1, {p}, 0):
+// CalcGradientU by central difference, eps = 1e-4 (pair_action_class.h:135-155,305-337)
+__global__ __launch_bounds__(64) void k_grad_compat(const Rep* __restrict__ reps, int out_est) {
+    const Rep& R = reps[blockIdx.z];
+    const int p = blockIdx.x * 64 + threadIdx.x, b = blockIdx.y;
+    if (p >= R.N || b >= R.M) return;
+    const int M = R.M, N = R.N;
+    const int bj = (b + 1) % M, bk = (b + M - 1) % M;
+    double r0[3], pj[3], pk[3], g[3] = {0., 0., 0.};
+    for (int d = 0; d < 3; ++d) {
+        r0[d] = pos_at(R, d, b, p);
+        pj[d] = pos_at(R, d, bj, p);
+        pk[d] = pos_at(R, d, bk, p);
+    }
+    const double eps = 1.e-4;
+    if (R.pair_type != 0 && N > 1)
+        for (int q = 0; q < N; ++q) {
+            if (q == p) continue;
+            double q0[3], qj[3], qk[3];
+            for (int d = 0; d < 3; ++d) {
+                q0[d] = pos_at(R, d, b, q);
+                qj[d] = pos_at(R, d, bj, q);
+                qk[d] = pos_at(R, d, bk, q);
+            }
+            for (int lk = 0; lk < 2; ++lk) {
+                const double* a1 = lk ? pk : pj;
+                const double* c1 = lk ? qk : qj;
+                for (int d = 0; d < 3; ++d) {
+                    double a0[3] = {r0[0], r0[1], r0[2]};
+                    LinkGeom gg;
+                    a0[d] = r0[d] + eps;
+                    link_geom(a0, q0, a1, c1, R.L, R.iL, gg);
+                    const double f1 = pair_u(R, gg.r, gg.rp, gg.s);
+                    a0[d] = r0[d] - eps;
+                    link_geom(a0, q0, a1, c1, R.L, R.iL, gg);
+                    const double f2 = pair_u(R, gg.r, gg.rp, gg.s);
+                    g[d] += (f1 - f2) / (2. * eps);
+                }
+            }
+        }
+    double* o = (out_est ? R.grad_est : R.grad) + 3 * ((size_t)b * N + p);
+    o[0] = g[0];
+    o[1] = g[1];
+    o[2] = g[2];
+}
+
+// Analytic adjoint, local terms: kinetic links (b-1,b), (b,b+1) and short-range
+// pair links of every other particle.  Writes (overwrites) grad.
+__global__ __launch_bounds__(64) void k_adjoint_local(const Rep* __restrict__ reps, unsigned int terms, int out_est) {
+    const Rep& R = reps[blockIdx.z];
+    const int i = blockIdx.x * 64 + threadIdx.x, b = blockIdx.y;
+    if (i >= R.N || b >= R.M) return;
+    const int M = R.M, N = R.N;
+    const int bn = (b + 1) % M, bp = (b + M - 1) % M;
+    double ri[3], rn[3], rp[3], g[3] = {0., 0., 0.};
+    for (int d = 0; d < 3; ++d) {
+        ri[d] = pos_at(R, d, b, i);
+        rn[d] = pos_at(R, d, bn, i);
+        rp[d] = pos_at(R, d, bp, i);
+    }
+    if (terms & 1u) {
+        // K(d) = sum_x img(d_x) + |d|^2 i4lt, d = r_b - r_{b+1}: dK/dr_b = K'(d_fwd) - K'(d_bwd)
+        double df[3], db[3];
+        for (int d = 0; d < 3; ++d) {
+            df[d] = minimg(ri[d] - rn[d], R.L, R.iL);
+            db[d] = minimg(rp[d] - ri[d], R.L, R.iL);
+        }
+        for (int d = 0; d < 3; ++d) {
+            double kf = 2. * df[d] * R.kin_i4lt, kb = 2. * db[d] * R.kin_i4lt;
+            if (R.kin_images > 0) {
+                int ia = spl_locate(R.kin[0], df[d]), ib = spl_locate(R.kin[0], db[d]);
+                kf += spl_der(R.kin[0], ia, 0, df[d] - R.kin[0].x[ia]);
+                kb += spl_der(R.kin[0], ib, 0, db[d] - R.kin[0].x[ib]);
+            }
+            g[d] += kf - kb;
+        }
+    }
+    if ((terms & 2u) && R.pair_type != 0)
+        for (int j = 0; j < N; ++j) {
+            if (j == i) continue;
+            double j0[3], jn[3], jp[3];
+            for (int d = 0; d < 3; ++d) {
+                j0[d] = pos_at(R, d, b, j);
+                jn[d] = pos_at(R, d, bn, j);
+                jp[d] = pos_at(R, d, bp, j);
+            }
+            LinkGeom gg;
+            double ur, urp, us;
+            // link (b, b+1), i first: r = r_j(b) - r_i(b); dU/dr_i(b) = -(U_r rhat + U_s shat)
+            link_geom(ri, j0, rn, jn, R.L, R.iL, gg);
+            pair_u_grad(R, gg.r, gg.rp, gg.s, ur, urp, us);
+            for (int d = 0; d < 3; ++d)
+                g[d] -= ur * gg.rv[d] / gg.r + ((gg.s > 0.) ? us * gg.sv[d] / gg.s : 0.);
+            // link (b-1, b), i first: r' = r_j(b) - r_i(b); dU/dr_i(b) = -(U_r' rphat - U_s shat)
+            link_geom(rp, jp, ri, j0, R.L, R.iL, gg);
+            pair_u_grad(R, gg.r, gg.rp, gg.s, ur, urp, us);
+            for (int d = 0; d < 3; ++d)
+                g[d] -= urp * gg.rpv[d] / gg.rp - ((gg.s > 0.) ? us * gg.sv[d] / gg.s : 0.);
+        }
+    double* o = (out_est ? R.grad_est : R.grad) + 3 * ((size_t)b * N + i);
+    o[0] = g[0];
+    o[1] = g[1];
+    o[2] = g[2];
+}
+
+// Analytic adjoint, Ewald term: d/dr_i of long_scale sum_k coef_s |rho(b,k)|^2
+// = -2 long_scale sum_k coef_s k Im(conj(rho) e^{i k.r_i}).  Block = 64 beads of
+// one slice; rho(b,:) staged through LDS in k tiles; per-bead phase tables in LDS.
+constexpr int kAdjTile = 128;
+__global__ __launch_bounds__(64) void k_adjoint_long(const Rep* __restrict__ reps, int out_est) {
+    const Rep& R = reps[blockIdx.z];
+    const int b = blockIdx.y;
+    if (b >= R.M || blockIdx.x * 64 >= R.N || !R.use_long_range || R.pair_type == 0) return;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int mx = R.max_idx, stride = mx + 1;
+    extern __shared__ double2 dyn[];
+    double2* tab = dyn;                            // [64][3][stride]
+    double2* rho_t = dyn + 64 * 3 * stride;        // [kAdjTile]
+    int* kp_t = (int*)(rho_t + kAdjTile);          // [kAdjTile]
+    double* cf_t = (double*)(kp_t + kAdjTile);     // [kAdjTile]
+    const bool valid = i < R.N;
+    double2* my = tab + threadIdx.x * 3 * stride;
+    if (valid)
+        for (int d = 0; d < 3; ++d) phase_chain(pos_at(R, d, b, i) * R.box, mx, my + d * stride);
+    double g[3] = {0., 0., 0.};
+    const double2* rb = R.rho + ((size_t)R.sel[b] * R.M + b) * R.nk;
+    for (int k0 = 0; k0 < R.nk; k0 += kAdjTile) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < kAdjTile; t += 64) {
+            const int k = k0 + t;
+            if (k < R.nk) {
+                rho_t[t] = rb[k];
+                kp_t[t] = R.kidx[k];
+                cf_t[t] = R.coef_s[k];
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            const int nt = min(kAdjTile, R.nk - k0);
+            for (int t = 0; t < nt; ++t) {
+                const int kp = kp_t[t];
+                const double2 e = phase_k(my, stride, kp);
+                const double2 rv = rho_t[t];
+                // Im(conj(rho) e) = rho.x e.y - rho.y e.x
+                const double im = cf_t[t] * (rv.x * e.y - rv.y * e.x);
+                g[0] += im * ((kp & 255) - 128);
+                g[1] += im * (((kp >> 8) & 255) - 128);
+                g[2] += im * (((kp >> 16) & 255) - 128);
+            }
+        }
+    }
+    if (valid) {
+        double* o = (out_est ? R.grad_est : R.grad) + 3 * ((size_t)b * R.N + i);
+        const double f = -2. * R.long_scale * R.box;
+        o[0] += f * g[0];
+        o[1] += f * g[1];
+        o[2] += f * g[2];
+    }
+}
+
+// Centroid term of Energy::VirialEstimator (energy_class.h:79-94):
+// sum_p grad_nonkinetic(b,p) . Dr(r_p(b), r_p(0)), per slice.  from_sweep: the
+// gradient is the sweep's full dS/dr, whose kinetic term is removed here.
+__global__ __launch_bounds__(256) void k_centroid(const Rep* __restrict__ reps, int slot, int from_sweep) {
+    const Rep& R = reps[blockIdx.y];
+    const int b = blockIdx.x;
+    if (b >= R.M) return;
+    const int bn = (b + 1) % R.M, bp = (b + R.M - 1) % R.M;
+    double t = 0.;
+    for (int p = threadIdx.x; p < R.N; p += 256) {
+        const double* g = (from_sweep ? R.grad : R.grad_est) + 3 * ((size_t)b * R.N + p);
+        for (int d = 0; d < 3; ++d) {
+            double gd = g[d];
+            if (from_sweep) {
+                const double x = pos_at(R, d, b, p);
+                const double df = minimg(x - pos_at(R, d, bn, p), R.L, R.iL);
+                const double db = minimg(pos_at(R, d, bp, p) - x, R.L, R.iL);
+                double kf = 2. * df * R.kin_i4lt, kb = 2. * db * R.kin_i4lt;
+                if (R.kin_images > 0) {
+                    int ia = spl_locate(R.kin[0], df), ib = spl_locate(R.kin[0], db);
+                    kf += spl_der(R.kin[0], ia, 0, df - R.kin[0].x[ia]);
+                    kb += spl_der(R.kin[0], ib, 0, db - R.kin[0].x[ib]);
+                }
+                gd -= kf - kb;
+            }
+            t += gd * minimg(pos_at(R, d, b, p) - pos_at(R, d, 0, p), R.L, R.iL);
+        }
+    }
+    __shared__ double sh[4];
+    t = block_sum<256>(t, sh);
+    if (threadIdx.x == 0) R.scratch[slot * R.M + b] = t;
+}
+
+// Combines estimator partials (fixed order) into results res[0..1] of a replica.
+//   what 0: kinetic dbeta -> res[0]; 1: pair dbeta or potential -> res[1];
+//   what 2: virial kinetic (virial slot + centroid / 2 tau) -> res[0]
+//   acc: add res[0..1] into the energy accumulators
+// Scratch layout per replica: [slot][M] for slots 0..3, then long-range chunk
+// partials, then the two results.
+
+__global__ void k_finalize(const Rep* __restrict__ reps, int n_reps, int what, int acc) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_reps) return;
+    const Rep& R = reps[r];
+    const int M = R.M;
+    const double* S = R.scratch;
+    double* res = R.scratch + scratch_res(M, R.nk);
+    if (what == 0) {
+        double t = R.kin_dbeta_const;
+        for (int b = 0; b < M; ++b) t += S[kSlotKin * M + b];
+        res[0] = t;
+    } else if (what == 1) {
+        double t = 0.;
+        for (int b = 0; b < M; ++b) t += S[kSlotPair * M + b];
+        if (R.use_long_range && R.pair_type != 0) {
+            const double* lw = R.scratch + scratch_long(M);
+            double l = 0.;
+            for (int c = 0; c < (R.nk + 255) / 256; ++c) l += lw[c];
+            t += l + R.const_v;
+        }
+        res[1] = t;
+    } else if (what == 2) {
+        double c = 0.;
+        for (int b = 0; b < M; ++b) c += S[kSlotCentroid * M + b];
+        res[0] = S[kSlotVirial * M] + c / (2. * R.tau);
+    }
+    if (acc) {
+        R.e_acc[0] += res[0];
+        R.e_acc[1] += res[1];
+        R.n_measure[0] += 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K8: StructureFactor::Accumulate (structure_factor_class.h:15-32)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sk(const Rep* __restrict__ reps, double k_cut) {
+    const Rep& R = reps[blockIdx.y];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= R.nk) return;
+    if (threadIdx.x == 0 && blockIdx.x == 0) R.n_measure[1] += 1;
+    if (!(R.kmag[k] < k_cut)) return;
+    double t = 0.;
+    for (int b = 0; b < R.M; ++b) {
+        const double2 v = R.rho[((size_t)R.sel[b] * R.M + b) * R.nk + k];
+        t += v.x * v.x + v.y * v.y;
+    }
+    R.sk_acc[k] += t;
+}
+
+// ---------------------------------------------------------------------------
+// K9: PairCorrelation::Accumulate (pair_correlation_class.h:15-28) with
+// LinearGrid::ReverseMap (observable_class.h:56-62); LDS histogram per slice.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gr(const Rep* __restrict__ reps, double r_min, double d_ir, int n_r,
+                                            const double* __restrict__ r_max_per_rep) {
+    const Rep& R = reps[blockIdx.y];
+    const int b = blockIdx.x;
+    if (b >= R.M) return;
+    __shared__ unsigned int hist[kGrBins];
+    for (int i = threadIdx.x; i < n_r; i += 256) hist[i] = 0;
+    __syncthreads();
+    const int N = R.N, half = N / 2;
+    double dir = d_ir;
+    if (r_max_per_rep) dir = 1. / ((r_max_per_rep[blockIdx.y] - r_min) / (n_r - 1.));
+    for (int i = threadIdx.x; i < N; i += 256) {
+        double a[3];
+        for (int d = 0; d < 3; ++d) a[d] = pos_at(R, d, b, i);
+        for (int dd = 1; dd <= half; ++dd) {
+            if ((N % 2 == 0) && dd == half && i >= half) continue;
+            const int j = (i + dd) % N;
+            const int lo = min(i, j), hi = max(i, j);
+            double v[3];
+            {
+#pragma clang fp contract(off)
+                for (int d = 0; d < 3; ++d) {
+                    const double x = (lo == i) ? a[d] : pos_at(R, d, b, lo);
+                    const double y = (hi == i) ? a[d] : pos_at(R, d, b, hi);
+                    v[d] = minimg(x - y, R.L, R.iL);
+                }
+                const double rr = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                const double f = rint((rr - r_min) * dir - 0.5);
+                const unsigned int bin = (unsigned int)(long long)f;
+                if (f >= 0. && bin < (unsigned int)n_r) atomicAdd(&hist[bin], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_r; i += 256)
+        if (hist[i]) atomicAdd(&R.gr_cnt[i], (unsigned long long)hist[i]);
+    if (threadIdx.x == 0 && b == 0) R.n_measure[2] += 1;
+}
+
+}  // namespace spimc

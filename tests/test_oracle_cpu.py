@@ -1,0 +1,305 @@
+"""CPU tests: the oracle pinned against closed forms, library routines and brute
+force, the host-side table logic, and the C ABI surface of the engine library.
+No test here launches device work."""
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy.interpolate import CubicSpline
+
+import oracle_ffi as of
+from oracle_ffi import Oracle
+from simpimc_amd import abi, system
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def minimg(d, L):
+    return d - np.rint(d / L) * L
+
+
+# ---------------------------------------------------------------------------
+# splines, k-space, free-particle images
+# ---------------------------------------------------------------------------
+def test_natural_spline_matches_scipy_golden():
+    g = np.load(os.path.join(GOLD, "natural_spline.npz"))
+    v, dv = of.spline_eval(g["x"], g["y"], g["t"])
+    assert np.max(np.abs(v - g["v"])) < 1e-12 * np.max(np.abs(g["v"]))
+    assert np.max(np.abs(dv - g["dv"])) < 1e-11 * np.max(np.abs(g["dv"]))
+
+
+@pytest.mark.parametrize("shell", [25, 61, 97, 153])
+def test_kspace_counts_golden(shell):
+    want = json.load(open(os.path.join(GOLD, "kspace_counts.json")))[str(shell)]
+    L = system.heg_box_length(256, 1.0)
+    kc = math.sqrt(shell + 0.5) * 2 * math.pi / L
+    a = system.kspace(L, kc)
+    b = of.kspace_build(L, kc)
+    c = abi.kspace_build(L, kc)
+    assert len(a) == want
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+    n2 = (a.astype(np.int64) ** 2).sum(axis=1)
+    assert n2.max() <= shell and n2.min() >= 1
+
+
+def test_kspace_strict_cutoff_tie():
+    # |n|^2 = 25 at k_cut = 5 (2 pi / L) is a floating-point tie of the strict test:
+    # each triple of the shell rounds on its own side, so part of the shell survives;
+    # the nudged cutoff keeps the whole shell
+    L = system.heg_box_length(32, 2.0)
+    tie = len(system.kspace(L, 5 * 2 * math.pi / L))
+    assert 242 <= tie <= 257
+    assert tie == len(of.kspace_build(L, 5 * 2 * math.pi / L)) == len(abi.kspace_build(L, 5 * 2 * math.pi / L))
+    assert len(system.kspace(L, (5 + 1e-9) * 2 * math.pi / L)) == 257
+
+
+def test_free_spline_image_sum_golden():
+    g = np.load(os.path.join(GOLD, "free_images.npz"))
+    L, lam, tau, n_im = float(g["L"]), float(g["lam"]), float(g["tau"]), int(g["n_images"])
+    x = g["x"]
+    r = np.stack([x, np.zeros_like(x), np.zeros_like(x)], axis=1)
+    got = of.free_log_rho(L, n_im, lam, tau, r)
+    img0 = g["image"][len(x) // 2]  # x = 0
+    want = -g["image"] - 2 * img0 - x * x / (4 * lam * tau)
+    assert np.max(np.abs(got - want)) < 1e-9
+
+
+def test_free_spline_no_images_is_gaussian():
+    rng = np.random.default_rng(1)
+    r = rng.uniform(-2, 2, size=(50, 3))
+    got = of.free_log_rho(5.0, 0, 0.5, 0.1, r)
+    assert np.array_equal(got, -(r[:, 0] ** 2 + r[:, 1] ** 2 + r[:, 2] ** 2) * (1 / (4 * 0.5 * 0.1)))
+
+
+# ---------------------------------------------------------------------------
+# actions
+# ---------------------------------------------------------------------------
+def test_kinetic_action_closed_form():
+    s = system.config_c1()
+    pos = system.random_path(s, seed=1, spread=0.4)
+    o = Oracle(s)
+    o.upload(pos)
+    p, b0, nb = 4, 29, 8
+    for level in range(3):
+        skip = 1 << level
+        want = 0.0
+        for a in range(b0, b0 + nb, skip):
+            d = minimg(pos[:, a % s.n_bead, p] - pos[:, (a + skip) % s.n_bead, p], s.L)
+            want += (d @ d) / (4 * s.lam * s.tau * skip)
+        got = o.get_action(abi.ACTION_KINETIC, abi.MODE_OLD, b0, b0 + nb, [p], level)
+        assert abs(got - want) < 1e-12 * want
+
+
+def test_rho_k_brute_force():
+    s = system.config_c1()
+    pos = system.random_path(s, seed=2, spread=0.3)
+    o = Oracle(s)
+    o.upload(pos)
+    k = s.k_n.astype(np.float64) * (2 * math.pi / s.L)
+    want = np.exp(1j * np.einsum("kd,dbp->bkp", k, pos)).sum(axis=2)
+    assert np.max(np.abs(o.rho_k() - want)) < 1e-12 * s.n_part
+
+
+def _ref_link_geometry(pos, L, b, b1, i, j):
+    """Path::DrDrpDrrp restated in numpy."""
+    r = pos[:, b, j] - pos[:, b, i]
+    rp = pos[:, b1, j] - pos[:, b1, i]
+    r = r - np.rint(r / L) * L
+    rp = rp + np.rint((r - rp) / L) * L
+    s = minimg(r - rp, L)
+    return np.linalg.norm(r), np.linalg.norm(rp), np.linalg.norm(s)
+
+
+def test_david_link_action_closed_form():
+    """The q,s expansion u = (u00(r)+u00(r'))/2 + sum_kj u_kj(q) z^2j s^2(k-j) evaluated
+    with scipy's natural splines, against the oracle's pair GetAction."""
+    s = system.config_c2(n_part=2, n_bead=4, n_level=1, k_shell=2.5, n_grid=300, seed=5)
+    s.use_long_range = False
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-1, 1, size=(3, 4, 2))
+    pos[:, :, 1] += 1.0
+    o = Oracle(s)
+    o.upload(pos)
+    spl = [CubicSpline(s.david_grid, s.david_u_kj[c], bc_type="natural") for c in range(s.david_u_kj.shape[0])]
+    want = 0.0
+    g0, g1 = s.david_grid[0], s.david_grid[-1]
+    for b in range(4):
+        r, rp, ss = _ref_link_geometry(pos, s.L, b, (b + 1) % 4, 0, 1)
+        q, z = 0.5 * (r + rp), r - rp
+        rc, rpc = np.clip(r, g0, g1), np.clip(rp, g0, g1)
+        u = 0.5 * (spl[1](rc) + spl[1](rpc))
+        if ss > 0 and q < g1:
+            for k in range(1, 3):
+                for j in range(k + 1):
+                    u += spl[k * (k + 1) // 2 + j + 1](q) * z ** (2 * j) * ss ** (2 * (k - j))
+        want += u
+    got = o.get_action(abi.ACTION_PAIR, abi.MODE_OLD, 0, 4, [0], 0)
+    assert abs(got - want) < 1e-12 * abs(want)
+    # the off-diagonal terms contribute
+    assert abs(want - sum(0.5 * (spl[1](np.clip(_ref_link_geometry(pos, s.L, b, (b + 1) % 4, 0, 1)[0], g0, g1))
+                                  + spl[1](np.clip(_ref_link_geometry(pos, s.L, b, (b + 1) % 4, 0, 1)[1], g0, g1)))
+                           for b in range(4))) > 1e-8
+
+
+def test_bare_primitive_ewald_action_closed_form():
+    """Bare primitive Coulomb with Ewald split: tau [ (1/r + 1/r')/2 - (vl(r) + vl(r'))/2 ]
+    per link plus tau sum_k v_k sum_b |rho_k(b)|^2 over the window."""
+    s = system.config_c1(n_part=3, n_bead=4, n_level=1)
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(-s.L / 2, s.L / 2, size=(3, 4, 3))
+    o = Oracle(s)
+    o.upload(pos)
+    vl = CubicSpline(s.bare_r_long, s.bare_v_long_r, bc_type="natural")
+    want = 0.0
+    p = 1
+    for b in range(4):
+        for q in (0, 2):
+            r, rp, _ = _ref_link_geometry(pos, s.L, b, (b + 1) % 4, p, q)
+            # SetLimits to each grid before use (bare_pair_action_class.h:101,115)
+            r, rp = np.clip([r, rp], s.bare_r[0], s.bare_r[-1])
+            want += s.tau * (0.5 / r + 0.5 / rp)
+            r, rp = np.clip([r, rp], s.bare_r_long[0], s.bare_r_long[-1])
+            want -= s.tau * (0.5 * vl(r) + 0.5 * vl(rp))
+    k = s.k_n.astype(np.float64) * (2 * math.pi / s.L)
+    rho = np.exp(1j * np.einsum("kd,dbp->bkp", k, pos)).sum(axis=2)
+    want += s.tau * np.sum(s.v_long_k[None, :] * np.abs(rho) ** 2)
+    got = o.get_action(abi.ACTION_PAIR, abi.MODE_OLD, 0, 4, [p], 0)
+    assert abs(got - want) < 1e-11 * abs(want)
+
+
+def test_madelung_bcc_wigner_crystal():
+    """Converged Ewald sum on the reference's BCC initial lattice (species_class.h:275-300):
+    potential energy per particle -0.895929255682 / r_s Hartree."""
+    N, rs = 16, 1.0
+    L = system.heg_box_length(N, rs)
+    alpha = 10.5 / L
+    kc = 11.0 * alpha
+    s = system.System(n_part=N, n_bead=2, beta=0.02, L=L, bisect_n_level=1)
+    s.k_n = system.kspace(L, kc)
+    system.bare_coulomb_ewald(s, alpha=alpha, n_r_long=8000)
+    o = Oracle(s)
+    o.upload(system.bcc_path(s))
+    e = o.potential(abi.ACTION_PAIR) / (s.n_bead * N)
+    assert abs(e - (-0.895929255682 / rs)) < 2e-7
+
+
+def test_david_loader_last_knot_and_potential_component():
+    grid, V, u, du = system.david_file_arrays(10.0, 0.7, 0.02, 50, 2, seed=1)
+    uk, duk = system.david_assemble(grid, V, u, du)
+    assert uk.shape == (7, 50)
+    assert np.all(uk[:, -1] == 0) and np.all(duk[:, -1] == 0)
+    assert np.array_equal(uk[0, :-1], V[:-1]) and np.array_equal(duk[0, :-1], V[:-1])
+    assert np.array_equal(uk[1:, :-1], u[:, :-1])
+
+
+def test_pair_correlation_brute_force_images():
+    """g(r) bins from a brute-force 27-image minimum distance and the reference's
+    ReverseMap rounding, against the oracle's counts (exact)."""
+    s = system.config_c1(n_part=12, n_bead=4, n_level=1)
+    pos = system.random_path(s, seed=9, spread=0.5)
+    o = Oracle(s)
+    o.upload(pos)
+    n_r, r_max = 40, s.L / 2
+    o.obs_configure(gr_n_r=n_r)
+    o.obs_accumulate(abi.OBS_GR)
+    got, nm = o.obs_read(abi.OBS_GR)
+    dr = r_max / (n_r - 1.0)
+    want = np.zeros(n_r)
+    shifts = np.array([[a, b, c] for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)]) * s.L
+    for b in range(4):
+        for i in range(12):
+            for j in range(i + 1, 12):
+                d = minimg(pos[:, b, i] - pos[:, b, j], s.L)
+                r = np.min(np.linalg.norm(d[None, :] + shifts, axis=1))
+                f = np.rint((r - 0.0) * (1.0 / dr) - 0.5)
+                if 0 <= f < n_r:
+                    want[int(f)] += 1
+    assert nm == 1 and np.array_equal(got, want)
+
+
+# ---------------------------------------------------------------------------
+# moves and invariants on the oracle
+# ---------------------------------------------------------------------------
+def test_replay_tape_roundtrip():
+    s = system.config_c1()
+    pos = system.random_path(s, seed=5, spread=0.05)
+    a = Oracle(s, seed=1)
+    a.upload(pos)
+    a.record(True)
+    acc_a = np.concatenate([a.bisect(200), a.displace(32)])
+    tape = a.recorded()
+    b = Oracle(s, seed=999)
+    b.upload(pos)
+    b.tape_set(tape)
+    acc_b = np.concatenate([b.bisect(200), b.displace(32)])
+    assert np.array_equal(acc_a, acc_b)
+    assert b.tape_cursor() == len(tape)
+    assert np.array_equal(a.download(), b.download())
+    assert 0 < acc_a.sum() < acc_a.size
+
+
+def test_incremental_rho_matches_rebuild():
+    s = system.config_c1()
+    o = Oracle(s, seed=4)
+    o.upload(system.random_path(s, seed=6, spread=0.05))
+    o.sweep(3)
+    inc = o.rho_k()
+    o.rho_k_rebuild()
+    assert np.max(np.abs(inc - o.rho_k())) < 1e-11 * s.n_part
+
+
+def test_displace_leaves_kinetic_action_zero():
+    # Kinetic::GetAction over [0, M) walks no links (kinetic_class.h:111-118)
+    s = system.config_c1()
+    o = Oracle(s)
+    o.upload(system.random_path(s, seed=7, spread=0.2))
+    assert o.get_action(abi.ACTION_KINETIC, abi.MODE_OLD, 0, s.n_bead, [3], 0) == 0.0
+
+
+def test_fd_gradient_is_consistent_with_action_totals():
+    """The oracle's whole-path central difference matches a direct difference of
+    its action totals for one bead (guards the local re-evaluation bookkeeping)."""
+    s = system.config_c1(n_part=6, n_bead=8, n_level=2)
+    pos = system.random_path(s, seed=8, spread=0.2)
+    o = Oracle(s)
+    o.upload(pos)
+    g = o.gradient(abi.GRAD_ALL, compat=False, h=1e-5)
+    b, p, d, h = 3, 2, 1, 1e-5
+    tot = []
+    for sgn in (1, -1):
+        q = pos.copy()
+        q[d, b, p] += sgn * h
+        o.upload(q)
+        tot.append(o.action_totals().sum())
+    fd = (tot[0] - tot[1]) / (2 * h)
+    assert abs(fd - g[b, p, d]) < 1e-6 * max(1.0, abs(fd))
+
+
+# ---------------------------------------------------------------------------
+# ABI surface of the engine library (no device work)
+# ---------------------------------------------------------------------------
+def test_library_exports_every_declared_symbol():
+    lib = abi.load()
+    hdr = open(os.path.join(ROOT, "include", "spimc.h")).read()
+    names = set(re.findall(r"^(?:int|const char\*)\s+(spimc_\w+)\(", hdr, flags=re.M))
+    assert names == set(abi.SIGNATURES)
+    for n in names:
+        assert hasattr(lib, n)
+    assert lib.spimc_abi_version() == 1
+
+
+def test_context_creation_errors_are_reported():
+    s = system.config_c1()
+    s.bisect_n_level = 9  # beyond floor(log2 32)
+    with pytest.raises(abi.SpimcError) as ei:
+        abi.Engine(s)
+    assert ei.value.code in (abi.ERR_UNSUPPORTED, abi.ERR_NODEVICE)
+    s = system.config_c1()
+    try:
+        abi.Engine(s).close()
+    except abi.SpimcError as e:  # CPU-only machine
+        assert e.code == abi.ERR_NODEVICE

@@ -1,0 +1,254 @@
+"""GPU parity: the HIP engine against the CPU oracle on the same seeded inputs.
+
+Tolerances: positions, accept/reject sequences and g(r) counts are compared
+bit for bit; actions, action differences, rho_k, gradients and estimators to
+1e-10 relative to the magnitude of the compared quantity (the engine and the
+oracle sum in different orders and use different libm/device sin, cos, exp).
+The analytic adjoint is compared with a central difference (h = 1e-5) to 1e-6.
+"""
+import numpy as np
+import pytest
+
+from simpimc_amd import abi, system
+from oracle_ffi import Oracle
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(1.0, float(np.max(np.abs(b))) if b.size else 1.0)
+    return float(np.max(np.abs(a - b))) / scale if a.size else 0.0
+
+
+def small_david(seed=7):
+    return system.config_c2(n_part=16, n_bead=16, r_s=1.0, tau=0.02, n_level=3, k_shell=20.5, n_grid=400, seed=seed)
+
+
+SYSTEMS = {
+    "bare_c1": lambda: system.config_c1(),
+    "david_small": small_david,
+}
+
+
+@pytest.fixture(params=sorted(SYSTEMS))
+def sysname(request):
+    return request.param
+
+
+def make(sysname, seed=3, spread=0.3):
+    s = SYSTEMS[sysname]()
+    pos = system.random_path(s, seed=seed, spread=spread)
+    return s, pos
+
+
+def test_rho_k_build(sysname):
+    s, pos = make(sysname)
+    o = Oracle(s)
+    o.upload(pos)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        got = e.rho_k()
+    ref = o.rho_k()
+    assert got.shape == ref.shape
+    assert rel_err(got, ref) < 1e-12 * s.n_part
+
+
+def test_get_action_interface(sysname):
+    s, pos = make(sysname)
+    o = Oracle(s)
+    o.upload(pos)
+    rng = np.random.default_rng(5)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        p, b0 = 5, 3
+        nb = 1 << s.bisect_n_level
+        new = pos[:, (b0 + 1 + np.arange(nb - 1)) % s.n_bead, p].T + rng.normal(0, 0.2, size=(nb - 1, 3))
+        for eng in (e, o):
+            eng.set_proposal(p, b0 + 1, new)
+        for action in (abi.ACTION_KINETIC, abi.ACTION_PAIR):
+            for level in range(s.bisect_n_level):
+                for mode in (abi.MODE_OLD, abi.MODE_NEW):
+                    a = e.get_action(action, mode, b0, b0 + nb, [p], level)
+                    b = o.get_action(action, mode, b0, b0 + nb, [p], level)
+                    assert abs(a - b) <= RTOL * max(1.0, abs(b)), (action, level, mode, a, b)
+        # displacement window: all slices
+        for mode in (abi.MODE_OLD, abi.MODE_NEW):
+            a = e.get_action(abi.ACTION_PAIR, mode, 0, s.n_bead, [p], 0)
+            b = o.get_action(abi.ACTION_PAIR, mode, 0, s.n_bead, [p], 0)
+            assert abs(a - b) <= RTOL * max(1.0, abs(b))
+        e.accept()
+        o.accept()
+        assert np.array_equal(e.download(), o.download())
+        assert rel_err(e.rho_k(), o.rho_k()) < 1e-12 * s.n_part
+
+
+def test_whole_path_estimators(sysname):
+    s, pos = make(sysname)
+    o = Oracle(s)
+    o.upload(pos)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        for action in (abi.ACTION_KINETIC, abi.ACTION_PAIR):
+            a, b = e.d_action_d_beta(action), o.d_action_d_beta(action)
+            assert abs(a - b) <= RTOL * max(1.0, abs(b)), (action, a, b)
+        a, b = e.potential(abi.ACTION_PAIR), o.potential(abi.ACTION_PAIR)
+        assert abs(a - b) <= RTOL * max(1.0, abs(b))
+        for w in (0, 4):
+            a, b = e.kinetic_virial_energy(w), o.kinetic_virial_energy(w)
+            assert abs(a - b) <= RTOL * max(1.0, abs(b)), (w, a, b)
+
+
+def test_replay_accept_reject_bit_exact(sysname):
+    s, pos = make(sysname, spread=0.05)
+    o = Oracle(s, seed=11)
+    o.upload(pos)
+    o.record(True)
+    n_b = s.n_part * s.n_bead >> s.bisect_n_level
+    ref = []
+    for _ in range(3):
+        ref.append(o.bisect(n_b))
+        ref.append(o.displace(s.n_part))
+    tape = o.recorded()
+    with abi.Engine(s, seed=99) as e:
+        e.upload(pos)
+        e.tape_set(tape)
+        got = []
+        for _ in range(3):
+            got.append(e.bisect(n_b)[0])
+            got.append(e.displace(s.n_part)[0])
+        assert e.tape_cursor() == len(tape)
+        for g, r in zip(got, ref):
+            assert np.array_equal(g, r)
+        assert np.array_equal(e.download(), o.download())
+        assert rel_err(e.rho_k(), o.rho_k()) < 1e-11 * s.n_part
+        assert np.array_equal(e.counters(), o.counters())
+
+
+def test_gradient_compat(sysname):
+    s, pos = make(sysname)
+    o = Oracle(s)
+    o.upload(pos)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        got = e.gradient(abi.GRAD_PAIR_SHORT, compat=True)
+    ref = o.gradient(abi.GRAD_PAIR_SHORT, compat=True)
+    # the reference's central difference (eps 1e-4) amplifies rounding by 1/eps
+    assert rel_err(got, ref) < 1e-9
+
+
+@pytest.mark.parametrize("terms", [abi.GRAD_KINETIC, abi.GRAD_PAIR_SHORT, abi.GRAD_PAIR_LONG, abi.GRAD_ALL])
+def test_adjoint_matches_central_difference(sysname, terms):
+    s, pos = make(sysname, spread=0.1)
+    o = Oracle(s)
+    o.upload(pos)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        got = e.gradient(terms)
+    ref = o.gradient(terms, compat=False, h=1e-5)
+    assert rel_err(got, ref) < 1e-6
+
+
+def test_observables_after_replay(sysname):
+    s, pos = make(sysname, spread=0.05)
+    o = Oracle(s, seed=21)
+    o.upload(pos)
+    o.record(True)
+    o.bisect(64)
+    o.displace(8)
+    tape = o.recorded()
+    kc = 0.8 * float(system.kmag(s.L, s.k_n).max())
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        e.tape_set(tape)
+        e.bisect(64)
+        e.displace(8)
+        for est in (abi.ENERGY_THERMAL, abi.ENERGY_VIRIAL_COMPAT):
+            for eng in (e, o):
+                eng.obs_configure(energy=est, sk_k_cut=kc, gr_n_r=50)
+                eng.obs_accumulate(abi.OBS_ENERGY | abi.OBS_SK | abi.OBS_GR)
+            (ge, gn), (re, rn) = e.obs_read(abi.OBS_ENERGY), o.obs_read(abi.OBS_ENERGY)
+            assert gn == rn == 1
+            assert rel_err(ge, re) < RTOL, (est, ge, re)
+            gs, _ = e.obs_read(abi.OBS_SK)
+            rs, _ = o.obs_read(abi.OBS_SK)
+            assert rel_err(gs, rs) < RTOL
+            gg, _ = e.obs_read(abi.OBS_GR)
+            rg, _ = o.obs_read(abi.OBS_GR)
+            assert np.array_equal(gg, rg)
+
+
+def test_virial_adjoint_estimator(sysname):
+    s, pos = make(sysname, spread=0.05)
+    o = Oracle(s)
+    o.upload(pos)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        for eng in (e, o):
+            eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT)
+            eng.obs_accumulate(abi.OBS_ENERGY)
+        ge, _ = e.obs_read(abi.OBS_ENERGY)
+    re, _ = o.obs_read(abi.OBS_ENERGY)
+    assert rel_err(ge, re) < 1e-6
+
+
+def test_batched_replicas_match_single_chains():
+    systems = [system.config_c1(n_part=n, n_bead=m, r_s=rs) for n, m, rs in [(16, 16, 1.5), (32, 32, 2.0), (8, 16, 3.0), (24, 8, 1.2)]]
+    tapes, refs, finals = [], [], []
+    for i, s in enumerate(systems):
+        pos = system.random_path(s, seed=40 + i, spread=0.05)
+        o = Oracle(s, seed=100 + i)
+        o.upload(pos)
+        o.record(True)
+        refs.append((o.bisect(50), o.displace(10)))
+        tapes.append((pos, o.recorded()))
+        finals.append(o.download())
+    with abi.Engine(systems, seed=1) as e:
+        for i, (pos, tape) in enumerate(tapes):
+            e.upload(pos, r=i)
+            e.tape_set(tape, r=i)
+        ab = e.bisect(50)
+        ad = e.displace(10)
+        for i in range(len(systems)):
+            assert np.array_equal(ab[i], refs[i][0])
+            assert np.array_equal(ad[i], refs[i][1])
+            assert np.array_equal(e.download(r=i), finals[i])
+
+
+def test_uneven_counts_and_graph_sweep():
+    systems = system.config_c4(seed=3, n_replicas=6)
+    with abi.Engine(systems, seed=5) as e, abi.Engine(systems, seed=5) as g:
+        for i, s in enumerate(systems):
+            pos = system.random_path(s, seed=i, spread=0.05)
+            e.upload(pos, r=i)
+            g.upload(pos, r=i)
+        e.sweep(2, 0)
+        g.sweep(2, abi.SWEEP_GRAPH)
+        for i, s in enumerate(systems):
+            assert np.array_equal(e.download(r=i), g.download(r=i))
+            c = e.counters(r=i)
+            assert c[0] == 2 * ((s.n_part * s.n_bead) >> s.bisect_n_level)
+            assert c[2] == 2 * s.n_part
+
+
+def test_c2_full_size_properties():
+    """At the full HEG N=256, M=128 size: the incrementally maintained rho_k after a
+    move stage equals a rebuild from the final positions, and the short-range
+    pair gradient is translation invariant (sum over particles vanishes)."""
+    s = system.config_c2()
+    pos = system.random_path(s, seed=2, spread=0.05)
+    with abi.Engine(s, seed=3) as e:
+        e.upload(pos)
+        e.bisect(256)
+        e.displace(16)
+        inc = e.rho_k()
+        e.rho_k_rebuild()
+        full = e.rho_k()
+        assert rel_err(inc, full) < 1e-11 * s.n_part
+        g = e.gradient(abi.GRAD_PAIR_SHORT)
+        tot = g.sum(axis=1)
+        assert np.max(np.abs(tot)) < 1e-9 * max(1.0, np.max(np.abs(g)))
+        c = e.counters()
+        assert c[0] == 256 and c[2] == 16 and 0 < c[1] < 256
