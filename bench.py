@@ -1,0 +1,167 @@
+"""Benchmark: bead updates/s on the HEG N=256, M=128 FP64 workload.
+
+A step is one sweep of every replica: N*M/2^4 = 2048 bisection attempts (15 bead
+updates each) and N = 256 displace attempts (M = 128 bead updates each), then
+the adjoint dS/dr of every bead and the device estimators (thermal energy,
+S(k), g(r)).  `value` is the whole-job throughput of R independent replicas
+advanced together (batched+adjoint), inputs resident in HBM.  The plain method
+(one chain, one attempt at a time through the same kernels) is timed beside it,
+and the CPU oracle on a bounded sample of the same workload is the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "bead updates/s, HEG N=256 M=128 FP64, batched+adjoint vs plain GPU kernel"
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs (disables per-kernel timing)")
+    ap.add_argument("--plain-attempts", type=int, default=1024)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(target_s: float):
+    """Oracle (CPU restatement) on a bounded sample of the same workload: one C2
+    replica, bisection and displace attempts in the sweep's 8:1 ratio."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi
+    from simpimc_amd import system
+
+    threads = len(os.sched_getaffinity(0))
+    oracle_ffi.set_threads(threads)
+    s = system.config_c2()
+    o = oracle_ffi.Oracle(s, seed=17)
+    o.upload(system.random_path(s, seed=17, spread=0.05))
+    o.bisect(64)  # warm caches and OpenMP pools
+    o.displace(8)
+    u0 = o.bead_updates()
+    t0 = time.perf_counter()
+    n_b = n_d = 0
+    while time.perf_counter() - t0 < target_s:
+        o.bisect(64)
+        o.displace(8)
+        n_b += 64
+        n_d += 8
+    dt = time.perf_counter() - t0
+    du = o.bead_updates() - u0
+    return {
+        "value": du / dt,
+        "unit": "bead updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"1 replica of the same workload, {n_b} bisect + {n_d} displace attempts ({du} bead updates) in {dt:.1f} s",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (replicas shard "
+              f"across GPUs without traffic, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from simpimc_amd import abi, system
+
+    base = system.config_c2()
+    R = a.replicas
+    systems = [base] * R
+    flags = abi.SWEEP_ADJOINT | abi.SWEEP_ENERGY | abi.SWEEP_SK | abi.SWEEP_GR
+    if a.graph:
+        flags |= abi.SWEEP_GRAPH
+    eng = abi.Engine(systems, seed=2026)
+    for r in range(R):
+        eng.upload(system.random_path(base, seed=1000 + r, spread=0.05), r=r)
+    kc = 0.95 * float(system.kmag(base.L, base.k_n).max())
+    eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, sk_k_cut=kc, gr_n_r=200)
+    for _ in range(a.warmup):
+        eng.sweep(1, flags)
+    eng.timing(not a.graph)
+    eng.timing_reset()
+    u0 = eng.bead_updates()
+    t0 = time.perf_counter()
+    eng.sweep(a.steps, flags)  # returns after the device work has finished
+    dt = time.perf_counter() - t0
+    du = eng.bead_updates() - u0
+    value = du / dt
+    kernels = {}
+    names = ["propose", "pair", "rho", "decide", "adjoint", "estimators"]
+    for i, n in enumerate(names):
+        ms, cnt, by = eng.kernel_timing(i)
+        kernels[n] = {"ms": ms, "launches": cnt, "avg_us": 1e3 * ms / cnt if cnt else None}
+    rho_ms, rho_n, rho_bytes = eng.kernel_timing(abi.KERNEL_RHO)
+    roofline = None
+    if rho_n:
+        ach = rho_bytes / (rho_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS,
+                    "traffic": None, "kernel": "k_rho (incremental rho_k update + long-range action)",
+                    "bytes_per_launch": rho_bytes / rho_n, "avg_launch_us": 1e3 * rho_ms / rho_n}
+    c0 = eng.counters(0)
+    eng.close()
+
+    plain = None
+    if not a.no_plain:
+        pe = abi.Engine(base, seed=7)
+        pe.upload(system.random_path(base, seed=7, spread=0.05))
+        pe.bisect(64)
+        pe.displace(8)
+        n_b = a.plain_attempts
+        n_d = max(1, n_b // 8)
+        u0p = pe.bead_updates()
+        t1 = time.perf_counter()
+        pe.bisect(n_b)
+        pe.displace(n_d)
+        dtp = time.perf_counter() - t1
+        dup = pe.bead_updates() - u0p
+        plain = {"value": dup / dtp, "attempts": {"bisect": n_b, "displace": n_d}, "seconds": dtp}
+        pe.close()
+
+    cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "bead updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * dt / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"HEG N=256 M=128 r_s=1 tau=0.02 FP64, David pair action n_order=2 (2000-point table), "
+                         f"Ewald with {base.n_k} stored k-vectors, bisect level 4 + displace; "
+                         f"{R} replicas batched, adjoint dS/dr and estimators every sweep"),
+            "replicas": R,
+            "graph": bool(a.graph),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_single_chain": plain,
+        "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "rho_k_update_GBps": roofline["achieved"] if roofline else None,
+        "kernels": kernels,
+        "acceptance_replica0": {"bisect": float(c0[1]) / max(1, c0[0]), "displace": float(c0[3]) / max(1, c0[2])},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -817,12 +817,15 @@ struct Oracle {
         b0_out = bead0;
         std::vector<uint32_t> parts{p};
         double prev_action_change = 0.;
+        // the walks compare bead objects: when 2^n_level == M the last bead is the
+        // first one and no midpoint is visited (bisect_class.h:51-58,78)
+        const bool wraps = (n_bisect_beads % M) == 0;
         for (int level = (int)n_level - 1; level >= 0; --level) {
             uint32_t skip = 1u << level;
             double level_tau = tau * skip;
             double sigma = std::sqrt(lambda * level_tau);
             double old_lsp = 0., new_lsp = 0.;
-            for (uint32_t a = bead0; a != bead1; a += 2 * skip) {
+            for (uint32_t a = bead0; !wraps && a != bead1; a += 2 * skip) {
                 uint32_t bb = a + skip, cc = a + 2 * skip;
                 double rbar[3], t[3], delta_old[3], delta_new[3];
                 // old sampling, OLD mode: RBar(c, a) = a + 0.5 Dr(c, a)

@@ -134,6 +134,9 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
         return;
     }
     const int n = R.n_level, nb = 1 << n, N = R.N, M = R.M;
+    // 2^n_level == M: the window's last bead is its first one, so the reference's
+    // bead walks visit no midpoint and no kinetic link (bisect_class.h:51-58,78)
+    const bool wraps = (nb % M) == 0;
     Rng rng{make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32)), 1u, s_seq};
     const bool tape = R.tape != nullptr;
     int bad = 0;
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
     int g_base = 0;  // normals drawn so far (device RNG indexing)
     for (int level = n - 1; level >= 0; --level) {
         const int skip = 1 << level;
-        const int nmid = nb >> (level + 1);
+        const int nmid = wraps ? 0 : nb >> (level + 1);
         const double level_tau = R.tau * skip;
         const double sigma = sqrt(R.lambda * level_tau);
         const double i4lt_b = 1. / (4. * R.lambda * (0.5 * R.tau * (double)skip));
@@ -207,7 +210,7 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
             lspn = -free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dnew);
         }
         __syncthreads();
-        const int nlinks = nb >> level;
+        const int nlinks = wraps ? 0 : nb >> level;
         const double i4lt_k = 1. / (4. * R.lambda * level_tau);
         double ko = 0., kn = 0.;
         for (int i = lane; i < nlinks; i += kWave) {
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
                 pr->b0 = b0;
                 pr->n_links = nb;
                 pr->first = (b0 + 1) % M;
-                pr->count = nb - 1;
+                pr->count = wraps ? 0 : nb - 1;
                 pr->lsr = lsr;
                 pr->k_old = ko;
                 pr->k_new = kn;
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
                 pr->u0 = u;
                 pr->tape_next = cur;
             }
-            for (int j = lane; j < nb - 1; j += kWave)
+            for (int j = lane; j < (wraps ? 0 : nb - 1); j += kWave)
                 for (int d = 0; d < 3; ++d) R.prop_new[3 * j + d] = sh_new[j + 1][d];
         }
     }

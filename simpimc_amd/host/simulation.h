@@ -1,0 +1,452 @@
+// simulation.h -- host side of the engine: simpimc's Path / Action / Move /
+// Observable / Loop plug-ins for the hot path, forwarding to the C ABI.
+//
+// The classes keep the reference's names, XML attributes, defaults and call
+// structure (framework/simulation_class.h:16-45, events/loop_class.h:34-60):
+// actions are built from <Actions>, moves from <Moves>, observables from
+// <Observables>, and the <Algorithm> loop runs them.  Compute goes to the
+// device through include/spimc.h; a Loop whose only event is one Move hands
+// its n_step attempts to the device as one stage.
+#pragma once
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/spimc.h"
+#include "table_file.h"
+#include "xml_input.h"
+
+namespace spimc_host {
+
+inline void check(int rc, const char* what) {
+    if (rc != SPIMC_OK)
+        throw std::runtime_error(std::string(what) + ": " + spimc_last_error() + " (code " + std::to_string(rc) + ")");
+}
+
+// ---------------------------------------------------------------------------
+// Path: <System> and one <Species> (data_structures/path_class.h:25-69,
+// species_class.h:48-98).
+// ---------------------------------------------------------------------------
+struct Path {
+    uint32_t n_d = 3, n_bead = 0, n_part = 0;
+    double beta = 0., L = 0., tau = 0., lambda = 0., k_cut = 0.;
+    bool pbc = true;
+    std::string species;
+    Input species_in;
+    std::vector<int32_t> k_n;  // the k-vector set, grown by SetupK
+
+    explicit Path(const Input& in) {
+        const Input& sys = in.GetChild("System");
+        n_d = sys.GetAttribute<uint32_t>("n_d");
+        n_bead = sys.GetAttribute<uint32_t>("n_bead");
+        beta = sys.GetAttribute<double>("beta");
+        pbc = sys.GetAttribute<bool>("pbc", true);
+        L = sys.GetAttribute<double>("L", 0.);
+        tau = beta / (1. * n_bead);
+        auto sp = in.GetChild("Particles").GetChildList("Species");
+        if (sp.size() != 1) throw std::runtime_error("exactly one <Species> is supported");
+        species_in = sp[0];
+        species = sp[0].GetAttribute<std::string>("name");
+        n_part = sp[0].GetAttribute<uint32_t>("n_part");
+        lambda = sp[0].GetAttribute<double>("lambda");
+        if (sp[0].GetAttribute<bool>("fermi", false)) throw std::runtime_error("fermions are outside this engine");
+        if (!pbc || !(L > 0.)) throw std::runtime_error("a periodic box (pbc=1, L>0) is required");
+        SetupK(sys.GetAttribute<double>("k_cut", 2. * M_PI / L));
+    }
+
+    // KSpace::Setup grows the set only (k_space_class.h:33-40)
+    void SetupK(double kc) {
+        if (kc <= k_cut) return;
+        k_cut = kc;
+        uint32_t n = 0;
+        check(spimc_kspace_build(L, kc, nullptr, 0, &n), "kspace");
+        k_n.assign(3 * (size_t)n, 0);
+        check(spimc_kspace_build(L, kc, k_n.data(), n, &n), "kspace");
+    }
+
+    uint32_t n_k() const { return (uint32_t)(k_n.size() / 3); }
+    double kmag(uint32_t k) const {
+        const double b = 2. * M_PI / L;
+        double x = k_n[3 * k] * b, y = k_n[3 * k + 1] * b, z = k_n[3 * k + 2] * b;
+        return std::sqrt(x * x + y * y + z * z);
+    }
+
+    // Initial path of one replica, positions r[3][M][N] (species_class.h:222-334)
+    std::vector<double> InitPath(uint64_t seed) const {
+        const std::string init = species_in.GetAttribute<std::string>("init_type");
+        std::vector<double> r((size_t)3 * n_bead * n_part);
+        auto set = [&](uint32_t p, const double* x) {
+            for (uint32_t d = 0; d < 3; ++d)
+                for (uint32_t b = 0; b < n_bead; ++b) r[((size_t)d * n_bead + b) * n_part + p] = x[d];
+        };
+        if (init == "Random") {
+            const double cof = species_in.GetAttribute<double>("cofactor", 1.);
+            std::mt19937_64 g(seed);
+            std::uniform_real_distribution<double> u(-cof * L / 2., cof * L / 2.);
+            for (uint32_t p = 0; p < n_part; ++p) {
+                double x[3] = {u(g), u(g), u(g)};
+                set(p, x);
+            }
+        } else if (init == "BCC" || init == "Cubic") {
+            const int n = (int)std::ceil(std::pow(0.5 * n_part, 1.0 / 3));
+            const double cof = init == "Cubic" ? species_in.GetAttribute<double>("cofactor", 1.) : 1.;
+            const double delta = cof * L / n;
+            for (uint32_t p_i = 0; p_i < n_part; ++p_i) {
+                const int p = init == "BCC" ? (int)p_i / 2 : (int)p_i;
+                int t0 = p / (n * n), t1 = (p - t0 * n * n) / n, t2 = p - t0 * n * n - t1 * n;
+                double x[3] = {delta * t0 - 0.5 * L, delta * t1 - 0.5 * L, delta * t2 - 0.5 * L};
+                if (init == "BCC" && (p_i % 2))
+                    for (auto& v : x) v += 0.5 * delta;
+                set(p_i, x);
+            }
+        } else if (init == "Line") {
+            const double dr = species_in.GetAttribute<double>("spacing", L / n_part);
+            for (uint32_t p = 0; p < n_part; ++p) {
+                double x[3] = {p * dr, 0., 0.};
+                set(p, x);
+            }
+        } else if (init == "File") {
+            const std::string fn = species_in.GetAttribute<std::string>("file_name");
+            const bool all = species_in.GetAttribute<bool>("all_beads", false);
+            std::ifstream f(fn);
+            if (!f) throw std::runtime_error("Init file '" + fn + "' does not exist.");
+            for (uint32_t p = 0; p < n_part; ++p) {
+                if (all) {
+                    for (uint32_t b = 0; b < n_bead; ++b)
+                        for (uint32_t d = 0; d < 3; ++d) f >> r[((size_t)d * n_bead + b) * n_part + p];
+                } else {
+                    double x[3];
+                    f >> x[0] >> x[1] >> x[2];
+                    set(p, x);
+                }
+            }
+        } else {
+            throw std::runtime_error("init_type " + init + " is outside this engine");
+        }
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Actions (actions/action_class.h, actions/actions.h:13-35)
+// ---------------------------------------------------------------------------
+struct Action {
+    std::string name, type;
+    int n_images = 0;
+    uint32_t max_level = 0;
+    explicit Action(const Input& in)
+        : name(in.GetAttribute<std::string>("name")),
+          type(in.GetAttribute<std::string>("type")),
+          n_images(in.GetAttribute<int>("n_images", 0)),
+          max_level(in.GetAttribute<uint32_t>("max_level", 0)) {}
+    virtual ~Action() = default;
+    virtual void Fill(spimc_system_desc& d, Path& path) = 0;
+    virtual int Selector() const = 0;
+};
+
+struct Kinetic : Action {
+    explicit Kinetic(const Input& in) : Action(in) {}
+    void Fill(spimc_system_desc& d, Path&) override { d.kinetic_n_images = n_images; }
+    int Selector() const override { return SPIMC_ACTION_KINETIC; }
+};
+
+struct PairActionBase : Action {
+    bool use_long_range = false;
+    uint32_t n_order = 0;
+    TableFile table;
+    std::vector<double> v_long_k;
+    explicit PairActionBase(const Input& in, Path& path) : Action(in) {
+        n_order = in.GetAttribute<uint32_t>("n_order", 0);
+        if (in.GetAttribute<std::string>("species_a") != path.species || in.GetAttribute<std::string>("species_b") != path.species)
+            throw std::runtime_error("pair action " + name + ": only the homologous pair of the one species is supported");
+        use_long_range = in.GetAttribute<bool>("use_long_range", false);
+        if (use_long_range) path.SetupK(in.GetAttribute<double>("k_cut", path.k_cut));
+        if (max_level != 0) throw std::runtime_error("pair action " + name + ": max_level must be 0");
+        table.Load(in.GetAttribute<std::string>("file"));
+    }
+    int Selector() const override { return SPIMC_ACTION_PAIR; }
+    // per stored k: the table entry whose |k| equals it within 1e-8 (bare_pair_action_class.h:79-85)
+    void MatchK(const Path& path, const std::vector<double>& k_v, const std::vector<double>& vals, double scale) {
+        v_long_k.assign(path.n_k(), 0.);
+        for (uint32_t k = 0; k < path.n_k(); ++k)
+            for (size_t i = 0; i < k_v.size(); ++i)
+                if (std::fabs(path.kmag(k) - k_v[i]) < 1e-8) v_long_k[k] = vals[i] * scale;
+    }
+};
+
+// BarePairAction loader (bare_pair_action_class.h:22-96)
+struct BarePairAction : PairActionBase {
+    std::vector<double> r, v_r, r_long, v_long_r;
+    double v_long_k_0 = 0., v_long_r_0 = 0.;
+    bool is_coulomb;
+    BarePairAction(const Input& in, Path& path) : PairActionBase(in, path) {
+        is_coulomb = in.GetAttribute<bool>("is_coulomb", false);
+        r = table.Read("v/diag/r");
+        v_r = table.Read("v/diag/v_r");
+        if (use_long_range) {
+            r_long = table.Read("v/diag/r_long");
+            v_long_r = table.Read("v/diag/v_long_r");
+            v_long_r_0 = table.Scalar("v/diag/v_long_r_0");
+            v_long_k_0 = table.Scalar("v/diag/v_long_k_0");
+        }
+    }
+    void Fill(spimc_system_desc& d, Path& path) override {
+        d.pair_type = SPIMC_PAIR_BARE;
+        d.pair_use_long_range = use_long_range;
+        d.bare_is_coulomb = is_coulomb;
+        d.bare_n_r = (uint32_t)r.size();
+        d.bare_r = r.data();
+        d.bare_v_r = v_r.data();
+        if (use_long_range) {
+            MatchK(path, table.Read("v/diag/k"), table.Read("v/diag/v_long_k"), 1.);
+            d.bare_n_r_long = (uint32_t)r_long.size();
+            d.bare_r_long = r_long.data();
+            d.bare_v_long_r = v_long_r.data();
+            d.pair_v_long_k = v_long_k.data();
+            d.pair_v_long_k_0 = v_long_k_0;
+            d.pair_v_long_r_0 = v_long_r_0;
+        }
+    }
+};
+
+// DavidPairAction loader (david_pair_action_class.h:377-545)
+struct DavidPairAction : PairActionBase {
+    std::vector<double> grid, u_kj, du_kj;
+    double v_long_k_0 = 0., v_long_r_0 = 0.;
+    DavidPairAction(const Input& in, Path& path) : PairActionBase(in, path) {
+        const std::string pre = "u_kj_" + std::to_string(n_order), dpre = "du_kj_dbeta_" + std::to_string(n_order);
+        const double start = table.Scalar(pre + "/grid/start"), end = table.Scalar(pre + "/grid/end");
+        const uint32_t n_grid = (uint32_t)table.Scalar(pre + "/grid/n_grid_points");
+        const std::string gtype = table.String(pre + "/grid/type");
+        if (gtype.find("LINEAR") != std::string::npos) {
+            grid.resize(n_grid);
+            for (uint32_t i = 0; i < n_grid; ++i) grid[i] = start + i * (end - start) / (n_grid - 1);
+        } else if (table.Has(pre + "/grid/grid_points")) {
+            grid = table.Read(pre + "/grid/grid_points");
+        } else {
+            throw std::runtime_error("grid type " + gtype + " needs grid_points in this engine");
+        }
+        // tau must be one of the table's taus within 1e-6 (david_pair_action_class.h:419-430)
+        const auto& taus = table.Read(pre + "/taus");
+        const uint32_t n_tau = max_level + 1;
+        int tau_i = -1;
+        for (uint32_t t = 0; t < n_tau && t < taus.size(); ++t)
+            if (std::fabs(taus[t] - path.tau) < 1.0e-6) tau_i = (int)t;
+        if (tau_i < 0) throw std::runtime_error("tau of " + std::to_string(path.tau) + " not found.");
+        uint32_t n_val = 1;
+        for (uint32_t i = 1; i <= n_order; ++i) n_val += 1 + i;
+        const auto& V = table.Read("potential/data");
+        const auto& u = table.Read(pre + "/data");
+        const auto& du = table.Read(dpre + "/data");
+        // component 0 = potential; knots 0 .. n_grid-2 filled, the last one left 0
+        u_kj.assign((size_t)(n_val + 1) * n_grid, 0.);
+        du_kj.assign((size_t)(n_val + 1) * n_grid, 0.);
+        for (uint32_t g = 0; g + 1 < n_grid; ++g) {
+            u_kj[g] = V[g];
+            du_kj[g] = V[g];
+            for (uint32_t c = 1; c <= n_val; ++c) {
+                const size_t src = (c - 1) + (size_t)n_val * (g + (size_t)n_grid * tau_i);
+                u_kj[(size_t)c * n_grid + g] = u[src];
+                du_kj[(size_t)c * n_grid + g] = du[src];
+            }
+        }
+        if (use_long_range) {
+            v_long_r_0 = table.Scalar("squarer/v_image");
+            const auto& kp = table.Read("long_range/k_points");
+            const auto& uk = table.Read("long_range/u_k");
+            const double vol = path.L * path.L * path.L;
+            for (size_t i = 0; i < kp.size(); ++i)
+                if (std::fabs(kp[i]) < 1e-8) v_long_k_0 = uk[i] / vol;
+        }
+    }
+    void Fill(spimc_system_desc& d, Path& path) override {
+        d.pair_type = SPIMC_PAIR_DAVID;
+        d.pair_use_long_range = use_long_range;
+        d.david_n_order = n_order;
+        d.david_n_grid = (uint32_t)grid.size();
+        d.david_grid = grid.data();
+        d.david_u_kj = u_kj.data();
+        d.david_du_kj = du_kj.data();
+        if (use_long_range) {
+            const double vol = path.L * path.L * path.L;
+            MatchK(path, table.Read("long_range/k_points"), table.Read("long_range/u_k"), 1. / vol);
+            d.pair_v_long_k = v_long_k.data();
+            d.pair_v_long_k_0 = v_long_k_0;
+            d.pair_v_long_r_0 = v_long_r_0;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Events (events/event_class.h, events/moves/move_class.h, observables/*)
+// ---------------------------------------------------------------------------
+struct Engine {
+    spimc_ctx* ctx = nullptr;
+    uint32_t n_replicas = 1;
+    ~Engine() {
+        if (ctx) spimc_ctx_destroy(ctx);
+    }
+};
+
+struct Event {
+    std::string name;
+    double time_spent = 0.;
+    virtual ~Event() = default;
+    virtual void DoEvent() = 0;
+    virtual void Write(std::ostream&, uint32_t /*block*/) {}
+    // A run of n consecutive DoEvent calls; moves hand them to the device at once.
+    virtual void DoEvents(uint32_t n) {
+        for (uint32_t i = 0; i < n; ++i) DoEvent();
+    }
+    double Now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    }
+};
+
+struct Move : Event {
+    Engine& eng;
+    std::string type;
+    std::vector<uint64_t> last;  // counters at the last Write
+    Move(const Input& in, Engine& e) : eng(e), type(in.GetAttribute<std::string>("type")) {
+        name = in.GetAttribute<std::string>("name");
+    }
+    virtual int Stage(uint32_t n) = 0;
+    virtual int CounterBase() const = 0;
+    void DoEvent() override { DoEvents(1); }
+    void DoEvents(uint32_t n) override {
+        double t = Now();
+        check(Stage(n), name.c_str());
+        time_spent += Now() - t;
+    }
+    // Move::Write (move_class.h:80-94): n_attempt, n_accept, ratio since the last write
+    void Write(std::ostream& os, uint32_t) override {
+        os << "\"" << name << "\": [";
+        last.resize(2 * eng.n_replicas, 0);
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) {
+            uint64_t c[4];
+            check(spimc_move_counters(eng.ctx, r, c), "counters");
+            const uint64_t att = c[CounterBase()] - last[2 * r], acc = c[CounterBase() + 1] - last[2 * r + 1];
+            last[2 * r] = c[CounterBase()];
+            last[2 * r + 1] = c[CounterBase() + 1];
+            char b[160];
+            snprintf(b, sizeof b, "%s{\"n_attempt\": %llu, \"n_accept\": %llu, \"x\": %.10g}", r ? ", " : "",
+                     (unsigned long long)att, (unsigned long long)acc, att ? (double)acc / att : 0.);
+            os << b;
+        }
+        os << "]";
+    }
+};
+
+// Bisect (bisect_class.h:167-190)
+struct Bisect : Move {
+    Bisect(const Input& in, Engine& e, spimc_system_desc& d, const Path& path) : Move(in, e) {
+        d.bisect_n_level = in.GetAttribute<uint32_t>("n_level");
+        d.bisect_n_images = in.GetAttribute<int>("n_images", 0);
+        if (in.GetAttribute<bool>("adaptive", false))
+            throw std::runtime_error("adaptive bisection changes n_level between blocks; not supported");
+        (void)path;
+    }
+    int Stage(uint32_t n) override { return spimc_bisect_stage(eng.ctx, nullptr, n, nullptr); }
+    int CounterBase() const override { return 0; }
+};
+
+// DisplaceParticle (displace_particle_class.h:93)
+struct DisplaceParticle : Move {
+    DisplaceParticle(const Input& in, Engine& e, spimc_system_desc& d, const Path& path) : Move(in, e) {
+        d.displace_step = in.GetAttribute<double>("step_size", path.L / 10.);
+    }
+    int Stage(uint32_t n) override { return spimc_displace_stage(eng.ctx, nullptr, n, nullptr); }
+    int CounterBase() const override { return 2; }
+};
+
+struct Observable : Event {
+    Engine& eng;
+    std::string type;
+    uint32_t which = 0;
+    Observable(const Input& in, Engine& e) : eng(e), type(in.GetAttribute<std::string>("type")) {
+        name = in.GetAttribute<std::string>("name");
+    }
+    void DoEvent() override {
+        double t = Now();
+        check(spimc_obs_accumulate(eng.ctx, which), name.c_str());
+        time_spent += Now() - t;
+    }
+};
+
+// Energy (energy_class.h:286-336, Write :339-432 without sectors)
+struct Energy : Observable {
+    std::vector<std::string> action_names;
+    int estimator;
+    Energy(const Input& in, Engine& e, const std::vector<std::string>& names) : Observable(in, e), action_names(names) {
+        which = SPIMC_OBS_ENERGY;
+        const bool virial = in.GetAttribute<bool>("use_virial_estimator", false);
+        // use_ewald_virial selects the analytic adjoint virial (no reference counterpart)
+        estimator = virial ? (in.GetAttribute<bool>("use_ewald_virial", false) ? SPIMC_ENERGY_VIRIAL_ADJOINT
+                                                                                : SPIMC_ENERGY_VIRIAL_COMPAT)
+                           : SPIMC_ENERGY_THERMAL;
+        if (in.GetAttribute<bool>("measure_per_sector", false) || in.GetAttribute<bool>("measure_potential", false))
+            throw std::runtime_error("Energy: measure_per_sector/measure_potential are outside this engine");
+    }
+    void Write(std::ostream& os, uint32_t n_bead) {
+        os << "\"" << name << "\": [";
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) {
+            double e[2];
+            uint64_t nm = 0;
+            check(spimc_obs_read(eng.ctx, r, SPIMC_OBS_ENERGY, e, &nm), "energy");
+            const double norm = (double)n_bead * (nm ? nm : 1);
+            char b[200];
+            snprintf(b, sizeof b, "%s{\"total\": %.15g, \"%s\": %.15g, \"%s\": %.15g, \"n_measure\": %llu}", r ? ", " : "",
+                     (e[0] + e[1]) / norm, action_names[0].c_str(), e[0] / norm, action_names[1].c_str(), e[1] / norm,
+                     (unsigned long long)nm);
+            os << b;
+        }
+        os << "]";
+        check(spimc_obs_reset(eng.ctx, SPIMC_OBS_ENERGY), "reset");
+    }
+    void Write(std::ostream& os, uint32_t) override;  // bound by the simulation
+};
+
+struct StructureFactor : Observable {
+    double k_cut;
+    StructureFactor(const Input& in, Engine& e, Path& path) : Observable(in, e) {
+        which = SPIMC_OBS_SK;
+        k_cut = in.GetAttribute<double>("k_cut", path.k_cut);
+        path.SetupK(k_cut);
+    }
+};
+
+struct PairCorrelation : Observable {
+    double r_min, r_max;
+    uint32_t n_r;
+    PairCorrelation(const Input& in, Engine& e, const Path& path) : Observable(in, e) {
+        which = SPIMC_OBS_GR;
+        r_min = in.GetAttribute<double>("r_min", 0.);
+        r_max = in.GetAttribute<double>("r_max", path.L / 2.);
+        n_r = (uint32_t)in.GetAttribute<double>("n_r", 100);
+    }
+};
+
+struct Loop : Event {
+    int n_steps = 1;
+    std::vector<std::shared_ptr<Event>> events;
+    void DoEvent() override {
+        // a loop over a single move is one device stage of n_steps attempts
+        if (events.size() == 1 && dynamic_cast<Move*>(events[0].get())) {
+            events[0]->DoEvents((uint32_t)n_steps);
+            return;
+        }
+        for (int i = 0; i < n_steps; ++i)
+            for (auto& e : events) e->DoEvent();
+    }
+};
+
+}  // namespace spimc_host

@@ -314,3 +314,64 @@ def config_c5(n_part: int = 1024, n_bead: int = 256, r_s: float = 1.0, tau: floa
     s = config_c2(n_part=n_part, n_bead=n_bead, r_s=r_s, tau=tau, k_shell=k_shell)
     s.name = "HEG-David-C5"
     return s
+
+
+# ---------------------------------------------------------------------------
+# pair-action table files
+# ---------------------------------------------------------------------------
+# Text container of named datasets ("spimc-table 1"): one header line per
+# dataset `<name> <f64|i64|str> <count>` followed by its values.  Dataset names
+# follow the HDF5 paths the reference loaders read.
+def write_table(path: str, datasets: dict) -> None:
+    with open(path, "w") as f:
+        f.write("spimc-table 1\n")
+        for name, val in datasets.items():
+            if isinstance(val, str):
+                f.write(f"{name} str 1\n{val}\n")
+                continue
+            a = np.atleast_1d(np.asarray(val))
+            kind = "i64" if np.issubdtype(a.dtype, np.integer) else "f64"
+            f.write(f"{name} {kind} {a.size}\n")
+            fmt = "%d" if kind == "i64" else "%.17g"
+            vals = [fmt % v for v in a.ravel()]
+            for i in range(0, len(vals), 8):
+                f.write(" ".join(vals[i:i + 8]) + "\n")
+
+
+def bare_table_datasets(sys: System) -> dict:
+    """v/diag/* of BarePairAction (bare_pair_action_class.h:39-86); v_long_k keyed
+    by the distinct |k| of the system's k-set."""
+    mags = kmag(sys.L, sys.k_n)
+    uniq, idx = np.unique(np.round(mags, 12), return_index=True)
+    return {
+        "v/diag/n_r": len(sys.bare_r), "v/diag/r": sys.bare_r, "v/diag/v_r": sys.bare_v_r,
+        "v/diag/n_r_long": len(sys.bare_r_long), "v/diag/r_long": sys.bare_r_long,
+        "v/diag/v_long_r": sys.bare_v_long_r, "v/diag/v_long_r_0": sys.v_long_r_0,
+        "v/diag/n_k": len(uniq), "v/diag/k": mags[idx], "v/diag/v_long_k": sys.v_long_k[idx],
+        "v/diag/v_long_k_0": sys.v_long_k_0,
+    }
+
+
+def david_table_datasets(L: float, alpha: float, tau: float, k_n: np.ndarray, n_grid: int = 2000, n_order: int = 2,
+                         seed: int = 7, amp: float = 1e-3, r_min: float = 0.01) -> dict:
+    """u_kj_<n>/*, du_kj_dbeta_<n>/*, potential/data, long_range/*, squarer/v_image of
+    DavidPairAction (david_pair_action_class.h:395-530).  data arrays are the
+    (n_val, n_grid, n_tau) cubes flattened with the first index fastest."""
+    grid, V, u, du = david_file_arrays(L, alpha, tau, n_grid, n_order, seed, amp, r_min)
+    vol = L ** 3
+    mags = kmag(L, k_n)
+    uniq, idx = np.unique(np.round(mags, 12), return_index=True)
+    vk, vk0, vr0 = ewald_k(L, alpha, k_n)
+    pre = f"u_kj_{n_order}"
+    dpre = f"du_kj_dbeta_{n_order}"
+    return {
+        f"{pre}/grid/start": grid[0], f"{pre}/grid/end": grid[-1], f"{pre}/grid/n_grid_points": n_grid,
+        f"{pre}/grid/type": "LINEAR", f"{pre}/taus": np.array([tau]),
+        "potential/data": V,
+        f"{pre}/data": u.T.ravel(), f"{dpre}/data": du.T.ravel(),
+        "long_range/k_cut": float(mags.max()) if len(mags) else 0.0,
+        "long_range/n_k": len(uniq) + 1,
+        "long_range/k_points": np.concatenate([[0.0], mags[idx]]),
+        "long_range/u_k": np.concatenate([[vk0 * vol], vk[idx] * vol]),
+        "squarer/v_image": vr0,
+    }

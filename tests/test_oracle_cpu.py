@@ -252,6 +252,20 @@ def test_incremental_rho_matches_rebuild():
     assert np.max(np.abs(inc - o.rho_k())) < 1e-11 * s.n_part
 
 
+def test_bisect_window_equal_to_path_is_a_no_op():
+    """2^n_level == M: the bisection walks visit no bead (the last bead is the first),
+    so every attempt draws p, b0 and one uniform per level, moves nothing and accepts."""
+    s = system.config_c1(n_part=6, n_bead=8, n_level=3)
+    pos = system.random_path(s, seed=3, spread=0.1)
+    o = Oracle(s, seed=2)
+    o.upload(pos)
+    o.record(True)
+    acc = o.bisect(20)
+    assert acc.all()
+    assert len(o.recorded()) == 20 * (2 + 3)
+    assert np.array_equal(o.download(), pos)
+
+
 def test_displace_leaves_kinetic_action_zero():
     # Kinetic::GetAction over [0, M) walks no links (kinetic_class.h:111-118)
     s = system.config_c1()
