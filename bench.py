@@ -87,15 +87,19 @@ def main():
     eng = abi.Engine(systems, seed=2026)
     for r in range(R):
         eng.upload(system.random_path(base, seed=1000 + r, spread=0.05), r=r)
+    print(f"{R} replicas uploaded", file=sys.stderr, flush=True)
     kc = 0.95 * float(system.kmag(base.L, base.k_n).max())
     eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, sk_k_cut=kc, gr_n_r=200)
-    for _ in range(a.warmup):
+    for i in range(a.warmup):
         eng.sweep(1, flags)
+        print(f"warmup sweep {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
     eng.timing(not a.graph)
     eng.timing_reset()
     u0 = eng.bead_updates()
     t0 = time.perf_counter()
-    eng.sweep(a.steps, flags)  # returns after the device work has finished
+    for i in range(a.steps):
+        eng.sweep(1, flags)  # returns after the device work has finished
+        print(f"timed sweep {i + 1}/{a.steps} done at {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
     du = eng.bead_updates() - u0
     value = du / dt
@@ -129,6 +133,7 @@ def main():
         dtp = time.perf_counter() - t1
         dup = pe.bead_updates() - u0p
         plain = {"value": dup / dtp, "attempts": {"bisect": n_b, "displace": n_d}, "seconds": dtp}
+        print(f"plain single chain: {dup / dtp:.4g} bead updates/s", file=sys.stderr, flush=True)
         pe.close()
 
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)

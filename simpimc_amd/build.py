@@ -21,6 +21,22 @@ FLAGS = [
 ]
 
 
+HOST_SRC = os.path.join(HERE, "host", "spimc_main.cpp")
+HOST_OUT = os.path.join(HERE, "spimc")
+
+
+def build_host() -> str:
+    """The XML driver `spimc` (host C++ over the C ABI), linked to libspimc.so."""
+    tmp = HOST_OUT + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wno-unused-parameter", "-o", tmp, HOST_SRC, f"-L{HERE}", "-lspimc",
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("host driver build failed:\n" + res.stderr[-4000:])
+    os.replace(tmp, HOST_OUT)
+    return HOST_OUT
+
+
 def build_engine(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
     cmd = [HIPCC, *FLAGS, "-o", tmp, SRC]
@@ -35,3 +51,4 @@ def build_engine(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build_engine(verbose=True))
+    print(build_host())

@@ -396,7 +396,7 @@ struct Energy : Observable {
         if (in.GetAttribute<bool>("measure_per_sector", false) || in.GetAttribute<bool>("measure_potential", false))
             throw std::runtime_error("Energy: measure_per_sector/measure_potential are outside this engine");
     }
-    void Write(std::ostream& os, uint32_t n_bead) {
+    void Write(std::ostream& os, uint32_t n_bead) override {
         os << "\"" << name << "\": [";
         for (uint32_t r = 0; r < eng.n_replicas; ++r) {
             double e[2];
@@ -412,7 +412,6 @@ struct Energy : Observable {
         os << "]";
         check(spimc_obs_reset(eng.ctx, SPIMC_OBS_ENERGY), "reset");
     }
-    void Write(std::ostream& os, uint32_t) override;  // bound by the simulation
 };
 
 struct StructureFactor : Observable {

@@ -104,8 +104,9 @@ class System:
 
     def desc(self) -> abi.SystemDesc:
         """Builds the C descriptor; arrays stay owned by this object."""
+        # arrays handed to C stay referenced for the lifetime of this object, also
+        # across several descriptors built from it (one per replica)
         keep = self.__dict__.setdefault("_keep", [])
-        keep.clear()
 
         def arr(a, dt=np.float64):
             if a is None:
