@@ -90,12 +90,14 @@ struct Rep {
     Spl kin_dtau;
     Spl bis[kMaxKinSplines];
     // per-attempt state
+    double2* phase;        // [M][2][3][max_idx+1] e^{i m phi} of the moved beads, OLD then NEW
     Prop* prop;
     double* prop_new;      // [M][3] NEW positions of the moved slices
     double2* pair_part;    // [n_links_max] (old, new)
     double2* rho_part;     // [M][n_chunks] (old, new)
     double2* kin_part;     // interface kinetic (old, new)
     Counters* cnt;
+    unsigned int* ctx_error;  // shared by all replicas of a context
     unsigned char* accepted; // [accepted_cap]
     unsigned int accepted_cap;
     unsigned int pad3;
@@ -142,14 +144,18 @@ __device__ __forceinline__ int spl_locate(const Spl& s, double t) {
     return i;
 }
 
+// The four coefficients of one interval and component are 32-byte aligned and
+// read as two 16-byte loads.
 __device__ __forceinline__ double spl_val(const Spl& s, int i, int comp, double dt) {
-    const double* c = s.c + ((size_t)i * s.ncomp + comp) * 4;
-    return c[0] + dt * (c[1] + dt * (c[2] + dt * c[3]));
+    const double2* c = reinterpret_cast<const double2*>(s.c + ((size_t)i * s.ncomp + comp) * 4);
+    const double2 a = c[0], b = c[1];
+    return a.x + dt * (a.y + dt * (b.x + dt * b.y));
 }
 
 __device__ __forceinline__ double spl_der(const Spl& s, int i, int comp, double dt) {
-    const double* c = s.c + ((size_t)i * s.ncomp + comp) * 4;
-    return c[1] + dt * (2. * c[2] + dt * (3. * c[3]));
+    const double2* c = reinterpret_cast<const double2*>(s.c + ((size_t)i * s.ncomp + comp) * 4);
+    const double2 a = c[0], b = c[1];
+    return a.y + dt * (2. * b.x + dt * (3. * b.y));
 }
 
 __device__ __forceinline__ double spl_eval1(const Spl& s, double t) {

@@ -29,7 +29,7 @@ int fail(int code, const std::string& msg) {
             return fail(SPIMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kRhoKpt = 2;                       // k-vectors per thread in k_rho
+constexpr int kRhoKpt = 4;                       // k-vectors per thread in k_rho
 constexpr int kRhoChunk = kRhoBlock * kRhoKpt;   // k-vectors per k_rho workgroup
 constexpr int kBuildKpt = 2;
 constexpr int kRoundsPerGraph = 16;
@@ -190,6 +190,7 @@ struct spimc_ctx {
     uint64_t t_n[SPIMC_KERNEL_COUNT] = {0};
     double t_bytes[SPIMC_KERNEL_COUNT] = {0};
     unsigned long long* d_work = nullptr;
+    unsigned int* d_error = nullptr;
     unsigned long long work_seen[WORK_COUNT] = {0, 0};
 };
 
@@ -383,6 +384,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         for (size_t l = 0; l < h.kin.size(); ++l) R.kin[l] = carve_spline(cv, h.kin[l]);
         if (h.kin_dtau.n) R.kin_dtau = carve_spline(cv, h.kin_dtau);
         for (size_t l = 0; l < h.bis.size(); ++l) R.bis[l] = carve_spline(cv, h.bis[l]);
+        R.phase = cv.take<double2>((size_t)M * 6 * (mx + 1));
         R.prop = cv.take<Prop>(1);
         R.prop_new = cv.take<double>((size_t)3 * M);
         R.pair_part = cv.take<double2>(M);
@@ -611,16 +613,21 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
     }
     if ((rc = sync_and_resolve(c))) return rc;
     c->adjoint_fresh = false;
-    // replay errors
-    for (int r = 0; r < R; ++r) {
-        Counters k;
-        HIPCHK(hipMemcpy(&k, c->reps[r].dev.cnt, sizeof(Counters), hipMemcpyDeviceToHost));
-        if (k.error) {
-            unsigned long long z = 0;
-            HIPCHK(hipMemcpy(&c->reps[r].dev.cnt->error, &z, sizeof z, hipMemcpyHostToDevice));
-            char b[128];
-            snprintf(b, sizeof b, "replica %d: replay tape exhausted or invalid draw", r);
-            return fail(SPIMC_ERR_STATE, b);
+    // replay errors: one context-wide flag, then the replica that raised it
+    unsigned int any = 0;
+    HIPCHK(hipMemcpy(&any, c->d_error, sizeof any, hipMemcpyDeviceToHost));
+    if (any) {
+        HIPCHK(hipMemset(c->d_error, 0, sizeof any));
+        for (int r = 0; r < R; ++r) {
+            Counters k;
+            HIPCHK(hipMemcpy(&k, c->reps[r].dev.cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+            if (k.error) {
+                unsigned long long z = 0;
+                HIPCHK(hipMemcpy(&c->reps[r].dev.cnt->error, &z, sizeof z, hipMemcpyHostToDevice));
+                char b[128];
+                snprintf(b, sizeof b, "replica %d: replay tape exhausted or invalid draw", r);
+                return fail(SPIMC_ERR_STATE, b);
+            }
         }
     }
     if (accepted)
@@ -791,6 +798,10 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         return bail(fail(SPIMC_ERR_HIP, "memset"));
     if (hipMalloc((void**)&c->d_gr_rmax, n_replicas * sizeof(double)) != hipSuccess)
         return bail(fail(SPIMC_ERR_NOMEM, "g(r) table"));
+    if (hipMalloc((void**)&c->d_error, sizeof(unsigned int)) != hipSuccess ||
+        hipMemset(c->d_error, 0, sizeof(unsigned int)) != hipSuccess)
+        return bail(fail(SPIMC_ERR_NOMEM, "error flag"));
+    for (auto& h : c->reps) h.dev.ctx_error = c->d_error;
     int rc = upload_reps(c);
     if (rc) return bail(rc);
     if ((rc = build_work_lists(c))) return bail(rc);
@@ -810,7 +821,7 @@ int spimc_ctx_destroy(spimc_ctx* c) {
         if (h.tape) (void)hipFree(h.tape);
         if (h.gr_cnt) (void)hipFree(h.gr_cnt);
     }
-    for (void* p : {(void*)c->d_reps, (void*)c->d_counts, (void*)c->d_work, (void*)c->d_gr_rmax, (void*)c->d_tmp,
+    for (void* p : {(void*)c->d_reps, (void*)c->d_counts, (void*)c->d_work, (void*)c->d_gr_rmax, (void*)c->d_error, (void*)c->d_tmp,
                     (void*)c->d_items_tmp, (void*)c->d_pair_bis, (void*)c->d_rho_bis, (void*)c->d_pair_dis,
                     (void*)c->d_rho_dis})
         if (p) (void)hipFree(p);
@@ -974,6 +985,9 @@ int spimc_get_action(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0
                 d_out = c->d_tmp + R.M;
                 HIPCHK(hipMemcpy(c->d_items_tmp, ri.data(), ri.size() * sizeof(uint2), hipMemcpyHostToDevice));
                 const int nri = (int)ri.size();
+                const int gph = (6 * c->prop_count + 63) / 64;
+                rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_phase_proposal<<<gph, 64, 0, c->stream>>>(c->d_reps, (int)r); });
+                if (rc) return rc;
                 rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_rho<kRhoKpt><<<nri, kRhoBlock, 0, c->stream>>>(c->d_reps, c->d_items_tmp, nri, 1, nullptr); });
                 if (rc) return rc;
                 c->rho_updated = true;

@@ -61,6 +61,13 @@ __device__ __forceinline__ double tape_get(const Rep& R, long long i, int* bad) 
     return R.tape[i];
 }
 
+// Phase tables of moved bead m (OLD position xo, NEW position xn) for k_rho:
+// chain c = o * 3 + d of length max_idx + 1, by the KSpace::CalcC recurrence.
+__device__ __forceinline__ void moved_phase_chain(const Rep& R, int m, int c, const double* xo, const double* xn) {
+    const int stride = R.max_idx + 1, o = c / 3, d = c % 3;
+    phase_chain((o ? xn[d] : xo[d]) * R.box, R.max_idx, R.phase + ((size_t)m * 6 + c) * stride);
+}
+
 // ---------------------------------------------------------------------------
 // K3: rho_k of every slice (Species::InitRhoK, species_class.h:380-403);
 // particle sums in particle order per k.
@@ -269,6 +276,9 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
             }
             for (int j = lane; j < (wraps ? 0 : nb - 1); j += kWave)
                 for (int d = 0; d < 3; ++d) R.prop_new[3 * j + d] = sh_new[j + 1][d];
+            if (R.use_long_range)
+                for (int t = lane; t < (wraps ? 0 : 6 * (nb - 1)); t += kWave)
+                    moved_phase_chain(R, t / 6, t % 6, sh_old[t / 6 + 1], sh_new[t / 6 + 1]);
         }
     }
 }
@@ -335,6 +345,31 @@ __global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__
     const int p = s_p;
     for (int s = lane; s < R.M; s += kWave)
         for (int d = 0; d < 3; ++d) R.prop_new[3 * s + d] = pos_at(R, d, s, p) + s_dr[d];
+    if (R.use_long_range)
+        for (int t = lane; t < 6 * R.M; t += kWave) {
+            const int s = t / 6;
+            double xo[3], xn[3];
+            for (int d = 0; d < 3; ++d) {
+                xo[d] = pos_at(R, d, s, p);
+                xn[d] = xo[d] + s_dr[d];
+            }
+            moved_phase_chain(R, s, t % 6, xo, xn);
+        }
+}
+
+// Phase tables of an interface proposal's moved beads (before k_rho in interface mode).
+__global__ __launch_bounds__(64) void k_phase_proposal(const Rep* __restrict__ reps, int r) {
+    const Rep& R = reps[r];
+    const Prop pr = *R.prop;
+    for (int t = blockIdx.x * 64 + threadIdx.x; t < 6 * pr.count; t += gridDim.x * 64) {
+        const int m = t / 6, s = (pr.first + m) % R.M;
+        double xo[3], xn[3];
+        for (int d = 0; d < 3; ++d) {
+            xo[d] = pos_at(R, d, s, pr.p);
+            xn[d] = R.prop_new[3 * m + d];
+        }
+        moved_phase_chain(R, m, t % 6, xo, xn);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -403,36 +438,42 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps,
     if (!interface_mode && pr.status != 1) return;
     const int m = (int)(it.y >> 8), chunk = (int)(it.y & 255);
     if (m >= pr.count) return;
-    const int M = R.M, nk = R.nk, mx = R.max_idx, stride = mx + 1;
+    const int M = R.M, nk = R.nk, stride = R.max_idx + 1;
     const int s = (pr.first + m) % M;
     __shared__ double2 tab[2 * 3 * (kMaxIdx + 1)];
     __shared__ double red[2][kRhoBlock / kWave];
-    if (threadIdx.x < 6) {
-        const int o = threadIdx.x / 3, d = threadIdx.x % 3;
-        const double x = o ? R.prop_new[3 * m + d] : pos_at(R, d, s, pr.p);
-        phase_chain(x * R.box, mx, &tab[(o * 3 + d) * stride]);
-    }
-    __syncthreads();
+    // the six phase chains of this moved bead (OLD x3, NEW x3)
+    const double2* src = R.phase + (size_t)m * 6 * stride;
+    for (int t = threadIdx.x; t < 6 * stride; t += kRhoBlock) tab[t] = src[t];
     const int cur = R.sel[s];
     const double2* __restrict__ rc = R.rho + ((size_t)cur * M + s) * nk;
     double2* __restrict__ rn = R.rho + ((size_t)(1 - cur) * M + s) * nk;
+    const int k0 = chunk * kRhoBlock * KPT + threadIdx.x;
+    // issue the rho_k and k-table loads before waiting on the tables
+    double2 rv[KPT];
+    int kp[KPT];
+    double cf[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const int k = k0 + i * kRhoBlock;
+        const bool ok = k < nk;
+        rv[i] = ok ? rc[k] : make_double2(0., 0.);
+        kp[i] = ok ? R.kidx[k] : (128 | (128 << 8) | (128 << 16));
+        cf[i] = ok ? R.coef_s[k] : 0.;
+    }
+    __syncthreads();
     double ao = 0., an = 0.;
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
-        const int k = chunk * kRhoBlock * KPT + i * kRhoBlock + threadIdx.x;
-        if (k < nk) {
-            const int kp = R.kidx[k];
-            const double2 en = phase_k(&tab[3 * stride], stride, kp);
-            const double2 eo = phase_k(tab, stride, kp);
-            const double2 rv = rc[k];
-            double2 nv;
-            nv.x = rv.x + (en.x - eo.x);
-            nv.y = rv.y + (en.y - eo.y);
-            rn[k] = nv;
-            const double cf = R.coef_s[k];
-            ao += cf * (rv.x * rv.x + rv.y * rv.y);
-            an += cf * (nv.x * nv.x + nv.y * nv.y);
-        }
+        const int k = k0 + i * kRhoBlock;
+        const double2 en = phase_k(&tab[3 * stride], stride, kp[i]);
+        const double2 eo = phase_k(tab, stride, kp[i]);
+        double2 nv;
+        nv.x = rv[i].x + (en.x - eo.x);
+        nv.y = rv[i].y + (en.y - eo.y);
+        if (k < nk) rn[k] = nv;
+        ao += cf[i] * (rv[i].x * rv[i].x + rv[i].y * rv[i].y);
+        an += cf[i] * (nv.x * nv.x + nv.y * nv.y);
     }
     ao = wave_sum(ao);
     an = wave_sum(an);
@@ -449,9 +490,8 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps,
             tn += red[1][i];
         }
         R.rho_part[m * R.n_chunks + chunk] = make_double2(to, tn);
+        if (work) atomicAdd(&work[WORK_RHO_SLICEK], (unsigned long long)min(KPT * kRhoBlock, nk - chunk * kRhoBlock * KPT));
     }
-    if (work && threadIdx.x == 0)
-        atomicAdd(&work[WORK_RHO_SLICEK], (unsigned long long)min(KPT * kRhoBlock, nk - chunk * kRhoBlock * KPT));
 }
 
 // ---------------------------------------------------------------------------
@@ -512,7 +552,10 @@ __global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int
         if (si < R.accepted_cap) R.accepted[si] = (unsigned char)acc;
         c->stage_i = si + 1;
         if (R.tape) *R.tape_cursor = pr.tape_next;
-        if (pr.status == 3) c->error = 1;  // replay tape overrun / bad draw
+        if (pr.status == 3) {  // replay tape overrun / bad draw
+            c->error = 1;
+            atomicOr(R.ctx_error, 1u);
+        }
         prp->status = 0;
     }
     __syncthreads();

@@ -7,6 +7,7 @@
 // goes to <output_prefix>.spimc.jsonl (one JSON object per <Write/>).
 #include <chrono>
 #include <cstring>
+#include <iomanip>
 #include <iostream>
 
 #include "simulation.h"
@@ -111,6 +112,7 @@ struct Simulation {
               "observables");
         const std::string prefix = in.GetChild("IO").GetAttribute<std::string>("output_prefix");
         out.open(prefix + ".spimc.jsonl");
+        out.precision(17);
         ReadLoop(in.GetChild("Algorithm"), main_loop);
     }
 
