@@ -34,7 +34,25 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one device-to-device hipMemcpy of this size (a copy kernel of known bytes "
+                         "for scaling rocprof FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int) -> None:
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"copy calibration: {mib} MiB device-to-device", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(target_s: float):
@@ -78,6 +96,8 @@ def main():
         sys.exit(2)
     from simpimc_amd import abi, system
 
+    if a.calibrate_copy_mib:
+        copy_calibration(a.calibrate_copy_mib)
     base = system.config_c2()
     R = a.replicas
     systems = [base] * R

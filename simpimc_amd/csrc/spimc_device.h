@@ -91,6 +91,7 @@ struct Rep {
     Spl bis[kMaxKinSplines];
     // per-attempt state
     double2* phase;        // [M][2][3][max_idx+1] e^{i m phi} of the moved beads, OLD then NEW
+    int2* rho_rows;        // [M] (slice, committed buffer) of each moved bead, written with the proposal
     Prop* prop;
     double* prop_new;      // [M][3] NEW positions of the moved slices
     double2* pair_part;    // [n_links_max] (old, new)

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -29,7 +30,7 @@ int fail(int code, const std::string& msg) {
             return fail(SPIMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kRhoKpt = 4;                       // k-vectors per thread in k_rho
+constexpr int kRhoKpt = 8;                       // k-vectors per thread in k_rho
 constexpr int kRhoChunk = kRhoBlock * kRhoKpt;   // k-vectors per k_rho workgroup
 constexpr int kBuildKpt = 2;
 constexpr int kRoundsPerGraph = 16;
@@ -182,6 +183,8 @@ struct spimc_ctx {
     uint32_t gr_n_r = 100;
     double* d_gr_rmax = nullptr;
     bool adjoint_fresh = false;
+    // one launch for the pair and rho_k work of a round (SPIMC_FUSED=0 separates them)
+    bool fused = true;
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -385,6 +388,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         if (h.kin_dtau.n) R.kin_dtau = carve_spline(cv, h.kin_dtau);
         for (size_t l = 0; l < h.bis.size(); ++l) R.bis[l] = carve_spline(cv, h.bis[l]);
         R.phase = cv.take<double2>((size_t)M * 6 * (mx + 1));
+        R.rho_rows = cv.take<int2>(M);
         R.prop = cv.take<Prop>(1);
         R.prop_new = cv.take<double>((size_t)3 * M);
         R.pair_part = cv.take<double2>(M);
@@ -538,14 +542,20 @@ int launch_round(spimc_ctx* c, int move) {
     const int np = move == 0 ? c->n_pair_bis : c->n_pair_dis;
     const uint2* ri = move == 0 ? c->d_rho_bis : c->d_rho_dis;
     const int nr = move == 0 ? c->n_rho_bis : c->n_rho_dis;
-    if (np) {
-        const int g = (np + 3) / 4;
-        rc = launch(c, SPIMC_KERNEL_PAIR, [&] { k_pair<<<g, kPairBlock, 0, st>>>(reps, pi, np, 0, work); });
+    if (c->fused && np && nr) {
+        const int g = (np + 3) / 4 + nr;
+        rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_eval<kRhoKpt><<<g, kRhoBlock, 0, st>>>(reps, pi, np, ri, nr, work); });
         if (rc) return rc;
-    }
-    if (nr) {
-        rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_rho<kRhoKpt><<<nr, kRhoBlock, 0, st>>>(reps, ri, nr, 0, work); });
-        if (rc) return rc;
+    } else {
+        if (np) {
+            const int g = (np + 3) / 4;
+            rc = launch(c, SPIMC_KERNEL_PAIR, [&] { k_pair<<<g, kPairBlock, 0, st>>>(reps, pi, np, 0, work); });
+            if (rc) return rc;
+        }
+        if (nr) {
+            rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_rho<kRhoKpt><<<nr, kRhoBlock, 0, st>>>(reps, ri, nr, 0, work); });
+            if (rc) return rc;
+        }
     }
     return launch(c, SPIMC_KERNEL_DECIDE, [&] { k_decide<<<R, 64, 0, st>>>(reps, R); });
 }
@@ -802,6 +812,7 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         hipMemset(c->d_error, 0, sizeof(unsigned int)) != hipSuccess)
         return bail(fail(SPIMC_ERR_NOMEM, "error flag"));
     for (auto& h : c->reps) h.dev.ctx_error = c->d_error;
+    if (const char* f = std::getenv("SPIMC_FUSED")) c->fused = std::atoi(f) != 0;
     int rc = upload_reps(c);
     if (rc) return bail(rc);
     if ((rc = build_work_lists(c))) return bail(rc);

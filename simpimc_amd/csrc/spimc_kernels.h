@@ -68,6 +68,9 @@ __device__ __forceinline__ void moved_phase_chain(const Rep& R, int m, int c, co
     phase_chain((o ? xn[d] : xo[d]) * R.box, R.max_idx, R.phase + ((size_t)m * 6 + c) * stride);
 }
 
+// Row descriptor of moved bead m at slice s: k_rho streams rho[sel][s] -> rho[1-sel][s].
+__device__ __forceinline__ void moved_row(const Rep& R, int m, int s) { R.rho_rows[m] = make_int2(s, R.sel[s]); }
+
 // ---------------------------------------------------------------------------
 // K3: rho_k of every slice (Species::InitRhoK, species_class.h:380-403);
 // particle sums in particle order per k.
@@ -276,9 +279,11 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
             }
             for (int j = lane; j < (wraps ? 0 : nb - 1); j += kWave)
                 for (int d = 0; d < 3; ++d) R.prop_new[3 * j + d] = sh_new[j + 1][d];
-            if (R.use_long_range)
+            if (R.use_long_range) {
                 for (int t = lane; t < (wraps ? 0 : 6 * (nb - 1)); t += kWave)
                     moved_phase_chain(R, t / 6, t % 6, sh_old[t / 6 + 1], sh_new[t / 6 + 1]);
+                for (int j = lane; j < (wraps ? 0 : nb - 1); j += kWave) moved_row(R, j, (b0 + 1 + j) % M);
+            }
         }
     }
 }
@@ -355,6 +360,8 @@ __global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__
             }
             moved_phase_chain(R, s, t % 6, xo, xn);
         }
+    if (R.use_long_range)
+        for (int s = lane; s < R.M; s += kWave) moved_row(R, s, s);
 }
 
 // Phase tables of an interface proposal's moved beads (before k_rho in interface mode).
@@ -369,6 +376,7 @@ __global__ __launch_bounds__(64) void k_phase_proposal(const Rep* __restrict__ r
             xn[d] = R.prop_new[3 * m + d];
         }
         moved_phase_chain(R, m, t % 6, xo, xn);
+        if (t % 6 == 0) moved_row(R, m, s);
     }
 }
 
@@ -376,10 +384,8 @@ __global__ __launch_bounds__(64) void k_phase_proposal(const Rep* __restrict__ r
 // K1: short-range pair action of the moved particle on the window links,
 // OLD and NEW (pair_action_class.h:279-290), one wavefront per (replica, link).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ reps, const uint2* __restrict__ items,
-                                                     int n_items, int interface_mode,
-                                                     unsigned long long* __restrict__ work) {
-    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void pair_block(const Rep* __restrict__ reps, const uint2* __restrict__ items, int n_items,
+                                           int interface_mode, unsigned long long* __restrict__ work, int blk) {
     const int item = blk * (kPairBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
     if (item >= n_items) return;
@@ -420,6 +426,12 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ rep
     }
 }
 
+__global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ reps, const uint2* __restrict__ items,
+                                                     int n_items, int interface_mode,
+                                                     unsigned long long* __restrict__ work) {
+    pair_block(reps, items, n_items, interface_mode, work, xcd_remap(blockIdx.x, gridDim.x));
+}
+
 // ---------------------------------------------------------------------------
 // K2: incremental rho_k of the moved slices (Species::UpdateRhoK,
 // species_class.h:406-425) written to the spare buffer, fused with the
@@ -427,10 +439,8 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ rep
 // david_pair_action_class.h:105-123) for OLD and NEW.
 // ---------------------------------------------------------------------------
 template <int KPT>
-__global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps, const uint2* __restrict__ items,
-                                                   int n_items, int interface_mode,
-                                                   unsigned long long* __restrict__ work) {
-    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void rho_block(const Rep* __restrict__ reps, const uint2* __restrict__ items, int n_items,
+                                          int interface_mode, unsigned long long* __restrict__ work, int blk) {
     if (blk >= n_items) return;
     const uint2 it = items[blk];
     const Rep& R = reps[it.x];
@@ -439,13 +449,13 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps,
     const int m = (int)(it.y >> 8), chunk = (int)(it.y & 255);
     if (m >= pr.count) return;
     const int M = R.M, nk = R.nk, stride = R.max_idx + 1;
-    const int s = (pr.first + m) % M;
+    const int2 row = R.rho_rows[m];
+    const int s = row.x, cur = row.y;
     __shared__ double2 tab[2 * 3 * (kMaxIdx + 1)];
     __shared__ double red[2][kRhoBlock / kWave];
     // the six phase chains of this moved bead (OLD x3, NEW x3)
     const double2* src = R.phase + (size_t)m * 6 * stride;
     for (int t = threadIdx.x; t < 6 * stride; t += kRhoBlock) tab[t] = src[t];
-    const int cur = R.sel[s];
     const double2* __restrict__ rc = R.rho + ((size_t)cur * M + s) * nk;
     double2* __restrict__ rn = R.rho + ((size_t)(1 - cur) * M + s) * nk;
     const int k0 = chunk * kRhoBlock * KPT + threadIdx.x;
@@ -492,6 +502,29 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps,
         R.rho_part[m * R.n_chunks + chunk] = make_double2(to, tn);
         if (work) atomicAdd(&work[WORK_RHO_SLICEK], (unsigned long long)min(KPT * kRhoBlock, nk - chunk * kRhoBlock * KPT));
     }
+}
+
+template <int KPT>
+__global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps, const uint2* __restrict__ items,
+                                                   int n_items, int interface_mode,
+                                                   unsigned long long* __restrict__ work) {
+    rho_block<KPT>(reps, items, n_items, interface_mode, work, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// K1 + K2 in one launch: Gp pair blocks spread evenly among Gr rho_k blocks, so
+// the VALU-bound pair work and the HBM-bound rho_k streams run side by side on
+// every CU.  Logical block i is a pair block when floor((i+1) Gp / G) > floor(i Gp / G).
+template <int KPT>
+__global__ __launch_bounds__(kRhoBlock) void k_eval(const Rep* __restrict__ reps, const uint2* __restrict__ pitems,
+                                                    int n_pitems, const uint2* __restrict__ ritems, int n_ritems,
+                                                    unsigned long long* __restrict__ work) {
+    const long long G = gridDim.x, Gp = (n_pitems + kPairBlock / kWave - 1) / (kPairBlock / kWave);
+    const long long i = xcd_remap(blockIdx.x, (int)G);
+    const long long a = i * Gp / G, b = (i + 1) * Gp / G;
+    if (b > a)
+        pair_block(reps, pitems, n_pitems, 0, work, (int)a);
+    else
+        rho_block<KPT>(reps, ritems, n_ritems, 0, work, (int)(i - a));
 }
 
 // ---------------------------------------------------------------------------
