@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="no per-kernel HIP events (for counter-collection runs)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device hipMemcpy of this size (a copy kernel of known bytes "
                          "for scaling rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -113,7 +114,7 @@ def main():
     for i in range(a.warmup):
         eng.sweep(1, flags)
         print(f"warmup sweep {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
-    eng.timing(not a.graph)
+    eng.timing(not (a.graph or a.no_timing))
     eng.timing_reset()
     u0 = eng.bead_updates()
     t0 = time.perf_counter()

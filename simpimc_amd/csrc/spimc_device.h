@@ -30,6 +30,19 @@ struct Spl {
     double inv_h;
 };
 
+// Uniform cubic B-spline (the natural cubic spline of a uniform grid in B-spline
+// form): coefficient j of component c at c[c * ncoef + j], ncoef = knots + 2;
+// s(t) = sum_{j=0..3} c[i + j] B_j(f) with u = (t - x0) / h, i = clamp(floor u), f = u - i.
+struct BTab {
+    const double* c;  // global copy; kernels stage it through LDS
+    int knots;
+    int ncoef;
+    int ncomp;
+    int pad;
+    double x0;
+    double ih;
+};
+
 // Per-attempt proposal of one replica.
 struct Prop {
     int status;   // 0 idle, 1 pending level-0 decision, 2 rejected at a coarse level
@@ -85,6 +98,11 @@ struct Rep {
     // pair tables
     Spl v_r, v_long_r;     // Bare
     Spl u, du;             // David, ncomp = n_val + 1
+    // LDS-staged B-spline form of the link action tables (uniform grids):
+    // Bare: bt_a = v_r, bt_b = v_long_r; David: bt_a = u components 1..n_val.
+    BTab bt_a, bt_b;
+    int lds_doubles;       // doubles of LDS the pair kernel stages (0: tables stay in global memory)
+    int pad4;
     // kinetic and sampling image splines (n_images > 0 only)
     Spl kin[kMaxKinSplines];
     Spl kin_dtau;
@@ -162,6 +180,28 @@ __device__ __forceinline__ double spl_der(const Spl& s, int i, int comp, double 
 __device__ __forceinline__ double spl_eval1(const Spl& s, double t) {
     int i = spl_locate(s, t);
     return spl_val(s, i, 0, t - s.x[i]);
+}
+
+// B-spline basis at fraction f and the interval of t
+struct BBasis {
+    int i;
+    double w0, w1, w2, w3;
+};
+__device__ __forceinline__ BBasis bt_basis(const BTab& T, double t) {
+    const double u = (t - T.x0) * T.ih;
+    double fl = floor(u);
+    fl = (fl < 0.) ? 0. : (fl > (double)(T.knots - 2) ? (double)(T.knots - 2) : fl);
+    const double f = u - fl, g = 1. - f, f2 = f * f, f3 = f2 * f;
+    BBasis b;
+    b.i = (int)fl;
+    b.w0 = g * g * g * (1. / 6.);
+    b.w1 = (3. * f3 - 6. * f2 + 4.) * (1. / 6.);
+    b.w2 = (-3. * f3 + 3. * f2 + 3. * f + 1.) * (1. / 6.);
+    b.w3 = f3 * (1. / 6.);
+    return b;
+}
+__device__ __forceinline__ double bt_apply(const double* c, const BBasis& b) {
+    return c[b.i] * b.w0 + c[b.i + 1] * b.w1 + c[b.i + 2] * b.w2 + c[b.i + 3] * b.w3;
 }
 
 // ---------------------------------------------------------------------------
@@ -274,6 +314,47 @@ __device__ __forceinline__ double pair_v(const Rep& R, double r0, double r1) {
     clamp2(R.g_min, R.g_max, r0, r1);
     int i0 = spl_locate(R.u, r0), i1 = spl_locate(R.u, r1);
     return 0.5 * (spl_val(R.u, i0, 0, r0 - R.u.x[i0]) + spl_val(R.u, i1, 0, r1 - R.u.x[i1]));
+}
+
+// Link action U(r, r', s) at level 0 from LDS-staged B-spline tables `lds`
+// (Bare: [v_r][v_long_r], David: u components 1..n_val).
+__device__ __forceinline__ double pair_u_lds(const Rep& R, const double* lds, double r0, double r1, double s) {
+    if (R.pair_type == 1) {
+        clamp2(R.r_v_min, R.r_v_max, r0, r1);
+        double v = 0.;
+        if (R.is_coulomb) {
+            v += (0.5 / r0) + (0.5 / r1);
+        } else {
+            v += 0.5 * bt_apply(lds, bt_basis(R.bt_a, r0));
+            v += 0.5 * bt_apply(lds, bt_basis(R.bt_a, r1));
+        }
+        if (R.use_long_range) {
+            const double* vl = lds + (R.is_coulomb ? 0 : R.bt_a.ncoef);
+            clamp2(R.r_l_min, R.r_l_max, r0, r1);
+            v -= 0.5 * bt_apply(vl, bt_basis(R.bt_b, r0));
+            v -= 0.5 * bt_apply(vl, bt_basis(R.bt_b, r1));
+        }
+        return R.tau * v;
+    }
+    const BTab& T = R.bt_a;
+    const double q = 0.5 * (r0 + r1), z = r0 - r1;
+    clamp2(R.g_min, R.g_max, r0, r1);
+    double u = 0.5 * (bt_apply(lds, bt_basis(T, r0)) + bt_apply(lds, bt_basis(T, r1)));
+    if (s > 0.0 && q < R.g_max) {
+        const BBasis bq = bt_basis(T, q);
+        double z_2 = z * z, s_2 = s * s, i_s_2 = 1. / s_2, s_2_k = s_2;
+        for (int k = 1; k <= R.n_order; ++k) {
+            double z_2_j = 1., cur_s = s_2_k;
+            for (int j = 0; j <= k; ++j) {
+                // table component k(k+1)/2 + j + 1 is LDS component k(k+1)/2 + j
+                u += bt_apply(lds + (size_t)(k * (k + 1) / 2 + j) * T.ncoef, bq) * z_2_j * cur_s;
+                z_2_j *= z_2;
+                cur_s *= i_s_2;
+            }
+            s_2_k *= s_2;
+        }
+    }
+    return u;
 }
 
 // Partial derivatives of the level-0 link action U(r, r', s) of pair_u.

@@ -68,6 +68,28 @@ void spline_coefs(const double* x, const double* y, int n, int ncomp, int comp, 
     }
 }
 
+// B-spline coefficients of the natural cubic spline through (x_i, y_i) on a
+// uniform grid: c_{i+1} = y_i - h^2 m_i / 6 (m = second derivatives, m_0 =
+// m_{n-1} = 0), c_0 = 2 c_1 - c_2, c_{n+1} = 2 c_n - c_{n-1}.
+void bspline_coefs(const double* x, const double* y, int n, double* c) {
+    std::vector<double> m(n, 0.);
+    if (n >= 3) {
+        std::vector<double> cp(n, 0.), dp(n, 0.);
+        for (int i = 1; i < n - 1; ++i) {
+            const double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+            const double rhs = 6. * ((y[i + 1] - y[i]) / h1 - (y[i] - y[i - 1]) / h0);
+            const double den = 2. * (h0 + h1) - h0 * cp[i - 1];
+            cp[i] = h1 / den;
+            dp[i] = (rhs - h0 * dp[i - 1]) / den;
+        }
+        for (int i = n - 2; i >= 1; --i) m[i] = dp[i] - cp[i] * m[i + 1];
+    }
+    const double h = (x[n - 1] - x[0]) / (n - 1);
+    for (int i = 0; i < n; ++i) c[i + 1] = y[i] - h * h * m[i] / 6.;
+    c[0] = 2. * c[1] - c[2];
+    c[n + 1] = 2. * c[n] - c[n - 1];
+}
+
 bool grid_is_uniform(const double* x, int n) {
     if (n < 3) return true;
     const double h = (x[n - 1] - x[0]) / (n - 1);
@@ -132,8 +154,28 @@ struct Carver {
     }
 };
 
+struct BTabHost {
+    std::vector<double> c;
+    int knots = 0, ncomp = 0;
+    double x0 = 0., ih = 0.;
+};
+
+BTabHost make_btab(const double* x, int n, const std::vector<const double*>& ys) {
+    BTabHost t;
+    t.knots = n;
+    t.ncomp = (int)ys.size();
+    t.x0 = x[0];
+    t.ih = (n - 1) / (x[n - 1] - x[0]);
+    t.c.assign((size_t)(n + 2) * t.ncomp, 0.);
+    for (int k = 0; k < t.ncomp; ++k) bspline_coefs(x, ys[k], n, t.c.data() + (size_t)k * (n + 2));
+    return t;
+}
+
+constexpr size_t kPairLdsMaxBytes = 150 * 1024;
+
 struct RepHost {
     spimc_system_desc d{};
+    BTabHost bt_a, bt_b;
     std::vector<int32_t> k_n;
     std::vector<int> kidx;
     std::vector<double> kmag, coef_s, coef_b;
@@ -185,6 +227,8 @@ struct spimc_ctx {
     bool adjoint_fresh = false;
     // one launch for the pair and rho_k work of a round (SPIMC_FUSED=0 separates them)
     bool fused = true;
+    // LDS bytes/8 of the largest pair table when every replica's tables stage through LDS, else 0
+    int lds_doubles_max = 0;
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -355,6 +399,21 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         h.u = make_spline(s.david_grid, s.david_n_grid, n_val + 1, yu);
         h.du = make_spline(s.david_grid, s.david_n_grid, n_val + 1, ydu);
     }
+    // LDS-staged B-spline tables for the pair kernel (uniform grids only)
+    h.bt_a = BTabHost();
+    h.bt_b = BTabHost();
+    if (s.pair_type == SPIMC_PAIR_BARE) {
+        const bool ua = grid_is_uniform(s.bare_r, s.bare_n_r);
+        const bool ub = !lr || grid_is_uniform(s.bare_r_long, s.bare_n_r_long);
+        if (ua && ub) {
+            if (!s.bare_is_coulomb) h.bt_a = make_btab(s.bare_r, s.bare_n_r, {s.bare_v_r});
+            if (lr) h.bt_b = make_btab(s.bare_r_long, s.bare_n_r_long, {s.bare_v_long_r});
+        }
+    } else if (s.pair_type == SPIMC_PAIR_DAVID && grid_is_uniform(s.david_grid, s.david_n_grid)) {
+        std::vector<const double*> comps;
+        for (int k = 1; k <= n_val; ++k) comps.push_back(s.david_u_kj + (size_t)k * s.david_n_grid);
+        h.bt_a = make_btab(s.david_grid, s.david_n_grid, comps);
+    }
     // kinetic and sampling image splines
     h.kin.clear();
     h.bis.clear();
@@ -387,6 +446,8 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         for (size_t l = 0; l < h.kin.size(); ++l) R.kin[l] = carve_spline(cv, h.kin[l]);
         if (h.kin_dtau.n) R.kin_dtau = carve_spline(cv, h.kin_dtau);
         for (size_t l = 0; l < h.bis.size(); ++l) R.bis[l] = carve_spline(cv, h.bis[l]);
+        R.bt_a.c = cv.take<double>(std::max<size_t>(h.bt_a.c.size(), 1));
+        R.bt_b.c = cv.take<double>(std::max<size_t>(h.bt_b.c.size(), 1));
         R.phase = cv.take<double2>((size_t)M * 6 * (mx + 1));
         R.rho_rows = cv.take<int2>(M);
         R.prop = cv.take<Prop>(1);
@@ -450,12 +511,29 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         R.g_max = s.david_grid[s.david_n_grid - 1];
     }
     R.key = splitmix64(seed ^ (0x9E3779B97F4A7C15ull * (r + 1)));
+    auto set_bt = [](BTab& d, const BTabHost& hh) {
+        d.knots = hh.knots;
+        d.ncoef = hh.knots ? hh.knots + 2 : 0;
+        d.ncomp = hh.ncomp;
+        d.x0 = hh.x0;
+        d.ih = hh.ih;
+    };
+    set_bt(R.bt_a, h.bt_a);
+    set_bt(R.bt_b, h.bt_b);
+    {
+        const size_t nd = h.bt_a.c.size() + h.bt_b.c.size();
+        const bool have = (s.pair_type == SPIMC_PAIR_DAVID && h.bt_a.knots) ||
+                          (s.pair_type == SPIMC_PAIR_BARE && (s.bare_is_coulomb || h.bt_a.knots) && (!lr || h.bt_b.knots));
+        R.lds_doubles = (have && nd * sizeof(double) <= kPairLdsMaxBytes) ? (int)std::max<size_t>(nd, 1) : 0;
+    }
     R.tape = nullptr;
     R.tape_len = 0;
     R.accepted = nullptr;
     R.accepted_cap = 0;
     R.gr_cnt = nullptr;
     // uploads
+    if (!h.bt_a.c.empty()) HIPCHK(hipMemcpy((void*)R.bt_a.c, h.bt_a.c.data(), h.bt_a.c.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!h.bt_b.c.empty()) HIPCHK(hipMemcpy((void*)R.bt_b.c, h.bt_b.c.data(), h.bt_b.c.size() * sizeof(double), hipMemcpyHostToDevice));
     if (nk) {
         HIPCHK(hipMemcpy((void*)R.kidx, h.kidx.data(), nk * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy((void*)R.kmag, h.kmag.data(), nk * sizeof(double), hipMemcpyHostToDevice));
@@ -542,7 +620,13 @@ int launch_round(spimc_ctx* c, int move) {
     const int np = move == 0 ? c->n_pair_bis : c->n_pair_dis;
     const uint2* ri = move == 0 ? c->d_rho_bis : c->d_rho_dis;
     const int nr = move == 0 ? c->n_rho_bis : c->n_rho_dis;
-    if (c->fused && np && nr) {
+    if (c->lds_doubles_max > 0 && np) {
+        const size_t shmem = (size_t)c->lds_doubles_max * sizeof(double);
+        rc = launch(c, SPIMC_KERNEL_PAIR, [&] { k_pair_lds<<<R, kPairLdsBlock, shmem, st>>>(reps, R, work); });
+        if (rc) return rc;
+        if (nr) rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_rho<kRhoKpt><<<nr, kRhoBlock, 0, st>>>(reps, ri, nr, 0, work); });
+        if (rc) return rc;
+    } else if (c->fused && np && nr) {
         const int g = (np + 3) / 4 + nr;
         rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_eval<kRhoKpt><<<g, kRhoBlock, 0, st>>>(reps, pi, np, ri, nr, work); });
         if (rc) return rc;
@@ -813,6 +897,23 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         return bail(fail(SPIMC_ERR_NOMEM, "error flag"));
     for (auto& h : c->reps) h.dev.ctx_error = c->d_error;
     if (const char* f = std::getenv("SPIMC_FUSED")) c->fused = std::atoi(f) != 0;
+    {
+        bool all = true;
+        for (auto& h : c->reps) {
+            if (h.dev.pair_type != SPIMC_PAIR_NONE && h.dev.lds_doubles == 0) all = false;
+            c->lds_doubles_max = std::max(c->lds_doubles_max, h.dev.lds_doubles);
+        }
+        const char* f = std::getenv("SPIMC_PAIR_LDS");
+        if (!all || (f && std::atoi(f) == 0)) c->lds_doubles_max = 0;
+        if (c->lds_doubles_max > 0 &&
+            hipFuncSetAttribute((const void*)k_pair_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                c->lds_doubles_max * (int)sizeof(double)) != hipSuccess)
+            c->lds_doubles_max = 0;
+    }
+    if (std::getenv("SPIMC_VERBOSE"))
+        fprintf(stderr, "spimc: %u replica(s), pair kernel %s (%d LDS doubles), rho_k kernel %s\n", n_replicas,
+                c->lds_doubles_max ? "k_pair_lds" : (c->fused ? "k_eval (fused)" : "k_pair"), c->lds_doubles_max,
+                c->fused && !c->lds_doubles_max ? "fused" : "k_rho");
     int rc = upload_reps(c);
     if (rc) return bail(rc);
     if ((rc = build_work_lists(c))) return bail(rc);

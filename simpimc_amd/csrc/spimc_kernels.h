@@ -426,6 +426,54 @@ __device__ __forceinline__ void pair_block(const Rep* __restrict__ reps, const u
     }
 }
 
+// K1 with the link-action tables staged in LDS: one 1024-thread workgroup per
+// replica loads its B-spline tables once, then each wavefront takes window links
+// j = w, w + 16, ...; lanes run over the partners q.
+constexpr int kPairLdsBlock = 1024;
+__global__ __launch_bounds__(kPairLdsBlock) void k_pair_lds(const Rep* __restrict__ reps, int n_reps,
+                                                            unsigned long long* __restrict__ work) {
+    extern __shared__ double lds[];
+    const int r = xcd_remap(blockIdx.x, gridDim.x);
+    if (r >= n_reps) return;
+    const Rep& R = reps[r];
+    const Prop pr = *R.prop;
+    if (pr.status != 1) return;
+    for (int t = threadIdx.x; t < R.lds_doubles; t += kPairLdsBlock) lds[t] = (t < R.bt_a.ncoef * R.bt_a.ncomp) ? R.bt_a.c[t] : R.bt_b.c[t - R.bt_a.ncoef * R.bt_a.ncomp];
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int M = R.M, N = R.N, p = pr.p;
+    for (int j = w; j < pr.n_links; j += kPairLdsBlock / kWave) {
+        const int s0 = (pr.b0 + j) % M, s1 = (s0 + 1) % M;
+        double po0[3], po1[3], pn0[3], pn1[3];
+        for (int d = 0; d < 3; ++d) {
+            po0[d] = pos_at(R, d, s0, p);
+            po1[d] = pos_at(R, d, s1, p);
+            pn0[d] = pos_new(R, pr, d, s0);
+            pn1[d] = pos_new(R, pr, d, s1);
+        }
+        double uo = 0., un = 0.;
+        for (int q = lane; q < N; q += kWave) {
+            if (q == p) continue;
+            double q0[3], q1[3];
+            for (int d = 0; d < 3; ++d) {
+                q0[d] = pos_at(R, d, s0, q);
+                q1[d] = pos_at(R, d, s1, q);
+            }
+            LinkGeom g;
+            link_geom(po0, q0, po1, q1, R.L, R.iL, g);
+            uo += pair_u_lds(R, lds, g.r, g.rp, g.s);
+            link_geom(pn0, q0, pn1, q1, R.L, R.iL, g);
+            un += pair_u_lds(R, lds, g.r, g.rp, g.s);
+        }
+        uo = wave_sum(uo);
+        un = wave_sum(un);
+        if (lane == 0) {
+            R.pair_part[j] = make_double2(uo, un);
+            if (work) atomicAdd(&work[WORK_PAIR_LINKS], (unsigned long long)(2 * (N - 1)));
+        }
+    }
+}
+
 __global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ reps, const uint2* __restrict__ items,
                                                      int n_items, int interface_mode,
                                                      unsigned long long* __restrict__ work) {
