@@ -101,7 +101,9 @@ struct Rep {
     // LDS-staged B-spline form of the link action tables (uniform grids):
     // Bare: bt_a = v_r, bt_b = v_long_r; David: bt_a = u components 1..n_val.
     BTab bt_a, bt_b;
+    BTab bt_du;            // David du_kj_dbeta components 0..n_val (estimators)
     int lds_doubles;       // doubles of LDS the pair kernel stages (0: tables stay in global memory)
+    int lds_doubles_du;    // doubles of LDS the beta-derivative estimator stages (0: global)
     int pad4;
     // kinetic and sampling image splines (n_images > 0 only)
     Spl kin[kMaxKinSplines];
@@ -202,6 +204,20 @@ __device__ __forceinline__ BBasis bt_basis(const BTab& T, double t) {
 }
 __device__ __forceinline__ double bt_apply(const double* c, const BBasis& b) {
     return c[b.i] * b.w0 + c[b.i + 1] * b.w1 + c[b.i + 2] * b.w2 + c[b.i + 3] * b.w3;
+}
+// basis weights of the first derivative d/dt (includes 1/h)
+__device__ __forceinline__ BBasis bt_basis_d(const BTab& T, double t) {
+    const double u = (t - T.x0) * T.ih;
+    double fl = floor(u);
+    fl = (fl < 0.) ? 0. : (fl > (double)(T.knots - 2) ? (double)(T.knots - 2) : fl);
+    const double f = u - fl, g = 1. - f, f2 = f * f, h = 0.5 * T.ih;
+    BBasis b;
+    b.i = (int)fl;
+    b.w0 = -g * g * h;
+    b.w1 = (3. * f2 - 4. * f) * h;
+    b.w2 = (-3. * f2 + 2. * f + 1.) * h;
+    b.w3 = f2 * h;
+    return b;
 }
 
 // ---------------------------------------------------------------------------
@@ -355,6 +371,102 @@ __device__ __forceinline__ double pair_u_lds(const Rep& R, const double* lds, do
         }
     }
     return u;
+}
+
+// dU/dbeta of a link (CalcdUdBeta) from LDS tables: Bare uses the pair tables
+// [v_r][v_long_r]; David uses du_kj_dbeta components 0..n_val at `ldu`.
+__device__ __forceinline__ double pair_du_dbeta_lds(const Rep& R, const double* lds, const double* ldu, double r0,
+                                                    double r1, double s) {
+    if (R.pair_type == 1) return pair_u_lds(R, lds, r0, r1, s) / R.tau;
+    const BTab& T = R.bt_du;
+    const double q = 0.5 * (r0 + r1), z = r0 - r1;
+    clamp2(R.g_min, R.g_max, r0, r1);
+    const BBasis b0 = bt_basis(T, r0), b1 = bt_basis(T, r1);
+    const double v = 0.5 * (bt_apply(ldu, b0) + bt_apply(ldu, b1));
+    double du = 0.5 * (bt_apply(ldu + T.ncoef, b0) + bt_apply(ldu + T.ncoef, b1));
+    du += v;
+    if (s > 0.0 && q < R.g_max) {
+        const BBasis bq = bt_basis(T, q);
+        double z_2 = z * z, s_2 = s * s, i_s_2 = 1. / s_2, s_2_k = s_2;
+        for (int k = 1; k <= R.n_order; ++k) {
+            double z_2_j = 1., cur_s = s_2_k;
+            for (int j = 0; j <= k; ++j) {
+                du += bt_apply(ldu + (size_t)(k * (k + 1) / 2 + j + 1) * T.ncoef, bq) * z_2_j * cur_s;
+                z_2_j *= z_2;
+                cur_s *= i_s_2;
+            }
+            s_2_k *= s_2;
+        }
+    }
+    return du;
+}
+
+// CalcV from LDS tables (Bare: [v_r][v_long_r]; David: component 0 of du).
+__device__ __forceinline__ double pair_v_lds(const Rep& R, const double* lds, const double* ldu, double r0, double r1) {
+    if (R.pair_type == 1) return pair_u_lds(R, lds, r0, r1, 0.) / R.tau;
+    clamp2(R.g_min, R.g_max, r0, r1);
+    return 0.5 * (bt_apply(ldu, bt_basis(R.bt_du, r0)) + bt_apply(ldu, bt_basis(R.bt_du, r1)));
+}
+
+// dU/dr, dU/dr', dU/ds of the level-0 link action from LDS tables.
+__device__ __forceinline__ void pair_u_grad_lds(const Rep& R, const double* lds, double r0, double r1, double s,
+                                                double& ur, double& urp, double& us) {
+    ur = urp = us = 0.;
+    if (R.pair_type == 1) {
+        double a = r0, b = r1;
+        const bool ina = (a >= R.r_v_min && a <= R.r_v_max), inb = (b >= R.r_v_min && b <= R.r_v_max);
+        clamp2(R.r_v_min, R.r_v_max, a, b);
+        double ga, gb;
+        if (R.is_coulomb) {
+            ga = -0.5 / (a * a);
+            gb = -0.5 / (b * b);
+        } else {
+            ga = 0.5 * bt_apply(lds, bt_basis_d(R.bt_a, a));
+            gb = 0.5 * bt_apply(lds, bt_basis_d(R.bt_a, b));
+        }
+        ga = ina ? ga : 0.;
+        gb = inb ? gb : 0.;
+        if (R.use_long_range) {
+            const double* vl = lds + (R.is_coulomb ? 0 : R.bt_a.ncoef);
+            const bool la = ina && (a >= R.r_l_min && a <= R.r_l_max), lb = inb && (b >= R.r_l_min && b <= R.r_l_max);
+            clamp2(R.r_l_min, R.r_l_max, a, b);
+            ga -= la ? 0.5 * bt_apply(vl, bt_basis_d(R.bt_b, a)) : 0.;
+            gb -= lb ? 0.5 * bt_apply(vl, bt_basis_d(R.bt_b, b)) : 0.;
+        }
+        ur = R.tau * ga;
+        urp = R.tau * gb;
+        return;
+    }
+    const BTab& T = R.bt_a;
+    const double q = 0.5 * (r0 + r1), z = r0 - r1;
+    double a = r0, b = r1;
+    const bool ina = (a >= R.g_min && a <= R.g_max), inb = (b >= R.g_min && b <= R.g_max);
+    clamp2(R.g_min, R.g_max, a, b);
+    ur = ina ? 0.5 * bt_apply(lds, bt_basis_d(T, a)) : 0.;
+    urp = inb ? 0.5 * bt_apply(lds, bt_basis_d(T, b)) : 0.;
+    if (s > 0.0 && q < R.g_max) {
+        const BBasis bq = bt_basis(T, q), bqd = bt_basis_d(T, q);
+        for (int k = 1; k <= R.n_order; ++k)
+            for (int j = 0; j <= k; ++j) {
+                const double* cc = lds + (size_t)(k * (k + 1) / 2 + j) * T.ncoef;
+                const double c = bt_apply(cc, bq), cd = bt_apply(cc, bqd);
+                const int e = k - j;
+                double zp = 1., zm = 0., sp = 1., sm = 0.;
+                for (int t = 0; t < 2 * j; ++t) {
+                    zm = zp;
+                    zp *= z;
+                }
+                for (int t = 0; t < 2 * e; ++t) {
+                    sm = sp;
+                    sp *= s;
+                }
+                const double dqterm = 0.5 * cd * zp * sp;
+                const double dz = c * (2. * j) * zm * sp;
+                ur += dqterm + dz;
+                urp += dqterm - dz;
+                us += c * zp * (2. * e) * sm;
+            }
+    }
 }
 
 // Partial derivatives of the level-0 link action U(r, r', s) of pair_u.

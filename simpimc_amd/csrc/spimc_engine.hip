@@ -175,7 +175,7 @@ constexpr size_t kPairLdsMaxBytes = 150 * 1024;
 
 struct RepHost {
     spimc_system_desc d{};
-    BTabHost bt_a, bt_b;
+    BTabHost bt_a, bt_b, bt_du;
     std::vector<int32_t> k_n;
     std::vector<int> kidx;
     std::vector<double> kmag, coef_s, coef_b;
@@ -225,10 +225,13 @@ struct spimc_ctx {
     uint32_t gr_n_r = 100;
     double* d_gr_rmax = nullptr;
     bool adjoint_fresh = false;
-    // one launch for the pair and rho_k work of a round (SPIMC_FUSED=0 separates them)
-    bool fused = true;
+    // one launch for the pair and rho_k work of a round (SPIMC_FUSED=1); measured slower
+    // than two launches on MI355X (the pair blocks' registers cap the rho_k blocks' occupancy)
+    bool fused = false;
     // LDS bytes/8 of the largest pair table when every replica's tables stage through LDS, else 0
     int lds_doubles_max = 0;
+    // the same for the estimator kernels (pair tables + du table), else 0
+    int lds_est_max = 0;
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -410,9 +413,11 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
             if (lr) h.bt_b = make_btab(s.bare_r_long, s.bare_n_r_long, {s.bare_v_long_r});
         }
     } else if (s.pair_type == SPIMC_PAIR_DAVID && grid_is_uniform(s.david_grid, s.david_n_grid)) {
-        std::vector<const double*> comps;
+        std::vector<const double*> comps, dcomps;
         for (int k = 1; k <= n_val; ++k) comps.push_back(s.david_u_kj + (size_t)k * s.david_n_grid);
+        for (int k = 0; k <= n_val; ++k) dcomps.push_back(s.david_du_kj + (size_t)k * s.david_n_grid);
         h.bt_a = make_btab(s.david_grid, s.david_n_grid, comps);
+        h.bt_du = make_btab(s.david_grid, s.david_n_grid, dcomps);
     }
     // kinetic and sampling image splines
     h.kin.clear();
@@ -448,6 +453,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         for (size_t l = 0; l < h.bis.size(); ++l) R.bis[l] = carve_spline(cv, h.bis[l]);
         R.bt_a.c = cv.take<double>(std::max<size_t>(h.bt_a.c.size(), 1));
         R.bt_b.c = cv.take<double>(std::max<size_t>(h.bt_b.c.size(), 1));
+        R.bt_du.c = cv.take<double>(std::max<size_t>(h.bt_du.c.size(), 1));
         R.phase = cv.take<double2>((size_t)M * 6 * (mx + 1));
         R.rho_rows = cv.take<int2>(M);
         R.prop = cv.take<Prop>(1);
@@ -520,11 +526,16 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     };
     set_bt(R.bt_a, h.bt_a);
     set_bt(R.bt_b, h.bt_b);
+    set_bt(R.bt_du, h.bt_du);
     {
         const size_t nd = h.bt_a.c.size() + h.bt_b.c.size();
         const bool have = (s.pair_type == SPIMC_PAIR_DAVID && h.bt_a.knots) ||
                           (s.pair_type == SPIMC_PAIR_BARE && (s.bare_is_coulomb || h.bt_a.knots) && (!lr || h.bt_b.knots));
         R.lds_doubles = (have && nd * sizeof(double) <= kPairLdsMaxBytes) ? (int)std::max<size_t>(nd, 1) : 0;
+        R.lds_doubles_du = 0;
+        if (s.pair_type == SPIMC_PAIR_DAVID && R.lds_doubles &&
+            (nd + h.bt_du.c.size()) * sizeof(double) <= kPairLdsMaxBytes)
+            R.lds_doubles_du = (int)h.bt_du.c.size();
     }
     R.tape = nullptr;
     R.tape_len = 0;
@@ -534,6 +545,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     // uploads
     if (!h.bt_a.c.empty()) HIPCHK(hipMemcpy((void*)R.bt_a.c, h.bt_a.c.data(), h.bt_a.c.size() * sizeof(double), hipMemcpyHostToDevice));
     if (!h.bt_b.c.empty()) HIPCHK(hipMemcpy((void*)R.bt_b.c, h.bt_b.c.data(), h.bt_b.c.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!h.bt_du.c.empty()) HIPCHK(hipMemcpy((void*)R.bt_du.c, h.bt_du.c.data(), h.bt_du.c.size() * sizeof(double), hipMemcpyHostToDevice));
     if (nk) {
         HIPCHK(hipMemcpy((void*)R.kidx, h.kidx.data(), nk * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy((void*)R.kmag, h.kmag.data(), nk * sizeof(double), hipMemcpyHostToDevice));
@@ -747,8 +759,15 @@ int est_kin_dbeta(spimc_ctx* c, Rep* base, int nrep) {
     return launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_kin_dbeta<<<g, 256, 0, c->stream>>>(base, kSlotKin); });
 }
 int est_pair_whole(spimc_ctx* c, Rep* base, int nrep, int which) {
-    dim3 g(c->max_M, nrep);
-    int rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_pair_whole<<<g, 256, 0, c->stream>>>(base, which, kSlotPair); });
+    int rc;
+    if (c->lds_est_max > 0) {
+        const size_t shmem = (size_t)c->lds_est_max * sizeof(double);
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS,
+                    [&] { k_pair_whole_lds<<<nrep, kAdjLdsBlock, shmem, c->stream>>>(base, which, kSlotPair); });
+    } else {
+        dim3 g(c->max_M, nrep);
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_pair_whole<<<g, 256, 0, c->stream>>>(base, which, kSlotPair); });
+    }
     if (rc || !c->any_long) return rc;
     dim3 g2((c->max_nk + 255) / 256, nrep);
     return launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_long_whole<<<g2, 256, 0, c->stream>>>(base); });
@@ -760,7 +779,14 @@ int est_finalize(spimc_ctx* c, Rep* base, int nrep, int what, int acc) {
 int est_gradient(spimc_ctx* c, Rep* base, int nrep, uint32_t terms, int compat, int out_est) {
     dim3 g((c->max_N + 63) / 64, c->max_M, nrep);
     if (compat) return launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_grad_compat<<<g, 64, 0, c->stream>>>(base, out_est); });
-    int rc = launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_adjoint_local<<<g, 64, 0, c->stream>>>(base, terms, out_est); });
+    int rc;
+    if (c->lds_est_max > 0) {
+        const size_t shmem = (size_t)c->lds_est_max * sizeof(double);
+        rc = launch(c, SPIMC_KERNEL_ADJOINT,
+                    [&] { k_adjoint_local_lds<<<nrep, kAdjLdsBlock, shmem, c->stream>>>(base, terms, out_est); });
+    } else {
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_adjoint_local<<<g, 64, 0, c->stream>>>(base, terms, out_est); });
+    }
     if (rc || !(terms & SPIMC_GRAD_PAIR_LONG) || !c->any_long) return rc;
     int mx = 0;
     for (auto& h : c->reps) mx = std::max(mx, h.dev.max_idx);
@@ -909,6 +935,18 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
             hipFuncSetAttribute((const void*)k_pair_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 c->lds_doubles_max * (int)sizeof(double)) != hipSuccess)
             c->lds_doubles_max = 0;
+        bool est = c->lds_doubles_max > 0;
+        for (auto& h : c->reps) {
+            if (h.dev.pair_type == SPIMC_PAIR_DAVID && h.dev.lds_doubles_du == 0) est = false;
+            c->lds_est_max = std::max(c->lds_est_max, h.dev.lds_doubles + h.dev.lds_doubles_du);
+        }
+        if (!est) c->lds_est_max = 0;
+        if (c->lds_est_max > 0) {
+            const int bytes = c->lds_est_max * (int)sizeof(double);
+            if (hipFuncSetAttribute((const void*)k_adjoint_local_lds, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess ||
+                hipFuncSetAttribute((const void*)k_pair_whole_lds, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+                c->lds_est_max = 0;
+        }
     }
     if (std::getenv("SPIMC_VERBOSE"))
         fprintf(stderr, "spimc: %u replica(s), pair kernel %s (%d LDS doubles), rho_k kernel %s\n", n_replicas,

@@ -965,6 +965,169 @@ __global__ __launch_bounds__(64) void k_adjoint_local(const Rep* __restrict__ re
     o[2] = g[2];
 }
 
+// Loads a replica's pair tables (and optionally the du table) into LDS.
+__device__ __forceinline__ void stage_tables(const Rep& R, double* lds, bool with_du) {
+    const int na = R.bt_a.ncoef * R.bt_a.ncomp;
+    for (int t = threadIdx.x; t < R.lds_doubles; t += blockDim.x) lds[t] = (t < na) ? R.bt_a.c[t] : R.bt_b.c[t - na];
+    if (with_du) {
+        double* ldu = lds + R.lds_doubles;
+        for (int t = threadIdx.x; t < R.lds_doubles_du; t += blockDim.x) ldu[t] = R.bt_du.c[t];
+    }
+    __syncthreads();
+}
+
+// Analytic adjoint, local terms with LDS tables: one 1024-thread workgroup per
+// replica; thread t owns particle i = t % N over a contiguous slice range and
+// walks it in order, so each link (b, b+1) of (i, j) is evaluated once for i:
+// its r_i(b) part goes to bead b, its r_i(b+1) part is carried to bead b+1.
+constexpr int kAdjLdsBlock = 1024;
+__global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_local_lds(const Rep* __restrict__ reps, unsigned int terms,
+                                                                    int out_est) {
+    extern __shared__ double lds[];
+    const Rep& R = reps[blockIdx.x];
+    stage_tables(R, lds, false);
+    const int M = R.M, N = R.N;
+    const int per = kAdjLdsBlock / N > 0 ? kAdjLdsBlock / N : 1;  // slice ranges per particle
+    const int n_units = N * per;
+    for (int unit = threadIdx.x; unit < n_units; unit += kAdjLdsBlock) {
+        const int i = unit % N, part = unit / N;
+        const int b_lo = (int)((long long)M * part / per), b_hi = (int)((long long)M * (part + 1) / per);
+        if (b_lo >= b_hi) continue;
+        double carry[3] = {0., 0., 0.};  // d/dr_i(b) from link (b-1, b)
+        // the link entering the range
+        {
+            const int bp = (b_lo + M - 1) % M;
+            double rp_[3], ri_[3];
+            for (int d = 0; d < 3; ++d) {
+                rp_[d] = pos_at(R, d, bp, i);
+                ri_[d] = pos_at(R, d, b_lo, i);
+            }
+            if ((terms & 2u) && R.pair_type != 0)
+                for (int j = 0; j < N; ++j) {
+                    if (j == i) continue;
+                    double jp[3], j0[3];
+                    for (int d = 0; d < 3; ++d) {
+                        jp[d] = pos_at(R, d, bp, j);
+                        j0[d] = pos_at(R, d, b_lo, j);
+                    }
+                    LinkGeom gg;
+                    double ur, urp, us;
+                    link_geom(rp_, jp, ri_, j0, R.L, R.iL, gg);
+                    pair_u_grad_lds(R, lds, gg.r, gg.rp, gg.s, ur, urp, us);
+                    for (int d = 0; d < 3; ++d)
+                        carry[d] -= urp * gg.rpv[d] / gg.rp - ((gg.s > 0.) ? us * gg.sv[d] / gg.s : 0.);
+                }
+        }
+        for (int b = b_lo; b < b_hi; ++b) {
+            const int bn = (b + 1) % M, bp = (b + M - 1) % M;
+            double ri[3], rn[3], rpv[3], g[3], nxt[3] = {0., 0., 0.};
+            for (int d = 0; d < 3; ++d) {
+                ri[d] = pos_at(R, d, b, i);
+                rn[d] = pos_at(R, d, bn, i);
+                rpv[d] = pos_at(R, d, bp, i);
+                g[d] = carry[d];
+            }
+            if (terms & 1u) {
+                for (int d = 0; d < 3; ++d) {
+                    const double df = minimg(ri[d] - rn[d], R.L, R.iL), db = minimg(rpv[d] - ri[d], R.L, R.iL);
+                    double kf = 2. * df * R.kin_i4lt, kb = 2. * db * R.kin_i4lt;
+                    if (R.kin_images > 0) {
+                        int ia = spl_locate(R.kin[0], df), ib = spl_locate(R.kin[0], db);
+                        kf += spl_der(R.kin[0], ia, 0, df - R.kin[0].x[ia]);
+                        kb += spl_der(R.kin[0], ib, 0, db - R.kin[0].x[ib]);
+                    }
+                    g[d] += kf - kb;
+                }
+            }
+            if ((terms & 2u) && R.pair_type != 0)
+                for (int j = 0; j < N; ++j) {
+                    if (j == i) continue;
+                    double j0[3], jn[3];
+                    for (int d = 0; d < 3; ++d) {
+                        j0[d] = pos_at(R, d, b, j);
+                        jn[d] = pos_at(R, d, bn, j);
+                    }
+                    LinkGeom gg;
+                    double ur, urp, us;
+                    link_geom(ri, j0, rn, jn, R.L, R.iL, gg);
+                    pair_u_grad_lds(R, lds, gg.r, gg.rp, gg.s, ur, urp, us);
+                    const double so = (gg.s > 0.) ? us / gg.s : 0.;
+                    for (int d = 0; d < 3; ++d) {
+                        g[d] -= ur * gg.rv[d] / gg.r + so * gg.sv[d];
+                        nxt[d] -= urp * gg.rpv[d] / gg.rp - so * gg.sv[d];
+                    }
+                }
+            double* o = (out_est ? R.grad_est : R.grad) + 3 * ((size_t)b * N + i);
+            o[0] = g[0];
+            o[1] = g[1];
+            o[2] = g[2];
+            carry[0] = nxt[0];
+            carry[1] = nxt[1];
+            carry[2] = nxt[2];
+        }
+    }
+}
+
+// Whole-path pair sums with LDS tables (which 0: CalcdUdBeta, 1: CalcV on plain
+// minimum images): one 1024-thread workgroup per replica, per-slice partials.
+__global__ __launch_bounds__(kAdjLdsBlock) void k_pair_whole_lds(const Rep* __restrict__ reps, int which, int slot) {
+    extern __shared__ double lds[];
+    const Rep& R = reps[blockIdx.x];
+    stage_tables(R, lds, R.pair_type == 2);
+    const double* ldu = lds + R.lds_doubles;
+    __shared__ double red[kAdjLdsBlock / kWave];
+    const int N = R.N, M = R.M, half = N / 2;
+    for (int b = 0; b < M; ++b) {
+        const int b1 = (b + 1) % M;
+        double t = 0.;
+        // pairs (i, i + d mod N), d = 1..N/2: thread handles i = tid % N and a slice of d
+        const int per = kAdjLdsBlock / N > 0 ? kAdjLdsBlock / N : 1;
+        for (int unit = threadIdx.x; unit < N * per; unit += kAdjLdsBlock) {
+            const int i = unit % N, part = unit / N;
+            const int d_lo = 1 + (int)((long long)half * part / per), d_hi = 1 + (int)((long long)half * (part + 1) / per);
+            double a0[3], a1[3];
+            for (int d = 0; d < 3; ++d) {
+                a0[d] = pos_at(R, d, b, i);
+                a1[d] = pos_at(R, d, b1, i);
+            }
+            for (int dd = d_lo; dd < d_hi; ++dd) {
+                if ((N % 2 == 0) && dd == half && i >= half) continue;
+                const int j = (i + dd) % N;
+                double c0[3], c1[3];
+                for (int d = 0; d < 3; ++d) {
+                    c0[d] = pos_at(R, d, b, j);
+                    c1[d] = pos_at(R, d, b1, j);
+                }
+                if (which == 0) {
+                    LinkGeom g;
+                    if (i < j)
+                        link_geom(a0, c0, a1, c1, R.L, R.iL, g);
+                    else
+                        link_geom(c0, a0, c1, a1, R.L, R.iL, g);
+                    t += pair_du_dbeta_lds(R, lds, ldu, g.r, g.rp, g.s);
+                } else {
+                    double v0[3], v1[3];
+                    for (int d = 0; d < 3; ++d) {
+                        v0[d] = minimg(a0[d] - c0[d], R.L, R.iL);
+                        v1[d] = minimg(a1[d] - c1[d], R.L, R.iL);
+                    }
+                    t += pair_v_lds(R, lds, ldu, sqrt(v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2]),
+                                    sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]));
+                }
+            }
+        }
+        t = wave_sum(t);
+        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tt = 0.;
+            for (int w = 0; w < kAdjLdsBlock / kWave; ++w) tt += red[w];
+            R.scratch[slot * M + b] = tt;
+        }
+        __syncthreads();
+    }
+}
+
 // Analytic adjoint, Ewald term: d/dr_i of long_scale sum_k coef_s |rho(b,k)|^2
 // = -2 long_scale sum_k coef_s k Im(conj(rho) e^{i k.r_i}).  Block = 64 beads of
 // one slice; rho(b,:) staged through LDS in k tiles; per-bead phase tables in LDS.
