@@ -255,6 +255,28 @@ __device__ __forceinline__ void link_geom(const double* a0, const double* c0, co
     g.s = sqrt(g.sv[0] * g.sv[0] + g.sv[1] * g.sv[1] + g.sv[2] * g.sv[2]);
 }
 
+// DrDrpDrrp magnitudes |r|, |r'| and the squared |s|^2 (the link action needs
+// only even powers of s).
+__device__ __forceinline__ void link_rr_s2(const double* a0, const double* c0, const double* a1, const double* c1,
+                                           double L, double iL, double& r, double& rp, double& s2) {
+#pragma clang fp contract(off)
+    double rr = 0., pp = 0., ss = 0.;
+    for (int d = 0; d < 3; ++d) {
+        double rv = c0[d] - a0[d];
+        double rpv = c1[d] - a1[d];
+        rv -= rint(rv * iL) * L;
+        rpv += rint((rv - rpv) * iL) * L;
+        double sv = rv - rpv;
+        sv -= rint(sv * iL) * L;
+        rr += rv * rv;
+        pp += rpv * rpv;
+        ss += sv * sv;
+    }
+    r = sqrt(rr);
+    rp = sqrt(pp);
+    s2 = ss;
+}
+
 __device__ __forceinline__ void clamp2(double lo, double hi, double& a, double& b) {
     a = (a > hi) ? hi : ((a < lo) ? lo : a);
     b = (b > hi) ? hi : ((b < lo) ? lo : b);
@@ -467,6 +489,32 @@ __device__ __forceinline__ void pair_u_grad_lds(const Rep& R, const double* lds,
                 us += c * zp * (2. * e) * sm;
             }
     }
+}
+
+// The same link action from |s|^2: sum_{k,j} c_kj(q) (z^2)^j (s^2)^(k-j) with the
+// powers formed directly (the reference forms s^2k (1/s^2)^j; equal in exact
+// arithmetic).
+__device__ __forceinline__ double pair_u_lds_s2(const Rep& R, const double* lds, double r0, double r1, double s2) {
+    if (R.pair_type == 1) return pair_u_lds(R, lds, r0, r1, 0.);
+    const BTab& T = R.bt_a;
+    const double q = 0.5 * (r0 + r1), z = r0 - r1;
+    clamp2(R.g_min, R.g_max, r0, r1);
+    double u = 0.5 * (bt_apply(lds, bt_basis(T, r0)) + bt_apply(lds, bt_basis(T, r1)));
+    if (s2 > 0.0 && q < R.g_max) {
+        const BBasis bq = bt_basis(T, q);
+        const double z2 = z * z;
+        double sp[4], zp[4];
+        sp[0] = 1.;
+        zp[0] = 1.;
+        for (int e = 1; e <= 3; ++e) {
+            sp[e] = sp[e - 1] * s2;
+            zp[e] = zp[e - 1] * z2;
+        }
+        for (int k = 1; k <= R.n_order; ++k)
+            for (int j = 0; j <= k; ++j)
+                u += bt_apply(lds + (size_t)(k * (k + 1) / 2 + j) * T.ncoef, bq) * zp[j] * sp[k - j];
+    }
+    return u;
 }
 
 // Partial derivatives of the level-0 link action U(r, r', s) of pair_u.

@@ -230,8 +230,9 @@ struct spimc_ctx {
     bool fused = false;
     // LDS bytes/8 of the largest pair table when every replica's tables stage through LDS, else 0
     int lds_doubles_max = 0;
-    // the same for the estimator kernels (pair tables + du table), else 0
-    int lds_est_max = 0;
+    // LDS doubles of the adjoint kernel (pair tables) and of the whole-path
+    // estimator kernel (Bare: pair tables, David: du table); 0 = global tables
+    int lds_adj_max = 0, lds_whole_max = 0;
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -533,8 +534,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
                           (s.pair_type == SPIMC_PAIR_BARE && (s.bare_is_coulomb || h.bt_a.knots) && (!lr || h.bt_b.knots));
         R.lds_doubles = (have && nd * sizeof(double) <= kPairLdsMaxBytes) ? (int)std::max<size_t>(nd, 1) : 0;
         R.lds_doubles_du = 0;
-        if (s.pair_type == SPIMC_PAIR_DAVID && R.lds_doubles &&
-            (nd + h.bt_du.c.size()) * sizeof(double) <= kPairLdsMaxBytes)
+        if (s.pair_type == SPIMC_PAIR_DAVID && h.bt_du.knots && h.bt_du.c.size() * sizeof(double) <= kPairLdsMaxBytes)
             R.lds_doubles_du = (int)h.bt_du.c.size();
     }
     R.tape = nullptr;
@@ -760,8 +760,8 @@ int est_kin_dbeta(spimc_ctx* c, Rep* base, int nrep) {
 }
 int est_pair_whole(spimc_ctx* c, Rep* base, int nrep, int which) {
     int rc;
-    if (c->lds_est_max > 0) {
-        const size_t shmem = (size_t)c->lds_est_max * sizeof(double);
+    if (c->lds_whole_max > 0) {
+        const size_t shmem = (size_t)c->lds_whole_max * sizeof(double);
         rc = launch(c, SPIMC_KERNEL_ESTIMATORS,
                     [&] { k_pair_whole_lds<<<nrep, kAdjLdsBlock, shmem, c->stream>>>(base, which, kSlotPair); });
     } else {
@@ -780,8 +780,8 @@ int est_gradient(spimc_ctx* c, Rep* base, int nrep, uint32_t terms, int compat, 
     dim3 g((c->max_N + 63) / 64, c->max_M, nrep);
     if (compat) return launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_grad_compat<<<g, 64, 0, c->stream>>>(base, out_est); });
     int rc;
-    if (c->lds_est_max > 0) {
-        const size_t shmem = (size_t)c->lds_est_max * sizeof(double);
+    if (c->lds_adj_max > 0) {
+        const size_t shmem = (size_t)c->lds_adj_max * sizeof(double);
         rc = launch(c, SPIMC_KERNEL_ADJOINT,
                     [&] { k_adjoint_local_lds<<<nrep, kAdjLdsBlock, shmem, c->stream>>>(base, terms, out_est); });
     } else {
@@ -935,23 +935,28 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
             hipFuncSetAttribute((const void*)k_pair_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 c->lds_doubles_max * (int)sizeof(double)) != hipSuccess)
             c->lds_doubles_max = 0;
-        bool est = c->lds_doubles_max > 0;
+        c->lds_adj_max = c->lds_doubles_max;
+        if (c->lds_adj_max > 0 &&
+            hipFuncSetAttribute((const void*)k_adjoint_local_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                c->lds_adj_max * (int)sizeof(double)) != hipSuccess)
+            c->lds_adj_max = 0;
+        bool whole = c->lds_doubles_max > 0;
         for (auto& h : c->reps) {
-            if (h.dev.pair_type == SPIMC_PAIR_DAVID && h.dev.lds_doubles_du == 0) est = false;
-            c->lds_est_max = std::max(c->lds_est_max, h.dev.lds_doubles + h.dev.lds_doubles_du);
+            const int need = h.dev.pair_type == SPIMC_PAIR_DAVID ? h.dev.lds_doubles_du : h.dev.lds_doubles;
+            if (h.dev.pair_type != SPIMC_PAIR_NONE && need == 0) whole = false;
+            c->lds_whole_max = std::max(c->lds_whole_max, need);
         }
-        if (!est) c->lds_est_max = 0;
-        if (c->lds_est_max > 0) {
-            const int bytes = c->lds_est_max * (int)sizeof(double);
-            if (hipFuncSetAttribute((const void*)k_adjoint_local_lds, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess ||
-                hipFuncSetAttribute((const void*)k_pair_whole_lds, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
-                c->lds_est_max = 0;
-        }
+        if (!whole) c->lds_whole_max = 0;
+        if (c->lds_whole_max > 0 &&
+            hipFuncSetAttribute((const void*)k_pair_whole_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                c->lds_whole_max * (int)sizeof(double)) != hipSuccess)
+            c->lds_whole_max = 0;
     }
     if (std::getenv("SPIMC_VERBOSE"))
-        fprintf(stderr, "spimc: %u replica(s), pair kernel %s (%d LDS doubles), rho_k kernel %s\n", n_replicas,
-                c->lds_doubles_max ? "k_pair_lds" : (c->fused ? "k_eval (fused)" : "k_pair"), c->lds_doubles_max,
-                c->fused && !c->lds_doubles_max ? "fused" : "k_rho");
+        fprintf(stderr, "spimc: %u replica(s), pair kernel %s (%d LDS doubles), rho_k kernel %s, adjoint %s, estimators %s\n",
+                n_replicas, c->lds_doubles_max ? "k_pair_lds" : (c->fused ? "k_eval (fused)" : "k_pair"),
+                c->lds_doubles_max, c->fused && !c->lds_doubles_max ? "fused" : "k_rho",
+                c->lds_adj_max ? "k_adjoint_local_lds" : "k_adjoint_local", c->lds_whole_max ? "k_pair_whole_lds" : "k_pair_whole");
     int rc = upload_reps(c);
     if (rc) return bail(rc);
     if ((rc = build_work_lists(c))) return bail(rc);

@@ -452,18 +452,29 @@ __global__ __launch_bounds__(kPairLdsBlock) void k_pair_lds(const Rep* __restric
             pn1[d] = pos_new(R, pr, d, s1);
         }
         double uo = 0., un = 0.;
-        for (int q = lane; q < N; q += kWave) {
-            if (q == p) continue;
-            double q0[3], q1[3];
+        // two partners per lane per step: independent evaluation chains for the scheduler
+        for (int q = lane; q < N; q += 2 * kWave) {
+            const int q2 = q + kWave;
+            const bool ok1 = q != p, ok2 = (q2 < N) && (q2 != p);
+            const int qq = (q2 < N) ? q2 : q;
+            double a0[3], a1[3], b0[3], b1[3];
             for (int d = 0; d < 3; ++d) {
-                q0[d] = pos_at(R, d, s0, q);
-                q1[d] = pos_at(R, d, s1, q);
+                a0[d] = pos_at(R, d, s0, q);
+                a1[d] = pos_at(R, d, s1, q);
+                b0[d] = pos_at(R, d, s0, qq);
+                b1[d] = pos_at(R, d, s1, qq);
             }
-            LinkGeom g;
-            link_geom(po0, q0, po1, q1, R.L, R.iL, g);
-            uo += pair_u_lds(R, lds, g.r, g.rp, g.s);
-            link_geom(pn0, q0, pn1, q1, R.L, R.iL, g);
-            un += pair_u_lds(R, lds, g.r, g.rp, g.s);
+            double r, rp, s2;
+            link_rr_s2(po0, a0, po1, a1, R.L, R.iL, r, rp, s2);
+            const double uo1 = pair_u_lds_s2(R, lds, r, rp, s2);
+            link_rr_s2(pn0, a0, pn1, a1, R.L, R.iL, r, rp, s2);
+            const double un1 = pair_u_lds_s2(R, lds, r, rp, s2);
+            link_rr_s2(po0, b0, po1, b1, R.L, R.iL, r, rp, s2);
+            const double uo2 = pair_u_lds_s2(R, lds, r, rp, s2);
+            link_rr_s2(pn0, b0, pn1, b1, R.L, R.iL, r, rp, s2);
+            const double un2 = pair_u_lds_s2(R, lds, r, rp, s2);
+            uo += (ok1 ? uo1 : 0.) + (ok2 ? uo2 : 0.);
+            un += (ok1 ? un1 : 0.) + (ok2 ? un2 : 0.);
         }
         uo = wave_sum(uo);
         un = wave_sum(un);
@@ -965,13 +976,14 @@ __global__ __launch_bounds__(64) void k_adjoint_local(const Rep* __restrict__ re
     o[2] = g[2];
 }
 
-// Loads a replica's pair tables (and optionally the du table) into LDS.
-__device__ __forceinline__ void stage_tables(const Rep& R, double* lds, bool with_du) {
-    const int na = R.bt_a.ncoef * R.bt_a.ncomp;
-    for (int t = threadIdx.x; t < R.lds_doubles; t += blockDim.x) lds[t] = (t < na) ? R.bt_a.c[t] : R.bt_b.c[t - na];
-    if (with_du) {
-        double* ldu = lds + R.lds_doubles;
-        for (int t = threadIdx.x; t < R.lds_doubles_du; t += blockDim.x) ldu[t] = R.bt_du.c[t];
+// Loads a replica's link-action tables (du_only: the David du_kj_dbeta table
+// instead) into LDS.
+__device__ __forceinline__ void stage_tables(const Rep& R, double* lds, bool du_only) {
+    if (du_only) {
+        for (int t = threadIdx.x; t < R.lds_doubles_du; t += blockDim.x) lds[t] = R.bt_du.c[t];
+    } else {
+        const int na = R.bt_a.ncoef * R.bt_a.ncomp;
+        for (int t = threadIdx.x; t < R.lds_doubles; t += blockDim.x) lds[t] = (t < na) ? R.bt_a.c[t] : R.bt_b.c[t - na];
     }
     __syncthreads();
 }
@@ -1073,8 +1085,9 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_local_lds(const Rep* _
 __global__ __launch_bounds__(kAdjLdsBlock) void k_pair_whole_lds(const Rep* __restrict__ reps, int which, int slot) {
     extern __shared__ double lds[];
     const Rep& R = reps[blockIdx.x];
+    // David: the du_kj_dbeta table (potential in component 0); Bare: the pair tables
     stage_tables(R, lds, R.pair_type == 2);
-    const double* ldu = lds + R.lds_doubles;
+    const double* ldu = lds;
     __shared__ double red[kAdjLdsBlock / kWave];
     const int N = R.N, M = R.M, half = N / 2;
     for (int b = 0; b < M; ++b) {
