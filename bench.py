@@ -29,12 +29,15 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--replicas", type=int, default=512)
-    ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs (disables per-kernel timing)")
+    ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs of 16 rounds")
     ap.add_argument("--plain-attempts", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-timing", action="store_true", help="no per-kernel HIP events (for counter-collection runs)")
+    ap.add_argument("--no-timing", action="store_true", help="no kernel clocks or events (for counter-collection runs)")
+    ap.add_argument("--kernel-events", action="store_true",
+                    help="time every launch with dispatch events instead of device clocks (slows the run; "
+                         "gives the per-kernel breakdown)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device hipMemcpy of this size (a copy kernel of known bytes "
                          "for scaling rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -114,7 +117,12 @@ def main():
     for i in range(a.warmup):
         eng.sweep(1, flags)
         print(f"warmup sweep {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
-    eng.timing(not (a.graph or a.no_timing))
+    if a.no_timing:
+        eng.timing(0)
+    elif a.kernel_events:
+        eng.timing(abi.TIMING_EVENTS)
+    else:
+        eng.timing(abi.TIMING_CLOCK)
     eng.timing_reset()
     u0 = eng.bead_updates()
     t0 = time.perf_counter()
@@ -128,14 +136,21 @@ def main():
     names = ["propose", "pair", "rho", "decide", "adjoint", "estimators"]
     for i, n in enumerate(names):
         ms, cnt, by = eng.kernel_timing(i)
-        kernels[n] = {"ms": ms, "launches": cnt, "avg_us": 1e3 * ms / cnt if cnt else None}
+        if cnt:
+            kernels[n] = {"ms": ms, "launches": cnt, "avg_us": 1e3 * ms / cnt}
+    if "pair" in kernels:
+        _, cnt, links = eng.kernel_timing(abi.KERNEL_PAIR)
+        kernels["pair"]["link_evals_per_s"] = links / (kernels["pair"]["ms"] * 1e-3)
     rho_ms, rho_n, rho_bytes = eng.kernel_timing(abi.KERNEL_RHO)
     roofline = None
     if rho_n:
         ach = rho_bytes / (rho_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS,
                     "traffic": None, "kernel": "k_rho (incremental rho_k update + long-range action)",
-                    "bytes_per_launch": rho_bytes / rho_n, "avg_launch_us": 1e3 * rho_ms / rho_n}
+                    "bytes_per_launch": rho_bytes / rho_n, "avg_launch_us": 1e3 * rho_ms / rho_n,
+                    "duration_from": ("dispatch events" if a.kernel_events else
+                                      "device wall clock, first block start to last block end, every launch "
+                                      "of the timed region")}
     c0 = eng.counters(0)
     eng.close()
 

@@ -89,6 +89,10 @@ extern "C" {
 #define SPIMC_KERNEL_ESTIMATORS 5
 #define SPIMC_KERNEL_COUNT 6
 
+/* timing modes of spimc_timing_enable (may be or-ed) */
+#define SPIMC_TIMING_CLOCK 1  /* device wall-clock spans of the pair and rho_k kernels; no events */
+#define SPIMC_TIMING_EVENTS 2 /* start/stop events written by every kernel's dispatch */
+
 /*
  * One replica's system.  Field groups follow the reference XML elements:
  *   <System n_d n_bead beta L>                 (data_structures/path_class.h:25-69)
@@ -270,10 +274,18 @@ int spimc_bead_updates(spimc_ctx* ctx, uint64_t* out);
 
 /* ---- measurement ------------------------------------------------------ */
 
-/* Enables HIP-event timing of every kernel launch on the context stream. */
-int spimc_timing_enable(spimc_ctx* ctx, int enable);
+/* Timing of the move kernels (0 disables).  SPIMC_TIMING_CLOCK: each pair and
+ * rho_k launch records its first block start and last block end on the GPU's
+ * constant-rate wall clock and the round's decide kernel sums the spans, so the
+ * timed run is the same program as an untimed one (also under graph replay).
+ * SPIMC_TIMING_EVENTS: every launch carries dispatch start/stop events
+ * (hipExtLaunchKernel); these flush caches around each kernel and slow the run.
+ * No counterpart in the reference (measurement only). */
+int spimc_timing_enable(spimc_ctx* ctx, int mode);
 /* Total milliseconds, launches and algorithmic bytes (rho_k: 32 B per
- * (slice, k) update) of one kernel class since timing was enabled. */
+ * (slice, k) update; pair: link evaluations) of one kernel class since the last
+ * reset.  With SPIMC_TIMING_CLOCK the pair and rho_k classes report the device
+ * clock spans of every launch (bytes count only the work done). */
 int spimc_kernel_timing(spimc_ctx* ctx, int kernel, double* total_ms, uint64_t* launches, double* bytes);
 int spimc_timing_reset(spimc_ctx* ctx);
 

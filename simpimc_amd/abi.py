@@ -24,6 +24,7 @@ ENERGY_THERMAL, ENERGY_VIRIAL_COMPAT, ENERGY_VIRIAL_ADJOINT = 0, 1, 2
 OBS_ENERGY, OBS_SK, OBS_GR = 1, 2, 4
 SWEEP_GRAPH, SWEEP_ADJOINT, SWEEP_ENERGY, SWEEP_SK, SWEEP_GR = 1, 2, 4, 8, 16
 KERNEL_PROPOSE, KERNEL_PAIR, KERNEL_RHO, KERNEL_DECIDE, KERNEL_ADJOINT, KERNEL_ESTIMATORS = range(6)
+TIMING_CLOCK, TIMING_EVENTS = 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -300,8 +301,10 @@ class Engine:
         return v.value
 
     # ---- measurement -----------------------------------------------------------
-    def timing(self, enable: bool):
-        check(self.lib.spimc_timing_enable(self.h, int(enable)))
+    def timing(self, mode: int):
+        """0 off; TIMING_CLOCK (device clocks of the pair and rho_k kernels) and/or
+        TIMING_EVENTS (dispatch events on every launch)."""
+        check(self.lib.spimc_timing_enable(self.h, int(mode)))
 
     def kernel_timing(self, kernel: int):
         ms, n, b = C.c_double(), C.c_uint64(), C.c_double()

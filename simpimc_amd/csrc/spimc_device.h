@@ -70,6 +70,8 @@ struct Counters {
     unsigned int remaining;           // attempts left in the running stage
     unsigned int stage_i;             // attempts done in the running stage
     unsigned long long error;         // replay tape overrun or invalid draw
+    unsigned long long pair_links;    // link evaluations of the pair kernel (moves)
+    unsigned long long rho_slicek;    // (moved slice, k) updates of the rho_k kernel
 };
 
 struct Rep {
@@ -185,21 +187,25 @@ __device__ __forceinline__ double spl_eval1(const Spl& s, double t) {
 }
 
 // B-spline basis at fraction f and the interval of t
+// clamp to [lo, hi] (lo <= hi, no NaN inputs): the same values as the
+// comparisons of the reference's SetLimits, as two min/max instructions
+__device__ __forceinline__ double clampd(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
+
 struct BBasis {
     int i;
     double w0, w1, w2, w3;
 };
 __device__ __forceinline__ BBasis bt_basis(const BTab& T, double t) {
     const double u = (t - T.x0) * T.ih;
-    double fl = floor(u);
-    fl = (fl < 0.) ? 0. : (fl > (double)(T.knots - 2) ? (double)(T.knots - 2) : fl);
+    const double fl = clampd(floor(u), 0., (double)(T.knots - 2));
     const double f = u - fl, g = 1. - f, f2 = f * f, f3 = f2 * f;
     BBasis b;
     b.i = (int)fl;
-    b.w0 = g * g * g * (1. / 6.);
-    b.w1 = (3. * f3 - 6. * f2 + 4.) * (1. / 6.);
-    b.w2 = (-3. * f3 + 3. * f2 + 3. * f + 1.) * (1. / 6.);
-    b.w3 = f3 * (1. / 6.);
+    // the cubic B-spline weights times 6 (the tables hold coefficients / 6)
+    b.w0 = g * g * g;
+    b.w1 = 3. * f3 - 6. * f2 + 4.;
+    b.w2 = -3. * f3 + 3. * f2 + 3. * f + 1.;
+    b.w3 = f3;
     return b;
 }
 __device__ __forceinline__ double bt_apply(const double* c, const BBasis& b) {
@@ -208,9 +214,8 @@ __device__ __forceinline__ double bt_apply(const double* c, const BBasis& b) {
 // basis weights of the first derivative d/dt (includes 1/h)
 __device__ __forceinline__ BBasis bt_basis_d(const BTab& T, double t) {
     const double u = (t - T.x0) * T.ih;
-    double fl = floor(u);
-    fl = (fl < 0.) ? 0. : (fl > (double)(T.knots - 2) ? (double)(T.knots - 2) : fl);
-    const double f = u - fl, g = 1. - f, f2 = f * f, h = 0.5 * T.ih;
+    const double fl = clampd(floor(u), 0., (double)(T.knots - 2));
+    const double f = u - fl, g = 1. - f, f2 = f * f, h = 3. * T.ih;
     BBasis b;
     b.i = (int)fl;
     b.w0 = -g * g * h;
@@ -256,10 +261,11 @@ __device__ __forceinline__ void link_geom(const double* a0, const double* c0, co
 }
 
 // DrDrpDrrp magnitudes |r|, |r'| and the squared |s|^2 (the link action needs
-// only even powers of s).
+// only even powers of s).  Contraction is allowed here: the image choices are
+// the same rint() of the same products, the lengths differ from the
+// uncontracted form in the last bits only.
 __device__ __forceinline__ void link_rr_s2(const double* a0, const double* c0, const double* a1, const double* c1,
                                            double L, double iL, double& r, double& rp, double& s2) {
-#pragma clang fp contract(off)
     double rr = 0., pp = 0., ss = 0.;
     for (int d = 0; d < 3; ++d) {
         double rv = c0[d] - a0[d];
@@ -278,8 +284,8 @@ __device__ __forceinline__ void link_rr_s2(const double* a0, const double* c0, c
 }
 
 __device__ __forceinline__ void clamp2(double lo, double hi, double& a, double& b) {
-    a = (a > hi) ? hi : ((a < lo) ? lo : a);
-    b = (b > hi) ? hi : ((b < lo) ? lo : b);
+    a = clampd(a, lo, hi);
+    b = clampd(b, lo, hi);
 }
 
 // BarePairAction::CalcV (bare_pair_action_class.h:99-123)
@@ -494,8 +500,11 @@ __device__ __forceinline__ void pair_u_grad_lds(const Rep& R, const double* lds,
 // The same link action from |s|^2: sum_{k,j} c_kj(q) (z^2)^j (s^2)^(k-j) with the
 // powers formed directly (the reference forms s^2k (1/s^2)^j; equal in exact
 // arithmetic).
+// PT: 1 Bare, 2 David, 0 read from the replica; NORD: David expansion order
+// (0 = read from the replica)
+template <int PT = 0, int NORD = 0>
 __device__ __forceinline__ double pair_u_lds_s2(const Rep& R, const double* lds, double r0, double r1, double s2) {
-    if (R.pair_type == 1) return pair_u_lds(R, lds, r0, r1, 0.);
+    if (PT == 1 || (PT == 0 && R.pair_type == 1)) return pair_u_lds(R, lds, r0, r1, 0.);
     const BTab& T = R.bt_a;
     const double q = 0.5 * (r0 + r1), z = r0 - r1;
     clamp2(R.g_min, R.g_max, r0, r1);
@@ -506,13 +515,19 @@ __device__ __forceinline__ double pair_u_lds_s2(const Rep& R, const double* lds,
         double sp[4], zp[4];
         sp[0] = 1.;
         zp[0] = 1.;
-        for (int e = 1; e <= 3; ++e) {
+#pragma unroll
+        for (int e = 1; e <= (NORD ? NORD : 3); ++e) {
             sp[e] = sp[e - 1] * s2;
             zp[e] = zp[e - 1] * z2;
         }
-        for (int k = 1; k <= R.n_order; ++k)
+        const int n_order = NORD ? NORD : R.n_order;
+#pragma unroll
+        for (int k = 1; k <= (NORD ? NORD : 3); ++k) {
+            if (k > n_order) break;
+#pragma unroll
             for (int j = 0; j <= k; ++j)
                 u += bt_apply(lds + (size_t)(k * (k + 1) / 2 + j) * T.ncoef, bq) * zp[j] * sp[k - j];
+        }
     }
     return u;
 }

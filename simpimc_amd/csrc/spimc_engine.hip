@@ -1,5 +1,6 @@
 // spimc_engine.hip -- context management and C ABI (include/spimc.h).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -7,6 +8,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/spimc.h"
@@ -30,8 +33,8 @@ int fail(int code, const std::string& msg) {
             return fail(SPIMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kRhoKpt = 8;                       // k-vectors per thread in k_rho
-constexpr int kRhoChunk = kRhoBlock * kRhoKpt;   // k-vectors per k_rho workgroup
+constexpr int kRhoKpt = 8;        // k-vectors per thread in k_rho (default; SPIMC_RHO_KPT 2, 4, 8, 16)
+constexpr int kRhoKptMin = 2;     // smallest chunking: bounds the number of k chunks
 constexpr int kBuildKpt = 2;
 constexpr int kRoundsPerGraph = 16;
 constexpr int kAdjMaxIdx = 40;
@@ -168,6 +171,7 @@ BTabHost make_btab(const double* x, int n, const std::vector<const double*>& ys)
     t.ih = (n - 1) / (x[n - 1] - x[0]);
     t.c.assign((size_t)(n + 2) * t.ncomp, 0.);
     for (int k = 0; k < t.ncomp; ++k) bspline_coefs(x, ys[k], n, t.c.data() + (size_t)k * (n + 2));
+    for (double& v : t.c) v *= 1. / 6.;  // the device basis weights omit the 1/6
     return t;
 }
 
@@ -183,6 +187,8 @@ struct RepHost {
     std::vector<SplHost> kin, bis;
     SplHost kin_dtau;
     Rep dev{};
+    int const_owner = -1;  // replica whose arena holds this replica's constant tables
+    uint64_t const_hash = 0;
     char* arena = nullptr;
     unsigned char* accepted = nullptr;
     double* tape = nullptr;
@@ -191,6 +197,64 @@ struct RepHost {
     int n_chunks = 1;
 };
 
+// Replicas with identical constant tables (k-set, Ewald coefficients, pair and
+// free-particle splines) share one device copy, so a batch of clones of one
+// system keeps a single set of tables hot in L2 instead of one per replica.
+void hash_bytes(uint64_t& h, const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001B3ull;
+}
+template <class T>
+void hash_vec(uint64_t& h, const std::vector<T>& v) {
+    const size_t n = v.size();
+    hash_bytes(h, &n, sizeof(n));
+    if (n) hash_bytes(h, v.data(), n * sizeof(T));
+}
+void hash_spline(uint64_t& h, const SplHost& s) {
+    hash_vec(h, s.x);
+    hash_vec(h, s.c);
+    const int m[3] = {s.n, s.ncomp, s.uniform ? 1 : 0};
+    hash_bytes(h, m, sizeof(m));
+}
+void hash_btab(uint64_t& h, const BTabHost& t) {
+    hash_vec(h, t.c);
+    const int m[2] = {t.knots, t.ncomp};
+    hash_bytes(h, m, sizeof(m));
+    hash_bytes(h, &t.x0, sizeof(double));
+    hash_bytes(h, &t.ih, sizeof(double));
+}
+bool same_spline(const SplHost& a, const SplHost& b) {
+    return a.n == b.n && a.ncomp == b.ncomp && a.uniform == b.uniform && a.x == b.x && a.c == b.c;
+}
+bool same_btab(const BTabHost& a, const BTabHost& b) {
+    return a.knots == b.knots && a.ncomp == b.ncomp && a.x0 == b.x0 && a.ih == b.ih && a.c == b.c;
+}
+uint64_t constants_hash(const RepHost& h) {
+    uint64_t x = 0xCBF29CE484222325ull;
+    hash_vec(x, h.kidx);
+    hash_vec(x, h.kmag);
+    hash_vec(x, h.coef_s);
+    hash_vec(x, h.coef_b);
+    for (const SplHost* s : {&h.v_r, &h.v_long_r, &h.u, &h.du, &h.kin_dtau}) hash_spline(x, *s);
+    for (const auto& s : h.kin) hash_spline(x, s);
+    for (const auto& s : h.bis) hash_spline(x, s);
+    for (const BTabHost* t : {&h.bt_a, &h.bt_b, &h.bt_du}) hash_btab(x, *t);
+    return x;
+}
+bool same_constants(const RepHost& a, const RepHost& b) {
+    if (a.const_hash != b.const_hash || a.kidx != b.kidx || a.kmag != b.kmag || a.coef_s != b.coef_s ||
+        a.coef_b != b.coef_b || a.kin.size() != b.kin.size() || a.bis.size() != b.bis.size())
+        return false;
+    if (!same_spline(a.v_r, b.v_r) || !same_spline(a.v_long_r, b.v_long_r) || !same_spline(a.u, b.u) ||
+        !same_spline(a.du, b.du) || !same_spline(a.kin_dtau, b.kin_dtau))
+        return false;
+    for (size_t l = 0; l < a.kin.size(); ++l)
+        if (!same_spline(a.kin[l], b.kin[l])) return false;
+    for (size_t l = 0; l < a.bis.size(); ++l)
+        if (!same_spline(a.bis[l], b.bis[l])) return false;
+    return same_btab(a.bt_a, b.bt_a) && same_btab(a.bt_b, b.bt_b) && same_btab(a.bt_du, b.bt_du);
+}
+
 struct TimedLaunch {
     int kernel;
     hipEvent_t a, b;
@@ -198,14 +262,30 @@ struct TimedLaunch {
 
 }  // namespace
 
+// A contiguous range of replicas whose move rounds run on a stream of their
+// own, so that one group's VALU-bound pair kernel and another group's
+// HBM-bound rho_k kernel (and the one-wavefront propose/decide kernels) share
+// the GPU at the same time.  Group 0 runs on the context stream.
+struct Group {
+    hipStream_t stream = nullptr;
+    int r0 = 0, n = 0;
+    // work lists (absolute replica indices)
+    uint2 *d_pair_bis = nullptr, *d_rho_bis = nullptr, *d_pair_dis = nullptr, *d_rho_dis = nullptr;
+    int n_pair_bis = 0, n_rho_bis = 0, n_pair_dis = 0, n_rho_dis = 0;
+    // captured round graphs and whether each was captured with the device clocks on
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool graph_clock[2] = {false, false};
+    unsigned long long* d_kt = nullptr;  // device clock slots of this group's rounds
+    hipEvent_t done = nullptr;
+};
+
 struct spimc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<RepHost> reps;
     Rep* d_reps = nullptr;
-    // work lists
-    uint2 *d_pair_bis = nullptr, *d_rho_bis = nullptr, *d_pair_dis = nullptr, *d_rho_dis = nullptr;
-    int n_pair_bis = 0, n_rho_bis = 0, n_pair_dis = 0, n_rho_dis = 0;
+    std::vector<Group> groups;
+    hipEvent_t stage_ready = nullptr;
     uint2* d_items_tmp = nullptr;
     size_t items_tmp_cap = 0;
     unsigned int* d_counts = nullptr;
@@ -213,8 +293,6 @@ struct spimc_ctx {
     size_t tmp_cap = 0;
     bool any_long = false;
     int max_M = 0, max_N = 0, max_nk = 0;
-    // graphs
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
     // interface proposal
     bool prop_set = false, rho_updated = false;
     int prop_rep = 0, prop_p = 0, prop_first = 0, prop_count = 0;
@@ -230,11 +308,19 @@ struct spimc_ctx {
     bool fused = false;
     // LDS bytes/8 of the largest pair table when every replica's tables stage through LDS, else 0
     int lds_doubles_max = 0;
+    // the LDS pair kernel specialised for the replicas' pair type (mixed types: generic)
+    void (*pair_lds_fn)(const Rep*, int, unsigned long long*) = nullptr;
+    // k_rho variant: k-vectors per thread and non-temporal rho_k traffic
+    int rho_kpt = kRhoKpt;
+    bool rho_nt = true;  // measured: slightly faster, and keeps the pair tables in L2
+    void (*rho_fn)(const Rep*, const uint2*, int, int, unsigned long long*) = nullptr;
     // LDS doubles of the adjoint kernel (pair tables) and of the whole-path
     // estimator kernel (Bare: pair tables, David: du table); 0 = global tables
     int lds_adj_max = 0, lds_whole_max = 0;
-    // timing
-    bool timing = false;
+    // timing: per-launch dispatch events (every kernel) and device clocks (pair, rho_k)
+    bool timing = false, clock = false;
+    double clock_hz = 0.;
+    unsigned long long kt_sum[2] = {0, 0}, kt_launches[2] = {0, 0};
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> pool;
     double t_ms[SPIMC_KERNEL_COUNT] = {0};
@@ -258,22 +344,40 @@ hipEvent_t take_event(spimc_ctx* c) {
     return e;
 }
 
-// Launch helper: brackets the launch with events when timing is on.
-template <class F>
-int launch(spimc_ctx* c, int kernel, F&& f) {
-    hipEvent_t a = nullptr, b = nullptr;
+// Launch helper.  With timing on, the start and stop events are written by the
+// dispatch packet itself (hipExtLaunchKernel), so an event pair measures the
+// kernel's own execution, as a kernel trace does, not the gaps around it.
+template <class... P, class... A>
+int launch(spimc_ctx* c, int kernel, void (*k)(P...), dim3 grid, dim3 block, size_t shmem, hipStream_t st,
+           A&&... a) {
+    std::tuple<std::decay_t<P>...> args{static_cast<std::decay_t<P>>(a)...};
+    void* ptrs[sizeof...(P) > 0 ? sizeof...(P) : 1];
+    std::apply([&](auto&... x) {
+        int i = 0;
+        ((ptrs[i++] = static_cast<void*>(&x)), ...);
+    }, args);
+    hipEvent_t ea = nullptr, eb = nullptr;
     if (c->timing) {
-        a = take_event(c);
-        b = take_event(c);
-        if (a) (void)hipEventRecord(a, c->stream);
+        ea = take_event(c);
+        eb = take_event(c);
     }
-    f();
-    hipError_t e = hipGetLastError();
+    hipError_t e;
+    if (ea && eb)
+        e = hipExtLaunchKernel(reinterpret_cast<const void*>(k), grid, block, ptrs, shmem, st, ea, eb, 0);
+    else
+        e = hipLaunchKernel(reinterpret_cast<const void*>(k), grid, block, ptrs, shmem, st);
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return fail(SPIMC_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
-    if (c->timing && a && b) {
-        (void)hipEventRecord(b, c->stream);
-        c->pending.push_back({kernel, a, b});
-    }
+    if (ea && eb) c->pending.push_back({kernel, ea, eb});
+    return SPIMC_OK;
+}
+
+// Work done by the move kernels so far, summed over replicas on the device.
+int work_totals(spimc_ctx* c, unsigned long long* w) {
+    k_work_sum<<<1, 64, 0, c->stream>>>(c->d_reps, (int)c->reps.size(), c->d_work);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(w, c->d_work, WORK_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     return SPIMC_OK;
 }
 
@@ -289,9 +393,21 @@ int sync_and_resolve(spimc_ctx* c) {
         c->pool.push_back(t.b);
     }
     c->pending.clear();
-    if (c->timing && c->d_work) {
+    if (c->clock) {
+        for (int k = 0; k < KT_CLASSES; ++k) c->kt_sum[k] = c->kt_launches[k] = 0;
+        for (auto& G : c->groups) {
+            unsigned long long kt[KT_WORDS];
+            HIPCHK(hipMemcpy(kt, G.d_kt, sizeof(kt), hipMemcpyDeviceToHost));
+            for (int k = 0; k < KT_CLASSES; ++k) {
+                c->kt_sum[k] += kt[2 + 2 * k];
+                c->kt_launches[k] += kt[3 + 2 * k];
+            }
+        }
+    }
+    if ((c->timing || c->clock) && c->d_work) {
         unsigned long long w[WORK_COUNT];
-        HIPCHK(hipMemcpy(w, c->d_work, sizeof(w), hipMemcpyDeviceToHost));
+        int rc = work_totals(c, w);
+        if (rc) return rc;
         c->t_bytes[SPIMC_KERNEL_RHO] += 32.0 * (double)(w[WORK_RHO_SLICEK] - c->work_seen[WORK_RHO_SLICEK]);
         c->t_bytes[SPIMC_KERNEL_PAIR] += (double)(w[WORK_PAIR_LINKS] - c->work_seen[WORK_PAIR_LINKS]);
         for (int i = 0; i < WORK_COUNT; ++i) c->work_seen[i] = w[i];
@@ -335,7 +451,8 @@ int validate(const spimc_system_desc& s, uint32_t r) {
         if (s.n_k == 0 || !s.k_n || !s.pair_v_long_k) return bad(SPIMC_ERR_ARG, "k-vectors and v_long_k required");
         for (uint32_t k = 0; k < 3 * s.n_k; ++k)
             if (std::abs(s.k_n[k]) > kMaxIdx) return bad(SPIMC_ERR_UNSUPPORTED, "|n| of a k-vector must be <= 63");
-        if ((s.n_k + kRhoChunk - 1) / kRhoChunk > 255) return bad(SPIMC_ERR_UNSUPPORTED, "too many k-vectors");
+        if ((s.n_k + kRhoBlock * kRhoKptMin - 1) / (kRhoBlock * kRhoKptMin) > 255)
+            return bad(SPIMC_ERR_UNSUPPORTED, "too many k-vectors");
     }
     return SPIMC_OK;
 }
@@ -368,7 +485,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     const int N = s.n_part, M = s.n_bead;
     const bool lr = s.pair_type != SPIMC_PAIR_NONE && s.pair_use_long_range;
     const int nk = lr ? (int)s.n_k : 0;
-    const double tau = s.beta / (1. * M), L = s.L, vol = L * L * L, box = 2. * M_PI / L;
+    const double tau = s.beta / (1. * M), L = s.L, box = 2. * M_PI / L;
     // k-space
     h.k_n.assign(s.k_n ? s.k_n : nullptr, s.k_n ? s.k_n + 3 * (size_t)nk : nullptr);
     h.kidx.resize(nk);
@@ -431,8 +548,17 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     if (s.bisect_n_images > 0)
         for (int l = 0; l < (int)s.bisect_n_level; ++l)
             h.bis.push_back(make_free_spline(L, s.bisect_n_images, s.lambda, 0.5 * tau * (double)(1 << l), false));
-    h.n_chunks = std::max(1, (nk + kRhoChunk - 1) / kRhoChunk);
+    h.n_chunks = std::max(1, (nk + kRhoBlock * c->rho_kpt - 1) / (kRhoBlock * c->rho_kpt));
 
+    // constant tables: shared with an earlier replica that has the same ones
+    h.const_hash = constants_hash(h);
+    h.const_owner = (int)r;
+    for (uint32_t o = 0; o < r; ++o)
+        if (c->reps[o].const_owner == (int)o && same_constants(c->reps[o], h)) {
+            h.const_owner = (int)o;
+            break;
+        }
+    const bool own = h.const_owner == (int)r;
     // lay out the arena (sizing pass, allocation, carving pass)
     Rep& R = h.dev;
     for (int pass = 0; pass < 2; ++pass) {
@@ -441,20 +567,22 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         R.pos = cv.take<double>((size_t)3 * M * N);
         R.rho = cv.take<double2>((size_t)2 * M * std::max(nk, 1));
         R.sel = cv.take<unsigned char>(M);
-        R.kidx = cv.take<int>(std::max(nk, 1));
-        R.kmag = cv.take<double>(std::max(nk, 1));
-        R.coef_s = cv.take<double>(std::max(nk, 1));
-        R.coef_b = cv.take<double>(std::max(nk, 1));
-        if (h.v_r.n) R.v_r = carve_spline(cv, h.v_r);
-        if (h.v_long_r.n) R.v_long_r = carve_spline(cv, h.v_long_r);
-        if (h.u.n) R.u = carve_spline(cv, h.u);
-        if (h.du.n) R.du = carve_spline(cv, h.du);
-        for (size_t l = 0; l < h.kin.size(); ++l) R.kin[l] = carve_spline(cv, h.kin[l]);
-        if (h.kin_dtau.n) R.kin_dtau = carve_spline(cv, h.kin_dtau);
-        for (size_t l = 0; l < h.bis.size(); ++l) R.bis[l] = carve_spline(cv, h.bis[l]);
-        R.bt_a.c = cv.take<double>(std::max<size_t>(h.bt_a.c.size(), 1));
-        R.bt_b.c = cv.take<double>(std::max<size_t>(h.bt_b.c.size(), 1));
-        R.bt_du.c = cv.take<double>(std::max<size_t>(h.bt_du.c.size(), 1));
+        if (own) {
+            R.kidx = cv.take<int>(std::max(nk, 1));
+            R.kmag = cv.take<double>(std::max(nk, 1));
+            R.coef_s = cv.take<double>(std::max(nk, 1));
+            R.coef_b = cv.take<double>(std::max(nk, 1));
+            if (h.v_r.n) R.v_r = carve_spline(cv, h.v_r);
+            if (h.v_long_r.n) R.v_long_r = carve_spline(cv, h.v_long_r);
+            if (h.u.n) R.u = carve_spline(cv, h.u);
+            if (h.du.n) R.du = carve_spline(cv, h.du);
+            for (size_t l = 0; l < h.kin.size(); ++l) R.kin[l] = carve_spline(cv, h.kin[l]);
+            if (h.kin_dtau.n) R.kin_dtau = carve_spline(cv, h.kin_dtau);
+            for (size_t l = 0; l < h.bis.size(); ++l) R.bis[l] = carve_spline(cv, h.bis[l]);
+            R.bt_a.c = cv.take<double>(std::max<size_t>(h.bt_a.c.size(), 1));
+            R.bt_b.c = cv.take<double>(std::max<size_t>(h.bt_b.c.size(), 1));
+            R.bt_du.c = cv.take<double>(std::max<size_t>(h.bt_du.c.size(), 1));
+        }
         R.phase = cv.take<double2>((size_t)M * 6 * (mx + 1));
         R.rho_rows = cv.take<int2>(M);
         R.prop = cv.take<Prop>(1);
@@ -542,6 +670,30 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     R.accepted = nullptr;
     R.accepted_cap = 0;
     R.gr_cnt = nullptr;
+    if (!own) {
+        const Rep& O = c->reps[h.const_owner].dev;
+        R.kidx = O.kidx;
+        R.kmag = O.kmag;
+        R.coef_s = O.coef_s;
+        R.coef_b = O.coef_b;
+        R.v_r = O.v_r;
+        R.v_long_r = O.v_long_r;
+        R.u = O.u;
+        R.du = O.du;
+        for (int l = 0; l < kMaxKinSplines; ++l) R.kin[l] = O.kin[l];
+        R.kin_dtau = O.kin_dtau;
+        for (int l = 0; l < kMaxKinSplines; ++l) R.bis[l] = O.bis[l];
+        R.bt_a.c = O.bt_a.c;
+        R.bt_b.c = O.bt_b.c;
+        R.bt_du.c = O.bt_du.c;
+        // the host copies are only compared against owners; drop the large ones
+        h.u = SplHost();
+        h.du = SplHost();
+        h.bt_a.c.clear();
+        h.bt_b.c.clear();
+        h.bt_du.c.clear();
+        return SPIMC_OK;
+    }
     // uploads
     if (!h.bt_a.c.empty()) HIPCHK(hipMemcpy((void*)R.bt_a.c, h.bt_a.c.data(), h.bt_a.c.size() * sizeof(double), hipMemcpyHostToDevice));
     if (!h.bt_b.c.empty()) HIPCHK(hipMemcpy((void*)R.bt_b.c, h.bt_b.c.data(), h.bt_b.c.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -578,9 +730,9 @@ int upload_items(const std::vector<uint2>& v, uint2** d, int* n) {
     return SPIMC_OK;
 }
 
-int build_work_lists(spimc_ctx* c) {
+int build_work_lists(spimc_ctx* c, Group& G) {
     std::vector<uint2> pb, rb, pd, rd;
-    for (uint32_t r = 0; r < c->reps.size(); ++r) {
+    for (uint32_t r = (uint32_t)G.r0; r < (uint32_t)(G.r0 + G.n); ++r) {
         const Rep& R = c->reps[r].dev;
         const int nb = 1 << R.n_level;
         for (int j = 0; j < nb; ++j) pb.push_back(make_uint2(r, j));
@@ -593,9 +745,44 @@ int build_work_lists(spimc_ctx* c) {
         }
     }
     int rc;
-    if ((rc = upload_items(pb, &c->d_pair_bis, &c->n_pair_bis)) || (rc = upload_items(rb, &c->d_rho_bis, &c->n_rho_bis)) ||
-        (rc = upload_items(pd, &c->d_pair_dis, &c->n_pair_dis)) || (rc = upload_items(rd, &c->d_rho_dis, &c->n_rho_dis)))
+    if ((rc = upload_items(pb, &G.d_pair_bis, &G.n_pair_bis)) || (rc = upload_items(rb, &G.d_rho_bis, &G.n_rho_bis)) ||
+        (rc = upload_items(pd, &G.d_pair_dis, &G.n_pair_dis)) || (rc = upload_items(rd, &G.d_rho_dis, &G.n_rho_dis)))
         return rc;
+    return SPIMC_OK;
+}
+
+// Splits the replicas into n_groups contiguous ranges of about equal bead count.
+int build_groups(spimc_ctx* c, int n_groups) {
+    const int n_reps = (int)c->reps.size();
+    n_groups = std::max(1, std::min(n_groups, n_reps));
+    double total = 0.;
+    for (auto& h : c->reps) total += (double)h.dev.N * h.dev.M;
+    c->groups.resize(n_groups);
+    int r = 0;
+    double acc = 0.;
+    for (int g = 0; g < n_groups; ++g) {
+        Group& G = c->groups[g];
+        G.r0 = r;
+        const double target = total * (g + 1) / n_groups;
+        while (r < n_reps && (r == G.r0 || acc + 0.5 * c->reps[r].dev.N * c->reps[r].dev.M <= target) &&
+               n_reps - r > n_groups - g - 1) {
+            acc += (double)c->reps[r].dev.N * c->reps[r].dev.M;
+            ++r;
+        }
+        if (g == n_groups - 1) r = n_reps;
+        G.n = r - G.r0;
+        if (g == 0)
+            G.stream = c->stream;
+        else if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(SPIMC_ERR_HIP, "stream creation failed");
+        if (hipEventCreateWithFlags(&G.done, hipEventDisableTiming) != hipSuccess) return fail(SPIMC_ERR_HIP, "event");
+        const unsigned long long kt0[KT_WORDS] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+        HIPCHK(hipMalloc((void**)&G.d_kt, sizeof(kt0)));
+        HIPCHK(hipMemcpy(G.d_kt, kt0, sizeof(kt0), hipMemcpyHostToDevice));
+        int rc = build_work_lists(c, G);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventCreateWithFlags(&c->stage_ready, hipEventDisableTiming));
     return SPIMC_OK;
 }
 
@@ -616,44 +803,45 @@ int ensure_tmp(spimc_ctx* c, size_t n_doubles, size_t n_items) {
 }
 
 // One round of a move stage for every replica.
-int launch_round(spimc_ctx* c, int move) {
-    const int R = (int)c->reps.size();
-    unsigned long long* work = c->timing ? c->d_work : nullptr;
-    Rep* reps = c->d_reps;
-    hipStream_t st = c->stream;
+int launch_round(spimc_ctx* c, const Group& G, int move) {
+    const int R = G.n;
+    unsigned long long* kt = c->clock ? G.d_kt : nullptr;
+    Rep* reps = c->d_reps + G.r0;  // kernels indexed by block: replicas of this group
+    Rep* all = c->d_reps;          // kernels driven by work items: absolute indices
+    hipStream_t st = G.stream;
     int rc;
     if (move == 0) {
-        rc = launch(c, SPIMC_KERNEL_PROPOSE, [&] { k_propose_bisect<<<R, 64, 0, st>>>(reps, R); });
+        rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_bisect, R, 64, 0, st, reps, R);
     } else {
-        rc = launch(c, SPIMC_KERNEL_PROPOSE, [&] { k_propose_displace<<<R, 64, 0, st>>>(reps, R); });
+        rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_displace, R, 64, 0, st, reps, R);
     }
     if (rc) return rc;
-    const uint2* pi = move == 0 ? c->d_pair_bis : c->d_pair_dis;
-    const int np = move == 0 ? c->n_pair_bis : c->n_pair_dis;
-    const uint2* ri = move == 0 ? c->d_rho_bis : c->d_rho_dis;
-    const int nr = move == 0 ? c->n_rho_bis : c->n_rho_dis;
+    const uint2* pi = move == 0 ? G.d_pair_bis : G.d_pair_dis;
+    const int np = move == 0 ? G.n_pair_bis : G.n_pair_dis;
+    const uint2* ri = move == 0 ? G.d_rho_bis : G.d_rho_dis;
+    const int nr = move == 0 ? G.n_rho_bis : G.n_rho_dis;
     if (c->lds_doubles_max > 0 && np) {
         const size_t shmem = (size_t)c->lds_doubles_max * sizeof(double);
-        rc = launch(c, SPIMC_KERNEL_PAIR, [&] { k_pair_lds<<<R, kPairLdsBlock, shmem, st>>>(reps, R, work); });
+        rc = launch(c, SPIMC_KERNEL_PAIR, c->pair_lds_fn, R, kPairLdsBlock, shmem, st, reps, R, kt);
         if (rc) return rc;
-        if (nr) rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_rho<kRhoKpt><<<nr, kRhoBlock, 0, st>>>(reps, ri, nr, 0, work); });
+        if (nr) rc = launch(c, SPIMC_KERNEL_RHO, c->rho_fn, nr, kRhoBlock, 0, st, all, ri, nr, 0, kt);
         if (rc) return rc;
     } else if (c->fused && np && nr) {
         const int g = (np + 3) / 4 + nr;
-        rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_eval<kRhoKpt><<<g, kRhoBlock, 0, st>>>(reps, pi, np, ri, nr, work); });
+        rc = launch(c, SPIMC_KERNEL_RHO, k_eval<kRhoKpt>, g, kRhoBlock, 0, st, all, pi, np, ri, nr);
         if (rc) return rc;
     } else {
         if (np) {
             const int g = (np + 3) / 4;
-            rc = launch(c, SPIMC_KERNEL_PAIR, [&] { k_pair<<<g, kPairBlock, 0, st>>>(reps, pi, np, 0, work); });
+            rc = launch(c, SPIMC_KERNEL_PAIR, k_pair, g, kPairBlock, 0, st, all, pi, np, 0, kt);
             if (rc) return rc;
         }
         if (nr) {
-            rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_rho<kRhoKpt><<<nr, kRhoBlock, 0, st>>>(reps, ri, nr, 0, work); });
+            rc = launch(c, SPIMC_KERNEL_RHO, c->rho_fn, nr, kRhoBlock, 0, st, all, ri, nr, 0, kt);
             if (rc) return rc;
         }
     }
-    return launch(c, SPIMC_KERNEL_DECIDE, [&] { k_decide<<<R, 64, 0, st>>>(reps, R); });
+    return launch(c, SPIMC_KERNEL_DECIDE, k_decide, R, 64, 0, st, reps, R, kt);
 }
 
 int ensure_accepted(spimc_ctx* c, uint32_t need) {
@@ -677,19 +865,24 @@ int ensure_accepted(spimc_ctx* c, uint32_t need) {
     return SPIMC_OK;
 }
 
-int capture_round_graph(spimc_ctx* c, int move) {
-    if (c->graph[move]) return SPIMC_OK;
+int capture_round_graph(spimc_ctx* c, Group& G, int move) {
+    if (G.graph[move] && G.graph_clock[move] != c->clock) {
+        (void)hipGraphExecDestroy(G.graph[move]);
+        G.graph[move] = nullptr;
+    }
+    if (G.graph[move]) return SPIMC_OK;
     hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
     bool saved = c->timing;
     c->timing = false;
     int rc = SPIMC_OK;
-    for (int i = 0; i < kRoundsPerGraph && rc == SPIMC_OK; ++i) rc = launch_round(c, move);
+    for (int i = 0; i < kRoundsPerGraph && rc == SPIMC_OK; ++i) rc = launch_round(c, G, move);
     c->timing = saved;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    G.graph_clock[move] = c->clock;
+    hipError_t e = hipStreamEndCapture(G.stream, &g);
     if (rc) return rc;
     if (e != hipSuccess) return fail(SPIMC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    HIPCHK(hipGraphInstantiate(&c->graph[move], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphInstantiate(&G.graph[move], g, nullptr, nullptr, 0));
     (void)hipGraphDestroy(g);
     return SPIMC_OK;
 }
@@ -706,16 +899,35 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
     if (mx == 0) return SPIMC_OK;
     int rc = ensure_accepted(c, mx);
     if (rc) return rc;
+    const bool graph = use_graph && !c->timing;
+    if (graph)
+        for (auto& G : c->groups)
+            if ((rc = capture_round_graph(c, G, move))) return rc;
     HIPCHK(hipMemcpyAsync(c->d_counts, cnt.data(), R * sizeof(unsigned int), hipMemcpyHostToDevice, c->stream));
-    rc = launch(c, SPIMC_KERNEL_DECIDE, [&] { k_stage_begin<<<(R + 63) / 64, 64, 0, c->stream>>>(c->d_reps, R, c->d_counts); });
+    rc = launch(c, SPIMC_KERNEL_DECIDE, k_stage_begin, (R + 63) / 64, 64, 0, c->stream, c->d_reps, R, c->d_counts);
     if (rc) return rc;
-    if (use_graph && !c->timing) {
-        rc = capture_round_graph(c, move);
-        if (rc) return rc;
-        for (uint32_t t = 0; t < mx; t += kRoundsPerGraph) HIPCHK(hipGraphLaunch(c->graph[move], c->stream));
-    } else {
-        for (uint32_t t = 0; t < mx; ++t)
-            if ((rc = launch_round(c, move))) return rc;
+    const int n_groups = (int)c->groups.size();
+    std::vector<uint32_t> gmx(n_groups, 0);
+    for (int g = 0; g < n_groups; ++g)
+        for (int r = c->groups[g].r0; r < c->groups[g].r0 + c->groups[g].n; ++r) gmx[g] = std::max(gmx[g], cnt[r]);
+    if (n_groups > 1) {
+        HIPCHK(hipEventRecord(c->stage_ready, c->stream));
+        for (int g = 1; g < n_groups; ++g) HIPCHK(hipStreamWaitEvent(c->groups[g].stream, c->stage_ready, 0));
+    }
+    // rounds issued group by group in turn, so the streams advance together
+    const uint32_t step = graph ? kRoundsPerGraph : 1;
+    for (uint32_t t = 0; t < mx; t += step)
+        for (int g = 0; g < n_groups; ++g) {
+            Group& G = c->groups[g];
+            if (t >= gmx[g]) continue;
+            if (graph)
+                HIPCHK(hipGraphLaunch(G.graph[move], G.stream));
+            else if ((rc = launch_round(c, G, move)))
+                return rc;
+        }
+    for (int g = 1; g < n_groups; ++g) {
+        HIPCHK(hipEventRecord(c->groups[g].done, c->groups[g].stream));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->groups[g].done, 0));
     }
     if ((rc = sync_and_resolve(c))) return rc;
     c->adjoint_fresh = false;
@@ -749,50 +961,47 @@ int rho_rebuild(spimc_ctx* c, uint32_t r) {
     HIPCHK(hipMemsetAsync(R.sel, 0, R.M, c->stream));
     const int chunks = (R.nk + 256 * kBuildKpt - 1) / (256 * kBuildKpt);
     dim3 g(R.M, chunks);
-    return launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_rho_build<kBuildKpt><<<g, 256, 0, c->stream>>>(c->d_reps, (int)r); });
+    return launch(c, SPIMC_KERNEL_ESTIMATORS, k_rho_build<kBuildKpt>, g, 256, 0, c->stream, c->d_reps, (int)r);
 }
 
 // ---- estimator drivers (launch over all replicas or one) -------------------
 // `base` offsets the replica table so that gridDim.y counts replicas from it.
 int est_kin_dbeta(spimc_ctx* c, Rep* base, int nrep) {
     dim3 g(c->max_M, nrep);
-    return launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_kin_dbeta<<<g, 256, 0, c->stream>>>(base, kSlotKin); });
+    return launch(c, SPIMC_KERNEL_ESTIMATORS, k_kin_dbeta, g, 256, 0, c->stream, base, kSlotKin);
 }
 int est_pair_whole(spimc_ctx* c, Rep* base, int nrep, int which) {
     int rc;
     if (c->lds_whole_max > 0) {
         const size_t shmem = (size_t)c->lds_whole_max * sizeof(double);
-        rc = launch(c, SPIMC_KERNEL_ESTIMATORS,
-                    [&] { k_pair_whole_lds<<<nrep, kAdjLdsBlock, shmem, c->stream>>>(base, which, kSlotPair); });
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_pair_whole_lds, nrep, kAdjLdsBlock, shmem, c->stream, base, which, kSlotPair);
     } else {
         dim3 g(c->max_M, nrep);
-        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_pair_whole<<<g, 256, 0, c->stream>>>(base, which, kSlotPair); });
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_pair_whole, g, 256, 0, c->stream, base, which, kSlotPair);
     }
     if (rc || !c->any_long) return rc;
     dim3 g2((c->max_nk + 255) / 256, nrep);
-    return launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_long_whole<<<g2, 256, 0, c->stream>>>(base); });
+    return launch(c, SPIMC_KERNEL_ESTIMATORS, k_long_whole, g2, 256, 0, c->stream, base);
 }
 int est_finalize(spimc_ctx* c, Rep* base, int nrep, int what, int acc) {
-    return launch(c, SPIMC_KERNEL_ESTIMATORS,
-                  [&] { k_finalize<<<(nrep + 63) / 64, 64, 0, c->stream>>>(base, nrep, what, acc); });
+    return launch(c, SPIMC_KERNEL_ESTIMATORS, k_finalize, (nrep + 63) / 64, 64, 0, c->stream, base, nrep, what, acc);
 }
 int est_gradient(spimc_ctx* c, Rep* base, int nrep, uint32_t terms, int compat, int out_est) {
     dim3 g((c->max_N + 63) / 64, c->max_M, nrep);
-    if (compat) return launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_grad_compat<<<g, 64, 0, c->stream>>>(base, out_est); });
+    if (compat) return launch(c, SPIMC_KERNEL_ADJOINT, k_grad_compat, g, 64, 0, c->stream, base, out_est);
     int rc;
     if (c->lds_adj_max > 0) {
         const size_t shmem = (size_t)c->lds_adj_max * sizeof(double);
-        rc = launch(c, SPIMC_KERNEL_ADJOINT,
-                    [&] { k_adjoint_local_lds<<<nrep, kAdjLdsBlock, shmem, c->stream>>>(base, terms, out_est); });
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_local_lds, nrep, kAdjLdsBlock, shmem, c->stream, base, terms, out_est);
     } else {
-        rc = launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_adjoint_local<<<g, 64, 0, c->stream>>>(base, terms, out_est); });
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_local, g, 64, 0, c->stream, base, terms, out_est);
     }
     if (rc || !(terms & SPIMC_GRAD_PAIR_LONG) || !c->any_long) return rc;
     int mx = 0;
     for (auto& h : c->reps) mx = std::max(mx, h.dev.max_idx);
     if (mx > kAdjMaxIdx) return fail(SPIMC_ERR_UNSUPPORTED, "adjoint long range supports |n| <= 40");
     const size_t shmem = (size_t)64 * 3 * (mx + 1) * sizeof(double2) + kAdjTile * (sizeof(double2) + sizeof(int) + sizeof(double));
-    return launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_adjoint_long<<<g, 64, shmem, c->stream>>>(base, out_est); });
+    return launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long, g, 64, shmem, c->stream, base, out_est);
 }
 
 int energy_accumulate(spimc_ctx* c, Rep* base, int nrep) {
@@ -804,7 +1013,7 @@ int energy_accumulate(spimc_ctx* c, Rep* base, int nrep) {
         return SPIMC_OK;
     }
     const int W = (int)c->virial_window;
-    rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_kin_virial<<<nrep, 256, 0, c->stream>>>(base, W, kSlotVirial); });
+    rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_kin_virial, nrep, 256, 0, c->stream, base, W, kSlotVirial);
     if (rc) return rc;
     int from_sweep = 0;
     if (c->energy_estimator == SPIMC_ENERGY_VIRIAL_COMPAT) {
@@ -816,8 +1025,7 @@ int energy_accumulate(spimc_ctx* c, Rep* base, int nrep) {
     }
     if (rc) return rc;
     dim3 g(c->max_M, nrep);
-    rc = launch(c, SPIMC_KERNEL_ESTIMATORS,
-                [&] { k_centroid<<<g, 256, 0, c->stream>>>(base, kSlotCentroid, from_sweep); });
+    rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_centroid, g, 256, 0, c->stream, base, kSlotCentroid, from_sweep);
     if (rc) return rc;
     if ((rc = est_finalize(c, base, nrep, 2, 0)) || (rc = est_pair_whole(c, base, nrep, 0)) ||
         (rc = est_finalize(c, base, nrep, 1, 1)))
@@ -833,15 +1041,14 @@ int obs_accumulate_all(spimc_ctx* c, uint32_t which) {
     if ((which & SPIMC_OBS_SK) && c->any_long) {
         dim3 g((c->max_nk + 255) / 256, R);
         const double kc = c->sk_k_cut;
-        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_sk<<<g, 256, 0, c->stream>>>(c->d_reps, kc); });
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_sk, g, 256, 0, c->stream, c->d_reps, kc);
         if (rc) return rc;
     }
     if (which & SPIMC_OBS_GR) {
         dim3 g(c->max_M, R);
         const double rmin = c->gr_r_min;
         const int nr = (int)c->gr_n_r;
-        rc = launch(c, SPIMC_KERNEL_ESTIMATORS,
-                    [&] { k_gr<<<g, 256, 0, c->stream>>>(c->d_reps, rmin, 0., nr, c->d_gr_rmax); });
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_gr, g, 256, 0, c->stream, c->d_reps, rmin, 0., nr, c->d_gr_rmax);
         if (rc) return rc;
     }
     return SPIMC_OK;
@@ -897,6 +1104,20 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         return rc;
     };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(SPIMC_ERR_HIP, "hipGetDevice"));
+    if (const char* f = std::getenv("SPIMC_FUSED")) c->fused = std::atoi(f) != 0;
+    if (const char* f = std::getenv("SPIMC_RHO_KPT")) {
+        const int v = std::atoi(f);
+        if (v == 2 || v == 4 || v == 8 || v == 16) c->rho_kpt = v;
+    }
+    if (const char* f = std::getenv("SPIMC_RHO_NT")) c->rho_nt = std::atoi(f) != 0;
+    if (c->fused) c->rho_kpt = kRhoKpt;  // the fused kernel is built for the default chunking
+    {
+        using RhoFn = void (*)(const Rep*, const uint2*, int, int, unsigned long long*);
+        const RhoFn fns[2][4] = {{k_rho<2, false>, k_rho<4, false>, k_rho<8, false>, k_rho<16, false>},
+                                 {k_rho<2, true>, k_rho<4, true>, k_rho<8, true>, k_rho<16, true>}};
+        const int ki = c->rho_kpt == 2 ? 0 : c->rho_kpt == 4 ? 1 : c->rho_kpt == 8 ? 2 : 3;
+        c->rho_fn = fns[c->rho_nt ? 1 : 0][ki];
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(SPIMC_ERR_HIP, "stream creation failed"));
     c->reps.resize(n_replicas);
@@ -916,13 +1137,19 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         return bail(fail(SPIMC_ERR_NOMEM, "work counters"));
     if (hipMemset(c->d_work, 0, WORK_COUNT * sizeof(unsigned long long)) != hipSuccess)
         return bail(fail(SPIMC_ERR_HIP, "memset"));
+    {
+        int dev = 0, khz = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            return bail(fail(SPIMC_ERR_HIP, "wall clock rate"));
+        c->clock_hz = 1e3 * khz;
+    }
     if (hipMalloc((void**)&c->d_gr_rmax, n_replicas * sizeof(double)) != hipSuccess)
         return bail(fail(SPIMC_ERR_NOMEM, "g(r) table"));
     if (hipMalloc((void**)&c->d_error, sizeof(unsigned int)) != hipSuccess ||
         hipMemset(c->d_error, 0, sizeof(unsigned int)) != hipSuccess)
         return bail(fail(SPIMC_ERR_NOMEM, "error flag"));
     for (auto& h : c->reps) h.dev.ctx_error = c->d_error;
-    if (const char* f = std::getenv("SPIMC_FUSED")) c->fused = std::atoi(f) != 0;
     {
         bool all = true;
         for (auto& h : c->reps) {
@@ -931,8 +1158,18 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         }
         const char* f = std::getenv("SPIMC_PAIR_LDS");
         if (!all || (f && std::atoi(f) == 0)) c->lds_doubles_max = 0;
+        // one pair type across the replicas selects the specialised instantiation
+        int pt = -1, nord = -1;
+        for (auto& h : c->reps)
+            if (h.dev.pair_type != SPIMC_PAIR_NONE) {
+                pt = (pt < 0 || pt == h.dev.pair_type) ? h.dev.pair_type : 0;
+                nord = (nord < 0 || nord == h.dev.n_order) ? h.dev.n_order : 0;
+            }
+        c->pair_lds_fn = pt == SPIMC_PAIR_BARE    ? k_pair_lds<1, 0>
+                         : pt == SPIMC_PAIR_DAVID ? (nord == 2 ? k_pair_lds<2, 2> : k_pair_lds<2, 0>)
+                                                  : k_pair_lds<0, 0>;
         if (c->lds_doubles_max > 0 &&
-            hipFuncSetAttribute((const void*)k_pair_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+            hipFuncSetAttribute((const void*)c->pair_lds_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 c->lds_doubles_max * (int)sizeof(double)) != hipSuccess)
             c->lds_doubles_max = 0;
         c->lds_adj_max = c->lds_doubles_max;
@@ -959,7 +1196,14 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
                 c->lds_adj_max ? "k_adjoint_local_lds" : "k_adjoint_local", c->lds_whole_max ? "k_pair_whole_lds" : "k_pair_whole");
     int rc = upload_reps(c);
     if (rc) return bail(rc);
-    if ((rc = build_work_lists(c))) return bail(rc);
+    {
+        // replica groups on separate streams (SPIMC_STREAMS, default 1)
+        int n_groups = 1;
+        if (const char* f = std::getenv("SPIMC_STREAMS")) n_groups = std::max(1, std::atoi(f));
+        if ((rc = build_groups(c, n_groups))) return bail(rc);
+        if (std::getenv("SPIMC_VERBOSE"))
+            fprintf(stderr, "spimc: %d replica group(s) on separate streams\n", (int)c->groups.size());
+    }
     if ((rc = spimc_obs_configure(c, SPIMC_ENERGY_THERMAL, 0, 1e300, 0., 0., 100))) return bail(rc);
     *out = c;
     return SPIMC_OK;
@@ -968,17 +1212,24 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
 int spimc_ctx_destroy(spimc_ctx* c) {
     if (!c) return SPIMC_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto& g : c->graph)
-        if (g) (void)hipGraphExecDestroy(g);
+    for (auto& G : c->groups) {
+        if (G.stream) (void)hipStreamSynchronize(G.stream);
+        for (auto& g : G.graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (void* p : {(void*)G.d_pair_bis, (void*)G.d_rho_bis, (void*)G.d_pair_dis, (void*)G.d_rho_dis, (void*)G.d_kt})
+            if (p) (void)hipFree(p);
+        if (G.done) (void)hipEventDestroy(G.done);
+        if (G.stream && G.stream != c->stream) (void)hipStreamDestroy(G.stream);
+    }
+    if (c->stage_ready) (void)hipEventDestroy(c->stage_ready);
     for (auto& h : c->reps) {
         if (h.arena) (void)hipFree(h.arena);
         if (h.accepted) (void)hipFree(h.accepted);
         if (h.tape) (void)hipFree(h.tape);
         if (h.gr_cnt) (void)hipFree(h.gr_cnt);
     }
-    for (void* p : {(void*)c->d_reps, (void*)c->d_counts, (void*)c->d_work, (void*)c->d_gr_rmax, (void*)c->d_error, (void*)c->d_tmp,
-                    (void*)c->d_items_tmp, (void*)c->d_pair_bis, (void*)c->d_rho_bis, (void*)c->d_pair_dis,
-                    (void*)c->d_rho_dis})
+    for (void* p : {(void*)c->d_reps, (void*)c->d_counts, (void*)c->d_work, (void*)c->d_gr_rmax, (void*)c->d_error,
+                    (void*)c->d_tmp, (void*)c->d_items_tmp})
         if (p) (void)hipFree(p);
     for (auto& t : c->pending) {
         (void)hipEventDestroy(t.a);
@@ -1078,7 +1329,7 @@ static int finish_proposal(spimc_ctx* c, uint32_t r, int accept) {
     if (!c->prop_set || c->prop_rep != (int)r) return fail(SPIMC_ERR_STATE, "no proposal pending on this replica");
     if ((rc = write_iface_prop(c, r, c->prop_p, c->prop_first, 0))) return rc;
     const int upd = c->rho_updated ? 1 : 0;
-    rc = launch(c, SPIMC_KERNEL_DECIDE, [&] { k_commit<<<1, 256, 0, c->stream>>>(c->d_reps, (int)r, accept, upd); });
+    rc = launch(c, SPIMC_KERNEL_DECIDE, k_commit, 1, 256, 0, c->stream, c->d_reps, (int)r, accept, upd);
     if (rc) return rc;
     c->prop_set = false;
     c->rho_updated = false;
@@ -1112,9 +1363,7 @@ int spimc_get_action(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0
         if (span % skip) return fail(SPIMC_ERR_ARG, "window not a multiple of the level step");
         const int nl = (int)(span / skip);
         if ((rc = write_iface_prop(c, r, p, (int)b0, 0))) return rc;
-        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] {
-            k_kinetic_window<<<1, 64, 0, c->stream>>>(c->d_reps, (int)r, mode, p, (int)b0, nl, (int)level, d_out);
-        });
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_kinetic_window, 1, 64, 0, c->stream, c->d_reps, (int)r, mode, p, (int)b0, nl, (int)level, d_out);
     } else {
         // PairAction::GetAction: 0 above max_level (= 0), and for is_constant
         if (level > 0 || R.pair_type == SPIMC_PAIR_NONE || R.N == 1) return SPIMC_OK;
@@ -1125,7 +1374,7 @@ int spimc_get_action(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0
         if (!items.empty()) {
             HIPCHK(hipMemcpy(c->d_items_tmp, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
             const int g = (nl + 3) / 4;
-            rc = launch(c, SPIMC_KERNEL_PAIR, [&] { k_pair<<<g, kPairBlock, 0, c->stream>>>(c->d_reps, c->d_items_tmp, nl, 1, nullptr); });
+            rc = launch(c, SPIMC_KERNEL_PAIR, k_pair, g, kPairBlock, 0, c->stream, c->d_reps, c->d_items_tmp, nl, 1, nullptr);
             if (rc) return rc;
             HIPCHK(hipStreamSynchronize(c->stream));
         }
@@ -1141,24 +1390,20 @@ int spimc_get_action(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0
                 HIPCHK(hipMemcpy(c->d_items_tmp, ri.data(), ri.size() * sizeof(uint2), hipMemcpyHostToDevice));
                 const int nri = (int)ri.size();
                 const int gph = (6 * c->prop_count + 63) / 64;
-                rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_phase_proposal<<<gph, 64, 0, c->stream>>>(c->d_reps, (int)r); });
+                rc = launch(c, SPIMC_KERNEL_RHO, k_phase_proposal, gph, 64, 0, c->stream, c->d_reps, (int)r);
                 if (rc) return rc;
-                rc = launch(c, SPIMC_KERNEL_RHO, [&] { k_rho<kRhoKpt><<<nri, kRhoBlock, 0, c->stream>>>(c->d_reps, c->d_items_tmp, nri, 1, nullptr); });
+                rc = launch(c, SPIMC_KERNEL_RHO, c->rho_fn, nri, kRhoBlock, 0, c->stream, c->d_reps, c->d_items_tmp, nri, 1, nullptr);
                 if (rc) return rc;
                 c->rho_updated = true;
             }
             n_long = (int)((b1 - b0 + skip - 1) / skip);
             const int upd = (mine && c->rho_updated) ? 1 : 0;
             if (n_long > 0) {
-                rc = launch(c, SPIMC_KERNEL_RHO, [&] {
-                    k_long_window<<<n_long, 256, 0, c->stream>>>(c->d_reps, (int)r, mode, (int)b0, upd, c->d_tmp);
-                });
+                rc = launch(c, SPIMC_KERNEL_RHO, k_long_window, n_long, 256, 0, c->stream, c->d_reps, (int)r, mode, (int)b0, upd, c->d_tmp);
                 if (rc) return rc;
             }
         }
-        rc = launch(c, SPIMC_KERNEL_PAIR, [&] {
-            k_action_sum<<<1, 64, 0, c->stream>>>(c->d_reps, (int)r, mode, nl, n_long, c->d_tmp, d_out);
-        });
+        rc = launch(c, SPIMC_KERNEL_PAIR, k_action_sum, 1, 64, 0, c->stream, c->d_reps, (int)r, mode, nl, n_long, c->d_tmp, d_out);
     }
     if (rc) return rc;
     if ((rc = sync_and_resolve(c))) return rc;
@@ -1206,7 +1451,7 @@ int spimc_kinetic_virial_energy(spimc_ctx* c, uint32_t r, uint32_t window, doubl
     if (!out) return fail(SPIMC_ERR_ARG, "null output");
     const Rep& R = c->reps[r].dev;
     const int W = window ? (int)window : 0;
-    rc = launch(c, SPIMC_KERNEL_ESTIMATORS, [&] { k_kin_virial<<<1, 256, 0, c->stream>>>(c->d_reps + r, W, kSlotVirial); });
+    rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_kin_virial, 1, 256, 0, c->stream, c->d_reps + r, W, kSlotVirial);
     if (rc || (rc = sync_and_resolve(c))) return rc;
     HIPCHK(hipMemcpy(out, R.scratch + kSlotVirial * R.M, sizeof(double), hipMemcpyDeviceToHost));
     return SPIMC_OK;
@@ -1223,14 +1468,14 @@ int spimc_action_gradient(spimc_ctx* c, uint32_t r, uint32_t terms, int compat, 
     Rep* base = c->d_reps + r;
     dim3 g((R.N + 63) / 64, R.M, 1);
     if (compat) {
-        rc = launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_grad_compat<<<g, 64, 0, c->stream>>>(base, 0); });
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, k_grad_compat, g, 64, 0, c->stream, base, 0);
     } else {
-        rc = launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_adjoint_local<<<g, 64, 0, c->stream>>>(base, terms, 0); });
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_local, g, 64, 0, c->stream, base, terms, 0);
         if (!rc && (terms & SPIMC_GRAD_PAIR_LONG) && R.use_long_range) {
             if (R.max_idx > kAdjMaxIdx) return fail(SPIMC_ERR_UNSUPPORTED, "adjoint long range supports |n| <= 40");
             const size_t shmem = (size_t)64 * 3 * (R.max_idx + 1) * sizeof(double2) +
                                  kAdjTile * (sizeof(double2) + sizeof(int) + sizeof(double));
-            rc = launch(c, SPIMC_KERNEL_ADJOINT, [&] { k_adjoint_long<<<g, 64, shmem, c->stream>>>(base, 0); });
+            rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long, g, 64, shmem, c->stream, base, 0);
         }
     }
     if (rc || (rc = sync_and_resolve(c))) return rc;
@@ -1427,12 +1672,15 @@ int spimc_bead_updates(spimc_ctx* c, uint64_t* out) {
 
 // ---- measurement ---------------------------------------------------------------
 
-int spimc_timing_enable(spimc_ctx* c, int enable) {
+int spimc_timing_enable(spimc_ctx* c, int mode) {
     if (!c) return fail(SPIMC_ERR_ARG, "null context");
     int rc = sync_and_resolve(c);
     if (rc) return rc;
-    c->timing = enable != 0;
-    HIPCHK(hipMemcpy(c->work_seen, c->d_work, sizeof(c->work_seen), hipMemcpyDeviceToHost));
+    if (mode & ~(SPIMC_TIMING_CLOCK | SPIMC_TIMING_EVENTS)) return fail(SPIMC_ERR_ARG, "unknown timing mode");
+    c->timing = (mode & SPIMC_TIMING_EVENTS) != 0;
+    c->clock = (mode & SPIMC_TIMING_CLOCK) != 0;
+    rc = work_totals(c, c->work_seen);
+    if (rc) return rc;
     return SPIMC_OK;
 }
 
@@ -1441,8 +1689,14 @@ int spimc_kernel_timing(spimc_ctx* c, int kernel, double* total_ms, uint64_t* la
     if (kernel < 0 || kernel >= SPIMC_KERNEL_COUNT) return fail(SPIMC_ERR_ARG, "unknown kernel class");
     int rc = sync_and_resolve(c);
     if (rc) return rc;
-    if (total_ms) *total_ms = c->t_ms[kernel];
-    if (launches) *launches = c->t_n[kernel];
+    const int kc = kernel == SPIMC_KERNEL_RHO ? KT_RHO : kernel == SPIMC_KERNEL_PAIR ? KT_PAIR : -1;
+    if (c->clock && kc >= 0) {
+        if (total_ms) *total_ms = 1e3 * (double)c->kt_sum[kc] / c->clock_hz;
+        if (launches) *launches = c->kt_launches[kc];
+    } else {
+        if (total_ms) *total_ms = c->t_ms[kernel];
+        if (launches) *launches = c->t_n[kernel];
+    }
     if (bytes) *bytes = c->t_bytes[kernel];
     return SPIMC_OK;
 }
@@ -1456,6 +1710,9 @@ int spimc_timing_reset(spimc_ctx* c) {
         c->t_n[i] = 0;
         c->t_bytes[i] = 0.;
     }
+    const unsigned long long kt0[KT_WORDS] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+    for (auto& G : c->groups) HIPCHK(hipMemcpy(G.d_kt, kt0, sizeof(kt0), hipMemcpyHostToDevice));
+    for (int k = 0; k < KT_CLASSES; ++k) c->kt_sum[k] = c->kt_launches[k] = 0;
     return SPIMC_OK;
 }
 

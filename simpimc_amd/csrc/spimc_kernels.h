@@ -42,8 +42,35 @@ __device__ __forceinline__ int xcd_remap(int hw, int G) {
     return (xcd < rem) ? xcd * (q + 1) + local : rem * (q + 1) + (xcd - rem) * q + local;
 }
 
+// Loads through the global address space (replica arrays live in device
+// memory; a generic pointer would compile to flat loads, which also count
+// against the LDS wait counter).
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+
+// 16-byte global loads and stores of one rho_k value; NT marks them
+// non-temporal (streamed through, not kept in the caches)
+typedef double d2v __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ double2 ld2(const double2* p) {
+    const __attribute__((address_space(1))) d2v* q = (const __attribute__((address_space(1))) d2v*)p;
+    const d2v v = NT ? __builtin_nontemporal_load(q) : *q;
+    return make_double2(v.x, v.y);
+}
+template <bool NT>
+__device__ __forceinline__ void st2(double2* p, double2 v) {
+    __attribute__((address_space(1))) d2v* q = (__attribute__((address_space(1))) d2v*)p;
+    const d2v w = {v.x, v.y};
+    if (NT)
+        __builtin_nontemporal_store(w, q);
+    else
+        *q = w;
+}
+
 __device__ __forceinline__ double pos_at(const Rep& R, int d, int s, int p) {
-    return R.pos[((size_t)d * R.M + s) * R.N + p];
+    return gld(R.pos + ((size_t)d * R.M + s) * R.N + p);
 }
 
 // NEW-mode position of the proposal's particle at slice s
@@ -267,7 +294,7 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
                 pr->move = 0;
                 pr->p = p;
                 pr->b0 = b0;
-                pr->n_links = nb;
+                pr->n_links = R.pair_type ? nb : 0;  // no pair action: no links to evaluate
                 pr->first = (b0 + 1) % M;
                 pr->count = wraps ? 0 : nb - 1;
                 pr->lsr = lsr;
@@ -332,7 +359,7 @@ __global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__
             pr->move = 1;
             pr->p = p;
             pr->b0 = 0;
-            pr->n_links = R.M;
+            pr->n_links = R.pair_type ? R.M : 0;
             pr->first = 0;
             pr->count = R.M;
             pr->lsr = 0.;
@@ -385,7 +412,7 @@ __global__ __launch_bounds__(64) void k_phase_proposal(const Rep* __restrict__ r
 // OLD and NEW (pair_action_class.h:279-290), one wavefront per (replica, link).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void pair_block(const Rep* __restrict__ reps, const uint2* __restrict__ items, int n_items,
-                                           int interface_mode, unsigned long long* __restrict__ work, int blk) {
+                                           int interface_mode, int blk) {
     const int item = blk * (kPairBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
     if (item >= n_items) return;
@@ -422,20 +449,49 @@ __device__ __forceinline__ void pair_block(const Rep* __restrict__ reps, const u
     un = wave_sum(un);
     if (lane == 0) {
         R.pair_part[j] = make_double2(uo, un);
-        if (work) atomicAdd(&work[WORK_PAIR_LINKS], (unsigned long long)(2 * (N - 1)));
     }
+}
+
+// Device-side kernel clocks.  Block 0 of a timed launch (the first dispatched)
+// stores its start on the constant-rate wall clock; the round's decide kernel
+// closes the spans at its own start: pair = rho_k start - pair start, rho_k =
+// decide start - rho_k start, i.e. each launch from its dispatch to the next
+// one's, as a kernel trace of back-to-back launches reports it.  No atomics, so
+// timed and untimed runs execute the same memory traffic.
+// kt[c]: open start of class c (0 = none); kt[2 + 2c], kt[3 + 2c]: summed ticks
+// and launches of class c.
+enum { KT_RHO = 0, KT_PAIR = 1, KT_CLASSES = 2, KT_WORDS = 6 };
+__device__ __forceinline__ void kt_begin(unsigned long long* kt, int cls) {
+    if (kt && blockIdx.x == 0 && threadIdx.x == 0) kt[cls] = (unsigned long long)wall_clock64();
+}
+__device__ __forceinline__ void kt_close(unsigned long long* kt) {
+    const unsigned long long now = (unsigned long long)wall_clock64();
+    const unsigned long long ps = kt[KT_PAIR], rs = kt[KT_RHO];
+    if (ps) {
+        const unsigned long long pe = rs ? rs : now;
+        if (pe >= ps) {
+            kt[2 + 2 * KT_PAIR] += pe - ps;
+            kt[3 + 2 * KT_PAIR] += 1ull;
+        }
+    }
+    if (rs && now >= rs) {
+        kt[2 + 2 * KT_RHO] += now - rs;
+        kt[3 + 2 * KT_RHO] += 1ull;
+    }
+    kt[KT_PAIR] = 0ull;
+    kt[KT_RHO] = 0ull;
 }
 
 // K1 with the link-action tables staged in LDS: one 1024-thread workgroup per
 // replica loads its B-spline tables once, then each wavefront takes window links
 // j = w, w + 16, ...; lanes run over the partners q.
 constexpr int kPairLdsBlock = 1024;
-__global__ __launch_bounds__(kPairLdsBlock) void k_pair_lds(const Rep* __restrict__ reps, int n_reps,
-                                                            unsigned long long* __restrict__ work) {
+template <int PT, int NORD>
+__device__ __forceinline__ void pair_lds_block(const Rep* __restrict__ reps, int n_reps) {
     extern __shared__ double lds[];
     const int r = xcd_remap(blockIdx.x, gridDim.x);
     if (r >= n_reps) return;
-    const Rep& R = reps[r];
+    const Rep R = reps[r];  // by value: the fields used stay in registers across the loops
     const Prop pr = *R.prop;
     if (pr.status != 1) return;
     for (int t = threadIdx.x; t < R.lds_doubles; t += kPairLdsBlock) lds[t] = (t < R.bt_a.ncoef * R.bt_a.ncomp) ? R.bt_a.c[t] : R.bt_b.c[t - R.bt_a.ncoef * R.bt_a.ncomp];
@@ -466,13 +522,13 @@ __global__ __launch_bounds__(kPairLdsBlock) void k_pair_lds(const Rep* __restric
             }
             double r, rp, s2;
             link_rr_s2(po0, a0, po1, a1, R.L, R.iL, r, rp, s2);
-            const double uo1 = pair_u_lds_s2(R, lds, r, rp, s2);
+            const double uo1 = pair_u_lds_s2<PT, NORD>(R, lds, r, rp, s2);
             link_rr_s2(pn0, a0, pn1, a1, R.L, R.iL, r, rp, s2);
-            const double un1 = pair_u_lds_s2(R, lds, r, rp, s2);
+            const double un1 = pair_u_lds_s2<PT, NORD>(R, lds, r, rp, s2);
             link_rr_s2(po0, b0, po1, b1, R.L, R.iL, r, rp, s2);
-            const double uo2 = pair_u_lds_s2(R, lds, r, rp, s2);
+            const double uo2 = pair_u_lds_s2<PT, NORD>(R, lds, r, rp, s2);
             link_rr_s2(pn0, b0, pn1, b1, R.L, R.iL, r, rp, s2);
-            const double un2 = pair_u_lds_s2(R, lds, r, rp, s2);
+            const double un2 = pair_u_lds_s2<PT, NORD>(R, lds, r, rp, s2);
             uo += (ok1 ? uo1 : 0.) + (ok2 ? uo2 : 0.);
             un += (ok1 ? un1 : 0.) + (ok2 ? un2 : 0.);
         }
@@ -480,15 +536,24 @@ __global__ __launch_bounds__(kPairLdsBlock) void k_pair_lds(const Rep* __restric
         un = wave_sum(un);
         if (lane == 0) {
             R.pair_part[j] = make_double2(uo, un);
-            if (work) atomicAdd(&work[WORK_PAIR_LINKS], (unsigned long long)(2 * (N - 1)));
         }
     }
 }
 
+// PT: 1 all replicas Bare, 2 all David, 0 mixed (type read per replica);
+// NORD: the David expansion order when all replicas share it, else 0
+template <int PT, int NORD>
+__global__ __launch_bounds__(kPairLdsBlock) void k_pair_lds(const Rep* __restrict__ reps, int n_reps,
+                                                            unsigned long long* __restrict__ kt) {
+    kt_begin(kt, KT_PAIR);
+    pair_lds_block<PT, NORD>(reps, n_reps);
+}
+
 __global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ reps, const uint2* __restrict__ items,
                                                      int n_items, int interface_mode,
-                                                     unsigned long long* __restrict__ work) {
-    pair_block(reps, items, n_items, interface_mode, work, xcd_remap(blockIdx.x, gridDim.x));
+                                                     unsigned long long* __restrict__ kt) {
+    kt_begin(kt, KT_PAIR);
+    pair_block(reps, items, n_items, interface_mode, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -497,9 +562,9 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ rep
 // long-range action sums of CalcULong (bare_pair_action_class.h:153-172,
 // david_pair_action_class.h:105-123) for OLD and NEW.
 // ---------------------------------------------------------------------------
-template <int KPT>
+template <int KPT, bool NT = false>
 __device__ __forceinline__ void rho_block(const Rep* __restrict__ reps, const uint2* __restrict__ items, int n_items,
-                                          int interface_mode, unsigned long long* __restrict__ work, int blk) {
+                                          int interface_mode, int blk) {
     if (blk >= n_items) return;
     const uint2 it = items[blk];
     const Rep& R = reps[it.x];
@@ -526,9 +591,9 @@ __device__ __forceinline__ void rho_block(const Rep* __restrict__ reps, const ui
     for (int i = 0; i < KPT; ++i) {
         const int k = k0 + i * kRhoBlock;
         const bool ok = k < nk;
-        rv[i] = ok ? rc[k] : make_double2(0., 0.);
-        kp[i] = ok ? R.kidx[k] : (128 | (128 << 8) | (128 << 16));
-        cf[i] = ok ? R.coef_s[k] : 0.;
+        rv[i] = ok ? ld2<NT>(rc + k) : make_double2(0., 0.);
+        kp[i] = ok ? gld(R.kidx + k) : (128 | (128 << 8) | (128 << 16));
+        cf[i] = ok ? gld(R.coef_s + k) : 0.;
     }
     __syncthreads();
     double ao = 0., an = 0.;
@@ -540,7 +605,7 @@ __device__ __forceinline__ void rho_block(const Rep* __restrict__ reps, const ui
         double2 nv;
         nv.x = rv[i].x + (en.x - eo.x);
         nv.y = rv[i].y + (en.y - eo.y);
-        if (k < nk) rn[k] = nv;
+        if (k < nk) st2<NT>(rn + k, nv);
         ao += cf[i] * (rv[i].x * rv[i].x + rv[i].y * rv[i].y);
         an += cf[i] * (nv.x * nv.x + nv.y * nv.y);
     }
@@ -559,15 +624,15 @@ __device__ __forceinline__ void rho_block(const Rep* __restrict__ reps, const ui
             tn += red[1][i];
         }
         R.rho_part[m * R.n_chunks + chunk] = make_double2(to, tn);
-        if (work) atomicAdd(&work[WORK_RHO_SLICEK], (unsigned long long)min(KPT * kRhoBlock, nk - chunk * kRhoBlock * KPT));
     }
 }
 
-template <int KPT>
+template <int KPT, bool NT>
 __global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps, const uint2* __restrict__ items,
                                                    int n_items, int interface_mode,
-                                                   unsigned long long* __restrict__ work) {
-    rho_block<KPT>(reps, items, n_items, interface_mode, work, xcd_remap(blockIdx.x, gridDim.x));
+                                                   unsigned long long* __restrict__ kt) {
+    kt_begin(kt, KT_RHO);
+    rho_block<KPT, NT>(reps, items, n_items, interface_mode, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // K1 + K2 in one launch: Gp pair blocks spread evenly among Gr rho_k blocks, so
@@ -575,22 +640,23 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps,
 // every CU.  Logical block i is a pair block when floor((i+1) Gp / G) > floor(i Gp / G).
 template <int KPT>
 __global__ __launch_bounds__(kRhoBlock) void k_eval(const Rep* __restrict__ reps, const uint2* __restrict__ pitems,
-                                                    int n_pitems, const uint2* __restrict__ ritems, int n_ritems,
-                                                    unsigned long long* __restrict__ work) {
+                                                    int n_pitems, const uint2* __restrict__ ritems, int n_ritems) {
     const long long G = gridDim.x, Gp = (n_pitems + kPairBlock / kWave - 1) / (kPairBlock / kWave);
     const long long i = xcd_remap(blockIdx.x, (int)G);
     const long long a = i * Gp / G, b = (i + 1) * Gp / G;
     if (b > a)
-        pair_block(reps, pitems, n_pitems, 0, work, (int)a);
+        pair_block(reps, pitems, n_pitems, 0, (int)a);
     else
-        rho_block<KPT>(reps, ritems, n_ritems, 0, work, (int)(i - a));
+        rho_block<KPT>(reps, ritems, n_ritems, 0, (int)(i - a));
 }
 
 // ---------------------------------------------------------------------------
 // K5: level-0 Metropolis decision and commit (bisect_class.h:100-123,24-36,127-139;
 // displace_particle_class.h:68-86), one wavefront per replica.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int n_reps) {
+__global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int n_reps,
+                                               unsigned long long* __restrict__ kt) {
+    if (kt && blockIdx.x == 0 && threadIdx.x == 0) kt_close(kt);
     const int r = blockIdx.x;
     if (r >= n_reps) return;
     const Rep& R = reps[r];
@@ -631,6 +697,10 @@ __global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int
         }
         s_acc = acc;
         Counters* c = R.cnt;
+        if (pr.status == 1) {  // the round's pair and rho_k work, for the roofline figures
+            c->rho_slicek += (unsigned long long)pr.count * (unsigned long long)R.nk;
+            if (R.pair_type) c->pair_links += 2ull * (unsigned long long)pr.n_links * (unsigned long long)(R.N - 1);
+        }
         if (pr.move == 0) {
             c->bis_att += 1;
             c->bis_acc += acc;
@@ -657,6 +727,24 @@ __global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int
             for (int d = 0; d < 3; ++d) R.pos[((size_t)d * R.M + s) * R.N + pr.p] = R.prop_new[3 * m + d];
             if (R.use_long_range) R.sel[s] ^= 1;
         }
+    }
+}
+
+// Sums the per-replica work counters into work[WORK_PAIR_LINKS], work[WORK_RHO_SLICEK].
+__global__ __launch_bounds__(64) void k_work_sum(const Rep* __restrict__ reps, int n_reps,
+                                                 unsigned long long* __restrict__ work) {
+    unsigned long long a = 0, b = 0;
+    for (int r = threadIdx.x; r < n_reps; r += kWave) {
+        a += reps[r].cnt->pair_links;
+        b += reps[r].cnt->rho_slicek;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    if (threadIdx.x == 0) {
+        work[WORK_PAIR_LINKS] = a;
+        work[WORK_RHO_SLICEK] = b;
     }
 }
 

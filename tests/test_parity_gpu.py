@@ -194,7 +194,11 @@ def test_virial_adjoint_estimator(sysname):
     assert rel_err(ge, re) < 1e-6
 
 
-def test_batched_replicas_match_single_chains():
+@pytest.mark.parametrize("streams", ["1", "3"])
+def test_batched_replicas_match_single_chains(streams, monkeypatch):
+    """Replicas advanced together (on one stream, or split into groups on three
+    streams) replay each single chain's tape bit for bit."""
+    monkeypatch.setenv("SPIMC_STREAMS", streams)
     systems = [system.config_c1(n_part=n, n_bead=m, r_s=rs) for n, m, rs in [(16, 16, 1.5), (32, 32, 2.0), (8, 16, 3.0), (24, 8, 1.2)]]
     tapes, refs, finals = [], [], []
     for i, s in enumerate(systems):
@@ -217,9 +221,12 @@ def test_batched_replicas_match_single_chains():
             assert np.array_equal(e.download(r=i), finals[i])
 
 
-def test_uneven_counts_and_graph_sweep():
+@pytest.mark.parametrize("streams", ["1", "2"])
+def test_uneven_counts_and_graph_sweep(streams, monkeypatch):
     systems = system.config_c4(seed=3, n_replicas=6)
-    with abi.Engine(systems, seed=5) as e, abi.Engine(systems, seed=5) as g:
+    with abi.Engine(systems, seed=5) as e:
+        monkeypatch.setenv("SPIMC_STREAMS", streams)
+        g = abi.Engine(systems, seed=5)
         for i, s in enumerate(systems):
             pos = system.random_path(s, seed=i, spread=0.05)
             e.upload(pos, r=i)
@@ -231,6 +238,8 @@ def test_uneven_counts_and_graph_sweep():
             c = e.counters(r=i)
             assert c[0] == 2 * ((s.n_part * s.n_bead) >> s.bisect_n_level)
             assert c[2] == 2 * s.n_part
+            assert np.array_equal(c, g.counters(r=i))
+        g.close()
 
 
 def test_c2_full_size_properties():
