@@ -28,7 +28,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--replicas", type=int, default=1024)
     ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs of 16 rounds")
     ap.add_argument("--plain-attempts", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

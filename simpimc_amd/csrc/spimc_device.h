@@ -95,6 +95,9 @@ struct Rep {
     // k-space
     const int* kidx;       // packed (n0+128) | (n1+128) << 8 | (n2+128) << 16
     const double* kmag;    // |k|
+    const int4* k_runs;    // [n_runs] runs of consecutive n2 at fixed (n0, n1): {n0, n1, n2 first, +-length}
+    const int* k_order;    // [nk] k index of the i-th k-vector in run order
+    int n_runs, pad_runs;
     const double* coef_s;  // per-slice action coefficient per k
     const double* coef_b;  // per-slice beta-derivative / potential coefficient per k
     // pair tables
@@ -437,10 +440,12 @@ __device__ __forceinline__ double pair_v_lds(const Rep& R, const double* lds, co
 }
 
 // dU/dr, dU/dr', dU/ds of the level-0 link action from LDS tables.
+// PT, NORD as for pair_u_lds_s2
+template <int PT = 0, int NORD = 0>
 __device__ __forceinline__ void pair_u_grad_lds(const Rep& R, const double* lds, double r0, double r1, double s,
                                                 double& ur, double& urp, double& us) {
     ur = urp = us = 0.;
-    if (R.pair_type == 1) {
+    if (PT == 1 || (PT == 0 && R.pair_type == 1)) {
         double a = r0, b = r1;
         const bool ina = (a >= R.r_v_min && a <= R.r_v_max), inb = (b >= R.r_v_min && b <= R.r_v_max);
         clamp2(R.r_v_min, R.r_v_max, a, b);
@@ -474,7 +479,11 @@ __device__ __forceinline__ void pair_u_grad_lds(const Rep& R, const double* lds,
     urp = inb ? 0.5 * bt_apply(lds, bt_basis_d(T, b)) : 0.;
     if (s > 0.0 && q < R.g_max) {
         const BBasis bq = bt_basis(T, q), bqd = bt_basis_d(T, q);
-        for (int k = 1; k <= R.n_order; ++k)
+        const int n_order = NORD ? NORD : R.n_order;
+#pragma unroll
+        for (int k = 1; k <= (NORD ? NORD : 3); ++k) {
+            if (k > n_order) break;
+#pragma unroll
             for (int j = 0; j <= k; ++j) {
                 const double* cc = lds + (size_t)(k * (k + 1) / 2 + j) * T.ncoef;
                 const double c = bt_apply(cc, bq), cd = bt_apply(cc, bqd);
@@ -494,6 +503,7 @@ __device__ __forceinline__ void pair_u_grad_lds(const Rep& R, const double* lds,
                 urp += dqterm - dz;
                 us += c * zp * (2. * e) * sm;
             }
+        }
     }
 }
 

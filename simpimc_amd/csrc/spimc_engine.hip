@@ -3,6 +3,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -33,8 +34,9 @@ int fail(int code, const std::string& msg) {
             return fail(SPIMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kRhoKpt = 8;        // k-vectors per thread in k_rho (default; SPIMC_RHO_KPT 2, 4, 8, 16)
+constexpr int kRhoKpt = 4;        // k-vectors per thread in k_rho (default; SPIMC_RHO_KPT 2, 4, 8, 16)
 constexpr int kRhoKptMin = 2;     // smallest chunking: bounds the number of k chunks
+constexpr int kNumCUs = 256;      // MI355X compute units (for workgroups-per-replica choices)
 constexpr int kBuildKpt = 2;
 constexpr int kRoundsPerGraph = 16;
 constexpr int kAdjMaxIdx = 40;
@@ -182,6 +184,8 @@ struct RepHost {
     BTabHost bt_a, bt_b, bt_du;
     std::vector<int32_t> k_n;
     std::vector<int> kidx;
+    std::vector<int4> k_runs;  // runs of consecutive n2 at fixed (n0, n1), ordered by (n0, n1)
+    std::vector<int> k_order;  // k index of each k-vector in run order
     std::vector<double> kmag, coef_s, coef_b;
     SplHost v_r, v_long_r, u, du;
     std::vector<SplHost> kin, bis;
@@ -269,6 +273,7 @@ struct TimedLaunch {
 struct Group {
     hipStream_t stream = nullptr;
     int r0 = 0, n = 0;
+    int max_bis_links = 0, max_M = 0;  // largest bisection window and slice count in the group
     // work lists (absolute replica indices)
     uint2 *d_pair_bis = nullptr, *d_rho_bis = nullptr, *d_pair_dis = nullptr, *d_rho_dis = nullptr;
     int n_pair_bis = 0, n_rho_bis = 0, n_pair_dis = 0, n_rho_dis = 0;
@@ -309,7 +314,10 @@ struct spimc_ctx {
     // LDS bytes/8 of the largest pair table when every replica's tables stage through LDS, else 0
     int lds_doubles_max = 0;
     // the LDS pair kernel specialised for the replicas' pair type (mixed types: generic)
-    void (*pair_lds_fn)(const Rep*, int, unsigned long long*) = nullptr;
+    void (*pair_lds_fn)(const Rep*, int, int, unsigned long long*) = nullptr;
+    int pair_block = kPairLdsBlock;
+    void (*adj_lds_fn)(const Rep*, unsigned int, int) = nullptr;
+    int groups_wanted = 1;
     // k_rho variant: k-vectors per thread and non-temporal rho_k traffic
     int rho_kpt = kRhoKpt;
     bool rho_nt = true;  // measured: slightly faster, and keeps the pair tables in L2
@@ -317,6 +325,8 @@ struct spimc_ctx {
     // LDS doubles of the adjoint kernel (pair tables) and of the whole-path
     // estimator kernel (Bare: pair tables, David: du table); 0 = global tables
     int lds_adj_max = 0, lds_whole_max = 0;
+    // dynamic LDS bytes of the run-ordered long-range adjoint (0: the per-particle table kernel)
+    size_t adj_runs_lds = 0;
     // timing: per-launch dispatch events (every kernel) and device clocks (pair, rho_k)
     bool timing = false, clock = false;
     double clock_hz = 0.;
@@ -504,6 +514,46 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         h.coef_s[k] = (s.pair_type == SPIMC_PAIR_DAVID) ? s.pair_v_long_k[k] * tau : s.pair_v_long_k[k];
         h.coef_b[k] = s.pair_v_long_k[k];
     }
+    // runs of the long-range adjoint: group by (n0, n1), split each group's sorted
+    // n2 into maximal consecutive sequences, and walk each from the n2 nearest 0
+    {
+        std::vector<int> idx(nk);
+        for (int k = 0; k < nk; ++k) idx[k] = k;
+        auto key = [&](int k) { return std::array<int, 3>{h.k_n[3 * k], h.k_n[3 * k + 1], h.k_n[3 * k + 2]}; };
+        std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
+        h.k_runs.clear();
+        h.k_order.clear();
+        auto emit = [&](int n0, int n1, const std::vector<int>& ks, const std::vector<int>& n2s, int lo, int hi) {
+            // ks[lo..hi] have consecutive n2s; split at 0 into an ascending and a descending run
+            int z = lo;
+            while (z <= hi && n2s[z] < 0) ++z;
+            if (z <= hi) {  // ascending from n2s[z] (>= 0)
+                h.k_runs.push_back(make_int4(n0, n1, n2s[z], hi - z + 1));
+                for (int i = z; i <= hi; ++i) h.k_order.push_back(ks[i]);
+            }
+            if (z > lo) {  // descending from n2s[z-1] (< 0)
+                h.k_runs.push_back(make_int4(n0, n1, n2s[z - 1], -(z - lo)));
+                for (int i = z - 1; i >= lo; --i) h.k_order.push_back(ks[i]);
+            }
+        };
+        for (int a = 0; a < nk;) {
+            const int n0 = h.k_n[3 * idx[a]], n1 = h.k_n[3 * idx[a] + 1];
+            std::vector<int> ks, n2s;
+            int e = a;
+            while (e < nk && h.k_n[3 * idx[e]] == n0 && h.k_n[3 * idx[e] + 1] == n1) {
+                ks.push_back(idx[e]);
+                n2s.push_back(h.k_n[3 * idx[e] + 2]);
+                ++e;
+            }
+            for (size_t lo = 0; lo < ks.size();) {
+                size_t hi = lo;
+                while (hi + 1 < ks.size() && n2s[hi + 1] == n2s[hi] + 1) ++hi;
+                emit(n0, n1, ks, n2s, (int)lo, (int)hi);
+                lo = hi + 1;
+            }
+            a = e;
+        }
+    }
     // pair tables
     int n_val = 0;
     if (s.pair_type == SPIMC_PAIR_BARE) {
@@ -572,6 +622,8 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
             R.kmag = cv.take<double>(std::max(nk, 1));
             R.coef_s = cv.take<double>(std::max(nk, 1));
             R.coef_b = cv.take<double>(std::max(nk, 1));
+            R.k_runs = cv.take<int4>(std::max<size_t>(h.k_runs.size(), 1));
+            R.k_order = cv.take<int>(std::max(nk, 1));
             if (h.v_r.n) R.v_r = carve_spline(cv, h.v_r);
             if (h.v_long_r.n) R.v_long_r = carve_spline(cv, h.v_long_r);
             if (h.u.n) R.u = carve_spline(cv, h.u);
@@ -619,6 +671,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     R.bis_images = s.bisect_n_images;
     R.n_links_max = M;
     R.n_chunks = h.n_chunks;
+    R.n_runs = (int)h.k_runs.size();
     R.index = r;
     R.L = L;
     R.iL = 1. / L;
@@ -676,6 +729,8 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         R.kmag = O.kmag;
         R.coef_s = O.coef_s;
         R.coef_b = O.coef_b;
+        R.k_runs = O.k_runs;
+        R.k_order = O.k_order;
         R.v_r = O.v_r;
         R.v_long_r = O.v_long_r;
         R.u = O.u;
@@ -703,6 +758,8 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         HIPCHK(hipMemcpy((void*)R.kmag, h.kmag.data(), nk * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy((void*)R.coef_s, h.coef_s.data(), nk * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy((void*)R.coef_b, h.coef_b.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void*)R.k_runs, h.k_runs.data(), h.k_runs.size() * sizeof(int4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void*)R.k_order, h.k_order.data(), nk * sizeof(int), hipMemcpyHostToDevice));
     }
     int rc;
     if ((rc = upload_spline(R.v_r, h.v_r)) || (rc = upload_spline(R.v_long_r, h.v_long_r)) ||
@@ -771,6 +828,10 @@ int build_groups(spimc_ctx* c, int n_groups) {
         }
         if (g == n_groups - 1) r = n_reps;
         G.n = r - G.r0;
+        for (int q = G.r0; q < r; ++q) {
+            G.max_bis_links = std::max(G.max_bis_links, 1 << c->reps[q].dev.n_level);
+            G.max_M = std::max(G.max_M, c->reps[q].dev.M);
+        }
         if (g == 0)
             G.stream = c->stream;
         else if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess)
@@ -822,7 +883,11 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
     const int nr = move == 0 ? G.n_rho_bis : G.n_rho_dis;
     if (c->lds_doubles_max > 0 && np) {
         const size_t shmem = (size_t)c->lds_doubles_max * sizeof(double);
-        rc = launch(c, SPIMC_KERNEL_PAIR, c->pair_lds_fn, R, kPairLdsBlock, shmem, st, reps, R, kt);
+        // workgroups per replica: one when the replicas fill the GPU, more (up to one
+        // per BS/64 links) when they are few
+        const int waves = c->pair_block / 64, max_links = move == 0 ? G.max_bis_links : G.max_M;
+        const int bpr = std::max(1, std::min((max_links + waves - 1) / waves, (2 * kNumCUs + R - 1) / R));
+        rc = launch(c, SPIMC_KERNEL_PAIR, c->pair_lds_fn, R * bpr, c->pair_block, shmem, st, reps, R, bpr, kt);
         if (rc) return rc;
         if (nr) rc = launch(c, SPIMC_KERNEL_RHO, c->rho_fn, nr, kRhoBlock, 0, st, all, ri, nr, 0, kt);
         if (rc) return rc;
@@ -992,11 +1057,19 @@ int est_gradient(spimc_ctx* c, Rep* base, int nrep, uint32_t terms, int compat, 
     int rc;
     if (c->lds_adj_max > 0) {
         const size_t shmem = (size_t)c->lds_adj_max * sizeof(double);
-        rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_local_lds, nrep, kAdjLdsBlock, shmem, c->stream, base, terms, out_est);
+        // workgroups per replica: enough to fill the GPU when the replicas are few
+        const int bpr = std::max(1, std::min(c->max_M, (2 * kNumCUs + nrep - 1) / nrep));
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, c->adj_lds_fn, dim3(nrep, bpr), kAdjLdsBlock, shmem, c->stream, base, terms,
+                    out_est);
     } else {
         rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_local, g, 64, 0, c->stream, base, terms, out_est);
     }
     if (rc || !(terms & SPIMC_GRAD_PAIR_LONG) || !c->any_long) return rc;
+    if (c->adj_runs_lds > 0) {
+        dim3 gr(c->max_M, nrep);
+        return launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long_runs, gr, kAdjRunBlock, c->adj_runs_lds, c->stream, base,
+                      out_est);
+    }
     int mx = 0;
     for (auto& h : c->reps) mx = std::max(mx, h.dev.max_idx);
     if (mx > kAdjMaxIdx) return fail(SPIMC_ERR_UNSUPPORTED, "adjoint long range supports |n| <= 40");
@@ -1111,6 +1184,10 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
     }
     if (const char* f = std::getenv("SPIMC_RHO_NT")) c->rho_nt = std::atoi(f) != 0;
     if (c->fused) c->rho_kpt = kRhoKpt;  // the fused kernel is built for the default chunking
+    // replica groups on separate streams: two when there are enough replicas to
+    // fill the GPU twice over (SPIMC_STREAMS overrides)
+    c->groups_wanted = n_replicas >= 64 ? 2 : 1;
+    if (const char* f = std::getenv("SPIMC_STREAMS")) c->groups_wanted = std::max(1, std::atoi(f));
     {
         using RhoFn = void (*)(const Rep*, const uint2*, int, int, unsigned long long*);
         const RhoFn fns[2][4] = {{k_rho<2, false>, k_rho<4, false>, k_rho<8, false>, k_rho<16, false>},
@@ -1165,16 +1242,32 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
                 pt = (pt < 0 || pt == h.dev.pair_type) ? h.dev.pair_type : 0;
                 nord = (nord < 0 || nord == h.dev.n_order) ? h.dev.n_order : 0;
             }
-        c->pair_lds_fn = pt == SPIMC_PAIR_BARE    ? k_pair_lds<1, 0>
-                         : pt == SPIMC_PAIR_DAVID ? (nord == 2 ? k_pair_lds<2, 2> : k_pair_lds<2, 0>)
-                                                  : k_pair_lds<0, 0>;
+        // 512-thread pair workgroups leave each CU room for the other stream's rho_k
+        // workgroups (measured best with two replica groups)
+        int bs = c->groups_wanted > 1 ? 512 : kPairLdsBlock;
+        if (const char* fb = std::getenv("SPIMC_PAIR_BLOCK")) {
+            const int v = std::atoi(fb);
+            if (v == 256 || v == 512 || v == 768 || v == 1024) bs = v;
+        }
+        c->pair_block = bs;
+        using PairFn = void (*)(const Rep*, int, int, unsigned long long*);
+        const PairFn fns[4][4] = {
+            {k_pair_lds<1, 0, 256>, k_pair_lds<2, 2, 256>, k_pair_lds<2, 0, 256>, k_pair_lds<0, 0, 256>},
+            {k_pair_lds<1, 0, 512>, k_pair_lds<2, 2, 512>, k_pair_lds<2, 0, 512>, k_pair_lds<0, 0, 512>},
+            {k_pair_lds<1, 0, 768>, k_pair_lds<2, 2, 768>, k_pair_lds<2, 0, 768>, k_pair_lds<0, 0, 768>},
+            {k_pair_lds<1, 0, 1024>, k_pair_lds<2, 2, 1024>, k_pair_lds<2, 0, 1024>, k_pair_lds<0, 0, 1024>}};
+        const int bi = bs == 256 ? 0 : bs == 512 ? 1 : bs == 768 ? 2 : 3;
+        const int vi = pt == SPIMC_PAIR_BARE ? 0 : pt == SPIMC_PAIR_DAVID ? (nord == 2 ? 1 : 2) : 3;
+        c->pair_lds_fn = fns[bi][vi];
         if (c->lds_doubles_max > 0 &&
             hipFuncSetAttribute((const void*)c->pair_lds_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 c->lds_doubles_max * (int)sizeof(double)) != hipSuccess)
             c->lds_doubles_max = 0;
         c->lds_adj_max = c->lds_doubles_max;
+        c->adj_lds_fn = vi == 0 ? k_adjoint_local_lds<1, 0> : vi == 1 ? k_adjoint_local_lds<2, 2>
+                                                        : vi == 2 ? k_adjoint_local_lds<2, 0> : k_adjoint_local_lds<0, 0>;
         if (c->lds_adj_max > 0 &&
-            hipFuncSetAttribute((const void*)k_adjoint_local_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+            hipFuncSetAttribute((const void*)c->adj_lds_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 c->lds_adj_max * (int)sizeof(double)) != hipSuccess)
             c->lds_adj_max = 0;
         bool whole = c->lds_doubles_max > 0;
@@ -1184,6 +1277,15 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
             c->lds_whole_max = std::max(c->lds_whole_max, need);
         }
         if (!whole) c->lds_whole_max = 0;
+        for (auto& h : c->reps)
+            if (h.dev.use_long_range)
+                c->adj_runs_lds = std::max(c->adj_runs_lds, (size_t)h.dev.nk * sizeof(double2) + (size_t)h.dev.n_runs * sizeof(int4));
+        const char* fr = std::getenv("SPIMC_ADJ_RUNS");
+        if (c->adj_runs_lds > kPairLdsMaxBytes || (fr && std::atoi(fr) == 0)) c->adj_runs_lds = 0;
+        if (c->adj_runs_lds > 0 &&
+            hipFuncSetAttribute((const void*)k_adjoint_long_runs, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)c->adj_runs_lds) != hipSuccess)
+            c->adj_runs_lds = 0;
         if (c->lds_whole_max > 0 &&
             hipFuncSetAttribute((const void*)k_pair_whole_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 c->lds_whole_max * (int)sizeof(double)) != hipSuccess)
@@ -1197,10 +1299,7 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
     int rc = upload_reps(c);
     if (rc) return bail(rc);
     {
-        // replica groups on separate streams (SPIMC_STREAMS, default 1)
-        int n_groups = 1;
-        if (const char* f = std::getenv("SPIMC_STREAMS")) n_groups = std::max(1, std::atoi(f));
-        if ((rc = build_groups(c, n_groups))) return bail(rc);
+        if ((rc = build_groups(c, c->groups_wanted))) return bail(rc);
         if (std::getenv("SPIMC_VERBOSE"))
             fprintf(stderr, "spimc: %d replica group(s) on separate streams\n", (int)c->groups.size());
     }
@@ -1471,7 +1570,10 @@ int spimc_action_gradient(spimc_ctx* c, uint32_t r, uint32_t terms, int compat, 
         rc = launch(c, SPIMC_KERNEL_ADJOINT, k_grad_compat, g, 64, 0, c->stream, base, 0);
     } else {
         rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_local, g, 64, 0, c->stream, base, terms, 0);
-        if (!rc && (terms & SPIMC_GRAD_PAIR_LONG) && R.use_long_range) {
+        if (!rc && (terms & SPIMC_GRAD_PAIR_LONG) && R.use_long_range && c->adj_runs_lds > 0) {
+            rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long_runs, dim3(R.M, 1), kAdjRunBlock, c->adj_runs_lds,
+                        c->stream, base, 0);
+        } else if (!rc && (terms & SPIMC_GRAD_PAIR_LONG) && R.use_long_range) {
             if (R.max_idx > kAdjMaxIdx) return fail(SPIMC_ERR_UNSUPPORTED, "adjoint long range supports |n| <= 40");
             const size_t shmem = (size_t)64 * 3 * (R.max_idx + 1) * sizeof(double2) +
                                  kAdjTile * (sizeof(double2) + sizeof(int) + sizeof(double));

@@ -486,19 +486,22 @@ __device__ __forceinline__ void kt_close(unsigned long long* kt) {
 // replica loads its B-spline tables once, then each wavefront takes window links
 // j = w, w + 16, ...; lanes run over the partners q.
 constexpr int kPairLdsBlock = 1024;
-template <int PT, int NORD>
-__device__ __forceinline__ void pair_lds_block(const Rep* __restrict__ reps, int n_reps) {
+template <int PT, int NORD, int BS>
+__device__ __forceinline__ void pair_lds_block(const Rep* __restrict__ reps, int n_reps, int bpr) {
     extern __shared__ double lds[];
-    const int r = xcd_remap(blockIdx.x, gridDim.x);
+    // bpr workgroups per replica (when replicas are few): part takes every bpr-th
+    // group of BS/64 links
+    const int idx = xcd_remap(blockIdx.x, gridDim.x), r = idx / bpr, part = idx % bpr;
     if (r >= n_reps) return;
     const Rep R = reps[r];  // by value: the fields used stay in registers across the loops
     const Prop pr = *R.prop;
     if (pr.status != 1) return;
-    for (int t = threadIdx.x; t < R.lds_doubles; t += kPairLdsBlock) lds[t] = (t < R.bt_a.ncoef * R.bt_a.ncomp) ? R.bt_a.c[t] : R.bt_b.c[t - R.bt_a.ncoef * R.bt_a.ncomp];
+    for (int t = threadIdx.x; t < R.lds_doubles; t += BS) lds[t] = (t < R.bt_a.ncoef * R.bt_a.ncomp) ? R.bt_a.c[t] : R.bt_b.c[t - R.bt_a.ncoef * R.bt_a.ncomp];
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int M = R.M, N = R.N, p = pr.p;
-    for (int j = w; j < pr.n_links; j += kPairLdsBlock / kWave) {
+    if (part * (BS / kWave) >= pr.n_links) return;
+    for (int j = part * (BS / kWave) + w; j < pr.n_links; j += bpr * (BS / kWave)) {
         const int s0 = (pr.b0 + j) % M, s1 = (s0 + 1) % M;
         double po0[3], po1[3], pn0[3], pn1[3];
         for (int d = 0; d < 3; ++d) {
@@ -542,11 +545,13 @@ __device__ __forceinline__ void pair_lds_block(const Rep* __restrict__ reps, int
 
 // PT: 1 all replicas Bare, 2 all David, 0 mixed (type read per replica);
 // NORD: the David expansion order when all replicas share it, else 0
-template <int PT, int NORD>
-__global__ __launch_bounds__(kPairLdsBlock) void k_pair_lds(const Rep* __restrict__ reps, int n_reps,
-                                                            unsigned long long* __restrict__ kt) {
+// BS: workgroup size; with BS < 1024 a CU keeps room for other kernels' waves
+// (at most 128 VGPRs per wave whatever BS is)
+template <int PT, int NORD, int BS = kPairLdsBlock>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_pair_lds(
+    const Rep* __restrict__ reps, int n_reps, int bpr, unsigned long long* __restrict__ kt) {
     kt_begin(kt, KT_PAIR);
-    pair_lds_block<PT, NORD>(reps, n_reps);
+    pair_lds_block<PT, NORD, BS>(reps, n_reps, bpr);
 }
 
 __global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ reps, const uint2* __restrict__ items,
@@ -1081,16 +1086,22 @@ __device__ __forceinline__ void stage_tables(const Rep& R, double* lds, bool du_
 // walks it in order, so each link (b, b+1) of (i, j) is evaluated once for i:
 // its r_i(b) part goes to bead b, its r_i(b+1) part is carried to bead b+1.
 constexpr int kAdjLdsBlock = 1024;
+// PT, NORD: the pair type and David order shared by all replicas (0: per replica)
+template <int PT, int NORD>
 __global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_local_lds(const Rep* __restrict__ reps, unsigned int terms,
                                                                     int out_est) {
     extern __shared__ double lds[];
-    const Rep& R = reps[blockIdx.x];
+    // gridDim.y workgroups per replica (when replicas are few), each taking a
+    // contiguous block of the slice ranges
+    const Rep R = reps[blockIdx.x];  // by value: the fields used stay in registers
     stage_tables(R, lds, false);
-    const int M = R.M, N = R.N;
-    const int per = kAdjLdsBlock / N > 0 ? kAdjLdsBlock / N : 1;  // slice ranges per particle
-    const int n_units = N * per;
+    const int M = R.M, N = R.N, bpr = gridDim.y;
+    const int per_blk = kAdjLdsBlock / N > 0 ? kAdjLdsBlock / N : 1;  // slice ranges per particle in one workgroup
+    const int per = min(M, per_blk * bpr);                               // ... over the replica's workgroups
+    const int p_lo = (int)((long long)per * blockIdx.y / bpr), p_hi = (int)((long long)per * (blockIdx.y + 1) / bpr);
+    const int n_units = N * (p_hi - p_lo);
     for (int unit = threadIdx.x; unit < n_units; unit += kAdjLdsBlock) {
-        const int i = unit % N, part = unit / N;
+        const int i = unit % N, part = p_lo + unit / N;
         const int b_lo = (int)((long long)M * part / per), b_hi = (int)((long long)M * (part + 1) / per);
         if (b_lo >= b_hi) continue;
         double carry[3] = {0., 0., 0.};  // d/dr_i(b) from link (b-1, b)
@@ -1113,9 +1124,9 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_local_lds(const Rep* _
                     LinkGeom gg;
                     double ur, urp, us;
                     link_geom(rp_, jp, ri_, j0, R.L, R.iL, gg);
-                    pair_u_grad_lds(R, lds, gg.r, gg.rp, gg.s, ur, urp, us);
-                    for (int d = 0; d < 3; ++d)
-                        carry[d] -= urp * gg.rpv[d] / gg.rp - ((gg.s > 0.) ? us * gg.sv[d] / gg.s : 0.);
+                    pair_u_grad_lds<PT, NORD>(R, lds, gg.r, gg.rp, gg.s, ur, urp, us);
+                    const double frp = urp / gg.rp, so = (gg.s > 0.) ? us / gg.s : 0.;
+                    for (int d = 0; d < 3; ++d) carry[d] -= frp * gg.rpv[d] - so * gg.sv[d];
                 }
         }
         for (int b = b_lo; b < b_hi; ++b) {
@@ -1150,11 +1161,11 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_local_lds(const Rep* _
                     LinkGeom gg;
                     double ur, urp, us;
                     link_geom(ri, j0, rn, jn, R.L, R.iL, gg);
-                    pair_u_grad_lds(R, lds, gg.r, gg.rp, gg.s, ur, urp, us);
-                    const double so = (gg.s > 0.) ? us / gg.s : 0.;
+                    pair_u_grad_lds<PT, NORD>(R, lds, gg.r, gg.rp, gg.s, ur, urp, us);
+                    const double fr = ur / gg.r, frp = urp / gg.rp, so = (gg.s > 0.) ? us / gg.s : 0.;
                     for (int d = 0; d < 3; ++d) {
-                        g[d] -= ur * gg.rv[d] / gg.r + so * gg.sv[d];
-                        nxt[d] -= urp * gg.rpv[d] / gg.rp - so * gg.sv[d];
+                        g[d] -= fr * gg.rv[d] + so * gg.sv[d];
+                        nxt[d] -= frp * gg.rpv[d] - so * gg.sv[d];
                     }
                 }
             double* o = (out_est ? R.grad_est : R.grad) + 3 * ((size_t)b * N + i);
@@ -1281,6 +1292,76 @@ __global__ __launch_bounds__(64) void k_adjoint_long(const Rep* __restrict__ rep
         o[0] += f * g[0];
         o[1] += f * g[1];
         o[2] += f * g[2];
+    }
+}
+
+// Long-range adjoint by runs: dS_long/dr_p(b) = -2 s box sum_k c_k n_k Im(conj(rho_k) e^{ik.r}).
+// One workgroup per (slice, replica), a thread per particle.  The k-vectors are
+// walked in runs of consecutive n2 at fixed (n0, n1), so each phase is the
+// previous one times e^{+-i phi_2} (one complex product), and the weights
+// w_k = c_k conj(rho_k) sit in LDS in run order: every lane reads the same
+// address (a broadcast).  Run starts step running phases e^{i n0 phi_0},
+// e^{i n1 phi_1}; the runs are ordered by (n0, n1) so those steps are short.
+constexpr int kAdjRunBlock = 256;
+__global__ __launch_bounds__(kAdjRunBlock) void k_adjoint_long_runs(const Rep* __restrict__ reps, int out_est) {
+    const Rep& R = reps[blockIdx.y];
+    const int b = blockIdx.x;
+    if (b >= R.M || !R.use_long_range || R.pair_type == 0) return;
+    extern __shared__ double2 wsh[];
+    int4* runs = (int4*)(wsh + R.nk);
+    const int nk = R.nk, n_runs = R.n_runs;
+    const double2* rb = R.rho + ((size_t)R.sel[b] * R.M + b) * nk;
+    for (int t = threadIdx.x; t < nk; t += kAdjRunBlock) {
+        const int k = R.k_order[t];
+        const double2 rv = rb[k];
+        const double cf = R.coef_s[k];
+        wsh[t] = make_double2(cf * rv.x, -cf * rv.y);
+    }
+    for (int t = threadIdx.x; t < n_runs; t += kAdjRunBlock) runs[t] = R.k_runs[t];
+    __syncthreads();
+    const double f = -2. * R.long_scale * R.box;
+    for (int p = threadIdx.x; p < R.N; p += kAdjRunBlock) {
+        double2 st[3];
+        for (int d = 0; d < 3; ++d) {
+            double sn, cs;
+            sincos(pos_at(R, d, b, p) * R.box, &sn, &cs);
+            st[d] = make_double2(cs, sn);
+        }
+        const double2 cst0 = make_double2(st[0].x, -st[0].y), cst1 = make_double2(st[1].x, -st[1].y);
+        const double2 cst2 = make_double2(st[2].x, -st[2].y);
+        double2 e0 = make_double2(1., 0.), e1 = make_double2(1., 0.);
+        int c0 = 0, c1 = 0, off = 0;
+        double g0 = 0., g1 = 0., g2 = 0.;
+        for (int u = 0; u < n_runs; ++u) {
+            const int4 h = runs[u];
+            for (; c0 < h.x; ++c0) e0 = cmul(e0, st[0]);
+            for (; c0 > h.x; --c0) e0 = cmul(e0, cst0);
+            for (; c1 < h.y; ++c1) e1 = cmul(e1, st[1]);
+            for (; c1 > h.y; --c1) e1 = cmul(e1, cst1);
+            double2 e = cmul(e0, e1);
+            const int n2 = h.z, len = h.w > 0 ? h.w : -h.w;
+            for (int m = 0; m < n2; ++m) e = cmul(e, st[2]);
+            for (int m = 0; m > n2; --m) e = cmul(e, cst2);
+            const double2 step = h.w > 0 ? st[2] : cst2;
+            const double dn = h.w > 0 ? 1. : -1.;
+            double sr = 0., tr = 0., n2d = (double)n2;
+            for (int i = 0; i < len; ++i) {
+                const double2 w = wsh[off + i];
+                const double im = w.x * e.y + w.y * e.x;  // Im(w e)
+                sr += im;
+                tr = fma(im, n2d, tr);
+                n2d += dn;
+                e = cmul(e, step);
+            }
+            off += len;
+            g0 = fma((double)h.x, sr, g0);
+            g1 = fma((double)h.y, sr, g1);
+            g2 += tr;
+        }
+        double* o = (out_est ? R.grad_est : R.grad) + 3 * ((size_t)b * R.N + p);
+        o[0] += f * g0;
+        o[1] += f * g1;
+        o[2] += f * g2;
     }
 }
 

@@ -151,6 +151,26 @@ def test_adjoint_matches_central_difference(sysname, terms):
     assert rel_err(got, ref) < 1e-6
 
 
+def test_adjoint_long_range_on_a_ragged_k_set():
+    """The long-range adjoint walks the k-set in runs of consecutive n2; a k-set
+    with holes (every third vector dropped, shuffled order) splits the runs at
+    the holes and at n2 = 0 in every way."""
+    s = system.config_c1()
+    rng = np.random.default_rng(1)
+    keep = np.array([i for i in range(s.n_k) if i % 3 != 1])
+    keep = keep[rng.permutation(len(keep))]
+    s.k_n = s.k_n[keep]
+    s.v_long_k = s.v_long_k[keep]
+    pos = system.random_path(s, seed=6, spread=0.1)
+    o = Oracle(s)
+    o.upload(pos)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        got = e.gradient(abi.GRAD_PAIR_LONG)
+    ref = o.gradient(abi.GRAD_PAIR_LONG, compat=False, h=1e-5)
+    assert rel_err(got, ref) < 1e-6
+
+
 def test_observables_after_replay(sysname):
     s, pos = make(sysname, spread=0.05)
     o = Oracle(s, seed=21)
@@ -261,6 +281,45 @@ def test_c2_full_size_properties():
         assert np.max(np.abs(tot)) < 1e-9 * max(1.0, np.max(np.abs(g)))
         c = e.counters()
         assert c[0] == 256 and c[2] == 16 and 0 < c[1] < 256
+
+
+def test_large_replica_spread_over_workgroups():
+    """N=1024, M=256, 4012 k-vectors (the large HEG case): a single replica's
+    pair links and adjoint slice ranges are spread over many workgroups.  The
+    replayed chain matches the oracle bit for bit; the incremental rho_k equals
+    a rebuild; short- and long-range pair gradients sum to zero over the
+    particles of every slice (translation invariance)."""
+    import os
+    import oracle_ffi
+
+    s = system.config_c5()
+    assert s.n_k == 4012
+    pos = system.random_path(s, seed=31, spread=0.05)
+    oracle_ffi.set_threads(min(16, len(os.sched_getaffinity(0))))
+    try:
+        o = Oracle(s, seed=5)
+        o.upload(pos)
+        o.record(True)
+        ref = np.concatenate([o.bisect(24), o.displace(3)])
+        tape = o.recorded()
+        final = o.download()
+        ref_rho = o.rho_k()
+    finally:
+        oracle_ffi.set_threads(1)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        e.tape_set(tape)
+        got = np.concatenate([e.bisect(24)[0], e.displace(3)[0]])
+        assert np.array_equal(got, ref)
+        assert np.array_equal(e.download(), final)
+        inc = e.rho_k()
+        assert rel_err(inc, ref_rho) < 1e-11 * s.n_part
+        e.rho_k_rebuild()
+        assert rel_err(inc, e.rho_k()) < 1e-11 * s.n_part
+        for terms in (abi.GRAD_PAIR_SHORT, abi.GRAD_PAIR_LONG):
+            g = e.gradient(terms)  # [M][N][3]
+            tot = g.sum(axis=1)
+            assert np.max(np.abs(tot)) < 1e-9 * max(1.0, float(np.max(np.abs(g)))), terms
 
 
 def test_global_table_kernels_match_lds_kernels(monkeypatch):
