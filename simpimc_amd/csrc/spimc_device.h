@@ -74,6 +74,19 @@ struct Counters {
     unsigned long long rho_slicek;    // (moved slice, k) updates of the rho_k kernel
 };
 
+// One rho_k work item of a move round: a (moved bead, k chunk) of one replica,
+// described in full so that the rho_k kernel reads one 64-byte record per item.
+struct __align__(64) RhoItem {
+    const double2* rc;    // committed rho_k row of the bead's slice
+    double2* rn;          // spare row: receives the proposal's rho_k
+    const double2* ph;    // the bead's six phase chains (OLD x3, NEW x3)
+    const int* kidx;      // packed k-vectors
+    const double* coef;   // long-range coefficients per k
+    double2* part;        // (OLD, NEW) partial sums of this item
+    int k0, k1;           // k range
+    int stride, valid;    // phase-chain stride; 0 = idle item
+};
+
 struct Rep {
     // sizes and flags
     int N, M, nk, max_idx;
@@ -97,7 +110,10 @@ struct Rep {
     const double* kmag;    // |k|
     const int4* k_runs;    // [n_runs] runs of consecutive n2 at fixed (n0, n1): {n0, n1, n2 first, +-length}
     const int* k_order;    // [nk] k index of the i-th k-vector in run order
-    int n_runs, pad_runs;
+    int n_runs, rho_kpt;   // rho_kpt: k-vectors per thread of the rho_k kernel (sets the k chunks)
+    RhoItem* ritems_bis;   // this replica's rho_k work items of a bisection round
+    RhoItem* ritems_dis;   // ... and of a displace round
+    int n_ritems_bis, n_ritems_dis;
     const double* coef_s;  // per-slice action coefficient per k
     const double* coef_b;  // per-slice beta-derivative / potential coefficient per k
     // pair tables

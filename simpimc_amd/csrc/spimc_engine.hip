@@ -34,7 +34,7 @@ int fail(int code, const std::string& msg) {
             return fail(SPIMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kRhoKpt = 4;        // k-vectors per thread in k_rho (default; SPIMC_RHO_KPT 2, 4, 8, 16)
+constexpr int kRhoKpt = 8;        // k-vectors per thread in k_rho (default; SPIMC_RHO_KPT 2, 4, 8, 16)
 constexpr int kRhoKptMin = 2;     // smallest chunking: bounds the number of k chunks
 constexpr int kNumCUs = 256;      // MI355X compute units (for workgroups-per-replica choices)
 constexpr int kBuildKpt = 2;
@@ -274,14 +274,21 @@ struct Group {
     hipStream_t stream = nullptr;
     int r0 = 0, n = 0;
     int max_bis_links = 0, max_M = 0;  // largest bisection window and slice count in the group
-    // work lists (absolute replica indices)
-    uint2 *d_pair_bis = nullptr, *d_rho_bis = nullptr, *d_pair_dis = nullptr, *d_rho_dis = nullptr;
-    int n_pair_bis = 0, n_rho_bis = 0, n_pair_dis = 0, n_rho_dis = 0;
+    // pair work lists (absolute replica indices; global-table pair kernel)
+    uint2 *d_pair_bis = nullptr, *d_pair_dis = nullptr;
+    int n_pair_bis = 0, n_pair_dis = 0;
+    // rho_k work items, written by the proposal kernels: the group's replicas' items in order
+    RhoItem *d_ritem_bis = nullptr, *d_ritem_dis = nullptr;
+    int n_ritem_bis = 0, n_ritem_dis = 0;
     // captured round graphs and whether each was captured with the device clocks on
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool graph_clock[2] = {false, false};
     unsigned long long* d_kt = nullptr;  // device clock slots of this group's rounds
     hipEvent_t done = nullptr;
+    // second stream of the group: the rho_k kernel of a round runs on it beside
+    // the pair kernel (both need only the proposal), joined before the decision
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 struct spimc_ctx {
@@ -308,9 +315,6 @@ struct spimc_ctx {
     uint32_t gr_n_r = 100;
     double* d_gr_rmax = nullptr;
     bool adjoint_fresh = false;
-    // one launch for the pair and rho_k work of a round (SPIMC_FUSED=1); measured slower
-    // than two launches on MI355X (the pair blocks' registers cap the rho_k blocks' occupancy)
-    bool fused = false;
     // LDS bytes/8 of the largest pair table when every replica's tables stage through LDS, else 0
     int lds_doubles_max = 0;
     // the LDS pair kernel specialised for the replicas' pair type (mixed types: generic)
@@ -318,10 +322,14 @@ struct spimc_ctx {
     int pair_block = kPairLdsBlock;
     void (*adj_lds_fn)(const Rep*, unsigned int, int) = nullptr;
     int groups_wanted = 1;
+    bool fork = false;  // rho_k beside the pair kernel on a second stream per group
     // k_rho variant: k-vectors per thread and non-temporal rho_k traffic
     int rho_kpt = kRhoKpt;
     bool rho_nt = true;  // measured: slightly faster, and keeps the pair tables in L2
     void (*rho_fn)(const Rep*, const uint2*, int, int, unsigned long long*) = nullptr;
+    // looping variant with rho_bpc workgroups per CU (0: one workgroup per item)
+    int rho_bpc = 0;
+    void (*rho_loop_fn)(const RhoItem*, int, unsigned long long*) = nullptr;
     // LDS doubles of the adjoint kernel (pair tables) and of the whole-path
     // estimator kernel (Bare: pair tables, David: du table); 0 = global tables
     int lds_adj_max = 0, lds_whole_max = 0;
@@ -406,7 +414,7 @@ int sync_and_resolve(spimc_ctx* c) {
     if (c->clock) {
         for (int k = 0; k < KT_CLASSES; ++k) c->kt_sum[k] = c->kt_launches[k] = 0;
         for (auto& G : c->groups) {
-            unsigned long long kt[KT_WORDS];
+            unsigned long long kt[KT_END];
             HIPCHK(hipMemcpy(kt, G.d_kt, sizeof(kt), hipMemcpyDeviceToHost));
             for (int k = 0; k < KT_CLASSES; ++k) {
                 c->kt_sum[k] += kt[2 + 2 * k];
@@ -672,6 +680,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     R.n_links_max = M;
     R.n_chunks = h.n_chunks;
     R.n_runs = (int)h.k_runs.size();
+    R.rho_kpt = c->rho_kpt;
     R.index = r;
     R.L = L;
     R.iL = 1. / L;
@@ -788,23 +797,42 @@ int upload_items(const std::vector<uint2>& v, uint2** d, int* n) {
 }
 
 int build_work_lists(spimc_ctx* c, Group& G) {
-    std::vector<uint2> pb, rb, pd, rd;
+    std::vector<uint2> pb, pd;
+    int nb_items = 0, nd_items = 0;
     for (uint32_t r = (uint32_t)G.r0; r < (uint32_t)(G.r0 + G.n); ++r) {
         const Rep& R = c->reps[r].dev;
         const int nb = 1 << R.n_level;
         for (int j = 0; j < nb; ++j) pb.push_back(make_uint2(r, j));
         for (int j = 0; j < R.M; ++j) pd.push_back(make_uint2(r, j));
         if (R.use_long_range) {
-            for (int m = 0; m < nb - 1; ++m)
-                for (int ch = 0; ch < R.n_chunks; ++ch) rb.push_back(make_uint2(r, (m << 8) | ch));
-            for (int m = 0; m < R.M; ++m)
-                for (int ch = 0; ch < R.n_chunks; ++ch) rd.push_back(make_uint2(r, (m << 8) | ch));
+            nb_items += (nb - 1) * R.n_chunks;
+            nd_items += R.M * R.n_chunks;
         }
     }
     int rc;
-    if ((rc = upload_items(pb, &G.d_pair_bis, &G.n_pair_bis)) || (rc = upload_items(rb, &G.d_rho_bis, &G.n_rho_bis)) ||
-        (rc = upload_items(pd, &G.d_pair_dis, &G.n_pair_dis)) || (rc = upload_items(rd, &G.d_rho_dis, &G.n_rho_dis)))
+    if ((rc = upload_items(pb, &G.d_pair_bis, &G.n_pair_bis)) || (rc = upload_items(pd, &G.d_pair_dis, &G.n_pair_dis)))
         return rc;
+    // rho_k items: zeroed (idle) until a proposal kernel fills its replica's range
+    for (int mv = 0; mv < 2; ++mv) {
+        const int n = mv == 0 ? nb_items : nd_items;
+        RhoItem*& d = mv == 0 ? G.d_ritem_bis : G.d_ritem_dis;
+        (mv == 0 ? G.n_ritem_bis : G.n_ritem_dis) = n;
+        if (!n) continue;
+        HIPCHK(hipMalloc((void**)&d, n * sizeof(RhoItem)));
+        HIPCHK(hipMemset(d, 0, n * sizeof(RhoItem)));
+    }
+    int ob = 0, od = 0;
+    for (int r = G.r0; r < G.r0 + G.n; ++r) {
+        Rep& R = c->reps[r].dev;
+        if (!R.use_long_range) continue;
+        const int nb = ((1 << R.n_level) - 1) * R.n_chunks, nd = R.M * R.n_chunks;
+        R.ritems_bis = G.d_ritem_bis + ob;
+        R.ritems_dis = G.d_ritem_dis + od;
+        R.n_ritems_bis = nb;
+        R.n_ritems_dis = nd;
+        ob += nb;
+        od += nd;
+    }
     return SPIMC_OK;
 }
 
@@ -837,9 +865,14 @@ int build_groups(spimc_ctx* c, int n_groups) {
         else if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess)
             return fail(SPIMC_ERR_HIP, "stream creation failed");
         if (hipEventCreateWithFlags(&G.done, hipEventDisableTiming) != hipSuccess) return fail(SPIMC_ERR_HIP, "event");
-        const unsigned long long kt0[KT_WORDS] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
-        HIPCHK(hipMalloc((void**)&G.d_kt, sizeof(kt0)));
-        HIPCHK(hipMemcpy(G.d_kt, kt0, sizeof(kt0), hipMemcpyHostToDevice));
+        if (c->fork) {
+            if (hipStreamCreateWithFlags(&G.side, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&G.ev_fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&G.ev_join, hipEventDisableTiming) != hipSuccess)
+                return fail(SPIMC_ERR_HIP, "fork stream");
+        }
+        HIPCHK(hipMalloc((void**)&G.d_kt, KT_WORDS * sizeof(unsigned long long)));
+        HIPCHK(hipMemset(G.d_kt, 0, KT_WORDS * sizeof(unsigned long long)));
         int rc = build_work_lists(c, G);
         if (rc) return rc;
     }
@@ -864,6 +897,13 @@ int ensure_tmp(spimc_ctx* c, size_t n_doubles, size_t n_items) {
 }
 
 // One round of a move stage for every replica.
+// The rho_k kernel of a round: rho_bpc workgroups per CU looping over the items
+// (0: one workgroup per item).
+int launch_rho(spimc_ctx* c, hipStream_t st, const RhoItem* ri, int nr, unsigned long long* kt) {
+    const int g = c->rho_bpc > 0 ? std::min(nr, c->rho_bpc * kNumCUs) : nr;
+    return launch(c, SPIMC_KERNEL_RHO, c->rho_loop_fn, g, kRhoBlock, 0, st, ri, nr, kt);
+}
+
 int launch_round(spimc_ctx* c, const Group& G, int move) {
     const int R = G.n;
     unsigned long long* kt = c->clock ? G.d_kt : nullptr;
@@ -879,9 +919,21 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
     if (rc) return rc;
     const uint2* pi = move == 0 ? G.d_pair_bis : G.d_pair_dis;
     const int np = move == 0 ? G.n_pair_bis : G.n_pair_dis;
-    const uint2* ri = move == 0 ? G.d_rho_bis : G.d_rho_dis;
-    const int nr = move == 0 ? G.n_rho_bis : G.n_rho_dis;
-    if (c->lds_doubles_max > 0 && np) {
+    const RhoItem* ri = move == 0 ? G.d_ritem_bis : G.d_ritem_dis;
+    const int nr = move == 0 ? G.n_ritem_bis : G.n_ritem_dis;
+    if (c->lds_doubles_max > 0 && np && G.side && nr) {
+        // fork: rho_k on the side stream beside the pair kernel, joined before the decision
+        HIPCHK(hipEventRecord(G.ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(G.side, G.ev_fork, 0));
+        if ((rc = launch_rho(c, G.side, ri, nr, kt))) return rc;
+        HIPCHK(hipEventRecord(G.ev_join, G.side));
+        const size_t shmem = (size_t)c->lds_doubles_max * sizeof(double);
+        const int waves = c->pair_block / 64, max_links = move == 0 ? G.max_bis_links : G.max_M;
+        const int bpr = std::max(1, std::min((max_links + waves - 1) / waves, (2 * kNumCUs + R - 1) / R));
+        rc = launch(c, SPIMC_KERNEL_PAIR, c->pair_lds_fn, R * bpr, c->pair_block, shmem, st, reps, R, bpr, kt);
+        if (rc) return rc;
+        HIPCHK(hipStreamWaitEvent(st, G.ev_join, 0));
+    } else if (c->lds_doubles_max > 0 && np) {
         const size_t shmem = (size_t)c->lds_doubles_max * sizeof(double);
         // workgroups per replica: one when the replicas fill the GPU, more (up to one
         // per BS/64 links) when they are few
@@ -889,11 +941,7 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
         const int bpr = std::max(1, std::min((max_links + waves - 1) / waves, (2 * kNumCUs + R - 1) / R));
         rc = launch(c, SPIMC_KERNEL_PAIR, c->pair_lds_fn, R * bpr, c->pair_block, shmem, st, reps, R, bpr, kt);
         if (rc) return rc;
-        if (nr) rc = launch(c, SPIMC_KERNEL_RHO, c->rho_fn, nr, kRhoBlock, 0, st, all, ri, nr, 0, kt);
-        if (rc) return rc;
-    } else if (c->fused && np && nr) {
-        const int g = (np + 3) / 4 + nr;
-        rc = launch(c, SPIMC_KERNEL_RHO, k_eval<kRhoKpt>, g, kRhoBlock, 0, st, all, pi, np, ri, nr);
+        if (nr) rc = launch_rho(c, st, ri, nr, kt);
         if (rc) return rc;
     } else {
         if (np) {
@@ -902,8 +950,7 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
             if (rc) return rc;
         }
         if (nr) {
-            rc = launch(c, SPIMC_KERNEL_RHO, c->rho_fn, nr, kRhoBlock, 0, st, all, ri, nr, 0, kt);
-            if (rc) return rc;
+            if ((rc = launch_rho(c, st, ri, nr, kt))) return rc;
         }
     }
     return launch(c, SPIMC_KERNEL_DECIDE, k_decide, R, 64, 0, st, reps, R, kt);
@@ -1177,23 +1224,33 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         return rc;
     };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(SPIMC_ERR_HIP, "hipGetDevice"));
-    if (const char* f = std::getenv("SPIMC_FUSED")) c->fused = std::atoi(f) != 0;
     if (const char* f = std::getenv("SPIMC_RHO_KPT")) {
         const int v = std::atoi(f);
         if (v == 2 || v == 4 || v == 8 || v == 16) c->rho_kpt = v;
     }
     if (const char* f = std::getenv("SPIMC_RHO_NT")) c->rho_nt = std::atoi(f) != 0;
-    if (c->fused) c->rho_kpt = kRhoKpt;  // the fused kernel is built for the default chunking
-    // replica groups on separate streams: two when there are enough replicas to
-    // fill the GPU twice over (SPIMC_STREAMS overrides)
-    c->groups_wanted = n_replicas >= 64 ? 2 : 1;
+    // Launch shape (measured best on MI355X at 1024 replicas of N=256, M=128):
+    // with enough replicas to fill the GPU, each round's rho_k kernel runs on a
+    // second stream beside its pair kernel, looping over its items with 2
+    // workgroups per CU so that every CU keeps room for a pair workgroup.
+    // SPIMC_STREAMS (replica groups), SPIMC_FORK and SPIMC_RHO_BPC override.
+    c->groups_wanted = 1;
     if (const char* f = std::getenv("SPIMC_STREAMS")) c->groups_wanted = std::max(1, std::atoi(f));
+    c->fork = n_replicas >= 64;
+    if (const char* f = std::getenv("SPIMC_FORK")) c->fork = std::atoi(f) != 0;
+    c->rho_bpc = c->fork ? 2 : c->groups_wanted > 1 ? 3 : 0;
+    if (const char* f = std::getenv("SPIMC_RHO_BPC")) c->rho_bpc = std::max(0, std::atoi(f));
     {
         using RhoFn = void (*)(const Rep*, const uint2*, int, int, unsigned long long*);
         const RhoFn fns[2][4] = {{k_rho<2, false>, k_rho<4, false>, k_rho<8, false>, k_rho<16, false>},
                                  {k_rho<2, true>, k_rho<4, true>, k_rho<8, true>, k_rho<16, true>}};
         const int ki = c->rho_kpt == 2 ? 0 : c->rho_kpt == 4 ? 1 : c->rho_kpt == 8 ? 2 : 3;
         c->rho_fn = fns[c->rho_nt ? 1 : 0][ki];
+        using RhoLoopFn = void (*)(const RhoItem*, int, unsigned long long*);
+        const RhoLoopFn lfns[2][4] = {
+            {k_rho_loop<2, false>, k_rho_loop<4, false>, k_rho_loop<8, false>, k_rho_loop<16, false>},
+            {k_rho_loop<2, true>, k_rho_loop<4, true>, k_rho_loop<8, true>, k_rho_loop<16, true>}};
+        c->rho_loop_fn = lfns[c->rho_nt ? 1 : 0][ki];
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(SPIMC_ERR_HIP, "stream creation failed"));
@@ -1242,9 +1299,9 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
                 pt = (pt < 0 || pt == h.dev.pair_type) ? h.dev.pair_type : 0;
                 nord = (nord < 0 || nord == h.dev.n_order) ? h.dev.n_order : 0;
             }
-        // 512-thread pair workgroups leave each CU room for the other stream's rho_k
-        // workgroups (measured best with two replica groups)
-        int bs = c->groups_wanted > 1 ? 512 : kPairLdsBlock;
+        // 512-thread pair workgroups leave each CU room for the rho_k workgroups of
+        // the other stream
+        int bs = (c->groups_wanted > 1 || c->fork) ? 512 : kPairLdsBlock;
         if (const char* fb = std::getenv("SPIMC_PAIR_BLOCK")) {
             const int v = std::atoi(fb);
             if (v == 256 || v == 512 || v == 768 || v == 1024) bs = v;
@@ -1292,16 +1349,22 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
             c->lds_whole_max = 0;
     }
     if (std::getenv("SPIMC_VERBOSE"))
-        fprintf(stderr, "spimc: %u replica(s), pair kernel %s (%d LDS doubles), rho_k kernel %s, adjoint %s, estimators %s\n",
-                n_replicas, c->lds_doubles_max ? "k_pair_lds" : (c->fused ? "k_eval (fused)" : "k_pair"),
-                c->lds_doubles_max, c->fused && !c->lds_doubles_max ? "fused" : "k_rho",
+        fprintf(stderr,
+                "spimc: %u replica(s), pair kernel %s (%d LDS doubles, %d threads), rho_k kernel %s (%d k/thread, %d "
+                "workgroups/CU), long-range adjoint %s, adjoint %s, estimators %s\n",
+                n_replicas, c->lds_doubles_max ? "k_pair_lds" : "k_pair",
+                c->lds_doubles_max, c->pair_block,
+                "k_rho_loop",
+                c->rho_kpt, c->rho_bpc, c->adj_runs_lds ? "k_adjoint_long_runs" : "k_adjoint_long",
                 c->lds_adj_max ? "k_adjoint_local_lds" : "k_adjoint_local", c->lds_whole_max ? "k_pair_whole_lds" : "k_pair_whole");
     int rc = upload_reps(c);
     if (rc) return bail(rc);
     {
         if ((rc = build_groups(c, c->groups_wanted))) return bail(rc);
+        if ((rc = upload_reps(c))) return bail(rc);  // the replicas now point at their rho_k items
         if (std::getenv("SPIMC_VERBOSE"))
-            fprintf(stderr, "spimc: %d replica group(s) on separate streams\n", (int)c->groups.size());
+            fprintf(stderr, "spimc: %d replica group(s), rho_k beside the pair kernel on a second stream: %s\n",
+                    (int)c->groups.size(), c->fork ? "yes" : "no");
     }
     if ((rc = spimc_obs_configure(c, SPIMC_ENERGY_THERMAL, 0, 1e300, 0., 0., 100))) return bail(rc);
     *out = c;
@@ -1315,9 +1378,15 @@ int spimc_ctx_destroy(spimc_ctx* c) {
         if (G.stream) (void)hipStreamSynchronize(G.stream);
         for (auto& g : G.graph)
             if (g) (void)hipGraphExecDestroy(g);
-        for (void* p : {(void*)G.d_pair_bis, (void*)G.d_rho_bis, (void*)G.d_pair_dis, (void*)G.d_rho_dis, (void*)G.d_kt})
+        for (void* p : {(void*)G.d_pair_bis, (void*)G.d_pair_dis, (void*)G.d_ritem_bis, (void*)G.d_ritem_dis, (void*)G.d_kt})
             if (p) (void)hipFree(p);
         if (G.done) (void)hipEventDestroy(G.done);
+        if (G.side) {
+            (void)hipStreamSynchronize(G.side);
+            (void)hipStreamDestroy(G.side);
+        }
+        if (G.ev_fork) (void)hipEventDestroy(G.ev_fork);
+        if (G.ev_join) (void)hipEventDestroy(G.ev_join);
         if (G.stream && G.stream != c->stream) (void)hipStreamDestroy(G.stream);
     }
     if (c->stage_ready) (void)hipEventDestroy(c->stage_ready);
@@ -1812,8 +1881,7 @@ int spimc_timing_reset(spimc_ctx* c) {
         c->t_n[i] = 0;
         c->t_bytes[i] = 0.;
     }
-    const unsigned long long kt0[KT_WORDS] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
-    for (auto& G : c->groups) HIPCHK(hipMemcpy(G.d_kt, kt0, sizeof(kt0), hipMemcpyHostToDevice));
+    for (auto& G : c->groups) HIPCHK(hipMemset(G.d_kt, 0, KT_WORDS * sizeof(unsigned long long)));
     for (int k = 0; k < KT_CLASSES; ++k) c->kt_sum[k] = c->kt_launches[k] = 0;
     return SPIMC_OK;
 }

@@ -98,6 +98,36 @@ __device__ __forceinline__ void moved_phase_chain(const Rep& R, int m, int c, co
 // Row descriptor of moved bead m at slice s: k_rho streams rho[sel][s] -> rho[1-sel][s].
 __device__ __forceinline__ void moved_row(const Rep& R, int m, int s) { R.rho_rows[m] = make_int2(s, R.sel[s]); }
 
+// The rho_k work items of a replica's round, written by its proposal kernel
+// (one wavefront): item t = (moved bead t / n_chunks, k chunk t % n_chunks),
+// valid when the proposal goes to level 0 and moves that bead (bead m sits at
+// slice first + m).  Idle items (no attempt left, rejected at a coarse level,
+// fewer moved beads) are marked invalid.
+__device__ __forceinline__ void rho_items_write(const Rep& R, int move, int status, int count, int first, int lane) {
+    RhoItem* items = move == 0 ? R.ritems_bis : R.ritems_dis;
+    const int n = move == 0 ? R.n_ritems_bis : R.n_ritems_dis;
+    if (!items) return;
+    const int stride = R.max_idx + 1, nch = R.n_chunks, ck = kRhoBlock * R.rho_kpt;
+    for (int t = lane; t < n; t += kWave) {
+        const int m = t / nch, ch = t % nch;
+        RhoItem d{};
+        if (status == 1 && m < count) {
+            const int s = (first + m) % R.M, cur = R.sel[s];
+            d.rc = R.rho + ((size_t)cur * R.M + s) * R.nk;
+            d.rn = R.rho + ((size_t)(1 - cur) * R.M + s) * R.nk;
+            d.ph = R.phase + (size_t)m * 6 * stride;
+            d.kidx = R.kidx;
+            d.coef = R.coef_s;
+            d.part = R.rho_part + m * nch + ch;
+            d.k0 = ch * ck;
+            d.k1 = min(R.nk, d.k0 + ck);
+            d.stride = stride;
+            d.valid = 1;
+        }
+        items[t] = d;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // K3: rho_k of every slice (Species::InitRhoK, species_class.h:380-403);
 // particle sums in particle order per k.
@@ -168,6 +198,7 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
     __syncthreads();
     if (!s_info[0]) {
         if (lane == 0) pr->status = 0;
+        rho_items_write(R, 0, 0, 0, 0, lane);
         return;
     }
     const int n = R.n_level, nb = 1 << n, N = R.N, M = R.M;
@@ -284,13 +315,15 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
                     pr->count = 0;
                     pr->tape_next = cur;
                 }
+                rho_items_write(R, 0, 2, 0, 0, lane);
                 return;
             }
             prev = change;
         } else {
             const int any_bad = __any(bad);
+            const int status = (any_bad || s_info[3]) ? 3 : 1;
             if (lane == 0) {
-                pr->status = (any_bad || s_info[3]) ? 3 : 1;
+                pr->status = status;
                 pr->move = 0;
                 pr->p = p;
                 pr->b0 = b0;
@@ -309,7 +342,7 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
             if (R.use_long_range) {
                 for (int t = lane; t < (wraps ? 0 : 6 * (nb - 1)); t += kWave)
                     moved_phase_chain(R, t / 6, t % 6, sh_old[t / 6 + 1], sh_new[t / 6 + 1]);
-                for (int j = lane; j < (wraps ? 0 : nb - 1); j += kWave) moved_row(R, j, (b0 + 1 + j) % M);
+                rho_items_write(R, 0, status, wraps ? 0 : nb - 1, (b0 + 1) % M, lane);
             }
         }
     }
@@ -322,7 +355,7 @@ __global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__
     if (r >= n_reps) return;
     const Rep& R = reps[r];
     const int lane = threadIdx.x;
-    __shared__ int s_act, s_p;
+    __shared__ int s_act, s_p, s_status;
     __shared__ double s_dr[3];
     Prop* pr = R.prop;
     if (lane == 0) {
@@ -368,12 +401,16 @@ __global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__
             pr->prev = 0.;
             pr->u0 = u0;
             pr->tape_next = cur;
+            s_status = bad ? 3 : 1;
         } else {
             pr->status = 0;
         }
     }
     __syncthreads();
-    if (!s_act) return;
+    if (!s_act) {
+        rho_items_write(R, 1, 0, 0, 0, lane);
+        return;
+    }
     const int p = s_p;
     for (int s = lane; s < R.M; s += kWave)
         for (int d = 0; d < 3; ++d) R.prop_new[3 * s + d] = pos_at(R, d, s, p) + s_dr[d];
@@ -387,8 +424,7 @@ __global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__
             }
             moved_phase_chain(R, s, t % 6, xo, xn);
         }
-    if (R.use_long_range)
-        for (int s = lane; s < R.M; s += kWave) moved_row(R, s, s);
+    if (R.use_long_range) rho_items_write(R, 1, s_status, R.M, 0, lane);
 }
 
 // Phase tables of an interface proposal's moved beads (before k_rho in interface mode).
@@ -453,33 +489,50 @@ __device__ __forceinline__ void pair_block(const Rep* __restrict__ reps, const u
 }
 
 // Device-side kernel clocks.  Block 0 of a timed launch (the first dispatched)
-// stores its start on the constant-rate wall clock; the round's decide kernel
-// closes the spans at its own start: pair = rho_k start - pair start, rho_k =
-// decide start - rho_k start, i.e. each launch from its dispatch to the next
-// one's, as a kernel trace of back-to-back launches reports it.  No atomics, so
-// timed and untimed runs execute the same memory traffic.
-// kt[c]: open start of class c (0 = none); kt[2 + 2c], kt[3 + 2c]: summed ticks
-// and launches of class c.
-enum { KT_RHO = 0, KT_PAIR = 1, KT_CLASSES = 2, KT_WORDS = 6 };
+// stores its start on the constant-rate wall clock; every block stores its end
+// in a slot of its own (plain stores; grids above KT_SLOTS blocks fold several
+// blocks into a slot with atomicMax).  The round's decide kernel (its first
+// wavefront) takes the latest end of each class, adds end - start to a running
+// sum and clears the slots: a launch is timed from its first block's start to
+// its last block's end, whatever runs beside it on other streams.
+// kt[c]: open start of class c (0 = none); kt[2 + 2c], kt[3 + 2c]: summed
+// ticks and launches; kt[KT_END + c * KT_SLOTS + slot]: block ends.
+enum { KT_RHO = 0, KT_PAIR = 1, KT_CLASSES = 2, KT_SLOTS = 1024, KT_END = 8, KT_WORDS = KT_END + KT_CLASSES * KT_SLOTS };
 __device__ __forceinline__ void kt_begin(unsigned long long* kt, int cls) {
     if (kt && blockIdx.x == 0 && threadIdx.x == 0) kt[cls] = (unsigned long long)wall_clock64();
 }
+__device__ __forceinline__ void kt_end(unsigned long long* kt, int cls) {
+    if (!kt) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long* slot = kt + KT_END + cls * KT_SLOTS + blockIdx.x % KT_SLOTS;
+        const unsigned long long now = (unsigned long long)wall_clock64();
+        if (gridDim.x <= KT_SLOTS)
+            *slot = now;
+        else
+            atomicMax(slot, now);
+    }
+}
+// called by one full wavefront
 __device__ __forceinline__ void kt_close(unsigned long long* kt) {
-    const unsigned long long now = (unsigned long long)wall_clock64();
-    const unsigned long long ps = kt[KT_PAIR], rs = kt[KT_RHO];
-    if (ps) {
-        const unsigned long long pe = rs ? rs : now;
-        if (pe >= ps) {
-            kt[2 + 2 * KT_PAIR] += pe - ps;
-            kt[3 + 2 * KT_PAIR] += 1ull;
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int c = 0; c < KT_CLASSES; ++c) {
+        unsigned long long e = 0;
+        for (int t = lane; t < KT_SLOTS; t += kWave) {
+            unsigned long long* slot = kt + KT_END + c * KT_SLOTS + t;
+            e = max(e, *slot);
+            *slot = 0ull;
+        }
+        for (int o = kWave / 2; o > 0; o >>= 1) e = max(e, (unsigned long long)__shfl_xor(e, o));
+        if (lane == 0) {
+            const unsigned long long b = kt[c];
+            if (b && e >= b) {
+                kt[2 + 2 * c] += e - b;
+                kt[3 + 2 * c] += 1ull;
+            }
+            kt[c] = 0ull;
         }
     }
-    if (rs && now >= rs) {
-        kt[2 + 2 * KT_RHO] += now - rs;
-        kt[3 + 2 * KT_RHO] += 1ull;
-    }
-    kt[KT_PAIR] = 0ull;
-    kt[KT_RHO] = 0ull;
 }
 
 // K1 with the link-action tables staged in LDS: one 1024-thread workgroup per
@@ -552,6 +605,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const Rep* __restrict__ reps, int n_reps, int bpr, unsigned long long* __restrict__ kt) {
     kt_begin(kt, KT_PAIR);
     pair_lds_block<PT, NORD, BS>(reps, n_reps, bpr);
+    kt_end(kt, KT_PAIR);
 }
 
 __global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ reps, const uint2* __restrict__ items,
@@ -559,6 +613,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pair(const Rep* __restrict__ rep
                                                      unsigned long long* __restrict__ kt) {
     kt_begin(kt, KT_PAIR);
     pair_block(reps, items, n_items, interface_mode, xcd_remap(blockIdx.x, gridDim.x));
+    kt_end(kt, KT_PAIR);
 }
 
 // ---------------------------------------------------------------------------
@@ -638,21 +693,82 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps,
                                                    unsigned long long* __restrict__ kt) {
     kt_begin(kt, KT_RHO);
     rho_block<KPT, NT>(reps, items, n_items, interface_mode, xcd_remap(blockIdx.x, gridDim.x));
+    kt_end(kt, KT_RHO);
 }
 
-// K1 + K2 in one launch: Gp pair blocks spread evenly among Gr rho_k blocks, so
-// the VALU-bound pair work and the HBM-bound rho_k streams run side by side on
-// every CU.  Logical block i is a pair block when floor((i+1) Gp / G) > floor(i Gp / G).
-template <int KPT>
-__global__ __launch_bounds__(kRhoBlock) void k_eval(const Rep* __restrict__ reps, const uint2* __restrict__ pitems,
-                                                    int n_pitems, const uint2* __restrict__ ritems, int n_ritems) {
-    const long long G = gridDim.x, Gp = (n_pitems + kPairBlock / kWave - 1) / (kPairBlock / kWave);
-    const long long i = xcd_remap(blockIdx.x, (int)G);
-    const long long a = i * Gp / G, b = (i + 1) * Gp / G;
-    if (b > a)
-        pair_block(reps, pitems, n_pitems, 0, (int)a);
-    else
-        rho_block<KPT>(reps, ritems, n_ritems, 0, (int)(i - a));
+// K2 with a fixed number of workgroups that loop over the work items: the
+// launch then holds at most that many workgroups per CU, so a concurrent pair
+// kernel on another stream always finds room on every CU.
+// Each item is one descriptor (scalar loads, prefetched while the previous
+// item computes); its rho_k, k-table and phase-chain loads are all issued
+// before the first wait.
+template <int KPT, bool NT>
+__global__ __launch_bounds__(kRhoBlock) void k_rho_loop(const RhoItem* __restrict__ items, int n_items,
+                                                        unsigned long long* __restrict__ kt) {
+    kt_begin(kt, KT_RHO);
+    __shared__ double2 tab[6 * (kMaxIdx + 1)];
+    __shared__ double red[2][kRhoBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    int it = blockIdx.x;
+    RhoItem d;
+    if (it < n_items) d = items[it];
+    for (; it < n_items; it += gridDim.x) {
+        RhoItem nd;
+        const int nx = it + gridDim.x;
+        if (nx < n_items) nd = items[nx];
+        if (d.valid) {
+            const int k0 = d.k0 + threadIdx.x;
+            double2 rv[KPT];
+            int kp[KPT];
+            double cf[KPT];
+#pragma unroll
+            for (int i = 0; i < KPT; ++i) {
+                const int k = k0 + i * kRhoBlock;
+                const bool ok = k < d.k1;
+                rv[i] = ok ? ld2<NT>(d.rc + k) : make_double2(0., 0.);
+                kp[i] = ok ? gld(d.kidx + k) : (128 | (128 << 8) | (128 << 16));
+                cf[i] = ok ? gld(d.coef + k) : 0.;
+            }
+            const int nt = 6 * d.stride;
+            const double2 t0 = threadIdx.x < nt ? ld2<false>(d.ph + threadIdx.x) : make_double2(0., 0.);
+            const double2 t1 =
+                threadIdx.x + kRhoBlock < nt ? ld2<false>(d.ph + threadIdx.x + kRhoBlock) : make_double2(0., 0.);
+            if (threadIdx.x < nt) tab[threadIdx.x] = t0;
+            if (threadIdx.x + kRhoBlock < nt) tab[threadIdx.x + kRhoBlock] = t1;
+            __syncthreads();
+            double ao = 0., an = 0.;
+#pragma unroll
+            for (int i = 0; i < KPT; ++i) {
+                const int k = k0 + i * kRhoBlock;
+                const double2 en = phase_k(&tab[3 * d.stride], d.stride, kp[i]);
+                const double2 eo = phase_k(tab, d.stride, kp[i]);
+                double2 nv;
+                nv.x = rv[i].x + (en.x - eo.x);
+                nv.y = rv[i].y + (en.y - eo.y);
+                if (k < d.k1) st2<NT>(d.rn + k, nv);
+                ao += cf[i] * (rv[i].x * rv[i].x + rv[i].y * rv[i].y);
+                an += cf[i] * (nv.x * nv.x + nv.y * nv.y);
+            }
+            ao = wave_sum(ao);
+            an = wave_sum(an);
+            if (lane == 0) {
+                red[0][w] = ao;
+                red[1][w] = an;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double to = 0., tn = 0.;
+                for (int i = 0; i < kRhoBlock / kWave; ++i) {
+                    to += red[0][i];
+                    tn += red[1][i];
+                }
+                *d.part = make_double2(to, tn);
+            }
+            __syncthreads();  // tab and red are reused by the next item
+        }
+        d = nd;
+    }
+    kt_end(kt, KT_RHO);
 }
 
 // ---------------------------------------------------------------------------
@@ -661,7 +777,7 @@ __global__ __launch_bounds__(kRhoBlock) void k_eval(const Rep* __restrict__ reps
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int n_reps,
                                                unsigned long long* __restrict__ kt) {
-    if (kt && blockIdx.x == 0 && threadIdx.x == 0) kt_close(kt);
+    if (kt && blockIdx.x == 0) kt_close(kt);
     const int r = blockIdx.x;
     if (r >= n_reps) return;
     const Rep& R = reps[r];

@@ -214,11 +214,14 @@ def test_virial_adjoint_estimator(sysname):
     assert rel_err(ge, re) < 1e-6
 
 
-@pytest.mark.parametrize("streams", ["1", "3"])
-def test_batched_replicas_match_single_chains(streams, monkeypatch):
-    """Replicas advanced together (on one stream, or split into groups on three
-    streams) replay each single chain's tape bit for bit."""
+@pytest.mark.parametrize("streams,fork", [("1", "0"), ("3", "0"), ("2", "1")])
+def test_batched_replicas_match_single_chains(streams, fork, monkeypatch):
+    """Replicas advanced together (on one stream, split into groups on three
+    streams, or two groups whose rho_k kernels run on a second stream beside the
+    pair kernels) replay each single chain's tape bit for bit."""
     monkeypatch.setenv("SPIMC_STREAMS", streams)
+    monkeypatch.setenv("SPIMC_FORK", fork)
+    monkeypatch.setenv("SPIMC_RHO_BPC", "3" if fork == "1" else "0")
     systems = [system.config_c1(n_part=n, n_bead=m, r_s=rs) for n, m, rs in [(16, 16, 1.5), (32, 32, 2.0), (8, 16, 3.0), (24, 8, 1.2)]]
     tapes, refs, finals = [], [], []
     for i, s in enumerate(systems):
@@ -241,11 +244,12 @@ def test_batched_replicas_match_single_chains(streams, monkeypatch):
             assert np.array_equal(e.download(r=i), finals[i])
 
 
-@pytest.mark.parametrize("streams", ["1", "2"])
-def test_uneven_counts_and_graph_sweep(streams, monkeypatch):
+@pytest.mark.parametrize("streams,fork", [("1", "0"), ("2", "0"), ("2", "1")])
+def test_uneven_counts_and_graph_sweep(streams, fork, monkeypatch):
     systems = system.config_c4(seed=3, n_replicas=6)
     with abi.Engine(systems, seed=5) as e:
         monkeypatch.setenv("SPIMC_STREAMS", streams)
+        monkeypatch.setenv("SPIMC_FORK", fork)
         g = abi.Engine(systems, seed=5)
         for i, s in enumerate(systems):
             pos = system.random_path(s, seed=i, spread=0.05)

@@ -9,8 +9,9 @@ from simpimc_amd import abi, system
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_clock_spans_per_round(graph):
+@pytest.mark.parametrize("graph,fork", [(False, "0"), (True, "0"), (False, "1"), (True, "1")])
+def test_clock_spans_per_round(graph, fork, monkeypatch):
+    monkeypatch.setenv("SPIMC_FORK", fork)
     s = system.config_c1()
     pos = system.random_path(s, seed=4, spread=0.05)
     n_b = (s.n_part * s.n_bead) >> s.bisect_n_level
