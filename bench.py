@@ -39,25 +39,41 @@ def parse():
     ap.add_argument("--kernel-events", action="store_true",
                     help="time every launch with dispatch events instead of device clocks (slows the run; "
                          "gives the per-kernel breakdown)")
-    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="first run one device-to-device hipMemcpy of this size (a copy kernel of known bytes "
-                         "for scaling rocprof FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--copy-mib", type=int, default=4096,
+                    help="size of the device-to-device copy timed first (the practical HBM ceiling for a "
+                         "read-one-write-one stream, reported as hbm_copy_GBps; 0 skips it)")
     return ap.parse_args()
 
 
-def copy_calibration(mib: int) -> None:
+def copy_ceiling(mib: int) -> float:
+    """GB/s (read + write bytes) of a device-to-device hipMemcpy of `mib` MiB, best
+    of three, timed with HIP events: the practical HBM rate of a stream that
+    reads one array and writes another, as the rho_k update does."""
     import ctypes as C
 
     hip = C.CDLL("libamdhip64.so")
     n = mib << 20
     a, b = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
     assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
     assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
-    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
-    assert hip.hipDeviceSynchronize() == 0
+    best = float("inf")
+    for _ in range(3):
+        assert hip.hipEventRecord(e0, None) == 0
+        assert hip.hipMemcpyAsync(b, a, C.c_size_t(n), 3, None) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+        best = min(best, ms.value)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
     hip.hipFree(a)
     hip.hipFree(b)
-    print(f"copy calibration: {mib} MiB device-to-device", file=sys.stderr, flush=True)
+    gbps = 2.0 * n / (best * 1e-3) / 1e9
+    print(f"device copy of {mib} MiB: {gbps:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
+    return gbps
 
 
 def cpu_baseline(target_s: float):
@@ -101,8 +117,7 @@ def main():
         sys.exit(2)
     from simpimc_amd import abi, system
 
-    if a.calibrate_copy_mib:
-        copy_calibration(a.calibrate_copy_mib)
+    copy_gbps = copy_ceiling(a.copy_mib) if a.copy_mib else None
     base = system.config_c2()
     R = a.replicas
     systems = [base] * R
@@ -152,7 +167,8 @@ def main():
                     "bytes_per_launch": rho_bytes / rho_n, "avg_launch_us": 1e3 * rho_ms / rho_n,
                     "duration_from": ("dispatch events" if a.kernel_events else
                                       "device wall clock, first block start to last block end, every launch "
-                                      "of the timed region")}
+                                      "of the timed region"),
+                    "hbm_copy_GBps": copy_gbps, "frac_of_copy": (ach / copy_gbps) if copy_gbps else None}
     c0 = eng.counters(0)
     eng.close()
 
@@ -200,6 +216,7 @@ def main():
         "plain_single_chain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "rho_k_update_GBps": roofline["achieved"] if roofline else None,
+        "hbm_copy_GBps": copy_gbps,
         "kernels": kernels,
         "acceptance_replica0": {"bisect": float(c0[1]) / max(1, c0[0]), "displace": float(c0[3]) / max(1, c0[2])},
     }
