@@ -279,10 +279,51 @@ __device__ __forceinline__ void link_geom(const double* a0, const double* c0, co
     g.s = sqrt(g.sv[0] * g.sv[0] + g.sv[1] * g.sv[1] + g.sv[2] * g.sv[2]);
 }
 
+// The same link geometry for the estimator kernels (compared at 1e-6 and
+// 1e-10): contraction allowed, s without its identity minimum image, lengths by
+// sqrt_pos.
+__device__ __forceinline__ double sqrt_pos(double x);
+__device__ __forceinline__ void link_geom_fast(const double* a0, const double* c0, const double* a1, const double* c1,
+                                               double L, double iL, LinkGeom& g) {
+    double rr = 0., pp = 0., ss = 0.;
+    for (int d = 0; d < 3; ++d) {
+        double rv = c0[d] - a0[d];
+        double rpv = c1[d] - a1[d];
+        rv -= rint(rv * iL) * L;
+        rpv += rint((rv - rpv) * iL) * L;
+        const double sv = rv - rpv;
+        g.rv[d] = rv;
+        g.rpv[d] = rpv;
+        g.sv[d] = sv;
+        rr += rv * rv;
+        pp += rpv * rpv;
+        ss += sv * sv;
+    }
+    g.r = sqrt_pos(rr);
+    g.rp = sqrt_pos(pp);
+    g.s = sqrt_pos(ss);
+}
+
+// Square root of a positive argument well inside the normal range (squared
+// link lengths): v_rsq_f64, one Goldschmidt step and one Newton correction,
+// without the range scaling of the general sqrt.  0 gives 0.
+__device__ __forceinline__ double sqrt_pos(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    const double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return x > 0. ? g : 0.;
+}
+
 // DrDrpDrrp magnitudes |r|, |r'| and the squared |s|^2 (the link action needs
-// only even powers of s).  Contraction is allowed here: the image choices are
-// the same rint() of the same products, the lengths differ from the
-// uncontracted form in the last bits only.
+// only even powers of s).  r' is shifted to the image nearest r, so s = r - r'
+// already lies in [-L/2, L/2] and its own minimum image is the identity (up to
+// the sign at exactly L/2, which leaves |s| unchanged).  Contraction is allowed:
+// the image choices are the same rint() of the same products, the lengths
+// differ from the uncontracted form in the last bits only.
 __device__ __forceinline__ void link_rr_s2(const double* a0, const double* c0, const double* a1, const double* c1,
                                            double L, double iL, double& r, double& rp, double& s2) {
     double rr = 0., pp = 0., ss = 0.;
@@ -291,14 +332,13 @@ __device__ __forceinline__ void link_rr_s2(const double* a0, const double* c0, c
         double rpv = c1[d] - a1[d];
         rv -= rint(rv * iL) * L;
         rpv += rint((rv - rpv) * iL) * L;
-        double sv = rv - rpv;
-        sv -= rint(sv * iL) * L;
+        const double sv = rv - rpv;
         rr += rv * rv;
         pp += rpv * rpv;
         ss += sv * sv;
     }
-    r = sqrt(rr);
-    rp = sqrt(pp);
+    r = sqrt_pos(rr);
+    rp = sqrt_pos(pp);
     s2 = ss;
 }
 

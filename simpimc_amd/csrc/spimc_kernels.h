@@ -1239,7 +1239,7 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_local_lds(const Rep* _
                     }
                     LinkGeom gg;
                     double ur, urp, us;
-                    link_geom(rp_, jp, ri_, j0, R.L, R.iL, gg);
+                    link_geom_fast(rp_, jp, ri_, j0, R.L, R.iL, gg);
                     pair_u_grad_lds<PT, NORD>(R, lds, gg.r, gg.rp, gg.s, ur, urp, us);
                     const double frp = urp / gg.rp, so = (gg.s > 0.) ? us / gg.s : 0.;
                     for (int d = 0; d < 3; ++d) carry[d] -= frp * gg.rpv[d] - so * gg.sv[d];
@@ -1276,7 +1276,7 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_local_lds(const Rep* _
                     }
                     LinkGeom gg;
                     double ur, urp, us;
-                    link_geom(ri, j0, rn, jn, R.L, R.iL, gg);
+                    link_geom_fast(ri, j0, rn, jn, R.L, R.iL, gg);
                     pair_u_grad_lds<PT, NORD>(R, lds, gg.r, gg.rp, gg.s, ur, urp, us);
                     const double fr = ur / gg.r, frp = urp / gg.rp, so = (gg.s > 0.) ? us / gg.s : 0.;
                     for (int d = 0; d < 3; ++d) {
@@ -1329,9 +1329,9 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_pair_whole_lds(const Rep* __re
                 if (which == 0) {
                     LinkGeom g;
                     if (i < j)
-                        link_geom(a0, c0, a1, c1, R.L, R.iL, g);
+                        link_geom_fast(a0, c0, a1, c1, R.L, R.iL, g);
                     else
-                        link_geom(c0, a0, c1, a1, R.L, R.iL, g);
+                        link_geom_fast(c0, a0, c1, a1, R.L, R.iL, g);
                     t += pair_du_dbeta_lds(R, lds, ldu, g.r, g.rp, g.s);
                 } else {
                     double v0[3], v1[3];
