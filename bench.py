@@ -33,6 +33,8 @@ def parse():
                     "faster here; rocprofv3 kernel traces of graph replays are not taken, so the default is eager)")
     ap.add_argument("--plain-attempts", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the engine's default launch shape instead of autotuning it before the warm-up")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no kernel clocks or events (for counter-collection runs)")
@@ -128,6 +130,10 @@ def main():
     for r in range(R):
         eng.upload(system.random_path(base, seed=1000 + r, spread=0.05), r=r)
     print(f"{R} replicas uploaded", file=sys.stderr, flush=True)
+    # launch shape: tried on short bisection stages (ordinary, untimed moves)
+    shape = eng.autotune(32) if a.autotune else None
+    if shape:
+        print(f"autotuned launch shape: {shape}", file=sys.stderr, flush=True)
     kc = 0.95 * float(system.kmag(base.L, base.k_n).max())
     eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, sk_k_cut=kc, gr_n_r=200)
     # the timing mode is set before the warm-up, so the graphs it captures are the timed ones
@@ -209,6 +215,7 @@ def main():
                          f"Ewald with {base.n_k} stored k-vectors, bisect level 4 + displace; "
                          f"{R} replicas batched, adjoint dS/dr and estimators every sweep"),
             "replicas": R,
+            "launch_shape": shape,
             "graph": bool(a.graph) and not a.kernel_events,
         },
         "roofline": roofline,

@@ -101,6 +101,7 @@ SIGNATURES = {
     "spimc_obs_reset": ([_c, _u32], _i32),
     "spimc_sweep": ([_c, _u32, _u32], _i32),
     "spimc_bead_updates": ([_c, _pu64], _i32),
+    "spimc_autotune": ([_c, _u32, C.POINTER(C.c_int32)], _i32),
     "spimc_timing_enable": ([_c, _i32], _i32),
     "spimc_kernel_timing": ([_c, _i32, _dp, _pu64, _dp], _i32),
     "spimc_timing_reset": ([_c], _i32),
@@ -294,6 +295,14 @@ class Engine:
 
     def sweep(self, n: int = 1, flags: int = 0):
         check(self.lib.spimc_sweep(self.h, n, flags))
+
+    def autotune(self, n_rounds: int = 32) -> dict:
+        """Try the launch shapes on n_rounds bisection attempts each (real moves of
+        the chains) and keep the fastest; returns the shape chosen."""
+        out = (C.c_int32 * 5)()
+        check(self.lib.spimc_autotune(self.h, n_rounds, out))
+        return {"fork": bool(out[0]), "rho_workgroups_per_cu": int(out[1]), "pair_workgroup": int(out[2]),
+                "shapes_tried": int(out[3]), "displace_rounds": int(out[4])}
 
     def bead_updates(self) -> int:
         v = C.c_uint64()

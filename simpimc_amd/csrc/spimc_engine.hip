@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -320,6 +321,7 @@ struct spimc_ctx {
     // the LDS pair kernel specialised for the replicas' pair type (mixed types: generic)
     void (*pair_lds_fn)(const Rep*, int, int, unsigned long long*) = nullptr;
     int pair_block = kPairLdsBlock;
+    int pair_variant = 3;  // 0 Bare, 1 David order 2, 2 David other order, 3 mixed
     void (*adj_lds_fn)(const Rep*, unsigned int, int) = nullptr;
     int groups_wanted = 1;
     bool fork = false;  // rho_k beside the pair kernel on a second stream per group
@@ -865,12 +867,11 @@ int build_groups(spimc_ctx* c, int n_groups) {
         else if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess)
             return fail(SPIMC_ERR_HIP, "stream creation failed");
         if (hipEventCreateWithFlags(&G.done, hipEventDisableTiming) != hipSuccess) return fail(SPIMC_ERR_HIP, "event");
-        if (c->fork) {
-            if (hipStreamCreateWithFlags(&G.side, hipStreamNonBlocking) != hipSuccess ||
-                hipEventCreateWithFlags(&G.ev_fork, hipEventDisableTiming) != hipSuccess ||
-                hipEventCreateWithFlags(&G.ev_join, hipEventDisableTiming) != hipSuccess)
-                return fail(SPIMC_ERR_HIP, "fork stream");
-        }
+        // the side stream exists whether or not the fork is on (the autotuner may turn it on)
+        if (hipStreamCreateWithFlags(&G.side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&G.ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&G.ev_join, hipEventDisableTiming) != hipSuccess)
+            return fail(SPIMC_ERR_HIP, "fork stream");
         HIPCHK(hipMalloc((void**)&G.d_kt, KT_WORDS * sizeof(unsigned long long)));
         HIPCHK(hipMemset(G.d_kt, 0, KT_WORDS * sizeof(unsigned long long)));
         int rc = build_work_lists(c, G);
@@ -897,6 +898,35 @@ int ensure_tmp(spimc_ctx* c, size_t n_doubles, size_t n_items) {
 }
 
 // One round of a move stage for every replica.
+// Selects the LDS pair kernel of workgroup size bs (256, 512, 768 or 1024).
+int set_pair_block(spimc_ctx* c, int bs) {
+    using PairFn = void (*)(const Rep*, int, int, unsigned long long*);
+    const PairFn fns[4][4] = {
+        {k_pair_lds<1, 0, 256>, k_pair_lds<2, 2, 256>, k_pair_lds<2, 0, 256>, k_pair_lds<0, 0, 256>},
+        {k_pair_lds<1, 0, 512>, k_pair_lds<2, 2, 512>, k_pair_lds<2, 0, 512>, k_pair_lds<0, 0, 512>},
+        {k_pair_lds<1, 0, 768>, k_pair_lds<2, 2, 768>, k_pair_lds<2, 0, 768>, k_pair_lds<0, 0, 768>},
+        {k_pair_lds<1, 0, 1024>, k_pair_lds<2, 2, 1024>, k_pair_lds<2, 0, 1024>, k_pair_lds<0, 0, 1024>}};
+    const int bi = bs == 256 ? 0 : bs == 512 ? 1 : bs == 768 ? 2 : bs == 1024 ? 3 : -1;
+    if (bi < 0) return fail(SPIMC_ERR_ARG, "pair workgroup size must be 256, 512, 768 or 1024");
+    c->pair_block = bs;
+    c->pair_lds_fn = fns[bi][c->pair_variant];
+    if (c->lds_doubles_max > 0 &&
+        hipFuncSetAttribute((const void*)c->pair_lds_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            c->lds_doubles_max * (int)sizeof(double)) != hipSuccess)
+        return fail(SPIMC_ERR_HIP, "pair kernel LDS size");
+    return SPIMC_OK;
+}
+
+// Drops the captured round graphs (the launch shape changed).
+void drop_graphs(spimc_ctx* c) {
+    for (auto& G : c->groups)
+        for (auto& g : G.graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+}
+
 // The rho_k kernel of a round: rho_bpc workgroups per CU looping over the items
 // (0: one workgroup per item).
 int launch_rho(spimc_ctx* c, hipStream_t st, const RhoItem* ri, int nr, unsigned long long* kt) {
@@ -921,7 +951,7 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
     const int np = move == 0 ? G.n_pair_bis : G.n_pair_dis;
     const RhoItem* ri = move == 0 ? G.d_ritem_bis : G.d_ritem_dis;
     const int nr = move == 0 ? G.n_ritem_bis : G.n_ritem_dis;
-    if (c->lds_doubles_max > 0 && np && G.side && nr) {
+    if (c->fork && c->lds_doubles_max > 0 && np && G.side && nr) {
         // fork: rho_k on the side stream beside the pair kernel, joined before the decision
         HIPCHK(hipEventRecord(G.ev_fork, st));
         HIPCHK(hipStreamWaitEvent(G.side, G.ev_fork, 0));
@@ -1306,20 +1336,9 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
             const int v = std::atoi(fb);
             if (v == 256 || v == 512 || v == 768 || v == 1024) bs = v;
         }
-        c->pair_block = bs;
-        using PairFn = void (*)(const Rep*, int, int, unsigned long long*);
-        const PairFn fns[4][4] = {
-            {k_pair_lds<1, 0, 256>, k_pair_lds<2, 2, 256>, k_pair_lds<2, 0, 256>, k_pair_lds<0, 0, 256>},
-            {k_pair_lds<1, 0, 512>, k_pair_lds<2, 2, 512>, k_pair_lds<2, 0, 512>, k_pair_lds<0, 0, 512>},
-            {k_pair_lds<1, 0, 768>, k_pair_lds<2, 2, 768>, k_pair_lds<2, 0, 768>, k_pair_lds<0, 0, 768>},
-            {k_pair_lds<1, 0, 1024>, k_pair_lds<2, 2, 1024>, k_pair_lds<2, 0, 1024>, k_pair_lds<0, 0, 1024>}};
-        const int bi = bs == 256 ? 0 : bs == 512 ? 1 : bs == 768 ? 2 : 3;
         const int vi = pt == SPIMC_PAIR_BARE ? 0 : pt == SPIMC_PAIR_DAVID ? (nord == 2 ? 1 : 2) : 3;
-        c->pair_lds_fn = fns[bi][vi];
-        if (c->lds_doubles_max > 0 &&
-            hipFuncSetAttribute((const void*)c->pair_lds_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                c->lds_doubles_max * (int)sizeof(double)) != hipSuccess)
-            c->lds_doubles_max = 0;
+        c->pair_variant = vi;
+        if (set_pair_block(c, bs) != SPIMC_OK) c->lds_doubles_max = 0;
         c->lds_adj_max = c->lds_doubles_max;
         c->adj_lds_fn = vi == 0 ? k_adjoint_local_lds<1, 0> : vi == 1 ? k_adjoint_local_lds<2, 2>
                                                         : vi == 2 ? k_adjoint_local_lds<2, 0> : k_adjoint_local_lds<0, 0>;
@@ -1824,6 +1843,65 @@ int spimc_sweep(spimc_ctx* c, uint32_t n_sweeps, uint32_t flags) {
         if (which && (rc = obs_accumulate_all(c, which))) return rc;
         if ((rc = sync_and_resolve(c))) return rc;
         c->adjoint_fresh = (flags & SPIMC_SWEEP_ADJOINT) != 0;
+    }
+    return SPIMC_OK;
+}
+
+int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    if (n_rounds == 0) return fail(SPIMC_ERR_ARG, "n_rounds must be positive");
+    if (c->prop_set) return fail(SPIMC_ERR_STATE, "an interface proposal is pending; accept or reject it first");
+    struct Shape {
+        int fork, bpc, bs;
+    };
+    // launch shapes tried: rho_k beside the pair kernel with 1-3 looping workgroups
+    // per CU and 256/512-thread pair workgroups, or the two kernels one after the other
+    std::vector<Shape> shapes = {{1, 2, 512}, {1, 3, 512}, {1, 1, 512}, {1, 2, 256}, {1, 3, 256}, {0, 0, 1024}, {0, 2, 512}};
+    if (!c->any_long || c->lds_doubles_max == 0) shapes = {{0, 0, 1024}, {0, 0, 512}, {0, 0, 256}};
+    const Shape saved{c->fork ? 1 : 0, c->rho_bpc, c->pair_block};
+    // the sweep's mix: N M / 2^l bisection attempts per N displace attempts
+    const Rep& R0 = c->reps[0].dev;
+    const uint32_t n_dis = std::max<uint32_t>(1, (uint32_t)((uint64_t)n_rounds * (1u << R0.n_level) / R0.M));
+    double best_t = 1e300;
+    Shape best = saved;
+    int tried = 0;
+    for (const Shape& sh : shapes) {
+        c->fork = sh.fork != 0;
+        c->rho_bpc = sh.bpc;
+        int rc = c->lds_doubles_max > 0 ? set_pair_block(c, sh.bs) : SPIMC_OK;
+        if (rc) continue;
+        drop_graphs(c);
+        // one untimed round of each move under the new shape, then the timed mix
+        if ((rc = run_stage(c, 0, nullptr, 1, nullptr, false)) == SPIMC_OK) rc = run_stage(c, 1, nullptr, 1, nullptr, false);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (rc == SPIMC_OK) rc = run_stage(c, 0, nullptr, n_rounds, nullptr, false);  // returns after the device work
+        if (rc == SPIMC_OK) rc = run_stage(c, 1, nullptr, n_dis, nullptr, false);
+        const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc) {
+            c->fork = saved.fork != 0;
+            c->rho_bpc = saved.bpc;
+            if (c->lds_doubles_max > 0) (void)set_pair_block(c, saved.bs);
+            return rc;
+        }
+        ++tried;
+        if (t < best_t) {
+            best_t = t;
+            best = sh;
+        }
+    }
+    c->fork = best.fork != 0;
+    c->rho_bpc = best.bpc;
+    if (c->lds_doubles_max > 0) {
+        int rc = set_pair_block(c, best.bs);
+        if (rc) return rc;
+    }
+    drop_graphs(c);
+    if (chosen) {
+        chosen[0] = best.fork;
+        chosen[1] = best.bpc;
+        chosen[2] = c->pair_block;
+        chosen[3] = tried;
+        chosen[4] = (int32_t)n_dis;
     }
     return SPIMC_OK;
 }

@@ -349,3 +349,33 @@ def test_global_table_kernels_match_lds_kernels(monkeypatch):
             eng.obs_configure(energy=abi.ENERGY_THERMAL)
             eng.obs_accumulate(abi.OBS_ENERGY)
         assert rel_err(e.obs_read(abi.OBS_ENERGY)[0], o.obs_read(abi.OBS_ENERGY)[0]) < RTOL
+
+
+def test_autotune_keeps_the_chain():
+    """The autotuner's stages are ordinary move stages under different launch
+    shapes: afterwards the chains equal those of an engine that ran the same
+    sequence of bisection and displace stages with its default shape, bit for
+    bit."""
+    s = system.config_c1()
+    n_rep, rounds = 64, 4
+    a = abi.Engine([s] * n_rep, seed=31)
+    b = abi.Engine([s] * n_rep, seed=31)
+    try:
+        for e in (a, b):
+            for r in range(n_rep):
+                e.upload(system.random_path(s, seed=700 + r, spread=0.2), r=r)
+        chosen = a.autotune(rounds)
+        assert chosen["shapes_tried"] >= 2 and chosen["pair_workgroup"] in (256, 512, 768, 1024)
+        for _ in range(chosen["shapes_tried"]):
+            b.bisect(1)
+            b.displace(1)
+            b.bisect(rounds)
+            b.displace(chosen["displace_rounds"])
+        for e in (a, b):
+            e.sweep(1, abi.SWEEP_GRAPH)
+        for r in (0, 17, n_rep - 1):
+            assert np.array_equal(a.download(r=r), b.download(r=r))
+            assert np.array_equal(a.counters(r=r), b.counters(r=r))
+    finally:
+        a.close()
+        b.close()
