@@ -704,7 +704,7 @@ struct Oracle {
             }
         return tot;
     }
-    double slice_long(uint32_t b, const cplx* rb) const {
+    double slice_long(uint32_t /* slice b, whose rho_k row is rb */, const cplx* rb) const {
         const std::vector<double>& coef = (pair_type == SPIMC_PAIR_BARE) ? v_long_k : u_long_k;
         double tot = 0.;
         for (uint32_t k = 0; k < nk; ++k) tot += coef[k] * cmag2(rb[k], rb[k]);

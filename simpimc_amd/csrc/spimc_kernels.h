@@ -729,12 +729,11 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_loop(const RhoItem* __restric
                 kp[i] = ok ? gld(d.kidx + k) : (128 | (128 << 8) | (128 << 16));
                 cf[i] = ok ? gld(d.coef + k) : 0.;
             }
-            const int nt = 6 * d.stride;
-            const double2 t0 = threadIdx.x < nt ? ld2<false>(d.ph + threadIdx.x) : make_double2(0., 0.);
-            const double2 t1 =
-                threadIdx.x + kRhoBlock < nt ? ld2<false>(d.ph + threadIdx.x + kRhoBlock) : make_double2(0., 0.);
-            if (threadIdx.x < nt) tab[threadIdx.x] = t0;
-            if (threadIdx.x + kRhoBlock < nt) tab[threadIdx.x + kRhoBlock] = t1;
+            const int nt = 6 * d.stride, tx = (int)threadIdx.x;
+            const double2 t0 = tx < nt ? ld2<false>(d.ph + tx) : make_double2(0., 0.);
+            const double2 t1 = tx + kRhoBlock < nt ? ld2<false>(d.ph + tx + kRhoBlock) : make_double2(0., 0.);
+            if (tx < nt) tab[tx] = t0;
+            if (tx + kRhoBlock < nt) tab[tx + kRhoBlock] = t1;
             __syncthreads();
             double ao = 0., an = 0.;
 #pragma unroll
@@ -1026,8 +1025,8 @@ __global__ __launch_bounds__(256) void k_pair_whole(const Rep* __restrict__ reps
 // sum_k coef_b(k) sum_b |rho(b,k)|^2 (CalcVLong / CalcdUdBetaLong sums), per k chunk
 __global__ __launch_bounds__(256) void k_long_whole(const Rep* __restrict__ reps) {
     const Rep& R = reps[blockIdx.y];
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (blockIdx.x * 256 >= R.nk) return;
+    const int k = (int)(blockIdx.x * 256 + threadIdx.x);
+    if ((int)blockIdx.x * 256 >= R.nk) return;
     double t = 0.;
     if (k < R.nk && R.use_long_range) {
         for (int b = 0; b < R.M; ++b) {
@@ -1363,7 +1362,7 @@ constexpr int kAdjTile = 128;
 __global__ __launch_bounds__(64) void k_adjoint_long(const Rep* __restrict__ reps, int out_est) {
     const Rep& R = reps[blockIdx.z];
     const int b = blockIdx.y;
-    if (b >= R.M || blockIdx.x * 64 >= R.N || !R.use_long_range || R.pair_type == 0) return;
+    if (b >= R.M || (int)blockIdx.x * 64 >= R.N || !R.use_long_range || R.pair_type == 0) return;
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int mx = R.max_idx, stride = mx + 1;
     extern __shared__ double2 dyn[];
