@@ -714,56 +714,53 @@ struct Oracle {
     // Central-difference gradient of the whole-path action terms for every bead,
     // using local re-evaluation of the terms a bead enters (test reference for
     // the adjoint; it restates no reference routine).
+    // Central difference of the terms bead (b, p) enters, in mode NEW.
+    void fd_bead(uint32_t terms, double h, uint32_t b, uint32_t p, double* g) {
+        std::vector<cplx> ph0, ph1, rb(nk);
+        double* pos = R(p, b);
+        for (int d = 0; d < 3; ++d) {
+            double x0 = pos[d], f[2];
+            for (int sgn = 0; sgn < 2; ++sgn) {
+                pos[d] = x0 + (sgn ? -h : h);
+                double s = 0.;
+                uint32_t bm = (b + M - 1) % M;
+                if (terms & SPIMC_GRAD_KINETIC) {
+                    double dd[3];
+                    dr(R(p, b), R(p, b + 1), dd);
+                    s -= kin_splines[0].log_rho_free(dd);
+                    dr(R(p, bm), R(p, b), dd);
+                    s -= kin_splines[0].log_rho_free(dd);
+                }
+                if ((terms & SPIMC_GRAD_PAIR_SHORT) && pair_type != SPIMC_PAIR_NONE) {
+                    for (uint32_t q = 0; q < N; ++q) {
+                        if (q == p) continue;
+                        uint32_t lo = std::min(p, q), hi = std::max(p, q);
+                        double rm, rpm, rrpm;
+                        dr_drp_drrp(b, b + 1, lo, hi, rm, rpm, rrpm);
+                        s += calc_u(rm, rpm, rrpm, 0);
+                        dr_drp_drrp(bm, bm + 1, lo, hi, rm, rpm, rrpm);
+                        s += calc_u(rm, rpm, rrpm, 0);
+                    }
+                }
+                if ((terms & SPIMC_GRAD_PAIR_LONG) && use_long_range && pair_type != SPIMC_PAIR_NONE) {
+                    pos[d] = x0;
+                    calc_rho_k(pos, ph0);
+                    pos[d] = x0 + (sgn ? -h : h);
+                    calc_rho_k(pos, ph1);
+                    for (uint32_t k = 0; k < nk; ++k) rb[k] = rho[(size_t)b * nk + k] + (ph1[k] - ph0[k]);
+                    s += slice_long(b, rb.data());
+                }
+                f[sgn] = s;
+            }
+            pos[d] = x0;
+            g[d] = (f[0] - f[1]) / (2. * h);
+        }
+    }
     void gradient_fd(uint32_t terms, double h, double* out) {
         int saved = mode;
         mode = SPIMC_MODE_NEW;
-        std::vector<cplx> ph0, ph1, rb(nk);
         for (uint32_t b = 0; b < M; ++b)
-            for (uint32_t p = 0; p < N; ++p) {
-                double* pos = R(p, b);
-                double g[3] = {0., 0., 0.};
-                for (int d = 0; d < 3; ++d) {
-                    double x0 = pos[d], f[2];
-                    for (int sgn = 0; sgn < 2; ++sgn) {
-                        pos[d] = x0 + (sgn ? -h : h);
-                        double s = 0.;
-                        uint32_t bm = (b + M - 1) % M;
-                        if (terms & SPIMC_GRAD_KINETIC) {
-                            double dd[3];
-                            dr(R(p, b), R(p, b + 1), dd);
-                            s -= kin_splines[0].log_rho_free(dd);
-                            dr(R(p, bm), R(p, b), dd);
-                            s -= kin_splines[0].log_rho_free(dd);
-                        }
-                        if ((terms & SPIMC_GRAD_PAIR_SHORT) && pair_type != SPIMC_PAIR_NONE) {
-                            for (uint32_t q = 0; q < N; ++q) {
-                                if (q == p) continue;
-                                uint32_t lo = std::min(p, q), hi = std::max(p, q);
-                                double rm, rpm, rrpm;
-                                dr_drp_drrp(b, b + 1, lo, hi, rm, rpm, rrpm);
-                                s += calc_u(rm, rpm, rrpm, 0);
-                                dr_drp_drrp(bm, bm + 1, lo, hi, rm, rpm, rrpm);
-                                s += calc_u(rm, rpm, rrpm, 0);
-                            }
-                        }
-                        if ((terms & SPIMC_GRAD_PAIR_LONG) && use_long_range && pair_type != SPIMC_PAIR_NONE) {
-                            pos[d] = x0;
-                            calc_rho_k(pos, ph0);
-                            pos[d] = x0 + (sgn ? -h : h);
-                            calc_rho_k(pos, ph1);
-                            for (uint32_t k = 0; k < nk; ++k) rb[k] = rho[(size_t)b * nk + k] + (ph1[k] - ph0[k]);
-                            s += slice_long(b, rb.data());
-                        }
-                        f[sgn] = s;
-                    }
-                    pos[d] = x0;
-                    g[d] = (f[0] - f[1]) / (2. * h);
-                }
-                double* o = out + 3 * ((size_t)b * N + p);
-                o[0] = g[0];
-                o[1] = g[1];
-                o[2] = g[2];
-            }
+            for (uint32_t p = 0; p < N; ++p) fd_bead(terms, h, b, p, out + 3 * ((size_t)b * N + p));
         mode = saved;
     }
 
@@ -1274,6 +1271,28 @@ int spimc_oracle_action_gradient(void* h, uint32_t terms, int compat, double fd_
         return SPIMC_OK;
     }
     o->gradient_fd(terms, fd_h, out);
+    return SPIMC_OK;
+}
+
+// The same central difference for a list of beads (slice, particle) only: out[n][3].
+int spimc_oracle_action_gradient_beads(void* h, uint32_t terms, double fd_h, uint32_t n, const uint32_t* beads,
+                                       double* out) {
+    Oracle* o = (Oracle*)h;
+    for (uint32_t i = 0; i < n; ++i)
+        if (beads[2 * i] >= o->M || beads[2 * i + 1] >= o->N) {
+            g_err = "bead out of range";
+            return SPIMC_ERR_ARG;
+        }
+    int saved = o->mode;
+    o->mode = SPIMC_MODE_NEW;
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        // each thread perturbs its own bead on a private copy of the path (the
+        // difference formulas read the neighbours' positions)
+        Oracle local = *o;
+        local.fd_bead(terms, fd_h, beads[2 * i], beads[2 * i + 1], out + 3 * i);
+    }
+    o->mode = saved;
     return SPIMC_OK;
 }
 

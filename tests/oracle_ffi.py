@@ -38,6 +38,7 @@ SIG = {
     "spimc_oracle_potential": ([_c, _i32, _dp], _i32),
     "spimc_oracle_kinetic_virial_energy": ([_c, _u32, _dp], _i32),
     "spimc_oracle_action_gradient": ([_c, _u32, _i32, _d, _dp], _i32),
+    "spimc_oracle_action_gradient_beads": ([_c, _u32, _d, _u32, C.POINTER(C.c_uint32), _dp], _i32),
     "spimc_oracle_action_totals": ([_c, _dp], _i32),
     "spimc_oracle_tape_set": ([_c, _dp, _u64], _i32),
     "spimc_oracle_tape_cursor": ([_c, _pu64], _i32),
@@ -172,6 +173,14 @@ class Oracle:
     def gradient(self, terms=abi.GRAD_ALL, compat=False, h=1e-5):
         out = np.zeros((self.s.n_bead, self.s.n_part, 3))
         check(self.L.spimc_oracle_action_gradient(self.h, terms, int(compat), h, dp(out)))
+        return out
+
+    def gradient_beads(self, beads, terms=abi.GRAD_ALL, h=1e-5):
+        """Central-difference dS/dr of the beads [(slice, particle), ...] only: [n][3]."""
+        b = np.ascontiguousarray(np.asarray(beads, dtype=np.uint32).reshape(-1, 2))
+        out = np.zeros((len(b), 3))
+        check(self.L.spimc_oracle_action_gradient_beads(self.h, terms, h, len(b),
+                                                         b.ctypes.data_as(C.POINTER(C.c_uint32)), dp(out)))
         return out
 
     def action_totals(self):
