@@ -162,16 +162,24 @@ struct Kinetic : Action {
 struct PairActionBase : Action {
     bool use_long_range = false;
     uint32_t n_order = 0;
+    std::string file, species_a, species_b;
+    double k_cut = 0.;
     TableFile table;
     std::vector<double> v_long_k;
     explicit PairActionBase(const Input& in, Path& path) : Action(in) {
         n_order = in.GetAttribute<uint32_t>("n_order", 0);
-        if (in.GetAttribute<std::string>("species_a") != path.species || in.GetAttribute<std::string>("species_b") != path.species)
+        species_a = in.GetAttribute<std::string>("species_a");
+        species_b = in.GetAttribute<std::string>("species_b");
+        if (species_a != path.species || species_b != path.species)
             throw std::runtime_error("pair action " + name + ": only the homologous pair of the one species is supported");
         use_long_range = in.GetAttribute<bool>("use_long_range", false);
-        if (use_long_range) path.SetupK(in.GetAttribute<double>("k_cut", path.k_cut));
+        if (use_long_range) {
+            k_cut = in.GetAttribute<double>("k_cut", path.k_cut);
+            path.SetupK(k_cut);
+        }
         if (max_level != 0) throw std::runtime_error("pair action " + name + ": max_level must be 0");
-        table.Load(in.GetAttribute<std::string>("file"));
+        file = in.GetAttribute<std::string>("file");
+        table.Load(file);
     }
     int Selector() const override { return SPIMC_ACTION_PAIR; }
     // per stored k: the table entry whose |k| equals it within 1e-8 (bare_pair_action_class.h:79-85)
@@ -245,8 +253,10 @@ struct DavidPairAction : PairActionBase {
         uint32_t n_val = 1;
         for (uint32_t i = 1; i <= n_order; ++i) n_val += 1 + i;
         const auto& V = table.Read("potential/data");
-        const auto& u = table.Read(pre + "/data");
-        const auto& du = table.Read(dpre + "/data");
+        // the cubes span every tau the file holds; tau is their slowest index
+        const std::vector<double> u = table.Cube(pre + "/data", n_val, n_grid, taus.size());
+        const std::vector<double> du = table.Cube(dpre + "/data", n_val, n_grid, taus.size());
+        if (V.size() < n_grid) throw std::runtime_error("potential/data holds fewer than n_grid_points values");
         // component 0 = potential; knots 0 .. n_grid-2 filled, the last one left 0
         u_kj.assign((size_t)(n_val + 1) * n_grid, 0.);
         du_kj.assign((size_t)(n_val + 1) * n_grid, 0.);
@@ -302,7 +312,6 @@ struct Event {
     double time_spent = 0.;
     virtual ~Event() = default;
     virtual void DoEvent() = 0;
-    virtual void Write(std::ostream&, uint32_t /*block*/) {}
     // A run of n consecutive DoEvent calls; moves hand them to the device at once.
     virtual void DoEvents(uint32_t n) {
         for (uint32_t i = 0; i < n; ++i) DoEvent();
@@ -327,22 +336,22 @@ struct Move : Event {
         check(Stage(n), name.c_str());
         time_spent += Now() - t;
     }
-    // Move::Write (move_class.h:80-94): n_attempt, n_accept, ratio since the last write
-    void Write(std::ostream& os, uint32_t) override {
-        os << "\"" << name << "\": [";
+    // Move::Write (move_class.h:80-94): per replica, attempts and acceptances
+    // since the last write
+    struct Counts {
+        uint64_t n_attempt, n_accept;
+    };
+    std::vector<Counts> Block() {
+        std::vector<Counts> out(eng.n_replicas);
         last.resize(2 * eng.n_replicas, 0);
         for (uint32_t r = 0; r < eng.n_replicas; ++r) {
             uint64_t c[4];
             check(spimc_move_counters(eng.ctx, r, c), "counters");
-            const uint64_t att = c[CounterBase()] - last[2 * r], acc = c[CounterBase() + 1] - last[2 * r + 1];
+            out[r] = {c[CounterBase()] - last[2 * r], c[CounterBase() + 1] - last[2 * r + 1]};
             last[2 * r] = c[CounterBase()];
             last[2 * r + 1] = c[CounterBase() + 1];
-            char b[160];
-            snprintf(b, sizeof b, "%s{\"n_attempt\": %llu, \"n_accept\": %llu, \"x\": %.10g}", r ? ", " : "",
-                     (unsigned long long)att, (unsigned long long)acc, att ? (double)acc / att : 0.);
-            os << b;
         }
-        os << "]";
+        return out;
     }
 };
 
@@ -396,28 +405,33 @@ struct Energy : Observable {
         if (in.GetAttribute<bool>("measure_per_sector", false) || in.GetAttribute<bool>("measure_potential", false))
             throw std::runtime_error("Energy: measure_per_sector/measure_potential are outside this engine");
     }
-    void Write(std::ostream& os, uint32_t n_bead) override {
-        os << "\"" << name << "\": [";
+    // per replica: the kinetic and pair energies per bead and measurement
+    // since the last write, and the number of measurements; resets the sums
+    struct Energies {
+        double kinetic, pair;
+        uint64_t n_measure;
+    };
+    std::vector<Energies> Block(uint32_t n_bead) {
+        std::vector<Energies> out(eng.n_replicas);
         for (uint32_t r = 0; r < eng.n_replicas; ++r) {
             double e[2];
             uint64_t nm = 0;
             check(spimc_obs_read(eng.ctx, r, SPIMC_OBS_ENERGY, e, &nm), "energy");
             const double norm = (double)n_bead * (nm ? nm : 1);
-            char b[200];
-            snprintf(b, sizeof b, "%s{\"total\": %.15g, \"%s\": %.15g, \"%s\": %.15g, \"n_measure\": %llu}", r ? ", " : "",
-                     (e[0] + e[1]) / norm, action_names[0].c_str(), e[0] / norm, action_names[1].c_str(), e[1] / norm,
-                     (unsigned long long)nm);
-            os << b;
+            out[r] = {e[0] / norm, e[1] / norm, nm};
         }
-        os << "]";
         check(spimc_obs_reset(eng.ctx, SPIMC_OBS_ENERGY), "reset");
+        return out;
     }
 };
 
 struct StructureFactor : Observable {
     double k_cut;
+    std::string species_a, species_b;
     StructureFactor(const Input& in, Engine& e, Path& path) : Observable(in, e) {
         which = SPIMC_OBS_SK;
+        species_a = in.GetAttribute<std::string>("species_a", path.species);
+        species_b = in.GetAttribute<std::string>("species_b", path.species);
         k_cut = in.GetAttribute<double>("k_cut", path.k_cut);
         path.SetupK(k_cut);
     }
@@ -426,8 +440,11 @@ struct StructureFactor : Observable {
 struct PairCorrelation : Observable {
     double r_min, r_max;
     uint32_t n_r;
+    std::string species_a, species_b;
     PairCorrelation(const Input& in, Engine& e, const Path& path) : Observable(in, e) {
         which = SPIMC_OBS_GR;
+        species_a = in.GetAttribute<std::string>("species_a", path.species);
+        species_b = in.GetAttribute<std::string>("species_b", path.species);
         r_min = in.GetAttribute<double>("r_min", 0.);
         r_max = in.GetAttribute<double>("r_max", path.L / 2.);
         n_r = (uint32_t)in.GetAttribute<double>("n_r", 100);

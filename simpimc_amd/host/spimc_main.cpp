@@ -4,7 +4,9 @@
 //
 // Replicas stand in for the reference's MPI clone groups: R independent chains
 // with seeds derived from <RNG seed>, advanced together on one GPU.  Block output
-// goes to <output_prefix>.spimc.jsonl (one JSON object per <Write/>).
+// goes to <output_prefix>.spimc.jsonl (one JSON object per <Write/>, all
+// replicas) and, when HDF5 is available, to <output_prefix>.<replica>.h5 in the
+// reference's layout (one file per clone group).
 #include <chrono>
 #include <cstring>
 #include <iomanip>
@@ -28,7 +30,7 @@ struct Simulation {
     uint32_t block = 0;
     double t_start = 0.;
 
-    Simulation(const std::string& file, uint32_t n_replicas)
+    Simulation(const std::string& file, uint32_t n_replicas, bool write_h5)
         : in(Input::Load(file)), path(in) {
         eng.n_replicas = n_replicas;
         std::memset(&desc, 0, sizeof desc);
@@ -113,6 +115,14 @@ struct Simulation {
         const std::string prefix = in.GetChild("IO").GetAttribute<std::string>("output_prefix");
         out.open(prefix + ".spimc.jsonl");
         out.precision(17);
+        if (write_h5) {
+#if SPIMC_HAVE_HDF5
+            OpenH5(prefix, file, seed);
+            energy_started_.assign(eng.n_replicas, false);
+#else
+            std::cerr << "WARNING: built without HDF5; block output goes to " << prefix << ".spimc.jsonl only" << std::endl;
+#endif
+        }
         ReadLoop(in.GetChild("Algorithm"), main_loop);
     }
 
@@ -157,7 +167,7 @@ struct Simulation {
         out << "{\"block\": " << block << ", \"Moves\": {";
         for (size_t i = 0; i < moves.size(); ++i) {
             if (i) out << ", ";
-            moves[i]->Write(out, path.n_bead);
+            WriteMove(*moves[i]);
         }
         out << "}, \"Observables\": {";
         bool first = true;
@@ -165,14 +175,17 @@ struct Simulation {
             if (!first) out << ", ";
             first = false;
             if (auto e = dynamic_cast<Energy*>(o.get()))
-                e->Write(out, path.n_bead);
-            else if (dynamic_cast<StructureFactor*>(o.get()))
-                WriteSk(*dynamic_cast<StructureFactor*>(o.get()));
-            else if (dynamic_cast<PairCorrelation*>(o.get()))
-                WriteGr(*dynamic_cast<PairCorrelation*>(o.get()));
+                WriteEnergy(*e);
+            else if (auto s = dynamic_cast<StructureFactor*>(o.get()))
+                WriteSk(*s);
+            else if (auto g = dynamic_cast<PairCorrelation*>(o.get()))
+                WriteGr(*g);
         }
         out << "}}\n";
         out.flush();
+#if SPIMC_HAVE_HDF5
+        for (auto& f : h5) f->Flush();
+#endif
         std::cout << "Clone #0, Block #" << block << ", Block Time: " << now - t_block << ", Total Time: " << now - t_start
                   << std::endl;
         t_block = now;
@@ -182,6 +195,57 @@ struct Simulation {
     double Now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     }
+
+    // Move::Write (move_class.h:80-94)
+    void WriteMove(Move& m) {
+        const auto b = m.Block();
+        out << "\"" << m.name << "\": [";
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) {
+            char s[160];
+            snprintf(s, sizeof s, "%s{\"n_attempt\": %llu, \"n_accept\": %llu, \"x\": %.10g}", r ? ", " : "",
+                     (unsigned long long)b[r].n_attempt, (unsigned long long)b[r].n_accept,
+                     b[r].n_attempt ? (double)b[r].n_accept / b[r].n_attempt : 0.);
+            out << s;
+        }
+        out << "]";
+#if SPIMC_HAVE_HDF5
+        for (uint32_t r = 0; r < h5.size(); ++r) {
+            const std::string g = "Moves/" + m.name + "/";
+            h5[r]->AppendInt(g + "n_attempt", (long long)b[r].n_attempt);
+            h5[r]->AppendInt(g + "n_accept", (long long)b[r].n_accept);
+            const double x = (1. * b[r].n_accept) / (1. * b[r].n_attempt);  // 0/0 is NaN, as written by the reference
+            h5[r]->Append(g + "x", &x, 0);
+        }
+#endif
+    }
+
+    // Energy::Write (energy_class.h:232-256), without sectors and potentials
+    void WriteEnergy(Energy& e) {
+        const auto b = e.Block(path.n_bead);
+        out << "\"" << e.name << "\": [";
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) {
+            char s[200];
+            snprintf(s, sizeof s, "%s{\"total\": %.15g, \"%s\": %.15g, \"%s\": %.15g, \"n_measure\": %llu}", r ? ", " : "",
+                     b[r].kinetic + b[r].pair, e.action_names[0].c_str(), b[r].kinetic, e.action_names[1].c_str(), b[r].pair,
+                     (unsigned long long)b[r].n_measure);
+            out << s;
+        }
+        out << "]";
+#if SPIMC_HAVE_HDF5
+        for (uint32_t r = 0; r < h5.size(); ++r) {
+            if (!b[r].n_measure) continue;
+            const std::string g = "Observables/" + e.name + "/";
+            const double vals[3] = {b[r].kinetic + b[r].pair, b[r].kinetic, b[r].pair};
+            const std::string names[3] = {"total", e.action_names[0], e.action_names[1]};
+            for (int i = 0; i < 3; ++i) {
+                if (!energy_started_[r]) h5[r]->String(g + names[i] + "/data_type", "scalar");
+                h5[r]->Append(g + names[i] + "/x", &vals[i], 0);
+            }
+            energy_started_[r] = true;
+        }
+#endif
+    }
+    std::vector<bool> energy_started_;
 
     // StructureFactor::Write (structure_factor_class.h:69-85)
     void WriteSk(const StructureFactor& s) {
@@ -193,24 +257,38 @@ struct Simulation {
             uint64_t nm = 0;
             check(spimc_obs_read(eng.ctx, r, SPIMC_OBS_SK, v.data(), &nm), "sk");
             const double norm = (double)(nm ? nm : 1) * path.n_bead * path.n_part * path.n_part;
+            for (uint32_t k = 0; k < path.n_k(); ++k) v[k] /= norm;
             out << (r ? ", [" : "[");
-            for (uint32_t k = 0; k < path.n_k(); ++k) out << (k ? ", " : "") << v[k] / norm;
+            for (uint32_t k = 0; k < path.n_k(); ++k) out << (k ? ", " : "") << v[k];
             out << "]";
+#if SPIMC_HAVE_HDF5
+            if (nm && r < h5.size()) h5[r]->Append("Observables/" + s.name + "/y", v.data(), path.n_k());
+#endif
         }
         out << "]}";
         check(spimc_obs_reset(eng.ctx, SPIMC_OBS_SK), "reset");
     }
 
-    // PairCorrelation::Write (pair_correlation_class.h:177-210)
+    // PairCorrelation bin centres (pair_correlation_class.h:64-74): the
+    // r^3-weighted mean radius of each of the n_r - 1 shells
+    static std::vector<double> GrCentres(const PairCorrelation& g) {
+        const double dr = (g.r_max - g.r_min) / (g.n_r - 1.);
+        std::vector<double> rs(g.n_r - 1);
+        for (uint32_t i = 0; i + 1 < g.n_r; ++i) {
+            const double r1 = g.r_min + i * dr, r2 = g.r_min + (i + 1) * dr;
+            rs[i] = 0.75 * (r2 * r2 * r2 * r2 - r1 * r1 * r1 * r1) / (r2 * r2 * r2 - r1 * r1 * r1);
+        }
+        return rs;
+    }
+
+    // PairCorrelation::Write (pair_correlation_class.h:89-124)
     void WriteGr(const PairCorrelation& g) {
         const double dr = (g.r_max - g.r_min) / (g.n_r - 1.);
         std::vector<double> x(g.n_r);
         for (uint32_t i = 0; i < g.n_r; ++i) x[i] = g.r_min + i * dr;
         out << "\"" << g.name << "\": {\"x\": [";
-        for (uint32_t i = 0; i + 1 < g.n_r; ++i) {
-            const double r1 = x[i], r2 = x[i + 1];
-            out << (i ? ", " : "") << 0.75 * (r2 * r2 * r2 * r2 - r1 * r1 * r1 * r1) / (r2 * r2 * r2 - r1 * r1 * r1);
-        }
+        const auto rs = GrCentres(g);
+        for (size_t i = 0; i < rs.size(); ++i) out << (i ? ", " : "") << rs[i];
         out << "], \"y\": [";
         std::vector<double> v(g.n_r);
         const double vol = path.L * path.L * path.L;
@@ -218,17 +296,84 @@ struct Simulation {
             uint64_t nm = 0;
             check(spimc_obs_read(eng.ctx, r, SPIMC_OBS_GR, v.data(), &nm), "gr");
             const double norm = 0.5 * (double)(nm ? nm : 1) * path.n_part * (path.n_part - 1.) * path.n_bead / vol;
-            out << (r ? ", [" : "[");
             for (uint32_t i = 0; i < g.n_r; ++i) {
                 const double r1 = x[i], r2 = (i + 1 < g.n_r) ? x[i + 1] : 2. * x[i] - x[i - 1];
                 const double bin_vol = 4. * M_PI / 3. * (r2 * r2 * r2 - r1 * r1 * r1);
-                out << (i ? ", " : "") << v[i] / (bin_vol * norm);
+                v[i] /= bin_vol * norm;
             }
+            out << (r ? ", [" : "[");
+            for (uint32_t i = 0; i < g.n_r; ++i) out << (i ? ", " : "") << v[i];
             out << "]";
+#if SPIMC_HAVE_HDF5
+            if (nm && r < h5.size()) h5[r]->Append("Observables/" + g.name + "/y", v.data(), g.n_r);
+#endif
         }
         out << "]}";
         check(spimc_obs_reset(eng.ctx, SPIMC_OBS_GR), "reset");
     }
+
+#if SPIMC_HAVE_HDF5
+    std::vector<std::unique_ptr<H5Out>> h5;
+
+    // One file per replica, <output_prefix>.<replica>.h5, as the reference
+    // writes one per clone group (framework_class.h:68-80); the set-up
+    // records of Framework, Simulation, Action, Move and Observable.
+    void OpenH5(const std::string& prefix, const std::string& in_file, uint64_t seed) {
+        std::ifstream f(in_file);
+        std::stringstream contents;
+        contents << f.rdbuf();
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) {
+            h5.push_back(std::make_unique<H5Out>(prefix + "." + std::to_string(r) + ".h5"));
+            H5Out& o = *h5.back();
+            o.String("Input/file_name", in_file);
+            o.String("Input/contents", contents.str());
+            o.Int("RNG/seed", (long long)seed);
+            o.Group("Actions");
+            for (auto& a : actions) {
+                const std::string g = "Actions/" + a->name + "/";
+                o.String(g + "type", a->type);
+                o.Int(g + "n_images", a->n_images);
+                o.Int(g + "max_level", a->max_level);
+                if (auto p = dynamic_cast<PairActionBase*>(a.get())) {
+                    o.Int(g + "n_order", p->n_order);
+                    o.String(g + "species_a", p->species_a);
+                    o.String(g + "species_b", p->species_b);
+                    o.Int(g + "use_long_range", p->use_long_range);
+                    if (p->use_long_range) o.Double(g + "k_cut", p->k_cut);
+                    o.String(g + "file", p->file);
+                }
+            }
+            o.Group("Moves");
+            for (auto& m : moves) {
+                o.String("Moves/" + m->name + "/type", m->type);
+                o.String("Moves/" + m->name + "/data_type", "scalar");
+            }
+            o.Group("Observables");
+            for (auto& ob : observables) {
+                const std::string g = "Observables/" + ob->name + "/";
+                o.String(g + "type", ob->type);
+                const bool hist = dynamic_cast<StructureFactor*>(ob.get()) || dynamic_cast<PairCorrelation*>(ob.get());
+                o.String(g + "data_type", hist ? "histogram" : "none");
+                if (auto s = dynamic_cast<StructureFactor*>(ob.get())) {
+                    o.String(g + "species_a", s->species_a);
+                    o.String(g + "species_b", s->species_b);
+                    o.Double(g + "k_cut", s->k_cut);
+                    std::vector<double> mags(path.n_k());
+                    for (uint32_t k = 0; k < path.n_k(); ++k) mags[k] = path.kmag(k);
+                    o.Doubles(g + "x", mags);
+                } else if (auto gr = dynamic_cast<PairCorrelation*>(ob.get())) {
+                    o.String(g + "species_a", gr->species_a);
+                    o.String(g + "species_b", gr->species_b);
+                    o.Double(g + "r_min", gr->r_min);
+                    o.Double(g + "r_max", gr->r_max);
+                    o.Int(g + "n_r", gr->n_r);
+                    o.Doubles(g + "x", GrCentres(*gr));
+                }
+            }
+            o.Flush();
+        }
+    }
+#endif
 
     void Run() {
         std::cout << "Running main loop..." << std::endl;
@@ -239,17 +384,84 @@ struct Simulation {
 
 }  // namespace spimc_host
 
+// `spimc --table-to-h5 in out [--tau-fastest]`: writes a pair-action table
+// (text container or HDF5) as an HDF5 file with the same dataset names.  The
+// David u_kj / du_kj_dbeta cubes become 3-D datasets [n_tau][n_grid][n_val],
+// or [n_val][n_grid][n_tau] with --tau-fastest; everything else keeps its shape.
+static int TableToH5(const std::string& in_path, const std::string& out_path, bool tau_fastest) {
+#if SPIMC_HAVE_HDF5
+    spimc_host::TableFile t;
+    t.Load(in_path);
+    const auto& d = t.Data();
+    spimc_host::H5Out o(out_path);
+    for (auto& kv : d.strings) o.String(kv.first, kv.second);
+    for (auto& kv : d.data) {
+        const std::string& name = kv.first;
+        const std::vector<double>& v = kv.second;
+        std::string pre;
+        for (const char* p : {"u_kj_", "du_kj_dbeta_"})
+            if (name.rfind(p, 0) == 0 && name.size() > 5 && name.compare(name.size() - 5, 5, "/data") == 0)
+                pre = "u_kj_" + name.substr(std::strlen(p), name.size() - 5 - std::strlen(p));
+        if (!pre.empty() && t.Has(pre + "/grid/n_grid_points") && t.Has(pre + "/taus")) {
+            const size_t n_grid = (size_t)t.Scalar(pre + "/grid/n_grid_points"), n_tau = t.Read(pre + "/taus").size();
+            const size_t n_val = v.size() / (n_grid * n_tau);
+            const std::vector<double> c = t.Cube(name, n_val, n_grid, n_tau);  // first index fastest
+            if (!tau_fastest) {
+                o.Doubles(name, c, {n_tau, n_grid, n_val});
+            } else {
+                std::vector<double> w(v.size());
+                for (size_t i = 0; i < n_val; ++i)
+                    for (size_t g = 0; g < n_grid; ++g)
+                        for (size_t k = 0; k < n_tau; ++k) w[(i * n_grid + g) * n_tau + k] = c[i + n_val * (g + n_grid * k)];
+                o.Doubles(name, w, {n_val, n_grid, n_tau});
+            }
+        } else if (d.ints.count(name)) {
+            std::vector<long long> iv(v.begin(), v.end());
+            o.Ints(name, iv);
+        } else {
+            const auto& dm = d.dims.at(name);
+            o.Doubles(name, v, std::vector<hsize_t>(dm.begin(), dm.end()));
+        }
+    }
+    std::cout << "wrote " << out_path << ": " << d.data.size() + d.strings.size() << " datasets" << std::endl;
+    return 0;
+#else
+    (void)in_path;
+    (void)out_path;
+    (void)tau_fastest;
+    std::cerr << "ERROR: built without HDF5" << std::endl;
+    return 1;
+#endif
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::cerr << "usage: spimc input.xml [--replicas R]" << std::endl;
+        std::cerr << "usage: spimc input.xml [--replicas R] [--no-h5]\n"
+                     "       spimc --table-to-h5 table_in table_out.h5 [--tau-fastest]"
+                  << std::endl;
         return 2;
     }
+    if (std::string(argv[1]) == "--table-to-h5") {
+        if (argc < 4) {
+            std::cerr << "usage: spimc --table-to-h5 table_in table_out.h5 [--tau-fastest]" << std::endl;
+            return 2;
+        }
+        try {
+            return TableToH5(argv[2], argv[3], argc > 4 && std::string(argv[4]) == "--tau-fastest");
+        } catch (const std::exception& e) {
+            std::cerr << "ERROR: " << e.what() << std::endl;
+            return 1;
+        }
+    }
     uint32_t R = 1;
-    for (int i = 2; i + 1 < argc; ++i)
-        if (std::string(argv[i]) == "--replicas") R = (uint32_t)std::atoi(argv[i + 1]);
+    bool h5 = true;
+    for (int i = 2; i < argc; ++i) {
+        if (std::string(argv[i]) == "--replicas" && i + 1 < argc) R = (uint32_t)std::atoi(argv[i + 1]);
+        if (std::string(argv[i]) == "--no-h5") h5 = false;
+    }
     try {
         std::cout << "Running spimc on " << argv[1] << " with " << R << " replica(s)" << std::endl;
-        spimc_host::Simulation sim(argv[1], R);
+        spimc_host::Simulation sim(argv[1], R, h5);
         sim.Run();
     } catch (const std::exception& e) {
         std::cerr << "ERROR: " << e.what() << std::endl;

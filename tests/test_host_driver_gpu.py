@@ -8,6 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import h5dump
 from simpimc_amd import abi, system
 
 pytestmark = pytest.mark.gpu
@@ -15,17 +16,21 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "simpimc_amd", "spimc")
 
+BARE = ('<Action name="Coulomb" type="BarePairAction" file="{file}" species_a="e" species_b="e" is_coulomb="1" '
+        'max_level="0" use_long_range="1" />')
+DAVID = ('<Action name="David" type="DavidPairAction" file="{file}" species_a="e" species_b="e" n_order="2" '
+         'max_level="0" use_long_range="1" />')
+
 XML = """<Input>
   <RNG seed="{seed}" />
-  <IO output_prefix="run" />
+  <IO output_prefix="{prefix}" />
   <System n_d="3" n_bead="{M}" beta="{beta!r}" L="{L!r}" pbc="1" k_cut="{kc!r}" />
   <Particles>
     <Species name="e" n_part="{N}" lambda="0.5" init_type="File" file_name="init.xyz" />
   </Particles>
   <Actions>
     <Action name="Kinetic" type="Kinetic" species="e" />
-    <Action name="Coulomb" type="BarePairAction" file="bare.table" species_a="e" species_b="e" is_coulomb="1"
-            max_level="0" use_long_range="1" />
+    {action}
   </Actions>
   <Moves>
     <Move name="Bisect" type="Bisect" species="e" n_level="3" />
@@ -63,7 +68,8 @@ def test_xml_driver_matches_binding(tmp_path):
     system.write_table(str(tmp_path / "bare.table"), system.bare_table_datasets(s))
     kc = (5.0 + 1e-9) * 2 * np.pi / s.L
     nb = (s.n_part * s.n_bead) >> s.bisect_n_level
-    (tmp_path / "in.xml").write_text(XML.format(seed=seed, M=s.n_bead, beta=s.beta, L=s.L, kc=kc, N=s.n_part, nb=nb))
+    (tmp_path / "in.xml").write_text(XML.format(seed=seed, M=s.n_bead, beta=s.beta, L=s.L, kc=kc, N=s.n_part, nb=nb,
+                                                prefix="run", action=BARE.format(file="bare.table")))
     res = subprocess.run([DRIVER, "in.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert res.returncode == 0, res.stderr
     blocks = [json.loads(line) for line in open(tmp_path / "run.spimc.jsonl")]
@@ -88,3 +94,91 @@ def test_xml_driver_matches_binding(tmp_path):
             np.testing.assert_allclose(y, sk / (nm * s.n_bead * s.n_part ** 2), rtol=1e-14)
             mv = blk["Moves"]["Bisect"][0]
             assert mv["n_attempt"] == 3 * nb
+
+
+def run_driver(tmp_path, s, action, prefix, replicas, seed=4242):
+    kc = (5.0 + 1e-9) * 2 * np.pi / s.L
+    nb = (s.n_part * s.n_bead) >> s.bisect_n_level
+    xml = tmp_path / f"{prefix}.xml"
+    xml.write_text(XML.format(seed=seed, M=s.n_bead, beta=s.beta, L=s.L, kc=kc, N=s.n_part, nb=nb, prefix=prefix,
+                              action=action))
+    print(f"driver: {prefix} ({replicas} replicas)", flush=True)
+    res = subprocess.run([DRIVER, xml.name, "--replicas", str(replicas)], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    return [json.loads(line) for line in open(tmp_path / f"{prefix}.spimc.jsonl")]
+
+
+def convert(tmp_path, src, dst, *extra):
+    res = subprocess.run([DRIVER, "--table-to-h5", src, dst, *extra], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+
+
+def write_init(tmp_path, s):
+    pos = system.random_path(s, seed=8)
+    with open(tmp_path / "init.xyz", "w") as f:
+        for p in range(s.n_part):
+            f.write("%.17g %.17g %.17g\n" % tuple(pos[:, 0, p]))
+
+
+@pytest.mark.skipif(h5dump.H5DUMP is None, reason="h5dump not installed")
+def test_hdf5_table_and_output(tmp_path):
+    """The Bare table as HDF5 gives the run of the text table, and the per-replica
+    HDF5 files hold the blocks of the JSON-lines output in the reference's layout."""
+    assert os.path.exists(DRIVER), "build the host driver first"
+    s = system.config_c1()
+    write_init(tmp_path, s)
+    system.write_table(str(tmp_path / "bare.table"), system.bare_table_datasets(s))
+    convert(tmp_path, "bare.table", "bare.h5")
+    R = 2
+    txt = run_driver(tmp_path, s, BARE.format(file="bare.table"), "txt", R)
+    h5 = run_driver(tmp_path, s, BARE.format(file="bare.h5"), "h5", R)
+    assert txt == h5
+    for r in range(R):
+        f = str(tmp_path / f"h5.{r}.h5")
+        ds = lambda name: h5dump.dataset(f, name)[1]  # noqa: E731
+        assert ds("/Actions/Coulomb/type") == ["BarePairAction"]
+        assert ds("/Actions/Coulomb/file") == ["bare.h5"]
+        assert ds("/RNG/seed")[0] == 4242
+        assert ds("/Input/file_name") == ["h5.xml"]
+        for mv in ("Bisect", "Displace"):
+            assert ds(f"/Moves/{mv}/data_type") == ["scalar"]
+            att = ds(f"/Moves/{mv}/n_attempt")
+            acc = ds(f"/Moves/{mv}/n_accept")
+            assert list(att) == [b["Moves"][mv][r]["n_attempt"] for b in h5]
+            assert list(acc) == [b["Moves"][mv][r]["n_accept"] for b in h5]
+            np.testing.assert_array_equal(ds(f"/Moves/{mv}/x"), acc / att)
+        assert ds("/Observables/Energy/type") == ["Energy"]
+        for key in ("total", "Kinetic", "Coulomb"):
+            assert ds(f"/Observables/Energy/{key}/data_type") == ["scalar"]
+            np.testing.assert_allclose(ds(f"/Observables/Energy/{key}/x"),
+                                       [b["Observables"]["Energy"][r][key] for b in h5], rtol=1e-14)
+        for ob in ("SK", "GR"):
+            assert ds(f"/Observables/{ob}/data_type") == ["histogram"]
+            assert ds(f"/Observables/{ob}/species_a") == ["e"]
+            dims, y = h5dump.dataset(f, f"/Observables/{ob}/y")
+            want = np.array([b["Observables"][ob]["y"][r] for b in h5])
+            assert dims == want.shape
+            np.testing.assert_array_equal(y.reshape(dims), want)
+            np.testing.assert_array_equal(ds(f"/Observables/{ob}/x"), h5[0]["Observables"][ob]["x"])
+        assert ds("/Observables/GR/n_r")[0] == 60
+
+
+@pytest.mark.skipif(h5dump.H5DUMP is None, reason="h5dump not installed")
+def test_david_hdf5_table_matches_text(tmp_path):
+    """A David table whose cubes hold three taus, as text and as HDF5 with the
+    tau-fastest cube layout, drives identical runs."""
+    from test_tables_h5_cpu import david_three_taus
+
+    assert os.path.exists(DRIVER), "build the host driver first"
+    s = system.config_c1()
+    write_init(tmp_path, s)
+    d, _, _ = david_three_taus(s)
+    system.write_table(str(tmp_path / "david.table"), d)
+    convert(tmp_path, "david.table", "david.h5", "--tau-fastest")
+    txt = run_driver(tmp_path, s, DAVID.format(file="david.table"), "txt", 1)
+    h5 = run_driver(tmp_path, s, DAVID.format(file="david.h5"), "h5", 1)
+    assert txt == h5
+    e = txt[-1]["Observables"]["Energy"][0]
+    assert np.isfinite(e["total"]) and e["n_measure"] == 3
