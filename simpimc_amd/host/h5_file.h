@@ -321,8 +321,23 @@ class H5Out {
 
     // Appends one block to an extendable dataset: a scalar row (n == 0) of a
     // 1-D dataset, or a row of n values of a 2-D dataset [block][n].
-    void Append(const std::string& name, const double* row, size_t n) { AppendRow(name, h_.t_double, row, n); }
-    void AppendInt(const std::string& name, long long v) { AppendRow(name, h_.t_int64, &v, 0); }
+    void Append(const std::string& name, const double* row, size_t n) {
+        AppendRow(name, h_.t_double, row, n ? std::vector<hsize_t>{n} : std::vector<hsize_t>{});
+    }
+    void AppendInt(const std::string& name, long long v) { AppendRow(name, h_.t_int64, &v, {}); }
+    // Appends one block of shape `row_dims` to the dataset [block][row_dims...].
+    void Append(const std::string& name, const double* row, const std::vector<hsize_t>& row_dims) {
+        AppendRow(name, h_.t_double, row, row_dims);
+    }
+    // A one-element integer dataset, created or overwritten.
+    void SetInt(const std::string& name, long long v) {
+        if (h_.H5Lexists(f_, name.c_str(), H5P_DEFAULT) <= 0) return Int(name, v);
+        const hid_t ds = h_.H5Dopen2(f_, name.c_str(), H5P_DEFAULT);
+        h_.Check(ds, path_ + ": opening " + name);
+        const herr_t rc = h_.H5Dwrite(ds, h_.t_int64, H5S_ALL, H5S_ALL, H5P_DEFAULT, &v);
+        h_.H5Dclose(ds);
+        h_.Check(rc, path_ + ": rewriting " + name);
+    }
 
     void Flush() { h_.H5Fflush(f_, H5F_SCOPE_LOCAL); }
 
@@ -348,17 +363,20 @@ class H5Out {
         h_.Check(rc, path_ + ": writing " + name);
     }
 
-    void AppendRow(const std::string& name, hid_t ty, const void* row, size_t n) {
-        const int rank = n ? 2 : 1;
-        const hsize_t cols = n ? n : 1;
+    void AppendRow(const std::string& name, hid_t ty, const void* row, const std::vector<hsize_t>& row_dims) {
+        const int rank = 1 + (int)row_dims.size();
+        std::vector<hsize_t> d0{0}, dmax{H5S_UNLIMITED}, chunk{row_dims.empty() ? (hsize_t)256 : (hsize_t)1};
+        for (hsize_t d : row_dims) {
+            d0.push_back(d);
+            dmax.push_back(d);
+            chunk.push_back(d);
+        }
         auto it = ext_.find(name);
         if (it == ext_.end()) {
             Parent(name);
-            const hsize_t d0[2] = {0, cols}, dmax[2] = {H5S_UNLIMITED, cols};
-            const hsize_t chunk[2] = {n ? 1 : 256, cols};
-            const hid_t sp = h_.H5Screate_simple(rank, d0, dmax);
+            const hid_t sp = h_.H5Screate_simple(rank, d0.data(), dmax.data());
             const hid_t pl = h_.H5Pcreate(h_.p_dataset_create);
-            h_.H5Pset_chunk(pl, rank, chunk);
+            h_.H5Pset_chunk(pl, rank, chunk.data());
             const hid_t ds = h_.H5Dcreate2(f_, name.c_str(), ty, sp, H5P_DEFAULT, pl, H5P_DEFAULT);
             h_.H5Pclose(pl);
             h_.H5Sclose(sp);
@@ -368,11 +386,14 @@ class H5Out {
         }
         const hid_t ds = it->second;
         hsize_t& rows = rows_[name];
-        const hsize_t size[2] = {rows + 1, cols}, start[2] = {rows, 0}, count[2] = {1, cols};
-        h_.Check(h_.H5Dset_extent(ds, size), path_ + ": extending " + name);
+        std::vector<hsize_t> size(d0), start(rank, 0), count(d0);
+        size[0] = rows + 1;
+        start[0] = rows;
+        count[0] = 1;
+        h_.Check(h_.H5Dset_extent(ds, size.data()), path_ + ": extending " + name);
         const hid_t fs = h_.H5Dget_space(ds);
-        h_.H5Sselect_hyperslab(fs, H5S_SELECT_SET, start, nullptr, count, nullptr);
-        const hid_t ms = h_.H5Screate_simple(rank, count, nullptr);
+        h_.H5Sselect_hyperslab(fs, H5S_SELECT_SET, start.data(), nullptr, count.data(), nullptr);
+        const hid_t ms = h_.H5Screate_simple(rank, count.data(), nullptr);
         const herr_t rc = h_.H5Dwrite(ds, ty, ms, fs, H5P_DEFAULT, row);
         h_.H5Sclose(ms);
         h_.H5Sclose(fs);

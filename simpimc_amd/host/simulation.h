@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <iostream>
 #include <map>
 #include <memory>
 #include <random>
@@ -80,8 +81,8 @@ struct Path {
         return std::sqrt(x * x + y * y + z * z);
     }
 
-    // Initial path of one replica, positions r[3][M][N] (species_class.h:222-334)
-    std::vector<double> InitPath(uint64_t seed) const {
+    // Initial path of one replica, positions r[3][M][N] (species_class.h:222-377)
+    std::vector<double> InitPath(uint64_t seed, uint32_t replica) const {
         const std::string init = species_in.GetAttribute<std::string>("init_type");
         std::vector<double> r((size_t)3 * n_bead * n_part);
         auto set = [&](uint32_t p, const double* x) {
@@ -129,10 +130,48 @@ struct Path {
                     set(p, x);
                 }
             }
+        } else if (init == "Restart") {
+            RestartPath(r, replica);
         } else {
             throw std::runtime_error("init_type " + init + " is outside this engine");
         }
         return r;
+    }
+
+    // init_type "Restart" (species_class.h:336-377): the last dump of a PathDump
+    // observable in <prefix>.<replica>.h5 (parallel="1") or <prefix>.0.h5.
+    // Positions are read as cube(n_d, n_part * n_bead, n_dump), entry
+    // (d, p * n_bead + b, n_dump - 1); the permutation must be the identity.
+    void RestartPath(std::vector<double>& r, uint32_t replica) const {
+        const std::string prefix = species_in.GetAttribute<std::string>("prefix");
+        const std::string dump = species_in.GetAttribute<std::string>("path_dump_name", "path_dump");
+        const bool parallel = species_in.GetAttribute<bool>("parallel", false);
+        const std::string file = prefix + "." + std::to_string(parallel ? replica : 0) + ".h5";
+        const TableData t = ReadHdf5(file);
+        const std::string g = "Observables/" + dump + "/" + species + "/";
+        auto get = [&](const std::string& n) -> const std::vector<double>& {
+            auto it = t.data.find(g + n);
+            if (it == t.data.end()) throw std::runtime_error(file + ": missing dataset " + g + n);
+            return it->second;
+        };
+        const auto& nd = get("n_dump");
+        if (nd.empty() || nd[0] < 1) throw std::runtime_error(file + ": no path dump");
+        const size_t n_dump = (size_t)nd[0];
+        const auto& pos = get("positions");
+        const auto& perm = get("permutation");
+        const size_t per = (size_t)n_d * n_part * n_bead;
+        if (pos.size() < per * n_dump || perm.size() < (size_t)2 * n_part * n_dump)
+            throw std::runtime_error(file + ": path dump smaller than n_dump paths of this system");
+        const double* last = pos.data() + per * (n_dump - 1);
+        for (uint32_t p = 0; p < n_part; ++p)
+            for (uint32_t b = 0; b < n_bead; ++b)
+                for (uint32_t d = 0; d < n_d; ++d)
+                    r[((size_t)d * n_bead + b) * n_part + p] = last[d + (size_t)n_d * (p * (size_t)n_bead + b)];
+        const double* pl = perm.data() + (size_t)2 * n_part * (n_dump - 1);
+        for (uint32_t p = 0; p < n_part; ++p)
+            if (pl[2 * p] != p || pl[2 * p + 1] != p)
+                throw std::runtime_error(file + ": restart from a permuted path is outside this engine");
+        std::cout << "Restarting " << species << " from " << file << "..." << std::endl;
     }
 };
 
@@ -435,6 +474,14 @@ struct StructureFactor : Observable {
         k_cut = in.GetAttribute<double>("k_cut", path.k_cut);
         path.SetupK(k_cut);
     }
+};
+
+// PathDump (path_dump_class.h:7-69): every `skip`-th Write appends the path of
+// each replica to its output file; accumulates nothing.
+struct PathDump : Observable {
+    uint32_t skip, n_dump = 0, n_write_calls = 0;
+    PathDump(const Input& in, Engine& e) : Observable(in, e) { skip = in.GetAttribute<uint32_t>("skip", 1); }
+    void DoEvent() override {}
 };
 
 struct PairCorrelation : Observable {

@@ -83,6 +83,8 @@ struct Simulation {
             } else if (type == "PairCorrelation") {
                 if (gr) throw std::runtime_error("one PairCorrelation observable per input");
                 ob = gr = std::make_shared<PairCorrelation>(o, eng, path);
+            } else if (type == "PathDump") {
+                ob = std::make_shared<PathDump>(o, eng);
             } else {
                 throw std::runtime_error("ERROR: Unrecognized observable, " + type);
             }
@@ -106,7 +108,7 @@ struct Simulation {
         std::vector<spimc_system_desc> descs(n_replicas, desc);
         check(spimc_ctx_create(descs.data(), n_replicas, seed, &eng.ctx), "context");
         for (uint32_t r = 0; r < n_replicas; ++r) {
-            auto pos = path.InitPath(seed * 1000003ull + r);
+            auto pos = path.InitPath(seed * 1000003ull + r, r);
             check(spimc_path_upload(eng.ctx, r, pos.data()), "path upload");
         }
         check(spimc_obs_configure(eng.ctx, en ? en->estimator : SPIMC_ENERGY_THERMAL, 0, sk ? sk->k_cut : 1e300,
@@ -180,6 +182,8 @@ struct Simulation {
                 WriteSk(*s);
             else if (auto g = dynamic_cast<PairCorrelation*>(o.get()))
                 WriteGr(*g);
+            else if (auto d = dynamic_cast<PathDump*>(o.get()))
+                WritePathDump(*d);
         }
         out << "}}\n";
         out.flush();
@@ -310,6 +314,34 @@ struct Simulation {
         }
         out << "]}";
         check(spimc_obs_reset(eng.ctx, SPIMC_OBS_GR), "reset");
+    }
+
+    // PathDump::Write (path_dump_class.h:29-66): positions [dump][particle][bead][d]
+    // (the reference's cube(n_d, n_bead, n_part) per dump, first index fastest)
+    // and permutation [dump][particle][2] (previous and next particle of the
+    // end beads, the identity for Boltzmann paths)
+    void WritePathDump(PathDump& d) {
+        const bool dump = !(d.n_write_calls % d.skip);
+        if (dump) d.n_dump += 1;
+        out << "\"" << d.name << "\": {\"n_dump\": " << d.n_dump << "}";
+#if SPIMC_HAVE_HDF5
+        if (dump && !h5.empty()) {
+            const uint32_t N = path.n_part, M = path.n_bead;
+            std::vector<double> r((size_t)3 * M * N), pd((size_t)N * M * 3), perm((size_t)2 * N);
+            for (uint32_t p = 0; p < N; ++p) perm[2 * p] = perm[2 * p + 1] = p;
+            const std::string g = "Observables/" + d.name + "/" + path.species + "/";
+            for (uint32_t rep = 0; rep < h5.size(); ++rep) {
+                check(spimc_path_download(eng.ctx, rep, r.data()), "path download");
+                for (uint32_t p = 0; p < N; ++p)
+                    for (uint32_t b = 0; b < M; ++b)
+                        for (uint32_t x = 0; x < 3; ++x) pd[((size_t)p * M + b) * 3 + x] = r[((size_t)x * M + b) * N + p];
+                h5[rep]->SetInt(g + "n_dump", d.n_dump);
+                h5[rep]->Append(g + "positions", pd.data(), {N, M, 3});
+                h5[rep]->Append(g + "permutation", perm.data(), {N, 2});
+            }
+        }
+#endif
+        d.n_write_calls += 1;
     }
 
 #if SPIMC_HAVE_HDF5

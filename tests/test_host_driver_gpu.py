@@ -16,6 +16,9 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "simpimc_amd", "spimc")
 
+INIT_FILE = 'init_type="File" file_name="init.xyz"'
+PATH_DUMP = '\n    <Observable name="path_dump" type="PathDump" />'
+
 BARE = ('<Action name="Coulomb" type="BarePairAction" file="{file}" species_a="e" species_b="e" is_coulomb="1" '
         'max_level="0" use_long_range="1" />')
 DAVID = ('<Action name="David" type="DavidPairAction" file="{file}" species_a="e" species_b="e" n_order="2" '
@@ -26,7 +29,7 @@ XML = """<Input>
   <IO output_prefix="{prefix}" />
   <System n_d="3" n_bead="{M}" beta="{beta!r}" L="{L!r}" pbc="1" k_cut="{kc!r}" />
   <Particles>
-    <Species name="e" n_part="{N}" lambda="0.5" init_type="File" file_name="init.xyz" />
+    <Species name="e" n_part="{N}" lambda="0.5" {init} />
   </Particles>
   <Actions>
     <Action name="Kinetic" type="Kinetic" species="e" />
@@ -39,7 +42,7 @@ XML = """<Input>
   <Observables>
     <Observable name="Energy" type="Energy" />
     <Observable name="SK" type="StructureFactor" species_a="e" species_b="e" />
-    <Observable name="GR" type="PairCorrelation" species_a="e" species_b="e" n_r="60" />
+    <Observable name="GR" type="PairCorrelation" species_a="e" species_b="e" n_r="60" />{extra}
   </Observables>
   <Algorithm>
     <Loop n_step="2">
@@ -69,7 +72,8 @@ def test_xml_driver_matches_binding(tmp_path):
     kc = (5.0 + 1e-9) * 2 * np.pi / s.L
     nb = (s.n_part * s.n_bead) >> s.bisect_n_level
     (tmp_path / "in.xml").write_text(XML.format(seed=seed, M=s.n_bead, beta=s.beta, L=s.L, kc=kc, N=s.n_part, nb=nb,
-                                                prefix="run", action=BARE.format(file="bare.table")))
+                                                prefix="run", action=BARE.format(file="bare.table"), init=INIT_FILE,
+                                                extra=""))
     res = subprocess.run([DRIVER, "in.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert res.returncode == 0, res.stderr
     blocks = [json.loads(line) for line in open(tmp_path / "run.spimc.jsonl")]
@@ -96,12 +100,12 @@ def test_xml_driver_matches_binding(tmp_path):
             assert mv["n_attempt"] == 3 * nb
 
 
-def run_driver(tmp_path, s, action, prefix, replicas, seed=4242):
+def run_driver(tmp_path, s, action, prefix, replicas, seed=4242, init=None, extra=""):
     kc = (5.0 + 1e-9) * 2 * np.pi / s.L
     nb = (s.n_part * s.n_bead) >> s.bisect_n_level
     xml = tmp_path / f"{prefix}.xml"
     xml.write_text(XML.format(seed=seed, M=s.n_bead, beta=s.beta, L=s.L, kc=kc, N=s.n_part, nb=nb, prefix=prefix,
-                              action=action))
+                              action=action, init=init or INIT_FILE, extra=extra))
     print(f"driver: {prefix} ({replicas} replicas)", flush=True)
     res = subprocess.run([DRIVER, xml.name, "--replicas", str(replicas)], cwd=tmp_path, capture_output=True, text=True,
                          timeout=300)
@@ -182,3 +186,59 @@ def test_david_hdf5_table_matches_text(tmp_path):
     assert txt == h5
     e = txt[-1]["Observables"]["Energy"][0]
     assert np.isfinite(e["total"]) and e["n_measure"] == 3
+
+
+@pytest.mark.skipif(h5dump.H5DUMP is None, reason="h5dump not installed")
+def test_path_dump_and_restart(tmp_path):
+    """PathDump appends each replica's path to its output file at every Write; the
+    last dump equals the chain the binding reaches with the same seed; a run with
+    init_type Restart (parallel: replica r from file r, else every replica from
+    file 0) starts from that dump."""
+    assert os.path.exists(DRIVER), "build the host driver first"
+    s = system.config_c1()
+    write_init(tmp_path, s)
+    system.write_table(str(tmp_path / "bare.table"), system.bare_table_datasets(s))
+    R = 2
+    blocks = run_driver(tmp_path, s, BARE.format(file="bare.table"), "A", R, extra=PATH_DUMP)
+    assert [b["Observables"]["path_dump"]["n_dump"] for b in blocks] == [1, 2]
+    N, M = s.n_part, s.n_bead
+    last = []
+    for r in range(R):
+        f = str(tmp_path / f"A.{r}.h5")
+        assert h5dump.dataset(f, "/Observables/path_dump/e/n_dump")[1][0] == 2
+        dims, pos = h5dump.dataset(f, "/Observables/path_dump/e/positions")
+        assert dims == (2, N, M, 3)
+        pdims, perm = h5dump.dataset(f, "/Observables/path_dump/e/permutation")
+        assert pdims == (2, N, 2)
+        assert np.array_equal(perm.reshape(pdims)[-1], np.repeat(np.arange(N), 2).reshape(N, 2))
+        last.append(pos.reshape(dims)[-1].transpose(2, 1, 0))  # -> [3][M][N]
+    # replica 0 of the driver is the binding's chain with the same seed
+    pos0 = system.random_path(s, seed=8)
+    nb = (N * M) >> s.bisect_n_level
+    kc = (5.0 + 1e-9) * 2 * np.pi / s.L
+    with abi.Engine([s] * R, seed=4242) as e:
+        for r in range(R):
+            e.upload(pos0, r=r)
+        e.obs_configure(energy=abi.ENERGY_THERMAL, sk_k_cut=kc, gr_n_r=60)
+        for _ in range(2 * 3):
+            e.bisect(nb)
+            e.displace(N)
+            e.obs_accumulate(abi.OBS_ENERGY | abi.OBS_SK | abi.OBS_GR)
+        for r in range(R):
+            assert np.array_equal(e.download(r=r), last[r]), r
+    for parallel in (1, 0):
+        prefix = f"B{parallel}"
+        init = f'init_type="Restart" prefix="A" parallel="{parallel}"'
+        run_driver(tmp_path, s, BARE.format(file="bare.table"), prefix, R, init=init, extra=PATH_DUMP)
+        # the restarted chain equals the binding's chain started from the dump
+        with abi.Engine([s] * R, seed=4242) as e:
+            for r in range(R):
+                e.upload(last[r if parallel else 0], r=r)
+            e.obs_configure(energy=abi.ENERGY_THERMAL, sk_k_cut=kc, gr_n_r=60)
+            for _ in range(2 * 3):
+                e.bisect(nb)
+                e.displace(N)
+                e.obs_accumulate(abi.OBS_ENERGY | abi.OBS_SK | abi.OBS_GR)
+            for r in range(R):
+                dims, pos = h5dump.dataset(str(tmp_path / f"{prefix}.{r}.h5"), "/Observables/path_dump/e/positions")
+                assert np.array_equal(e.download(r=r), pos.reshape(dims)[-1].transpose(2, 1, 0)), (parallel, r)
