@@ -21,6 +21,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "bead updates/s, HEG N=256 M=128 FP64, batched+adjoint vs plain GPU kernel"
 HBM_PEAK_GBPS = 8000.0
+MEASURED_RHO_ALG_BYTES = 1786919139.56  # algorithmic bytes per k_rho_loop launch, default workload
+MEASURED_RHO_TRAFFIC_BYTES = 1841.70e6  # HBM bytes per k_rho_loop launch, same workload (rocprofv3 --pmc)
 
 
 def parse():
@@ -114,8 +116,8 @@ def cpu_baseline(target_s: float):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (replicas shard "
-              f"across GPUs without traffic, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (replicas would "
+              f"shard across GPUs with no traffic between them)", file=sys.stderr)
         sys.exit(2)
     from simpimc_amd import abi, system
 
@@ -168,8 +170,15 @@ def main():
     roofline = None
     if rho_n:
         ach = rho_bytes / (rho_ms * 1e-3) / 1e9
+        # HBM bytes per k_rho_loop launch of this workload, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+        # (reads x2.000, writes x1.000, the factors a 1 GiB device copy gives), for the run whose algorithmic
+        # bytes per launch were 1786.9 MB; reported only when this run's algorithmic bytes are the same
+        same = abs(rho_bytes / rho_n - MEASURED_RHO_ALG_BYTES) < 1e-3 * MEASURED_RHO_ALG_BYTES
+        traffic = MEASURED_RHO_TRAFFIC_BYTES if same else None
         roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS,
-                    "traffic": None, "kernel": "k_rho (incremental rho_k update + long-range action)",
+                    "traffic": traffic, "traffic_unit": "bytes per launch",
+                    "traffic_from": ("profiles/k_rho_loop_k_pair_lds_heg256_m128_1024rep_counters.md" if same else None),
+                    "kernel": "k_rho (incremental rho_k update + long-range action)",
                     "bytes_per_launch": rho_bytes / rho_n, "avg_launch_us": 1e3 * rho_ms / rho_n,
                     "duration_from": ("dispatch events" if a.kernel_events else
                                       "device wall clock, first block start to last block end, every launch "
