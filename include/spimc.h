@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SPIMC_ABI_VERSION 1
+#define SPIMC_ABI_VERSION 2
 
 /* status codes */
 #define SPIMC_OK 0
@@ -48,6 +48,7 @@ extern "C" {
 #define SPIMC_PAIR_NONE 0
 #define SPIMC_PAIR_BARE 1  /* "BarePairAction"  pair_action/bare_pair_action_class.h */
 #define SPIMC_PAIR_DAVID 2 /* "DavidPairAction" pair_action/david_pair_action_class.h */
+#define SPIMC_PAIR_ILKKA 3 /* "IlkkaPairAction" pair_action/ilkka_pair_action_class.h */
 
 /* action selectors */
 #define SPIMC_ACTION_KINETIC 0
@@ -98,7 +99,7 @@ extern "C" {
  *   <System n_d n_bead beta L>                 (data_structures/path_class.h:25-69)
  *   <Species n_part lambda>                    (data_structures/species_class.h:48-98)
  *   <Action type="Kinetic" n_images>           (actions/single_action/kinetic_class.h:28-32)
- *   <Action type="BarePairAction"|"DavidPairAction" n_order max_level use_long_range>
+ *   <Action type="BarePairAction"|"DavidPairAction"|"IlkkaPairAction" n_order max_level use_long_range>
  *                                              (actions/pair_action/pair_action_class.h:205-238)
  *   <Move type="Bisect" n_level n_images>      (events/moves/single_species_move/bisect/bisect_class.h:167-190)
  *   <Move type="DisplaceParticle" step_size>   (events/moves/single_species_move/displace_particle_class.h:93)
@@ -150,6 +151,42 @@ typedef struct spimc_system_desc {
     const double* pair_v_long_k; /* [n_k] */
     double pair_v_long_k_0;
     double pair_v_long_r_0;
+    /* IlkkaPairAction knots (ilkka_pair_action_class.h:253-432), every spline a
+     * natural cubic one on its own general grid:
+     *   u/off_diag: U(x, y) on x_u[n_x_u] x y_u[n_y_u], values u_xy[n_y_u][n_x_u]
+     *     (x fastest, the memory order of the reference's mat(n_x, n_y));
+     *   du/off_diag: dU/dbeta(x, y) likewise;
+     *   v/diag: v_r on r_v (the potential; ilkka_v_long_r on ilkka_r_v_long is its
+     *     long-range part);
+     *   u/diag, du/diag: u_long_r, du_long_r on their own grids (use_long_range).
+     * Per stored k-vector: ilkka_u_long_k (action), ilkka_du_long_k (dU/dbeta);
+     * pair_v_long_k holds v/diag v_long_k (potential).  The raw k=0 and r=0
+     * constants of du/diag are ilkka_du_long_k_0, ilkka_du_long_r_0; those of
+     * v/diag are pair_v_long_k_0, pair_v_long_r_0. */
+    uint32_t ilkka_n_x_u, ilkka_n_y_u;
+    const double* ilkka_x_u;
+    const double* ilkka_y_u;
+    const double* ilkka_u_xy;
+    uint32_t ilkka_n_x_du, ilkka_n_y_du;
+    const double* ilkka_x_du;
+    const double* ilkka_y_du;
+    const double* ilkka_du_xy;
+    uint32_t ilkka_n_r_v;
+    const double* ilkka_r_v;
+    const double* ilkka_v_r;
+    uint32_t ilkka_n_r_v_long;
+    const double* ilkka_r_v_long;
+    const double* ilkka_v_long_r;
+    uint32_t ilkka_n_r_u_long;
+    const double* ilkka_r_u_long;
+    const double* ilkka_u_long_r;
+    uint32_t ilkka_n_r_du_long;
+    const double* ilkka_r_du_long;
+    const double* ilkka_du_long_r;
+    const double* ilkka_u_long_k;  /* [n_k] */
+    const double* ilkka_du_long_k; /* [n_k] */
+    double ilkka_du_long_k_0;
+    double ilkka_du_long_r_0;
 } spimc_system_desc;
 
 typedef struct spimc_ctx spimc_ctx;
@@ -215,9 +252,13 @@ int spimc_kinetic_virial_energy(spimc_ctx* ctx, uint32_t replica, uint32_t windo
 
 /* Per-bead action gradient out[b][p][3] of the committed path.
  *   compat = 1: the reference's PairAction::GetActionGradient(b, b+1, {p}, 0)
- *               (pair_action_class.h:305-337) for every bead: central finite
- *               difference (eps 1e-4) of the short-range link action, long range 0.
- *               `terms` must be SPIMC_GRAD_PAIR_SHORT.
+ *               (pair_action_class.h:305-337) for every bead.  SPIMC_GRAD_PAIR_SHORT:
+ *               its CalcGradientU on the links (b, b+1) and (b, b-1) -- a central
+ *               difference (eps 1e-4) of the link action for Bare and David, the
+ *               analytic form of ilkka_pair_action_class.h:172-222 for Ilkka.
+ *               SPIMC_GRAD_PAIR_LONG: CalcGradientULong once per particle pair --
+ *               zero for Bare and David, ilkka_pair_action_class.h:231-249 for Ilkka.
+ *               `terms` must not hold SPIMC_GRAD_KINETIC.
  *   compat = 0: analytic adjoint dS/dr of the terms selected in `terms`
  *               (kinetic, short-range pair, Ewald long range) in one pass.
  *               Replaces the numeric gradients feeding energy_class.h:79-94. */

@@ -122,6 +122,43 @@ struct NatSpline {
 };
 
 // ---------------------------------------------------------------------------
+// Tensor-product natural cubic spline on general grids (stands in for einspline
+// create_NUBspline_2d_d with BCtype_d{NATURAL, NATURAL} on both axes,
+// ilkka_pair_action_class.h:276-278, 322-324).  Evaluated as a spline of
+// splines: one natural spline along x per y knot, whose values at x are then
+// interpolated by a natural spline along y.
+// ---------------------------------------------------------------------------
+struct NatSpline2 {
+    std::vector<double> x, y;
+    std::vector<NatSpline> rows;  // rows[j]: along x through vals[j][*]
+
+    void build(const double* xs, uint32_t nx, const double* ys, uint32_t ny, const double* vals) {
+        x.assign(xs, xs + nx);
+        y.assign(ys, ys + ny);
+        rows.resize(ny);
+        for (uint32_t j = 0; j < ny; ++j) rows[j].build(xs, vals + (size_t)j * nx, nx, false);
+    }
+    // value and, when gx/gy are non-null, the partial derivatives at (xv, yv)
+    double eval(double xv, double yv, double* gx = nullptr, double* gy = nullptr) const {
+        const uint32_t ny = (uint32_t)y.size();
+        std::vector<double> g(ny), gd(ny);
+        for (uint32_t j = 0; j < ny; ++j) {
+            g[j] = rows[j].eval(xv);
+            if (gx) gd[j] = rows[j].deriv(xv);
+        }
+        NatSpline col;
+        col.build(y.data(), g.data(), ny, false);
+        if (gy) *gy = col.deriv(yv);
+        if (gx) {
+            NatSpline dcol;
+            dcol.build(y.data(), gd.data(), ny, false);
+            *gx = dcol.eval(yv);
+        }
+        return col.eval(yv);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // FreeSpline (actions/free_spline_class.h:25-122)
 // ---------------------------------------------------------------------------
 struct FreeSpline {
@@ -253,6 +290,12 @@ struct Oracle {
     uint32_t n_order = 0, n_val = 0;
     std::vector<NatSpline> u_kj, du_kj;  // n_val+1 components each
     double g_start = 0., g_end = 0.;
+    // Ilkka (ilkka_pair_action_class.h:9-30): U and dU/dbeta tensor splines, the
+    // long-range parts of U and dU/dbeta on their r grids; v_r and v_long_r above
+    // hold its potential
+    NatSpline2 u_xy, du_xy;
+    NatSpline u_long_r_s, du_long_r_s;
+    double r_u_min = 0., r_u_max = 0., r_du_min = 0., r_du_max = 0.;
     // long range
     std::vector<double> v_long_k, u_long_k, du_long_k;
     double v_long_k_0 = 0., v_long_r_0 = 0., du_long_k_0 = 0., du_long_r_0 = 0.;
@@ -439,6 +482,21 @@ struct Oracle {
         return u;
     }
 
+    // IlkkaPairAction::CalcU / CalcdUdBeta (ilkka_pair_action_class.h:83-107,126-150)
+    double ilkka_calc_u(double r0, double r1, double s, const NatSpline2& t, const NatSpline& lr, double lo,
+                        double hi) const {
+        double q = 0.5 * (r0 + r1);
+        double x = q + 0.5 * s;
+        double y = q - 0.5 * s;
+        double u = t.eval(x, y);
+        if (use_long_range) {
+            set_limits(lo, hi, r0, r1);
+            u -= 0.5 * lr.eval(r0);
+            u -= 0.5 * lr.eval(r1);
+        }
+        return u;
+    }
+
     // CalcU of the selected family at `level` (max_level = 0, so level = 0 here)
     double calc_u(double r0, double r1, double s, uint32_t level) const {
         if (pair_type == SPIMC_PAIR_BARE) {
@@ -446,10 +504,12 @@ struct Oracle {
             double level_tau = skip * tau;
             return level_tau * bare_calc_v(r0, r1);
         }
+        if (pair_type == SPIMC_PAIR_ILKKA) return ilkka_calc_u(r0, r1, s, u_xy, u_long_r_s, r_u_min, r_u_max);
         return david_calc_u(r0, r1, s, u_kj, false);
     }
     double calc_du_dbeta(double r0, double r1, double s) const {
         if (pair_type == SPIMC_PAIR_BARE) return bare_calc_v(r0, r1);  // bare_pair_action_class.h:175-177
+        if (pair_type == SPIMC_PAIR_ILKKA) return ilkka_calc_u(r0, r1, s, du_xy, du_long_r_s, r_du_min, r_du_max);
         // david_pair_action_class.h:126-168: du = 0.5(du_kj[1](r)+du_kj[1](r')) + V + off-diagonal
         double q = 0.5 * (r0 + r1), z = r0 - r1;
         double r_max = g_end;
@@ -473,8 +533,9 @@ struct Oracle {
         }
         return du;
     }
+    // CalcV; IlkkaPairAction::CalcV (ilkka_pair_action_class.h:34-54) is the Bare form
     double calc_v(double r0, double r1) const {
-        return pair_type == SPIMC_PAIR_BARE ? bare_calc_v(r0, r1) : david_calc_v(r0, r1);
+        return (pair_type == SPIMC_PAIR_BARE || pair_type == SPIMC_PAIR_ILKKA) ? bare_calc_v(r0, r1) : david_calc_v(r0, r1);
     }
 
     // CalcULong (bare_pair_action_class.h:153-172, david_pair_action_class.h:105-123)
@@ -499,7 +560,8 @@ struct Oracle {
     // david_pair_action_class.h:42-60,171-188)
     double calc_whole_long(bool dbeta) {
         std::vector<cplx>& rk = rho_mode();
-        const std::vector<double>& coef = (pair_type == SPIMC_PAIR_DAVID && dbeta) ? du_long_k : v_long_k;
+        const bool du = (pair_type == SPIMC_PAIR_DAVID || pair_type == SPIMC_PAIR_ILKKA) && dbeta;
+        const std::vector<double>& coef = du ? du_long_k : v_long_k;
         double tot = 0.;
         const long nkl = nk;
 #pragma omp parallel for collapse(2) reduction(+ : tot)
@@ -508,7 +570,7 @@ struct Oracle {
                 const cplx& a = rk[(size_t)b * nk + k];
                 tot += coef[k] * cmag2(a, a);
             }
-        if (pair_type == SPIMC_PAIR_DAVID && dbeta) return tot + du_long_k_0 + du_long_r_0;
+        if (du) return tot + du_long_k_0 + du_long_r_0;
         return tot + v_long_k_0 + v_long_r_0;
     }
 
@@ -585,8 +647,42 @@ struct Oracle {
         return tot;
     }
 
-    // PairAction::CalcGradientU (pair_action_class.h:135-155)
+    // PairAction::CalcGradientU (pair_action_class.h:135-155); Ilkka's analytic
+    // override (ilkka_pair_action_class.h:172-222)
     void calc_gradient_u(uint32_t b_i, uint32_t b_j, uint32_t p_i, uint32_t p_j, double* tot) {
+        if (pair_type == SPIMC_PAIR_ILKKA) {
+            // DrDrpDrrp with vectors (path_class.h:153-166): r = r_{p_j} - r_{p_i}
+            double rv[3], rpv[3], rrp[3];
+            const double *a0 = R(p_i, b_i), *a1 = R(p_j, b_i), *c0 = R(p_i, b_j), *c1 = R(p_j, b_j);
+            for (int d = 0; d < 3; ++d) {
+                rv[d] = a1[d] - a0[d];
+                rpv[d] = c1[d] - c0[d];
+            }
+            for (int d = 0; d < 3; ++d) {
+                rv[d] -= std::nearbyint(rv[d] * iL) * L;
+                rpv[d] += std::nearbyint((rv[d] - rpv[d]) * iL) * L;
+            }
+            for (int d = 0; d < 3; ++d) rrp[d] = rv[d] - rpv[d];
+            for (int d = 0; d < 3; ++d) rrp[d] -= std::nearbyint(rrp[d] * iL) * L;
+            double r_mag = mag(rv), r_p_mag = mag(rpv), r_r_p_mag = mag(rrp);
+            double q = 0.5 * (r_mag + r_p_mag);
+            double x = q + 0.5 * r_r_p_mag;
+            double y = q - 0.5 * r_r_p_mag;
+            double ux = 0., uy = 0.;
+            u_xy.eval(x, y, &ux, &uy);
+            double rh[3];
+            for (int d = 0; d < 3; ++d) {
+                rh[d] = r_mag == 0. ? 0. : rv[d] / r_mag;
+                double sh = r_r_p_mag == 0. ? 0. : rrp[d] / r_r_p_mag;
+                tot[d] = -0.5 * (ux * (rh[d] + sh) + uy * (rh[d] - sh));
+            }
+            if (use_long_range) {
+                set_limits(r_u_min, r_u_max, r_mag, r_p_mag);
+                double du_dr = u_long_r_s.deriv(r_mag);
+                for (int d = 0; d < 3; ++d) tot[d] -= 0.5 * du_dr * rh[d];
+            }
+            return;
+        }
         double* b = R(p_i, b_i);
         double r0[3] = {b[0], b[1], b[2]};
         const double eps = 1.e-4;
@@ -603,11 +699,13 @@ struct Oracle {
         }
     }
 
-    // PairAction::GetActionGradient(b, b+1, {p}, 0) (pair_action_class.h:305-337)
-    void pair_gradient_window(uint32_t b0, uint32_t b1, uint32_t p, double* out) {
+    // PairAction::GetActionGradient(b, b+1, {p}, 0) (pair_action_class.h:305-337);
+    // terms selects the link terms (SPIMC_GRAD_PAIR_SHORT) and the long-range
+    // term (SPIMC_GRAD_PAIR_LONG)
+    void pair_gradient_window(uint32_t b0, uint32_t b1, uint32_t p, double* out, uint32_t terms) {
         out[0] = out[1] = out[2] = 0.;
         if (pair_type == SPIMC_PAIR_NONE || N == 1) return;
-        for (uint32_t b_i = b0; b_i < b1; ++b_i) {
+        for (uint32_t b_i = b0; b_i < b1 && (terms & SPIMC_GRAD_PAIR_SHORT); ++b_i) {
             uint32_t b_j = b_i + 1;
             uint32_t b_k = b_i - 1 + M;
             for (uint32_t q = 0; q < N; ++q) {
@@ -619,7 +717,25 @@ struct Oracle {
                 for (int d = 0; d < 3; ++d) out[d] += g[d];
             }
         }
-        // long range: CalcGradientULong of the base class is zero (pair_action_class.h:158-165)
+        // long range: CalcGradientULong of the base class is zero (pair_action_class.h:158-165);
+        // Ilkka's (ilkka_pair_action_class.h:231-249) is added once per particle pair
+        // (pair_action_class.h:331-334)
+        if (pair_type == SPIMC_PAIR_ILKKA && use_long_range && (terms & SPIMC_GRAD_PAIR_LONG)) {
+            double g[3] = {0., 0., 0.};
+            std::vector<cplx> ri;
+            const std::vector<cplx>& rk = rho_mode();
+            for (uint32_t b_i = b0; b_i < b1; ++b_i) {
+                calc_rho_k(R(p, b_i), ri);
+                const cplx* rb = &rk[(size_t)bead_loop(b_i) * nk];
+                for (uint32_t k = 0; k < nk; ++k) {
+                    double w = u_long_k[k] * (ri[k].real() * rb[k].imag() - ri[k].imag() * rb[k].real());
+                    for (int d = 0; d < 3; ++d) g[d] += w * kn[3 * k + d] * box;
+                }
+            }
+            for (uint32_t q = 0; q < N; ++q)
+                if (q != p)
+                    for (int d = 0; d < 3; ++d) out[d] += g[d];
+        }
     }
 
     // ---- kinetic -----------------------------------------------------------
@@ -926,7 +1042,7 @@ struct Oracle {
                     double d[3], grad[3];
                     dr(R(p, b), R(p, 0), d);
                     if (energy_estimator == SPIMC_ENERGY_VIRIAL_COMPAT) {
-                        pair_gradient_window(b, b + 1, p, grad);
+                        pair_gradient_window(b, b + 1, p, grad, SPIMC_GRAD_PAIR_SHORT | SPIMC_GRAD_PAIR_LONG);
                     } else {
                         const double* gg = &g[3 * ((size_t)b * N + p)];
                         grad[0] = gg[0];
@@ -1128,6 +1244,33 @@ int spimc_oracle_create(const spimc_system_desc* s, uint64_t seed, void** out) {
             o->v_long_k_0 = vk0 * 0.5 * nn * nn * o->M;
             o->v_long_r_0 = vr0 * (-0.5) * nn * o->M;
         }
+    } else if (o->pair_type == SPIMC_PAIR_ILKKA) {
+        // IlkkaPairAction ctor (ilkka_pair_action_class.h:252-431)
+        o->u_xy.build(s->ilkka_x_u, s->ilkka_n_x_u, s->ilkka_y_u, s->ilkka_n_y_u, s->ilkka_u_xy);
+        o->du_xy.build(s->ilkka_x_du, s->ilkka_n_x_du, s->ilkka_y_du, s->ilkka_n_y_du, s->ilkka_du_xy);
+        o->is_coulomb = false;
+        o->v_r.build(s->ilkka_r_v, s->ilkka_v_r, s->ilkka_n_r_v, false);
+        o->r_v_min = s->ilkka_r_v[0];
+        o->r_v_max = s->ilkka_r_v[s->ilkka_n_r_v - 1];
+        if (o->use_long_range) {
+            o->u_long_r_s.build(s->ilkka_r_u_long, s->ilkka_u_long_r, s->ilkka_n_r_u_long, false);
+            o->r_u_min = s->ilkka_r_u_long[0];
+            o->r_u_max = s->ilkka_r_u_long[s->ilkka_n_r_u_long - 1];
+            o->du_long_r_s.build(s->ilkka_r_du_long, s->ilkka_du_long_r, s->ilkka_n_r_du_long, false);
+            o->r_du_min = s->ilkka_r_du_long[0];
+            o->r_du_max = s->ilkka_r_du_long[s->ilkka_n_r_du_long - 1];
+            o->v_long_r.build(s->ilkka_r_v_long, s->ilkka_v_long_r, s->ilkka_n_r_v_long, false);
+            o->r_v_long_min = s->ilkka_r_v_long[0];
+            o->r_v_long_max = s->ilkka_r_v_long[s->ilkka_n_r_v_long - 1];
+            o->u_long_k.assign(s->ilkka_u_long_k, s->ilkka_u_long_k + o->nk);
+            o->du_long_k.assign(s->ilkka_du_long_k, s->ilkka_du_long_k + o->nk);
+            o->v_long_k.assign(s->pair_v_long_k, s->pair_v_long_k + o->nk);
+            // homologous constants (ilkka_pair_action_class.h:418-423)
+            o->du_long_k_0 = s->ilkka_du_long_k_0 * 0.5 * nn * nn * o->M;
+            o->du_long_r_0 = s->ilkka_du_long_r_0 * (-0.5) * nn * o->M;
+            o->v_long_k_0 = s->pair_v_long_k_0 * 0.5 * nn * nn * o->M;
+            o->v_long_r_0 = s->pair_v_long_r_0 * (-0.5) * nn * o->M;
+        }
     } else if (o->pair_type != SPIMC_PAIR_NONE) {
         delete o;
         return fail(SPIMC_ERR_ARG, "unknown pair type");
@@ -1260,14 +1403,16 @@ int spimc_oracle_kinetic_virial_energy(void* h, uint32_t window, double* out) {
     return SPIMC_OK;
 }
 
-// compat=1: reference FD gradient per bead (pair_action_class.h:305-337);
+// compat=1: the reference's GetActionGradient per bead (pair_action_class.h:305-337),
+// link terms and/or long-range term per `terms`;
 // compat=0: central difference (step h) of the selected whole-path terms.
 int spimc_oracle_action_gradient(void* h, uint32_t terms, int compat, double fd_h, double* out) {
     Oracle* o = (Oracle*)h;
     o->mode = SPIMC_MODE_NEW;
     if (compat) {
         for (uint32_t b = 0; b < o->M; ++b)
-            for (uint32_t p = 0; p < o->N; ++p) o->pair_gradient_window(b, b + 1, p, out + 3 * ((size_t)b * o->N + p));
+            for (uint32_t p = 0; p < o->N; ++p)
+                o->pair_gradient_window(b, b + 1, p, out + 3 * ((size_t)b * o->N + p), terms);
         return SPIMC_OK;
     }
     o->gradient_fd(terms, fd_h, out);
@@ -1466,6 +1611,22 @@ int spimc_oracle_spline_eval(const double* x, const double* y, uint32_t n, const
         out[i] = s.eval(t[i]);
         if (dout) dout[i] = s.deriv(t[i]);
     }
+    return SPIMC_OK;
+}
+
+// Tensor natural spline through vals[ny][nx] at n points: values and partials.
+int spimc_oracle_spline2_eval(const double* x, uint32_t nx, const double* y, uint32_t ny, const double* vals,
+                              const double* px, const double* py, uint32_t n, double* out, double* gx, double* gy) {
+    NatSpline2 s;
+    s.build(x, nx, y, ny, vals);
+    for (uint32_t i = 0; i < n; ++i) out[i] = s.eval(px[i], py[i], gx ? gx + i : nullptr, gy ? gy + i : nullptr);
+    return SPIMC_OK;
+}
+
+// One link of the pair action from its lengths: which 0 CalcU, 1 CalcdUdBeta, 2 CalcV.
+int spimc_oracle_pair_link(void* h, int which, double r0, double r1, double s, double* out) {
+    Oracle* o = (Oracle*)h;
+    *out = which == 0 ? o->calc_u(r0, r1, s, 0) : which == 1 ? o->calc_du_dbeta(r0, r1, s) : o->calc_v(r0, r1);
     return SPIMC_OK;
 }
 

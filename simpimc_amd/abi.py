@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libspimc.so")
 OK = 0
 ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_NODEVICE = -1, -2, -3, -4, -5, -6
 
-PAIR_NONE, PAIR_BARE, PAIR_DAVID = 0, 1, 2
+PAIR_NONE, PAIR_BARE, PAIR_DAVID, PAIR_ILKKA = 0, 1, 2, 3
 ACTION_KINETIC, ACTION_PAIR = 0, 1
 MODE_OLD, MODE_NEW = 0, 1
 GRAD_KINETIC, GRAD_PAIR_SHORT, GRAD_PAIR_LONG, GRAD_ALL = 1, 2, 4, 7
@@ -64,6 +64,32 @@ class SystemDesc(C.Structure):
         ("pair_v_long_k", _dp),
         ("pair_v_long_k_0", C.c_double),
         ("pair_v_long_r_0", C.c_double),
+        ("ilkka_n_x_u", C.c_uint32),
+        ("ilkka_n_y_u", C.c_uint32),
+        ("ilkka_x_u", _dp),
+        ("ilkka_y_u", _dp),
+        ("ilkka_u_xy", _dp),
+        ("ilkka_n_x_du", C.c_uint32),
+        ("ilkka_n_y_du", C.c_uint32),
+        ("ilkka_x_du", _dp),
+        ("ilkka_y_du", _dp),
+        ("ilkka_du_xy", _dp),
+        ("ilkka_n_r_v", C.c_uint32),
+        ("ilkka_r_v", _dp),
+        ("ilkka_v_r", _dp),
+        ("ilkka_n_r_v_long", C.c_uint32),
+        ("ilkka_r_v_long", _dp),
+        ("ilkka_v_long_r", _dp),
+        ("ilkka_n_r_u_long", C.c_uint32),
+        ("ilkka_r_u_long", _dp),
+        ("ilkka_u_long_r", _dp),
+        ("ilkka_n_r_du_long", C.c_uint32),
+        ("ilkka_r_du_long", _dp),
+        ("ilkka_du_long_r", _dp),
+        ("ilkka_u_long_k", _dp),
+        ("ilkka_du_long_k", _dp),
+        ("ilkka_du_long_k_0", C.c_double),
+        ("ilkka_du_long_r_0", C.c_double),
     ]
 
 

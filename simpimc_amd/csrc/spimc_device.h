@@ -30,6 +30,16 @@ struct Spl {
     double inv_h;
 };
 
+// Tensor-product natural cubic spline on general grids x[nx] x y[ny]: cell
+// (i, j) holds s(x, y) = sum_{a,b} c[a][b] dx^a dy^b, dx = x - x[i], dy = y - y[j],
+// its 16 coefficients contiguous at c[(j * (nx - 1) + i) * 16 + 4 a + b].
+struct Spl2 {
+    const double* x;
+    const double* y;
+    const double* c;
+    int nx, ny;
+};
+
 // Uniform cubic B-spline (the natural cubic spline of a uniform grid in B-spline
 // form): coefficient j of component c at c[c * ncoef + j], ncoef = knots + 2;
 // s(t) = sum_{j=0..3} c[i + j] B_j(f) with u = (t - x0) / h, i = clamp(floor u), f = u - i.
@@ -115,10 +125,14 @@ struct Rep {
     RhoItem* ritems_dis;   // ... and of a displace round
     int n_ritems_bis, n_ritems_dis;
     const double* coef_s;  // per-slice action coefficient per k
-    const double* coef_b;  // per-slice beta-derivative / potential coefficient per k
+    const double* coef_b;  // per-slice beta-derivative coefficient per k
+    const double* coef_v;  // per-slice potential coefficient per k
     // pair tables
     Spl v_r, v_long_r;     // Bare
     Spl u, du;             // David, ncomp = n_val + 1
+    Spl2 u2, du2;          // Ilkka U(x, y), dU/dbeta(x, y); v_r and v_long_r hold its potential
+    Spl u_long_r, du_long_r;  // Ilkka long-range parts of U and dU/dbeta
+    double r_u_min, r_u_max, r_du_min, r_du_max;
     // LDS-staged B-spline form of the link action tables (uniform grids):
     // Bare: bt_a = v_r, bt_b = v_long_r; David: bt_a = u components 1..n_val.
     BTab bt_a, bt_b;
@@ -399,21 +413,73 @@ __device__ __forceinline__ double david_u(const Rep& R, const Spl& T, double r0,
     return u;
 }
 
+// Interval of t on the general grid x[n]: the largest i <= n - 2 with x[i] <= t,
+// 0 below the grid (binary search).
+__device__ __forceinline__ int grid_locate(const double* x, int n, double t) {
+    if (t <= x[0]) return 0;
+    if (t >= x[n - 1]) return n - 2;
+    int lo = 0, hi = n - 1;
+    while (hi - lo >= 2) {
+        const int mid = (hi + lo) >> 1;
+        if (x[mid] > t)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return lo;
+}
+
+// s(x, y) of a tensor spline; outside the grid the edge cell's polynomial
+// continues.  With gx/gy non-null also ds/dx, ds/dy.
+__device__ __forceinline__ double spl2_eval(const Spl2& S, double x, double y, double* gx = nullptr,
+                                            double* gy = nullptr) {
+    const int i = grid_locate(S.x, S.nx, x), j = grid_locate(S.y, S.ny, y);
+    const double dx = x - S.x[i], dy = y - S.y[j];
+    const double2* c = reinterpret_cast<const double2*>(S.c + ((size_t)j * (S.nx - 1) + i) * 16);
+    double a[4], ay[4];
+    for (int k = 0; k < 4; ++k) {
+        const double2 lo = c[2 * k], hi = c[2 * k + 1];
+        a[k] = lo.x + dy * (lo.y + dy * (hi.x + dy * hi.y));
+        ay[k] = lo.y + dy * (2. * hi.x + dy * (3. * hi.y));
+    }
+    if (gx) *gx = a[1] + dx * (2. * a[2] + dx * (3. * a[3]));
+    if (gy) *gy = ay[0] + dx * (ay[1] + dx * (ay[2] + dx * ay[3]));
+    return a[0] + dx * (a[1] + dx * (a[2] + dx * a[3]));
+}
+
+// IlkkaPairAction::CalcU / CalcdUdBeta (ilkka_pair_action_class.h:83-107,
+// 126-150): the tensor spline at x = q + s/2, y = q - s/2, less half the
+// long-range spline at r and r' (clamped to its grid) when use_long_range.
+__device__ __forceinline__ double ilkka_u(const Rep& R, const Spl2& T, const Spl& lr, double lo, double hi, double r0,
+                                          double r1, double s) {
+    const double q = 0.5 * (r0 + r1);
+    double u = spl2_eval(T, q + 0.5 * s, q - 0.5 * s);
+    if (R.use_long_range) {
+        clamp2(lo, hi, r0, r1);
+        u -= 0.5 * spl_eval1(lr, r0);
+        u -= 0.5 * spl_eval1(lr, r1);
+    }
+    return u;
+}
+
 // CalcU of the pair family at level 0 (bare_pair_action_class.h:146-150)
 __device__ __forceinline__ double pair_u(const Rep& R, double r0, double r1, double s) {
     if (R.pair_type == 1) return R.tau * bare_v(R, r0, r1);
+    if (R.pair_type == 3) return ilkka_u(R, R.u2, R.u_long_r, R.r_u_min, R.r_u_max, r0, r1, s);
     return david_u(R, R.u, r0, r1, s, false);
 }
 
 // CalcdUdBeta (bare_pair_action_class.h:175-177, david_pair_action_class.h:126-168)
 __device__ __forceinline__ double pair_du_dbeta(const Rep& R, double r0, double r1, double s) {
     if (R.pair_type == 1) return bare_v(R, r0, r1);
+    if (R.pair_type == 3) return ilkka_u(R, R.du2, R.du_long_r, R.r_du_min, R.r_du_max, r0, r1, s);
     return david_u(R, R.du, r0, r1, s, true);
 }
 
-// CalcV (bare_pair_action_class.h:99-123, david_pair_action_class.h:42-57)
+// CalcV (bare_pair_action_class.h:99-123, david_pair_action_class.h:42-57,
+// ilkka_pair_action_class.h:34-54: the Bare form with a tabulated v_r)
 __device__ __forceinline__ double pair_v(const Rep& R, double r0, double r1) {
-    if (R.pair_type == 1) return bare_v(R, r0, r1);
+    if (R.pair_type == 1 || R.pair_type == 3) return bare_v(R, r0, r1);
     clamp2(R.g_min, R.g_max, r0, r1);
     int i0 = spl_locate(R.u, r0), i1 = spl_locate(R.u, r1);
     return 0.5 * (spl_val(R.u, i0, 0, r0 - R.u.x[i0]) + spl_val(R.u, i1, 0, r1 - R.u.x[i1]));
@@ -602,6 +668,23 @@ __device__ __forceinline__ double pair_u_lds_s2(const Rep& R, const double* lds,
 __device__ __forceinline__ void pair_u_grad(const Rep& R, double r0, double r1, double s, double& ur, double& urp,
                                             double& us) {
     ur = urp = us = 0.;
+    if (R.pair_type == 3) {
+        // x = q + s/2, y = q - s/2, q = (r + r')/2
+        const double q = 0.5 * (r0 + r1);
+        double sx, sy;
+        spl2_eval(R.u2, q + 0.5 * s, q - 0.5 * s, &sx, &sy);
+        ur = urp = 0.5 * (sx + sy);
+        us = 0.5 * (sx - sy);
+        if (R.use_long_range) {
+            double a = r0, b = r1;
+            const bool la = (a >= R.r_u_min && a <= R.r_u_max), lb = (b >= R.r_u_min && b <= R.r_u_max);
+            clamp2(R.r_u_min, R.r_u_max, a, b);
+            const int ia = spl_locate(R.u_long_r, a), ib = spl_locate(R.u_long_r, b);
+            ur -= la ? 0.5 * spl_der(R.u_long_r, ia, 0, a - R.u_long_r.x[ia]) : 0.;
+            urp -= lb ? 0.5 * spl_der(R.u_long_r, ib, 0, b - R.u_long_r.x[ib]) : 0.;
+        }
+        return;
+    }
     if (R.pair_type == 1) {
         double a = r0, b = r1;
         bool ina = (a >= R.r_v_min && a <= R.r_v_max), inb = (b >= R.r_v_min && b <= R.r_v_max);

@@ -122,6 +122,36 @@ SplHost make_spline(const double* x, int n, int ncomp, const std::vector<const d
     return s;
 }
 
+// Tensor-product natural cubic spline through vals[ny][nx] on x[nx] x y[ny]
+// (einspline create_NUBspline_2d_d with NATURAL conditions on both axes,
+// ilkka_pair_action_class.h:276-278): a natural spline along x through each
+// row gives, per x interval, four polynomial coefficients at every y knot; a
+// natural spline along y through each of them gives the cell polynomials.
+struct Spl2Host {
+    std::vector<double> x, y, c;
+    int nx = 0, ny = 0;
+};
+
+Spl2Host make_spline2(const double* x, int nx, const double* y, int ny, const double* vals) {
+    Spl2Host s;
+    s.nx = nx;
+    s.ny = ny;
+    s.x.assign(x, x + nx);
+    s.y.assign(y, y + ny);
+    s.c.assign((size_t)(nx - 1) * (ny - 1) * 16, 0.);
+    std::vector<double> ax((size_t)ny * (nx - 1) * 4);  // [row j][interval i][power a]
+    for (int j = 0; j < ny; ++j) spline_coefs(x, vals + (size_t)j * nx, nx, 1, 0, ax.data() + (size_t)j * (nx - 1) * 4);
+    std::vector<double> col(ny), cy((size_t)(ny - 1) * 4);
+    for (int i = 0; i < nx - 1; ++i)
+        for (int a = 0; a < 4; ++a) {
+            for (int j = 0; j < ny; ++j) col[j] = ax[((size_t)j * (nx - 1) + i) * 4 + a];
+            spline_coefs(y, col.data(), ny, 1, 0, cy.data());
+            for (int j = 0; j < ny - 1; ++j)
+                for (int b = 0; b < 4; ++b) s.c[((size_t)j * (nx - 1) + i) * 16 + 4 * a + b] = cy[(size_t)j * 4 + b];
+        }
+    return s;
+}
+
 // FreeSpline tables (free_spline_class.h:25-63): image action or its tau derivative
 // on 10000 uniform points over [-L/2, L/2].
 SplHost make_free_spline(double L, int n_images, double lambda, double tau, bool dtau) {
@@ -187,8 +217,9 @@ struct RepHost {
     std::vector<int> kidx;
     std::vector<int4> k_runs;  // runs of consecutive n2 at fixed (n0, n1), ordered by (n0, n1)
     std::vector<int> k_order;  // k index of each k-vector in run order
-    std::vector<double> kmag, coef_s, coef_b;
-    SplHost v_r, v_long_r, u, du;
+    std::vector<double> kmag, coef_s, coef_b, coef_v;
+    SplHost v_r, v_long_r, u, du, u_long_r, du_long_r;
+    Spl2Host u2, du2;
     std::vector<SplHost> kin, bis;
     SplHost kin_dtau;
     Rep dev{};
@@ -240,7 +271,13 @@ uint64_t constants_hash(const RepHost& h) {
     hash_vec(x, h.kmag);
     hash_vec(x, h.coef_s);
     hash_vec(x, h.coef_b);
-    for (const SplHost* s : {&h.v_r, &h.v_long_r, &h.u, &h.du, &h.kin_dtau}) hash_spline(x, *s);
+    hash_vec(x, h.coef_v);
+    for (const SplHost* s : {&h.v_r, &h.v_long_r, &h.u, &h.du, &h.u_long_r, &h.du_long_r, &h.kin_dtau}) hash_spline(x, *s);
+    for (const Spl2Host* s : {&h.u2, &h.du2}) {
+        hash_vec(x, s->x);
+        hash_vec(x, s->y);
+        hash_vec(x, s->c);
+    }
     for (const auto& s : h.kin) hash_spline(x, s);
     for (const auto& s : h.bis) hash_spline(x, s);
     for (const BTabHost* t : {&h.bt_a, &h.bt_b, &h.bt_du}) hash_btab(x, *t);
@@ -248,11 +285,14 @@ uint64_t constants_hash(const RepHost& h) {
 }
 bool same_constants(const RepHost& a, const RepHost& b) {
     if (a.const_hash != b.const_hash || a.kidx != b.kidx || a.kmag != b.kmag || a.coef_s != b.coef_s ||
-        a.coef_b != b.coef_b || a.kin.size() != b.kin.size() || a.bis.size() != b.bis.size())
+        a.coef_b != b.coef_b || a.coef_v != b.coef_v || a.kin.size() != b.kin.size() || a.bis.size() != b.bis.size())
         return false;
     if (!same_spline(a.v_r, b.v_r) || !same_spline(a.v_long_r, b.v_long_r) || !same_spline(a.u, b.u) ||
-        !same_spline(a.du, b.du) || !same_spline(a.kin_dtau, b.kin_dtau))
+        !same_spline(a.du, b.du) || !same_spline(a.u_long_r, b.u_long_r) || !same_spline(a.du_long_r, b.du_long_r) ||
+        !same_spline(a.kin_dtau, b.kin_dtau))
         return false;
+    for (auto pr : {std::make_pair(&a.u2, &b.u2), std::make_pair(&a.du2, &b.du2)})
+        if (pr.first->x != pr.second->x || pr.first->y != pr.second->y || pr.first->c != pr.second->c) return false;
     for (size_t l = 0; l < a.kin.size(); ++l)
         if (!same_spline(a.kin[l], b.kin[l])) return false;
     for (size_t l = 0; l < a.bis.size(); ++l)
@@ -456,7 +496,27 @@ int validate(const spimc_system_desc& s, uint32_t r) {
         return bad(SPIMC_ERR_UNSUPPORTED, "bisect n_level must be in [1, min(6, floor(log2 n_bead))]");
     if (s.pair_max_level != 0) return bad(SPIMC_ERR_UNSUPPORTED, "pair action max_level must be 0");
     if (s.kinetic_n_images < 0 || s.bisect_n_images < 0) return bad(SPIMC_ERR_ARG, "n_images must be >= 0");
-    if (s.pair_type < SPIMC_PAIR_NONE || s.pair_type > SPIMC_PAIR_DAVID) return bad(SPIMC_ERR_ARG, "unknown pair type");
+    if (s.pair_type < SPIMC_PAIR_NONE || s.pair_type > SPIMC_PAIR_ILKKA) return bad(SPIMC_ERR_ARG, "unknown pair type");
+    if (s.pair_type == SPIMC_PAIR_ILKKA) {
+        if (!s.ilkka_x_u || !s.ilkka_y_u || !s.ilkka_u_xy || s.ilkka_n_x_u < 2 || s.ilkka_n_y_u < 2 || !s.ilkka_x_du ||
+            !s.ilkka_y_du || !s.ilkka_du_xy || s.ilkka_n_x_du < 2 || s.ilkka_n_y_du < 2)
+            return bad(SPIMC_ERR_ARG, "ilkka u/off_diag or du/off_diag table missing");
+        if (!s.ilkka_r_v || !s.ilkka_v_r || s.ilkka_n_r_v < 3) return bad(SPIMC_ERR_ARG, "ilkka v/diag table missing");
+        if (s.pair_use_long_range &&
+            (!s.ilkka_r_v_long || !s.ilkka_v_long_r || s.ilkka_n_r_v_long < 3 || !s.ilkka_r_u_long || !s.ilkka_u_long_r ||
+             s.ilkka_n_r_u_long < 3 || !s.ilkka_r_du_long || !s.ilkka_du_long_r || s.ilkka_n_r_du_long < 3 ||
+             !s.ilkka_u_long_k || !s.ilkka_du_long_k))
+            return bad(SPIMC_ERR_ARG, "ilkka long-range tables missing");
+        auto increasing = [](const double* x, uint32_t n) {
+            for (uint32_t i = 1; i < n; ++i)
+                if (!(x[i] > x[i - 1])) return false;
+            return true;
+        };
+        if (!increasing(s.ilkka_x_u, s.ilkka_n_x_u) || !increasing(s.ilkka_y_u, s.ilkka_n_y_u) ||
+            !increasing(s.ilkka_x_du, s.ilkka_n_x_du) || !increasing(s.ilkka_y_du, s.ilkka_n_y_du) ||
+            !increasing(s.ilkka_r_v, s.ilkka_n_r_v))
+            return bad(SPIMC_ERR_ARG, "ilkka grids must increase strictly");
+    }
     if (s.pair_type == SPIMC_PAIR_BARE) {
         if (!s.bare_r || !s.bare_v_r || s.bare_n_r < 3) return bad(SPIMC_ERR_ARG, "bare v/diag table missing");
         if (s.pair_use_long_range && (!s.bare_r_long || !s.bare_v_long_r || s.bare_n_r_long < 3))
@@ -512,6 +572,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     h.kmag.resize(nk);
     h.coef_s.resize(nk);
     h.coef_b.resize(nk);
+    h.coef_v.resize(nk);
     int mx = 0;
     for (int k = 0; k < nk; ++k) {
         int n0 = h.k_n[3 * k], n1 = h.k_n[3 * k + 1], n2 = h.k_n[3 * k + 2];
@@ -521,8 +582,14 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         h.kmag[k] = std::sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
         // Bare: CalcULong multiplies the k sum by level_tau (bare_pair_action_class.h:170);
         // David: u_long_k = v_long_k * tau (david_pair_action_class.h:519).
+        // Ilkka: u_long_k, du_long_k and v_long_k as read (ilkka_pair_action_class.h:63-80, 109-124, 152-168).
         h.coef_s[k] = (s.pair_type == SPIMC_PAIR_DAVID) ? s.pair_v_long_k[k] * tau : s.pair_v_long_k[k];
         h.coef_b[k] = s.pair_v_long_k[k];
+        h.coef_v[k] = s.pair_v_long_k[k];
+        if (s.pair_type == SPIMC_PAIR_ILKKA) {
+            h.coef_s[k] = s.ilkka_u_long_k[k];
+            h.coef_b[k] = s.ilkka_du_long_k[k];
+        }
     }
     // runs of the long-range adjoint: group by (n0, n1), split each group's sorted
     // n2 into maximal consecutive sequences, and walk each from the n2 nearest 0
@@ -579,6 +646,15 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         }
         h.u = make_spline(s.david_grid, s.david_n_grid, n_val + 1, yu);
         h.du = make_spline(s.david_grid, s.david_n_grid, n_val + 1, ydu);
+    } else if (s.pair_type == SPIMC_PAIR_ILKKA) {
+        h.u2 = make_spline2(s.ilkka_x_u, s.ilkka_n_x_u, s.ilkka_y_u, s.ilkka_n_y_u, s.ilkka_u_xy);
+        h.du2 = make_spline2(s.ilkka_x_du, s.ilkka_n_x_du, s.ilkka_y_du, s.ilkka_n_y_du, s.ilkka_du_xy);
+        h.v_r = make_spline(s.ilkka_r_v, s.ilkka_n_r_v, 1, {s.ilkka_v_r});
+        if (lr) {
+            h.v_long_r = make_spline(s.ilkka_r_v_long, s.ilkka_n_r_v_long, 1, {s.ilkka_v_long_r});
+            h.u_long_r = make_spline(s.ilkka_r_u_long, s.ilkka_n_r_u_long, 1, {s.ilkka_u_long_r});
+            h.du_long_r = make_spline(s.ilkka_r_du_long, s.ilkka_n_r_du_long, 1, {s.ilkka_du_long_r});
+        }
     }
     // LDS-staged B-spline tables for the pair kernel (uniform grids only)
     h.bt_a = BTabHost();
@@ -632,12 +708,23 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
             R.kmag = cv.take<double>(std::max(nk, 1));
             R.coef_s = cv.take<double>(std::max(nk, 1));
             R.coef_b = cv.take<double>(std::max(nk, 1));
+            R.coef_v = cv.take<double>(std::max(nk, 1));
             R.k_runs = cv.take<int4>(std::max<size_t>(h.k_runs.size(), 1));
             R.k_order = cv.take<int>(std::max(nk, 1));
             if (h.v_r.n) R.v_r = carve_spline(cv, h.v_r);
             if (h.v_long_r.n) R.v_long_r = carve_spline(cv, h.v_long_r);
             if (h.u.n) R.u = carve_spline(cv, h.u);
             if (h.du.n) R.du = carve_spline(cv, h.du);
+            if (h.u_long_r.n) R.u_long_r = carve_spline(cv, h.u_long_r);
+            if (h.du_long_r.n) R.du_long_r = carve_spline(cv, h.du_long_r);
+            for (auto pr : {std::make_pair(&R.u2, &h.u2), std::make_pair(&R.du2, &h.du2)})
+                if (pr.second->nx) {
+                    pr.first->nx = pr.second->nx;
+                    pr.first->ny = pr.second->ny;
+                    pr.first->x = cv.take<double>(pr.second->x.size());
+                    pr.first->y = cv.take<double>(pr.second->y.size());
+                    pr.first->c = cv.take<double>(pr.second->c.size());
+                }
             for (size_t l = 0; l < h.kin.size(); ++l) R.kin[l] = carve_spline(cv, h.kin[l]);
             if (h.kin_dtau.n) R.kin_dtau = carve_spline(cv, h.kin_dtau);
             for (size_t l = 0; l < h.bis.size(); ++l) R.bis[l] = carve_spline(cv, h.bis[l]);
@@ -674,7 +761,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     R.n_level = s.bisect_n_level;
     R.pair_type = s.pair_type;
     R.use_long_range = lr ? 1 : 0;
-    R.is_coulomb = s.bare_is_coulomb ? 1 : 0;
+    R.is_coulomb = (s.pair_type == SPIMC_PAIR_BARE && s.bare_is_coulomb) ? 1 : 0;
     R.n_order = s.david_n_order;
     R.n_val = n_val;
     R.kin_images = s.kinetic_n_images;
@@ -698,6 +785,9 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     // homologous constants (bare_pair_action_class.h:89-95, david_pair_action_class.h:533-543)
     R.const_v = lr ? s.pair_v_long_k_0 * 0.5 * nn * nn * M + s.pair_v_long_r_0 * (-0.5) * nn * M : 0.;
     R.const_du = R.const_v;
+    // Ilkka: du/diag constants for dU/dbeta (ilkka_pair_action_class.h:418-423)
+    if (s.pair_type == SPIMC_PAIR_ILKKA)
+        R.const_du = lr ? s.ilkka_du_long_k_0 * 0.5 * nn * nn * M + s.ilkka_du_long_r_0 * (-0.5) * nn * M : 0.;
     if (s.pair_type == SPIMC_PAIR_BARE) {
         R.r_v_min = s.bare_r[0];
         R.r_v_max = s.bare_r[s.bare_n_r - 1];
@@ -708,6 +798,17 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     } else if (s.pair_type == SPIMC_PAIR_DAVID) {
         R.g_min = s.david_grid[0];
         R.g_max = s.david_grid[s.david_n_grid - 1];
+    } else if (s.pair_type == SPIMC_PAIR_ILKKA) {
+        R.r_v_min = s.ilkka_r_v[0];
+        R.r_v_max = s.ilkka_r_v[s.ilkka_n_r_v - 1];
+        if (lr) {
+            R.r_l_min = s.ilkka_r_v_long[0];
+            R.r_l_max = s.ilkka_r_v_long[s.ilkka_n_r_v_long - 1];
+            R.r_u_min = s.ilkka_r_u_long[0];
+            R.r_u_max = s.ilkka_r_u_long[s.ilkka_n_r_u_long - 1];
+            R.r_du_min = s.ilkka_r_du_long[0];
+            R.r_du_max = s.ilkka_r_du_long[s.ilkka_n_r_du_long - 1];
+        }
     }
     R.key = splitmix64(seed ^ (0x9E3779B97F4A7C15ull * (r + 1)));
     auto set_bt = [](BTab& d, const BTabHost& hh) {
@@ -740,12 +841,17 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         R.kmag = O.kmag;
         R.coef_s = O.coef_s;
         R.coef_b = O.coef_b;
+        R.coef_v = O.coef_v;
         R.k_runs = O.k_runs;
         R.k_order = O.k_order;
         R.v_r = O.v_r;
         R.v_long_r = O.v_long_r;
         R.u = O.u;
         R.du = O.du;
+        R.u2 = O.u2;
+        R.du2 = O.du2;
+        R.u_long_r = O.u_long_r;
+        R.du_long_r = O.du_long_r;
         for (int l = 0; l < kMaxKinSplines; ++l) R.kin[l] = O.kin[l];
         R.kin_dtau = O.kin_dtau;
         for (int l = 0; l < kMaxKinSplines; ++l) R.bis[l] = O.bis[l];
@@ -755,6 +861,8 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         // the host copies are only compared against owners; drop the large ones
         h.u = SplHost();
         h.du = SplHost();
+        h.u2.c.clear();
+        h.du2.c.clear();
         h.bt_a.c.clear();
         h.bt_b.c.clear();
         h.bt_du.c.clear();
@@ -769,13 +877,21 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         HIPCHK(hipMemcpy((void*)R.kmag, h.kmag.data(), nk * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy((void*)R.coef_s, h.coef_s.data(), nk * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy((void*)R.coef_b, h.coef_b.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void*)R.coef_v, h.coef_v.data(), nk * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy((void*)R.k_runs, h.k_runs.data(), h.k_runs.size() * sizeof(int4), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy((void*)R.k_order, h.k_order.data(), nk * sizeof(int), hipMemcpyHostToDevice));
     }
     int rc;
     if ((rc = upload_spline(R.v_r, h.v_r)) || (rc = upload_spline(R.v_long_r, h.v_long_r)) ||
-        (rc = upload_spline(R.u, h.u)) || (rc = upload_spline(R.du, h.du)) || (rc = upload_spline(R.kin_dtau, h.kin_dtau)))
+        (rc = upload_spline(R.u, h.u)) || (rc = upload_spline(R.du, h.du)) || (rc = upload_spline(R.kin_dtau, h.kin_dtau)) ||
+        (rc = upload_spline(R.u_long_r, h.u_long_r)) || (rc = upload_spline(R.du_long_r, h.du_long_r)))
         return rc;
+    for (auto pr : {std::make_pair(&R.u2, &h.u2), std::make_pair(&R.du2, &h.du2)})
+        if (pr.second->nx) {
+            HIPCHK(hipMemcpy((void*)pr.first->x, pr.second->x.data(), pr.second->x.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy((void*)pr.first->y, pr.second->y.data(), pr.second->y.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy((void*)pr.first->c, pr.second->c.data(), pr.second->c.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
     for (size_t l = 0; l < h.kin.size(); ++l)
         if ((rc = upload_spline(R.kin[l], h.kin[l]))) return rc;
     for (size_t l = 0; l < h.bis.size(); ++l)
@@ -1123,16 +1239,24 @@ int est_pair_whole(spimc_ctx* c, Rep* base, int nrep, int which) {
     }
     if (rc || !c->any_long) return rc;
     dim3 g2((c->max_nk + 255) / 256, nrep);
-    return launch(c, SPIMC_KERNEL_ESTIMATORS, k_long_whole, g2, 256, 0, c->stream, base);
+    return launch(c, SPIMC_KERNEL_ESTIMATORS, k_long_whole, g2, 256, 0, c->stream, base, which);
 }
 int est_finalize(spimc_ctx* c, Rep* base, int nrep, int what, int acc) {
     return launch(c, SPIMC_KERNEL_ESTIMATORS, k_finalize, (nrep + 63) / 64, 64, 0, c->stream, base, nrep, what, acc);
 }
 int est_gradient(spimc_ctx* c, Rep* base, int nrep, uint32_t terms, int compat, int out_est) {
     dim3 g((c->max_N + 63) / 64, c->max_M, nrep);
-    if (compat) return launch(c, SPIMC_KERNEL_ADJOINT, k_grad_compat, g, 64, 0, c->stream, base, out_est);
     int rc;
-    if (c->lds_adj_max > 0) {
+    if (compat) {
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, k_grad_compat, g, 64, 0, c->stream, base, out_est,
+                    (terms & SPIMC_GRAD_PAIR_SHORT) ? 1 : 0);
+        bool ilkka_long = false;
+        for (int r = 0; r < nrep; ++r) {
+            const Rep& Rr = c->reps[(base - c->d_reps) + r].dev;
+            ilkka_long |= Rr.pair_type == SPIMC_PAIR_ILKKA && Rr.use_long_range;
+        }
+        if (!ilkka_long) return rc;
+    } else if (c->lds_adj_max > 0) {
         const size_t shmem = (size_t)c->lds_adj_max * sizeof(double);
         // workgroups per replica: enough to fill the GPU when the replicas are few
         const int bpr = std::max(1, std::min(c->max_M, (2 * kNumCUs + nrep - 1) / nrep));
@@ -1145,13 +1269,13 @@ int est_gradient(spimc_ctx* c, Rep* base, int nrep, uint32_t terms, int compat, 
     if (c->adj_runs_lds > 0) {
         dim3 gr(c->max_M, nrep);
         return launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long_runs, gr, kAdjRunBlock, c->adj_runs_lds, c->stream, base,
-                      out_est);
+                      out_est, compat);
     }
     int mx = 0;
     for (auto& h : c->reps) mx = std::max(mx, h.dev.max_idx);
     if (mx > kAdjMaxIdx) return fail(SPIMC_ERR_UNSUPPORTED, "adjoint long range supports |n| <= 40");
     const size_t shmem = (size_t)64 * 3 * (mx + 1) * sizeof(double2) + kAdjTile * (sizeof(double2) + sizeof(int) + sizeof(double));
-    return launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long, g, 64, shmem, c->stream, base, out_est);
+    return launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long, g, 64, shmem, c->stream, base, out_est, compat);
 }
 
 int energy_accumulate(spimc_ctx* c, Rep* base, int nrep) {
@@ -1167,7 +1291,7 @@ int energy_accumulate(spimc_ctx* c, Rep* base, int nrep) {
     if (rc) return rc;
     int from_sweep = 0;
     if (c->energy_estimator == SPIMC_ENERGY_VIRIAL_COMPAT) {
-        rc = est_gradient(c, base, nrep, SPIMC_GRAD_PAIR_SHORT, 1, 1);
+        rc = est_gradient(c, base, nrep, SPIMC_GRAD_PAIR_SHORT | SPIMC_GRAD_PAIR_LONG, 1, 1);
     } else if (c->adjoint_fresh) {
         from_sweep = 1;  // dS/dr of the sweep holds the kinetic term too; the centroid kernel removes it
     } else {
@@ -1648,25 +1772,33 @@ int spimc_action_gradient(spimc_ctx* c, uint32_t r, uint32_t terms, int compat, 
     int rc = check_rep(c, r);
     if (rc) return rc;
     if (!out) return fail(SPIMC_ERR_ARG, "null output");
-    if (compat && terms != SPIMC_GRAD_PAIR_SHORT)
-        return fail(SPIMC_ERR_ARG, "compat gradient covers the short-range pair action only");
     if (terms & ~SPIMC_GRAD_ALL) return fail(SPIMC_ERR_ARG, "unknown gradient terms");
+    if (compat && (terms & SPIMC_GRAD_KINETIC))
+        return fail(SPIMC_ERR_ARG, "compat gradient covers the pair action only");
     const Rep& R = c->reps[r].dev;
     Rep* base = c->d_reps + r;
     dim3 g((R.N + 63) / 64, R.M, 1);
+    // Long range.  Adjoint: d/dr of sum_k coef_s |rho_k|^2 per slice.  Compat:
+    // IlkkaPairAction::CalcGradientULong (ilkka_pair_action_class.h:231-249),
+    // sum_k u_long_k k Im(conj(rho_k) e^{i k.r}) -- half the adjoint term -- added
+    // once per particle pair (pair_action_class.h:331-334), i.e. (N - 1) times;
+    // zero for the Bare and David actions (pair_action_class.h:158-165).
+    const bool long_on = (terms & SPIMC_GRAD_PAIR_LONG) && R.use_long_range &&
+                         (!compat || R.pair_type == SPIMC_PAIR_ILKKA);
     if (compat) {
-        rc = launch(c, SPIMC_KERNEL_ADJOINT, k_grad_compat, g, 64, 0, c->stream, base, 0);
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, k_grad_compat, g, 64, 0, c->stream, base, 0,
+                    (terms & SPIMC_GRAD_PAIR_SHORT) ? 1 : 0);
     } else {
         rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_local, g, 64, 0, c->stream, base, terms, 0);
-        if (!rc && (terms & SPIMC_GRAD_PAIR_LONG) && R.use_long_range && c->adj_runs_lds > 0) {
-            rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long_runs, dim3(R.M, 1), kAdjRunBlock, c->adj_runs_lds,
-                        c->stream, base, 0);
-        } else if (!rc && (terms & SPIMC_GRAD_PAIR_LONG) && R.use_long_range) {
-            if (R.max_idx > kAdjMaxIdx) return fail(SPIMC_ERR_UNSUPPORTED, "adjoint long range supports |n| <= 40");
-            const size_t shmem = (size_t)64 * 3 * (R.max_idx + 1) * sizeof(double2) +
-                                 kAdjTile * (sizeof(double2) + sizeof(int) + sizeof(double));
-            rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long, g, 64, shmem, c->stream, base, 0);
-        }
+    }
+    if (!rc && long_on && c->adj_runs_lds > 0) {
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long_runs, dim3(R.M, 1), kAdjRunBlock, c->adj_runs_lds,
+                    c->stream, base, 0, compat);
+    } else if (!rc && long_on) {
+        if (R.max_idx > kAdjMaxIdx) return fail(SPIMC_ERR_UNSUPPORTED, "adjoint long range supports |n| <= 40");
+        const size_t shmem = (size_t)64 * 3 * (R.max_idx + 1) * sizeof(double2) +
+                             kAdjTile * (sizeof(double2) + sizeof(int) + sizeof(double));
+        rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_long, g, 64, shmem, c->stream, base, 0, compat);
     }
     if (rc || (rc = sync_and_resolve(c))) return rc;
     HIPCHK(hipMemcpy(out, R.grad, (size_t)3 * R.M * R.N * sizeof(double), hipMemcpyDeviceToHost));

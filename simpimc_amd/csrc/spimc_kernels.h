@@ -1022,8 +1022,10 @@ __global__ __launch_bounds__(256) void k_pair_whole(const Rep* __restrict__ reps
     if (threadIdx.x == 0) R.scratch[slot * R.M + b] = t;
 }
 
-// sum_k coef_b(k) sum_b |rho(b,k)|^2 (CalcVLong / CalcdUdBetaLong sums), per k chunk
-__global__ __launch_bounds__(256) void k_long_whole(const Rep* __restrict__ reps) {
+// sum_k coef(k) sum_b |rho(b,k)|^2 per k chunk, plus (chunk 0) the k=0 and r=0
+// constants: which 0 CalcdUdBetaLong (coef_b, const_du), 1 CalcVLong (coef_v,
+// const_v)
+__global__ __launch_bounds__(256) void k_long_whole(const Rep* __restrict__ reps, int which) {
     const Rep& R = reps[blockIdx.y];
     const int k = (int)(blockIdx.x * 256 + threadIdx.x);
     if ((int)blockIdx.x * 256 >= R.nk) return;
@@ -1033,11 +1035,11 @@ __global__ __launch_bounds__(256) void k_long_whole(const Rep* __restrict__ reps
             const double2 v = R.rho[((size_t)R.sel[b] * R.M + b) * R.nk + k];
             t += v.x * v.x + v.y * v.y;
         }
-        t *= R.coef_b[k];
+        t *= which ? R.coef_v[k] : R.coef_b[k];
     }
     __shared__ double sh[4];
     t = block_sum<256>(t, sh);
-    if (threadIdx.x == 0) R.scratch[scratch_long(R.M) + blockIdx.x] = t;
+    if (threadIdx.x == 0) R.scratch[scratch_long(R.M) + blockIdx.x] = t + (blockIdx.x == 0 ? (which ? R.const_v : R.const_du) : 0.);
 }
 
 // Kinetic::VirialEnergy (kinetic_class.h:48-102), thread per particle
@@ -1081,7 +1083,7 @@ __global__ __launch_bounds__(256) void k_kin_virial(const Rep* __restrict__ reps
 // ---------------------------------------------------------------------------
 // Reference-compatible PairAction::GetActionGradient(b, b+1, {p}, 0):
 // CalcGradientU by central difference, eps = 1e-4 (pair_action_class.h:135-155,305-337)
-__global__ __launch_bounds__(64) void k_grad_compat(const Rep* __restrict__ reps, int out_est) {
+__global__ __launch_bounds__(64) void k_grad_compat(const Rep* __restrict__ reps, int out_est, int short_on) {
     const Rep& R = reps[blockIdx.z];
     const int p = blockIdx.x * 64 + threadIdx.x, b = blockIdx.y;
     if (p >= R.N || b >= R.M) return;
@@ -1094,7 +1096,41 @@ __global__ __launch_bounds__(64) void k_grad_compat(const Rep* __restrict__ reps
         pk[d] = pos_at(R, d, bk, p);
     }
     const double eps = 1.e-4;
-    if (R.pair_type != 0 && N > 1)
+    if (!short_on) {
+    } else if (R.pair_type == 3 && N > 1) {
+        // IlkkaPairAction::CalcGradientU (ilkka_pair_action_class.h:172-222) per link,
+        // as the reference forms it from DrDrpDrrp's r = r_q - r_p (path_class.h:153-166):
+        // -(1/2)[U_x (r/|r| + (r - r')/|r - r'|) + U_y (r/|r| - (r - r')/|r - r'|)]
+        // - (1/2) u_long'(|r|) r/|r|, |r| and |r'| clamped to the u/diag grid.  The
+        // first term is dU/dr_p; the second has the opposite sign of d/dr_p of the
+        // -(1/2) u_long(|r|) term of CalcU.
+        for (int q = 0; q < N; ++q) {
+            if (q == p) continue;
+            double q0[3];
+            for (int d = 0; d < 3; ++d) q0[d] = pos_at(R, d, b, q);
+            for (int lk = 0; lk < 2; ++lk) {
+                const double* a1 = lk ? pk : pj;
+                double c1[3];
+                for (int d = 0; d < 3; ++d) c1[d] = pos_at(R, d, lk ? bk : bj, q);
+                LinkGeom gg;
+                link_geom(r0, q0, a1, c1, R.L, R.iL, gg);
+                const double qq = 0.5 * (gg.r + gg.rp);
+                double ux, uy;
+                spl2_eval(R.u2, qq + 0.5 * gg.s, qq - 0.5 * gg.s, &ux, &uy);
+                double dl = 0.;
+                if (R.use_long_range) {
+                    double a = gg.r, bb = gg.rp;
+                    clamp2(R.r_u_min, R.r_u_max, a, bb);
+                    const int ia = spl_locate(R.u_long_r, a);
+                    dl = spl_der(R.u_long_r, ia, 0, a - R.u_long_r.x[ia]);
+                }
+                for (int d = 0; d < 3; ++d) {
+                    const double rh = gg.r == 0. ? 0. : gg.rv[d] / gg.r, sh = gg.s == 0. ? 0. : gg.sv[d] / gg.s;
+                    g[d] += -0.5 * (ux * (rh + sh) + uy * (rh - sh)) - 0.5 * dl * rh;
+                }
+            }
+        }
+    } else if (R.pair_type != 0 && N > 1)
         for (int q = 0; q < N; ++q) {
             if (q == p) continue;
             double q0[3], qj[3], qk[3];
@@ -1355,11 +1391,21 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_pair_whole_lds(const Rep* __re
     }
 }
 
+// Factor on the long-range gradient: 1 for the adjoint; with compat, the
+// reference's GetActionGradient (pair_action_class.h:331-334) adds
+// IlkkaPairAction::CalcGradientULong (ilkka_pair_action_class.h:231-249), half
+// the adjoint term, once per particle pair, i.e. (N - 1) / 2; the Bare and David
+// actions add nothing (pair_action_class.h:158-165).
+__device__ __forceinline__ double long_grad_scale(const Rep& R, int compat) {
+    if (!compat) return 1.;
+    return R.pair_type == 3 ? 0.5 * (R.N - 1.) : 0.;
+}
+
 // Analytic adjoint, Ewald term: d/dr_i of long_scale sum_k coef_s |rho(b,k)|^2
 // = -2 long_scale sum_k coef_s k Im(conj(rho) e^{i k.r_i}).  Block = 64 beads of
 // one slice; rho(b,:) staged through LDS in k tiles; per-bead phase tables in LDS.
 constexpr int kAdjTile = 128;
-__global__ __launch_bounds__(64) void k_adjoint_long(const Rep* __restrict__ reps, int out_est) {
+__global__ __launch_bounds__(64) void k_adjoint_long(const Rep* __restrict__ reps, int out_est, int compat) {
     const Rep& R = reps[blockIdx.z];
     const int b = blockIdx.y;
     if (b >= R.M || (int)blockIdx.x * 64 >= R.N || !R.use_long_range || R.pair_type == 0) return;
@@ -1403,7 +1449,7 @@ __global__ __launch_bounds__(64) void k_adjoint_long(const Rep* __restrict__ rep
     }
     if (valid) {
         double* o = (out_est ? R.grad_est : R.grad) + 3 * ((size_t)b * R.N + i);
-        const double f = -2. * R.long_scale * R.box;
+        const double f = -2. * long_grad_scale(R, compat) * R.long_scale * R.box;
         o[0] += f * g[0];
         o[1] += f * g[1];
         o[2] += f * g[2];
@@ -1418,7 +1464,7 @@ __global__ __launch_bounds__(64) void k_adjoint_long(const Rep* __restrict__ rep
 // address (a broadcast).  Run starts step running phases e^{i n0 phi_0},
 // e^{i n1 phi_1}; the runs are ordered by (n0, n1) so those steps are short.
 constexpr int kAdjRunBlock = 256;
-__global__ __launch_bounds__(kAdjRunBlock) void k_adjoint_long_runs(const Rep* __restrict__ reps, int out_est) {
+__global__ __launch_bounds__(kAdjRunBlock) void k_adjoint_long_runs(const Rep* __restrict__ reps, int out_est, int compat) {
     const Rep& R = reps[blockIdx.y];
     const int b = blockIdx.x;
     if (b >= R.M || !R.use_long_range || R.pair_type == 0) return;
@@ -1434,7 +1480,7 @@ __global__ __launch_bounds__(kAdjRunBlock) void k_adjoint_long_runs(const Rep* _
     }
     for (int t = threadIdx.x; t < n_runs; t += kAdjRunBlock) runs[t] = R.k_runs[t];
     __syncthreads();
-    const double f = -2. * R.long_scale * R.box;
+    const double f = -2. * long_grad_scale(R, compat) * R.long_scale * R.box;
     for (int p = threadIdx.x; p < R.N; p += kAdjRunBlock) {
         double2 st[3];
         for (int d = 0; d < 3; ++d) {
@@ -1538,7 +1584,7 @@ __global__ void k_finalize(const Rep* __restrict__ reps, int n_reps, int what, i
             const double* lw = R.scratch + scratch_long(M);
             double l = 0.;
             for (int c = 0; c < (R.nk + 255) / 256; ++c) l += lw[c];
-            t += l + R.const_v;
+            t += l;
         }
         res[1] = t;
     } else if (what == 2) {

@@ -335,6 +335,81 @@ struct DavidPairAction : PairActionBase {
     }
 };
 
+// IlkkaPairAction loader (ilkka_pair_action_class.h:252-431)
+struct IlkkaPairAction : PairActionBase {
+    std::vector<double> x_u, y_u, u_xy, x_du, y_du, du_xy, r_v, v_r;
+    std::vector<double> r_v_long, v_long_r, r_u_long, u_long_r, r_du_long, du_long_r;
+    std::vector<double> u_long_k, du_long_k;
+    double v_long_k_0 = 0., v_long_r_0 = 0., du_long_k_0 = 0., du_long_r_0 = 0.;
+    IlkkaPairAction(const Input& in, Path& path) : PairActionBase(in, path) {
+        auto grid2 = [&](const std::string& g, const std::string& v, std::vector<double>& x, std::vector<double>& y,
+                         std::vector<double>& xy) {
+            const size_t n_x = (size_t)table.Scalar(g + "/off_diag/n_x"), n_y = (size_t)table.Scalar(g + "/off_diag/n_y");
+            x = table.Read(g + "/off_diag/x");
+            y = table.Read(g + "/off_diag/y");
+            if (x.size() != n_x || y.size() != n_y) throw std::runtime_error(g + "/off_diag: grid sizes differ from n_x, n_y");
+            xy = table.Matrix(g + "/off_diag/" + v, n_x, n_y);
+        };
+        grid2("u", "u_xy", x_u, y_u, u_xy);
+        grid2("du", "du_xy", x_du, y_du, du_xy);
+        r_v = table.Read("v/diag/r");
+        v_r = table.Read("v/diag/v_r");
+        if (use_long_range) {
+            r_u_long = table.Read("u/diag/r_long");
+            u_long_r = table.Read("u/diag/u_long_r");
+            r_du_long = table.Read("du/diag/r_long");
+            du_long_r = table.Read("du/diag/du_long_r");
+            du_long_r_0 = table.Scalar("du/diag/du_long_r_0");
+            du_long_k_0 = table.Scalar("du/diag/du_long_k_0");
+            r_v_long = table.Read("v/diag/r_long");
+            v_long_r = table.Read("v/diag/v_long_r");
+            v_long_r_0 = table.Scalar("v/diag/v_long_r_0");
+            v_long_k_0 = table.Scalar("v/diag/v_long_k_0");
+        }
+    }
+    void Fill(spimc_system_desc& d, Path& path) override {
+        d.pair_type = SPIMC_PAIR_ILKKA;
+        d.pair_use_long_range = use_long_range;
+        d.ilkka_n_x_u = (uint32_t)x_u.size();
+        d.ilkka_n_y_u = (uint32_t)y_u.size();
+        d.ilkka_x_u = x_u.data();
+        d.ilkka_y_u = y_u.data();
+        d.ilkka_u_xy = u_xy.data();
+        d.ilkka_n_x_du = (uint32_t)x_du.size();
+        d.ilkka_n_y_du = (uint32_t)y_du.size();
+        d.ilkka_x_du = x_du.data();
+        d.ilkka_y_du = y_du.data();
+        d.ilkka_du_xy = du_xy.data();
+        d.ilkka_n_r_v = (uint32_t)r_v.size();
+        d.ilkka_r_v = r_v.data();
+        d.ilkka_v_r = v_r.data();
+        if (use_long_range) {
+            d.ilkka_n_r_v_long = (uint32_t)r_v_long.size();
+            d.ilkka_r_v_long = r_v_long.data();
+            d.ilkka_v_long_r = v_long_r.data();
+            d.ilkka_n_r_u_long = (uint32_t)r_u_long.size();
+            d.ilkka_r_u_long = r_u_long.data();
+            d.ilkka_u_long_r = u_long_r.data();
+            d.ilkka_n_r_du_long = (uint32_t)r_du_long.size();
+            d.ilkka_r_du_long = r_du_long.data();
+            d.ilkka_du_long_r = du_long_r.data();
+            // per stored k: the table entry whose |k| equals it within 1e-8 (ilkka_pair_action_class.h:301-308)
+            MatchK(path, table.Read("u/diag/k"), table.Read("u/diag/u_long_k"), 1.);
+            u_long_k = v_long_k;
+            MatchK(path, table.Read("du/diag/k"), table.Read("du/diag/du_long_k"), 1.);
+            du_long_k = v_long_k;
+            MatchK(path, table.Read("v/diag/k"), table.Read("v/diag/v_long_k"), 1.);
+            d.ilkka_u_long_k = u_long_k.data();
+            d.ilkka_du_long_k = du_long_k.data();
+            d.pair_v_long_k = v_long_k.data();
+            d.pair_v_long_k_0 = v_long_k_0;
+            d.pair_v_long_r_0 = v_long_r_0;
+            d.ilkka_du_long_k_0 = du_long_k_0;
+            d.ilkka_du_long_r_0 = du_long_r_0;
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Events (events/event_class.h, events/moves/move_class.h, observables/*)
 // ---------------------------------------------------------------------------

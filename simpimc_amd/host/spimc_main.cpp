@@ -49,6 +49,9 @@ struct Simulation {
             } else if (type == "DavidPairAction") {
                 act = std::make_shared<DavidPairAction>(a, path);
                 action_names[1] = act->name;
+            } else if (type == "IlkkaPairAction") {
+                act = std::make_shared<IlkkaPairAction>(a, path);
+                action_names[1] = act->name;
             } else {
                 throw std::runtime_error("ERROR: Unrecognized Action, " + type);
             }
@@ -416,10 +419,12 @@ struct Simulation {
 
 }  // namespace spimc_host
 
-// `spimc --table-to-h5 in out [--tau-fastest]`: writes a pair-action table
+// `spimc --table-to-h5 in out [--transposed]`: writes a pair-action table
 // (text container or HDF5) as an HDF5 file with the same dataset names.  The
-// David u_kj / du_kj_dbeta cubes become 3-D datasets [n_tau][n_grid][n_val],
-// or [n_val][n_grid][n_tau] with --tau-fastest; everything else keeps its shape.
+// David u_kj / du_kj_dbeta cubes become 3-D datasets [n_tau][n_grid][n_val] and
+// the Ilkka u_xy / du_xy matrices 2-D datasets [n_y][n_x] (the reference's
+// memory order), or [n_val][n_grid][n_tau] and [n_x][n_y] with --transposed;
+// everything else keeps its shape.
 static int TableToH5(const std::string& in_path, const std::string& out_path, bool tau_fastest) {
 #if SPIMC_HAVE_HDF5
     spimc_host::TableFile t;
@@ -447,6 +452,19 @@ static int TableToH5(const std::string& in_path, const std::string& out_path, bo
                         for (size_t k = 0; k < n_tau; ++k) w[(i * n_grid + g) * n_tau + k] = c[i + n_val * (g + n_grid * k)];
                 o.Doubles(name, w, {n_val, n_grid, n_tau});
             }
+        } else if ((name == "u/off_diag/u_xy" || name == "du/off_diag/du_xy") &&
+                   t.Has(name.substr(0, name.find('/')) + "/off_diag/n_x")) {
+            const std::string g = name.substr(0, name.find('/'));
+            const size_t n_x = (size_t)t.Scalar(g + "/off_diag/n_x"), n_y = (size_t)t.Scalar(g + "/off_diag/n_y");
+            const std::vector<double> m = t.Matrix(name, n_x, n_y);  // x fastest
+            if (!tau_fastest) {
+                o.Doubles(name, m, {n_y, n_x});
+            } else {
+                std::vector<double> w(m.size());
+                for (size_t i = 0; i < n_x; ++i)
+                    for (size_t j = 0; j < n_y; ++j) w[i * n_y + j] = m[i + n_x * j];
+                o.Doubles(name, w, {n_x, n_y});
+            }
         } else if (d.ints.count(name)) {
             std::vector<long long> iv(v.begin(), v.end());
             o.Ints(name, iv);
@@ -469,17 +487,17 @@ static int TableToH5(const std::string& in_path, const std::string& out_path, bo
 int main(int argc, char** argv) {
     if (argc < 2) {
         std::cerr << "usage: spimc input.xml [--replicas R] [--no-h5]\n"
-                     "       spimc --table-to-h5 table_in table_out.h5 [--tau-fastest]"
+                     "       spimc --table-to-h5 table_in table_out.h5 [--transposed]"
                   << std::endl;
         return 2;
     }
     if (std::string(argv[1]) == "--table-to-h5") {
         if (argc < 4) {
-            std::cerr << "usage: spimc --table-to-h5 table_in table_out.h5 [--tau-fastest]" << std::endl;
+            std::cerr << "usage: spimc --table-to-h5 table_in table_out.h5 [--transposed]" << std::endl;
             return 2;
         }
         try {
-            return TableToH5(argv[2], argv[3], argc > 4 && std::string(argv[4]) == "--tau-fastest");
+            return TableToH5(argv[2], argv[3], argc > 4 && std::string(argv[4]) == "--transposed");
         } catch (const std::exception& e) {
             std::cerr << "ERROR: " << e.what() << std::endl;
             return 1;

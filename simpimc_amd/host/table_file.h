@@ -95,6 +95,24 @@ class TableFile {
         return out;
     }
 
+    // The (n_x, n_y) matrix `name` with x fastest, the memory order of the
+    // reference's mat(n_x, n_y) (ilkka_pair_action_class.h:269-272).  A 2-D dataset
+    // stored [n_y][n_x] is already in that order; one stored [n_x][n_y] is
+    // transposed (when n_x == n_y the first is taken).
+    std::vector<double> Matrix(const std::string& name, size_t n_x, size_t n_y) const {
+        const auto& v = Read(name);
+        if (v.size() != n_x * n_y)
+            throw std::runtime_error(file_ + ": " + name + " holds " + std::to_string(v.size()) + " values, expected " +
+                                     std::to_string(n_x) + " x " + std::to_string(n_y));
+        const auto& d = t_.dims.at(name);
+        const bool y_fastest = d.size() == 2 && d[0] == n_x && d[1] == n_y && n_x != n_y;
+        if (!y_fastest) return v;
+        std::vector<double> out(v.size());
+        for (size_t i = 0; i < n_x; ++i)
+            for (size_t j = 0; j < n_y; ++j) out[i + n_x * j] = v[i * n_y + j];
+        return out;
+    }
+
     const TableData& Data() const { return t_; }
 
    private:

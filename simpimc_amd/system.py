@@ -5,7 +5,9 @@ one replica's <System>/<Species>/<Action>/<Move> parameters, builds the
 k-vector set of KSpace::Setup, the Ewald-split pair-action tables in the
 layouts the reference loaders read (Bare: v/diag/*, bare_pair_action_class.h:39-86;
 David: u_kj_<n>/du_kj_dbeta_<n>/potential/long_range/squarer,
-david_pair_action_class.h:192-360), and initial paths (species_class.h:222-377).
+david_pair_action_class.h:192-360; Ilkka: u/off_diag, du/off_diag, u/diag,
+du/diag, v/diag, ilkka_pair_action_class.h:253-432), and initial paths
+(species_class.h:222-377).
 The synthetic tables stand in for pagen output, which is not available.
 """
 from __future__ import annotations
@@ -92,6 +94,8 @@ class System:
     v_long_k: Optional[np.ndarray] = None
     v_long_k_0: float = 0.0
     v_long_r_0: float = 0.0
+    # IlkkaPairAction knots: keys of ILKKA_KEYS (grids, values, per-k vectors, constants)
+    ilkka: Optional[dict] = None
     name: str = ""
 
     @property
@@ -146,6 +150,23 @@ class System:
         d.pair_v_long_k = arr(self.v_long_k)
         d.pair_v_long_k_0 = self.v_long_k_0
         d.pair_v_long_r_0 = self.v_long_r_0
+        if self.ilkka is not None:
+            t = self.ilkka
+            for g, x, y, v in (("u", "x_u", "y_u", "u_xy"), ("du", "x_du", "y_du", "du_xy")):
+                setattr(d, f"ilkka_n_x_{g}", len(t[x]))
+                setattr(d, f"ilkka_n_y_{g}", len(t[y]))
+                setattr(d, f"ilkka_{x}", arr(t[x]))
+                setattr(d, f"ilkka_{y}", arr(t[y]))
+                setattr(d, f"ilkka_{v}", arr(t[v]))  # [n_y][n_x], x fastest
+            for r, v in (("r_v", "v_r"), ("r_v_long", "v_long_r"), ("r_u_long", "u_long_r"), ("r_du_long", "du_long_r")):
+                if t.get(r) is not None:
+                    setattr(d, f"ilkka_n_{r}", len(t[r]))
+                    setattr(d, f"ilkka_{r}", arr(t[r]))
+                    setattr(d, f"ilkka_{v}", arr(t[v]))
+            d.ilkka_u_long_k = arr(t.get("u_long_k"))
+            d.ilkka_du_long_k = arr(t.get("du_long_k"))
+            d.ilkka_du_long_k_0 = t.get("du_long_k_0", 0.0)
+            d.ilkka_du_long_r_0 = t.get("du_long_r_0", 0.0)
         return d
 
 
@@ -233,6 +254,59 @@ def david_ewald(sys: System, alpha: float, n_grid: int = 2000, n_order: int = 2,
     # long_range/u_k holds 4 pi e^{-k^2/4a^2}/k^2 unnormalised; the loader divides by the volume
     sys.v_long_k, sys.v_long_k_0, sys.v_long_r_0 = ewald_k(sys.L, alpha, sys.k_n)
     return sys
+
+
+def _graded_grid(a: float, b: float, n: int, power: float = 1.6) -> np.ndarray:
+    """n strictly increasing knots on [a, b], denser near a (a general grid)."""
+    t = np.linspace(0.0, 1.0, n)
+    return a + (b - a) * (0.5 * t + 0.5 * t ** power)
+
+
+def _soft(q: np.ndarray, a: float) -> np.ndarray:
+    return 1.0 / np.sqrt(q * q + a * a)
+
+
+def ilkka_ewald(sys: System, alpha: float, n_x: int = 40, n_y: int = 30, n_r: int = 200, soft: float = 0.3,
+                eps: float = 1e-3) -> System:
+    """IlkkaPairAction tables on general (graded) grids for a soft-core Coulomb
+    pair with an Ewald split.  U(x, y) = tau v(q) + eps tau s^2 e^{-q} and
+    dU/dbeta(x, y) = v(q) + 2 eps s^2 e^{-q}, with q = (x + y)/2, s = x - y and
+    v(q) = 1/sqrt(q^2 + soft^2); x spans [0, sqrt(3) L + 0.1] and y [0,
+    sqrt(3) L/2 + 0.1], the range the minimum-image link geometry reaches, so the
+    tensor splines are never evaluated outside their grid.  v/diag v_r = v(r);
+    the long-range parts are erf(alpha r)/r (u/diag scaled by tau) on [0.01,
+    sqrt(3) L/2] with the Ewald k components of ewald_k (u_long_k = tau v_k)."""
+    tau = sys.tau
+    r_cap = math.sqrt(3.0) * sys.L / 2.0
+    x = _graded_grid(0.0, 2.0 * r_cap + 0.1, n_x)
+    y = _graded_grid(0.0, r_cap + 0.1, n_y, power=1.3)
+    X, Y = np.meshgrid(x, y, indexing="xy")  # [n_y][n_x], x fastest
+    q, sd = 0.5 * (X + Y), X - Y
+    u_xy = tau * _soft(q, soft) + eps * tau * sd * sd * np.exp(-q)
+    du_xy = _soft(q, soft) + 2.0 * eps * sd * sd * np.exp(-q)
+    r = _graded_grid(0.01, r_cap, n_r, power=1.2)
+    lr = _erf(alpha * r) / r
+    vk, vk0, vr0 = ewald_k(sys.L, alpha, sys.k_n)
+    sys.pair_type = abi.PAIR_ILKKA
+    sys.use_long_range = True
+    sys.v_long_k, sys.v_long_k_0, sys.v_long_r_0 = vk, vk0, vr0
+    sys.ilkka = {
+        "x_u": x, "y_u": y, "u_xy": u_xy, "x_du": x, "y_du": y, "du_xy": du_xy,
+        "r_v": r, "v_r": _soft(r, soft), "r_v_long": r, "v_long_r": lr,
+        "r_u_long": r, "u_long_r": tau * lr, "r_du_long": r, "du_long_r": lr,
+        "u_long_k": tau * vk, "du_long_k": vk, "du_long_k_0": vk0, "du_long_r_0": vr0,
+    }
+    return sys
+
+
+def config_ilkka(n_part: int = 32, n_bead: int = 32, r_s: float = 2.0, tau: float = 0.05, n_level: int = 3) -> System:
+    """The C1 electron gas with an IlkkaPairAction (soft-core Coulomb, Ewald alpha
+    = 5/L, |k| <= 5 2pi/L)."""
+    s = config_c1(n_part=n_part, n_bead=n_bead, r_s=r_s, tau=tau, n_level=n_level)
+    s.bare_r = s.bare_v_r = s.bare_r_long = s.bare_v_long_r = None
+    s.bare_is_coulomb = False
+    s.name = "HEG-Ilkka"
+    return ilkka_ewald(s, alpha=5.0 / s.L)
 
 
 # ---------------------------------------------------------------------------
@@ -351,6 +425,39 @@ def bare_table_datasets(sys: System) -> dict:
         "v/diag/n_k": len(uniq), "v/diag/k": mags[idx], "v/diag/v_long_k": sys.v_long_k[idx],
         "v/diag/v_long_k_0": sys.v_long_k_0,
     }
+
+
+def ilkka_table_datasets(sys: System) -> dict:
+    """u/off_diag, du/off_diag, u/diag, du/diag and v/diag of IlkkaPairAction
+    (ilkka_pair_action_class.h:262-410); off-diagonal values flat with x
+    fastest; the per-k datasets keyed by the distinct |k| of the system's k-set."""
+    t = sys.ilkka
+    mags = kmag(sys.L, sys.k_n)
+    uniq, idx = np.unique(np.round(mags, 12), return_index=True)
+    out = {}
+    for g, x, y, v in (("u", "x_u", "y_u", "u_xy"), ("du", "x_du", "y_du", "du_xy")):
+        out[f"{g}/off_diag/n_x"] = len(t[x])
+        out[f"{g}/off_diag/n_y"] = len(t[y])
+        out[f"{g}/off_diag/x"] = t[x]
+        out[f"{g}/off_diag/y"] = t[y]
+        out[f"{g}/off_diag/{v}"] = np.ascontiguousarray(t[v]).ravel()
+    for g, r, v, k, k0, r0 in (("u", "r_u_long", "u_long_r", t["u_long_k"], 0.0, 0.0),
+                                ("du", "r_du_long", "du_long_r", t["du_long_k"], t["du_long_k_0"], t["du_long_r_0"])):
+        out[f"{g}/diag/n_r_long"] = len(t[r])
+        out[f"{g}/diag/r_long"] = t[r]
+        out[f"{g}/diag/{g}_long_r"] = t[v]
+        out[f"{g}/diag/{g}_long_r_0"] = r0
+        out[f"{g}/diag/n_k"] = len(uniq)
+        out[f"{g}/diag/k"] = mags[idx]
+        out[f"{g}/diag/{g}_long_k"] = k[idx]
+        out[f"{g}/diag/{g}_long_k_0"] = k0
+    out.update({
+        "v/diag/n_r": len(t["r_v"]), "v/diag/r": t["r_v"], "v/diag/v_r": t["v_r"],
+        "v/diag/n_r_long": len(t["r_v_long"]), "v/diag/r_long": t["r_v_long"], "v/diag/v_long_r": t["v_long_r"],
+        "v/diag/v_long_r_0": sys.v_long_r_0, "v/diag/n_k": len(uniq), "v/diag/k": mags[idx],
+        "v/diag/v_long_k": sys.v_long_k[idx], "v/diag/v_long_k_0": sys.v_long_k_0,
+    })
+    return out
 
 
 def david_table_datasets(L: float, alpha: float, tau: float, k_n: np.ndarray, n_grid: int = 2000, n_order: int = 2,

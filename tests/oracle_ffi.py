@@ -53,6 +53,8 @@ SIG = {
     "spimc_oracle_obs_read": ([_c, _u32, _dp, _pu64], _i32),
     "spimc_oracle_sweep": ([_c, _u32, _u32], _i32),
     "spimc_oracle_spline_eval": ([_dp, _dp, _u32, _dp, _u32, _dp, _dp], _i32),
+    "spimc_oracle_spline2_eval": ([_dp, _u32, _dp, _u32, _dp, _dp, _dp, _u32, _dp, _dp, _dp], _i32),
+    "spimc_oracle_pair_link": ([_c, _i32, _d, _d, _d, _dp], _i32),
     "spimc_oracle_free_spline_log_rho": ([_d, _i32, _d, _d, _dp, _u32, _dp], _i32),
 }
 
@@ -92,6 +94,17 @@ def kspace_build(L, k_cut):
     out = np.zeros((n.value, 3), dtype=np.int32)
     check(lib().spimc_oracle_kspace_build(L, k_cut, out.ctypes.data_as(_ip), n.value, C.byref(n)))
     return out
+
+
+def spline2_eval(x, y, vals, px, py):
+    """Tensor natural spline through vals[ny][nx]: values and d/dx, d/dy at the points."""
+    x, y = np.ascontiguousarray(x, dtype=np.float64), np.ascontiguousarray(y, dtype=np.float64)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    px, py = np.ascontiguousarray(px, dtype=np.float64), np.ascontiguousarray(py, dtype=np.float64)
+    out, gx, gy = np.zeros(len(px)), np.zeros(len(px)), np.zeros(len(px))
+    check(lib().spimc_oracle_spline2_eval(dp(x), len(x), dp(y), len(y), dp(vals), dp(px), dp(py), len(px), dp(out),
+                                          dp(gx), dp(gy)))
+    return out, gx, gy
 
 
 def spline_eval(x, y, t):
@@ -153,6 +166,13 @@ class Oracle:
         arr = (C.c_uint32 * max(1, len(parts)))(*parts)
         out = C.c_double()
         check(self.L.spimc_oracle_get_action(self.h, action, mode, b0, b1, arr, len(parts), level, C.byref(out)))
+        return out.value
+
+    def pair_link(self, which, r0, r1, s):
+        """One link of the pair action from |r|, |r'|, |r - r'|: which 0 CalcU,
+        1 CalcdUdBeta, 2 CalcV."""
+        out = C.c_double()
+        check(self.L.spimc_oracle_pair_link(self.h, which, r0, r1, s, C.byref(out)))
         return out.value
 
     def d_action_d_beta(self, action):

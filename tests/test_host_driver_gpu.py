@@ -21,6 +21,8 @@ PATH_DUMP = '\n    <Observable name="path_dump" type="PathDump" />'
 
 BARE = ('<Action name="Coulomb" type="BarePairAction" file="{file}" species_a="e" species_b="e" is_coulomb="1" '
         'max_level="0" use_long_range="1" />')
+ILKKA = ('<Action name="Ilkka" type="IlkkaPairAction" file="{file}" species_a="e" species_b="e" max_level="0" '
+         'use_long_range="1" />')
 DAVID = ('<Action name="David" type="DavidPairAction" file="{file}" species_a="e" species_b="e" n_order="2" '
          'max_level="0" use_long_range="1" />')
 
@@ -172,7 +174,7 @@ def test_hdf5_table_and_output(tmp_path):
 @pytest.mark.skipif(h5dump.H5DUMP is None, reason="h5dump not installed")
 def test_david_hdf5_table_matches_text(tmp_path):
     """A David table whose cubes hold three taus, as text and as HDF5 with the
-    tau-fastest cube layout, drives identical runs."""
+    transposed cube layout, drives identical runs."""
     from test_tables_h5_cpu import david_three_taus
 
     assert os.path.exists(DRIVER), "build the host driver first"
@@ -180,7 +182,7 @@ def test_david_hdf5_table_matches_text(tmp_path):
     write_init(tmp_path, s)
     d, _, _ = david_three_taus(s)
     system.write_table(str(tmp_path / "david.table"), d)
-    convert(tmp_path, "david.table", "david.h5", "--tau-fastest")
+    convert(tmp_path, "david.table", "david.h5", "--transposed")
     txt = run_driver(tmp_path, s, DAVID.format(file="david.table"), "txt", 1)
     h5 = run_driver(tmp_path, s, DAVID.format(file="david.h5"), "h5", 1)
     assert txt == h5
@@ -242,3 +244,35 @@ def test_path_dump_and_restart(tmp_path):
             for r in range(R):
                 dims, pos = h5dump.dataset(str(tmp_path / f"{prefix}.{r}.h5"), "/Observables/path_dump/e/positions")
                 assert np.array_equal(e.download(r=r), pos.reshape(dims)[-1].transpose(2, 1, 0)), (parallel, r)
+
+
+@pytest.mark.skipif(h5dump.H5DUMP is None, reason="h5dump not installed")
+def test_ilkka_driver_matches_binding_and_hdf5_table(tmp_path):
+    """The IlkkaPairAction loader: the driver on the Ilkka tables (text, and HDF5
+    with transposed matrices) gives the chain the binding gives on the same system."""
+    assert os.path.exists(DRIVER), "build the host driver first"
+    s = system.config_ilkka()
+    write_init(tmp_path, s)
+    system.write_table(str(tmp_path / "ilkka.table"), system.ilkka_table_datasets(s))
+    convert(tmp_path, "ilkka.table", "ilkka.h5", "--transposed")
+    txt = run_driver(tmp_path, s, ILKKA.format(file="ilkka.table"), "txt", 1)
+    h5 = run_driver(tmp_path, s, ILKKA.format(file="ilkka.h5"), "h5", 1)
+    assert txt == h5
+    pos = system.random_path(s, seed=8)
+    nb = (s.n_part * s.n_bead) >> s.bisect_n_level
+    kc = (5.0 + 1e-9) * 2 * np.pi / s.L
+    with abi.Engine(s, seed=4242) as e:
+        e.upload(pos)
+        e.obs_configure(energy=abi.ENERGY_THERMAL, sk_k_cut=kc, gr_n_r=60)
+        for blk in txt:
+            for _ in range(3):
+                e.bisect(nb)
+                e.displace(s.n_part)
+                e.obs_accumulate(abi.OBS_ENERGY | abi.OBS_SK | abi.OBS_GR)
+            en, nm = e.obs_read(abi.OBS_ENERGY)
+            e.obs_reset(abi.OBS_ENERGY | abi.OBS_SK | abi.OBS_GR)
+            got = blk["Observables"]["Energy"][0]
+            assert got["n_measure"] == nm == 3
+            assert got["Kinetic"] == pytest.approx(en[0] / (s.n_bead * nm), rel=1e-14)
+            assert got["Ilkka"] == pytest.approx(en[1] / (s.n_bead * nm), rel=1e-14)
+            assert blk["Moves"]["Bisect"][0]["n_attempt"] == 3 * nb

@@ -1,10 +1,12 @@
 """CPU tests: the oracle pinned against closed forms, library routines and brute
 force, the host-side table logic, and the C ABI surface of the engine library.
 No test here launches device work."""
+import ctypes
 import json
 import math
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -303,7 +305,7 @@ def test_library_exports_every_declared_symbol():
     assert names == set(abi.SIGNATURES)
     for n in names:
         assert hasattr(lib, n)
-    assert lib.spimc_abi_version() == 1
+    assert lib.spimc_abi_version() == 2
 
 
 def test_context_creation_errors_are_reported():
@@ -317,3 +319,125 @@ def test_context_creation_errors_are_reported():
         abi.Engine(s).close()
     except abi.SpimcError as e:  # CPU-only machine
         assert e.code == abi.ERR_NODEVICE
+
+
+# ---------------------------------------------------------------------------
+# IlkkaPairAction: tensor natural splines, link actions, reference gradient
+# ---------------------------------------------------------------------------
+def _scipy_tensor(x, y, vals, px, py):
+    """The tensor natural spline through vals[ny][nx], evaluated column-first:
+    a natural spline along y through each column x_i, then along x."""
+    cols = [CubicSpline(y, vals[:, i], bc_type="natural") for i in range(len(x))]
+    v, gx, gy = [], [], []
+    for a, b in zip(px, py):
+        f = CubicSpline(x, [c(b) for c in cols], bc_type="natural")
+        fd = CubicSpline(x, [c(b, 1) for c in cols], bc_type="natural")
+        v.append(f(a))
+        gx.append(f(a, 1))
+        gy.append(fd(a))
+    return np.array(v), np.array(gx), np.array(gy)
+
+
+def test_spline2_interpolates_reproduces_bilinear_and_matches_scipy():
+    rng = np.random.default_rng(5)
+    x = np.cumsum(rng.uniform(0.05, 0.4, 12))
+    y = np.cumsum(rng.uniform(0.05, 0.4, 9)) - 0.3
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    # knots are interpolated
+    vals = rng.standard_normal((len(y), len(x)))
+    v, _, _ = of.spline2_eval(x, y, vals, X.ravel(), Y.ravel())
+    np.testing.assert_allclose(v, vals.ravel(), rtol=0, atol=1e-13)
+    # a bilinear function is reproduced everywhere, with its partials
+    f = 0.7 + 1.3 * X - 0.4 * Y + 2.1 * X * Y
+    px = rng.uniform(x[0] - 0.2, x[-1] + 0.2, 40)
+    py = rng.uniform(y[0] - 0.2, y[-1] + 0.2, 40)
+    v, gx, gy = of.spline2_eval(x, y, f, px, py)
+    np.testing.assert_allclose(v, 0.7 + 1.3 * px - 0.4 * py + 2.1 * px * py, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(gx, 1.3 + 2.1 * py, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(gy, -0.4 + 2.1 * px, rtol=1e-11, atol=1e-11)
+    # random data, inside and just outside the grid, against scipy's splines taken in the other order
+    v, gx, gy = of.spline2_eval(x, y, vals, px, py)
+    sv, sgx, sgy = _scipy_tensor(x, y, vals, px, py)
+    np.testing.assert_allclose(v, sv, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(gx, sgx, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(gy, sgy, rtol=1e-10, atol=1e-10)
+
+
+def _ilkka_small(long_range=True):
+    s = system.config_ilkka(n_part=8, n_bead=8, n_level=2)
+    if not long_range:
+        s.use_long_range = False
+    return s
+
+
+def test_ilkka_link_actions_match_scipy_splines():
+    s = _ilkka_small()
+    t = s.ilkka
+    o = Oracle(s, seed=1)
+    rng = np.random.default_rng(3)
+    S = CubicSpline  # natural splines of the 1-D tables
+    ul = S(t["r_u_long"], t["u_long_r"], bc_type="natural")
+    dul = S(t["r_du_long"], t["du_long_r"], bc_type="natural")
+    vr = S(t["r_v"], t["v_r"], bc_type="natural")
+    vl = S(t["r_v_long"], t["v_long_r"], bc_type="natural")
+    cl = lambda r, g: np.clip(r, g[0], g[-1])  # noqa: E731  SetLimits
+    for _ in range(25):
+        r0, r1 = rng.uniform(0.0, 0.9 * math.sqrt(3.0) * s.L / 2.0, 2)
+        sd = rng.uniform(0.0, 1.0) * abs(r0 - r1) + rng.uniform(0.0, 0.5)
+        q = 0.5 * (r0 + r1)
+        u2 = _scipy_tensor(t["x_u"], t["y_u"], t["u_xy"], [q + 0.5 * sd], [q - 0.5 * sd])[0][0]
+        du2 = _scipy_tensor(t["x_du"], t["y_du"], t["du_xy"], [q + 0.5 * sd], [q - 0.5 * sd])[0][0]
+        ru = cl(np.array([r0, r1]), t["r_u_long"])
+        rdu = cl(np.array([r0, r1]), t["r_du_long"])
+        want_u = u2 - 0.5 * ul(ru).sum()
+        want_du = du2 - 0.5 * dul(rdu).sum()
+        rv = cl(np.array([r0, r1]), t["r_v"])
+        want_v = 0.5 * vr(rv).sum() - 0.5 * vl(cl(rv, t["r_v_long"])).sum()
+        assert o.pair_link(0, r0, r1, sd) == pytest.approx(want_u, rel=1e-11, abs=1e-13)
+        assert o.pair_link(1, r0, r1, sd) == pytest.approx(want_du, rel=1e-11, abs=1e-13)
+        assert o.pair_link(2, r0, r1, sd) == pytest.approx(want_v, rel=1e-11, abs=1e-13)
+
+
+def test_ilkka_reference_gradient_pins():
+    """The reference's Ilkka gradient (analytic CalcGradientU, CalcGradientULong)
+    against central differences of the oracle's own actions: without the long
+    range the link gradient is dS/dr; the long-range radial term of
+    CalcGradientU enters with the opposite sign of its derivative; the k-space
+    term is (N - 1)/2 times the derivative of the long-range action.  The path
+    keeps every pair vector within half a box, where DrDrpDrrp gives the same
+    lengths for a link read forwards (GetAction) and backwards (the b - 1 link of
+    GetActionGradient)."""
+    pos = None
+    res = {}
+    for lr in (False, True):
+        s = _ilkka_small(lr)
+        if pos is None:
+            pos = system.random_path(s, seed=21, spread=0.1, cofactor=0.3)
+        o = Oracle(s, seed=2)
+        o.upload(pos)
+        res[lr] = (o.gradient(abi.GRAD_PAIR_SHORT, compat=True), o.gradient(abi.GRAD_PAIR_SHORT, h=1e-5),
+                   o.gradient(abi.GRAD_PAIR_LONG, compat=True), o.gradient(abi.GRAD_PAIR_LONG, h=1e-5), s.n_part)
+    cs0, fd0, cl0, _, _ = res[False]
+    cs1, fd1, cl1, fl1, n = res[True]
+    np.testing.assert_allclose(cs0, fd0, rtol=1e-6, atol=1e-8)
+    assert np.abs(cl0).max() == 0.0
+    d_comp, d_fd = cs1 - cs0, fd1 - fd0
+    assert np.abs(d_fd).max() > 1e-3
+    np.testing.assert_allclose(d_comp, -d_fd, rtol=1e-6, atol=1e-8)
+    np.testing.assert_allclose(cl1, 0.5 * (n - 1) * fl1, rtol=1e-6, atol=1e-8)
+
+
+def test_system_desc_mirror_matches_c_layout(tmp_path):
+    """The ctypes mirror of spimc_system_desc has the C struct's size and field offsets."""
+    names = [n for n, _ in abi.SystemDesc._fields_]
+    c_names = ["lambda" if n == "lam" else n for n in names]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "spimc.h"\nint main(void) {\n'
+                   '  printf("%zu\\n", sizeof(spimc_system_desc));\n'
+                   + "".join(f'  printf("%zu\\n", offsetof(spimc_system_desc, {n}));\n' for n in c_names)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(abi.SystemDesc)
+    assert got[1:] == [getattr(abi.SystemDesc, n).offset for n in names]

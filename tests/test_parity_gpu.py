@@ -30,6 +30,7 @@ def small_david(seed=7):
 SYSTEMS = {
     "bare_c1": lambda: system.config_c1(),
     "david_small": small_david,
+    "ilkka_c1": lambda: system.config_ilkka(),
 }
 
 
@@ -137,6 +138,22 @@ def test_gradient_compat(sysname):
     ref = o.gradient(abi.GRAD_PAIR_SHORT, compat=True)
     # the reference's central difference (eps 1e-4) amplifies rounding by 1/eps
     assert rel_err(got, ref) < 1e-9
+
+
+@pytest.mark.parametrize("terms", [abi.GRAD_PAIR_LONG, abi.GRAD_PAIR_SHORT | abi.GRAD_PAIR_LONG])
+def test_gradient_compat_long_range(sysname, terms):
+    """The reference's GetActionGradient with its long-range term: zero for Bare
+    and David, Ilkka's CalcGradientULong added once per particle pair."""
+    s, pos = make(sysname)
+    o = Oracle(s)
+    o.upload(pos)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        got = e.gradient(terms, compat=True)
+    ref = o.gradient(terms, compat=True)
+    assert rel_err(got, ref) < 1e-9
+    if terms == abi.GRAD_PAIR_LONG:
+        assert (np.abs(ref).max() > 0) == (s.pair_type == abi.PAIR_ILKKA)
 
 
 @pytest.mark.parametrize("terms", [abi.GRAD_KINETIC, abi.GRAD_PAIR_SHORT, abi.GRAD_PAIR_LONG, abi.GRAD_ALL])
