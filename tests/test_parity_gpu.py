@@ -261,6 +261,61 @@ def test_batched_replicas_match_single_chains(streams, fork, monkeypatch):
             assert np.array_equal(e.download(r=i), finals[i])
 
 
+def mixed_systems():
+    return [system.config_c1(n_part=16, n_bead=16), small_david(), system.config_ilkka(n_part=24, n_bead=8, n_level=2)]
+
+
+def test_mixed_pair_actions_in_one_context():
+    """Bare, David and Ilkka replicas of different sizes batched in one context
+    (the global-table kernels, pair type read per replica) replay each single
+    chain's tape bit for bit, and their estimators match the oracle."""
+    systems = mixed_systems()
+    tapes, refs, finals, energies = [], [], [], []
+    for i, s in enumerate(systems):
+        pos = system.random_path(s, seed=60 + i, spread=0.05)
+        o = Oracle(s, seed=200 + i)
+        o.upload(pos)
+        o.record(True)
+        refs.append((o.bisect(40), o.displace(8)))
+        tapes.append((pos, o.recorded()))
+        finals.append(o.download())
+        energies.append((o.d_action_d_beta(abi.ACTION_PAIR), o.potential(abi.ACTION_PAIR)))
+    with abi.Engine(systems, seed=1) as e:
+        for i, (pos, tape) in enumerate(tapes):
+            e.upload(pos, r=i)
+            e.tape_set(tape, r=i)
+        ab = e.bisect(40)
+        ad = e.displace(8)
+        for i in range(len(systems)):
+            assert np.array_equal(ab[i], refs[i][0]), i
+            assert np.array_equal(ad[i], refs[i][1]), i
+            assert np.array_equal(e.download(r=i), finals[i]), i
+            a, b = e.d_action_d_beta(abi.ACTION_PAIR, r=i), energies[i][0]
+            assert abs(a - b) <= RTOL * max(1.0, abs(b)), (i, a, b)
+            a, b = e.potential(abi.ACTION_PAIR, r=i), energies[i][1]
+            assert abs(a - b) <= RTOL * max(1.0, abs(b)), (i, a, b)
+
+
+def test_mixed_pair_actions_graph_sweep():
+    """The same mixed context swept eagerly and from captured graphs (device RNG)
+    ends in the same state, with the sweep's estimators and adjoint."""
+    systems = mixed_systems()
+    flags = abi.SWEEP_ADJOINT | abi.SWEEP_ENERGY | abi.SWEEP_SK | abi.SWEEP_GR
+    with abi.Engine(systems, seed=5) as e, abi.Engine(systems, seed=5) as g:
+        for i, s in enumerate(systems):
+            pos = system.random_path(s, seed=70 + i, spread=0.05)
+            e.upload(pos, r=i)
+            g.upload(pos, r=i)
+        for eng in (e, g):
+            eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, gr_n_r=50)
+        e.sweep(2, flags)
+        g.sweep(2, flags | abi.SWEEP_GRAPH)
+        for i in range(len(systems)):
+            assert np.array_equal(e.download(r=i), g.download(r=i)), i
+            assert np.array_equal(e.counters(r=i), g.counters(r=i)), i
+            assert np.array_equal(e.obs_read(abi.OBS_ENERGY, r=i)[0], g.obs_read(abi.OBS_ENERGY, r=i)[0]), i
+
+
 @pytest.mark.parametrize("streams,fork", [("1", "0"), ("2", "0"), ("2", "1")])
 def test_uneven_counts_and_graph_sweep(streams, fork, monkeypatch):
     systems = system.config_c4(seed=3, n_replicas=6)
