@@ -1735,7 +1735,7 @@ int spimc_d_action_d_beta(spimc_ctx* c, uint32_t r, int action, double* out) {
     }
     if ((rc = sync_and_resolve(c))) return rc;
     double res[2];
-    HIPCHK(hipMemcpy(res, R.scratch + scratch_res(R.M, std::max(R.nk, 1)), sizeof res, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(res, R.scratch + scratch_res(R.M, R.nk), sizeof res, hipMemcpyDeviceToHost));
     *out = action == SPIMC_ACTION_KINETIC ? res[0] : (R.pair_type == SPIMC_PAIR_NONE ? 0. : res[1]);
     return SPIMC_OK;
 }
@@ -1751,7 +1751,7 @@ int spimc_potential(spimc_ctx* c, uint32_t r, int action, double* out) {
     if ((rc = est_pair_whole(c, base, 1, 1)) || (rc = est_finalize(c, base, 1, 1, 0))) return rc;
     if ((rc = sync_and_resolve(c))) return rc;
     double res[2];
-    HIPCHK(hipMemcpy(res, R.scratch + scratch_res(R.M, std::max(R.nk, 1)), sizeof res, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(res, R.scratch + scratch_res(R.M, R.nk), sizeof res, hipMemcpyDeviceToHost));
     *out = res[1];
     return SPIMC_OK;
 }

@@ -493,3 +493,67 @@ def test_adjoint_at_full_size_on_sampled_beads():
         g = e.gradient(abi.GRAD_ALL)
     got = g[beads[:, 0], beads[:, 1]]
     assert rel_err(got, ref) < 1e-6
+
+
+def _edge(name):
+    """Small and degenerate inputs the reference accepts."""
+    if name == "one_particle":
+        return system.config_c1(n_part=1, n_bead=8, n_level=2)
+    if name == "two_slices":
+        return system.config_c1(n_part=4, n_bead=2, n_level=1)
+    if name == "window_is_path":  # 2^n_level = M: the bead walks visit nothing
+        return system.config_c1(n_part=6, n_bead=8, n_level=3)
+    if name == "no_long_range":
+        s = system.config_c1(n_part=8, n_bead=16, n_level=2)
+        s.use_long_range = False
+        return s
+    if name == "images":
+        s = system.config_c1(n_part=6, n_bead=8, n_level=2)
+        s.kinetic_n_images = 1
+        s.bisect_n_images = 1
+        return s
+    if name == "three_k_vectors":
+        s = system.config_c1(n_part=8, n_bead=8, n_level=2)
+        s.k_n = system.kspace(s.L, (1.0 + 1e-9) * 2 * np.pi / s.L)
+        return system.bare_coulomb_ewald(s, alpha=5.0 / s.L)
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["one_particle", "two_slices", "window_is_path", "no_long_range", "images",
+                                  "three_k_vectors"])
+def test_edge_configurations(name):
+    """Replay, estimators and gradients on edge inputs: one particle (no pairs),
+    two slices, a bisection window as long as the path, no long range, image
+    sums in the kinetic action and the sampling, and a three-vector k-set."""
+    s = _edge(name)
+    if name == "three_k_vectors":
+        assert s.n_k == 3
+    pos = system.random_path(s, seed=17, spread=0.1)
+    o = Oracle(s, seed=23)
+    o.upload(pos)
+    o.record(True)
+    n_b = max(1, (s.n_part * s.n_bead) >> s.bisect_n_level)
+    ref = [o.bisect(n_b), o.displace(s.n_part), o.bisect(n_b)]
+    tape = o.recorded()
+    with abi.Engine(s, seed=3) as e:
+        e.upload(pos)
+        e.tape_set(tape)
+        got = [e.bisect(n_b)[0], e.displace(s.n_part)[0], e.bisect(n_b)[0]]
+        for g, r in zip(got, ref):
+            assert np.array_equal(g, r), name
+        assert np.array_equal(e.download(), o.download())
+        assert np.array_equal(e.counters(), o.counters())
+        e.tape_set(None)
+        for action in (abi.ACTION_KINETIC, abi.ACTION_PAIR):
+            a, b = e.d_action_d_beta(action), o.d_action_d_beta(action)
+            assert abs(a - b) <= RTOL * max(1.0, abs(b)), (name, action, a, b)
+        a, b = e.potential(abi.ACTION_PAIR), o.potential(abi.ACTION_PAIR)
+        assert abs(a - b) <= RTOL * max(1.0, abs(b)), (name, a, b)
+        a, b = e.kinetic_virial_energy(0), o.kinetic_virial_energy(0)
+        assert abs(a - b) <= RTOL * max(1.0, abs(b)), (name, a, b)
+        assert rel_err(e.gradient(abi.GRAD_PAIR_SHORT, compat=True), o.gradient(abi.GRAD_PAIR_SHORT, compat=True)) < 1e-9
+        # with one particle the pair action is constant, so the reference never
+        # updates rho_k during moves (pair_action_class.h:267-272); the central
+        # difference is taken on rho_k of the current positions
+        o.rho_k_rebuild()
+        assert rel_err(e.gradient(abi.GRAD_ALL), o.gradient(abi.GRAD_ALL, h=1e-5)) < 1e-6
