@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--kernel-events", action="store_true",
                     help="time every launch with dispatch events instead of device clocks (slows the run; "
                          "gives the per-kernel breakdown)")
+    ap.add_argument("--no-graph-line", dest="graph_line", action="store_false",
+                    help="skip the second timing of the same sweeps replayed from captured hipGraphs")
     ap.add_argument("--copy-mib", type=int, default=4096,
                     help="size of the device-to-device copy timed first (the practical HBM ceiling for a "
                          "read-one-write-one stream, reported as hbm_copy_GBps; 0 skips it)")
@@ -185,6 +187,18 @@ def main():
                                       "of the timed region"),
                     "hbm_copy_GBps": copy_gbps, "frac_of_copy": (ach / copy_gbps) if copy_gbps else None}
     c0 = eng.counters(0)
+    # the same sweeps replayed from captured hipGraphs (16 rounds per graph, the
+    # fork and join included); one untimed sweep captures the graphs
+    graph_line = None
+    if a.graph_line and not a.graph and not a.kernel_events:
+        eng.sweep(1, flags | abi.SWEEP_GRAPH)
+        u0g = eng.bead_updates()
+        t0g = time.perf_counter()
+        for i in range(a.steps):
+            eng.sweep(1, flags | abi.SWEEP_GRAPH)
+        dtg = time.perf_counter() - t0g
+        graph_line = {"value": (eng.bead_updates() - u0g) / dtg, "ms_per_step": 1e3 * dtg / a.steps, "steps": a.steps}
+        print(f"graph replay: {graph_line['value']:.4g} bead updates/s", file=sys.stderr, flush=True)
     eng.close()
 
     plain = None
@@ -231,6 +245,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_single_chain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "graph_replay": graph_line,
         "rho_k_update_GBps": roofline["achieved"] if roofline else None,
         "hbm_copy_GBps": copy_gbps,
         "kernels": kernels,
