@@ -266,6 +266,12 @@ int spimc_action_gradient(spimc_ctx* ctx, uint32_t replica, uint32_t terms, int 
 
 /* ---- Move interface (events/moves/move_class.h:21-77) ----------------- */
 
+/* Bisect::Reset with adaptive="1" (events/moves/single_species_move/bisect/
+ * bisect_class.h:143-151): sets one replica's bisection level for the stages
+ * and sweeps that follow (1 <= n_level <= min(6, floor(log2 n_bead))).
+ * Synchronises the device. */
+int spimc_set_bisect_level(spimc_ctx* ctx, uint32_t replica, uint32_t n_level);
+
 /* Replay draws for one replica, in reference call order (post-transform):
  *   Bisect::Attempt (bisect_class.h:39-124): p, b0 (integers as doubles), then per
  *     level from n_level-1 down to 0: 3 standard normals per midpoint in path

@@ -1614,6 +1614,18 @@ int spimc_oracle_spline_eval(const double* x, const double* y, uint32_t n, const
     return SPIMC_OK;
 }
 
+// Bisect::Reset with adaptive="1" (bisect_class.h:143-151): the next level.
+int spimc_oracle_set_bisect_level(void* h, uint32_t n_level) {
+    Oracle* o = (Oracle*)h;
+    if (n_level < 1 || n_level > (uint32_t)std::floor(std::log2((double)o->M))) {
+        g_err = "bisect n_level out of range";
+        return SPIMC_ERR_ARG;
+    }
+    o->n_level = n_level;
+    o->n_bisect_beads = 1u << n_level;
+    return SPIMC_OK;
+}
+
 // Tensor natural spline through vals[ny][nx] at n points: values and partials.
 int spimc_oracle_spline2_eval(const double* x, uint32_t nx, const double* y, uint32_t ny, const double* vals,
                               const double* px, const double* py, uint32_t n, double* out, double* gx, double* gy) {

@@ -117,6 +117,7 @@ SIGNATURES = {
     "spimc_kinetic_virial_energy": ([_c, _u32, _u32, _dp], _i32),
     "spimc_action_gradient": ([_c, _u32, _u32, _i32, _dp], _i32),
     "spimc_tape_set": ([_c, _u32, _dp, _u64], _i32),
+    "spimc_set_bisect_level": ([_c, _u32, _u32], _i32),
     "spimc_tape_cursor": ([_c, _u32, _pu64], _i32),
     "spimc_bisect_stage": ([_c, _pu32, _u32, _pu8], _i32),
     "spimc_displace_stage": ([_c, _pu32, _u32, _pu8], _i32),
@@ -269,6 +270,9 @@ class Engine:
         return out
 
     # ---- moves ---------------------------------------------------------------
+    def set_bisect_level(self, n_level: int, r: int = 0):
+        check(self.lib.spimc_set_bisect_level(self.h, r, n_level))
+
     def tape_set(self, draws, r: int = 0):
         if draws is None:
             check(self.lib.spimc_tape_set(self.h, r, None, 0))

@@ -681,8 +681,11 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
             h.kin.push_back(make_free_spline(L, s.kinetic_n_images, s.lambda, tau * (double)(1 << l), false));
         h.kin_dtau = make_free_spline(L, s.kinetic_n_images, s.lambda, tau, true);
     }
+    // the sampling splines of every level the bisection may use (Bisect::SetupSpline,
+    // bisect_class.h:155-162: floor(log2 M) of them; at most kMaxLevel here), so that
+    // spimc_set_bisect_level needs no new tables
     if (s.bisect_n_images > 0)
-        for (int l = 0; l < (int)s.bisect_n_level; ++l)
+        for (int l = 0; l < std::min((int)std::floor(std::log2((double)M)), kMaxLevel); ++l)
             h.bis.push_back(make_free_spline(L, s.bisect_n_images, s.lambda, 0.5 * tau * (double)(1 << l), false));
     h.n_chunks = std::max(1, (nk + kRhoBlock * c->rho_kpt - 1) / (kRhoBlock * c->rho_kpt));
 
@@ -1804,6 +1807,36 @@ int spimc_action_gradient(spimc_ctx* c, uint32_t r, uint32_t terms, int compat, 
     HIPCHK(hipMemcpy(out, R.grad, (size_t)3 * R.M * R.N * sizeof(double), hipMemcpyDeviceToHost));
     if (!compat && terms == SPIMC_GRAD_ALL) c->adjoint_fresh = true;
     return SPIMC_OK;
+}
+
+// Bisect::Reset with adaptive="1" (bisect_class.h:143-151): the replica's
+// bisection level for the following stages; the work lists are rebuilt and the
+// captured graphs dropped.
+int spimc_set_bisect_level(spimc_ctx* c, uint32_t r, uint32_t n_level) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    RepHost& h = c->reps[r];
+    const int max_lvl = std::min((int)std::floor(std::log2((double)h.dev.M)), kMaxLevel);
+    if (n_level < 1 || (int)n_level > max_lvl)
+        return fail(SPIMC_ERR_UNSUPPORTED, "bisect n_level must be in [1, min(6, floor(log2 n_bead))]");
+    if ((int)n_level == h.dev.n_level) return SPIMC_OK;
+    HIPCHK(hipDeviceSynchronize());
+    h.dev.n_level = (int)n_level;
+    h.d.bisect_n_level = n_level;
+    for (auto& G : c->groups) {
+        if ((int)r < G.r0 || (int)r >= G.r0 + G.n) continue;
+        for (void** p : {(void**)&G.d_pair_bis, (void**)&G.d_pair_dis, (void**)&G.d_ritem_bis, (void**)&G.d_ritem_dis})
+            if (*p) {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
+        G.max_bis_links = 0;
+        for (int q = G.r0; q < G.r0 + G.n; ++q)
+            G.max_bis_links = std::max(G.max_bis_links, 1 << c->reps[q].dev.n_level);
+        if ((rc = build_work_lists(c, G))) return rc;
+    }
+    drop_graphs(c);
+    return upload_reps(c);
 }
 
 // ---- Move interface ----------------------------------------------------------

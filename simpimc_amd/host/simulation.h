@@ -471,15 +471,40 @@ struct Move : Event {
 
 // Bisect (bisect_class.h:167-190)
 struct Bisect : Move {
+    bool adaptive = false;
+    double target_ratio = 0.;
+    uint32_t n_bead = 0, n_level0 = 0;
+    std::vector<uint32_t> n_level;  // per replica, the level of the running block
+    bool first_write = true;
     Bisect(const Input& in, Engine& e, spimc_system_desc& d, const Path& path) : Move(in, e) {
-        d.bisect_n_level = in.GetAttribute<uint32_t>("n_level");
+        d.bisect_n_level = n_level0 = in.GetAttribute<uint32_t>("n_level");
         d.bisect_n_images = in.GetAttribute<int>("n_images", 0);
-        if (in.GetAttribute<bool>("adaptive", false))
-            throw std::runtime_error("adaptive bisection changes n_level between blocks; not supported");
-        (void)path;
+        adaptive = in.GetAttribute<bool>("adaptive", false);
+        if (adaptive) target_ratio = in.GetAttribute<double>("target_ratio");
+        n_bead = path.n_bead;
     }
     int Stage(uint32_t n) override { return spimc_bisect_stage(eng.ctx, nullptr, n, nullptr); }
     int CounterBase() const override { return 0; }
+    // Bisect::Reset with adaptive="1" (bisect_class.h:143-151), after the first
+    // write: one level down when the block's acceptance is below target_ratio,
+    // else one up while 2^n_level < n_bead / 2; each replica on its own counts
+    void Adapt(const std::vector<Counts>& b) {
+        n_level.resize(eng.n_replicas, n_level0);
+        if (!adaptive || first_write) {
+            first_write = false;
+            return;
+        }
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) {
+            const double ratio = (double)b[r].n_accept / (double)b[r].n_attempt;
+            uint32_t lv = n_level[r];
+            if (ratio < target_ratio && lv > 1)
+                --lv;
+            else if ((1u << lv) < n_bead / 2)
+                ++lv;
+            if (lv != n_level[r]) check(spimc_set_bisect_level(eng.ctx, r, lv), "adaptive bisection level");
+            n_level[r] = lv;
+        }
+    }
 };
 
 // DisplaceParticle (displace_particle_class.h:93)

@@ -203,18 +203,32 @@ struct Simulation {
         return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     }
 
-    // Move::Write (move_class.h:80-94)
+    // Move::Write (move_class.h:80-94); Bisect::Write first records the block's
+    // level and adapts it (bisect_class.h:193-211)
     void WriteMove(Move& m) {
         const auto b = m.Block();
+        auto* bis = dynamic_cast<Bisect*>(&m);
+        std::vector<uint32_t> level;
+        if (bis) {
+            bis->n_level.resize(eng.n_replicas, bis->n_level0);
+            level = bis->n_level;
+            bis->Adapt(b);
+        }
         out << "\"" << m.name << "\": [";
         for (uint32_t r = 0; r < eng.n_replicas; ++r) {
-            char s[160];
-            snprintf(s, sizeof s, "%s{\"n_attempt\": %llu, \"n_accept\": %llu, \"x\": %.10g}", r ? ", " : "",
+            char s[200];
+            snprintf(s, sizeof s, "%s{\"n_attempt\": %llu, \"n_accept\": %llu, \"x\": %.10g", r ? ", " : "",
                      (unsigned long long)b[r].n_attempt, (unsigned long long)b[r].n_accept,
                      b[r].n_attempt ? (double)b[r].n_accept / b[r].n_attempt : 0.);
             out << s;
+            if (bis && bis->adaptive) out << ", \"n_level\": " << level[r];
+            out << "}";
         }
         out << "]";
+#if SPIMC_HAVE_HDF5
+        if (bis && bis->adaptive)
+            for (uint32_t r = 0; r < h5.size(); ++r) h5[r]->AppendInt("Moves/" + m.name + "/n_level", level[r]);
+#endif
 #if SPIMC_HAVE_HDF5
         for (uint32_t r = 0; r < h5.size(); ++r) {
             const std::string g = "Moves/" + m.name + "/";

@@ -55,6 +55,7 @@ SIG = {
     "spimc_oracle_spline_eval": ([_dp, _dp, _u32, _dp, _u32, _dp, _dp], _i32),
     "spimc_oracle_spline2_eval": ([_dp, _u32, _dp, _u32, _dp, _dp, _dp, _u32, _dp, _dp, _dp], _i32),
     "spimc_oracle_pair_link": ([_c, _i32, _d, _d, _d, _dp], _i32),
+    "spimc_oracle_set_bisect_level": ([_c, _u32], _i32),
     "spimc_oracle_free_spline_log_rho": ([_d, _i32, _d, _d, _dp, _u32, _dp], _i32),
 }
 
@@ -167,6 +168,9 @@ class Oracle:
         out = C.c_double()
         check(self.L.spimc_oracle_get_action(self.h, action, mode, b0, b1, arr, len(parts), level, C.byref(out)))
         return out.value
+
+    def set_bisect_level(self, n_level):
+        check(self.L.spimc_oracle_set_bisect_level(self.h, n_level))
 
     def pair_link(self, which, r0, r1, s):
         """One link of the pair action from |r|, |r'|, |r - r'|: which 0 CalcU,

@@ -276,3 +276,62 @@ def test_ilkka_driver_matches_binding_and_hdf5_table(tmp_path):
             assert got["Kinetic"] == pytest.approx(en[0] / (s.n_bead * nm), rel=1e-14)
             assert got["Ilkka"] == pytest.approx(en[1] / (s.n_bead * nm), rel=1e-14)
             assert blk["Moves"]["Bisect"][0]["n_attempt"] == 3 * nb
+
+
+@pytest.mark.skipif(h5dump.H5DUMP is None, reason="h5dump not installed")
+def test_adaptive_bisection(tmp_path):
+    """<Move type="Bisect" adaptive="1" target_ratio=...>: from the second write on,
+    each replica's level steps down when its block acceptance is below the
+    target and up otherwise (bisect_class.h:143-151); the driver's chain equals the
+    binding's with the same level sequence, and the levels are written per block."""
+    assert os.path.exists(DRIVER), "build the host driver first"
+    s = system.config_c1()
+    write_init(tmp_path, s)
+    system.write_table(str(tmp_path / "bare.table"), system.bare_table_datasets(s))
+    target = 0.985
+    adapt = lambda x: x.replace('<Move name="Bisect" type="Bisect" species="e" n_level="3" />',  # noqa: E731
+                                f'<Move name="Bisect" type="Bisect" species="e" n_level="3" adaptive="1" '
+                                f'target_ratio="{target}" />').replace('<Loop n_step="2">', '<Loop n_step="5">')
+    kc = (5.0 + 1e-9) * 2 * np.pi / s.L
+    nb = (s.n_part * s.n_bead) >> s.bisect_n_level
+    xml = tmp_path / "ad.xml"
+    xml.write_text(adapt(XML.format(seed=4242, M=s.n_bead, beta=s.beta, L=s.L, kc=kc, N=s.n_part, nb=nb, prefix="ad",
+                                    action=BARE.format(file="bare.table"), init=INIT_FILE, extra="")))
+    res = subprocess.run([DRIVER, xml.name, "--replicas", "2"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    blocks = [json.loads(line) for line in open(tmp_path / "ad.spimc.jsonl")]
+    assert len(blocks) == 5
+    pos = system.random_path(s, seed=8)
+    with abi.Engine([s, s], seed=4242) as e:
+        for r in range(2):
+            e.upload(pos, r=r)
+        e.obs_configure(energy=abi.ENERGY_THERMAL, sk_k_cut=kc, gr_n_r=60)
+        level = [3, 3]
+        c_prev = [e.counters(r=r) for r in range(2)]
+        for k, blk in enumerate(blocks):
+            for _ in range(3):
+                e.bisect(nb)
+                e.displace(s.n_part)
+                e.obs_accumulate(abi.OBS_ENERGY | abi.OBS_SK | abi.OBS_GR)
+            for r in range(2):
+                en, nm = e.obs_read(abi.OBS_ENERGY, r=r)
+                got = blk["Observables"]["Energy"][r]
+                assert got["Kinetic"] == pytest.approx(en[0] / (s.n_bead * nm), rel=1e-14), (k, r)
+                mv = blk["Moves"]["Bisect"][r]
+                assert mv["n_level"] == level[r], (k, r)
+                c = e.counters(r=r)
+                att, acc = c[0] - c_prev[r][0], c[1] - c_prev[r][1]
+                c_prev[r] = c
+                assert (mv["n_attempt"], mv["n_accept"]) == (att, acc)
+                if k > 0:
+                    if acc / att < target and level[r] > 1:
+                        level[r] -= 1
+                    elif (1 << level[r]) < s.n_bead // 2:
+                        level[r] += 1
+                    e.set_bisect_level(level[r], r=r)
+            e.obs_reset(abi.OBS_ENERGY | abi.OBS_SK | abi.OBS_GR)
+    levels = [b["Moves"]["Bisect"][0]["n_level"] for b in blocks]
+    assert len(set(levels)) > 1, levels  # the level did change
+    for r in range(2):
+        got = h5dump.dataset(str(tmp_path / f"ad.{r}.h5"), "/Moves/Bisect/n_level")[1]
+        assert list(got) == [b["Moves"]["Bisect"][r]["n_level"] for b in blocks]

@@ -557,3 +557,55 @@ def test_edge_configurations(name):
         # difference is taken on rho_k of the current positions
         o.rho_k_rebuild()
         assert rel_err(e.gradient(abi.GRAD_ALL), o.gradient(abi.GRAD_ALL, h=1e-5)) < 1e-6
+
+
+@pytest.mark.parametrize("images", [0, 1])
+def test_bisect_level_changes(images):
+    """Adaptive bisection (bisect_class.h:143-151): stages at levels 3, 2, 4, 1
+    and 3 of two replicas, each replaying its oracle's tape, with the sampling
+    image splines of every level; the sweep's attempt count follows the level."""
+    systems = [system.config_c1(n_part=16, n_bead=32), system.config_c1(n_part=8, n_bead=16)]
+    for s in systems:
+        s.bisect_n_images = images
+    levels = [3, 2, 4, 1, 3]
+    tapes, refs, finals = [], [], []
+    for i, s in enumerate(systems):
+        pos = system.random_path(s, seed=80 + i, spread=0.05)
+        o = Oracle(s, seed=300 + i)
+        o.upload(pos)
+        o.record(True)
+        acc = []
+        for lv in levels:
+            o.set_bisect_level(lv)
+            acc.append(o.bisect(12))
+            acc.append(o.displace(4))
+        refs.append(acc)
+        tapes.append((pos, o.recorded()))
+        finals.append(o.download())
+    with abi.Engine(systems, seed=1) as e:
+        for i, (pos, tape) in enumerate(tapes):
+            e.upload(pos, r=i)
+            e.tape_set(tape, r=i)
+        got = [[] for _ in systems]
+        for lv in levels:
+            for i in range(len(systems)):
+                e.set_bisect_level(lv, r=i)
+            ab, ad = e.bisect(12), e.displace(4)
+            for i in range(len(systems)):
+                got[i] += [ab[i], ad[i]]
+        for i in range(len(systems)):
+            for g, r in zip(got[i], refs[i]):
+                assert np.array_equal(g, r), i
+            assert np.array_equal(e.download(r=i), finals[i]), i
+        with pytest.raises(abi.SpimcError):
+            e.set_bisect_level(5, r=1)  # beyond floor(log2 16)
+    with abi.Engine(systems[0], seed=4) as e:
+        e.upload(system.random_path(systems[0], seed=1, spread=0.05))
+        e.set_bisect_level(2)
+        c0 = e.counters()
+        e.sweep(1, abi.SWEEP_GRAPH)
+        e.set_bisect_level(4)
+        e.sweep(1, abi.SWEEP_GRAPH)
+        c = e.counters() - c0
+        s = systems[0]
+        assert c[0] == ((s.n_part * s.n_bead) >> 2) + ((s.n_part * s.n_bead) >> 4)
