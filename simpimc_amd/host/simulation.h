@@ -576,6 +576,15 @@ struct StructureFactor : Observable {
     }
 };
 
+// Time (time_class.h:7-71): at every Write, the block's wall time and each
+// event's time spent in it, then a fresh block.  Moves time the host call that
+// hands their attempts to the device, which returns when the stage is done.
+struct TimeObs : Observable {
+    double start;
+    TimeObs(const Input& in, Engine& e) : Observable(in, e) { start = Now(); }
+    void DoEvent() override {}
+};
+
 // PathDump (path_dump_class.h:7-69): every `skip`-th Write appends the path of
 // each replica to its output file; accumulates nothing.
 struct PathDump : Observable {

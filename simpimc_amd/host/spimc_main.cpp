@@ -88,6 +88,8 @@ struct Simulation {
                 ob = gr = std::make_shared<PairCorrelation>(o, eng, path);
             } else if (type == "PathDump") {
                 ob = std::make_shared<PathDump>(o, eng);
+            } else if (type == "Time") {
+                ob = std::make_shared<TimeObs>(o, eng);
             } else {
                 throw std::runtime_error("ERROR: Unrecognized observable, " + type);
             }
@@ -187,6 +189,8 @@ struct Simulation {
                 WriteGr(*g);
             else if (auto d = dynamic_cast<PathDump*>(o.get()))
                 WritePathDump(*d);
+            else if (auto t = dynamic_cast<TimeObs*>(o.get()))
+                WriteTime(*t);
         }
         out << "}}\n";
         out.flush();
@@ -332,6 +336,31 @@ struct Simulation {
         out << "]}";
         check(spimc_obs_reset(eng.ctx, SPIMC_OBS_GR), "reset");
     }
+
+    // Time::Write (time_class.h:38-67): block time and per-event times (seconds),
+    // each an extendable scalar with data_type "scalar"; then every event's
+    // time_spent is cleared (Time::Reset)
+    void WriteTime(TimeObs& t) {
+        const double now = Now(), total = now - t.start;
+        out << "\"" << t.name << "\": {\"block\": " << total;
+        for (auto& e : events) out << ", \"" << e->name << "\": " << e->time_spent;
+        out << "}";
+#if SPIMC_HAVE_HDF5
+        for (auto& f : h5) {
+            const std::string g = "Observables/" + t.name + "/";
+            if (!time_started_) {
+                f->String(g + "block/data_type", "scalar");
+                for (auto& e : events) f->String(g + e->name + "/data_type", "scalar");
+            }
+            f->Append(g + "block/x", &total, 0);
+            for (auto& e : events) f->Append(g + e->name + "/x", &e->time_spent, 0);
+        }
+#endif
+        time_started_ = true;
+        for (auto& e : events) e->time_spent = 0.;
+        t.start = now;
+    }
+    bool time_started_ = false;
 
     // PathDump::Write (path_dump_class.h:29-66): positions [dump][particle][bead][d]
     // (the reference's cube(n_d, n_bead, n_part) per dump, first index fastest)

@@ -59,9 +59,9 @@ def test_unknown_move_and_observable(tmp_path):
               tweak=lambda x: x.replace('type="DisplaceParticle"', 'type="PermBisectIterative"'))
     assert res.returncode == 1
     assert "Unrecognized Move, PermBisectIterative" in res.stderr
-    res = run(tmp_path, BARE.format(file="bare.table"), extra='\n    <Observable name="T" type="Time" />')
+    res = run(tmp_path, BARE.format(file="bare.table"), extra='\n    <Observable name="S" type="Sign" />')
     assert res.returncode == 1
-    assert "Unrecognized observable, Time" in res.stderr
+    assert "Unrecognized observable, Sign" in res.stderr
 
 
 @pytest.mark.parametrize("attr", ["fermi=\"1\"", "pbc=\"0\""])

@@ -335,3 +335,26 @@ def test_adaptive_bisection(tmp_path):
     for r in range(2):
         got = h5dump.dataset(str(tmp_path / f"ad.{r}.h5"), "/Moves/Bisect/n_level")[1]
         assert list(got) == [b["Moves"]["Bisect"][r]["n_level"] for b in blocks]
+
+
+@pytest.mark.skipif(h5dump.H5DUMP is None, reason="h5dump not installed")
+def test_time_observable(tmp_path):
+    """<Observable type="Time">: per block, the wall time and each event's time
+    (time_class.h:38-67), as positive scalars in both outputs."""
+    assert os.path.exists(DRIVER), "build the host driver first"
+    s = system.config_c1()
+    write_init(tmp_path, s)
+    system.write_table(str(tmp_path / "bare.table"), system.bare_table_datasets(s))
+    blocks = run_driver(tmp_path, s, BARE.format(file="bare.table"), "tm", 1,
+                        extra='\n    <Observable name="T" type="Time" />')
+    for b in blocks:
+        t = b["Observables"]["T"]
+        assert t["block"] > 0 and t["Bisect"] > 0 and t["Displace"] > 0
+        assert t["Bisect"] + t["Displace"] + t["Energy"] + t["SK"] + t["GR"] <= t["block"] * 1.001
+    f = str(tmp_path / "tm.0.h5")
+    assert h5dump.dataset(f, "/Observables/T/block/data_type")[1] == ["scalar"]
+    for ev in ("block", "Bisect", "Displace", "Energy"):
+        dims, x = h5dump.dataset(f, f"/Observables/T/{ev}/x")
+        assert dims == (2,) and np.all(x >= 0)
+    np.testing.assert_allclose(h5dump.dataset(f, "/Observables/T/block/x")[1], [b["Observables"]["T"]["block"] for b in blocks],
+                               rtol=1e-12)
