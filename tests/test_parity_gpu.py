@@ -609,3 +609,82 @@ def test_bisect_level_changes(images):
         c = e.counters() - c0
         s = systems[0]
         assert c[0] == ((s.n_part * s.n_bead) >> 2) + ((s.n_part * s.n_bead) >> 4)
+
+
+def _random_system(rng):
+    """A small system drawn at random: size, level, pair action (none, Bare
+    Coulomb, tabulated Bare, David, Ilkka), long range, images and k cut."""
+    n = int(rng.integers(1, 40))
+    m = int(rng.choice([2, 4, 8, 16, 32], p=[0.1, 0.15, 0.25, 0.25, 0.25]))
+    lv = int(rng.integers(1, min(int(np.log2(m)), 4) + 1))
+    kind = rng.choice(["none", "coulomb", "tabulated", "david", "ilkka"])
+    rs = float(rng.uniform(1.0, 3.0))
+    if kind == "david":
+        s = system.config_c2(n_part=n, n_bead=m, r_s=rs, tau=0.02, n_level=lv, k_shell=float(rng.uniform(3.0, 12.0)),
+                             n_grid=300, seed=int(rng.integers(1, 100)))
+    elif kind == "ilkka":
+        s = system.config_ilkka(n_part=n, n_bead=m, r_s=rs, n_level=lv)
+    else:
+        s = system.config_c1(n_part=n, n_bead=m, r_s=rs, n_level=lv)
+        s.k_n = system.kspace(s.L, float(rng.uniform(1.0, 4.0)) * 2 * np.pi / s.L + 1e-9)
+        system.bare_coulomb_ewald(s, alpha=5.0 / s.L)
+        if kind == "tabulated":
+            s.bare_is_coulomb = False
+            s.bare_v_r = 1.0 / np.sqrt(s.bare_r ** 2 + 0.1)
+        elif kind == "none":
+            s.pair_type = abi.PAIR_NONE
+    if kind != "none" and rng.random() < 0.25:
+        s.use_long_range = False
+    if rng.random() < 0.3:
+        s.kinetic_n_images = 1
+    if rng.random() < 0.3:
+        s.bisect_n_images = 1
+    s.name = f"{kind}-N{n}-M{m}-l{lv}-lr{int(s.use_long_range)}-img{s.kinetic_n_images}{s.bisect_n_images}"
+    return s
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_randomized_configurations(seed):
+    """Seeded random small systems: a replayed bisect/displace chain bit for bit,
+    rho_k, the whole-path estimators, the reference gradient and the adjoint."""
+    rng = np.random.default_rng(1000 + seed)
+    s = _random_system(rng)
+    print(s.name, flush=True)
+    pos = system.random_path(s, seed=seed, spread=float(rng.uniform(0.02, 0.3)))
+    o = Oracle(s, seed=seed)
+    o.upload(pos)
+    o.record(True)
+    n_b = max(1, (s.n_part * s.n_bead) >> s.bisect_n_level)
+    ref = [o.bisect(n_b), o.displace(s.n_part), o.bisect(n_b)]
+    tape = o.recorded()
+    with abi.Engine(s, seed=seed) as e:
+        e.upload(pos)
+        e.tape_set(tape)
+        got = [e.bisect(n_b)[0], e.displace(s.n_part)[0], e.bisect(n_b)[0]]
+        for g, r in zip(got, ref):
+            assert np.array_equal(g, r), s.name
+        assert np.array_equal(e.download(), o.download()), s.name
+        e.tape_set(None)
+        o.rho_k_rebuild()  # one particle: the reference leaves rho_k stale (constant pair action)
+        e.rho_k_rebuild()
+        if s.use_long_range and s.pair_type != abi.PAIR_NONE:
+            assert rel_err(e.rho_k(), o.rho_k()) < 1e-12 * max(1, s.n_part), s.name
+        for action in (abi.ACTION_KINETIC, abi.ACTION_PAIR):
+            a, b = e.d_action_d_beta(action), o.d_action_d_beta(action)
+            assert abs(a - b) <= RTOL * max(1.0, abs(b)), (s.name, action, a, b)
+        a, b = e.potential(abi.ACTION_PAIR), o.potential(abi.ACTION_PAIR)
+        assert abs(a - b) <= RTOL * max(1.0, abs(b)), (s.name, a, b)
+        a, b = e.kinetic_virial_energy(0), o.kinetic_virial_energy(0)
+        assert abs(a - b) <= RTOL * max(1.0, abs(b)), (s.name, a, b)
+        terms = abi.GRAD_PAIR_SHORT | abi.GRAD_PAIR_LONG
+        assert rel_err(e.gradient(terms, compat=True), o.gradient(terms, compat=True)) < 1e-9, s.name
+        # The link action jumps where a pair vector crosses half a box: DrDrpDrrp
+        # takes r' at the image nearest r, so when r changes sides |r'| changes
+        # (path_class.h:137-150).  A central difference that straddles such a
+        # point is not a derivative; beads whose differences at h and h/5 disagree
+        # are left out (there are at most a few).
+        g, f5, f1 = e.gradient(abi.GRAD_ALL), o.gradient(abi.GRAD_ALL, h=1e-5), o.gradient(abi.GRAD_ALL, h=2e-6)
+        scale = max(1.0, float(np.abs(f5).max()))
+        smooth = np.all(np.abs(f5 - f1) < 1e-6 * scale, axis=2)
+        assert smooth.sum() >= smooth.size - 3, (s.name, int(smooth.size - smooth.sum()))
+        assert float(np.abs(g - f5)[smooth].max()) < 1e-6 * scale, s.name
