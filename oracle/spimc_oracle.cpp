@@ -328,8 +328,41 @@ struct Oracle {
     std::vector<double> sk_acc, gr_acc;
     uint64_t n_measure[3] = {0, 0, 0};
 
+    // Path topology.  sigma[p] is the particle whose slice-0 bead follows the
+    // slice-(M-1) bead of particle p: Bead::next_ across the wrap, where
+    // PermBisect::AssignParticleLabels (perm_bisect_class.h:47-56) leaves every
+    // permutation.  Within slices 0..M-1 a bead is followed by the same label.
+    // During a permuting attempt the NEW-mode path is also relinked at slice
+    // att_e: bead (x, att_e) is followed by the bead that followed (cyc[x],
+    // att_e) (PermBisect::PermuteBeads, perm_bisect_class.h:32-44).
+    std::vector<uint32_t> sigma, isigma;
+    bool att_on = false;
+    uint32_t att_e = 0;
+    std::vector<uint32_t> cyc;
+    // PermBisect (perm_bisect_class.h:20-29, perm_bisect_iterative_class.h)
+    double perm_log_epsilon = std::log(1.e-100);
+    std::vector<uint64_t> perm_attempt, perm_accept;  // per cycle length - 1
+    uint64_t perm_counters[2] = {0, 0};               // Move::n_attempt, n_accept
+
     // ---- helpers ------------------------------------------------------------
     uint32_t bead_loop(uint32_t b) const { return b >= M ? b - M : b; }  // species_class.h:101
+    struct Bd {
+        uint32_t p, b;  // label, slice
+        bool operator==(const Bd& o) const { return p == o.p && b == o.b; }
+        bool operator!=(const Bd& o) const { return !(*this == o); }
+    };
+    // Bead::GetNextBead(1) in the current mode
+    Bd next1(Bd x) const {
+        const uint32_t y = (att_on && mode == SPIMC_MODE_NEW && x.b == att_e) ? cyc[x.p] : x.p;
+        return x.b + 1 == M ? Bd{sigma[y], 0} : Bd{y, x.b + 1};
+    }
+    Bd next(Bd x, uint32_t n) const {
+        while (n--) x = next1(x);
+        return x;
+    }
+    // Bead::GetPrevBead(1) outside an attempt
+    Bd prev1(Bd x) const { return x.b == 0 ? Bd{isigma[x.p], M - 1} : Bd{x.p, x.b - 1}; }
+    double* R(Bd x) { return R(x.p, x.b); }
     double* R(uint32_t p, uint32_t b) { return (mode ? r.data() : rc.data()) + 3 * ((size_t)bead_loop(b) * N + p); }
     double* Rm(int m, uint32_t p, uint32_t b) { return (m ? r.data() : rc.data()) + 3 * ((size_t)bead_loop(b) * N + p); }
     std::vector<cplx>& rho_mode() { return mode ? rho : rhoc; }  // species_class.h:428
@@ -739,18 +772,20 @@ struct Oracle {
     }
 
     // ---- kinetic -----------------------------------------------------------
-    // Kinetic::GetAction (kinetic_class.h:105-122)
+    // Kinetic::GetAction (kinetic_class.h:105-122): walks the path from bead
+    // (p, b0) to the bead b1 - b0 links later; the walk compares beads, so it is
+    // empty when it returns to its start.
     double kinetic_get_action(uint32_t b0, uint32_t b1, const std::vector<uint32_t>& parts, uint32_t level) {
         uint32_t skip = 1u << level;
         double tot = 0.;
         for (uint32_t p : parts) {
-            uint32_t a = bead_loop(b0);
-            uint32_t end = (b0 + (b1 - b0)) % M;
+            Bd a{p, bead_loop(b0)};
+            const Bd end = next(a, b1 - b0);
             uint32_t guard = 0;
-            while (a != end && guard++ <= M) {
-                uint32_t nx = (a + skip) % M;
+            while (a != end && guard++ <= N * M) {
+                Bd nx = next(a, skip);
                 double d[3];
-                dr(R(p, a), R(p, nx), d);
+                dr(R(a), R(nx), d);
                 tot -= kin_splines[skip - 1].log_rho_free(d);
                 a = nx;
             }
@@ -764,7 +799,7 @@ struct Oracle {
         for (uint32_t b = 0; b < M; ++b)
             for (uint32_t p = 0; p < N; ++p) {
                 double d[3];
-                dr(R(p, b), R(p, b + 1), d);
+                dr(R(p, b), R(next1(Bd{p, b})), d);
                 tot += kin_splines[0].dlog_rho_free_dtau(d);
             }
         return tot;
@@ -773,25 +808,25 @@ struct Oracle {
     double kinetic_virial_energy(uint32_t W) {
         double tot = kin_dbeta_const / W;
         for (uint32_t p = 0; p < N; ++p) {
-            uint32_t b_0 = 0, b_1 = 1 % M, b_2 = 2 % M;
+            Bd b_0{p, 0}, b_1 = next1(b_0), b_2 = next1(b_1);
             double dr_i_1[3], dr_i_L[3], t[3];
-            dr(R(p, b_1), R(p, b_0), dr_i_1);
+            dr(R(b_1), R(b_0), dr_i_1);
             for (int d = 0; d < 3; ++d) dr_i_L[d] = dr_i_1[d];
             for (uint32_t skip = 1; skip < W; ++skip) {
-                dr(R(p, b_2), R(p, b_1), t);
+                dr(R(b_2), R(b_1), t);
                 for (int d = 0; d < 3; ++d) dr_i_L[d] += t[d];
                 b_1 = b_2;
-                b_2 = (b_2 + 1) % M;
+                b_2 = next1(b_2);
             }
             tot -= (dr_i_L[0] * dr_i_1[0] + dr_i_L[1] * dr_i_1[1] + dr_i_L[2] * dr_i_1[2]) * kin_i4lt / (W * tau);
             for (uint32_t b_i = 1; b_i < M; ++b_i) {
                 for (int d = 0; d < 3; ++d) dr_i_L[d] -= dr_i_1[d];
-                dr(R(p, b_2), R(p, b_1), t);
+                dr(R(b_2), R(b_1), t);
                 for (int d = 0; d < 3; ++d) dr_i_L[d] += t[d];
-                b_0 = (b_0 + 1) % M;
-                dr(R(p, (b_0 + 1) % M), R(p, b_0), dr_i_1);
+                b_0 = next1(b_0);
+                dr(R(next1(b_0)), R(b_0), dr_i_1);
                 b_1 = b_2;
-                b_2 = (b_2 + 1) % M;
+                b_2 = next1(b_2);
                 tot -= (dr_i_L[0] * dr_i_1[0] + dr_i_L[1] * dr_i_1[1] + dr_i_L[2] * dr_i_1[2]) * kin_i4lt / (W * tau);
             }
         }
@@ -804,7 +839,7 @@ struct Oracle {
         for (uint32_t b = 0; b < M; ++b)
             for (uint32_t p = 0; p < N; ++p) {
                 double d[3];
-                dr(R(p, b), R(p, b + 1), d);
+                dr(R(p, b), R(next1(Bd{p, b})), d);
                 tot -= kin_splines[0].log_rho_free(d);
             }
         return tot;
@@ -842,9 +877,9 @@ struct Oracle {
                 uint32_t bm = (b + M - 1) % M;
                 if (terms & SPIMC_GRAD_KINETIC) {
                     double dd[3];
-                    dr(R(p, b), R(p, b + 1), dd);
+                    dr(R(p, b), R(next1(Bd{p, b})), dd);
                     s -= kin_splines[0].log_rho_free(dd);
-                    dr(R(p, bm), R(p, b), dd);
+                    dr(R(prev1(Bd{p, b})), R(p, b), dd);
                     s -= kin_splines[0].log_rho_free(dd);
                 }
                 if ((terms & SPIMC_GRAD_PAIR_SHORT) && pair_type != SPIMC_PAIR_NONE) {
@@ -921,41 +956,53 @@ struct Oracle {
             std::memcpy(&rho[(size_t)bead_loop(b) * nk], &rhoc[(size_t)bead_loop(b) * nk], nk * sizeof(cplx));
     }
 
-    // Bisect::Attempt (bisect_class.h:39-124)
-    bool bisect_attempt(uint32_t& p_out, uint32_t& b0_out) {
-        uint32_t p = draw_index(N);
-        uint32_t bead0 = draw_index(M);
-        uint32_t bead1 = bead0 + n_bisect_beads;
-        p_out = p;
-        b0_out = bead0;
-        std::vector<uint32_t> parts{p};
-        double prev_action_change = 0.;
-        // the walks compare bead objects: when 2^n_level == M the last bead is the
-        // first one and no midpoint is visited (bisect_class.h:51-58,78)
-        const bool wraps = (n_bisect_beads % M) == 0;
+    // The beads the last attempt moved (Bisect::affected_beads) and its bead0
+    std::vector<Bd> affected;
+    uint32_t last_bead0 = 0;
+
+    // Bisection down to level 0 of the chains that start at the beads `start`
+    // and end at the bead n_bisect_beads links later in each mode: the level
+    // loop of Bisect::Attempt (bisect_class.h:65-122) and of
+    // PermBisectIterative::Attempt (perm_bisect_iterative_class.h:159-217),
+    // which samples the chains one after the other.  The chain ends are fixed;
+    // the beads between are resampled top-down.  parts is the particle list the
+    // actions see; prev_action_change is carried into the first level.
+    bool bisect_levels(const std::vector<Bd>& start, uint32_t bead0, uint32_t bead1, const std::vector<uint32_t>& parts,
+                       double prev_action_change) {
+        const size_t n = start.size();
+        std::vector<Bd> end_new(n);
+        mode = SPIMC_MODE_NEW;
+        for (size_t i = 0; i < n; ++i) end_new[i] = next(start[i], n_bisect_beads);
         for (int level = (int)n_level - 1; level >= 0; --level) {
             uint32_t skip = 1u << level;
             double level_tau = tau * skip;
-            double sigma = std::sqrt(lambda * level_tau);
+            double sigma_l = std::sqrt(lambda * level_tau);
             double old_lsp = 0., new_lsp = 0.;
-            for (uint32_t a = bead0; !wraps && a != bead1; a += 2 * skip) {
-                uint32_t bb = a + skip, cc = a + 2 * skip;
-                double rbar[3], t[3], delta_old[3], delta_new[3];
-                // old sampling, OLD mode: RBar(c, a) = a + 0.5 Dr(c, a)
-                mode = SPIMC_MODE_OLD;
-                dr(R(p, cc), R(p, a), t);
-                for (int d = 0; d < 3; ++d) rbar[d] = R(p, a)[d] + 0.5 * t[d];
-                dr(R(p, bb), rbar, delta_old);
-                old_lsp += bis_splines[level].log_rho_free(delta_old);
-                // new sampling, NEW mode
-                mode = SPIMC_MODE_NEW;
-                for (int d = 0; d < 3; ++d) delta_new[d] = sigma * draw_normal();
-                put_in_box(delta_new);
-                dr(R(p, cc), R(p, a), t);
-                for (int d = 0; d < 3; ++d) rbar[d] = R(p, a)[d] + 0.5 * t[d];
-                double* nb = R(p, bb);
-                for (int d = 0; d < 3; ++d) nb[d] = rbar[d] + delta_new[d];
-                new_lsp += bis_splines[level].log_rho_free(delta_new);
+            for (size_t i = 0; i < n; ++i) {
+                Bd a = start[i];
+                uint32_t guard = 0;
+                while (a != end_new[i] && guard++ <= N * M) {
+                    double rbar[3], t[3], delta_old[3], delta_new[3];
+                    // old sampling, OLD mode: RBar(c, a) = a + 0.5 Dr(c, a)
+                    mode = SPIMC_MODE_OLD;
+                    Bd bb = next(a, skip), cc = next(bb, skip);
+                    dr(R(cc), R(a), t);
+                    for (int d = 0; d < 3; ++d) rbar[d] = R(a)[d] + 0.5 * t[d];
+                    dr(R(bb), rbar, delta_old);
+                    old_lsp += bis_splines[level].log_rho_free(delta_old);
+                    // new sampling, NEW mode
+                    mode = SPIMC_MODE_NEW;
+                    bb = next(a, skip);
+                    cc = next(bb, skip);
+                    for (int d = 0; d < 3; ++d) delta_new[d] = sigma_l * draw_normal();
+                    put_in_box(delta_new);
+                    dr(R(cc), R(a), t);
+                    for (int d = 0; d < 3; ++d) rbar[d] = R(a)[d] + 0.5 * t[d];
+                    double* nb = R(bb);
+                    for (int d = 0; d < 3; ++d) nb[d] = rbar[d] + delta_new[d];
+                    new_lsp += bis_splines[level].log_rho_free(delta_new);
+                    a = cc;
+                }
             }
             double old_action = 0., new_action = 0.;
             // action list order: Kinetic, pair action
@@ -975,15 +1022,188 @@ struct Oracle {
         }
         return true;
     }
-    void bisect_accept(uint32_t p, uint32_t bead0) {  // bisect_class.h:24-36
-        store_r_all_beads_of(p, bead0, n_bisect_beads + 1);
+
+    // Bisect::Attempt (bisect_class.h:39-124)
+    bool bisect_attempt(uint32_t& p_out, uint32_t& b0_out) {
+        uint32_t p = draw_index(N);
+        uint32_t bead0 = draw_index(M);
+        uint32_t bead1 = bead0 + n_bisect_beads;
+        p_out = p;
+        b0_out = bead0;
+        mode = SPIMC_MODE_NEW;
+        const Bd bead_i{p, bead0};
+        affected.clear();
+        Bd bead_f = bead_i;
+        affected.push_back(bead_i);
+        for (uint32_t i = 0; i < n_bisect_beads; ++i) {
+            bead_f = next1(bead_f);
+            affected.push_back(bead_f);
+        }
+        std::vector<uint32_t> parts{p};
+        if (bead_f.p != p) parts.push_back(bead_f.p);
+        return bisect_levels({bead_i}, bead0, bead1, parts, 0.);
+    }
+    void store_affected() {
+        for (const Bd& x : affected) std::memcpy(Rm(0, x.p, x.b), Rm(1, x.p, x.b), 3 * sizeof(double));
+    }
+    void restore_affected() {
+        for (const Bd& x : affected) std::memcpy(Rm(1, x.p, x.b), Rm(0, x.p, x.b), 3 * sizeof(double));
+    }
+    void bisect_accept(uint32_t /* p */, uint32_t bead0) {  // bisect_class.h:24-36
+        store_affected();
         store_rho(bead0, bead0 + n_bisect_beads);
         need_update_rho_k = true;
     }
-    void bisect_reject(uint32_t p, uint32_t bead0) {  // bisect_class.h:127-139
-        restore_r_all_beads_of(p, bead0, n_bisect_beads + 1);
+    void bisect_reject(uint32_t /* p */, uint32_t bead0) {  // bisect_class.h:127-139
+        restore_affected();
         restore_rho(bead0, bead0 + n_bisect_beads);
         need_update_rho_k = true;
+    }
+
+    // PermBisectIterative::UpdatePermTable (perm_bisect_iterative_class.h:10-29),
+    // row p only: t(p, j) from the bead (p, bead0) to the bead n_bisect_beads
+    // links after (j, bead0)
+    void perm_table_row(uint32_t p, uint32_t bead0, std::vector<double>& row) {
+        row.resize(N);
+        const double i4ltn = 1. / (4. * lambda * tau) / n_bisect_beads;
+        const double* a = R(p, bead0);
+        for (uint32_t j = 0; j < N; ++j) {
+            double d[3];
+            dr(a, R(next(Bd{j, bead0}, n_bisect_beads)), d);
+            double exponent = (-(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])) * i4ltn;
+            row[j] = exponent > perm_log_epsilon ? std::exp(exponent) : 0.;
+        }
+    }
+
+    // PermBisectIterative::SelectCycleIterative (perm_bisect_iterative_class.h:32-110):
+    // false when the walk stops before it closes; else the cycle's particles
+    // (ps) and weight
+    bool select_cycle(uint32_t bead0, std::vector<uint32_t>& ps, double& weight) {
+        mode = SPIMC_MODE_NEW;
+        ps.clear();
+        uint32_t p0 = draw_index(N);
+        uint32_t p = p0;
+        std::vector<double> t_row, t_c(N);
+        weight = 1.;
+        std::vector<double> w_num, w_den;
+        do {
+            ps.push_back(p);
+            perm_table_row(p, bead0, t_row);
+            // a row is taken from t with the visited particles removed and the
+            // closing link restored (the reference's t_c rows)
+            for (uint32_t i = 0; i < N; ++i) t_c[i] = t_row[i];
+            for (uint32_t q : ps) t_c[q] = 0.;
+            t_c[p0] = t_row[p0];
+            double Q_p = 0., Q_p_c = 0.;
+            for (uint32_t i = 0; i < N; ++i) {
+                Q_p += t_row[i];
+                Q_p_c += t_c[i];
+            }
+            if (Q_p_c / Q_p < draw()) return false;
+            double x = draw();
+            double t_Q = 0.;
+            const uint32_t from = p;
+            for (uint32_t i = 0; i < N; ++i) {
+                t_Q += t_c[i] / Q_p_c;
+                if (t_Q > x) {
+                    p = i;
+                    break;
+                }
+            }
+            // the weight's factors t(ps[i], ps[i+1]) / t(ps[i], ps[i]), in order
+            w_num.push_back(t_row[p]);
+            w_den.push_back(t_row[from]);
+            if (ps.size() > (size_t)N + 1) {
+                g_err = "permutation walk did not close";
+                return false;
+            }
+        } while (p != p0);
+        for (size_t i = 0; i < w_num.size(); ++i) weight *= w_num[i] / w_den[i];
+        return true;
+    }
+
+    // PermBisectIterative::Attempt (perm_bisect_iterative_class.h:113-223).
+    // n_chain_out = 0 when the cycle walk stopped (no bisection).
+    bool perm_bisect_attempt(uint32_t& n_chain_out) {
+        uint32_t bead0 = draw_index(M);
+        last_bead0 = bead0;
+        uint32_t bead1 = bead0 + n_bisect_beads;
+        bool roll_over = bead1 > M - 1;
+        std::vector<uint32_t> ps;
+        double weight = 1.;
+        n_chain_out = 0;
+        if (!select_cycle(bead0, ps, weight)) return false;
+        const uint32_t n = (uint32_t)ps.size();
+        n_chain_out = n;
+        mode = SPIMC_MODE_NEW;
+        std::vector<Bd> start(n), fm1(n);
+        std::vector<uint32_t> parts;
+        for (uint32_t i = 0; i < n; ++i) {
+            start[i] = Bd{ps[i], bead0};
+            fm1[i] = next(start[i], n_bisect_beads - 1);
+            parts.push_back(ps[i]);
+        }
+        if (roll_over) {
+            for (uint32_t i = 0; i < n; ++i) parts.push_back(next1(fm1[i]).p);
+            std::sort(parts.begin(), parts.end());
+            parts.erase(std::unique(parts.begin(), parts.end()), parts.end());
+        }
+        // PermuteBeads: bead fm1(i) is followed by the old successor of fm1(i + 1)
+        cyc.resize(N);
+        for (uint32_t x = 0; x < N; ++x) cyc[x] = x;
+        for (uint32_t i = 0; i < n; ++i) cyc[fm1[i].p] = fm1[(i + 1) % n].p;
+        att_e = fm1[0].b;
+        att_on = true;
+        affected.clear();
+        for (uint32_t i = 0; i < n; ++i) {
+            Bd a = next1(start[i]);
+            for (uint32_t j = 1; j < n_bisect_beads; ++j, a = next1(a)) affected.push_back(a);
+        }
+        return bisect_levels(start, bead0, bead1, parts, -std::log(weight));
+    }
+    // PermBisect::Accept / AssignParticleLabels (perm_bisect_class.h:47-73) then
+    // Bisect::Accept: slices after the relinked one take the labels of the path
+    // they now continue, which moves the permutation to the wrap.
+    void perm_bisect_accept(uint32_t bead0, uint32_t n) {
+        store_affected();
+        store_rho(bead0, bead0 + n_bisect_beads);
+        need_update_rho_k = true;
+        if (n > 1) {
+            std::vector<double> tmp(3 * (size_t)N);
+            for (uint32_t b = att_e + 1; b < M; ++b)
+                for (int m = 0; m < 2; ++m) {
+                    for (uint32_t x = 0; x < N; ++x) std::memcpy(&tmp[3 * x], Rm(m, x, b), 3 * sizeof(double));
+                    for (uint32_t x = 0; x < N; ++x) std::memcpy(Rm(m, x, b), &tmp[3 * cyc[x]], 3 * sizeof(double));
+                }
+            std::vector<uint32_t> s2(N);
+            for (uint32_t x = 0; x < N; ++x) s2[x] = sigma[cyc[x]];
+            sigma = s2;
+            for (uint32_t x = 0; x < N; ++x) isigma[sigma[x]] = x;
+        }
+        att_on = false;
+    }
+    void perm_bisect_reject(uint32_t bead0, uint32_t n) {  // PermBisect::Reject (perm_bisect_class.h:76-90)
+        att_on = false;
+        if (n > 0) restore_affected();
+        restore_rho(bead0, bead0 + n_bisect_beads);
+        need_update_rho_k = true;
+    }
+    // Species::SetCycleCount (species_class.h:28-44)
+    std::vector<uint32_t> cycle_lengths() const {
+        std::vector<uint32_t> out;
+        std::vector<char> seen(N, 0);
+        for (uint32_t p = 0; p < N; ++p) {
+            if (seen[p]) continue;
+            uint32_t len = 1, q = sigma[p];
+            seen[p] = 1;
+            while (q != p) {
+                seen[q] = 1;
+                ++len;
+                q = sigma[q];
+            }
+            out.push_back(len);
+        }
+        return out;
     }
 
     // DisplaceParticle::Attempt (displace_particle_class.h:28-72)
@@ -1160,6 +1380,11 @@ int spimc_oracle_create(const spimc_system_desc* s, uint64_t seed, void** out) {
     o->lambda = s->lambda;
     o->r.assign((size_t)3 * o->M * o->N, 0.);
     o->rc = o->r;
+    o->sigma.resize(o->N);
+    for (uint32_t p = 0; p < o->N; ++p) o->sigma[p] = p;
+    o->isigma = o->sigma;
+    o->perm_attempt.assign(o->N, 0);
+    o->perm_accept.assign(o->N, 0);
     // k-space
     o->nk = s->n_k;
     o->kn.assign(s->k_n, s->k_n + 3 * (size_t)s->n_k);
@@ -1525,6 +1750,80 @@ int spimc_oracle_displace_stage(void* h, uint32_t n_attempts, uint8_t* accepted)
         if (accepted) accepted[i] = ok ? 1 : 0;
     }
     return g_err.empty() ? SPIMC_OK : SPIMC_ERR_STATE;
+}
+
+int spimc_oracle_perm_bisect_stage(void* h, uint32_t n_attempts, uint8_t* accepted) {
+    Oracle* o = (Oracle*)h;
+    g_err.clear();
+    for (uint32_t i = 0; i < n_attempts; ++i) {
+        uint32_t n = 0;
+        o->perm_counters[0]++;
+        bool ok = o->perm_bisect_attempt(n);
+        if (ok) {
+            o->perm_counters[1]++;
+            o->perm_attempt[n - 1]++;
+            o->perm_accept[n - 1]++;
+            o->perm_bisect_accept(o->last_bead0, n);
+        } else {
+            if (n > 0) o->perm_attempt[n - 1]++;
+            o->perm_bisect_reject(o->last_bead0, n);
+        }
+        o->mode = SPIMC_MODE_NEW;
+        o->bead_updates += (uint64_t)n * (o->n_bisect_beads - 1);
+        if (accepted) accepted[i] = ok ? 1 : 0;
+        if (!g_err.empty()) break;
+    }
+    return g_err.empty() ? SPIMC_OK : SPIMC_ERR_STATE;
+}
+
+// out[0] attempts, out[1] accepts, out[2 + c] perm_attempt and out[2 + N + c]
+// perm_accept of cycle length c + 1 (perm_bisect_class.h:25-26,66-87)
+int spimc_oracle_perm_counters(void* h, uint64_t* out) {
+    Oracle* o = (Oracle*)h;
+    out[0] = o->perm_counters[0];
+    out[1] = o->perm_counters[1];
+    for (uint32_t c = 0; c < o->N; ++c) {
+        out[2 + c] = o->perm_attempt[c];
+        out[2 + o->N + c] = o->perm_accept[c];
+    }
+    return SPIMC_OK;
+}
+
+int spimc_oracle_perm_configure(void* h, double epsilon) {
+    Oracle* o = (Oracle*)h;
+    if (!(epsilon > 0.)) return fail(SPIMC_ERR_ARG, "epsilon must be > 0");
+    o->perm_log_epsilon = std::log(epsilon);
+    return SPIMC_OK;
+}
+
+// sigma[p]: the particle whose slice-0 bead follows particle p's last bead
+int spimc_oracle_perm_get(void* h, uint32_t* sigma) {
+    Oracle* o = (Oracle*)h;
+    for (uint32_t p = 0; p < o->N; ++p) sigma[p] = o->sigma[p];
+    return SPIMC_OK;
+}
+
+int spimc_oracle_perm_set(void* h, const uint32_t* sigma) {
+    Oracle* o = (Oracle*)h;
+    std::vector<char> seen(o->N, 0);
+    for (uint32_t p = 0; p < o->N; ++p) {
+        if (sigma[p] >= o->N || seen[sigma[p]]) return fail(SPIMC_ERR_ARG, "not a permutation");
+        seen[sigma[p]] = 1;
+    }
+    for (uint32_t p = 0; p < o->N; ++p) {
+        o->sigma[p] = sigma[p];
+        o->isigma[sigma[p]] = p;
+    }
+    return SPIMC_OK;
+}
+
+// cycle lengths of the path's permutation in particle order (Species::SetCycleCount)
+int spimc_oracle_cycle_lengths(void* h, uint32_t* out, uint32_t* n) {
+    Oracle* o = (Oracle*)h;
+    auto c = o->cycle_lengths();
+    for (size_t i = 0; i < c.size(); ++i) out[i] = c[i];
+    *n = (uint32_t)c.size();
+    return SPIMC_OK;
 }
 
 int spimc_oracle_move_counters(void* h, uint64_t* out) {

@@ -57,6 +57,12 @@ SIG = {
     "spimc_oracle_pair_link": ([_c, _i32, _d, _d, _d, _dp], _i32),
     "spimc_oracle_set_bisect_level": ([_c, _u32], _i32),
     "spimc_oracle_free_spline_log_rho": ([_d, _i32, _d, _d, _dp, _u32, _dp], _i32),
+    "spimc_oracle_perm_bisect_stage": ([_c, _u32, _pu8], _i32),
+    "spimc_oracle_perm_counters": ([_c, _pu64], _i32),
+    "spimc_oracle_perm_configure": ([_c, _d], _i32),
+    "spimc_oracle_perm_get": ([_c, _pu32], _i32),
+    "spimc_oracle_perm_set": ([_c, _pu32], _i32),
+    "spimc_oracle_cycle_lengths": ([_c, _pu32, _pu32], _i32),
 }
 
 _lib = None
@@ -126,6 +132,7 @@ class Oracle:
     def __init__(self, system, seed: int = 1):
         self.L = lib()
         self.s = system
+        self.n_part = system.n_part
         self._desc = system.desc()
         h = C.c_void_p()
         check(self.L.spimc_oracle_create(C.byref(self._desc), seed, C.byref(h)))
@@ -243,6 +250,35 @@ class Oracle:
         acc = np.zeros(max(n, 1), dtype=np.uint8)
         check(self.L.spimc_oracle_displace_stage(self.h, n, acc.ctypes.data_as(_pu8)))
         return acc[:n]
+
+    def perm_bisect(self, n):
+        acc = np.zeros(max(n, 1), dtype=np.uint8)
+        check(self.L.spimc_oracle_perm_bisect_stage(self.h, n, acc.ctypes.data_as(_pu8)))
+        return acc[:n]
+
+    def perm_counters(self):
+        """(attempts, accepts, perm_attempt[N], perm_accept[N]) of the permuting bisection"""
+        out = np.zeros(2 + 2 * self.n_part, dtype=np.uint64)
+        check(self.L.spimc_oracle_perm_counters(self.h, out.ctypes.data_as(_pu64)))
+        return int(out[0]), int(out[1]), out[2:2 + self.n_part], out[2 + self.n_part:]
+
+    def perm_configure(self, epsilon=1e-100):
+        check(self.L.spimc_oracle_perm_configure(self.h, epsilon))
+
+    def perm(self):
+        out = np.zeros(self.n_part, dtype=np.uint32)
+        check(self.L.spimc_oracle_perm_get(self.h, out.ctypes.data_as(_pu32)))
+        return out
+
+    def set_perm(self, sigma):
+        s = np.ascontiguousarray(sigma, dtype=np.uint32)
+        check(self.L.spimc_oracle_perm_set(self.h, s.ctypes.data_as(_pu32)))
+
+    def cycle_lengths(self):
+        out = np.zeros(self.n_part, dtype=np.uint32)
+        n = C.c_uint32()
+        check(self.L.spimc_oracle_cycle_lengths(self.h, out.ctypes.data_as(_pu32), C.byref(n)))
+        return out[:n.value]
 
     def counters(self):
         out = np.zeros(4, dtype=np.uint64)
