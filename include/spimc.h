@@ -291,6 +291,36 @@ int spimc_displace_stage(spimc_ctx* ctx, const uint32_t* n_attempts, uint32_t n_
  * out[0..3] = bisect attempts, bisect accepts, displace attempts, displace accepts. */
 int spimc_move_counters(spimc_ctx* ctx, uint32_t replica, uint64_t* out);
 
+/* ---- Permutations (events/moves/single_species_move/bisect/perm_bisect/) --
+ * A replica's path topology is the permutation sigma at the wrap: the bead after
+ * (p, n_bead - 1) is (sigma[p], 0), where PermBisect::AssignParticleLabels
+ * (perm_bisect_class.h:47-56) leaves every permutation.  The kinetic action,
+ * the bisection walks, Kinetic::DActionDBeta / VirialEnergy and the adjoint's
+ * kinetic term follow it; the pair action and rho_k stay per label and slice
+ * (Path::DrDrpDrrp, path_class.h:137-150).  Identity at creation. */
+
+/* PermBisectIterative's attributes n_level and epsilon (perm_bisect_class.h:104,
+ * bisect_class.h:168) for one replica (defaults: the descriptor's bisect level,
+ * 1e-100).  Synchronises the device. */
+int spimc_perm_configure(spimc_ctx* ctx, uint32_t replica, uint32_t n_level, double epsilon);
+/* n_attempts[r] consecutive Move::DoEvent calls of PermBisectIterative
+ * (perm_bisect_iterative_class.h:113-223, perm_bisect_class.h:59-90) on every
+ * replica; arguments as spimc_bisect_stage.  Replay draws per attempt: bead0, the
+ * first particle (integers as doubles); per walk step the continue uniform and,
+ * unless the walk stops there, the select uniform; then per level the chains'
+ * normals chain by chain (3 per midpoint in path order) and one uniform.
+ * Replicas hold at most 4096 particles. */
+int spimc_perm_bisect_stage(spimc_ctx* ctx, const uint32_t* n_attempts, uint32_t n_uniform, uint8_t* accepted);
+/* sigma[p] for p < n_part: PathDump's permutation row 1 (path_dump_class.h:43-47);
+ * upload is the Restart path of species_class.h:366-374.  Upload checks that
+ * sigma is a permutation. */
+int spimc_perm_download(spimc_ctx* ctx, uint32_t replica, uint32_t* sigma);
+int spimc_perm_upload(spimc_ctx* ctx, uint32_t replica, const uint32_t* sigma);
+/* PermBisectIterative counters since creation: out[0] attempts, out[1] accepts
+ * (Move::DoEvent), out[2 + c] perm_attempt and out[2 + n_part + c] perm_accept of
+ * cycle length c + 1 (perm_bisect_class.h:25-26,66-87).  out holds 2 + 2 n_part. */
+int spimc_perm_counters(spimc_ctx* ctx, uint32_t replica, uint64_t* out);
+
 /* ---- Observable interface (events/observables/observable_class.h:72-104) */
 
 /* Energy: estimator SPIMC_ENERGY_*, virial window (0 = n_bead).

@@ -1103,12 +1103,20 @@ struct Oracle {
             double x = draw();
             double t_Q = 0.;
             const uint32_t from = p;
+            bool found = false;
             for (uint32_t i = 0; i < N; ++i) {
                 t_Q += t_c[i] / Q_p_c;
                 if (t_Q > x) {
                     p = i;
+                    found = true;
                     break;
                 }
+            }
+            // a row whose running sum never passes x selects no particle; the
+            // reference would then revisit the same particle
+            if (!found) {
+                g_err = "permutation walk selected no particle";
+                return false;
             }
             // the weight's factors t(ps[i], ps[i+1]) / t(ps[i], ps[i]), in order
             w_num.push_back(t_row[p]);

@@ -122,6 +122,11 @@ SIGNATURES = {
     "spimc_bisect_stage": ([_c, _pu32, _u32, _pu8], _i32),
     "spimc_displace_stage": ([_c, _pu32, _u32, _pu8], _i32),
     "spimc_move_counters": ([_c, _u32, _pu64], _i32),
+    "spimc_perm_configure": ([_c, _u32, _u32, _d], _i32),
+    "spimc_perm_bisect_stage": ([_c, _pu32, _u32, _pu8], _i32),
+    "spimc_perm_download": ([_c, _u32, _pu32], _i32),
+    "spimc_perm_upload": ([_c, _u32, _pu32], _i32),
+    "spimc_perm_counters": ([_c, _u32, _pu64], _i32),
     "spimc_obs_configure": ([_c, _i32, _u32, _d, _d, _d, _u32], _i32),
     "spimc_obs_accumulate": ([_c, _u32], _i32),
     "spimc_obs_read": ([_c, _u32, _u32, _dp, _pu64], _i32),
@@ -303,6 +308,29 @@ class Engine:
         out = np.zeros(4, dtype=np.uint64)
         check(self.lib.spimc_move_counters(self.h, r, out.ctypes.data_as(_pu64)))
         return out
+
+    # ---- permutations ----------------------------------------------------------
+    def perm_configure(self, n_level: int, epsilon: float = 1e-100, r: int = 0):
+        check(self.lib.spimc_perm_configure(self.h, r, n_level, epsilon))
+
+    def perm_bisect(self, n):
+        return self._stage(self.lib.spimc_perm_bisect_stage, n)
+
+    def perm(self, r: int = 0) -> np.ndarray:
+        out = np.zeros(self.systems[r].n_part, dtype=np.uint32)
+        check(self.lib.spimc_perm_download(self.h, r, out.ctypes.data_as(_pu32)))
+        return out
+
+    def set_perm(self, sigma, r: int = 0):
+        s = np.ascontiguousarray(sigma, dtype=np.uint32)
+        check(self.lib.spimc_perm_upload(self.h, r, s.ctypes.data_as(_pu32)))
+
+    def perm_counters(self, r: int = 0):
+        """(attempts, accepts, perm_attempt[N], perm_accept[N]) of the permuting bisection"""
+        n = self.systems[r].n_part
+        out = np.zeros(2 + 2 * n, dtype=np.uint64)
+        check(self.lib.spimc_perm_counters(self.h, r, out.ctypes.data_as(_pu64)))
+        return int(out[0]), int(out[1]), out[2:2 + n], out[2 + n:]
 
     # ---- observables ---------------------------------------------------------
     def obs_configure(self, energy=ENERGY_THERMAL, virial_window=0, sk_k_cut=1e300, gr_r_min=0.0, gr_r_max=0.0, gr_n_r=100):

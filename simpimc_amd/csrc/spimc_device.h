@@ -68,9 +68,13 @@ struct Prop {
     double k_new;
     double prev;  // action change of the previous (coarser) level
     double u0;    // level-0 uniform
-    double pad;
+    int p2;       // label of the moved chain after the wrap (bisect; else p)
+    int generic;  // 1: the label-general pair kernel evaluates this proposal
     long long tape_next;  // tape cursor after this attempt
-    long long pad2;
+    int n_chain;  // chains of a permuting bisection (0: the cycle walk stopped)
+    int e_slice;  // slice after which a permuting bisection relinks the chains
+    int n_level;  // bisection level of a permuting bisection
+    int pad3;
 };
 
 struct Counters {
@@ -169,7 +173,31 @@ struct Rep {
     double* grad;           // [M][N][3] dS/dr of the last adjoint
     double* grad_est;       // [M][N][3] estimator scratch gradient
     unsigned long long* n_measure;  // [3]
+    // path topology: perm[p] is the particle whose slice-0 bead follows the
+    // slice-(M-1) bead of particle p, iperm its inverse (nullptr: the identity)
+    int* perm;
+    int* iperm;
+    // permuting bisection (nullptr until the first permuting stage)
+    int* ch_lbl;         // [N] chain labels before the wrap, in cycle order
+    int* ch_lbl2;        // [N] chain labels after the wrap
+    int* mv_pre;         // [N] chain of a label before the wrap, -1 if none
+    int* mv_post;        // [N] chain of a label after the wrap, -1 if none
+    double* ch_new;      // [N][M - 1][3] NEW positions of the chains' moved beads
+    unsigned long long* perm_cnt;  // [2 + 2N] attempts, accepts, perm_attempt[N], perm_accept[N]
+    double perm_log_eps;            // log of the table cutoff epsilon
+    int perm_n_level;               // bisection level of the permuting move
+    int pad5;
 };
+
+// Path topology (Bead::next_ / prev_ across the wrap, which is where
+// PermBisect::AssignParticleLabels leaves every permutation,
+// perm_bisect_class.h:47-56); within slices 0..M-1 a bead is followed by the
+// same label.
+__device__ __forceinline__ int label_after_wrap(const Rep& R, int p) { return R.perm ? R.perm[p] : p; }
+__device__ __forceinline__ int label_before_wrap(const Rep& R, int p) { return R.iperm ? R.iperm[p] : p; }
+// label of the bead after (p, s) and before (p, s)
+__device__ __forceinline__ int next_label(const Rep& R, int s, int p) { return s == R.M - 1 ? label_after_wrap(R, p) : p; }
+__device__ __forceinline__ int prev_label(const Rep& R, int s, int p) { return s == 0 ? label_before_wrap(R, p) : p; }
 
 // ---------------------------------------------------------------------------
 // splines

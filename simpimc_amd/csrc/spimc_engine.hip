@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "../../include/spimc.h"
-#include "spimc_kernels.h"
+#include "spimc_perm.h"
 
 using namespace spimc;
 
@@ -231,6 +231,8 @@ struct RepHost {
     unsigned long long* gr_cnt = nullptr;
     uint32_t accepted_cap = 0;
     int n_chunks = 1;
+    char* perm_arena = nullptr;    // perm, iperm
+    char* perm_scratch = nullptr;  // the permuting move's chain arrays and counters
 };
 
 // Replicas with identical constant tables (k-set, Ewald coefficients, pair and
@@ -389,6 +391,8 @@ struct spimc_ctx {
     unsigned long long* d_work = nullptr;
     unsigned int* d_error = nullptr;
     unsigned long long work_seen[WORK_COUNT] = {0, 0};
+    // a replica's path may be permuted: rounds also launch the label-general pair kernel
+    bool perm_on = false;
 };
 
 namespace {
@@ -762,6 +766,8 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     R.nk = nk;
     R.max_idx = mx;
     R.n_level = s.bisect_n_level;
+    R.perm_n_level = s.bisect_n_level;
+    R.perm_log_eps = std::log(1.e-100);  // PermBisect epsilon default (perm_bisect_class.h:104)
     R.pair_type = s.pair_type;
     R.use_long_range = lr ? 1 : 0;
     R.is_coulomb = (s.pair_type == SPIMC_PAIR_BARE && s.bare_is_coulomb) ? 1 : 0;
@@ -1053,7 +1059,34 @@ int launch_rho(spimc_ctx* c, hipStream_t st, const RhoItem* ri, int nr, unsigned
     return launch(c, SPIMC_KERNEL_RHO, c->rho_loop_fn, g, kRhoBlock, 0, st, ri, nr, kt);
 }
 
+// Label-general pair kernel of a round: workgroups per replica that loop over
+// the window links (only proposals marked generic do work).
+constexpr int kPairMultiBpr = 16;
+int launch_pair_multi(spimc_ctx* c, const Group& G, hipStream_t st) {
+    return launch(c, SPIMC_KERNEL_PAIR, k_pair_multi, G.n * kPairMultiBpr, kPairMultiBlock, 0, st, c->d_reps + G.r0, G.n,
+                  kPairMultiBpr);
+}
+
+// A permuting bisection round: propose, label-general pair and rho_k, decide.
+int launch_perm_round(spimc_ctx* c, const Group& G) {
+    const int R = G.n;
+    Rep* reps = c->d_reps + G.r0;
+    hipStream_t st = G.stream;
+    int rc;
+    const size_t shmem = (size_t)c->max_N * (sizeof(double) + sizeof(int));
+    if ((rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_perm, R, kPermBlock, shmem, st, reps, R))) return rc;
+    if ((rc = launch_pair_multi(c, G, st))) return rc;
+    if (c->any_long) {
+        int max_chunks = 1;
+        for (int r = G.r0; r < G.r0 + G.n; ++r) max_chunks = std::max(max_chunks, c->reps[r].dev.n_chunks);
+        dim3 grid(max_chunks, std::max(1, G.max_M - 1), R);
+        if ((rc = launch(c, SPIMC_KERNEL_RHO, k_rho_multi, grid, kRhoBlock, 0, st, reps))) return rc;
+    }
+    return launch(c, SPIMC_KERNEL_DECIDE, k_decide, R, 64, 0, st, reps, R, (unsigned long long*)nullptr);
+}
+
 int launch_round(spimc_ctx* c, const Group& G, int move) {
+    if (move == 3) return launch_perm_round(c, G);
     const int R = G.n;
     unsigned long long* kt = c->clock ? G.d_kt : nullptr;
     Rep* reps = c->d_reps + G.r0;  // kernels indexed by block: replicas of this group
@@ -1066,6 +1099,9 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
         rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_displace, R, 64, 0, st, reps, R);
     }
     if (rc) return rc;
+    // a bisection chain that crosses the wrap of a permuted path changes label:
+    // the label-general pair kernel takes those proposals
+    if (move == 0 && c->perm_on && (rc = launch_pair_multi(c, G, st))) return rc;
     const uint2* pi = move == 0 ? G.d_pair_bis : G.d_pair_dis;
     const int np = move == 0 ? G.n_pair_bis : G.n_pair_dis;
     const RhoItem* ri = move == 0 ? G.d_ritem_bis : G.d_ritem_dis;
@@ -1148,6 +1184,49 @@ int capture_round_graph(spimc_ctx* c, Group& G, int move) {
     return SPIMC_OK;
 }
 
+// The path topology arrays of every replica (the identity at first) and, with
+// scratch, the permuting move's chain arrays.  Once a path may be permuted the
+// rounds also launch the label-general pair kernel.
+int ensure_perm(spimc_ctx* c, bool scratch) {
+    bool changed = false;
+    for (auto& h : c->reps) {
+        Rep& R = h.dev;
+        if (!h.perm_arena) {
+            if (!changed) HIPCHK(hipDeviceSynchronize());
+            std::vector<int> id(2 * (size_t)R.N);
+            for (int p = 0; p < R.N; ++p) id[p] = id[R.N + p] = p;
+            HIPCHK(hipMalloc((void**)&h.perm_arena, id.size() * sizeof(int)));
+            HIPCHK(hipMemcpy(h.perm_arena, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice));
+            R.perm = reinterpret_cast<int*>(h.perm_arena);
+            R.iperm = R.perm + R.N;
+            changed = true;
+        }
+        if (scratch && !h.perm_scratch) {
+            if (R.N > kPermMaxN) return fail(SPIMC_ERR_UNSUPPORTED, "permuting moves support at most 4096 particles");
+            if (!changed) HIPCHK(hipDeviceSynchronize());
+            const size_t N = (size_t)R.N, M = (size_t)R.M;
+            auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+            const size_t b_int = up(N * sizeof(int)), b_cnt = up((2 + 2 * N) * sizeof(unsigned long long));
+            const size_t b_new = up(N * (M - 1) * 3 * sizeof(double));
+            HIPCHK(hipMalloc((void**)&h.perm_scratch, 4 * b_int + b_cnt + b_new));
+            char* q = h.perm_scratch;
+            R.ch_lbl = reinterpret_cast<int*>(q);
+            R.ch_lbl2 = reinterpret_cast<int*>(q + b_int);
+            R.mv_pre = reinterpret_cast<int*>(q + 2 * b_int);
+            R.mv_post = reinterpret_cast<int*>(q + 3 * b_int);
+            R.perm_cnt = reinterpret_cast<unsigned long long*>(q + 4 * b_int);
+            R.ch_new = reinterpret_cast<double*>(q + 4 * b_int + b_cnt);
+            HIPCHK(hipMemset(q, 0, 4 * b_int + b_cnt));
+            HIPCHK(hipMemset(R.mv_pre, 0xff, 2 * b_int));  // -1: no chain
+            changed = true;
+        }
+    }
+    if (!changed) return SPIMC_OK;
+    c->perm_on = true;
+    drop_graphs(c);
+    return upload_reps(c);
+}
+
 int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform, uint8_t* accepted, bool use_graph) {
     if (c->prop_set) return fail(SPIMC_ERR_STATE, "an interface proposal is pending; accept or reject it first");
     const int R = (int)c->reps.size();
@@ -1160,7 +1239,7 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
     if (mx == 0) return SPIMC_OK;
     int rc = ensure_accepted(c, mx);
     if (rc) return rc;
-    const bool graph = use_graph && !c->timing;
+    const bool graph = use_graph && !c->timing && move < 2;
     if (graph)
         for (auto& G : c->groups)
             if ((rc = capture_round_graph(c, G, move))) return rc;
@@ -1541,6 +1620,8 @@ int spimc_ctx_destroy(spimc_ctx* c) {
         if (h.accepted) (void)hipFree(h.accepted);
         if (h.tape) (void)hipFree(h.tape);
         if (h.gr_cnt) (void)hipFree(h.gr_cnt);
+        if (h.perm_arena) (void)hipFree(h.perm_arena);
+        if (h.perm_scratch) (void)hipFree(h.perm_scratch);
     }
     for (void* p : {(void*)c->d_reps, (void*)c->d_counts, (void*)c->d_work, (void*)c->d_gr_rmax, (void*)c->d_error,
                     (void*)c->d_tmp, (void*)c->d_items_tmp})
@@ -1628,6 +1709,7 @@ static int write_iface_prop(spimc_ctx* c, uint32_t r, int p, int b0, int n_links
     pr.status = 0;
     pr.move = 2;
     pr.p = p;
+    pr.p2 = p;
     pr.b0 = b0;
     pr.n_links = n_links;
     const bool mine = c->prop_set && c->prop_rep == (int)r && c->prop_p == p;
@@ -1673,7 +1755,15 @@ int spimc_get_action(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0
     if (action == SPIMC_ACTION_KINETIC) {
         if (level >= kMaxKinSplines || (R.kin_images > 0 && level >= c->reps[r].kin.size()))
             return fail(SPIMC_ERR_UNSUPPORTED, "kinetic level out of range");
-        const uint32_t span = (b1 - b0) % (uint32_t)R.M;
+        uint32_t span = (b1 - b0) % (uint32_t)R.M;
+        // a whole-particle walk returns to its first bead only when the particle's
+        // path closes on itself (kinetic_class.h:111-118)
+        if (span == 0 && b1 > b0 && R.perm) {
+            int nxt = p;
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipMemcpy(&nxt, R.perm + p, sizeof(int), hipMemcpyDeviceToHost));
+            if (nxt != p) span = (uint32_t)R.M;
+        }
         if (span % skip) return fail(SPIMC_ERR_ARG, "window not a multiple of the level step");
         const int nl = (int)(span / skip);
         if ((rc = write_iface_prop(c, r, p, (int)b0, 0))) return rc;
@@ -1881,6 +1971,81 @@ int spimc_bisect_stage(spimc_ctx* c, const uint32_t* n_attempts, uint32_t n_unif
 int spimc_displace_stage(spimc_ctx* c, const uint32_t* n_attempts, uint32_t n_uniform, uint8_t* accepted) {
     if (!c) return fail(SPIMC_ERR_ARG, "null context");
     return run_stage(c, 1, n_attempts, n_uniform, accepted, false);
+}
+
+int spimc_perm_configure(spimc_ctx* c, uint32_t r, uint32_t n_level, double epsilon) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    RepHost& h = c->reps[r];
+    const int max_lvl = std::min((int)std::floor(std::log2((double)h.dev.M)), kMaxLevel);
+    if (n_level < 1 || (int)n_level > max_lvl)
+        return fail(SPIMC_ERR_UNSUPPORTED, "permuting bisection n_level must be in [1, min(6, floor(log2 n_bead))]");
+    if (!(epsilon > 0.)) return fail(SPIMC_ERR_ARG, "epsilon must be > 0");
+    HIPCHK(hipDeviceSynchronize());
+    h.dev.perm_n_level = (int)n_level;
+    h.dev.perm_log_eps = std::log(epsilon);
+    return upload_reps(c);
+}
+
+int spimc_perm_bisect_stage(spimc_ctx* c, const uint32_t* n_attempts, uint32_t n_uniform, uint8_t* accepted) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    int rc = ensure_perm(c, true);
+    if (rc) return rc;
+    return run_stage(c, 3, n_attempts, n_uniform, accepted, false);
+}
+
+int spimc_perm_download(spimc_ctx* c, uint32_t r, uint32_t* sigma) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!sigma) return fail(SPIMC_ERR_ARG, "null argument");
+    const Rep& R = c->reps[r].dev;
+    HIPCHK(hipDeviceSynchronize());
+    if (!R.perm) {
+        for (int p = 0; p < R.N; ++p) sigma[p] = (uint32_t)p;
+        return SPIMC_OK;
+    }
+    std::vector<int> v(R.N);
+    HIPCHK(hipMemcpy(v.data(), R.perm, R.N * sizeof(int), hipMemcpyDeviceToHost));
+    for (int p = 0; p < R.N; ++p) sigma[p] = (uint32_t)v[p];
+    return SPIMC_OK;
+}
+
+int spimc_perm_upload(spimc_ctx* c, uint32_t r, const uint32_t* sigma) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!sigma) return fail(SPIMC_ERR_ARG, "null argument");
+    const int N = c->reps[r].dev.N;
+    std::vector<int> v(2 * (size_t)N, -1);
+    bool ident = true;
+    for (int p = 0; p < N; ++p) {
+        if (sigma[p] >= (uint32_t)N || v[N + sigma[p]] >= 0) return fail(SPIMC_ERR_ARG, "not a permutation");
+        v[p] = (int)sigma[p];
+        v[N + sigma[p]] = p;
+        ident = ident && sigma[p] == (uint32_t)p;
+    }
+    if (ident && !c->reps[r].dev.perm) return SPIMC_OK;
+    if ((rc = ensure_perm(c, false))) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(c->reps[r].dev.perm, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->adjoint_fresh = false;
+    return SPIMC_OK;
+}
+
+int spimc_perm_counters(spimc_ctx* c, uint32_t r, uint64_t* out) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!out) return fail(SPIMC_ERR_ARG, "null argument");
+    const Rep& R = c->reps[r].dev;
+    const size_t n = 2 + 2 * (size_t)R.N;
+    HIPCHK(hipDeviceSynchronize());
+    if (!R.perm_cnt) {
+        for (size_t i = 0; i < n; ++i) out[i] = 0;
+        return SPIMC_OK;
+    }
+    std::vector<unsigned long long> v(n);
+    HIPCHK(hipMemcpy(v.data(), R.perm_cnt, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) out[i] = v[i];
+    return SPIMC_OK;
 }
 
 int spimc_move_counters(spimc_ctx* c, uint32_t r, uint64_t* out) {

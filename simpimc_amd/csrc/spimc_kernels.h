@@ -202,9 +202,6 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
         return;
     }
     const int n = R.n_level, nb = 1 << n, N = R.N, M = R.M;
-    // 2^n_level == M: the window's last bead is its first one, so the reference's
-    // bead walks visit no midpoint and no kinetic link (bisect_class.h:51-58,78)
-    const bool wraps = (nb % M) == 0;
     Rng rng{make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32)), 1u, s_seq};
     const bool tape = R.tape != nullptr;
     int bad = 0;
@@ -232,10 +229,17 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
     }
     __syncthreads();
     const int p = s_info[1], b0 = s_info[2];
+    // the chain walked from (p, b0) carries label p2 after the wrap
+    const int p2 = label_after_wrap(R, p);
+    // 2^n_level == M and the chain closes on itself: the window's last bead is
+    // its first one, so the reference's bead walks visit no midpoint and no
+    // kinetic link (bisect_class.h:51-58,78)
+    const bool wraps = (nb % M) == 0 && p2 == p;
     for (int j = lane; j <= nb; j += kWave) {
         int s = (b0 + j) % M;
+        const int lbl = (b0 + j >= M) ? p2 : p;
         for (int d = 0; d < 3; ++d) {
-            double x = pos_at(R, d, s, p);
+            double x = pos_at(R, d, s, lbl);
             sh_old[j][d] = x;
             sh_new[j][d] = x;
         }
@@ -311,6 +315,8 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
                     pr->status = 2;
                     pr->move = 0;
                     pr->p = p;
+                    pr->p2 = p2;
+                    pr->generic = 0;
                     pr->b0 = b0;
                     pr->count = 0;
                     pr->tape_next = cur;
@@ -326,6 +332,10 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
                 pr->status = status;
                 pr->move = 0;
                 pr->p = p;
+                pr->p2 = p2;
+                // moved beads on both sides of the wrap under two labels: the
+                // label-general pair kernel takes the proposal
+                pr->generic = (p2 != p) && (b0 + nb > M);
                 pr->b0 = b0;
                 pr->n_links = R.pair_type ? nb : 0;  // no pair action: no links to evaluate
                 pr->first = (b0 + 1) % M;
@@ -348,8 +358,12 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
     }
 }
 
-// DisplaceParticle::Attempt (displace_particle_class.h:28-72): NEW beads only;
-// the kinetic action of a rigid shift is 0 (the bead walk is empty).
+// DisplaceParticle::Attempt (displace_particle_class.h:28-72): NEW beads only.
+// Kinetic::GetAction(0, M, {p}) walks from (p, 0) to the bead M links later:
+// the walk is empty when the particle's path closes on itself, else it holds
+// the wrap link (p, M-1) -> (perm[p], 0), the one link of the walk a rigid shift
+// changes.  The link into (p, 0) from (iperm[p], M-1), which the shift also
+// changes, is outside the walk and not counted, as in the reference.
 __global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__ reps, int n_reps) {
     const int r = blockIdx.x;
     if (r >= n_reps) return;
@@ -388,16 +402,29 @@ __global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__
                 u0 = rng.unif(2, 1);
             }
             s_p = p;
+            double k_old = 0., k_new = 0.;
+            const int pn = label_after_wrap(R, p);
+            if (pn != p) {
+                const double i4lt = 1. / (4. * R.lambda * R.tau);
+                double dd[3];
+                for (int d = 0; d < 3; ++d) dd[d] = minimg(pos_at(R, d, R.M - 1, p) - pos_at(R, d, 0, pn), R.L, R.iL);
+                k_old = free_neg_log(&R.kin[0], R.kin_images, i4lt, dd);
+                for (int d = 0; d < 3; ++d)
+                    dd[d] = minimg((pos_at(R, d, R.M - 1, p) + s_dr[d]) - pos_at(R, d, 0, pn), R.L, R.iL);
+                k_new = free_neg_log(&R.kin[0], R.kin_images, i4lt, dd);
+            }
             pr->status = bad ? 3 : 1;
             pr->move = 1;
             pr->p = p;
+            pr->p2 = p;
+            pr->generic = 0;
             pr->b0 = 0;
             pr->n_links = R.pair_type ? R.M : 0;
             pr->first = 0;
             pr->count = R.M;
             pr->lsr = 0.;
-            pr->k_old = 0.;
-            pr->k_new = 0.;
+            pr->k_old = k_old;
+            pr->k_new = k_new;
             pr->prev = 0.;
             pr->u0 = u0;
             pr->tape_next = cur;
@@ -455,7 +482,7 @@ __device__ __forceinline__ void pair_block(const Rep* __restrict__ reps, const u
     const uint2 it = items[item];
     const Rep& R = reps[it.x];
     const Prop pr = *R.prop;
-    if (!interface_mode && pr.status != 1) return;
+    if (!interface_mode && (pr.status != 1 || pr.generic)) return;
     const int j = (int)it.y;
     if (j >= pr.n_links) return;
     const int M = R.M, N = R.N, p = pr.p;
@@ -548,7 +575,7 @@ __device__ __forceinline__ void pair_lds_block(const Rep* __restrict__ reps, int
     if (r >= n_reps) return;
     const Rep R = reps[r];  // by value: the fields used stay in registers across the loops
     const Prop pr = *R.prop;
-    if (pr.status != 1) return;
+    if (pr.status != 1 || pr.generic) return;
     for (int t = threadIdx.x; t < R.lds_doubles; t += BS) lds[t] = (t < R.bt_a.ncoef * R.bt_a.ncomp) ? R.bt_a.c[t] : R.bt_b.c[t - R.bt_a.ncoef * R.bt_a.ncomp];
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
@@ -817,7 +844,7 @@ __global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int
         }
         s_acc = acc;
         Counters* c = R.cnt;
-        if (pr.status == 1) {  // the round's pair and rho_k work, for the roofline figures
+        if (pr.status == 1 && pr.move != 3) {  // the round's pair and rho_k work, for the roofline figures
             c->rho_slicek += (unsigned long long)pr.count * (unsigned long long)R.nk;
             if (R.pair_type) c->pair_links += 2ull * (unsigned long long)pr.n_links * (unsigned long long)(R.N - 1);
         }
@@ -825,6 +852,17 @@ __global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int
             c->bis_att += 1;
             c->bis_acc += acc;
             c->bead_updates += (1ull << R.n_level) - 1;
+        } else if (pr.move == 3) {
+            // Move::DoEvent counts every attempt; PermBisect counts per cycle length
+            // the attempts that reached the bisection (perm_bisect_class.h:66-87)
+            unsigned long long* pc = R.perm_cnt;
+            pc[0] += 1;
+            pc[1] += acc;
+            if (pr.n_chain > 0) {
+                pc[2 + pr.n_chain - 1] += 1;
+                pc[2 + R.N + pr.n_chain - 1] += acc;
+            }
+            c->bead_updates += (unsigned long long)pr.n_chain * ((1ull << pr.n_level) - 1);
         } else {
             c->dis_att += 1;
             c->dis_acc += acc;
@@ -841,11 +879,67 @@ __global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int
         prp->status = 0;
     }
     __syncthreads();
-    if (s_acc) {
+    const int M = R.M, N = R.N;
+    if (s_acc && pr.move == 3) {
+        // Bisect::Accept of every chain, then PermBisect::AssignParticleLabels
+        // (perm_bisect_class.h:47-56): the slices after the relinked slice e take
+        // the labels of the path they now continue, so chain label a_i there
+        // holds what a_{i+1} held, and the wrap permutation follows.
+        const int n = pr.n_chain, nm = pr.count, nb = nm + 1;
+        for (int idx = lane; idx < n * nm; idx += kWave) {
+            const int i = idx / nm, t = idx % nm + 1;
+            const int s = (pr.b0 + t) % M, l = (pr.b0 + t >= M) ? R.ch_lbl2[i] : R.ch_lbl[i];
+            for (int d = 0; d < 3; ++d) R.pos[((size_t)d * M + s) * N + l] = R.ch_new[(size_t)idx * 3 + d];
+        }
+        if (R.use_long_range)
+            for (int m = lane; m < nm; m += kWave) R.sel[(pr.first + m) % M] ^= 1;
+        __syncthreads();
+        if (n > 1) {
+            const bool wrapped = pr.b0 + nb - 1 >= M;
+            const int e = pr.e_slice;
+            for (int idx = lane; idx < (M - 1 - e) * 3; idx += kWave) {
+                const int s = e + 1 + idx / 3, d = idx % 3;
+                double* col = R.pos + ((size_t)d * M + s) * N;
+                const int a0 = wrapped ? R.ch_lbl2[0] : R.ch_lbl[0];
+                const double keep = col[a0];
+                int ai = a0;
+                for (int i = 0; i + 1 < n; ++i) {
+                    const int an = wrapped ? R.ch_lbl2[i + 1] : R.ch_lbl[i + 1];
+                    col[ai] = col[an];
+                    ai = an;
+                }
+                col[ai] = keep;
+            }
+            if (lane == 0) {
+                const int a0 = wrapped ? R.ch_lbl2[0] : R.ch_lbl[0];
+                const int keep = R.perm[a0];
+                int ai = a0;
+                for (int i = 0; i + 1 < n; ++i) {
+                    const int an = wrapped ? R.ch_lbl2[i + 1] : R.ch_lbl[i + 1];
+                    R.perm[ai] = R.perm[an];
+                    ai = an;
+                }
+                R.perm[ai] = keep;
+                for (int i = 0; i < n; ++i) {
+                    const int a = wrapped ? R.ch_lbl2[i] : R.ch_lbl[i];
+                    R.iperm[R.perm[a]] = a;
+                }
+            }
+        }
+    } else if (s_acc) {
         for (int m = lane; m < pr.count; m += kWave) {
-            const int s = (pr.first + m) % R.M;
-            for (int d = 0; d < 3; ++d) R.pos[((size_t)d * R.M + s) * R.N + pr.p] = R.prop_new[3 * m + d];
+            const int s = (pr.first + m) % M;
+            // a bisection chain carries label p2 past the wrap
+            const int l = (pr.move == 0 && pr.b0 + m + 1 >= M) ? pr.p2 : pr.p;
+            for (int d = 0; d < 3; ++d) R.pos[((size_t)d * M + s) * N + l] = R.prop_new[3 * m + d];
             if (R.use_long_range) R.sel[s] ^= 1;
+        }
+    }
+    if (pr.move == 3 && pr.status == 1) {
+        __syncthreads();
+        for (int i = lane; i < pr.n_chain; i += kWave) {
+            R.mv_pre[R.ch_lbl[i]] = -1;
+            R.mv_post[R.ch_lbl2[i]] = -1;
         }
     }
 }
@@ -887,12 +981,17 @@ __global__ __launch_bounds__(64) void k_kinetic_window(const Rep* __restrict__ r
     const int skip = 1 << level, M = R.M;
     const double i4lt = 1. / (4. * R.lambda * (R.tau * (double)skip));
     double tot = 0.;
+    // the walk from (p, b0) carries label perm[p] past the wrap; NEW mode reads
+    // the proposal's beads where the label is the proposal's particle
+    const int p2 = label_after_wrap(R, p);
     for (int i = threadIdx.x; i < n_links; i += kWave) {
-        const int a = (b0 + i * skip) % M, b = (a + skip) % M;
+        const int ta = i * skip, tb = ta + skip;
+        const int a = (b0 + ta) % M, b = (b0 + tb) % M;
+        const int la = (b0 + ta >= M) ? p2 : p, lb = (b0 + tb >= M) ? p2 : p;
         double dd[3];
         for (int d = 0; d < 3; ++d) {
-            double xa = (mode && pr.p == p) ? pos_new(R, pr, d, a) : pos_at(R, d, a, p);
-            double xb = (mode && pr.p == p) ? pos_new(R, pr, d, b) : pos_at(R, d, b, p);
+            double xa = (mode && pr.p == la) ? pos_new(R, pr, d, a) : pos_at(R, d, a, la);
+            double xb = (mode && pr.p == lb) ? pos_new(R, pr, d, b) : pos_at(R, d, b, lb);
             dd[d] = minimg(xa - xb, R.L, R.iL);
         }
         tot += free_neg_log(&R.kin[level], R.kin_images, i4lt, dd);
@@ -959,8 +1058,9 @@ __global__ __launch_bounds__(256) void k_kin_dbeta(const Rep* __restrict__ reps,
     const int b1 = (b + 1) % R.M;
     double t = 0.;
     for (int p = threadIdx.x; p < R.N; p += 256) {
+        const int pn = next_label(R, b, p);
         double dd[3];
-        for (int d = 0; d < 3; ++d) dd[d] = minimg(pos_at(R, d, b, p) - pos_at(R, d, b1, p), R.L, R.iL);
+        for (int d = 0; d < 3; ++d) dd[d] = minimg(pos_at(R, d, b, p) - pos_at(R, d, b1, pn), R.L, R.iL);
         double img = 0.;
         if (R.kin_images > 0)
             for (int d = 0; d < 3; ++d) img -= spl_eval1(R.kin_dtau, dd[d]);
@@ -1049,26 +1149,34 @@ __global__ __launch_bounds__(256) void k_kin_virial(const Rep* __restrict__ reps
     const int W = W_in > 0 ? W_in : M;
     const double wt = W * R.tau;
     double tot = 0.;
+    // the walks follow the path: past slice M - 1 a bead is followed by slice 0
+    // of the particle perm[label]
     for (int p = threadIdx.x; p < R.N; p += 256) {
         int b_0 = 0, b_1 = 1 % M, b_2 = 2 % M;
+        int l_0 = p, l_1 = next_label(R, 0, p), l_2 = next_label(R, b_1, l_1);
         double d1[3], dL[3], t[3];
-        for (int d = 0; d < 3; ++d) d1[d] = minimg(pos_at(R, d, b_1, p) - pos_at(R, d, b_0, p), R.L, R.iL);
+        for (int d = 0; d < 3; ++d) d1[d] = minimg(pos_at(R, d, b_1, l_1) - pos_at(R, d, b_0, l_0), R.L, R.iL);
         for (int d = 0; d < 3; ++d) dL[d] = d1[d];
         for (int sk = 1; sk < W; ++sk) {
-            for (int d = 0; d < 3; ++d) t[d] = minimg(pos_at(R, d, b_2, p) - pos_at(R, d, b_1, p), R.L, R.iL);
+            for (int d = 0; d < 3; ++d) t[d] = minimg(pos_at(R, d, b_2, l_2) - pos_at(R, d, b_1, l_1), R.L, R.iL);
             for (int d = 0; d < 3; ++d) dL[d] += t[d];
             b_1 = b_2;
+            l_1 = l_2;
+            l_2 = next_label(R, b_2, l_2);
             b_2 = (b_2 + 1) % M;
         }
         tot -= (dL[0] * d1[0] + dL[1] * d1[1] + dL[2] * d1[2]) * R.kin_i4lt / wt;
         for (int bi = 1; bi < M; ++bi) {
             for (int d = 0; d < 3; ++d) dL[d] -= d1[d];
-            for (int d = 0; d < 3; ++d) t[d] = minimg(pos_at(R, d, b_2, p) - pos_at(R, d, b_1, p), R.L, R.iL);
+            for (int d = 0; d < 3; ++d) t[d] = minimg(pos_at(R, d, b_2, l_2) - pos_at(R, d, b_1, l_1), R.L, R.iL);
             for (int d = 0; d < 3; ++d) dL[d] += t[d];
+            l_0 = next_label(R, b_0, l_0);
             b_0 = (b_0 + 1) % M;
-            const int bn = (b_0 + 1) % M;
-            for (int d = 0; d < 3; ++d) d1[d] = minimg(pos_at(R, d, bn, p) - pos_at(R, d, b_0, p), R.L, R.iL);
+            const int bn = (b_0 + 1) % M, ln = next_label(R, b_0, l_0);
+            for (int d = 0; d < 3; ++d) d1[d] = minimg(pos_at(R, d, bn, ln) - pos_at(R, d, b_0, l_0), R.L, R.iL);
             b_1 = b_2;
+            l_1 = l_2;
+            l_2 = next_label(R, b_2, l_2);
             b_2 = (b_2 + 1) % M;
             tot -= (dL[0] * d1[0] + dL[1] * d1[1] + dL[2] * d1[2]) * R.kin_i4lt / wt;
         }
@@ -1176,11 +1284,13 @@ __global__ __launch_bounds__(64) void k_adjoint_local(const Rep* __restrict__ re
         rp[d] = pos_at(R, d, bp, i);
     }
     if (terms & 1u) {
-        // K(d) = sum_x img(d_x) + |d|^2 i4lt, d = r_b - r_{b+1}: dK/dr_b = K'(d_fwd) - K'(d_bwd)
+        // K(d) = sum_x img(d_x) + |d|^2 i4lt, d = r_b - r_{b+1}: dK/dr_b = K'(d_fwd) - K'(d_bwd);
+        // the kinetic links follow the path across the wrap
+        const int kn = next_label(R, b, i), kp = prev_label(R, b, i);
         double df[3], db[3];
         for (int d = 0; d < 3; ++d) {
-            df[d] = minimg(ri[d] - rn[d], R.L, R.iL);
-            db[d] = minimg(rp[d] - ri[d], R.L, R.iL);
+            df[d] = minimg(ri[d] - (kn == i ? rn[d] : pos_at(R, d, bn, kn)), R.L, R.iL);
+            db[d] = minimg((kp == i ? rp[d] : pos_at(R, d, bp, kp)) - ri[d], R.L, R.iL);
         }
         for (int d = 0; d < 3; ++d) {
             double kf = 2. * df[d] * R.kin_i4lt, kb = 2. * db[d] * R.kin_i4lt;
@@ -1290,8 +1400,11 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_local_lds(const Rep* _
                 g[d] = carry[d];
             }
             if (terms & 1u) {
+                // the kinetic links follow the path across the wrap
+                const int kn = next_label(R, b, i), kp = prev_label(R, b, i);
                 for (int d = 0; d < 3; ++d) {
-                    const double df = minimg(ri[d] - rn[d], R.L, R.iL), db = minimg(rpv[d] - ri[d], R.L, R.iL);
+                    const double xn = kn == i ? rn[d] : pos_at(R, d, bn, kn), xp = kp == i ? rpv[d] : pos_at(R, d, bp, kp);
+                    const double df = minimg(ri[d] - xn, R.L, R.iL), db = minimg(xp - ri[d], R.L, R.iL);
                     double kf = 2. * df * R.kin_i4lt, kb = 2. * db * R.kin_i4lt;
                     if (R.kin_images > 0) {
                         int ia = spl_locate(R.kin[0], df), ib = spl_locate(R.kin[0], db);
@@ -1541,8 +1654,8 @@ __global__ __launch_bounds__(256) void k_centroid(const Rep* __restrict__ reps, 
             double gd = g[d];
             if (from_sweep) {
                 const double x = pos_at(R, d, b, p);
-                const double df = minimg(x - pos_at(R, d, bn, p), R.L, R.iL);
-                const double db = minimg(pos_at(R, d, bp, p) - x, R.L, R.iL);
+                const double df = minimg(x - pos_at(R, d, bn, next_label(R, b, p)), R.L, R.iL);
+                const double db = minimg(pos_at(R, d, bp, prev_label(R, b, p)) - x, R.L, R.iL);
                 double kf = 2. * df * R.kin_i4lt, kb = 2. * db * R.kin_i4lt;
                 if (R.kin_images > 0) {
                     int ia = spl_locate(R.kin[0], df), ib = spl_locate(R.kin[0], db);

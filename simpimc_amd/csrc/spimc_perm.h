@@ -1,0 +1,514 @@
+// spimc_perm.h -- permuting bisection (PermBisectIterative) and the kernels
+// that evaluate a proposal whose moved beads carry more than one particle
+// label: the chains of a permutation cycle, or a bisection chain whose label
+// changes where it crosses the wrap of a permuted path.
+//
+//   k_propose_perm  one workgroup per replica: bead0, the iterative cycle walk,
+//                   the bisection of every chain down to level 0
+//   k_pair_multi    workgroups per replica loop over the window links: the
+//                   label-based pair action of every label with a moved bead
+//                   on the link, OLD and NEW
+//   k_rho_multi     one workgroup per (replica, moved slice, k chunk): the
+//                   incremental rho_k of a slice with several moved beads
+// The decision and commit (relabelling included) stay in k_decide.
+#pragma once
+
+#include "spimc_kernels.h"
+
+namespace spimc {
+
+constexpr int kPermBlock = 256;
+constexpr int kPermMaxN = 4096;  // particles of a replica that runs permuting moves
+constexpr int kRhoKptMax = 16;   // largest k-vectors per thread of the rho_k chunks
+
+// Window position t (0..nb) of chain i of a permuting proposal: the chain is
+// walked from (ch_lbl[i], b0) and carries label ch_lbl2[i] past the wrap.
+__device__ __forceinline__ int chain_label(const Rep& R, int b0, int t, int i) {
+    return (b0 + t >= R.M) ? R.ch_lbl2[i] : R.ch_lbl[i];
+}
+__device__ __forceinline__ double chain_old(const Rep& R, int b0, int t, int i, int d) {
+    return pos_at(R, d, (b0 + t) % R.M, chain_label(R, b0, t, i));
+}
+// NEW mode: the start is shared; the end of chain i is the old end of chain
+// i + 1 (PermBisect::PermuteBeads, perm_bisect_class.h:32-44); the beads between
+// are the proposal's.
+__device__ __forceinline__ double chain_new(const Rep& R, int b0, int nb, int n, int t, int i, int d) {
+    if (t == 0) return chain_old(R, b0, 0, i, d);
+    if (t == nb) return chain_old(R, b0, nb, (i + 1) % n, d);
+    return R.ch_new[((size_t)i * (nb - 1) + t - 1) * 3 + d];
+}
+
+// ---------------------------------------------------------------------------
+// PermBisectIterative::Attempt (perm_bisect_iterative_class.h:113-223) down to
+// level 0.  Draws in the reference's order: bead0, the first particle, per
+// walk step the continue and select uniforms (SelectCycleIterative, :32-110),
+// then per level the chains' normals chain by chain and one uniform.  The coarse
+// levels carry only the kinetic action (pair_action_class.h:269).  The
+// permutation-table row of the current particle is rebuilt at each step: row p
+// of the reference's t_c is t(p, .) with the visited particles removed and the
+// closing link t(p, p0) restored (:48-53).
+// Dynamic LDS: N doubles (the row) and N ints (visited step, -1 if not).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restrict__ reps, int n_reps) {
+    const int r = blockIdx.x;
+    if (r >= n_reps) return;
+    const Rep& R = reps[r];
+    const int tid = threadIdx.x;
+    extern __shared__ double lds[];
+    double* t_row = lds;
+    int* vis = reinterpret_cast<int*>(lds + R.N);
+    __shared__ double red[kPermBlock / kWave];
+    __shared__ int s_i[8];
+    __shared__ double s_w[2];
+    __shared__ unsigned long long s_seq;
+    __shared__ long long s_cur;
+    Prop* pr = R.prop;
+    if (tid == 0) {
+        Counters* c = R.cnt;
+        unsigned int rem = c->remaining;
+        s_i[0] = rem > 0;
+        if (rem > 0) {
+            c->remaining = rem - 1;
+            s_seq = c->seq;
+            c->seq = s_seq + 1;
+        }
+    }
+    __syncthreads();
+    if (!s_i[0]) {
+        if (tid == 0) pr->status = 0;
+        return;
+    }
+    const int N = R.N, M = R.M, n_lev = R.perm_n_level, nb = 1 << n_lev;
+    const Rng rng{make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32)), 3u, s_seq};
+    const bool tape = R.tape != nullptr;
+    // device draw counters: 0 bead0 and p0; 1 + step the walk; N + 3 + level the
+    // level uniforms; N + 3 + n_level + g / 2 the normals
+    const uint32_t c_lev = (uint32_t)N + 3u, c_norm = c_lev + (uint32_t)n_lev;
+    for (int i = tid; i < N; i += kPermBlock) vis[i] = -1;
+    if (tid == 0) {
+        int bad = 0, b0, p0;
+        long long cur = 0;
+        if (tape) {
+            cur = *R.tape_cursor;
+            b0 = (int)tape_get(R, cur, &bad);
+            p0 = (int)tape_get(R, cur + 1, &bad);
+            cur += 2;
+            if (b0 < 0 || b0 >= M || p0 < 0 || p0 >= N) {
+                bad = 1;
+                b0 = 0;
+                p0 = 0;
+            }
+        } else {
+            b0 = (int)rng.index(0, 0, M);
+            p0 = (int)rng.index(0, 1, N);
+        }
+        s_i[1] = b0;
+        s_i[2] = p0;
+        s_i[3] = bad;
+        s_cur = cur;
+        s_w[0] = 1.;
+    }
+    __syncthreads();
+    const int b0 = s_i[1], p0 = s_i[2];
+    const int e_end = (b0 + nb) % M;
+    const bool end_wrapped = b0 + nb >= M;
+    const double i4ltn = 1. / (4. * R.lambda * R.tau) / nb;
+    // ---- the cycle walk ----
+    int p = p0, n = 0, stop = 0;  // stop: 1 closed, 2 walk stopped, 3 error
+    for (int step = 0; !stop; ++step) {
+        if (tid == 0) {
+            R.ch_lbl[n] = p;
+            vis[p] = n;
+        }
+        ++n;
+        {
+#pragma clang fp contract(off)
+            double a[3];
+            for (int d = 0; d < 3; ++d) a[d] = pos_at(R, d, b0, p);
+            for (int j = tid; j < N; j += kPermBlock) {
+                const int lj = end_wrapped ? label_after_wrap(R, j) : j;
+                double dd[3];
+                for (int d = 0; d < 3; ++d) dd[d] = minimg(a[d] - pos_at(R, d, e_end, lj), R.L, R.iL);
+                const double e = (-(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2])) * i4ltn;
+                t_row[j] = e > R.perm_log_eps ? exp(e) : 0.;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int bad = s_i[3];
+            long long cur = s_cur;
+            double q_p = 0., q_c = 0.;
+            for (int i = 0; i < N; ++i) {
+                const double ti = t_row[i];
+                q_p += ti;
+                q_c += (i == p0) ? ti : (vis[i] >= 0 ? 0. : ti);
+            }
+            double u = tape ? tape_get(R, cur++, &bad) : rng.unif(1u + (uint32_t)step, 0);
+            int st = 0, nx = p;
+            if (q_c / q_p < u) {
+                st = 2;
+            } else {
+                const double x = tape ? tape_get(R, cur++, &bad) : rng.unif(1u + (uint32_t)step, 1);
+                double t_q = 0.;
+                int found = -1;
+                for (int i = 0; i < N; ++i) {
+                    const double tc = (i == p0) ? t_row[i] : (vis[i] >= 0 ? 0. : t_row[i]);
+                    t_q += tc / q_c;
+                    if (t_q > x) {
+                        found = i;
+                        break;
+                    }
+                }
+                if (found < 0) {
+                    st = 3;  // no particle selected (a degenerate row)
+                } else {
+                    nx = found;
+                    // the cycle weight's factors t(ps[i], ps[i+1]) / t(ps[i], ps[i]), in order
+                    s_w[0] = s_w[0] * (t_row[nx] / t_row[p]);
+                    if (nx == p0)
+                        st = 1;
+                    else if (n >= N)
+                        st = 3;
+                }
+            }
+            s_i[3] = bad;
+            s_cur = cur;
+            s_i[4] = st;
+            s_i[5] = nx;
+        }
+        __syncthreads();
+        stop = s_i[4];
+        p = s_i[5];
+    }
+    long long cur = s_cur;
+    int bad = s_i[3];
+    // the visited map becomes the chain map before the wrap; chain labels after it
+    for (int i = tid; i < n; i += kPermBlock) {
+        const int l = R.ch_lbl[i];
+        R.mv_pre[l] = i;
+        const int l2 = label_after_wrap(R, l);
+        R.ch_lbl2[i] = l2;
+        R.mv_post[l2] = i;
+    }
+    if (stop != 1) {
+        if (tid == 0) {
+            pr->status = stop == 3 ? 3 : 2;
+            pr->move = 3;
+            pr->p = p0;
+            pr->p2 = p0;
+            pr->generic = 1;
+            pr->b0 = b0;
+            pr->count = 0;
+            pr->n_chain = 0;
+            pr->n_level = n_lev;
+            pr->tape_next = cur;
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += kPermBlock) {
+            R.mv_pre[R.ch_lbl[i]] = -1;
+            R.mv_post[R.ch_lbl2[i]] = -1;
+        }
+        return;
+    }
+    const double prev0 = -log(s_w[0]);
+    // NEW positions start as the old ones
+    const int nm = nb - 1;
+    for (int idx = tid; idx < n * nm; idx += kPermBlock) {
+        const int i = idx / nm, t = idx % nm + 1;
+        for (int d = 0; d < 3; ++d) R.ch_new[(size_t)idx * 3 + d] = chain_old(R, b0, t, i, d);
+    }
+    __syncthreads();
+    double prev = prev0;
+    uint32_t g_base = 0;
+    for (int level = n_lev - 1; level >= 0; --level) {
+        const int skip = 1 << level;
+        const int nmid = nb >> (level + 1);
+        const double level_tau = R.tau * skip;
+        const double sigma = sqrt(R.lambda * level_tau);
+        const double i4lt_b = 1. / (4. * R.lambda * (0.5 * R.tau * (double)skip));
+        double lspo = 0., lspn = 0.;
+        for (int it = tid; it < n * nmid; it += kPermBlock) {
+#pragma clang fp contract(off)
+            const int i = it / nmid, k = it % nmid;
+            const int a = 2 * k * skip, b = a + skip, c = a + 2 * skip;
+            double t[3], rbar[3], dold[3], dnew[3], nbv[3];
+            for (int d = 0; d < 3; ++d) {
+                const double xa = chain_old(R, b0, a, i, d);
+                t[d] = minimg(chain_old(R, b0, c, i, d) - xa, R.L, R.iL);
+                rbar[d] = xa + 0.5 * t[d];
+                dold[d] = minimg(chain_old(R, b0, b, i, d) - rbar[d], R.L, R.iL);
+            }
+            for (int d = 0; d < 3; ++d) {
+                double z;
+                if (tape) {
+                    z = tape_get(R, cur + 3 * it + d, &bad);
+                } else {
+                    const uint32_t g = g_base + 3u * (uint32_t)it + (uint32_t)d;
+                    z = rng.normal(c_norm + g / 2u, (int)(g & 1u));
+                }
+                dnew[d] = sigma * z;
+                dnew[d] = minimg(dnew[d], R.L, R.iL);
+                const double xa = chain_new(R, b0, nb, n, a, i, d);
+                t[d] = minimg(chain_new(R, b0, nb, n, c, i, d) - xa, R.L, R.iL);
+                rbar[d] = xa + 0.5 * t[d];
+                nbv[d] = rbar[d] + dnew[d];
+            }
+            for (int d = 0; d < 3; ++d) R.ch_new[((size_t)i * nm + b - 1) * 3 + d] = nbv[d];
+            lspo -= free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dold);
+            lspn -= free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dnew);
+        }
+        __syncthreads();
+        const int nlinks = nb >> level;
+        const double i4lt_k = 1. / (4. * R.lambda * level_tau);
+        double ko = 0., kn = 0.;
+        for (int it = tid; it < n * nlinks; it += kPermBlock) {
+            const int i = it / nlinks, a = (it % nlinks) * skip, b = a + skip;
+            double dd[3];
+            for (int d = 0; d < 3; ++d) dd[d] = minimg(chain_old(R, b0, a, i, d) - chain_old(R, b0, b, i, d), R.L, R.iL);
+            ko += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+            for (int d = 0; d < 3; ++d)
+                dd[d] = minimg(chain_new(R, b0, nb, n, a, i, d) - chain_new(R, b0, nb, n, b, i, d), R.L, R.iL);
+            kn += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+        }
+        lspo = block_sum<kPermBlock>(lspo, red);
+        lspn = block_sum<kPermBlock>(lspn, red);
+        ko = block_sum<kPermBlock>(ko, red);
+        kn = block_sum<kPermBlock>(kn, red);
+        cur += 3 * n * nmid;
+        g_base += 3u * (uint32_t)(n * nmid);
+        if (tid == 0) {
+            const double u = tape ? tape_get(R, cur, &bad) : rng.unif(c_lev + (uint32_t)level, 0);
+            const double lsr = -lspn + lspo;
+            int rej = 0;
+            if (level > 0) {
+                const double change = (0. + kn) - (0. + ko);
+                const double la = lsr - change + prev;
+                rej = la < log(u);
+                s_w[1] = change;
+            } else {
+                pr->status = bad ? 3 : 1;
+                pr->move = 3;
+                pr->p = R.ch_lbl[0];
+                pr->p2 = R.ch_lbl2[0];
+                pr->generic = 1;
+                pr->b0 = b0;
+                pr->n_links = R.pair_type ? nb : 0;
+                pr->first = (b0 + 1) % M;
+                pr->count = nm;
+                pr->lsr = lsr;
+                pr->k_old = ko;
+                pr->k_new = kn;
+                pr->prev = prev;
+                pr->u0 = u;
+                pr->tape_next = cur + 1;
+                pr->n_chain = n;
+                pr->e_slice = (b0 + nb - 1) % M;
+                pr->n_level = n_lev;
+            }
+            if (rej) {
+                pr->status = 2;
+                pr->move = 3;
+                pr->p = R.ch_lbl[0];
+                pr->p2 = R.ch_lbl2[0];
+                pr->generic = 1;
+                pr->b0 = b0;
+                pr->count = 0;
+                pr->n_chain = n;
+                pr->n_level = n_lev;
+                pr->tape_next = cur + 1;
+            }
+            s_i[6] = rej;
+        }
+        cur += 1;
+        __syncthreads();
+        if (s_i[6]) {
+            for (int i = tid; i < n; i += kPermBlock) {
+                R.mv_pre[R.ch_lbl[i]] = -1;
+                R.mv_post[R.ch_lbl2[i]] = -1;
+            }
+            return;
+        }
+        prev = s_w[1];
+    }
+    // level 0 reached: the rho_k rows of the moved slices
+    if (R.use_long_range)
+        for (int m = tid; m < nm; m += kPermBlock) moved_row(R, m, (b0 + 1 + m) % M);
+}
+
+// ---------------------------------------------------------------------------
+// Label-based pair action change of a proposal whose moved beads carry several
+// labels (PairAction::GetAction, pair_action_class.h:267-302, with the particle
+// list of Bisect::Attempt, bisect_class.h:60-63, or
+// PermBisectIterative::Attempt, perm_bisect_iterative_class.h:133-146).  The
+// reference sums every pair of its particle list over every window link; the
+// pairs and links without a moved bead are the same in OLD and NEW and are
+// left out here.  On link j the labels with a moved bead on it are listed
+// (chains before the wrap, then chains after it not already listed); each
+// listed label is paired with every particle not listed before it.  A bead's
+// NEW position is the proposal's where it moved, else its committed one.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int pre_chain(const Rep& R, const Prop& pr, int y) {
+    return pr.move == 3 ? R.mv_pre[y] : (y == pr.p ? 0 : -1);
+}
+__device__ __forceinline__ int post_chain(const Rep& R, const Prop& pr, int y) {
+    return pr.move == 3 ? R.mv_post[y] : (y == pr.p2 ? 0 : -1);
+}
+// chain whose moved bead label y is at window position t, or -1
+__device__ __forceinline__ int moved_chain(const Rep& R, const Prop& pr, int nb, int y, int t) {
+    if (t < 1 || t > nb - 1) return -1;
+    return (pr.b0 + t >= R.M) ? post_chain(R, pr, y) : pre_chain(R, pr, y);
+}
+__device__ __forceinline__ double moved_new(const Rep& R, const Prop& pr, int nb, int c, int t, int d) {
+    return pr.move == 3 ? R.ch_new[((size_t)c * (nb - 1) + t - 1) * 3 + d] : R.prop_new[3 * (t - 1) + d];
+}
+
+constexpr int kPairMultiBlock = 256;
+__global__ __launch_bounds__(kPairMultiBlock) void k_pair_multi(const Rep* __restrict__ reps, int n_reps, int bpr) {
+    const int r = blockIdx.x / bpr, part = blockIdx.x % bpr;
+    if (r >= n_reps) return;
+    const Rep& R = reps[r];
+    const Prop pr = *R.prop;
+    if (pr.status != 1 || !pr.generic) return;
+    __shared__ double sh_o[kPairMultiBlock / kWave], sh_n[kPairMultiBlock / kWave];
+    const int M = R.M, N = R.N, nb = pr.count + 1, b0 = pr.b0;
+    const int nch = pr.move == 3 ? pr.n_chain : 1;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    constexpr int kWaves = kPairMultiBlock / kWave;
+    for (int j = part; j < pr.n_links; j += bpr) {
+        const int t0 = j, t1 = j + 1;
+        const int s0 = (b0 + j) % M, s1 = (s0 + 1) % M;
+        const bool post0 = b0 + t0 >= M, post1 = b0 + t1 >= M;
+        const bool in0 = t0 >= 1 && t0 <= nb - 1, in1 = t1 <= nb - 1;
+        const bool pre_on = (in0 && !post0) || (in1 && !post1);
+        const bool post_on = (in0 && post0) || (in1 && post1);
+        const int n_items = (pre_on ? nch : 0) + (post_on ? nch : 0);
+        double uo = 0., un = 0.;
+        for (int it = w; it < n_items; it += kWaves) {
+            const bool is_post = !pre_on || it >= nch;
+            const int ci = it < nch ? it : it - nch;
+            int x;
+            if (pr.move == 3)
+                x = is_post ? R.ch_lbl2[ci] : R.ch_lbl[ci];
+            else
+                x = is_post ? pr.p2 : pr.p;
+            if (is_post && pre_on && pre_chain(R, pr, x) >= 0) continue;  // listed before the wrap already
+            double xo0[3], xo1[3], xn0[3], xn1[3];
+            const int cx0 = moved_chain(R, pr, nb, x, t0), cx1 = moved_chain(R, pr, nb, x, t1);
+            for (int d = 0; d < 3; ++d) {
+                xo0[d] = pos_at(R, d, s0, x);
+                xo1[d] = pos_at(R, d, s1, x);
+                xn0[d] = cx0 >= 0 ? moved_new(R, pr, nb, cx0, t0, d) : xo0[d];
+                xn1[d] = cx1 >= 0 ? moved_new(R, pr, nb, cx1, t1, d) : xo1[d];
+            }
+            for (int q = lane; q < N; q += kWave) {
+                if (q == x) continue;
+                // a particle listed on this link before x was paired with x already
+                int iq = 1 << 30;
+                const int qa = pre_on ? pre_chain(R, pr, q) : -1;
+                if (qa >= 0) {
+                    iq = qa;
+                } else if (post_on) {
+                    const int qb = post_chain(R, pr, q);
+                    if (qb >= 0) iq = (pre_on ? nch : 0) + qb;
+                }
+                if (iq < it) continue;
+                const int cq0 = moved_chain(R, pr, nb, q, t0), cq1 = moved_chain(R, pr, nb, q, t1);
+                double qo0[3], qo1[3], qn0[3], qn1[3];
+                for (int d = 0; d < 3; ++d) {
+                    qo0[d] = pos_at(R, d, s0, q);
+                    qo1[d] = pos_at(R, d, s1, q);
+                    qn0[d] = cq0 >= 0 ? moved_new(R, pr, nb, cq0, t0, d) : qo0[d];
+                    qn1[d] = cq1 >= 0 ? moved_new(R, pr, nb, cq1, t1, d) : qo1[d];
+                }
+                LinkGeom g;
+                link_geom(xo0, qo0, xo1, qo1, R.L, R.iL, g);
+                uo += pair_u(R, g.r, g.rp, g.s);
+                link_geom(xn0, qn0, xn1, qn1, R.L, R.iL, g);
+                un += pair_u(R, g.r, g.rp, g.s);
+            }
+        }
+        uo = wave_sum(uo);
+        un = wave_sum(un);
+        if (lane == 0) {
+            sh_o[w] = uo;
+            sh_n[w] = un;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = 0., b = 0.;
+            for (int i = 0; i < kWaves; ++i) {
+                a += sh_o[i];
+                b += sh_n[i];
+            }
+            R.pair_part[j] = make_double2(a, b);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Species::UpdateRhoK (species_class.h:406-425) for a permuting proposal: the
+// slice of moved slot m holds one moved bead per chain; its rho_k row gets the
+// sum of their phase changes, fused with the long-range sums of CalcULong like
+// k_rho.  The beads' phase chains (KSpace::CalcC recurrence) are built in LDS a
+// few chains at a time.  Grid (k chunk, slot, replica).
+// ---------------------------------------------------------------------------
+constexpr int kPermPhaseBatch = 4;
+__global__ __launch_bounds__(kRhoBlock) void k_rho_multi(const Rep* __restrict__ reps) {
+    const Rep& R = reps[blockIdx.z];
+    const Prop pr = *R.prop;
+    if (pr.status != 1 || pr.move != 3 || !R.use_long_range) return;
+    const int m = blockIdx.y, chunk = blockIdx.x;
+    if (m >= pr.count || chunk >= R.n_chunks) return;
+    __shared__ double2 tab[kPermPhaseBatch * 6 * (kMaxIdx + 1)];
+    __shared__ double red[kRhoBlock / kWave];
+    const int M = R.M, nk = R.nk, stride = R.max_idx + 1, n = pr.n_chain, nb = pr.count + 1;
+    const int2 row = R.rho_rows[m];
+    const int s = row.x, cur = row.y;
+    const double2* rc = R.rho + ((size_t)cur * M + s) * nk;
+    double2* rn = R.rho + ((size_t)(1 - cur) * M + s) * nk;
+    const int ck = kRhoBlock * R.rho_kpt, k_lo = chunk * ck, k_hi = min(nk, k_lo + ck);
+    const int kpt = R.rho_kpt;
+    // this thread's k-vectors: k_lo + threadIdx.x + i * kRhoBlock, i < kpt (at most kRhoKptMax)
+    double dx[kRhoKptMax], dy[kRhoKptMax];
+    for (int i = 0; i < kRhoKptMax; ++i) dx[i] = dy[i] = 0.;
+    for (int i0 = 0; i0 < n; i0 += kPermPhaseBatch) {
+        const int nbat = min(kPermPhaseBatch, n - i0);
+        __syncthreads();
+        for (int c = threadIdx.x; c < nbat * 6; c += kRhoBlock) {
+            const int i = i0 + c / 6, o = (c % 6) / 3, d = c % 3;
+            const double x = o ? chain_new(R, pr.b0, nb, n, m + 1, i, d) : chain_old(R, pr.b0, m + 1, i, d);
+            phase_chain(x * R.box, R.max_idx, tab + (size_t)c * stride);
+        }
+        __syncthreads();
+        for (int i = 0; i < kpt && i < kRhoKptMax; ++i) {
+            const int k = k_lo + threadIdx.x + i * kRhoBlock;
+            if (k >= k_hi) break;
+            const int kp = R.kidx[k];
+            for (int b = 0; b < nbat; ++b) {
+                const double2 en = phase_k(tab + (size_t)(b * 6 + 3) * stride, stride, kp);
+                const double2 eo = phase_k(tab + (size_t)(b * 6) * stride, stride, kp);
+                dx[i] += en.x - eo.x;
+                dy[i] += en.y - eo.y;
+            }
+        }
+    }
+    double ao = 0., an = 0.;
+    for (int i = 0; i < kpt && i < kRhoKptMax; ++i) {
+        const int k = k_lo + threadIdx.x + i * kRhoBlock;
+        if (k >= k_hi) break;
+        const double2 rv = rc[k];
+        double2 nv;
+        nv.x = rv.x + dx[i];
+        nv.y = rv.y + dy[i];
+        rn[k] = nv;
+        const double cf = R.coef_s[k];
+        ao += cf * (rv.x * rv.x + rv.y * rv.y);
+        an += cf * (nv.x * nv.x + nv.y * nv.y);
+    }
+    ao = block_sum<kRhoBlock>(ao, red);
+    an = block_sum<kRhoBlock>(an, red);
+    if (threadIdx.x == 0) R.rho_part[m * R.n_chunks + chunk] = make_double2(ao, an);
+}
+
+}  // namespace spimc

@@ -1,0 +1,179 @@
+"""GPU parity of permutation sampling (PermBisectIterative,
+perm_bisect_iterative_class.h:10-233, perm_bisect_class.h:7-135) and of the
+moves and estimators on a permuted path, against the CPU oracle.
+
+* Replay: the oracle's draws are fed to the engine; the accept/reject
+  sequences, positions (relabelled slices included), the wrap permutation and
+  the move counters must match bit for bit, rho_k to 1e-11 relative.  Stages of
+  permuting bisections are interleaved with Bisect and DisplaceParticle, which
+  then run on permuted paths (a bisection chain that changes label at the wrap,
+  the one-sided wrap link of a displaced particle).
+* Estimators on a fixed permuted path: Kinetic::GetAction windows across the
+  wrap, DActionDBeta, VirialEnergy and the adjoint's kinetic term, to 1e-10
+  relative (1e-6 against the central difference for the gradient).
+* Free running: free bosons sampled with the device RNG reproduce the ideal
+  Bose gas closed form (energy and mean k-cycle counts), like the oracle's
+  CPU test, within 3.5 standard errors over independent replicas.
+"""
+import numpy as np
+import pytest
+
+from oracle_ffi import Oracle
+from simpimc_amd import abi, system
+from test_permutation_cpu import BETA, LAM, L, M, N, free_system, ideal_bose
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(1.0, float(np.max(np.abs(b))) if b.size else 1.0)
+    return float(np.max(np.abs(a - b))) / scale if a.size else 0.0
+
+
+def dense_bare():
+    # 8 electrons at r_s = 1, beta = 0.8: neighbours within a few thermal lengths
+    return system.config_c1(n_part=8, n_bead=16, r_s=1.0, tau=0.05, n_level=3)
+
+
+def small_david():
+    return system.config_c2(n_part=16, n_bead=16, r_s=1.0, tau=0.02, n_level=3, k_shell=20.5, n_grid=400, seed=7)
+
+
+SYSTEMS = {
+    "bare_dense": dense_bare,
+    "david_small": small_david,
+    "ilkka_c1": lambda: system.config_ilkka(),
+    "free_bose": free_system,
+}
+
+
+@pytest.mark.parametrize("name", sorted(SYSTEMS))
+def test_replay_with_permutations_bit_exact(name):
+    s = SYSTEMS[name]()
+    pos = system.random_path(s, seed=3, spread=0.3)
+    o = Oracle(s, seed=11)
+    o.upload(pos)
+    o.record(True)
+    n_b = max(1, s.n_part * s.n_bead >> s.bisect_n_level)
+    ref = []
+    for _ in range(4):
+        ref.append(o.perm_bisect(n_b))
+        ref.append(o.bisect(n_b))
+        ref.append(o.displace(s.n_part))
+    tape = o.recorded()
+    print(f"{name}: {len(tape)} draws", flush=True)
+    with abi.Engine(s, seed=99) as e:
+        e.upload(pos)
+        e.tape_set(tape)
+        got = []
+        for _ in range(4):
+            got.append(e.perm_bisect(n_b)[0])
+            got.append(e.bisect(n_b)[0])
+            got.append(e.displace(s.n_part)[0])
+        assert e.tape_cursor() == len(tape)
+        for i, (g, r) in enumerate(zip(got, ref)):
+            assert np.array_equal(g, r), (i, g, r)
+        assert np.array_equal(e.perm(), o.perm())
+        assert np.array_equal(e.download(), o.download())
+        if s.n_k:
+            assert rel_err(e.rho_k(), o.rho_k()) < 1e-11 * s.n_part
+        assert np.array_equal(e.counters(), o.counters())
+        eg, og = e.perm_counters(), o.perm_counters()
+        assert eg[0] == og[0] and eg[1] == og[1]
+        assert np.array_equal(eg[2], og[2]) and np.array_equal(eg[3], og[3])
+        assert e.bead_updates() == o.bead_updates()
+        sigma = o.perm()
+        print(f"{name}: perm accepts per cycle length {og[3].tolist()}, sigma {sigma.tolist()}", flush=True)
+    if name == "bare_dense":
+        # the run relinked the path: multi-particle cycles were accepted
+        assert og[3][1:].sum() > 0
+
+
+PERMS = {4: [2, 0, 1, 3], 8: [1, 0, 3, 4, 2, 5, 7, 6], 16: list(range(1, 16)) + [0]}
+
+
+def permuted(name):
+    s = SYSTEMS[name]()
+    pos = system.random_path(s, seed=5, spread=0.3)
+    sigma = np.array(PERMS[s.n_part] if s.n_part in PERMS else np.roll(np.arange(s.n_part), 1), dtype=np.uint32)
+    return s, pos, sigma
+
+
+@pytest.mark.parametrize("name", ["bare_dense", "david_small"])
+def test_estimators_on_a_permuted_path(name):
+    s, pos, sigma = permuted(name)
+    o = Oracle(s)
+    o.upload(pos)
+    o.set_perm(sigma)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        e.set_perm(sigma)
+        assert np.array_equal(e.perm(), sigma)
+        for p in range(s.n_part):
+            for b0, span, lvl in ((s.n_bead - 3, 4, 0), (s.n_bead - 4, 8, 1), (0, s.n_bead, 0), (5, s.n_bead, 0)):
+                a = e.get_action(abi.ACTION_KINETIC, abi.MODE_NEW, b0, b0 + span, [p], lvl)
+                b = o.get_action(abi.ACTION_KINETIC, abi.MODE_NEW, b0, b0 + span, [p], lvl)
+                assert abs(a - b) <= 1e-10 * max(1.0, abs(b)), (p, b0, span, a, b)
+        a, b = e.d_action_d_beta(abi.ACTION_KINETIC), o.d_action_d_beta(abi.ACTION_KINETIC)
+        assert abs(a - b) <= 1e-10 * max(1.0, abs(b)), (a, b)
+        for w in (0, 4):
+            a, b = e.kinetic_virial_energy(w), o.kinetic_virial_energy(w)
+            assert abs(a - b) <= 1e-10 * max(1.0, abs(b)), (w, a, b)
+        g = e.gradient(abi.GRAD_ALL)
+        ref = o.gradient(abi.GRAD_ALL, compat=False, h=1e-5)
+        assert rel_err(g, ref) < 1e-6
+        # the same path with the identity permutation differs on the wrap links
+        g_id = e.gradient(abi.GRAD_KINETIC)
+        e.set_perm(np.arange(s.n_part))
+        assert rel_err(g_id, e.gradient(abi.GRAD_KINETIC)) > 1e-3
+
+
+def test_free_bosons_match_the_closed_form():
+    s = free_system()
+    n_rep, n_eq, n_meas, every = 64, 300, 2500, 5
+    e_exact, nk_exact = ideal_bose(N, BETA, L, LAM)
+    with abi.Engine([s] * n_rep, seed=4242) as e:
+        for r in range(n_rep):
+            e.upload(system.random_path(s, seed=300 + r), r=r)
+        e.obs_configure(energy=abi.ENERGY_THERMAL)
+        for _ in range(n_eq):
+            e.perm_bisect(3)
+            e.bisect(3)
+        e.obs_reset(abi.OBS_ENERGY)
+        cyc = np.zeros((n_rep, N))
+        n_cyc = 0
+        for it in range(n_meas):
+            e.perm_bisect(3)
+            e.bisect(3)
+            e.obs_accumulate(abi.OBS_ENERGY)
+            if it % every == 0:
+                n_cyc += 1
+                for r in range(n_rep):
+                    sig = e.perm(r=r)
+                    seen = np.zeros(N, dtype=bool)
+                    for p in range(N):
+                        if seen[p]:
+                            continue
+                        ln, q = 0, p
+                        while not seen[q]:
+                            seen[q] = True
+                            q = sig[q]
+                            ln += 1
+                        cyc[r, ln - 1] += 1
+            if it % 500 == 0:
+                print(f"free bosons: sample {it} of {n_meas}", flush=True)
+        en = []
+        for r in range(n_rep):
+            v, nm = e.obs_read(abi.OBS_ENERGY, r=r)
+            assert nm == n_meas
+            en.append((v[0] + v[1]) / (M * nm))
+    en = np.array(en)
+    em, ese = en.mean(), en.std(ddof=1) / np.sqrt(n_rep)
+    print(f"energy {em:.5f} +- {ese:.5f}, exact {e_exact:.5f}", flush=True)
+    assert abs(em - e_exact) < 3.5 * ese
+    nk = cyc / n_cyc
+    nkm, nkse = nk.mean(axis=0), nk.std(axis=0, ddof=1) / np.sqrt(n_rep)
+    print(f"cycles {nkm} +- {nkse}, exact {nk_exact}", flush=True)
+    for k in range(N):
+        assert abs(nkm[k] - nk_exact[k]) < 3.5 * nkse[k] + 1e-3, (k, nkm, nkse, nk_exact)
