@@ -317,7 +317,10 @@ class H5Out {
         if (dims.empty()) dims = {v.size()};
         Fixed(name, h_.t_double, dims, v.data());
     }
-    void Ints(const std::string& name, const std::vector<long long>& v) { Fixed(name, h_.t_int64, {v.size()}, v.data()); }
+    void Ints(const std::string& name, const std::vector<long long>& v, std::vector<hsize_t> dims = {}) {
+        if (dims.empty()) dims = {v.size()};
+        Fixed(name, h_.t_int64, dims, v.data());
+    }
 
     // Appends one block to an extendable dataset: a scalar row (n == 0) of a
     // 1-D dataset, or a row of n values of a 2-D dataset [block][n].
@@ -325,6 +328,9 @@ class H5Out {
         AppendRow(name, h_.t_double, row, n ? std::vector<hsize_t>{n} : std::vector<hsize_t>{});
     }
     void AppendInt(const std::string& name, long long v) { AppendRow(name, h_.t_int64, &v, {}); }
+    void AppendInts(const std::string& name, const std::vector<long long>& row) {
+        AppendRow(name, h_.t_int64, row.data(), {(hsize_t)row.size()});
+    }
     // Appends one block of shape `row_dims` to the dataset [block][row_dims...].
     void Append(const std::string& name, const double* row, const std::vector<hsize_t>& row_dims) {
         AppendRow(name, h_.t_double, row, row_dims);

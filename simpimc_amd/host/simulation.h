@@ -9,6 +9,7 @@
 // its n_step attempts to the device as one stage.
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -82,7 +83,10 @@ struct Path {
     }
 
     // Initial path of one replica, positions r[3][M][N] (species_class.h:222-377)
-    std::vector<double> InitPath(uint64_t seed, uint32_t replica) const {
+    // sigma receives the path permutation (the identity except for a restart)
+    std::vector<double> InitPath(uint64_t seed, uint32_t replica, std::vector<uint32_t>& sigma) const {
+        sigma.resize(n_part);
+        for (uint32_t p = 0; p < n_part; ++p) sigma[p] = p;
         const std::string init = species_in.GetAttribute<std::string>("init_type");
         std::vector<double> r((size_t)3 * n_bead * n_part);
         auto set = [&](uint32_t p, const double* x) {
@@ -131,7 +135,7 @@ struct Path {
                 }
             }
         } else if (init == "Restart") {
-            RestartPath(r, replica);
+            RestartPath(r, sigma, replica);
         } else {
             throw std::runtime_error("init_type " + init + " is outside this engine");
         }
@@ -141,8 +145,10 @@ struct Path {
     // init_type "Restart" (species_class.h:336-377): the last dump of a PathDump
     // observable in <prefix>.<replica>.h5 (parallel="1") or <prefix>.0.h5.
     // Positions are read as cube(n_d, n_part * n_bead, n_dump), entry
-    // (d, p * n_bead + b, n_dump - 1); the permutation must be the identity.
-    void RestartPath(std::vector<double>& r, uint32_t replica) const {
+    // (d, p * n_bead + b, n_dump - 1); the permutation of the last dump gives, per
+    // particle, the previous particle of its first bead and the next particle of
+    // its last bead (species_class.h:366-374).
+    void RestartPath(std::vector<double>& r, std::vector<uint32_t>& sigma, uint32_t replica) const {
         const std::string prefix = species_in.GetAttribute<std::string>("prefix");
         const std::string dump = species_in.GetAttribute<std::string>("path_dump_name", "path_dump");
         const bool parallel = species_in.GetAttribute<bool>("parallel", false);
@@ -168,9 +174,13 @@ struct Path {
                 for (uint32_t d = 0; d < n_d; ++d)
                     r[((size_t)d * n_bead + b) * n_part + p] = last[d + (size_t)n_d * (p * (size_t)n_bead + b)];
         const double* pl = perm.data() + (size_t)2 * n_part * (n_dump - 1);
+        for (uint32_t p = 0; p < n_part; ++p) {
+            const double nx = pl[2 * p + 1];
+            if (!(nx >= 0. && nx < n_part)) throw std::runtime_error(file + ": permutation out of range");
+            sigma[p] = (uint32_t)nx;
+        }
         for (uint32_t p = 0; p < n_part; ++p)
-            if (pl[2 * p] != p || pl[2 * p + 1] != p)
-                throw std::runtime_error(file + ": restart from a permuted path is outside this engine");
+            if (pl[2 * sigma[p]] != p) throw std::runtime_error(file + ": permutation rows disagree");
         std::cout << "Restarting " << species << " from " << file << "..." << std::endl;
     }
 };
@@ -442,8 +452,10 @@ struct Move : Event {
     Move(const Input& in, Engine& e) : eng(e), type(in.GetAttribute<std::string>("type")) {
         name = in.GetAttribute<std::string>("name");
     }
+    virtual ~Move() = default;
     virtual int Stage(uint32_t n) = 0;
     virtual int CounterBase() const = 0;
+    virtual void Counters(uint32_t r, uint64_t* c) const { check(spimc_move_counters(eng.ctx, r, c), "counters"); }
     void DoEvent() override { DoEvents(1); }
     void DoEvents(uint32_t n) override {
         double t = Now();
@@ -460,7 +472,7 @@ struct Move : Event {
         last.resize(2 * eng.n_replicas, 0);
         for (uint32_t r = 0; r < eng.n_replicas; ++r) {
             uint64_t c[4];
-            check(spimc_move_counters(eng.ctx, r, c), "counters");
+            Counters(r, c);
             out[r] = {c[CounterBase()] - last[2 * r], c[CounterBase() + 1] - last[2 * r + 1]};
             last[2 * r] = c[CounterBase()];
             last[2 * r + 1] = c[CounterBase() + 1];
@@ -476,15 +488,17 @@ struct Bisect : Move {
     uint32_t n_bead = 0, n_level0 = 0;
     std::vector<uint32_t> n_level;  // per replica, the level of the running block
     bool first_write = true;
-    Bisect(const Input& in, Engine& e, spimc_system_desc& d, const Path& path) : Move(in, e) {
-        d.bisect_n_level = n_level0 = in.GetAttribute<uint32_t>("n_level");
-        d.bisect_n_images = in.GetAttribute<int>("n_images", 0);
+    int n_images = 0;
+    Bisect(const Input& in, Engine& e, const Path& path) : Move(in, e) {
+        n_level0 = in.GetAttribute<uint32_t>("n_level");
+        n_images = in.GetAttribute<int>("n_images", 0);
         adaptive = in.GetAttribute<bool>("adaptive", false);
         if (adaptive) target_ratio = in.GetAttribute<double>("target_ratio");
         n_bead = path.n_bead;
     }
     int Stage(uint32_t n) override { return spimc_bisect_stage(eng.ctx, nullptr, n, nullptr); }
     int CounterBase() const override { return 0; }
+    virtual void SetLevel(uint32_t r, uint32_t lv) { check(spimc_set_bisect_level(eng.ctx, r, lv), "adaptive bisection level"); }
     // Bisect::Reset with adaptive="1" (bisect_class.h:143-151), after the first
     // write: one level down when the block's acceptance is below target_ratio,
     // else one up while 2^n_level < n_bead / 2; each replica on its own counts
@@ -501,8 +515,53 @@ struct Bisect : Move {
                 --lv;
             else if ((1u << lv) < n_bead / 2)
                 ++lv;
-            if (lv != n_level[r]) check(spimc_set_bisect_level(eng.ctx, r, lv), "adaptive bisection level");
+            if (lv != n_level[r]) SetLevel(r, lv);
             n_level[r] = lv;
+        }
+    }
+};
+
+// PermBisectIterative (perm_bisect_iterative_class.h:226-230, perm_bisect_class.h:100-116):
+// the bisection attributes plus epsilon; its own level per replica, and the
+// per-cycle-length counters of each block (perm_bisect_class.h:119-132)
+struct PermBisectIterative : Bisect {
+    double epsilon;
+    uint32_t n_part;
+    std::vector<uint64_t> perm_last;                    // [replica][2 n_part] at the last write
+    std::vector<std::vector<uint64_t>> perm_att, perm_acc;  // the block's counts per replica
+    bool h5_started = false;
+    PermBisectIterative(const Input& in, Engine& e, const Path& path) : Bisect(in, e, path) {
+        epsilon = in.GetAttribute<double>("epsilon", 1.e-100);
+        n_part = path.n_part;
+    }
+    int Stage(uint32_t n) override { return spimc_perm_bisect_stage(eng.ctx, nullptr, n, nullptr); }
+    void Counters(uint32_t r, uint64_t* c) const override {
+        std::vector<uint64_t> v(2 + 2 * (size_t)n_part);
+        check(spimc_perm_counters(eng.ctx, r, v.data()), "counters");
+        c[0] = v[0];
+        c[1] = v[1];
+    }
+    void SetLevel(uint32_t r, uint32_t lv) override {
+        check(spimc_perm_configure(eng.ctx, r, lv, epsilon), "adaptive bisection level");
+    }
+    void Configure() {
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) check(spimc_perm_configure(eng.ctx, r, n_level0, epsilon), name.c_str());
+    }
+    // perm_attempt / perm_accept since the last write (PermBisect::Reset zeroes them)
+    void PermBlock() {
+        const size_t w = 2 + 2 * (size_t)n_part;
+        perm_last.resize(eng.n_replicas * w, 0);
+        perm_att.assign(eng.n_replicas, std::vector<uint64_t>(n_part));
+        perm_acc.assign(eng.n_replicas, std::vector<uint64_t>(n_part));
+        std::vector<uint64_t> v(w);
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) {
+            check(spimc_perm_counters(eng.ctx, r, v.data()), "counters");
+            uint64_t* l = &perm_last[r * w];
+            for (uint32_t c = 0; c < n_part; ++c) {
+                perm_att[r][c] = v[2 + c] - l[2 + c];
+                perm_acc[r][c] = v[2 + n_part + c] - l[2 + n_part + c];
+            }
+            std::copy(v.begin(), v.end(), l);
         }
     }
 };
@@ -583,6 +642,89 @@ struct TimeObs : Observable {
     double start;
     TimeObs(const Input& in, Engine& e) : Observable(in, e) { start = Now(); }
     void DoEvent() override {}
+};
+
+// Permutation (permutation_class.h:7-119): per measurement, the cycle lengths
+// of each replica's path permutation (Species::SetCycleCount,
+// species_class.h:28-44) and its sector, the index of the sorted cycle lengths
+// among the integer partitions of n_part in the order of
+// Species::SetupPermSectors (species_class.h:178-219, ascending compositions).
+// The reference never fills its partition map (the setup runs only when the
+// map is already non-empty), so its sector lookup reads an empty map; here the
+// map is built as the setup describes, for n_part <= 60 (more partitions than
+// that are not enumerated: the sector series is then left out).
+struct Permutation : Observable {
+    std::string species;
+    uint32_t sector_max, n_part;
+    std::map<std::vector<uint32_t>, uint32_t> sectors_of;
+    std::vector<std::vector<uint32_t>> partitions;
+    std::vector<std::vector<double>> cycles;             // [replica][n_part] counts since the last write
+    std::vector<std::map<uint32_t, uint32_t>> sector_n;  // [replica] sector -> measurements
+    std::vector<uint64_t> n_measure;
+    bool h5_started = false;
+    Permutation(const Input& in, Engine& e, const Path& path) : Observable(in, e) {
+        species = in.GetAttribute<std::string>("species");
+        sector_max = in.GetAttribute<uint32_t>("sector_max", 0);
+        n_part = path.n_part;
+        if (n_part <= 60) {
+            // ascending compositions of n (the loop of SetupPermSectors)
+            const int n = (int)n_part;
+            std::vector<int> a(n + 1, 0);
+            int k = 1, y = n - 1;
+            while (k != 0) {
+                int x = a[k - 1] + 1;
+                k -= 1;
+                while (2 * x <= y) {
+                    a[k] = x;
+                    y -= x;
+                    k += 1;
+                }
+                const int l = k + 1;
+                while (x <= y) {
+                    a[k] = x;
+                    a[l] = y;
+                    partitions.emplace_back(a.begin(), a.begin() + k + 2);
+                    x += 1;
+                    y -= 1;
+                }
+                a[k] = x + y;
+                y = x + y - 1;
+                partitions.emplace_back(a.begin(), a.begin() + k + 1);
+            }
+            for (uint32_t j = 0; j < partitions.size(); ++j) sectors_of[partitions[j]] = j;
+        }
+        Reset();
+    }
+    void Reset() {
+        cycles.assign(eng.n_replicas, std::vector<double>(n_part, 0.));
+        sector_n.assign(eng.n_replicas, {});
+        n_measure.assign(eng.n_replicas, 0);
+    }
+    void DoEvent() override {
+        double t = Now();
+        std::vector<uint32_t> sigma(n_part);
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) {
+            check(spimc_perm_download(eng.ctx, r, sigma.data()), name.c_str());
+            std::vector<uint32_t> len;
+            std::vector<char> seen(n_part, 0);
+            for (uint32_t p = 0; p < n_part; ++p) {
+                if (seen[p]) continue;
+                uint32_t c = 0, q = p;
+                while (!seen[q]) {
+                    seen[q] = 1;
+                    q = sigma[q];
+                    ++c;
+                }
+                len.push_back(c);
+            }
+            for (uint32_t c : len) cycles[r][c - 1] += 1.;
+            std::sort(len.begin(), len.end());
+            auto it = sectors_of.find(len);
+            if (it != sectors_of.end()) sector_n[r][it->second] += 1;
+            n_measure[r] += 1;
+        }
+        time_spent += Now() - t;
+    }
 };
 
 // PathDump (path_dump_class.h:7-69): every `skip`-th Write appends the path of

@@ -63,7 +63,9 @@ struct Simulation {
             const std::string type = m.GetAttribute<std::string>("type");
             std::shared_ptr<Move> mv;
             if (type == "Bisect")
-                mv = std::make_shared<Bisect>(m, eng, desc, path);
+                mv = std::make_shared<Bisect>(m, eng, path);
+            else if (type == "PermBisectIterative")
+                mv = std::make_shared<PermBisectIterative>(m, eng, path);
             else if (type == "DisplaceParticle")
                 mv = std::make_shared<DisplaceParticle>(m, eng, desc, path);
             else
@@ -88,6 +90,8 @@ struct Simulation {
                 ob = gr = std::make_shared<PairCorrelation>(o, eng, path);
             } else if (type == "PathDump") {
                 ob = std::make_shared<PathDump>(o, eng);
+            } else if (type == "Permutation") {
+                ob = std::make_shared<Permutation>(o, eng, path);
             } else if (type == "Time") {
                 ob = std::make_shared<TimeObs>(o, eng);
             } else {
@@ -107,15 +111,32 @@ struct Simulation {
         desc.k_n = path.k_n.data();
         if (desc.displace_step == 0.) desc.displace_step = path.L / 10.;
         for (auto& a : actions) a->Fill(desc, path);
-        if (!desc.bisect_n_level) desc.bisect_n_level = 1;
+        // one set of bisection splines per replica: the Bisect move's level (else
+        // the permuting one's) seeds the descriptor; all must share n_images
+        uint32_t plain_level = 0, perm_level = 0;
+        bool images_set = false;
+        for (auto& m : moves)
+            if (auto* b = dynamic_cast<Bisect*>(m.get())) {
+                uint32_t& lv = dynamic_cast<PermBisectIterative*>(m.get()) ? perm_level : plain_level;
+                if (!lv) lv = b->n_level0;
+                if (images_set && desc.bisect_n_images != b->n_images)
+                    throw std::runtime_error("ERROR: bisection moves with different n_images are outside this engine");
+                desc.bisect_n_images = b->n_images;
+                images_set = true;
+            }
+        desc.bisect_n_level = plain_level ? plain_level : (perm_level ? perm_level : 1);
         const uint64_t seed = in.HasChild("RNG") ? (uint64_t)in.GetChild("RNG").GetAttribute<double>("seed", 0.)
                                                  : (uint64_t)std::chrono::system_clock::now().time_since_epoch().count();
         std::vector<spimc_system_desc> descs(n_replicas, desc);
         check(spimc_ctx_create(descs.data(), n_replicas, seed, &eng.ctx), "context");
         for (uint32_t r = 0; r < n_replicas; ++r) {
-            auto pos = path.InitPath(seed * 1000003ull + r, r);
+            std::vector<uint32_t> sigma;
+            auto pos = path.InitPath(seed * 1000003ull + r, r, sigma);
             check(spimc_path_upload(eng.ctx, r, pos.data()), "path upload");
+            check(spimc_perm_upload(eng.ctx, r, sigma.data()), "permutation upload");
         }
+        for (auto& m : moves)
+            if (auto* pb = dynamic_cast<PermBisectIterative*>(m.get())) pb->Configure();
         check(spimc_obs_configure(eng.ctx, en ? en->estimator : SPIMC_ENERGY_THERMAL, 0, sk ? sk->k_cut : 1e300,
                                   gr ? gr->r_min : 0., gr ? gr->r_max : 0., gr ? gr->n_r : 100),
               "observables");
@@ -191,6 +212,8 @@ struct Simulation {
                 WritePathDump(*d);
             else if (auto t = dynamic_cast<TimeObs*>(o.get()))
                 WriteTime(*t);
+            else if (auto pm = dynamic_cast<Permutation*>(o.get()))
+                WritePermutation(*pm);
         }
         out << "}}\n";
         out.flush();
@@ -212,6 +235,10 @@ struct Simulation {
     void WriteMove(Move& m) {
         const auto b = m.Block();
         auto* bis = dynamic_cast<Bisect*>(&m);
+        auto* pb = dynamic_cast<PermBisectIterative*>(&m);
+        // PermBisect::Write (perm_bisect_class.h:119-132): the block's counts per
+        // cycle length, before the bisection level and Move::Write
+        if (pb) pb->PermBlock();
         std::vector<uint32_t> level;
         if (bis) {
             bis->n_level.resize(eng.n_replicas, bis->n_level0);
@@ -225,11 +252,27 @@ struct Simulation {
                      (unsigned long long)b[r].n_attempt, (unsigned long long)b[r].n_accept,
                      b[r].n_attempt ? (double)b[r].n_accept / b[r].n_attempt : 0.);
             out << s;
+            if (pb) {
+                for (int w = 0; w < 2; ++w) {
+                    out << (w ? ", \"perm_accept\": [" : ", \"perm_attempt\": [");
+                    const auto& v = w ? pb->perm_acc[r] : pb->perm_att[r];
+                    for (size_t c = 0; c < v.size(); ++c) out << (c ? ", " : "") << v[c];
+                    out << "]";
+                }
+            }
             if (bis && bis->adaptive) out << ", \"n_level\": " << level[r];
             out << "}";
         }
         out << "]";
 #if SPIMC_HAVE_HDF5
+        if (pb)
+            for (uint32_t r = 0; r < h5.size(); ++r) {
+                const std::string g = "Moves/" + m.name + "/";
+                if (!pb->h5_started) h5[r]->Int(g + "n_perm_type", pb->n_part);
+                h5[r]->AppendInts(g + "perm_attempt", std::vector<long long>(pb->perm_att[r].begin(), pb->perm_att[r].end()));
+                h5[r]->AppendInts(g + "perm_accept", std::vector<long long>(pb->perm_acc[r].begin(), pb->perm_acc[r].end()));
+            }
+        if (pb) pb->h5_started = true;
         if (bis && bis->adaptive)
             for (uint32_t r = 0; r < h5.size(); ++r) h5[r]->AppendInt("Moves/" + m.name + "/n_level", level[r]);
 #endif
@@ -362,6 +405,60 @@ struct Simulation {
     }
     bool time_started_ = false;
 
+    // Permutation::Write (permutation_class.h:76-118): per replica the block's
+    // (sector, count) pairs in sector order, appended row by row to sectors/y,
+    // and the cycle-length histogram normalised by the number of cycles to
+    // cycles/y; the constructor's datasets go out with the first write.
+    void WritePermutation(Permutation& pm) {
+        out << "\"" << pm.name << "\": [";
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) {
+            double n_cyc = 0.;
+            for (double c : pm.cycles[r]) n_cyc += c;
+            out << (r ? ", " : "") << "{\"n_measure\": " << pm.n_measure[r] << ", \"cycles\": [";
+            for (uint32_t c = 0; c < pm.n_part; ++c)
+                out << (c ? ", " : "") << (n_cyc > 0. ? pm.cycles[r][c] / n_cyc : 0.);
+            out << "], \"sectors\": [";
+            bool f = true;
+            for (auto& kv : pm.sector_n[r]) {
+                out << (f ? "" : ", ") << "[" << kv.first << ", " << kv.second << "]";
+                f = false;
+            }
+            out << "]}";
+        }
+        out << "]";
+#if SPIMC_HAVE_HDF5
+        const std::string g = "Observables/" + pm.name + "/";
+        for (uint32_t r = 0; r < h5.size(); ++r) {
+            if (!pm.h5_started) {
+                h5[r]->String(g + "species", pm.species);
+                h5[r]->Int(g + "sector_max", pm.sector_max);
+                h5[r]->String(g + "sectors/data_type", "pairs");
+                std::vector<long long> x(pm.partitions.size());
+                std::vector<long long> poss((size_t)pm.n_part * pm.partitions.size(), 0);
+                for (size_t j = 0; j < pm.partitions.size(); ++j) {
+                    x[j] = (long long)j;
+                    for (uint32_t len : pm.partitions[j]) poss[(size_t)(len - 1) * pm.partitions.size() + j] += 1;
+                }
+                h5[r]->Ints(g + "sectors/x", x);
+                h5[r]->Ints(g + "sectors/possPerms", poss, {pm.n_part, (hsize_t)pm.partitions.size()});
+                h5[r]->String(g + "cycles/data_type", "histogram");
+                std::vector<long long> cx(pm.n_part);
+                for (uint32_t c = 0; c < pm.n_part; ++c) cx[c] = c + 1;
+                h5[r]->Ints(g + "cycles/x", cx);
+            }
+            if (pm.n_measure[r] == 0) continue;
+            for (auto& kv : pm.sector_n[r]) h5[r]->AppendInts(g + "sectors/y", {(long long)kv.first, (long long)kv.second});
+            double n_cyc = 0.;
+            for (double c : pm.cycles[r]) n_cyc += c;
+            std::vector<double> y(pm.n_part);
+            for (uint32_t c = 0; c < pm.n_part; ++c) y[c] = pm.cycles[r][c] / n_cyc;
+            h5[r]->Append(g + "cycles/y", y.data(), pm.n_part);
+        }
+#endif
+        pm.h5_started = true;
+        pm.Reset();
+    }
+
     // PathDump::Write (path_dump_class.h:29-66): positions [dump][particle][bead][d]
     // (the reference's cube(n_d, n_bead, n_part) per dump, first index fastest)
     // and permutation [dump][particle][2] (previous and next particle of the
@@ -374,10 +471,15 @@ struct Simulation {
         if (dump && !h5.empty()) {
             const uint32_t N = path.n_part, M = path.n_bead;
             std::vector<double> r((size_t)3 * M * N), pd((size_t)N * M * 3), perm((size_t)2 * N);
-            for (uint32_t p = 0; p < N; ++p) perm[2 * p] = perm[2 * p + 1] = p;
+            std::vector<uint32_t> sigma(N);
             const std::string g = "Observables/" + d.name + "/" + path.species + "/";
             for (uint32_t rep = 0; rep < h5.size(); ++rep) {
                 check(spimc_path_download(eng.ctx, rep, r.data()), "path download");
+                check(spimc_perm_download(eng.ctx, rep, sigma.data()), "permutation download");
+                for (uint32_t p = 0; p < N; ++p) {
+                    perm[2 * sigma[p]] = p;  // previous particle of sigma[p]'s first bead
+                    perm[2 * p + 1] = sigma[p];
+                }
                 for (uint32_t p = 0; p < N; ++p)
                     for (uint32_t b = 0; b < M; ++b)
                         for (uint32_t x = 0; x < 3; ++x) pd[((size_t)p * M + b) * 3 + x] = r[((size_t)x * M + b) * N + p];

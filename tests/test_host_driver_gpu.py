@@ -358,3 +358,132 @@ def test_time_observable(tmp_path):
         assert dims == (2,) and np.all(x >= 0)
     np.testing.assert_allclose(h5dump.dataset(f, "/Observables/T/block/x")[1], [b["Observables"]["T"]["block"] for b in blocks],
                                rtol=1e-12)
+
+
+PERM_XML = """<Input>
+  <RNG seed="{seed}" />
+  <IO output_prefix="{prefix}" />
+  <System n_d="3" n_bead="{M}" beta="{beta!r}" L="{L!r}" pbc="1" k_cut="{kc!r}" />
+  <Particles>
+    <Species name="e" n_part="{N}" lambda="0.5" {init} />
+  </Particles>
+  <Actions>
+    <Action name="Kinetic" type="Kinetic" species="e" />
+    {action}
+  </Actions>
+  <Moves>
+    <Move name="Perm" type="PermBisectIterative" species="e" n_level="3" epsilon="1e-100" />
+    <Move name="Bisect" type="Bisect" species="e" n_level="3" />
+    <Move name="Displace" type="DisplaceParticle" species="e" />
+  </Moves>
+  <Observables>
+    <Observable name="Energy" type="Energy" />
+    <Observable name="Cycles" type="Permutation" species="e" />
+    <Observable name="path_dump" type="PathDump" />
+  </Observables>
+  <Algorithm>
+    <Loop n_step="2">
+      <Loop n_step="3">
+        <Loop n_step="{nb}"><Move name="Perm" /></Loop>
+        <Loop n_step="{nb}"><Move name="Bisect" /></Loop>
+        <Loop n_step="{N}"><Move name="Displace" /></Loop>
+        <Observable name="Energy" />
+        <Observable name="Cycles" />
+      </Loop>
+      <Write />
+    </Loop>
+  </Algorithm>
+</Input>
+"""
+
+
+def cycle_lengths(sigma):
+    seen, out = np.zeros(len(sigma), dtype=bool), []
+    for p in range(len(sigma)):
+        if not seen[p]:
+            n, q = 0, p
+            while not seen[q]:
+                seen[q] = True
+                q = sigma[q]
+                n += 1
+            out.append(n)
+    return out
+
+
+@pytest.mark.skipif(h5dump.H5DUMP is None, reason="h5dump not installed")
+def test_permuting_driver_matches_binding_and_restarts(tmp_path):
+    """<Move type="PermBisectIterative"> with a Permutation observable and a
+    PathDump: the driver's chain equals the binding's with the same seed (energies,
+    per-cycle-length counters, the cycle histogram, the dumped path and its
+    permutation rows); a Restart from the dump carries the permutation on."""
+    assert os.path.exists(DRIVER), "build the host driver first"
+    s = system.config_c1(n_part=8, n_bead=16, r_s=1.0, tau=0.05, n_level=3)
+    write_init(tmp_path, s)
+    system.write_table(str(tmp_path / "bare.table"), system.bare_table_datasets(s))
+    kc = (5.0 + 1e-9) * 2 * np.pi / s.L
+    N, M = s.n_part, s.n_bead
+    nb = (N * M) >> s.bisect_n_level
+
+    def run(prefix, init):
+        (tmp_path / f"{prefix}.xml").write_text(PERM_XML.format(
+            seed=4242, M=M, beta=s.beta, L=s.L, kc=kc, N=N, nb=nb, prefix=prefix, init=init,
+            action=BARE.format(file="bare.table")))
+        res = subprocess.run([DRIVER, f"{prefix}.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+        assert res.returncode == 0, res.stdout + res.stderr
+        return [json.loads(line) for line in open(tmp_path / f"{prefix}.spimc.jsonl")]
+
+    def binding(pos, sigma, blocks, h5file):
+        with abi.Engine(s, seed=4242) as e:
+            e.upload(pos)
+            e.set_perm(sigma)
+            e.obs_configure(energy=abi.ENERGY_THERMAL, sk_k_cut=kc, gr_n_r=100)
+            last = e.perm_counters()
+            for blk in blocks:
+                cyc = np.zeros(N)
+                for _ in range(3):
+                    e.perm_bisect(nb)
+                    e.bisect(nb)
+                    e.displace(N)
+                    e.obs_accumulate(abi.OBS_ENERGY)
+                    for ln in cycle_lengths(e.perm()):
+                        cyc[ln - 1] += 1
+                en, nm = e.obs_read(abi.OBS_ENERGY)
+                e.obs_reset(abi.OBS_ENERGY)
+                got = blk["Observables"]["Energy"][0]
+                assert got["Kinetic"] == pytest.approx(en[0] / (M * nm), rel=1e-14)
+                assert got["Coulomb"] == pytest.approx(en[1] / (M * nm), rel=1e-14)
+                pc = e.perm_counters()
+                mv = blk["Moves"]["Perm"][0]
+                assert (mv["n_attempt"], mv["n_accept"]) == (pc[0] - last[0], pc[1] - last[1])
+                assert mv["perm_attempt"] == (pc[2] - last[2]).tolist()
+                assert mv["perm_accept"] == (pc[3] - last[3]).tolist()
+                last = pc
+                np.testing.assert_allclose(blk["Observables"]["Cycles"][0]["cycles"], cyc / cyc.sum(), rtol=1e-15)
+            dims, pdump = h5dump.dataset(h5file, "/Observables/path_dump/e/positions")
+            assert np.array_equal(e.download(), pdump.reshape(dims)[-1].transpose(2, 1, 0))
+            pdims, perm = h5dump.dataset(h5file, "/Observables/path_dump/e/permutation")
+            rows = perm.reshape(pdims)[-1]
+            sig = e.perm()
+            assert np.array_equal(rows[:, 1], sig)
+            assert np.array_equal(rows[sig, 0], np.arange(N))
+            return sig
+
+    blocks = run("P", INIT_FILE)
+    sig = binding(system.random_path(s, seed=8), np.arange(N), blocks, str(tmp_path / "P.0.h5"))
+    assert not np.array_equal(sig, np.arange(N)), "the run left the path unpermuted"
+    f = str(tmp_path / "P.0.h5")
+    assert h5dump.dataset(f, "/Moves/Perm/n_perm_type")[1][0] == N
+    dims, pa = h5dump.dataset(f, "/Moves/Perm/perm_attempt")
+    assert dims == (2, N)
+    assert pa.reshape(dims)[1].tolist() == blocks[1]["Moves"]["Perm"][0]["perm_attempt"]
+    dims, cy = h5dump.dataset(f, "/Observables/Cycles/cycles/y")
+    assert dims == (2, N)
+    np.testing.assert_allclose(cy.reshape(dims)[1], blocks[1]["Observables"]["Cycles"][0]["cycles"], rtol=1e-15)
+    # sector pairs: every measurement of the block lands in one sector
+    dims, sy = h5dump.dataset(f, "/Observables/Cycles/sectors/y")
+    assert sum(v for _, v in sy.reshape(-1, 2)) == 2 * 3
+    # restart from the dump: the permuted path goes on as the binding's does
+    dims, pdump = h5dump.dataset(f, "/Observables/path_dump/e/positions")
+    last_pos = pdump.reshape(dims)[-1].transpose(2, 1, 0)
+    blocks2 = run("Q", 'init_type="Restart" prefix="P"')
+    binding(last_pos, sig, blocks2, str(tmp_path / "Q.0.h5"))
