@@ -45,6 +45,9 @@ def parse():
                          "gives the per-kernel breakdown)")
     ap.add_argument("--no-graph-line", dest="graph_line", action="store_false",
                     help="skip the second timing of the same sweeps replayed from captured hipGraphs")
+    ap.add_argument("--perm", action="store_true",
+                    help="time the permuting bisection (PermBisectIterative) on the same workload with Bose "
+                         "statistics instead of the headline sweep; prints its own JSON line")
     ap.add_argument("--copy-mib", type=int, default=4096,
                     help="size of the device-to-device copy timed first (the practical HBM ceiling for a "
                          "read-one-write-one stream, reported as hbm_copy_GBps; 0 skips it)")
@@ -123,6 +126,8 @@ def main():
         sys.exit(2)
     from simpimc_amd import abi, system
 
+    if a.perm:
+        return perm_bench(a, abi, system)
     copy_gbps = copy_ceiling(a.copy_mib) if a.copy_mib else None
     base = system.config_c2()
     R = a.replicas
@@ -252,6 +257,49 @@ def main():
         "acceptance_replica0": {"bisect": float(c0[1]) / max(1, c0[0]), "displace": float(c0[3]) / max(1, c0[2])},
     }
     print(json.dumps(out))
+
+
+def perm_bench(a, abi, system):
+    """Permuting bisection on the headline workload's system: a step is
+    N M / 2^l PermBisectIterative attempts on every replica.  Reported: bead
+    updates/s (n_chain (2^l - 1) per attempt that reached the bisection), the
+    attempts' cycle lengths and acceptance, and the time per kernel class."""
+    base = system.config_c2()
+    R = a.replicas
+    eng = abi.Engine([base] * R, seed=2027)
+    for r in range(R):
+        eng.upload(system.random_path(base, seed=3000 + r, spread=0.05), r=r)
+    n_att = (base.n_part * base.n_bead) >> base.bisect_n_level
+    eng.timing(abi.TIMING_EVENTS)
+    for i in range(a.warmup):
+        eng.perm_bisect(n_att)
+        eng.bisect(n_att)
+        print(f"warmup step {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
+    eng.timing_reset()
+    u0 = eng.bead_updates()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        eng.perm_bisect(n_att)
+        print(f"timed step {i + 1}/{a.steps} done at {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    value = (eng.bead_updates() - u0) / dt
+    kernels = {}
+    for i, n in enumerate(["propose", "pair", "rho", "decide"]):
+        ms, cnt, _ = eng.kernel_timing(i)
+        if cnt:
+            kernels[n] = {"ms": ms, "launches": cnt, "avg_us": 1e3 * ms / cnt}
+    att, acc, pa, pc = eng.perm_counters(0)
+    eng.close()
+    print(json.dumps({
+        "metric": "bead updates/s, permuting bisection, HEG N=256 M=128 FP64",
+        "value": value, "unit": "bead updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * dt / a.steps, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"HEG N=256 M=128 bosons, David n_order=2, Ewald {base.n_k} k-vectors, "
+                               f"PermBisectIterative level {base.bisect_n_level}, {n_att} attempts per step on each "
+                               f"of {R} replicas", "replicas": R},
+        "kernels": kernels,
+        "replica0": {"attempts": att, "accepts": acc, "perm_attempt": pa[:8].tolist(), "perm_accept": pc[:8].tolist()},
+    }))
 
 
 if __name__ == "__main__":

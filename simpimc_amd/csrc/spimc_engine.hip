@@ -1075,13 +1075,27 @@ int launch_perm_round(spimc_ctx* c, const Group& G) {
     int rc;
     const size_t shmem = (size_t)c->max_N * (sizeof(double) + sizeof(int));
     if ((rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_perm, R, kPermBlock, shmem, st, reps, R))) return rc;
-    if ((rc = launch_pair_multi(c, G, st))) return rc;
-    if (c->any_long) {
-        int max_chunks = 1;
-        for (int r = G.r0; r < G.r0 + G.n; ++r) max_chunks = std::max(max_chunks, c->reps[r].dev.n_chunks);
-        dim3 grid(max_chunks, std::max(1, G.max_M - 1), R);
-        if ((rc = launch(c, SPIMC_KERNEL_RHO, k_rho_multi, grid, kRhoBlock, 0, st, reps))) return rc;
+    // rho_k beside the pair kernel on the group's second stream when it has one
+    const bool fork = c->any_long && G.side;
+    hipStream_t rst = fork ? G.side : st;
+    if (fork) {
+        HIPCHK(hipEventRecord(G.ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(G.side, G.ev_fork, 0));
     }
+    if (c->any_long) {
+        int max_chunks = 1, max_slots = 1;
+        for (int r = G.r0; r < G.r0 + G.n; ++r) {
+            max_chunks = std::max(max_chunks, c->reps[r].dev.n_chunks);
+            max_slots = std::max(max_slots, (1 << c->reps[r].dev.perm_n_level) - 1);
+        }
+        dim3 grid(max_chunks, max_slots, R);
+        auto fn = c->rho_kpt == 2 ? k_rho_multi<2> : c->rho_kpt == 4 ? k_rho_multi<4> : c->rho_kpt == 16 ? k_rho_multi<16>
+                                                                                                : k_rho_multi<8>;
+        if ((rc = launch(c, SPIMC_KERNEL_RHO, fn, grid, kRhoBlock, 0, rst, reps))) return rc;
+    }
+    if (fork) HIPCHK(hipEventRecord(G.ev_join, G.side));
+    if ((rc = launch_pair_multi(c, G, st))) return rc;
+    if (fork) HIPCHK(hipStreamWaitEvent(st, G.ev_join, 0));
     return launch(c, SPIMC_KERNEL_DECIDE, k_decide, R, 64, 0, st, reps, R, (unsigned long long*)nullptr);
 }
 
@@ -1100,8 +1114,9 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
     }
     if (rc) return rc;
     // a bisection chain that crosses the wrap of a permuted path changes label:
-    // the label-general pair kernel takes those proposals
-    if (move == 0 && c->perm_on && (rc = launch_pair_multi(c, G, st))) return rc;
+    // the label-general pair kernel takes those proposals, after the tuned pair
+    // kernel on the same stream (beside rho_k when that is forked)
+    const bool multi = move == 0 && c->perm_on;
     const uint2* pi = move == 0 ? G.d_pair_bis : G.d_pair_dis;
     const int np = move == 0 ? G.n_pair_bis : G.n_pair_dis;
     const RhoItem* ri = move == 0 ? G.d_ritem_bis : G.d_ritem_dis;
@@ -1117,6 +1132,7 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
         const int bpr = std::max(1, std::min((max_links + waves - 1) / waves, (2 * kNumCUs + R - 1) / R));
         rc = launch(c, SPIMC_KERNEL_PAIR, c->pair_lds_fn, R * bpr, c->pair_block, shmem, st, reps, R, bpr, kt);
         if (rc) return rc;
+        if (multi && (rc = launch_pair_multi(c, G, st))) return rc;
         HIPCHK(hipStreamWaitEvent(st, G.ev_join, 0));
     } else if (c->lds_doubles_max > 0 && np) {
         const size_t shmem = (size_t)c->lds_doubles_max * sizeof(double);
@@ -1126,6 +1142,7 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
         const int bpr = std::max(1, std::min((max_links + waves - 1) / waves, (2 * kNumCUs + R - 1) / R));
         rc = launch(c, SPIMC_KERNEL_PAIR, c->pair_lds_fn, R * bpr, c->pair_block, shmem, st, reps, R, bpr, kt);
         if (rc) return rc;
+        if (multi && (rc = launch_pair_multi(c, G, st))) return rc;
         if (nr) rc = launch_rho(c, st, ri, nr, kt);
         if (rc) return rc;
     } else {
@@ -1134,6 +1151,7 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
             rc = launch(c, SPIMC_KERNEL_PAIR, k_pair, g, kPairBlock, 0, st, all, pi, np, 0, kt);
             if (rc) return rc;
         }
+        if (multi && (rc = launch_pair_multi(c, G, st))) return rc;
         if (nr) {
             if ((rc = launch_rho(c, st, ri, nr, kt))) return rc;
         }

@@ -19,7 +19,6 @@ namespace spimc {
 
 constexpr int kPermBlock = 256;
 constexpr int kPermMaxN = 4096;  // particles of a replica that runs permuting moves
-constexpr int kRhoKptMax = 16;   // largest k-vectors per thread of the rho_k chunks
 
 // Window position t (0..nb) of chain i of a permuting proposal: the chain is
 // walked from (ch_lbl[i], b0) and carries label ch_lbl2[i] past the wrap.
@@ -134,32 +133,60 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
             }
         }
         __syncthreads();
-        if (tid == 0) {
-            int bad = s_i[3];
-            long long cur = s_cur;
+        // the first wavefront takes the row: its sums, the continue draw and the
+        // selection; the terms are summed in a different order than the
+        // reference's running sums (a difference in the last bits)
+        if (tid < kWave) {
+            const int lane = tid;
             double q_p = 0., q_c = 0.;
-            for (int i = 0; i < N; ++i) {
+            for (int i = lane; i < N; i += kWave) {
                 const double ti = t_row[i];
                 q_p += ti;
                 q_c += (i == p0) ? ti : (vis[i] >= 0 ? 0. : ti);
             }
-            double u = tape ? tape_get(R, cur++, &bad) : rng.unif(1u + (uint32_t)step, 0);
-            int st = 0, nx = p;
-            if (q_c / q_p < u) {
-                st = 2;
-            } else {
-                const double x = tape ? tape_get(R, cur++, &bad) : rng.unif(1u + (uint32_t)step, 1);
-                double t_q = 0.;
-                int found = -1;
-                for (int i = 0; i < N; ++i) {
+            q_p = wave_sum(q_p);
+            q_c = wave_sum(q_c);
+            int bad = s_i[3];
+            long long cur = s_cur;
+            double u = 0., x = 0.;
+            if (lane == 0) u = tape ? tape_get(R, cur++, &bad) : rng.unif(1u + (uint32_t)step, 0);
+            u = __shfl(u, 0, kWave);
+            const bool go = !(q_c / q_p < u);
+            if (go && lane == 0) x = tape ? tape_get(R, cur++, &bad) : rng.unif(1u + (uint32_t)step, 1);
+            x = __shfl(x, 0, kWave);
+            int found = -1;
+            if (go) {
+                // t_Q of the reference's scan, lane by lane over contiguous blocks:
+                // the first index whose running sum of t_c / Q_p_c exceeds x
+                const int c = (N + kWave - 1) / kWave, lo = min(N, lane * c), hi = min(N, lo + c);
+                double part = 0.;
+                for (int i = lo; i < hi; ++i) {
+                    const double tc = (i == p0) ? t_row[i] : (vis[i] >= 0 ? 0. : t_row[i]);
+                    part += tc / q_c;
+                }
+                double incl = part;  // inclusive scan over the lanes
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const double v = __shfl_up(incl, o, kWave);
+                    if (lane >= o) incl += v;
+                }
+                double t_q = incl - part;
+                int mine = 1 << 30;
+                for (int i = lo; i < hi; ++i) {
                     const double tc = (i == p0) ? t_row[i] : (vis[i] >= 0 ? 0. : t_row[i]);
                     t_q += tc / q_c;
                     if (t_q > x) {
-                        found = i;
+                        mine = i;
                         break;
                     }
                 }
-                if (found < 0) {
+                for (int o = kWave / 2; o > 0; o >>= 1) mine = min(mine, __shfl_xor(mine, o, kWave));
+                found = mine < N ? mine : -1;
+            }
+            if (lane == 0) {
+                int st = 0, nx = p;
+                if (!go) {
+                    st = 2;
+                } else if (found < 0) {
                     st = 3;  // no particle selected (a degenerate row)
                 } else {
                     nx = found;
@@ -170,11 +197,11 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
                     else if (n >= N)
                         st = 3;
                 }
+                s_i[3] = bad;
+                s_cur = cur;
+                s_i[4] = st;
+                s_i[5] = nx;
             }
-            s_i[3] = bad;
-            s_cur = cur;
-            s_i[4] = st;
-            s_i[5] = nx;
         }
         __syncthreads();
         stop = s_i[4];
@@ -383,7 +410,9 @@ __global__ __launch_bounds__(kPairMultiBlock) void k_pair_multi(const Rep* __res
         const bool post_on = (in0 && post0) || (in1 && post1);
         const int n_items = (pre_on ? nch : 0) + (post_on ? nch : 0);
         double uo = 0., un = 0.;
-        for (int it = w; it < n_items; it += kWaves) {
+        // the link's listed labels one after the other; the workgroup's threads
+        // share the partners of each
+        for (int it = 0; it < n_items; ++it) {
             const bool is_post = !pre_on || it >= nch;
             const int ci = it < nch ? it : it - nch;
             int x;
@@ -400,7 +429,7 @@ __global__ __launch_bounds__(kPairMultiBlock) void k_pair_multi(const Rep* __res
                 xn0[d] = cx0 >= 0 ? moved_new(R, pr, nb, cx0, t0, d) : xo0[d];
                 xn1[d] = cx1 >= 0 ? moved_new(R, pr, nb, cx1, t1, d) : xo1[d];
             }
-            for (int q = lane; q < N; q += kWave) {
+            for (int q = threadIdx.x; q < N; q += kPairMultiBlock) {
                 if (q == x) continue;
                 // a particle listed on this link before x was paired with x already
                 int iq = 1 << 30;
@@ -454,6 +483,8 @@ __global__ __launch_bounds__(kPairMultiBlock) void k_pair_multi(const Rep* __res
 // few chains at a time.  Grid (k chunk, slot, replica).
 // ---------------------------------------------------------------------------
 constexpr int kPermPhaseBatch = 4;
+// KPT: k-vectors per thread, the rho_kpt of the replicas' k chunks
+template <int KPT>
 __global__ __launch_bounds__(kRhoBlock) void k_rho_multi(const Rep* __restrict__ reps) {
     const Rep& R = reps[blockIdx.z];
     const Prop pr = *R.prop;
@@ -467,11 +498,20 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_multi(const Rep* __restrict__
     const int s = row.x, cur = row.y;
     const double2* rc = R.rho + ((size_t)cur * M + s) * nk;
     double2* rn = R.rho + ((size_t)(1 - cur) * M + s) * nk;
-    const int ck = kRhoBlock * R.rho_kpt, k_lo = chunk * ck, k_hi = min(nk, k_lo + ck);
-    const int kpt = R.rho_kpt;
-    // this thread's k-vectors: k_lo + threadIdx.x + i * kRhoBlock, i < kpt (at most kRhoKptMax)
-    double dx[kRhoKptMax], dy[kRhoKptMax];
-    for (int i = 0; i < kRhoKptMax; ++i) dx[i] = dy[i] = 0.;
+    const int ck = kRhoBlock * KPT, k_lo = chunk * ck, k_hi = min(nk, k_lo + ck);
+    // this thread's k-vectors: k_lo + threadIdx.x + i * kRhoBlock, i < KPT; the row
+    // and k-table loads are issued before the phase chains are built
+    double dx[KPT], dy[KPT];
+    double2 rv[KPT];
+    int kp[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        dx[i] = dy[i] = 0.;
+        const int k = k_lo + threadIdx.x + i * kRhoBlock;
+        const bool ok = k < k_hi;
+        rv[i] = ok ? ld2<true>(rc + k) : make_double2(0., 0.);
+        kp[i] = ok ? gld(R.kidx + k) : 0;
+    }
     for (int i0 = 0; i0 < n; i0 += kPermPhaseBatch) {
         const int nbat = min(kPermPhaseBatch, n - i0);
         __syncthreads();
@@ -481,29 +521,29 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_multi(const Rep* __restrict__
             phase_chain(x * R.box, R.max_idx, tab + (size_t)c * stride);
         }
         __syncthreads();
-        for (int i = 0; i < kpt && i < kRhoKptMax; ++i) {
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) {
             const int k = k_lo + threadIdx.x + i * kRhoBlock;
-            if (k >= k_hi) break;
-            const int kp = R.kidx[k];
+            if (k >= k_hi) continue;
             for (int b = 0; b < nbat; ++b) {
-                const double2 en = phase_k(tab + (size_t)(b * 6 + 3) * stride, stride, kp);
-                const double2 eo = phase_k(tab + (size_t)(b * 6) * stride, stride, kp);
+                const double2 en = phase_k(tab + (size_t)(b * 6 + 3) * stride, stride, kp[i]);
+                const double2 eo = phase_k(tab + (size_t)(b * 6) * stride, stride, kp[i]);
                 dx[i] += en.x - eo.x;
                 dy[i] += en.y - eo.y;
             }
         }
     }
     double ao = 0., an = 0.;
-    for (int i = 0; i < kpt && i < kRhoKptMax; ++i) {
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
         const int k = k_lo + threadIdx.x + i * kRhoBlock;
-        if (k >= k_hi) break;
-        const double2 rv = rc[k];
+        if (k >= k_hi) continue;
         double2 nv;
-        nv.x = rv.x + dx[i];
-        nv.y = rv.y + dy[i];
-        rn[k] = nv;
-        const double cf = R.coef_s[k];
-        ao += cf * (rv.x * rv.x + rv.y * rv.y);
+        nv.x = rv[i].x + dx[i];
+        nv.y = rv[i].y + dy[i];
+        st2<true>(rn + k, nv);
+        const double cf = gld(R.coef_s + k);
+        ao += cf * (rv[i].x * rv[i].x + rv[i].y * rv[i].y);
         an += cf * (nv.x * nv.x + nv.y * nv.y);
     }
     ao = block_sum<kRhoBlock>(ao, red);
