@@ -393,6 +393,8 @@ struct spimc_ctx {
     unsigned long long work_seen[WORK_COUNT] = {0, 0};
     // a replica's path may be permuted: rounds also launch the label-general pair kernel
     bool perm_on = false;
+    // a round's decision runs in the next round's proposal launch (SPIMC_FUSE_ROUNDS=0: its own launch)
+    bool fuse_rounds = true;
 };
 
 namespace {
@@ -1107,7 +1109,11 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
     Rep* all = c->d_reps;          // kernels driven by work items: absolute indices
     hipStream_t st = G.stream;
     int rc;
-    if (move == 0) {
+    if (c->fuse_rounds) {
+        // the previous round's decision and this round's proposal in one launch
+        rc = move == 0 ? launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_bisect, R, 64, 0, st, reps, R, kt)
+                       : launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_displace, R, 64, 0, st, reps, R, kt);
+    } else if (move == 0) {
         rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_bisect, R, 64, 0, st, reps, R);
     } else {
         rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_displace, R, 64, 0, st, reps, R);
@@ -1156,7 +1162,15 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
             if ((rc = launch_rho(c, st, ri, nr, kt))) return rc;
         }
     }
+    if (c->fuse_rounds) return SPIMC_OK;  // decided by the next round's first launch or the stage's end
     return launch(c, SPIMC_KERNEL_DECIDE, k_decide, R, 64, 0, st, reps, R, kt);
+}
+
+// The decision of a stage's last round when rounds are fused.
+int launch_final_decide(spimc_ctx* c, const Group& G) {
+    if (!c->fuse_rounds) return SPIMC_OK;
+    unsigned long long* kt = c->clock ? G.d_kt : nullptr;
+    return launch(c, SPIMC_KERNEL_DECIDE, k_decide, G.n, 64, 0, G.stream, c->d_reps + G.r0, G.n, kt);
 }
 
 int ensure_accepted(spimc_ctx* c, uint32_t need) {
@@ -1283,6 +1297,9 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
             else if ((rc = launch_round(c, G, move)))
                 return rc;
         }
+    if (move != 3)
+        for (int g = 0; g < n_groups; ++g)
+            if (gmx[g] && (rc = launch_final_decide(c, c->groups[g]))) return rc;
     for (int g = 1; g < n_groups; ++g) {
         HIPCHK(hipEventRecord(c->groups[g].done, c->groups[g].stream));
         HIPCHK(hipStreamWaitEvent(c->stream, c->groups[g].done, 0));
@@ -1482,6 +1499,7 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         const int v = std::atoi(f);
         if (v == 2 || v == 4 || v == 8 || v == 16) c->rho_kpt = v;
     }
+    if (const char* f = std::getenv("SPIMC_FUSE_ROUNDS")) c->fuse_rounds = std::atoi(f) != 0;
     if (const char* f = std::getenv("SPIMC_RHO_NT")) c->rho_nt = std::atoi(f) != 0;
     // Launch shape (measured best on MI355X at 1024 replicas of N=256, M=128):
     // with enough replicas to fill the GPU, each round's rho_k kernel runs on a

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_rho_build(const Rep* __restrict__ reps,
 // per replica.  Coarse levels carry only the kinetic action (the pair action
 // returns 0 above max_level = 0, pair_action_class.h:269).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ reps, int n_reps) {
+__device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps, int n_reps) {
     const int r = blockIdx.x;
     if (r >= n_reps) return;
     const Rep& R = reps[r];
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ r
 // the wrap link (p, M-1) -> (perm[p], 0), the one link of the walk a rigid shift
 // changes.  The link into (p, 0) from (iperm[p], M-1), which the shift also
 // changes, is outside the walk and not counted, as in the reference.
-__global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__ reps, int n_reps) {
+__device__ __forceinline__ void propose_displace_body(const Rep* __restrict__ reps, int n_reps) {
     const int r = blockIdx.x;
     if (r >= n_reps) return;
     const Rep& R = reps[r];
@@ -801,8 +801,8 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_loop(const RhoItem* __restric
 // K5: level-0 Metropolis decision and commit (bisect_class.h:100-123,24-36,127-139;
 // displace_particle_class.h:68-86), one wavefront per replica.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int n_reps,
-                                               unsigned long long* __restrict__ kt) {
+__device__ __forceinline__ void decide_body(const Rep* __restrict__ reps, int n_reps,
+                                            unsigned long long* __restrict__ kt) {
     if (kt && blockIdx.x == 0) kt_close(kt);
     const int r = blockIdx.x;
     if (r >= n_reps) return;
@@ -942,6 +942,33 @@ __global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int
             R.mv_post[R.ch_lbl2[i]] = -1;
         }
     }
+}
+
+__global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ reps, int n_reps) {
+    propose_bisect_body(reps, n_reps);
+}
+__global__ __launch_bounds__(64) void k_propose_displace(const Rep* __restrict__ reps, int n_reps) {
+    propose_displace_body(reps, n_reps);
+}
+__global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int n_reps,
+                                               unsigned long long* __restrict__ kt) {
+    decide_body(reps, n_reps, kt);
+}
+// K5 of one round and K4 of the next in one launch: a replica's pending
+// proposal is decided and committed (nothing is pending at the first round of
+// a stage), then its next proposal is made.  Saves one dependent launch per
+// round; the stage's last decision is a k_decide of its own.
+__global__ __launch_bounds__(64) void k_decide_propose_bisect(const Rep* __restrict__ reps, int n_reps,
+                                                              unsigned long long* __restrict__ kt) {
+    decide_body(reps, n_reps, kt);
+    __syncthreads();
+    propose_bisect_body(reps, n_reps);
+}
+__global__ __launch_bounds__(64) void k_decide_propose_displace(const Rep* __restrict__ reps, int n_reps,
+                                                                unsigned long long* __restrict__ kt) {
+    decide_body(reps, n_reps, kt);
+    __syncthreads();
+    propose_displace_body(reps, n_reps);
 }
 
 // Sums the per-replica work counters into work[WORK_PAIR_LINKS], work[WORK_RHO_SLICEK].
