@@ -177,3 +177,63 @@ def test_free_bosons_match_the_closed_form():
     print(f"cycles {nkm} +- {nkse}, exact {nk_exact}", flush=True)
     for k in range(N):
         assert abs(nkm[k] - nk_exact[k]) < 3.5 * nkse[k] + 1e-3, (k, nkm, nkse, nk_exact)
+
+
+def test_uneven_replicas_with_permutations():
+    """Replicas of different systems and sizes in one context, each with its own
+    number of attempts, replay their single chains' tapes bit for bit through
+    permuting, bisect and displace stages."""
+    systems = [dense_bare(), small_david(), system.config_ilkka(n_part=24, n_bead=8, n_level=2), free_system()]
+    counts = np.array([30, 20, 25, 12], dtype=np.uint32)
+    tapes, refs, finals, perms = [], [], [], []
+    for i, s in enumerate(systems):
+        pos = system.random_path(s, seed=70 + i, spread=0.3)
+        o = Oracle(s, seed=300 + i)
+        o.upload(pos)
+        o.record(True)
+        refs.append((o.perm_bisect(int(counts[i])), o.bisect(int(counts[i])), o.displace(int(counts[i]) // 2)))
+        tapes.append((pos, o.recorded()))
+        finals.append(o.download())
+        perms.append(o.perm())
+    with abi.Engine(systems, seed=5) as e:
+        for i, (pos, tape) in enumerate(tapes):
+            e.upload(pos, r=i)
+            e.tape_set(tape, r=i)
+        ap = e.perm_bisect(counts)
+        ab = e.bisect(counts)
+        ad = e.displace(counts // 2)
+        for i in range(len(systems)):
+            n = int(counts[i])
+            assert np.array_equal(ap[i, :n], refs[i][0]), i
+            assert np.array_equal(ab[i, :n], refs[i][1]), i
+            assert np.array_equal(ad[i, :n // 2], refs[i][2]), i
+            assert np.array_equal(e.download(r=i), finals[i]), i
+            assert np.array_equal(e.perm(r=i), perms[i]), i
+            assert e.tape_cursor(r=i) == len(tapes[i][1]), i
+
+
+def test_graph_sweep_on_permuted_paths():
+    """On permuted paths (after permuting stages on the device RNG) a sweep
+    replayed from captured graphs, which then also hold the label-general pair
+    kernel, equals the eager sweep of an identical context."""
+    s = dense_bare()
+    R = 6
+    flags = abi.SWEEP_ENERGY | abi.SWEEP_SK | abi.SWEEP_GR | abi.SWEEP_ADJOINT
+    out = []
+    for graph in (False, True):
+        with abi.Engine([s] * R, seed=77) as e:
+            for r in range(R):
+                e.upload(system.random_path(s, seed=90 + r, spread=0.3), r=r)
+            e.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, sk_k_cut=1e300, gr_n_r=50)
+            for _ in range(3):
+                e.perm_bisect(40)
+            e.sweep(2, flags | (abi.SWEEP_GRAPH if graph else 0))
+            out.append(([e.download(r=r) for r in range(R)], [e.perm(r=r) for r in range(R)],
+                        [e.counters(r=r) for r in range(R)], [e.obs_read(abi.OBS_ENERGY, r=r)[0] for r in range(R)]))
+    (p0, s0, c0, e0), (p1, s1, c1, e1) = out
+    assert any(not np.array_equal(x, np.arange(s.n_part)) for x in s0), "no replica was permuted"
+    for r in range(R):
+        assert np.array_equal(p0[r], p1[r]), r
+        assert np.array_equal(s0[r], s1[r]), r
+        assert np.array_equal(c0[r], c1[r]), r
+        assert np.array_equal(e0[r], e1[r]), r
