@@ -74,7 +74,7 @@ struct Prop {
     int n_chain;  // chains of a permuting bisection (0: the cycle walk stopped)
     int e_slice;  // slice after which a permuting bisection relinks the chains
     int n_level;  // bisection level of a permuting bisection
-    int pad3;
+    int rho_multi;  // 1: k_rho_multi updates rho_k (several beads per slice); 0: the k_rho items
 };
 
 struct Counters {
@@ -186,7 +186,12 @@ struct Rep {
     unsigned long long* perm_cnt;  // [2 + 2N] attempts, accepts, perm_attempt[N], perm_accept[N]
     double perm_log_eps;            // log of the table cutoff epsilon
     int perm_n_level;               // bisection level of the permuting move
-    int pad5;
+    int multi_cap;                  // replicas of the group: the second list's offset
+    // per group: the proposals of a permuting round that the label-general pair
+    // kernel (cnt[0], list[0..]) and k_rho_multi (cnt[1], list[multi_cap..]) take,
+    // as replica indices within the group
+    unsigned int* multi_cnt;
+    int* multi_list;
 };
 
 // Path topology (Bead::next_ / prev_ across the wrap, which is where

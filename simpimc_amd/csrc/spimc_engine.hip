@@ -332,6 +332,8 @@ struct Group {
     // the pair kernel (both need only the proposal), joined before the decision
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // permuting rounds: the proposals for the label-general kernels (2 counters, 2 lists of n)
+    unsigned int* d_multi = nullptr;
 };
 
 struct spimc_ctx {
@@ -1069,15 +1071,19 @@ int launch_pair_multi(spimc_ctx* c, const Group& G, hipStream_t st) {
                   kPairMultiBpr);
 }
 
-// A permuting bisection round: propose, label-general pair and rho_k, decide.
+// A permuting bisection round.  One-chain proposals (most of them) run on the
+// Boltzmann bisection's kernels: the LDS pair kernel and the rho_k items;
+// several chains, or a chain whose label changes inside the window, on the
+// label-general pair kernel and k_rho_multi.  rho_k runs beside the pair
+// kernels on the group's second stream.
 int launch_perm_round(spimc_ctx* c, const Group& G) {
     const int R = G.n;
     Rep* reps = c->d_reps + G.r0;
     hipStream_t st = G.stream;
     int rc;
+    const bool tuned = c->lds_doubles_max > 0;
     const size_t shmem = (size_t)c->max_N * (sizeof(double) + sizeof(int));
-    if ((rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_perm, R, kPermBlock, shmem, st, reps, R))) return rc;
-    // rho_k beside the pair kernel on the group's second stream when it has one
+    if ((rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_perm, R, kPermBlock, shmem, st, reps, R, (int)tuned))) return rc;
     const bool fork = c->any_long && G.side;
     hipStream_t rst = fork ? G.side : st;
     if (fork) {
@@ -1085,20 +1091,39 @@ int launch_perm_round(spimc_ctx* c, const Group& G) {
         HIPCHK(hipStreamWaitEvent(G.side, G.ev_fork, 0));
     }
     if (c->any_long) {
+        if (G.n_ritem_bis && (rc = launch_rho(c, rst, G.d_ritem_bis, G.n_ritem_bis, nullptr))) return rc;
         int max_chunks = 1, max_slots = 1;
         for (int r = G.r0; r < G.r0 + G.n; ++r) {
             max_chunks = std::max(max_chunks, c->reps[r].dev.n_chunks);
             max_slots = std::max(max_slots, (1 << c->reps[r].dev.perm_n_level) - 1);
         }
-        dim3 grid(max_chunks, max_slots, R);
-        auto fn = c->rho_kpt == 2 ? k_rho_multi<2> : c->rho_kpt == 4 ? k_rho_multi<4> : c->rho_kpt == 16 ? k_rho_multi<16>
-                                                                                                : k_rho_multi<8>;
-        if ((rc = launch(c, SPIMC_KERNEL_RHO, fn, grid, kRhoBlock, 0, rst, reps))) return rc;
+        // a fixed grid over the listed proposals' (replica, slot, chunk) items
+        auto fn = c->rho_kpt == 2    ? k_rho_multi_list<2>
+                  : c->rho_kpt == 4  ? k_rho_multi_list<4>
+                  : c->rho_kpt == 16 ? k_rho_multi_list<16>
+                                     : k_rho_multi_list<8>;
+        const int* list = reinterpret_cast<const int*>(G.d_multi + 2) + G.n;
+        if ((rc = launch(c, SPIMC_KERNEL_RHO, fn, 4 * kNumCUs, kRhoBlock, 0, rst, reps, list, G.d_multi, max_slots,
+                         max_chunks)))
+            return rc;
     }
     if (fork) HIPCHK(hipEventRecord(G.ev_join, G.side));
-    if ((rc = launch_pair_multi(c, G, st))) return rc;
+    if (tuned) {
+        const size_t lds = (size_t)c->lds_doubles_max * sizeof(double);
+        int max_links = 1;
+        for (int r = G.r0; r < G.r0 + G.n; ++r) max_links = std::max(max_links, 1 << c->reps[r].dev.perm_n_level);
+        const int waves = c->pair_block / 64;
+        const int bpr = std::max(1, std::min((max_links + waves - 1) / waves, (2 * kNumCUs + R - 1) / R));
+        if ((rc = launch(c, SPIMC_KERNEL_PAIR, c->pair_lds_fn, R * bpr, c->pair_block, lds, st, reps, R, bpr,
+                         (unsigned long long*)nullptr)))
+            return rc;
+    }
+    // as many workgroups as a full grid: a listed item each, the others return at once
+    if ((rc = launch(c, SPIMC_KERNEL_PAIR, k_pair_multi_list, R * kPairMultiBpr, kPairMultiBlock, 0, st, reps,
+                     reinterpret_cast<const int*>(G.d_multi + 2), (const unsigned int*)G.d_multi, kPairMultiBpr)))
+        return rc;
     if (fork) HIPCHK(hipStreamWaitEvent(st, G.ev_join, 0));
-    return launch(c, SPIMC_KERNEL_DECIDE, k_decide, R, 64, 0, st, reps, R, (unsigned long long*)nullptr);
+    return launch(c, SPIMC_KERNEL_DECIDE, k_decide_perm, R, 64, 0, st, reps, R, G.d_multi);
 }
 
 int launch_round(spimc_ctx* c, const Group& G, int move) {
@@ -1253,6 +1278,19 @@ int ensure_perm(spimc_ctx* c, bool scratch) {
             changed = true;
         }
     }
+    if (scratch)
+        for (auto& G : c->groups) {
+            if (G.d_multi) continue;
+            HIPCHK(hipMalloc((void**)&G.d_multi, (2 + 2 * (size_t)G.n) * sizeof(unsigned int)));
+            HIPCHK(hipMemset(G.d_multi, 0, (2 + 2 * (size_t)G.n) * sizeof(unsigned int)));
+            for (int r = G.r0; r < G.r0 + G.n; ++r) {
+                Rep& R = c->reps[r].dev;
+                R.multi_cnt = G.d_multi;
+                R.multi_list = reinterpret_cast<int*>(G.d_multi + 2);
+                R.multi_cap = G.n;
+            }
+            changed = true;
+        }
     if (!changed) return SPIMC_OK;
     c->perm_on = true;
     drop_graphs(c);
@@ -1639,7 +1677,8 @@ int spimc_ctx_destroy(spimc_ctx* c) {
         if (G.stream) (void)hipStreamSynchronize(G.stream);
         for (auto& g : G.graph)
             if (g) (void)hipGraphExecDestroy(g);
-        for (void* p : {(void*)G.d_pair_bis, (void*)G.d_pair_dis, (void*)G.d_ritem_bis, (void*)G.d_ritem_dis, (void*)G.d_kt})
+        for (void* p : {(void*)G.d_pair_bis, (void*)G.d_pair_dis, (void*)G.d_ritem_bis, (void*)G.d_ritem_dis, (void*)G.d_kt,
+                        (void*)G.d_multi})
             if (p) (void)hipFree(p);
         if (G.done) (void)hipEventDestroy(G.done);
         if (G.side) {

@@ -48,7 +48,9 @@ __device__ __forceinline__ double chain_new(const Rep& R, int b0, int nb, int n,
 // closing link t(p, p0) restored (:48-53).
 // Dynamic LDS: N doubles (the row) and N ints (visited step, -1 if not).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restrict__ reps, int n_reps) {
+// tuned: the round also runs the LDS pair kernel, which takes a one-chain
+// proposal whose label does not change inside the window
+__global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restrict__ reps, int n_reps, int tuned) {
     const int r = blockIdx.x;
     if (r >= n_reps) return;
     const Rep& R = reps[r];
@@ -73,8 +75,10 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
         }
     }
     __syncthreads();
+    // an attempt that does not reach level 0 leaves the bisection's rho_k items idle
     if (!s_i[0]) {
         if (tid == 0) pr->status = 0;
+        if (tid < kWave) rho_items_write(R, 0, 0, 0, 0, tid);
         return;
     }
     const int N = R.N, M = R.M, n_lev = R.perm_n_level, nb = 1 << n_lev;
@@ -235,6 +239,7 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
             R.mv_pre[R.ch_lbl[i]] = -1;
             R.mv_post[R.ch_lbl2[i]] = -1;
         }
+        if (tid < kWave) rho_items_write(R, 0, 0, 0, 0, tid);
         return;
     }
     const double prev0 = -log(s_w[0]);
@@ -301,6 +306,7 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
         lspn = block_sum<kPermBlock>(lspn, red);
         ko = block_sum<kPermBlock>(ko, red);
         kn = block_sum<kPermBlock>(kn, red);
+        bad = __syncthreads_or(bad);  // a tape overrun in any thread's draws
         cur += 3 * n * nmid;
         g_base += 3u * (uint32_t)(n * nmid);
         if (tid == 0) {
@@ -314,6 +320,7 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
                 s_w[1] = change;
             } else {
                 pr->status = bad ? 3 : 1;
+                s_i[7] = bad ? 3 : 1;
                 pr->move = 3;
                 pr->p = R.ch_lbl[0];
                 pr->p2 = R.ch_lbl2[0];
@@ -353,13 +360,50 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
                 R.mv_pre[R.ch_lbl[i]] = -1;
                 R.mv_post[R.ch_lbl2[i]] = -1;
             }
+            if (tid < kWave) rho_items_write(R, 0, 0, 0, 0, tid);
             return;
         }
         prev = s_w[1];
     }
-    // level 0 reached: the rho_k rows of the moved slices
-    if (R.use_long_range)
-        for (int m = tid; m < nm; m += kPermBlock) moved_row(R, m, (b0 + 1 + m) % M);
+    // Level 0 reached.  A one-chain proposal (a 1-cycle: plain bisection of p0,
+    // whose label may change at the wrap) goes to the kernels of the Boltzmann
+    // bisection: its beads in prop_new, its phase chains and the bisection's
+    // rho_k items, and the LDS pair kernel unless its label changes in the
+    // window.  Several chains go to k_pair_multi and k_rho_multi.
+    const bool single = n == 1 && (!R.use_long_range || (R.ritems_bis && nm * R.n_chunks <= R.n_ritems_bis));
+    if (single) {
+        const int l1 = R.ch_lbl[0], l2 = R.ch_lbl2[0];
+        for (int idx = tid; idx < nm * 3; idx += kPermBlock) R.prop_new[idx] = R.ch_new[idx];
+        if (R.use_long_range)
+            for (int t = tid; t < 6 * nm; t += kPermBlock) {
+                const int m = t / 6;
+                double xo[3], xn[3];
+                for (int d = 0; d < 3; ++d) {
+                    xo[d] = chain_old(R, b0, m + 1, 0, d);
+                    xn[d] = R.ch_new[(size_t)m * 3 + d];
+                }
+                moved_phase_chain(R, m, t % 6, xo, xn);
+            }
+        if (tid == 0) {
+            const int gen = !tuned || ((l2 != l1) && (b0 + nb > M));
+            pr->generic = gen;
+            pr->rho_multi = 0;
+            if (gen && s_i[7] == 1 && R.multi_cnt) R.multi_list[atomicAdd(&R.multi_cnt[0], 1u)] = r;
+        }
+        if (tid < kWave) rho_items_write(R, 0, s_i[7], nm, (b0 + 1) % M, tid);
+    } else {
+        if (tid == 0) {
+            pr->rho_multi = 1;
+            // the label-general kernels of this round take the proposal
+            if (s_i[7] == 1 && R.multi_cnt) {
+                R.multi_list[atomicAdd(&R.multi_cnt[0], 1u)] = r;
+                if (R.use_long_range) R.multi_list[R.multi_cap + atomicAdd(&R.multi_cnt[1], 1u)] = r;
+            }
+        }
+        if (tid < kWave) rho_items_write(R, 0, 0, 0, 0, tid);
+        if (R.use_long_range)
+            for (int m = tid; m < nm; m += kPermBlock) moved_row(R, m, (b0 + 1 + m) % M);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -390,9 +434,8 @@ __device__ __forceinline__ double moved_new(const Rep& R, const Prop& pr, int nb
 }
 
 constexpr int kPairMultiBlock = 256;
-__global__ __launch_bounds__(kPairMultiBlock) void k_pair_multi(const Rep* __restrict__ reps, int n_reps, int bpr) {
-    const int r = blockIdx.x / bpr, part = blockIdx.x % bpr;
-    if (r >= n_reps) return;
+// part of bpr workgroups of replica r: links part, part + bpr, ...
+__device__ __forceinline__ void pair_multi_item(const Rep* __restrict__ reps, int r, int part, int bpr) {
     const Rep& R = reps[r];
     const Prop pr = *R.prop;
     if (pr.status != 1 || !pr.generic) return;
@@ -475,6 +518,19 @@ __global__ __launch_bounds__(kPairMultiBlock) void k_pair_multi(const Rep* __res
     }
 }
 
+__global__ __launch_bounds__(kPairMultiBlock) void k_pair_multi(const Rep* __restrict__ reps, int n_reps, int bpr) {
+    const int r = blockIdx.x / bpr;
+    if (r < n_reps) pair_multi_item(reps, r, blockIdx.x % bpr, bpr);
+}
+// The same over the replicas a permuting round listed (cnt[0] of them in
+// list[0..]): a fixed grid loops over their (replica, part) items.
+__global__ __launch_bounds__(kPairMultiBlock) void k_pair_multi_list(const Rep* __restrict__ reps,
+                                                                     const int* __restrict__ list,
+                                                                     const unsigned int* __restrict__ cnt, int bpr) {
+    const int n = (int)cnt[0] * bpr;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) pair_multi_item(reps, list[it / bpr], it % bpr, bpr);
+}
+
 // ---------------------------------------------------------------------------
 // Species::UpdateRhoK (species_class.h:406-425) for a permuting proposal: the
 // slice of moved slot m holds one moved bead per chain; its rho_k row gets the
@@ -485,11 +541,10 @@ __global__ __launch_bounds__(kPairMultiBlock) void k_pair_multi(const Rep* __res
 constexpr int kPermPhaseBatch = 4;
 // KPT: k-vectors per thread, the rho_kpt of the replicas' k chunks
 template <int KPT>
-__global__ __launch_bounds__(kRhoBlock) void k_rho_multi(const Rep* __restrict__ reps) {
-    const Rep& R = reps[blockIdx.z];
+__device__ __forceinline__ void rho_multi_item(const Rep* __restrict__ reps, int r, int m, int chunk) {
+    const Rep& R = reps[r];
     const Prop pr = *R.prop;
-    if (pr.status != 1 || pr.move != 3 || !R.use_long_range) return;
-    const int m = blockIdx.y, chunk = blockIdx.x;
+    if (pr.status != 1 || pr.move != 3 || !pr.rho_multi || !R.use_long_range) return;
     if (m >= pr.count || chunk >= R.n_chunks) return;
     __shared__ double2 tab[kPermPhaseBatch * 6 * (kMaxIdx + 1)];
     __shared__ double red[kRhoBlock / kWave];
@@ -549,6 +604,34 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_multi(const Rep* __restrict__
     ao = block_sum<kRhoBlock>(ao, red);
     an = block_sum<kRhoBlock>(an, red);
     if (threadIdx.x == 0) R.rho_part[m * R.n_chunks + chunk] = make_double2(ao, an);
+    __syncthreads();  // the LDS tables and partials are reused by the block's next item
+}
+template <int KPT>
+__global__ __launch_bounds__(kRhoBlock) void k_rho_multi(const Rep* __restrict__ reps) {
+    rho_multi_item<KPT>(reps, blockIdx.z, blockIdx.y, blockIdx.x);
+}
+// The same over the replicas a permuting round listed (cnt[1] of them in
+// list[0..]): a fixed grid loops over their (replica, slot, chunk) items.
+template <int KPT>
+__global__ __launch_bounds__(kRhoBlock) void k_rho_multi_list(const Rep* __restrict__ reps, const int* __restrict__ list,
+                                                              const unsigned int* __restrict__ cnt, int slots,
+                                                              int chunks) {
+    const int per = slots * chunks, n = (int)cnt[1] * per;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int q = it % per;
+        rho_multi_item<KPT>(reps, list[it / per], q / chunks, q % chunks);
+    }
+}
+
+// k_decide for a permuting round, which also empties the group's lists for the
+// next round
+__global__ __launch_bounds__(64) void k_decide_perm(const Rep* __restrict__ reps, int n_reps,
+                                                    unsigned int* __restrict__ multi_cnt) {
+    decide_body(reps, n_reps, nullptr);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        multi_cnt[0] = 0u;
+        multi_cnt[1] = 0u;
+    }
 }
 
 }  // namespace spimc
