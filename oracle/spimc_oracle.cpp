@@ -341,6 +341,7 @@ struct Oracle {
     std::vector<uint32_t> cyc;
     // PermBisect (perm_bisect_class.h:20-29, perm_bisect_iterative_class.h)
     double perm_log_epsilon = std::log(1.e-100);
+    uint32_t perm_n_level = 0;  // the permuting move's own level (its Bisect base, bisect_class.h:168)
     std::vector<uint64_t> perm_attempt, perm_accept;  // per cycle length - 1
     uint64_t perm_counters[2] = {0, 0};               // Move::n_attempt, n_accept
 
@@ -1420,6 +1421,7 @@ int spimc_oracle_create(const spimc_system_desc* s, uint64_t seed, void** out) {
     // Bisect (bisect_class.h:158-190)
     o->n_level = s->bisect_n_level;
     o->n_bisect_beads = 1u << o->n_level;
+    o->perm_n_level = o->n_level;
     o->bis_n_images = s->bisect_n_images;
     uint32_t n_bs = (uint32_t)std::floor(std::log2((double)o->M));
     if (o->n_level < 1 || o->n_level > n_bs) {
@@ -1763,6 +1765,10 @@ int spimc_oracle_displace_stage(void* h, uint32_t n_attempts, uint8_t* accepted)
 int spimc_oracle_perm_bisect_stage(void* h, uint32_t n_attempts, uint8_t* accepted) {
     Oracle* o = (Oracle*)h;
     g_err.clear();
+    // the permuting move bisects at its own level
+    const uint32_t saved_level = o->n_level;
+    o->n_level = o->perm_n_level;
+    o->n_bisect_beads = 1u << o->n_level;
     for (uint32_t i = 0; i < n_attempts; ++i) {
         uint32_t n = 0;
         o->perm_counters[0]++;
@@ -1781,6 +1787,8 @@ int spimc_oracle_perm_bisect_stage(void* h, uint32_t n_attempts, uint8_t* accept
         if (accepted) accepted[i] = ok ? 1 : 0;
         if (!g_err.empty()) break;
     }
+    o->n_level = saved_level;
+    o->n_bisect_beads = 1u << o->n_level;
     return g_err.empty() ? SPIMC_OK : SPIMC_ERR_STATE;
 }
 
@@ -1797,10 +1805,13 @@ int spimc_oracle_perm_counters(void* h, uint64_t* out) {
     return SPIMC_OK;
 }
 
-int spimc_oracle_perm_configure(void* h, double epsilon) {
+int spimc_oracle_perm_configure(void* h, uint32_t n_level, double epsilon) {
     Oracle* o = (Oracle*)h;
     if (!(epsilon > 0.)) return fail(SPIMC_ERR_ARG, "epsilon must be > 0");
+    if (n_level < 1 || n_level > (uint32_t)std::floor(std::log2((double)o->M)))
+        return fail(SPIMC_ERR_ARG, "permuting bisection n_level out of range");
     o->perm_log_epsilon = std::log(epsilon);
+    o->perm_n_level = n_level;
     return SPIMC_OK;
 }
 

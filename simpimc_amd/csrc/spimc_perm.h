@@ -249,7 +249,22 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
         const int i = idx / nm, t = idx % nm + 1;
         for (int d = 0; d < 3; ++d) R.ch_new[(size_t)idx * 3 + d] = chain_old(R, b0, t, i, d);
     }
+    // A window of the whole path (2^n_level == M) can close a chain on itself:
+    // its end, (label_after_wrap(ps[i + 1]), b0) in NEW mode or
+    // (label_after_wrap(ps[i]), b0) in OLD mode, is its first bead.  The
+    // reference's bead walks then stop at once (perm_bisect_iterative_class.h:163-185,
+    // kinetic_class.h:111-118): a chain closed in NEW mode has no midpoint and
+    // draws no normal, and a closed chain adds no kinetic link in that mode.
+    // The list of open chains (NEW mode) reuses the walk's visited map.
+    __shared__ int s_open;
+    if (tid == 0) {
+        int k = 0;
+        for (int i = 0; i < n; ++i)
+            if (nb != M || R.ch_lbl2[(i + 1) % n] != R.ch_lbl[i]) vis[k++] = i;
+        s_open = k;
+    }
     __syncthreads();
+    const int n_open = s_open;
     double prev = prev0;
     uint32_t g_base = 0;
     for (int level = n_lev - 1; level >= 0; --level) {
@@ -259,9 +274,9 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
         const double sigma = sqrt(R.lambda * level_tau);
         const double i4lt_b = 1. / (4. * R.lambda * (0.5 * R.tau * (double)skip));
         double lspo = 0., lspn = 0.;
-        for (int it = tid; it < n * nmid; it += kPermBlock) {
+        for (int it = tid; it < n_open * nmid; it += kPermBlock) {
 #pragma clang fp contract(off)
-            const int i = it / nmid, k = it % nmid;
+            const int i = vis[it / nmid], k = it % nmid;
             const int a = 2 * k * skip, b = a + skip, c = a + 2 * skip;
             double t[3], rbar[3], dold[3], dnew[3], nbv[3];
             for (int d = 0; d < 3; ++d) {
@@ -295,20 +310,26 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
         double ko = 0., kn = 0.;
         for (int it = tid; it < n * nlinks; it += kPermBlock) {
             const int i = it / nlinks, a = (it % nlinks) * skip, b = a + skip;
+            const bool ring = nb == M;
             double dd[3];
-            for (int d = 0; d < 3; ++d) dd[d] = minimg(chain_old(R, b0, a, i, d) - chain_old(R, b0, b, i, d), R.L, R.iL);
-            ko += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
-            for (int d = 0; d < 3; ++d)
-                dd[d] = minimg(chain_new(R, b0, nb, n, a, i, d) - chain_new(R, b0, nb, n, b, i, d), R.L, R.iL);
-            kn += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+            if (!ring || R.ch_lbl2[i] != R.ch_lbl[i]) {
+                for (int d = 0; d < 3; ++d)
+                    dd[d] = minimg(chain_old(R, b0, a, i, d) - chain_old(R, b0, b, i, d), R.L, R.iL);
+                ko += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+            }
+            if (!ring || R.ch_lbl2[(i + 1) % n] != R.ch_lbl[i]) {
+                for (int d = 0; d < 3; ++d)
+                    dd[d] = minimg(chain_new(R, b0, nb, n, a, i, d) - chain_new(R, b0, nb, n, b, i, d), R.L, R.iL);
+                kn += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+            }
         }
         lspo = block_sum<kPermBlock>(lspo, red);
         lspn = block_sum<kPermBlock>(lspn, red);
         ko = block_sum<kPermBlock>(ko, red);
         kn = block_sum<kPermBlock>(kn, red);
         bad = __syncthreads_or(bad);  // a tape overrun in any thread's draws
-        cur += 3 * n * nmid;
-        g_base += 3u * (uint32_t)(n * nmid);
+        cur += 3 * n_open * nmid;
+        g_base += 3u * (uint32_t)(n_open * nmid);
         if (tid == 0) {
             const double u = tape ? tape_get(R, cur, &bad) : rng.unif(c_lev + (uint32_t)level, 0);
             const double lsr = -lspn + lspo;

@@ -59,7 +59,7 @@ SIG = {
     "spimc_oracle_free_spline_log_rho": ([_d, _i32, _d, _d, _dp, _u32, _dp], _i32),
     "spimc_oracle_perm_bisect_stage": ([_c, _u32, _pu8], _i32),
     "spimc_oracle_perm_counters": ([_c, _pu64], _i32),
-    "spimc_oracle_perm_configure": ([_c, _d], _i32),
+    "spimc_oracle_perm_configure": ([_c, _u32, _d], _i32),
     "spimc_oracle_perm_get": ([_c, _pu32], _i32),
     "spimc_oracle_perm_set": ([_c, _pu32], _i32),
     "spimc_oracle_cycle_lengths": ([_c, _pu32, _pu32], _i32),
@@ -262,8 +262,8 @@ class Oracle:
         check(self.L.spimc_oracle_perm_counters(self.h, out.ctypes.data_as(_pu64)))
         return int(out[0]), int(out[1]), out[2:2 + self.n_part], out[2 + self.n_part:]
 
-    def perm_configure(self, epsilon=1e-100):
-        check(self.L.spimc_oracle_perm_configure(self.h, epsilon))
+    def perm_configure(self, n_level, epsilon=1e-100):
+        check(self.L.spimc_oracle_perm_configure(self.h, n_level, epsilon))
 
     def perm(self):
         out = np.zeros(self.n_part, dtype=np.uint32)

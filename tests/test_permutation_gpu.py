@@ -237,3 +237,75 @@ def test_graph_sweep_on_permuted_paths():
         assert np.array_equal(s0[r], s1[r]), r
         assert np.array_equal(c0[r], c1[r]), r
         assert np.array_equal(e0[r], e1[r]), r
+
+
+EDGE = {
+    # one particle: every cycle is a 1-cycle and the pair action is constant
+    "one_particle": (lambda: system.config_c1(n_part=1, n_bead=8, r_s=1.0, tau=0.1, n_level=2), None, None),
+    # 2^n_level = M on a permuted path: the window runs once around into the next particle
+    "window_is_path": (lambda: system.config_c1(n_part=4, n_bead=8, r_s=1.0, tau=0.1, n_level=3), None, [1, 2, 3, 0]),
+    # the permuting level above the bisection's: one-chain proposals outgrow the
+    # bisection's rho_k items and take the label-general rho_k kernel
+    "perm_level_above_bisect": (lambda: system.config_c1(n_part=8, n_bead=16, r_s=1.0, tau=0.05, n_level=2), 3,
+                                [1, 0, 2, 3, 5, 4, 6, 7]),
+    # two particles: the only cycles are the identity and the swap
+    "two_particles": (lambda: system.config_c1(n_part=2, n_bead=16, r_s=1.0, tau=0.1, n_level=2), None, None),
+}
+
+
+@pytest.mark.parametrize("case", sorted(EDGE))
+def test_permutation_edge_cases(case):
+    make, perm_level, sigma = EDGE[case]
+    s = make()
+    pos = system.random_path(s, seed=17, spread=0.3)
+    o = Oracle(s, seed=23)
+    o.upload(pos)
+    if sigma is not None:
+        o.set_perm(sigma)
+    if perm_level is not None:
+        o.perm_configure(perm_level)
+    o.record(True)
+    n = max(16, s.n_part * s.n_bead >> s.bisect_n_level)
+    ref = []
+    for _ in range(3):
+        ref.append(o.perm_bisect(n))
+        ref.append(o.bisect(n))
+        ref.append(o.displace(max(1, s.n_part)))
+    tape = o.recorded()
+    print(f"{case}: {len(tape)} draws, perm accepts {o.perm_counters()[3].tolist()}", flush=True)
+    with abi.Engine(s, seed=1) as e:
+        e.upload(pos)
+        if sigma is not None:
+            e.set_perm(sigma)
+        if perm_level is not None:
+            e.perm_configure(perm_level)
+        e.tape_set(tape)
+        got = []
+        for _ in range(3):
+            got.append(e.perm_bisect(n)[0])
+            got.append(e.bisect(n)[0])
+            got.append(e.displace(max(1, s.n_part))[0])
+        assert e.tape_cursor() == len(tape)
+        for i, (g, r) in enumerate(zip(got, ref)):
+            assert np.array_equal(g, r), (i, g, r)
+        assert np.array_equal(e.perm(), o.perm())
+        assert np.array_equal(e.download(), o.download())
+        if s.n_part > 1:
+            assert rel_err(e.rho_k(), o.rho_k()) < 1e-11 * s.n_part
+        else:
+            # one particle: the pair action is constant, so the reference never
+            # updates rho_k (pair_action_class.h:225,269) and the oracle's stays at
+            # the start; the engine keeps it in step with the beads.  |rho_k|^2 = 1,
+            # all that S(k) reads, is the same in both.
+            x = e.download()
+            want = np.exp(1j * np.einsum("kd,dm->mk", 2 * np.pi * s.k_n / s.L, x[:, :, 0]))
+            assert np.abs(e.rho_k() - want).max() < 1e-11
+            assert np.allclose(np.abs(o.rho_k()), 1.0, atol=1e-13)
+        assert np.array_equal(e.counters(), o.counters())
+        eg, og = e.perm_counters(), o.perm_counters()
+        assert eg[:2] == og[:2] and np.array_equal(eg[2], og[2]) and np.array_equal(eg[3], og[3])
+        assert e.bead_updates() == o.bead_updates()
+        a, b = e.d_action_d_beta(abi.ACTION_KINETIC), o.d_action_d_beta(abi.ACTION_KINETIC)
+        assert abs(a - b) <= 1e-10 * max(1.0, abs(b)), (a, b)
+    if case == "two_particles":
+        assert og[3][1] > 0  # swaps were accepted
