@@ -221,6 +221,7 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
         R.ch_lbl2[i] = l2;
         R.mv_post[l2] = i;
     }
+    __syncthreads();  // chain_old reads the labels of every chain across the wrap
     if (stop != 1) {
         if (tid == 0) {
             pr->status = stop == 3 ? 3 : 2;
