@@ -37,6 +37,68 @@ __device__ __forceinline__ double chain_new(const Rep& R, int b0, int nb, int n,
     return R.ch_new[((size_t)i * (nb - 1) + t - 1) * 3 + d];
 }
 
+constexpr int kPermStage = 256;  // beads of a proposal bisected on LDS by one wavefront
+
+// Orders the LDS traffic of one wavefront (the compiler may not move loads or
+// stores across it).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The Metropolis step closing level `level` of a permuting proposal
+// (perm_bisect_iterative_class.h:206-215), on one thread: returns 1 if the
+// proposal is rejected.  A coarse level returns its action change in *change;
+// level 0 fills the proposal for the pair, rho_k and decision kernels, and
+// *status (1, or 3 after a replay-tape overrun).
+__device__ __forceinline__ int perm_level_step(const Rep& R, Prop* pr, int level, double lspo, double lspn, double ko,
+                                               double kn, double prev, double u, int bad, int b0, int nb, int n,
+                                               int n_lev, long long cur, int* status, double* change) {
+    const int M = R.M, nm = nb - 1;
+    const double lsr = -lspn + lspo;
+    int rej = 0;
+    if (level > 0) {
+        const double ch = (0. + kn) - (0. + ko);
+        const double la = lsr - ch + prev;
+        rej = la < log(u);
+        *change = ch;
+    } else {
+        pr->status = bad ? 3 : 1;
+        *status = bad ? 3 : 1;
+        pr->move = 3;
+        pr->p = R.ch_lbl[0];
+        pr->p2 = R.ch_lbl2[0];
+        pr->generic = 1;
+        pr->b0 = b0;
+        pr->n_links = R.pair_type ? nb : 0;
+        pr->first = (b0 + 1) % M;
+        pr->count = nm;
+        pr->lsr = lsr;
+        pr->k_old = ko;
+        pr->k_new = kn;
+        pr->prev = prev;
+        pr->u0 = u;
+        pr->tape_next = cur + 1;
+        pr->n_chain = n;
+        pr->e_slice = (b0 + nb - 1) % M;
+        pr->n_level = n_lev;
+    }
+    if (rej) {
+        pr->status = 2;
+        pr->move = 3;
+        pr->p = R.ch_lbl[0];
+        pr->p2 = R.ch_lbl2[0];
+        pr->generic = 1;
+        pr->b0 = b0;
+        pr->count = 0;
+        pr->n_chain = n;
+        pr->n_level = n_lev;
+        pr->tape_next = cur + 1;
+    }
+    return rej;
+}
+
 // ---------------------------------------------------------------------------
 // PermBisectIterative::Attempt (perm_bisect_iterative_class.h:113-223) down to
 // level 0.  Draws in the reference's order: bead0, the first particle, per
@@ -244,12 +306,7 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
         return;
     }
     const double prev0 = -log(s_w[0]);
-    // NEW positions start as the old ones
     const int nm = nb - 1;
-    for (int idx = tid; idx < n * nm; idx += kPermBlock) {
-        const int i = idx / nm, t = idx % nm + 1;
-        for (int d = 0; d < 3; ++d) R.ch_new[(size_t)idx * 3 + d] = chain_old(R, b0, t, i, d);
-    }
     // A window of the whole path (2^n_level == M) can close a chain on itself:
     // its end, (label_after_wrap(ps[i + 1]), b0) in NEW mode or
     // (label_after_wrap(ps[i]), b0) in OLD mode, is its first bead.  The
@@ -264,118 +321,120 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
             if (nb != M || R.ch_lbl2[(i + 1) % n] != R.ch_lbl[i]) vis[k++] = i;
         s_open = k;
     }
+    // A proposal of at most kPermStage beads (a few chains) is bisected by the
+    // first wavefront on its beads staged in LDS, [chain][t = 0..nb][d]; a
+    // larger one by the whole workgroup on global memory.  Both do the same
+    // arithmetic in the same order per bead, with the draws in the same order.
+    const bool fast = n * (nb + 1) <= kPermStage;
+    __shared__ double st_old[kPermStage * 3], st_new[kPermStage * 3];
+    const int per = (nb + 1) * 3;
+    if (fast) {
+        for (int idx = tid; idx < n * per; idx += kPermBlock) {
+            const int i = idx / per, t = (idx % per) / 3, d = idx % 3;
+            const double x = chain_old(R, b0, t, i, d);
+            st_old[idx] = x;
+            // NEW mode: the end of chain i is the old end of chain i + 1
+            st_new[idx] = t == nb ? chain_old(R, b0, nb, (i + 1) % n, d) : x;
+        }
+    } else {
+        // NEW positions start as the old ones
+        for (int idx = tid; idx < n * nm; idx += kPermBlock) {
+            const int i = idx / nm, t = idx % nm + 1;
+            for (int d = 0; d < 3; ++d) R.ch_new[(size_t)idx * 3 + d] = chain_old(R, b0, t, i, d);
+        }
+    }
     __syncthreads();
     const int n_open = s_open;
-    double prev = prev0;
-    uint32_t g_base = 0;
-    for (int level = n_lev - 1; level >= 0; --level) {
-        const int skip = 1 << level;
-        const int nmid = nb >> (level + 1);
-        const double level_tau = R.tau * skip;
-        const double sigma = sqrt(R.lambda * level_tau);
-        const double i4lt_b = 1. / (4. * R.lambda * (0.5 * R.tau * (double)skip));
-        double lspo = 0., lspn = 0.;
-        for (int it = tid; it < n_open * nmid; it += kPermBlock) {
-#pragma clang fp contract(off)
-            const int i = vis[it / nmid], k = it % nmid;
-            const int a = 2 * k * skip, b = a + skip, c = a + 2 * skip;
-            double t[3], rbar[3], dold[3], dnew[3], nbv[3];
-            for (int d = 0; d < 3; ++d) {
-                const double xa = chain_old(R, b0, a, i, d);
-                t[d] = minimg(chain_old(R, b0, c, i, d) - xa, R.L, R.iL);
-                rbar[d] = xa + 0.5 * t[d];
-                dold[d] = minimg(chain_old(R, b0, b, i, d) - rbar[d], R.L, R.iL);
-            }
-            for (int d = 0; d < 3; ++d) {
-                double z;
-                if (tape) {
-                    z = tape_get(R, cur + 3 * it + d, &bad);
-                } else {
-                    const uint32_t g = g_base + 3u * (uint32_t)it + (uint32_t)d;
-                    z = rng.normal(c_norm + g / 2u, (int)(g & 1u));
-                }
-                dnew[d] = sigma * z;
-                dnew[d] = minimg(dnew[d], R.L, R.iL);
-                const double xa = chain_new(R, b0, nb, n, a, i, d);
-                t[d] = minimg(chain_new(R, b0, nb, n, c, i, d) - xa, R.L, R.iL);
-                rbar[d] = xa + 0.5 * t[d];
-                nbv[d] = rbar[d] + dnew[d];
-            }
-            for (int d = 0; d < 3; ++d) R.ch_new[((size_t)i * nm + b - 1) * 3 + d] = nbv[d];
-            lspo -= free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dold);
-            lspn -= free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dnew);
-        }
-        __syncthreads();
-        const int nlinks = nb >> level;
-        const double i4lt_k = 1. / (4. * R.lambda * level_tau);
-        double ko = 0., kn = 0.;
-        for (int it = tid; it < n * nlinks; it += kPermBlock) {
-            const int i = it / nlinks, a = (it % nlinks) * skip, b = a + skip;
-            const bool ring = nb == M;
-            double dd[3];
-            if (!ring || R.ch_lbl2[i] != R.ch_lbl[i]) {
-                for (int d = 0; d < 3; ++d)
-                    dd[d] = minimg(chain_old(R, b0, a, i, d) - chain_old(R, b0, b, i, d), R.L, R.iL);
-                ko += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
-            }
-            if (!ring || R.ch_lbl2[(i + 1) % n] != R.ch_lbl[i]) {
-                for (int d = 0; d < 3; ++d)
-                    dd[d] = minimg(chain_new(R, b0, nb, n, a, i, d) - chain_new(R, b0, nb, n, b, i, d), R.L, R.iL);
-                kn += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
-            }
-        }
-        lspo = block_sum<kPermBlock>(lspo, red);
-        lspn = block_sum<kPermBlock>(lspn, red);
-        ko = block_sum<kPermBlock>(ko, red);
-        kn = block_sum<kPermBlock>(kn, red);
-        bad = __syncthreads_or(bad);  // a tape overrun in any thread's draws
-        cur += 3 * n_open * nmid;
-        g_base += 3u * (uint32_t)(n_open * nmid);
-        if (tid == 0) {
-            const double u = tape ? tape_get(R, cur, &bad) : rng.unif(c_lev + (uint32_t)level, 0);
-            const double lsr = -lspn + lspo;
+    if (fast) {
+        if (tid < kWave) {
+            const int lane = tid;
+            double prev = prev0;
+            uint32_t g_base = 0;
             int rej = 0;
-            if (level > 0) {
-                const double change = (0. + kn) - (0. + ko);
-                const double la = lsr - change + prev;
-                rej = la < log(u);
-                s_w[1] = change;
-            } else {
-                pr->status = bad ? 3 : 1;
-                s_i[7] = bad ? 3 : 1;
-                pr->move = 3;
-                pr->p = R.ch_lbl[0];
-                pr->p2 = R.ch_lbl2[0];
-                pr->generic = 1;
-                pr->b0 = b0;
-                pr->n_links = R.pair_type ? nb : 0;
-                pr->first = (b0 + 1) % M;
-                pr->count = nm;
-                pr->lsr = lsr;
-                pr->k_old = ko;
-                pr->k_new = kn;
-                pr->prev = prev;
-                pr->u0 = u;
-                pr->tape_next = cur + 1;
-                pr->n_chain = n;
-                pr->e_slice = (b0 + nb - 1) % M;
-                pr->n_level = n_lev;
+            for (int level = n_lev - 1; level >= 0 && !rej; --level) {
+                const int skip = 1 << level;
+                const int nmid = nb >> (level + 1);
+                const double level_tau = R.tau * skip;
+                const double sigma = sqrt(R.lambda * level_tau);
+                const double i4lt_b = 1. / (4. * R.lambda * (0.5 * R.tau * (double)skip));
+                double lspo = 0., lspn = 0.;
+                for (int it = lane; it < n_open * nmid; it += kWave) {
+#pragma clang fp contract(off)
+                    const int i = vis[it / nmid], k = it % nmid;
+                    const int a = 2 * k * skip, b = a + skip, c = a + 2 * skip;
+                    const double* so = st_old + i * per;
+                    double* sn = st_new + i * per;
+                    double t[3], rbar[3], dold[3], dnew[3], nbv[3];
+                    for (int d = 0; d < 3; ++d) {
+                        const double xa = so[3 * a + d];
+                        t[d] = minimg(so[3 * c + d] - xa, R.L, R.iL);
+                        rbar[d] = xa + 0.5 * t[d];
+                        dold[d] = minimg(so[3 * b + d] - rbar[d], R.L, R.iL);
+                    }
+                    for (int d = 0; d < 3; ++d) {
+                        double z;
+                        if (tape) {
+                            z = tape_get(R, cur + 3 * it + d, &bad);
+                        } else {
+                            const uint32_t g = g_base + 3u * (uint32_t)it + (uint32_t)d;
+                            z = rng.normal(c_norm + g / 2u, (int)(g & 1u));
+                        }
+                        dnew[d] = sigma * z;
+                        dnew[d] = minimg(dnew[d], R.L, R.iL);
+                        const double xa = sn[3 * a + d];
+                        t[d] = minimg(sn[3 * c + d] - xa, R.L, R.iL);
+                        rbar[d] = xa + 0.5 * t[d];
+                        nbv[d] = rbar[d] + dnew[d];
+                    }
+                    for (int d = 0; d < 3; ++d) sn[3 * b + d] = nbv[d];
+                    lspo -= free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dold);
+                    lspn -= free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dnew);
+                }
+                wave_lds_sync();
+                const int nlinks = nb >> level;
+                const double i4lt_k = 1. / (4. * R.lambda * level_tau);
+                double ko = 0., kn = 0.;
+                for (int it = lane; it < n * nlinks; it += kWave) {
+                    const int i = it / nlinks, a = (it % nlinks) * skip, b = a + skip;
+                    const bool ring = nb == M;
+                    const double* so = st_old + i * per;
+                    const double* sn = st_new + i * per;
+                    double dd[3];
+                    if (!ring || R.ch_lbl2[i] != R.ch_lbl[i]) {
+                        for (int d = 0; d < 3; ++d) dd[d] = minimg(so[3 * a + d] - so[3 * b + d], R.L, R.iL);
+                        ko += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+                    }
+                    if (!ring || R.ch_lbl2[(i + 1) % n] != R.ch_lbl[i]) {
+                        for (int d = 0; d < 3; ++d) dd[d] = minimg(sn[3 * a + d] - sn[3 * b + d], R.L, R.iL);
+                        kn += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+                    }
+                }
+                lspo = wave_sum(lspo);
+                lspn = wave_sum(lspn);
+                ko = wave_sum(ko);
+                kn = wave_sum(kn);
+                bad = __any(bad) ? 1 : 0;  // a tape overrun in any lane's draws
+                cur += 3 * n_open * nmid;
+                g_base += 3u * (uint32_t)(n_open * nmid);
+                double change = 0.;
+                if (lane == 0) {
+                    const double u = tape ? tape_get(R, cur, &bad) : rng.unif(c_lev + (uint32_t)level, 0);
+                    rej = perm_level_step(R, pr, level, lspo, lspn, ko, kn, prev, u, bad, b0, nb, n, n_lev, cur, &s_i[7],
+                                          &change);
+                }
+                rej = __shfl(rej, 0, kWave);
+                prev = __shfl(change, 0, kWave);
+                cur += 1;
+                wave_lds_sync();
             }
-            if (rej) {
-                pr->status = 2;
-                pr->move = 3;
-                pr->p = R.ch_lbl[0];
-                pr->p2 = R.ch_lbl2[0];
-                pr->generic = 1;
-                pr->b0 = b0;
-                pr->count = 0;
-                pr->n_chain = n;
-                pr->n_level = n_lev;
-                pr->tape_next = cur + 1;
-            }
-            s_i[6] = rej;
+            // the NEW beads for the round's other kernels and the decision
+            if (!rej)
+                for (int idx = lane; idx < n * nm; idx += kWave) {
+                    const int i = idx / nm, t = idx % nm + 1;
+                    for (int d = 0; d < 3; ++d) R.ch_new[(size_t)idx * 3 + d] = st_new[i * per + 3 * t + d];
+                }
+            if (lane == 0) s_i[6] = rej;
         }
-        cur += 1;
         __syncthreads();
         if (s_i[6]) {
             for (int i = tid; i < n; i += kPermBlock) {
@@ -385,7 +444,91 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
             if (tid < kWave) rho_items_write(R, 0, 0, 0, 0, tid);
             return;
         }
-        prev = s_w[1];
+    } else {
+        double prev = prev0;
+        uint32_t g_base = 0;
+        for (int level = n_lev - 1; level >= 0; --level) {
+            const int skip = 1 << level;
+            const int nmid = nb >> (level + 1);
+            const double level_tau = R.tau * skip;
+            const double sigma = sqrt(R.lambda * level_tau);
+            const double i4lt_b = 1. / (4. * R.lambda * (0.5 * R.tau * (double)skip));
+            double lspo = 0., lspn = 0.;
+            for (int it = tid; it < n_open * nmid; it += kPermBlock) {
+#pragma clang fp contract(off)
+                const int i = vis[it / nmid], k = it % nmid;
+                const int a = 2 * k * skip, b = a + skip, c = a + 2 * skip;
+                double t[3], rbar[3], dold[3], dnew[3], nbv[3];
+                for (int d = 0; d < 3; ++d) {
+                    const double xa = chain_old(R, b0, a, i, d);
+                    t[d] = minimg(chain_old(R, b0, c, i, d) - xa, R.L, R.iL);
+                    rbar[d] = xa + 0.5 * t[d];
+                    dold[d] = minimg(chain_old(R, b0, b, i, d) - rbar[d], R.L, R.iL);
+                }
+                for (int d = 0; d < 3; ++d) {
+                    double z;
+                    if (tape) {
+                        z = tape_get(R, cur + 3 * it + d, &bad);
+                    } else {
+                        const uint32_t g = g_base + 3u * (uint32_t)it + (uint32_t)d;
+                        z = rng.normal(c_norm + g / 2u, (int)(g & 1u));
+                    }
+                    dnew[d] = sigma * z;
+                    dnew[d] = minimg(dnew[d], R.L, R.iL);
+                    const double xa = chain_new(R, b0, nb, n, a, i, d);
+                    t[d] = minimg(chain_new(R, b0, nb, n, c, i, d) - xa, R.L, R.iL);
+                    rbar[d] = xa + 0.5 * t[d];
+                    nbv[d] = rbar[d] + dnew[d];
+                }
+                for (int d = 0; d < 3; ++d) R.ch_new[((size_t)i * nm + b - 1) * 3 + d] = nbv[d];
+                lspo -= free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dold);
+                lspn -= free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dnew);
+            }
+            __syncthreads();
+            const int nlinks = nb >> level;
+            const double i4lt_k = 1. / (4. * R.lambda * level_tau);
+            double ko = 0., kn = 0.;
+            for (int it = tid; it < n * nlinks; it += kPermBlock) {
+                const int i = it / nlinks, a = (it % nlinks) * skip, b = a + skip;
+                const bool ring = nb == M;
+                double dd[3];
+                if (!ring || R.ch_lbl2[i] != R.ch_lbl[i]) {
+                    for (int d = 0; d < 3; ++d)
+                        dd[d] = minimg(chain_old(R, b0, a, i, d) - chain_old(R, b0, b, i, d), R.L, R.iL);
+                    ko += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+                }
+                if (!ring || R.ch_lbl2[(i + 1) % n] != R.ch_lbl[i]) {
+                    for (int d = 0; d < 3; ++d)
+                        dd[d] = minimg(chain_new(R, b0, nb, n, a, i, d) - chain_new(R, b0, nb, n, b, i, d), R.L, R.iL);
+                    kn += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
+                }
+            }
+            lspo = block_sum<kPermBlock>(lspo, red);
+            lspn = block_sum<kPermBlock>(lspn, red);
+            ko = block_sum<kPermBlock>(ko, red);
+            kn = block_sum<kPermBlock>(kn, red);
+            bad = __syncthreads_or(bad);  // a tape overrun in any thread's draws
+            cur += 3 * n_open * nmid;
+            g_base += 3u * (uint32_t)(n_open * nmid);
+            if (tid == 0) {
+                const double u = tape ? tape_get(R, cur, &bad) : rng.unif(c_lev + (uint32_t)level, 0);
+                double change = 0.;
+                s_i[6] = perm_level_step(R, pr, level, lspo, lspn, ko, kn, prev, u, bad, b0, nb, n, n_lev, cur, &s_i[7],
+                                         &change);
+                s_w[1] = change;
+            }
+            cur += 1;
+            __syncthreads();
+            if (s_i[6]) {
+                for (int i = tid; i < n; i += kPermBlock) {
+                    R.mv_pre[R.ch_lbl[i]] = -1;
+                    R.mv_post[R.ch_lbl2[i]] = -1;
+                }
+                if (tid < kWave) rho_items_write(R, 0, 0, 0, 0, tid);
+                return;
+            }
+            prev = s_w[1];
+        }
     }
     // Level 0 reached.  A one-chain proposal (a 1-cycle: plain bisection of p0,
     // whose label may change at the wrap) goes to the kernels of the Boltzmann
