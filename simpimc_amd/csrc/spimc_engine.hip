@@ -369,6 +369,7 @@ struct spimc_ctx {
     void (*adj_lds_fn)(const Rep*, unsigned int, int) = nullptr;
     int groups_wanted = 1;
     bool fork = false;  // rho_k beside the pair kernel on a second stream per group
+    int perm_stage = kPermStage;  // largest permuting proposal (chain beads) bisected on LDS
     // k_rho variant: k-vectors per thread and non-temporal rho_k traffic
     int rho_kpt = kRhoKpt;
     bool rho_nt = true;  // measured: slightly faster, and keeps the pair tables in L2
@@ -1083,7 +1084,9 @@ int launch_perm_round(spimc_ctx* c, const Group& G) {
     int rc;
     const bool tuned = c->lds_doubles_max > 0;
     const size_t shmem = (size_t)c->max_N * (sizeof(double) + sizeof(int));
-    if ((rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_perm, R, kPermBlock, shmem, st, reps, R, (int)tuned))) return rc;
+    if ((rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_perm, R, kPermBlock, shmem, st, reps, R, (int)tuned,
+                     c->perm_stage)))
+        return rc;
     const bool fork = c->any_long && G.side;
     hipStream_t rst = fork ? G.side : st;
     if (fork) {
@@ -1548,6 +1551,7 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
     if (const char* f = std::getenv("SPIMC_STREAMS")) c->groups_wanted = std::max(1, std::atoi(f));
     c->fork = n_replicas >= 64;
     if (const char* f = std::getenv("SPIMC_FORK")) c->fork = std::atoi(f) != 0;
+    if (const char* f = std::getenv("SPIMC_PERM_STAGE")) c->perm_stage = std::max(0, std::min(kPermStage, std::atoi(f)));
     c->rho_bpc = c->fork ? 2 : c->groups_wanted > 1 ? 3 : 0;
     if (const char* f = std::getenv("SPIMC_RHO_BPC")) c->rho_bpc = std::max(0, std::atoi(f));
     {

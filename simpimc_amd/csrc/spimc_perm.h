@@ -112,7 +112,10 @@ __device__ __forceinline__ int perm_level_step(const Rep& R, Prop* pr, int level
 // ---------------------------------------------------------------------------
 // tuned: the round also runs the LDS pair kernel, which takes a one-chain
 // proposal whose label does not change inside the window
-__global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restrict__ reps, int n_reps, int tuned) {
+// stage_max: the largest proposal, in chain beads, bisected on LDS (at most
+// kPermStage)
+__global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restrict__ reps, int n_reps, int tuned,
+                                                             int stage_max) {
     const int r = blockIdx.x;
     if (r >= n_reps) return;
     const Rep& R = reps[r];
@@ -325,7 +328,7 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
     // first wavefront on its beads staged in LDS, [chain][t = 0..nb][d]; a
     // larger one by the whole workgroup on global memory.  Both do the same
     // arithmetic in the same order per bead, with the draws in the same order.
-    const bool fast = n * (nb + 1) <= kPermStage;
+    const bool fast = n * (nb + 1) <= min(stage_max, kPermStage);
     __shared__ double st_old[kPermStage * 3], st_new[kPermStage * 3];
     const int per = (nb + 1) * 3;
     if (fast) {

@@ -48,8 +48,17 @@ SYSTEMS = {
 }
 
 
-@pytest.mark.parametrize("name", sorted(SYSTEMS))
-def test_replay_with_permutations_bit_exact(name):
+REPLAY = [(name, "lds") for name in sorted(SYSTEMS)] + [("bare_dense", "global"), ("david_small", "global")]
+
+
+# "lds": proposals of a few chains are bisected by one wavefront on LDS-staged
+# beads; "global": SPIMC_PERM_STAGE=0 sends every proposal to the workgroup-wide
+# form on global memory, which otherwise serves only proposals of more than 256
+# chain beads
+@pytest.mark.parametrize("name,form", REPLAY)
+def test_replay_with_permutations_bit_exact(name, form, monkeypatch):
+    if form == "global":
+        monkeypatch.setenv("SPIMC_PERM_STAGE", "0")
     s = SYSTEMS[name]()
     pos = system.random_path(s, seed=3, spread=0.3)
     o = Oracle(s, seed=11)
