@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--perm", action="store_true",
                     help="time the permuting bisection (PermBisectIterative) on the same workload with Bose "
                          "statistics instead of the headline sweep; prints its own JSON line")
+    ap.add_argument("--uneven", action="store_true",
+                    help="time sweeps of the 64-replica uneven batch (N 16-128, M 16-64, r_s 1-4, each with its own "
+                         "box and k-set), eager and replayed from captured hipGraphs, and print that line instead")
     ap.add_argument("--copy-mib", type=int, default=4096,
                     help="size of the device-to-device copy timed first (the practical HBM ceiling for a "
                          "read-one-write-one stream, reported as hbm_copy_GBps; 0 skips it)")
@@ -128,6 +131,8 @@ def main():
 
     if a.perm:
         return perm_bench(a, abi, system)
+    if a.uneven:
+        return uneven_bench(a, abi, system)
     copy_gbps = copy_ceiling(a.copy_mib) if a.copy_mib else None
     base = system.config_c2()
     R = a.replicas
@@ -299,6 +304,48 @@ def perm_bench(a, abi, system):
                                f"of {R} replicas", "replicas": R},
         "kernels": kernels,
         "replica0": {"attempts": att, "accepts": acc, "perm_attempt": pa[:8].tolist(), "perm_accept": pc[:8].tolist()},
+    }))
+
+
+def uneven_bench(a, abi, system):
+    """Sweeps of the uneven batch: 64 replicas of N in {16, 32, 64, 128},
+    M in {16, 32, 64} and r_s in [1, 4], each with the C1 interaction on its
+    own box and k-set, in one context after the autotuner.  A sweep is every
+    replica's own N M / 2^l bisect and N displace attempts, the adjoint and the
+    estimators.  Reported: bead updates/s of eager sweeps and of the same sweeps
+    replayed from captured hipGraphs."""
+    systems = system.config_c4()
+    R = len(systems)
+    eng = abi.Engine(systems, seed=2028)
+    for r, s in enumerate(systems):
+        eng.upload(system.random_path(s, seed=4000 + r, spread=0.05), r=r)
+    shape = eng.autotune(32) if a.autotune else None
+    kc = min(0.95 * float(system.kmag(s.L, s.k_n).max()) for s in systems)
+    eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, sk_k_cut=kc, gr_n_r=200)
+    flags = abi.SWEEP_ADJOINT | abi.SWEEP_ENERGY | abi.SWEEP_SK | abi.SWEEP_GR
+    lines = {}
+    for mode, f in (("eager", flags), ("graph", flags | abi.SWEEP_GRAPH)):
+        for i in range(a.warmup):
+            eng.sweep(1, f)  # the first graph sweep captures the graphs
+        if mode == "graph" and a.warmup == 0:
+            eng.sweep(1, f)
+        u0 = eng.bead_updates()
+        t0 = time.perf_counter()
+        for i in range(a.steps):
+            eng.sweep(1, f)
+        dt = time.perf_counter() - t0
+        lines[mode] = {"value": (eng.bead_updates() - u0) / dt, "ms_per_step": 1e3 * dt / a.steps}
+        print(f"{mode}: {lines[mode]['value']:.4g} bead updates/s", file=sys.stderr, flush=True)
+    eng.close()
+    print(json.dumps({
+        "metric": "bead updates/s, uneven batch of 64 replicas FP64",
+        "value": lines["eager"]["value"], "unit": "bead updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": lines["eager"]["ms_per_step"], "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "64 replicas, N in {16,32,64,128}, M in {16,32,64}, r_s in [1,4], Bare Ewald pair "
+                               "action on each replica's own box and k-set, bisect level 3 + displace, adjoint and "
+                               "estimators every sweep", "replicas": R,
+                   "beads": int(sum(s.n_part * s.n_bead for s in systems)), "launch_shape": shape},
+        "graph_replay": lines["graph"],
     }))
 
 
