@@ -291,6 +291,25 @@ int spimc_displace_stage(spimc_ctx* ctx, const uint32_t* n_attempts, uint32_t n_
  * out[0..3] = bisect attempts, bisect accepts, displace attempts, displace accepts. */
 int spimc_move_counters(spimc_ctx* ctx, uint32_t replica, uint64_t* out);
 
+/* Action trace of the move stages: with it on, every attempt of a Bisect,
+ * DisplaceParticle or PermBisectIterative stage records the terms its level-0
+ * Metropolis decision adds up -- the values the reference's Attempt takes from
+ * Action::GetAction in OLD and NEW mode at level 0 (bisect_class.h:100-114,
+ * displace_particle_class.h:52-68, perm_bisect_iterative_class.h:194-213):
+ *   out[i][0..7] = kinetic OLD, kinetic NEW, short-range pair OLD, NEW,
+ *                  long-range (Ewald k-space) pair OLD, NEW,
+ *                  log sample ratio at level 0, action change of level 1
+ * for attempt i of the replica's last stage (NaN for an attempt that made no
+ * level-0 decision: rejected at a coarser level, or its cycle walk stopped).
+ * Short-range pair terms sum PairAction::CalcU over the window links and
+ * partners (pair_action_class.h:279-290); long-range terms are CalcULong
+ * (bare_pair_action_class.h:153-172, david_pair_action_class.h:105-123) on the
+ * updated rho_k.  Enabling synchronises the device.  Read n_attempts <= the
+ * replica's attempts in the last stage.  Diagnostic; no counterpart in the
+ * reference, which keeps these values in locals. */
+int spimc_action_trace_enable(spimc_ctx* ctx, int enable);
+int spimc_action_trace_read(spimc_ctx* ctx, uint32_t replica, uint32_t n_attempts, double* out);
+
 /* ---- Permutations (events/moves/single_species_move/bisect/perm_bisect/) --
  * A replica's path topology is the permutation sigma at the wrap: the bead after
  * (p, n_bead - 1) is (sigma[p], 0), where PermBisect::AssignParticleLabels

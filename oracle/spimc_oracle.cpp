@@ -632,9 +632,12 @@ struct Oracle {
                 tot += calc_u(rm, rpm, rrpm, level);
             }
         }
+        last_pair_short = tot;
+        last_pair_long = 0.;
         if (use_long_range) {
             if (need_update_rho_k && mode == SPIMC_MODE_NEW) update_rho_k(b0, b1, parts, level);
-            tot += calc_u_long(b0, b1, level);
+            last_pair_long = calc_u_long(b0, b1, level);
+            tot += last_pair_long;
         }
         return tot;
     }
@@ -916,6 +919,38 @@ struct Oracle {
         mode = saved;
     }
 
+    // ---- action trace: the level-0 decision's terms per attempt --------------
+    // kinetic OLD, NEW; short-range pair OLD, NEW; long-range pair OLD, NEW;
+    // log sample ratio; previous level's change; the long-range action of the
+    // window's slices where no bead moves (in both OLD and NEW) (NaN rows: no
+    // level-0 decision)
+    static constexpr int kTraceWords = 9;
+    bool tracing = false;
+    std::vector<double> trace;
+    double trace_row[kTraceWords];
+    bool trace_row_set = false;
+    double last_pair_short = 0., last_pair_long = 0.;
+    void trace_begin() { trace_row_set = false; }
+    void trace_end() {
+        if (!tracing) return;
+        for (int i = 0; i < kTraceWords; ++i) trace.push_back(trace_row_set ? trace_row[i] : std::nan(""));
+    }
+    // CalcULong at level 0 over the slices of [b0, b1) whose beads all keep their
+    // OLD positions
+    double unmoved_long(uint32_t b0, uint32_t b1) {
+        if (!use_long_range || pair_type == SPIMC_PAIR_NONE || N == 1 || lambda == 0.) return 0.;
+        const int saved = mode;
+        mode = SPIMC_MODE_OLD;
+        double tot = 0.;
+        for (uint32_t b = b0; b < b1; ++b) {
+            bool same = true;
+            for (uint32_t p = 0; p < N && same; ++p) same = std::memcmp(Rm(0, p, b), Rm(1, p, b), 3 * sizeof(double)) == 0;
+            if (same) tot += calc_u_long(b, b + 1, 0);
+        }
+        mode = saved;
+        return tot;
+    }
+
     // ---- draws (post-transform values, reference call order) ---------------
     bool recording = false;
     std::vector<double> rec;
@@ -1008,14 +1043,25 @@ struct Oracle {
             double old_action = 0., new_action = 0.;
             // action list order: Kinetic, pair action
             mode = SPIMC_MODE_OLD;
-            old_action += kinetic_get_action(bead0, bead1, parts, level);
+            const double k_old = kinetic_get_action(bead0, bead1, parts, level);
+            old_action += k_old;
             mode = SPIMC_MODE_NEW;
-            new_action += kinetic_get_action(bead0, bead1, parts, level);
+            const double k_new = kinetic_get_action(bead0, bead1, parts, level);
+            new_action += k_new;
             mode = SPIMC_MODE_OLD;
+            last_pair_short = last_pair_long = 0.;
             old_action += pair_get_action(bead0, bead1, parts, level);
+            const double ps_old = last_pair_short, pl_old = last_pair_long;
             mode = SPIMC_MODE_NEW;
+            last_pair_short = last_pair_long = 0.;
             new_action += pair_get_action(bead0, bead1, parts, level);
             double log_sample_ratio = -new_lsp + old_lsp;
+            if (level == 0 && tracing) {
+                const double row[kTraceWords] = {k_old, k_new, ps_old, last_pair_short, pl_old, last_pair_long,
+                                                 log_sample_ratio, prev_action_change, unmoved_long(bead0, bead1)};
+                std::memcpy(trace_row, row, sizeof row);
+                trace_row_set = true;
+            }
             double current_action_change = new_action - old_action;
             double log_accept = log_sample_ratio - current_action_change + prev_action_change;
             if (log_accept < std::log(draw())) return false;
@@ -1229,13 +1275,24 @@ struct Oracle {
         }
         double old_action = 0., new_action = 0.;
         mode = SPIMC_MODE_OLD;
-        old_action += kinetic_get_action(0, M, parts, 0);
+        const double k_old = kinetic_get_action(0, M, parts, 0);
+        old_action += k_old;
         mode = SPIMC_MODE_NEW;
-        new_action += kinetic_get_action(0, M, parts, 0);
+        const double k_new = kinetic_get_action(0, M, parts, 0);
+        new_action += k_new;
         mode = SPIMC_MODE_OLD;
+        last_pair_short = last_pair_long = 0.;
         old_action += pair_get_action(0, M, parts, 0);
+        const double ps_old = last_pair_short, pl_old = last_pair_long;
         mode = SPIMC_MODE_NEW;
+        last_pair_short = last_pair_long = 0.;
         new_action += pair_get_action(0, M, parts, 0);
+        if (tracing) {
+            const double row[kTraceWords] = {k_old, k_new, ps_old, last_pair_short, pl_old, last_pair_long, 0., 0.,
+                                             unmoved_long(0, M)};
+            std::memcpy(trace_row, row, sizeof row);
+            trace_row_set = true;
+        }
         double log_accept = old_action - new_action;
         return !(log_accept < std::log(draw()));
     }
@@ -1702,6 +1759,22 @@ int spimc_oracle_tape_set(void* h, const double* draws, uint64_t n) {
     return SPIMC_OK;
 }
 
+// Action trace of the move stages: 9 doubles per attempt (see Oracle::trace).
+// Enabling clears it; read copies up to cap attempts and returns their count.
+int spimc_oracle_trace(void* h, int enable) {
+    Oracle* o = (Oracle*)h;
+    o->tracing = enable != 0;
+    o->trace.clear();
+    return SPIMC_OK;
+}
+int spimc_oracle_trace_read(void* h, double* out, uint64_t cap, uint64_t* n) {
+    Oracle* o = (Oracle*)h;
+    const uint64_t w = Oracle::kTraceWords, have = o->trace.size() / w, k = have < cap ? have : cap;
+    if (k) std::memcpy(out, o->trace.data(), k * w * sizeof(double));
+    *n = have;
+    return SPIMC_OK;
+}
+
 // Records every consumed draw (post-transform) while enabled; read with spimc_oracle_recorded.
 int spimc_oracle_record(void* h, int enable) {
     Oracle* o = (Oracle*)h;
@@ -1728,7 +1801,9 @@ int spimc_oracle_bisect_stage(void* h, uint32_t n_attempts, uint8_t* accepted) {
     for (uint32_t i = 0; i < n_attempts; ++i) {
         uint32_t p, b0;
         o->counters[0]++;
+        o->trace_begin();
         bool ok = o->bisect_attempt(p, b0);
+        o->trace_end();
         if (ok) {
             o->counters[1]++;
             o->bisect_accept(p, b0);
@@ -1748,7 +1823,9 @@ int spimc_oracle_displace_stage(void* h, uint32_t n_attempts, uint8_t* accepted)
     for (uint32_t i = 0; i < n_attempts; ++i) {
         uint32_t p;
         o->counters[2]++;
+        o->trace_begin();
         bool ok = o->displace_attempt(p);
+        o->trace_end();
         if (ok) {
             o->counters[3]++;
             o->displace_accept(p);
@@ -1772,7 +1849,9 @@ int spimc_oracle_perm_bisect_stage(void* h, uint32_t n_attempts, uint8_t* accept
     for (uint32_t i = 0; i < n_attempts; ++i) {
         uint32_t n = 0;
         o->perm_counters[0]++;
+        o->trace_begin();
         bool ok = o->perm_bisect_attempt(n);
+        o->trace_end();
         if (ok) {
             o->perm_counters[1]++;
             o->perm_attempt[n - 1]++;

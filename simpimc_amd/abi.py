@@ -25,6 +25,7 @@ OBS_ENERGY, OBS_SK, OBS_GR = 1, 2, 4
 SWEEP_GRAPH, SWEEP_ADJOINT, SWEEP_ENERGY, SWEEP_SK, SWEEP_GR = 1, 2, 4, 8, 16
 KERNEL_PROPOSE, KERNEL_PAIR, KERNEL_RHO, KERNEL_DECIDE, KERNEL_ADJOINT, KERNEL_ESTIMATORS = range(6)
 TIMING_CLOCK, TIMING_EVENTS = 1, 2
+TRACE_WORDS = 8  # action trace record of one move attempt
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -122,6 +123,8 @@ SIGNATURES = {
     "spimc_bisect_stage": ([_c, _pu32, _u32, _pu8], _i32),
     "spimc_displace_stage": ([_c, _pu32, _u32, _pu8], _i32),
     "spimc_move_counters": ([_c, _u32, _pu64], _i32),
+    "spimc_action_trace_enable": ([_c, _i32], _i32),
+    "spimc_action_trace_read": ([_c, _u32, _u32, _dp], _i32),
     "spimc_perm_configure": ([_c, _u32, _u32, _d], _i32),
     "spimc_perm_bisect_stage": ([_c, _pu32, _u32, _pu8], _i32),
     "spimc_perm_download": ([_c, _u32, _pu32], _i32),
@@ -307,6 +310,18 @@ class Engine:
     def counters(self, r: int = 0) -> np.ndarray:
         out = np.zeros(4, dtype=np.uint64)
         check(self.lib.spimc_move_counters(self.h, r, out.ctypes.data_as(_pu64)))
+        return out
+
+    def trace(self, enable: bool = True):
+        """Records each move attempt's level-0 action terms from now on."""
+        check(self.lib.spimc_action_trace_enable(self.h, int(enable)))
+
+    def trace_read(self, n: int, r: int = 0) -> np.ndarray:
+        """[n][8]: kinetic OLD, NEW; short-range pair OLD, NEW; long-range pair
+        OLD, NEW; log sample ratio; previous level's change (NaN rows: no
+        level-0 decision) of the last stage's first n attempts."""
+        out = np.zeros((n, TRACE_WORDS), dtype=np.float64)
+        check(self.lib.spimc_action_trace_read(self.h, r, n, dptr(out)))
         return out
 
     # ---- permutations ----------------------------------------------------------

@@ -17,6 +17,9 @@ namespace spimc {
 constexpr int kMaxLevel = 6;      // bisection levels supported (window of 64 links)
 constexpr int kWave = 64;         // CDNA wavefront
 constexpr int kMaxKinSplines = 8; // kinetic / bisect image splines at tau * 2^l
+// action trace record of one attempt: kinetic OLD, NEW; short-range pair OLD,
+// NEW; long-range pair OLD, NEW; log sample ratio; previous level's change
+constexpr int kTraceWords = 8;
 
 // Natural cubic spline: s(t) = c0 + dt (c1 + dt (c2 + dt c3)), dt = t - x[i].
 struct Spl {
@@ -161,6 +164,9 @@ struct Rep {
     unsigned char* accepted; // [accepted_cap]
     unsigned int accepted_cap;
     unsigned int pad3;
+    // action trace of the move stages (nullptr: off): per attempt of the running
+    // stage, kTraceWords doubles of what the level-0 Metropolis decision adds up
+    double* act_trace;       // [accepted_cap][kTraceWords]
     // replay tape
     const double* tape;
     long long tape_len;

@@ -230,6 +230,8 @@ struct RepHost {
     double* tape = nullptr;
     unsigned long long* gr_cnt = nullptr;
     uint32_t accepted_cap = 0;
+    double* trace = nullptr;        // action trace, [accepted_cap][kTraceWords] (when on)
+    uint32_t stage_attempts = 0;    // attempts of this replica in the last move stage
     int n_chunks = 1;
     char* perm_arena = nullptr;    // perm, iperm
     char* perm_scratch = nullptr;  // the permuting move's chain arrays and counters
@@ -398,6 +400,8 @@ struct spimc_ctx {
     bool perm_on = false;
     // a round's decision runs in the next round's proposal launch (SPIMC_FUSE_ROUNDS=0: its own launch)
     bool fuse_rounds = true;
+    // the move stages record each attempt's level-0 action terms (spimc_action_trace_enable)
+    bool trace_on = false;
 };
 
 namespace {
@@ -1203,17 +1207,32 @@ int launch_final_decide(spimc_ctx* c, const Group& G) {
 
 int ensure_accepted(spimc_ctx* c, uint32_t need) {
     bool changed = false;
-    for (auto& h : c->reps)
+    for (auto& h : c->reps) {
         if (h.accepted_cap < need) {
             if (h.accepted) (void)hipFree(h.accepted);
+            if (h.trace) (void)hipFree(h.trace);
             h.accepted = nullptr;
+            h.trace = nullptr;
             uint32_t cap = std::max<uint32_t>(need, 1024);
             HIPCHK(hipMalloc((void**)&h.accepted, cap));
             h.accepted_cap = cap;
             h.dev.accepted = h.accepted;
             h.dev.accepted_cap = cap;
+            h.dev.act_trace = nullptr;
             changed = true;
         }
+        if (c->trace_on && !h.trace) {
+            HIPCHK(hipMalloc((void**)&h.trace, (size_t)h.accepted_cap * kTraceWords * sizeof(double)));
+            h.dev.act_trace = h.trace;
+            changed = true;
+        } else if (c->trace_on && h.dev.act_trace != h.trace) {
+            h.dev.act_trace = h.trace;
+            changed = true;
+        } else if (!c->trace_on && h.dev.act_trace) {
+            h.dev.act_trace = nullptr;
+            changed = true;
+        }
+    }
     if (changed) {
         int rc = upload_reps(c);
         if (rc) return rc;
@@ -1364,6 +1383,7 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
             }
         }
     }
+    for (int r = 0; r < R; ++r) c->reps[r].stage_attempts = cnt[r];
     if (accepted)
         for (int r = 0; r < R; ++r)
             HIPCHK(hipMemcpy(accepted + (size_t)r * mx, c->reps[r].accepted, cnt[r], hipMemcpyDeviceToHost));
@@ -1697,6 +1717,7 @@ int spimc_ctx_destroy(spimc_ctx* c) {
     for (auto& h : c->reps) {
         if (h.arena) (void)hipFree(h.arena);
         if (h.accepted) (void)hipFree(h.accepted);
+        if (h.trace) (void)hipFree(h.trace);
         if (h.tape) (void)hipFree(h.tape);
         if (h.gr_cnt) (void)hipFree(h.gr_cnt);
         if (h.perm_arena) (void)hipFree(h.perm_arena);
@@ -2137,6 +2158,31 @@ int spimc_move_counters(spimc_ctx* c, uint32_t r, uint64_t* out) {
     out[1] = k.bis_acc;
     out[2] = k.dis_att;
     out[3] = k.dis_acc;
+    return SPIMC_OK;
+}
+
+int spimc_action_trace_enable(spimc_ctx* c, int enable) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    if (c->prop_set) return fail(SPIMC_ERR_STATE, "an interface proposal is pending; accept or reject it first");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->trace_on = enable != 0;
+    uint32_t cap = 0;
+    for (auto& h : c->reps) cap = std::max(cap, h.accepted_cap);
+    int rc = ensure_accepted(c, std::max<uint32_t>(cap, 1));
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    return SPIMC_OK;
+}
+
+int spimc_action_trace_read(spimc_ctx* c, uint32_t r, uint32_t n_attempts, double* out) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!out && n_attempts) return fail(SPIMC_ERR_ARG, "null output");
+    const RepHost& h = c->reps[r];
+    if (!c->trace_on || !h.trace) return fail(SPIMC_ERR_STATE, "the action trace is off");
+    if (n_attempts > h.stage_attempts) return fail(SPIMC_ERR_ARG, "more attempts than the last stage made");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, h.trace, (size_t)n_attempts * kTraceWords * sizeof(double), hipMemcpyDeviceToHost));
     return SPIMC_OK;
 }
 

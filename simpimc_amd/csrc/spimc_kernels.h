@@ -870,6 +870,20 @@ __device__ __forceinline__ void decide_body(const Rep* __restrict__ reps, int n_
         }
         unsigned int si = c->stage_i;
         if (si < R.accepted_cap) R.accepted[si] = (unsigned char)acc;
+        if (R.act_trace && si < R.accepted_cap) {
+            // the terms of the level-0 decision as summed above (NaN: no level-0 decision)
+            double* t = R.act_trace + (size_t)si * kTraceWords;
+            const bool l0 = pr.status == 1;
+            const double nan = __builtin_nan("");
+            t[0] = l0 ? pr.k_old : nan;
+            t[1] = l0 ? pr.k_new : nan;
+            t[2] = l0 ? po : nan;
+            t[3] = l0 ? pn : nan;
+            t[4] = l0 ? R.long_scale * lo : nan;
+            t[5] = l0 ? R.long_scale * ln : nan;
+            t[6] = l0 ? pr.lsr : nan;
+            t[7] = l0 ? pr.prev : nan;
+        }
         c->stage_i = si + 1;
         if (R.tape) *R.tape_cursor = pr.tape_next;
         if (pr.status == 3) {  // replay tape overrun / bad draw

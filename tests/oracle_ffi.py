@@ -44,6 +44,8 @@ SIG = {
     "spimc_oracle_tape_cursor": ([_c, _pu64], _i32),
     "spimc_oracle_record": ([_c, _i32], _i32),
     "spimc_oracle_recorded": ([_c, _dp, _u64, _pu64], _i32),
+    "spimc_oracle_trace": ([_c, _i32], _i32),
+    "spimc_oracle_trace_read": ([_c, _dp, _u64, _pu64], _i32),
     "spimc_oracle_bisect_stage": ([_c, _u32, _pu8], _i32),
     "spimc_oracle_displace_stage": ([_c, _u32, _pu8], _i32),
     "spimc_oracle_move_counters": ([_c, _pu64], _i32),
@@ -239,6 +241,17 @@ class Oracle:
         check(self.L.spimc_oracle_recorded(self.h, None, 0, C.byref(n)))
         out = np.zeros(n.value)
         check(self.L.spimc_oracle_recorded(self.h, dp(out), n.value, C.byref(n)))
+        return out
+
+    def trace(self, enable=True):
+        """Records each move attempt's level-0 action terms (cleared here)."""
+        check(self.L.spimc_oracle_trace(self.h, int(enable)))
+
+    def trace_read(self):
+        n = C.c_uint64()
+        check(self.L.spimc_oracle_trace_read(self.h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 9))
+        check(self.L.spimc_oracle_trace_read(self.h, dp(out), n.value, C.byref(n)))
         return out
 
     def bisect(self, n):
