@@ -242,6 +242,22 @@ int spimc_reject_proposal(spimc_ctx* ctx, uint32_t replica);
 int spimc_get_action(spimc_ctx* ctx, uint32_t replica, int action, int mode, uint32_t b0, uint32_t b1,
                      const uint32_t* particles, uint32_t n_particles, uint32_t level, double* out);
 
+/* spimc_get_action without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as spimc_get_action, whose kernels it shares;
+ * its scratch (long-range partials, the result, work items) is a fixed device
+ * array of SPIMC_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends,
+ * instead of context buffers grown on demand.  It calls no hipMalloc or
+ * hipFree.  Limits: n_bead + 8 <= SPIMC_NOALLOC_SCRATCH_BYTES / 32 and
+ * n_bead * (k chunks) <= 3 * SPIMC_NOALLOC_SCRATCH_BYTES / 32; larger replicas
+ * get SPIMC_ERR_ARG.  Refused (SPIMC_ERR_STATE) while SPIMC_TIMING_EVENTS is
+ * on.  All contexts on one GPU share the array: calls that overlap on one GPU
+ * (from several threads or contexts) race. */
+#define SPIMC_NOALLOC_SCRATCH_BYTES (256u * 1024u)
+int spimc_get_action_noalloc(spimc_ctx* ctx, uint32_t replica, int action, int mode, uint32_t b0, uint32_t b1,
+                             const uint32_t* particles, uint32_t n_particles, uint32_t level, double* out);
+
 /* Action::DActionDBeta (action_class.h:37): kinetic_class.h:35-45,
  * pair_action_class.h:241-264, committed path. */
 int spimc_d_action_d_beta(spimc_ctx* ctx, uint32_t replica, int action, double* out);

@@ -113,6 +113,7 @@ SIGNATURES = {
     "spimc_accept_proposal": ([_c, _u32], _i32),
     "spimc_reject_proposal": ([_c, _u32], _i32),
     "spimc_get_action": ([_c, _u32, _i32, _i32, _u32, _u32, _pu32, _u32, _u32, _dp], _i32),
+    "spimc_get_action_noalloc": ([_c, _u32, _i32, _i32, _u32, _u32, _pu32, _u32, _u32, _dp], _i32),
     "spimc_d_action_d_beta": ([_c, _u32, _i32, _dp], _i32),
     "spimc_potential": ([_c, _u32, _i32, _dp], _i32),
     "spimc_kinetic_virial_energy": ([_c, _u32, _u32, _dp], _i32),

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -19,7 +20,21 @@
 
 using namespace spimc;
 
+// Device scratch of spimc_get_action_noalloc (long-range partials, the result
+// and the work items of one query), so that the call allocates nothing.  Every
+// device the library runs on holds its own copy until the process ends; calls
+// on one device share it.  In host code the name is only a stand-in: the
+// device address comes from hipGetSymbolAddress (noalloc_scratch_ptr).
+constexpr size_t kNoAllocScratchBytes = SPIMC_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
+
+// Per-device address of `scratch`, looked up once per device with that device
+// current (at context creation).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
 
 thread_local std::string g_err;
 
@@ -34,6 +49,19 @@ int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                               \
             return fail(SPIMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+int noalloc_scratch_ptr(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(SPIMC_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(SPIMC_ERR_HIP, "hipGetSymbolAddress returned no address for the device scratch");
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return SPIMC_OK;
+}
 
 constexpr int kRhoKpt = 8;        // k-vectors per thread in k_rho (default; SPIMC_RHO_KPT 2, 4, 8, 16)
 constexpr int kRhoKptMin = 2;     // smallest chunking: bounds the number of k chunks
@@ -402,6 +430,9 @@ struct spimc_ctx {
     bool fuse_rounds = true;
     // the move stages record each attempt's level-0 action terms (spimc_action_trace_enable)
     bool trace_on = false;
+    // device address of this device's `scratch` array (spimc_get_action_noalloc)
+    void* noalloc_scratch = nullptr;
+    std::string noalloc_error;  // why the address lookup failed, if it did
 };
 
 namespace {
@@ -1556,6 +1587,11 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         return rc;
     };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(SPIMC_ERR_HIP, "hipGetDevice"));
+    // the device scratch of spimc_get_action_noalloc; a failed lookup disables only that entry point
+    if (noalloc_scratch_ptr(c->device, &c->noalloc_scratch) != SPIMC_OK) {
+        c->noalloc_error = g_err;
+        c->noalloc_scratch = nullptr;
+    }
     if (const char* f = std::getenv("SPIMC_RHO_KPT")) {
         const int v = std::atoi(f);
         if (v == 2 || v == 4 || v == 8 || v == 16) c->rho_kpt = v;
@@ -1836,8 +1872,20 @@ static int finish_proposal(spimc_ctx* c, uint32_t r, int accept) {
 int spimc_accept_proposal(spimc_ctx* c, uint32_t r) { return finish_proposal(c, r, 1); }
 int spimc_reject_proposal(spimc_ctx* c, uint32_t r) { return finish_proposal(c, r, 0); }
 
-int spimc_get_action(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0, uint32_t b1, const uint32_t* parts,
-                     uint32_t n_parts, uint32_t level, double* out) {
+namespace {
+
+// Device memory of one GetAction query beyond its inputs and outputs: per-slice
+// long-range partials and the result (tmp, doubles) and the pair / rho_k work
+// items (items).  spimc_get_action grows context buffers for them;
+// spimc_get_action_noalloc carves them from the device scratch array.
+struct IfaceScratch {
+    double* tmp;
+    uint2* items;
+    size_t tmp_cap, items_cap;
+};
+
+int get_action_impl(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0, uint32_t b1, const uint32_t* parts,
+                    uint32_t n_parts, uint32_t level, double* out, const IfaceScratch* fixed) {
     int rc = check_rep(c, r);
     if (rc) return rc;
     if (!out || (n_parts && !parts)) return fail(SPIMC_ERR_ARG, "null argument");
@@ -1850,8 +1898,18 @@ int spimc_get_action(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0
     if (n_parts == 0) return SPIMC_OK;
     const int p = (int)parts[0];
     const uint32_t skip = 1u << level;
-    if ((rc = ensure_tmp(c, (size_t)R.M + 8, (size_t)R.M * std::max(R.n_chunks, 1)))) return rc;
-    double* d_out = c->d_tmp + R.M;
+    const size_t need_tmp = (size_t)R.M + 8, need_items = (size_t)R.M * std::max(R.n_chunks, 1);
+    IfaceScratch sc;
+    if (fixed) {
+        sc = *fixed;
+        if (need_tmp > sc.tmp_cap || need_items > sc.items_cap)
+            return fail(SPIMC_ERR_ARG, "query too large for the fixed scratch: n_bead + 8 partials and n_bead * k chunks "
+                                       "work items must fit spimc_get_action_noalloc's scratch (SPIMC_NOALLOC_SCRATCH_BYTES)");
+    } else {
+        if ((rc = ensure_tmp(c, need_tmp, need_items))) return rc;
+        sc = IfaceScratch{c->d_tmp, c->d_items_tmp, c->tmp_cap, c->items_tmp_cap};
+    }
+    double* d_out = sc.tmp + R.M;
     if (action == SPIMC_ACTION_KINETIC) {
         if (level >= kMaxKinSplines || (R.kin_images > 0 && level >= c->reps[r].kin.size()))
             return fail(SPIMC_ERR_UNSUPPORTED, "kinetic level out of range");
@@ -1873,46 +1931,64 @@ int spimc_get_action(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0
         if (level > 0 || R.pair_type == SPIMC_PAIR_NONE || R.N == 1) return SPIMC_OK;
         const int nl = (int)(b1 - b0);
         if ((rc = write_iface_prop(c, r, p, (int)b0, nl))) return rc;
-        std::vector<uint2> items;
-        for (int j = 0; j < nl; ++j) items.push_back(make_uint2(r, j));
-        if (!items.empty()) {
-            HIPCHK(hipMemcpy(c->d_items_tmp, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
-            const int g = (nl + 3) / 4;
-            rc = launch(c, SPIMC_KERNEL_PAIR, k_pair, g, kPairBlock, 0, c->stream, c->d_reps, c->d_items_tmp, nl, 1, nullptr);
+        if (nl > 0) {
+            // work items (r, link j), written on the device
+            rc = launch(c, SPIMC_KERNEL_PAIR, k_iface_items, (nl + 255) / 256, 256, 0, c->stream, sc.items, (int)r, nl, 0);
             if (rc) return rc;
-            HIPCHK(hipStreamSynchronize(c->stream));
+            const int g = (nl + 3) / 4;
+            rc = launch(c, SPIMC_KERNEL_PAIR, k_pair, g, kPairBlock, 0, c->stream, c->d_reps, (const uint2*)sc.items, nl, 1, nullptr);
+            if (rc) return rc;
         }
         int n_long = 0;
         if (R.use_long_range) {
             const bool mine = c->prop_set && c->prop_rep == (int)r && c->prop_p == p;
             if (mode == SPIMC_MODE_NEW && mine && !c->rho_updated) {
-                std::vector<uint2> ri;
-                for (int m = 0; m < c->prop_count; ++m)
-                    for (int ch = 0; ch < R.n_chunks; ++ch) ri.push_back(make_uint2(r, (m << 8) | ch));
-                if ((rc = ensure_tmp(c, (size_t)R.M + 8, ri.size()))) return rc;
-                d_out = c->d_tmp + R.M;
-                HIPCHK(hipMemcpy(c->d_items_tmp, ri.data(), ri.size() * sizeof(uint2), hipMemcpyHostToDevice));
-                const int nri = (int)ri.size();
+                // work items (r, moved bead m << 8 | k chunk), after the pair kernel has read its items
+                const int nri = c->prop_count * R.n_chunks;
+                rc = launch(c, SPIMC_KERNEL_RHO, k_iface_items, (nri + 255) / 256, 256, 0, c->stream, sc.items, (int)r, nri,
+                            R.n_chunks);
+                if (rc) return rc;
                 const int gph = (6 * c->prop_count + 63) / 64;
                 rc = launch(c, SPIMC_KERNEL_RHO, k_phase_proposal, gph, 64, 0, c->stream, c->d_reps, (int)r);
                 if (rc) return rc;
-                rc = launch(c, SPIMC_KERNEL_RHO, c->rho_fn, nri, kRhoBlock, 0, c->stream, c->d_reps, c->d_items_tmp, nri, 1, nullptr);
+                rc = launch(c, SPIMC_KERNEL_RHO, c->rho_fn, nri, kRhoBlock, 0, c->stream, c->d_reps, (const uint2*)sc.items, nri, 1, nullptr);
                 if (rc) return rc;
                 c->rho_updated = true;
             }
             n_long = (int)((b1 - b0 + skip - 1) / skip);
             const int upd = (mine && c->rho_updated) ? 1 : 0;
             if (n_long > 0) {
-                rc = launch(c, SPIMC_KERNEL_RHO, k_long_window, n_long, 256, 0, c->stream, c->d_reps, (int)r, mode, (int)b0, upd, c->d_tmp);
+                rc = launch(c, SPIMC_KERNEL_RHO, k_long_window, n_long, 256, 0, c->stream, c->d_reps, (int)r, mode, (int)b0, upd, sc.tmp);
                 if (rc) return rc;
             }
         }
-        rc = launch(c, SPIMC_KERNEL_PAIR, k_action_sum, 1, 64, 0, c->stream, c->d_reps, (int)r, mode, nl, n_long, c->d_tmp, d_out);
+        rc = launch(c, SPIMC_KERNEL_PAIR, k_action_sum, 1, 64, 0, c->stream, c->d_reps, (int)r, mode, nl, n_long, (const double*)sc.tmp, d_out);
     }
     if (rc) return rc;
     if ((rc = sync_and_resolve(c))) return rc;
     HIPCHK(hipMemcpy(out, d_out, sizeof(double), hipMemcpyDeviceToHost));
     return SPIMC_OK;
+}
+
+}  // namespace
+
+int spimc_get_action(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0, uint32_t b1, const uint32_t* parts,
+                     uint32_t n_parts, uint32_t level, double* out) {
+    return get_action_impl(c, r, action, mode, b0, b1, parts, n_parts, level, out, nullptr);
+}
+
+int spimc_get_action_noalloc(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0, uint32_t b1,
+                             const uint32_t* parts, uint32_t n_parts, uint32_t level, double* out) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    if (c->timing) return fail(SPIMC_ERR_STATE, "per-launch timing events are on; spimc_get_action_noalloc records none");
+    if (!c->noalloc_scratch)
+        return fail(SPIMC_ERR_HIP, "device scratch of this context's device is not available: " + c->noalloc_error);
+    // the scratch array: long-range partials and result first, then the work items
+    constexpr size_t kTmpBytes = kNoAllocScratchBytes / 4;
+    IfaceScratch sc{reinterpret_cast<double*>(c->noalloc_scratch),
+                    reinterpret_cast<uint2*>(static_cast<char*>(c->noalloc_scratch) + kTmpBytes),
+                    kTmpBytes / sizeof(double), (kNoAllocScratchBytes - kTmpBytes) / sizeof(uint2)};
+    return get_action_impl(c, r, action, mode, b0, b1, parts, n_parts, level, out, &sc);
 }
 
 int spimc_d_action_d_beta(spimc_ctx* c, uint32_t r, int action, double* out) {

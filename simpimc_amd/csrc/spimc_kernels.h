@@ -1064,6 +1064,14 @@ __global__ __launch_bounds__(256) void k_long_window(const Rep* __restrict__ rep
     if (threadIdx.x == 0) part[w] = t;
 }
 
+// Work items of an interface query: (r, t) for the pair kernel's links
+// (n_chunks == 0), or (r, moved bead << 8 | k chunk) for the rho_k kernel.
+__global__ __launch_bounds__(256) void k_iface_items(uint2* __restrict__ items, int r, int n, int n_chunks) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    items[t] = make_uint2((unsigned)r, n_chunks ? (unsigned)(((t / n_chunks) << 8) | (t % n_chunks)) : (unsigned)t);
+}
+
 // Sums a pair action query: links (mode column) + long-range window partials.
 __global__ void k_action_sum(const Rep* __restrict__ reps, int r, int mode, int n_links, int n_long,
                              const double* __restrict__ long_part, double* __restrict__ out) {
