@@ -51,6 +51,11 @@ def parse():
     ap.add_argument("--uneven", action="store_true",
                     help="time sweeps of the 64-replica uneven batch (N 16-128, M 16-64, r_s 1-4, each with its own "
                          "box and k-set), eager and replayed from captured hipGraphs, and print that line instead")
+    ap.add_argument("--no-extra", dest="extra", action="store_false",
+                    help="skip the uneven-batch (64 replicas) and large-case (8 x N=1024, M=256) sub-lines of the "
+                         "default run")
+    ap.add_argument("--c5", action="store_true",
+                    help="time only the large case (8 replicas of N=1024, M=256, 4012 k-vectors) and print that line")
     ap.add_argument("--copy-mib", type=int, default=4096,
                     help="size of the device-to-device copy timed first (the practical HBM ceiling for a "
                          "read-one-write-one stream, reported as hbm_copy_GBps; 0 skips it)")
@@ -132,7 +137,11 @@ def main():
     if a.perm:
         return perm_bench(a, abi, system)
     if a.uneven:
-        return uneven_bench(a, abi, system)
+        print(json.dumps(uneven_bench(a, abi, system)))
+        return
+    if a.c5:
+        print(json.dumps(c5_bench(a, abi, system, None)))
+        return
     copy_gbps = copy_ceiling(a.copy_mib) if a.copy_mib else None
     base = system.config_c2()
     R = a.replicas
@@ -229,6 +238,10 @@ def main():
         print(f"plain single chain: {dup / dtp:.4g} bead updates/s", file=sys.stderr, flush=True)
         pe.close()
 
+    extra_uneven = extra_c5 = None
+    if a.extra:
+        extra_uneven = uneven_bench(a, abi, system, steps=5)
+        extra_c5 = c5_bench(a, abi, system, copy_gbps, steps=2)
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     out = {
         "metric": METRIC,
@@ -260,6 +273,8 @@ def main():
         "hbm_copy_GBps": copy_gbps,
         "kernels": kernels,
         "acceptance_replica0": {"bisect": float(c0[1]) / max(1, c0[0]), "displace": float(c0[3]) / max(1, c0[2])},
+        "uneven": extra_uneven,
+        "c5": extra_c5,
     }
     print(json.dumps(out))
 
@@ -307,46 +322,102 @@ def perm_bench(a, abi, system):
     }))
 
 
-def uneven_bench(a, abi, system):
+def uneven_bench(a, abi, system, steps=None):
     """Sweeps of the uneven batch: 64 replicas of N in {16, 32, 64, 128},
     M in {16, 32, 64} and r_s in [1, 4], each with the C1 interaction on its
-    own box and k-set, in one context after the autotuner.  A sweep is every
-    replica's own N M / 2^l bisect and N displace attempts, the adjoint and the
-    estimators.  Reported: bead updates/s of eager sweeps and of the same sweeps
-    replayed from captured hipGraphs."""
+    own box and k-set, in one context.  A sweep is every replica's own
+    N M / 2^l bisect and N displace attempts, the adjoint and the estimators.
+    Two launch designs are timed: the autotuned rounds of launches, and the
+    persistent stage kernel (one launch per stage, a workgroup per replica);
+    `value` is the faster one's eager rate, and each is also replayed from
+    captured hipGraphs."""
+    steps = steps or a.steps
     systems = system.config_c4()
     R = len(systems)
-    eng = abi.Engine(systems, seed=2028)
-    for r, s in enumerate(systems):
-        eng.upload(system.random_path(s, seed=4000 + r, spread=0.05), r=r)
-    shape = eng.autotune(32) if a.autotune else None
     kc = min(0.95 * float(system.kmag(s.L, s.k_n).max()) for s in systems)
-    eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, sk_k_cut=kc, gr_n_r=200)
     flags = abi.SWEEP_ADJOINT | abi.SWEEP_ENERGY | abi.SWEEP_SK | abi.SWEEP_GR
-    lines = {}
-    for mode, f in (("eager", flags), ("graph", flags | abi.SWEEP_GRAPH)):
-        for i in range(a.warmup):
-            eng.sweep(1, f)  # the first graph sweep captures the graphs
-        if mode == "graph" and a.warmup == 0:
-            eng.sweep(1, f)
-        u0 = eng.bead_updates()
-        t0 = time.perf_counter()
-        for i in range(a.steps):
-            eng.sweep(1, f)
-        dt = time.perf_counter() - t0
-        lines[mode] = {"value": (eng.bead_updates() - u0) / dt, "ms_per_step": 1e3 * dt / a.steps}
-        print(f"{mode}: {lines[mode]['value']:.4g} bead updates/s", file=sys.stderr, flush=True)
-    eng.close()
-    print(json.dumps({
+    designs = {}
+    for design in ("rounds", "persistent"):
+        os.environ["SPIMC_PERSISTENT"] = "512" if design == "persistent" else "0"
+        eng = abi.Engine(systems, seed=2028)
+        for r, s in enumerate(systems):
+            eng.upload(system.random_path(s, seed=4000 + r, spread=0.05), r=r)
+        shape = eng.autotune(32) if (a.autotune and design == "rounds") else None
+        eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, sk_k_cut=kc, gr_n_r=200)
+        lines = {}
+        for mode, f in (("eager", flags), ("graph", flags | abi.SWEEP_GRAPH)):
+            for i in range(max(1, a.warmup)):
+                eng.sweep(1, f)  # the first graph sweep captures the graphs
+            u0 = eng.bead_updates()
+            t0 = time.perf_counter()
+            for i in range(steps):
+                eng.sweep(1, f)
+            dt = time.perf_counter() - t0
+            lines[mode] = {"value": (eng.bead_updates() - u0) / dt, "ms_per_step": 1e3 * dt / steps}
+            print(f"uneven {design} {mode}: {lines[mode]['value']:.4g} bead updates/s", file=sys.stderr, flush=True)
+        eng.close()
+        designs[design] = {"eager": lines["eager"], "graph_replay": lines["graph"], "launch_shape": shape}
+    os.environ.pop("SPIMC_PERSISTENT", None)
+    best = max(designs, key=lambda d: designs[d]["eager"]["value"])
+    return {
         "metric": "bead updates/s, uneven batch of 64 replicas FP64",
-        "value": lines["eager"]["value"], "unit": "bead updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": lines["eager"]["ms_per_step"], "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "value": designs[best]["eager"]["value"], "unit": "bead updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": a.warmup, "ms_per_step": designs[best]["eager"]["ms_per_step"], "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic",
         "config": {"workload": "64 replicas, N in {16,32,64,128}, M in {16,32,64}, r_s in [1,4], Bare Ewald pair "
                                "action on each replica's own box and k-set, bisect level 3 + displace, adjoint and "
                                "estimators every sweep", "replicas": R,
-                   "beads": int(sum(s.n_part * s.n_bead for s in systems)), "launch_shape": shape},
-        "graph_replay": lines["graph"],
-    }))
+                   "beads": int(sum(s.n_part * s.n_bead for s in systems)), "design": best},
+        "designs": designs,
+    }
+
+
+def c5_bench(a, abi, system, copy_gbps, steps=None):
+    """The combined large case: 8 replicas of HEG N=1024, M=256, r_s=1, 4012
+    stored k-vectors, David n_order=2, batched with the adjoint, the virial
+    energy from it, S(k) on all k-vectors and g(r) on 200 bins every sweep.
+    Reported: bead updates/s and the rho_k kernel's algorithmic GB/s from the
+    device clocks (32 B per (slice, k) update)."""
+    steps = steps or a.steps
+    base = system.config_c5()
+    R = 8
+    eng = abi.Engine([base] * R, seed=2029)
+    for r in range(R):
+        eng.upload(system.random_path(base, seed=5000 + r, spread=0.05), r=r)
+    shape = eng.autotune(16) if a.autotune else None
+    eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, sk_k_cut=1e300, gr_n_r=200)
+    flags = abi.SWEEP_ADJOINT | abi.SWEEP_ENERGY | abi.SWEEP_SK | abi.SWEEP_GR
+    eng.timing(abi.TIMING_CLOCK)
+    for i in range(max(1, a.warmup)):
+        eng.sweep(1, flags)
+        print(f"c5 warmup sweep {i + 1} done", file=sys.stderr, flush=True)
+    eng.timing_reset()
+    u0 = eng.bead_updates()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        eng.sweep(1, flags)
+        print(f"c5 timed sweep {i + 1}/{steps} done at {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    value = (eng.bead_updates() - u0) / dt
+    rho_ms, rho_n, rho_bytes = eng.kernel_timing(abi.KERNEL_RHO)
+    pair_ms, pair_n, links = eng.kernel_timing(abi.KERNEL_PAIR)
+    eng.close()
+    ach = rho_bytes / (rho_ms * 1e-3) / 1e9 if rho_n else None
+    return {
+        "metric": "bead updates/s, HEG N=1024 M=256 FP64, 8 replicas batched+adjoint",
+        "value": value, "unit": "bead updates/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * dt / steps, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"HEG N=1024 M=256 r_s=1 tau=0.02, David n_order=2, Ewald {base.n_k} stored k-vectors, "
+                               f"bisect level 4 + displace; {R} replicas, adjoint, virial energy, S(k) on all "
+                               f"k-vectors and g(r) on 200 bins every sweep", "replicas": R, "launch_shape": shape},
+        "rho_k_update_GBps": ach,
+        "roofline": ({"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                      "frac": ach / HBM_PEAK_GBPS, "kernel": "k_rho (incremental rho_k update + long-range action)",
+                      "bytes_per_launch": rho_bytes / rho_n, "avg_launch_us": 1e3 * rho_ms / rho_n,
+                      "hbm_copy_GBps": copy_gbps, "frac_of_copy": (ach / copy_gbps) if copy_gbps else None}
+                     if rho_n else None),
+        "pair": ({"avg_us": 1e3 * pair_ms / pair_n, "link_evals_per_s": links / (pair_ms * 1e-3)} if pair_n else None),
+    }
 
 
 if __name__ == "__main__":

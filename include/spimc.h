@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SPIMC_ABI_VERSION 2
+#define SPIMC_ABI_VERSION 3
 
 /* status codes */
 #define SPIMC_OK 0
@@ -384,14 +384,18 @@ int spimc_sweep(spimc_ctx* ctx, uint32_t n_sweeps, uint32_t flags);
 /* Bead updates performed so far (Bisect: 2^l - 1 per attempt, Displace: M). */
 int spimc_bead_updates(spimc_ctx* ctx, uint64_t* out);
 /* Launch-shape autotuning.  For each candidate shape (rho_k kernel beside the
- * pair kernel or after it, rho_k workgroups per CU, pair workgroup size) it
- * runs one bisection and one displace attempt per replica untimed, then times
- * n_rounds bisection attempts followed by n_displace displace attempts (the
- * sweep's ratio, N M / 2^l : N, for replica 0); it keeps the fastest shape for
- * later stages and drops captured graphs.  The stages are ordinary moves of the
- * chains (call it during equilibration); the shape never changes results.
- * chosen (optional, 5 ints): fork, rho_k workgroups per CU, pair workgroup
- * size, number of shapes tried, n_displace.  No counterpart in the reference. */
+ * pair kernel or after it, rho_k workgroups per CU, pair workgroup size; and,
+ * when every replica has N M <= 16384 on unpermuted paths, the persistent
+ * stage kernel that runs a replica's whole stage in one workgroup, with 256,
+ * 512 or 1024 threads) it runs one bisection and one displace attempt per
+ * replica untimed, then times n_rounds bisection attempts followed by
+ * n_displace displace attempts (the sweep's ratio, N M / 2^l : N, for replica
+ * 0); it keeps the fastest shape for later stages and drops captured graphs.
+ * The stages are ordinary moves of the chains (call it during equilibration);
+ * the shape never changes results beyond rounding.  chosen (optional, 6 ints):
+ * fork, rho_k workgroups per CU, pair workgroup size, number of shapes tried,
+ * n_displace, persistent stage workgroup size (0: rounds of launches).  No
+ * counterpart in the reference. */
 int spimc_autotune(spimc_ctx* ctx, uint32_t n_rounds, int32_t* chosen);
 
 /* ---- measurement ------------------------------------------------------ */

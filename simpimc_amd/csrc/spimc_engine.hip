@@ -17,6 +17,7 @@
 
 #include "../../include/spimc.h"
 #include "spimc_perm.h"
+#include "spimc_stage.h"
 
 using namespace spimc;
 
@@ -432,6 +433,10 @@ struct spimc_ctx {
     bool trace_on = false;
     // device address of this device's `scratch` array (spimc_get_action_noalloc)
     void* noalloc_scratch = nullptr;
+    // persistent stage kernel for small replicas: its workgroup size (0: stages run in rounds)
+    int pers_block = 0;
+    void (*pers_fn)(const Rep*, int, int, int, int) = nullptr;
+    hipGraphExec_t pers_graph[2] = {nullptr, nullptr};  // stage begin + persistent stage, per move
     std::string noalloc_error;  // why the address lookup failed, if it did
 };
 
@@ -1082,7 +1087,62 @@ int set_pair_block(spimc_ctx* c, int bs) {
     return SPIMC_OK;
 }
 
-// Drops the captured round graphs (the launch shape changed).
+// Dynamic LDS of the persistent kernel: the pair tables (rounded to 16 bytes),
+// then the phase chains of a stage's moved beads (up to M of them).
+constexpr size_t kPersLdsBytes = 160 * 1024 - 24 * 1024;  // the CU's LDS less the kernel's static arrays
+size_t pers_tables(const spimc_ctx* c) { return ((size_t)c->lds_doubles_max + 1) & ~(size_t)1; }
+size_t pers_chain_bytes(const spimc_ctx* c) {
+    size_t b = 0;
+    for (const auto& h : c->reps) b = std::max(b, (size_t)h.dev.M * 6 * (h.dev.max_idx + 1) * sizeof(double2));
+    return b;
+}
+size_t pers_shmem(const spimc_ctx* c) { return pers_tables(c) * sizeof(double) + pers_chain_bytes(c); }
+
+// Replicas small enough for the persistent stage kernel (a whole stage of one
+// replica in one workgroup, its per-attempt sums in LDS), on paths that are not
+// permuted.
+constexpr int kPersMaxBeads = 16384;  // N M of the largest replica
+bool pers_eligible(const spimc_ctx* c) {
+    if (c->perm_on) return false;
+    for (const auto& h : c->reps)
+        if ((long long)h.dev.N * h.dev.M > kPersMaxBeads || h.dev.M > kStageMaxM || h.dev.n_chunks > 1 ||
+            stage_draws(0, h.dev.n_level) > kStageMaxDraws)
+            return false;
+    return pers_shmem(c) <= kPersLdsBytes;
+}
+
+
+
+// Selects the persistent stage kernel of workgroup size bs (256, 512 or 1024;
+// 0: stages run in rounds).
+int set_pers_block(spimc_ctx* c, int bs) {
+    using StageFn = void (*)(const Rep*, int, int, int, int);
+    const StageFn fns[3][4] = {
+        {k_stage_persistent<1, 0, 256>, k_stage_persistent<2, 2, 256>, k_stage_persistent<2, 0, 256>,
+         k_stage_persistent<0, 0, 256>},
+        {k_stage_persistent<1, 0, 512>, k_stage_persistent<2, 2, 512>, k_stage_persistent<2, 0, 512>,
+         k_stage_persistent<0, 0, 512>},
+        {k_stage_persistent<1, 0, 1024>, k_stage_persistent<2, 2, 1024>, k_stage_persistent<2, 0, 1024>,
+         k_stage_persistent<0, 0, 1024>}};
+    if (bs == 0) {
+        c->pers_block = 0;
+        c->pers_fn = nullptr;
+        return SPIMC_OK;
+    }
+    const int bi = bs == 256 ? 0 : bs == 512 ? 1 : bs == 1024 ? 2 : -1;
+    if (bi < 0) return fail(SPIMC_ERR_ARG, "persistent stage workgroup size must be 0, 256, 512 or 1024");
+    const StageFn fn = fns[bi][c->pair_variant];
+    if (pers_shmem(c) > kPersLdsBytes)
+        return fail(SPIMC_ERR_UNSUPPORTED, "persistent stage kernel: the tables and phase chains exceed its LDS");
+    if (hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pers_shmem(c)) !=
+        hipSuccess)
+        return fail(SPIMC_ERR_HIP, "persistent stage kernel LDS size");
+    c->pers_block = bs;
+    c->pers_fn = fn;
+    return SPIMC_OK;
+}
+
+// Drops the captured round and stage graphs (the launch shape changed).
 void drop_graphs(spimc_ctx* c) {
     for (auto& G : c->groups)
         for (auto& g : G.graph)
@@ -1090,6 +1150,41 @@ void drop_graphs(spimc_ctx* c) {
                 (void)hipGraphExecDestroy(g);
                 g = nullptr;
             }
+    for (auto& g : c->pers_graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// A stage in one launch of the persistent kernel, after the stage's counts are
+// set (k_stage_begin reads them from d_counts).
+int launch_pers_stage(spimc_ctx* c, int move) {
+    const int R = (int)c->reps.size();
+    int rc = launch(c, SPIMC_KERNEL_DECIDE, k_stage_begin, (R + 63) / 64, 64, 0, c->stream, c->d_reps, R, c->d_counts);
+    if (rc) return rc;
+    return launch(c, SPIMC_KERNEL_PROPOSE, c->pers_fn, R, c->pers_block, pers_shmem(c), c->stream, c->d_reps, R, move,
+                  c->lds_doubles_max > 0 ? 1 : 0, (int)pers_tables(c));
+}
+
+int capture_pers_graph(spimc_ctx* c, int move) {
+    if (c->pers_graph[move]) return SPIMC_OK;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const bool saved = c->timing;
+    c->timing = false;
+    int rc = launch_pers_stage(c, move);
+    c->timing = saved;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(SPIMC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->pers_graph[move], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(SPIMC_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    return SPIMC_OK;
 }
 
 // The rho_k kernel of a round: rho_bpc workgroups per CU looping over the items
@@ -1350,6 +1445,8 @@ int ensure_perm(spimc_ctx* c, bool scratch) {
     return upload_reps(c);
 }
 
+int finish_stage(spimc_ctx* c, const std::vector<unsigned int>& cnt, uint32_t mx, uint8_t* accepted);
+
 int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform, uint8_t* accepted, bool use_graph) {
     if (c->prop_set) return fail(SPIMC_ERR_STATE, "an interface proposal is pending; accept or reject it first");
     const int R = (int)c->reps.size();
@@ -1362,6 +1459,17 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
     if (mx == 0) return SPIMC_OK;
     int rc = ensure_accepted(c, mx);
     if (rc) return rc;
+    if (c->pers_fn && move < 2 && pers_eligible(c)) {
+        // the whole stage in one launch (from a captured graph with use_graph)
+        const bool pgraph = use_graph && !c->timing;
+        if (pgraph && (rc = capture_pers_graph(c, move))) return rc;
+        HIPCHK(hipMemcpyAsync(c->d_counts, cnt.data(), R * sizeof(unsigned int), hipMemcpyHostToDevice, c->stream));
+        if (pgraph)
+            HIPCHK(hipGraphLaunch(c->pers_graph[move], c->stream));
+        else if ((rc = launch_pers_stage(c, move)))
+            return rc;
+        return finish_stage(c, cnt, mx, accepted);
+    }
     const bool graph = use_graph && !c->timing && move < 2;
     if (graph)
         for (auto& G : c->groups)
@@ -1395,6 +1503,13 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
         HIPCHK(hipEventRecord(c->groups[g].done, c->groups[g].stream));
         HIPCHK(hipStreamWaitEvent(c->stream, c->groups[g].done, 0));
     }
+    return finish_stage(c, cnt, mx, accepted);
+}
+
+// After a stage's launches: wait, report replay errors, copy the accept flags.
+int finish_stage(spimc_ctx* c, const std::vector<unsigned int>& cnt, uint32_t mx, uint8_t* accepted) {
+    const int R = (int)c->reps.size();
+    int rc;
     if ((rc = sync_and_resolve(c))) return rc;
     c->adjoint_fresh = false;
     // replay errors: one context-wide flag, then the replica that raised it
@@ -1679,6 +1794,12 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         const int vi = pt == SPIMC_PAIR_BARE ? 0 : pt == SPIMC_PAIR_DAVID ? (nord == 2 ? 1 : 2) : 3;
         c->pair_variant = vi;
         if (set_pair_block(c, bs) != SPIMC_OK) c->lds_doubles_max = 0;
+        // stages of small replicas in one persistent launch: off by default, set by
+        // spimc_autotune or SPIMC_PERSISTENT (workgroup size 256, 512 or 1024)
+        if (const char* fp = std::getenv("SPIMC_PERSISTENT")) {
+            const int v = std::atoi(fp);
+            if (v && set_pers_block(c, v) != SPIMC_OK) return bail(SPIMC_ERR_ARG);
+        }
         c->lds_adj_max = c->lds_doubles_max;
         c->adj_lds_fn = vi == 0 ? k_adjoint_local_lds<1, 0> : vi == 1 ? k_adjoint_local_lds<2, 2>
                                                         : vi == 2 ? k_adjoint_local_lds<2, 0> : k_adjoint_local_lds<0, 0>;
@@ -2383,13 +2504,17 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
     if (n_rounds == 0) return fail(SPIMC_ERR_ARG, "n_rounds must be positive");
     if (c->prop_set) return fail(SPIMC_ERR_STATE, "an interface proposal is pending; accept or reject it first");
     struct Shape {
-        int fork, bpc, bs;
+        int fork, bpc, bs, pers;
     };
     // launch shapes tried: rho_k beside the pair kernel with 1-3 looping workgroups
-    // per CU and 256/512-thread pair workgroups, or the two kernels one after the other
-    std::vector<Shape> shapes = {{1, 2, 512}, {1, 3, 512}, {1, 1, 512}, {1, 2, 256}, {1, 3, 256}, {0, 0, 1024}, {0, 2, 512}};
-    if (!c->any_long || c->lds_doubles_max == 0) shapes = {{0, 0, 1024}, {0, 0, 512}, {0, 0, 256}};
-    const Shape saved{c->fork ? 1 : 0, c->rho_bpc, c->pair_block};
+    // per CU and 256/512-thread pair workgroups, or the two kernels one after the
+    // other; for small replicas also the persistent stage kernel with 256-1024 threads
+    std::vector<Shape> shapes = {{1, 2, 512, 0}, {1, 3, 512, 0}, {1, 1, 512, 0}, {1, 2, 256, 0},
+                                 {1, 3, 256, 0}, {0, 0, 1024, 0}, {0, 2, 512, 0}};
+    if (!c->any_long || c->lds_doubles_max == 0) shapes = {{0, 0, 1024, 0}, {0, 0, 512, 0}, {0, 0, 256, 0}};
+    if (pers_eligible(c))
+        for (int pb : {256, 512, 1024}) shapes.push_back({c->fork ? 1 : 0, c->rho_bpc, c->pair_block, pb});
+    const Shape saved{c->fork ? 1 : 0, c->rho_bpc, c->pair_block, c->pers_block};
     // the sweep's mix: N M / 2^l bisection attempts per N displace attempts
     const Rep& R0 = c->reps[0].dev;
     const uint32_t n_dis = std::max<uint32_t>(1, (uint32_t)((uint64_t)n_rounds * (1u << R0.n_level) / R0.M));
@@ -2400,6 +2525,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
         c->fork = sh.fork != 0;
         c->rho_bpc = sh.bpc;
         int rc = c->lds_doubles_max > 0 ? set_pair_block(c, sh.bs) : SPIMC_OK;
+        if (rc == SPIMC_OK) rc = set_pers_block(c, sh.pers);
         if (rc) continue;
         drop_graphs(c);
         // one untimed round of each move under the new shape, then the timed mix
@@ -2412,6 +2538,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
             c->fork = saved.fork != 0;
             c->rho_bpc = saved.bpc;
             if (c->lds_doubles_max > 0) (void)set_pair_block(c, saved.bs);
+            (void)set_pers_block(c, saved.pers);
             return rc;
         }
         ++tried;
@@ -2426,6 +2553,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
         int rc = set_pair_block(c, best.bs);
         if (rc) return rc;
     }
+    if (int rc = set_pers_block(c, best.pers)) return rc;
     drop_graphs(c);
     if (chosen) {
         chosen[0] = best.fork;
@@ -2433,6 +2561,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
         chosen[2] = c->pair_block;
         chosen[3] = tried;
         chosen[4] = (int32_t)n_dis;
+        chosen[5] = c->pers_block;
     }
     return SPIMC_OK;
 }

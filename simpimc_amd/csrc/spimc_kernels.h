@@ -174,11 +174,16 @@ __global__ __launch_bounds__(256) void k_rho_build(const Rep* __restrict__ reps,
 // per replica.  Coarse levels carry only the kinetic action (the pair action
 // returns 0 above max_level = 0, pair_action_class.h:269).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps, int n_reps) {
-    const int r = blockIdx.x;
-    if (r >= n_reps) return;
-    const Rep& R = reps[r];
-    const int lane = threadIdx.x;
+// Synchronisation of the one-wavefront bodies below (the whole 64-thread
+// workgroup of the round kernels).
+struct BlockSync {
+    __device__ void operator()() const { __syncthreads(); }
+};
+
+// Bisect::Attempt up to the level-0 decision, one wavefront (lane 0..63) on
+// replica R; items: also write the round's rho_k work items.
+template <class Sync>
+__device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync sync, bool items) {
     __shared__ double sh_old[(1 << kMaxLevel) + 1][3];
     __shared__ double sh_new[(1 << kMaxLevel) + 1][3];
     __shared__ int s_info[4];
@@ -195,10 +200,10 @@ __device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps
             c->seq = s_seq + 1;
         }
     }
-    __syncthreads();
+    sync();
     if (!s_info[0]) {
         if (lane == 0) pr->status = 0;
-        rho_items_write(R, 0, 0, 0, 0, lane);
+        if (items) rho_items_write(R, 0, 0, 0, 0, lane);
         return;
     }
     const int n = R.n_level, nb = 1 << n, N = R.N, M = R.M;
@@ -227,7 +232,7 @@ __device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps
         s_info[3] = bad;
         s_cur = cur;
     }
-    __syncthreads();
+    sync();
     const int p = s_info[1], b0 = s_info[2];
     // the chain walked from (p, b0) carries label p2 after the wrap
     const int p2 = label_after_wrap(R, p);
@@ -244,7 +249,7 @@ __device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps
             sh_new[j][d] = x;
         }
     }
-    __syncthreads();
+    sync();
     long long cur = s_cur;
     double prev = 0.;
     int g_base = 0;  // normals drawn so far (device RNG indexing)
@@ -281,7 +286,7 @@ __device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps
             lspo = -free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dold);
             lspn = -free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dnew);
         }
-        __syncthreads();
+        sync();
         const int nlinks = wraps ? 0 : nb >> level;
         const double i4lt_k = 1. / (4. * R.lambda * level_tau);
         double ko = 0., kn = 0.;
@@ -321,7 +326,7 @@ __device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps
                     pr->count = 0;
                     pr->tape_next = cur;
                 }
-                rho_items_write(R, 0, 2, 0, 0, lane);
+                if (items) rho_items_write(R, 0, 2, 0, 0, lane);
                 return;
             }
             prev = change;
@@ -352,10 +357,16 @@ __device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps
             if (R.use_long_range) {
                 for (int t = lane; t < (wraps ? 0 : 6 * (nb - 1)); t += kWave)
                     moved_phase_chain(R, t / 6, t % 6, sh_old[t / 6 + 1], sh_new[t / 6 + 1]);
-                rho_items_write(R, 0, status, wraps ? 0 : nb - 1, (b0 + 1) % M, lane);
+                if (items) rho_items_write(R, 0, status, wraps ? 0 : nb - 1, (b0 + 1) % M, lane);
             }
         }
     }
+}
+
+__device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps, int n_reps) {
+    const int r = blockIdx.x;
+    if (r >= n_reps) return;
+    propose_bisect_impl(reps[r], (int)threadIdx.x, BlockSync{}, true);
 }
 
 // DisplaceParticle::Attempt (displace_particle_class.h:28-72): NEW beads only.
@@ -364,11 +375,8 @@ __device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps
 // the wrap link (p, M-1) -> (perm[p], 0), the one link of the walk a rigid shift
 // changes.  The link into (p, 0) from (iperm[p], M-1), which the shift also
 // changes, is outside the walk and not counted, as in the reference.
-__device__ __forceinline__ void propose_displace_body(const Rep* __restrict__ reps, int n_reps) {
-    const int r = blockIdx.x;
-    if (r >= n_reps) return;
-    const Rep& R = reps[r];
-    const int lane = threadIdx.x;
+template <class Sync>
+__device__ __forceinline__ void propose_displace_impl(const Rep& R, int lane, Sync sync, bool items) {
     __shared__ int s_act, s_p, s_status;
     __shared__ double s_dr[3];
     Prop* pr = R.prop;
@@ -433,9 +441,9 @@ __device__ __forceinline__ void propose_displace_body(const Rep* __restrict__ re
             pr->status = 0;
         }
     }
-    __syncthreads();
+    sync();
     if (!s_act) {
-        rho_items_write(R, 1, 0, 0, 0, lane);
+        if (items) rho_items_write(R, 1, 0, 0, 0, lane);
         return;
     }
     const int p = s_p;
@@ -451,7 +459,13 @@ __device__ __forceinline__ void propose_displace_body(const Rep* __restrict__ re
             }
             moved_phase_chain(R, s, t % 6, xo, xn);
         }
-    if (R.use_long_range) rho_items_write(R, 1, s_status, R.M, 0, lane);
+    if (R.use_long_range) if (items) rho_items_write(R, 1, s_status, R.M, 0, lane);
+}
+
+__device__ __forceinline__ void propose_displace_body(const Rep* __restrict__ reps, int n_reps) {
+    const int r = blockIdx.x;
+    if (r >= n_reps) return;
+    propose_displace_impl(reps[r], (int)threadIdx.x, BlockSync{}, true);
 }
 
 // Phase tables of an interface proposal's moved beads (before k_rho in interface mode).
@@ -566,22 +580,13 @@ __device__ __forceinline__ void kt_close(unsigned long long* kt) {
 // replica loads its B-spline tables once, then each wavefront takes window links
 // j = w, w + 16, ...; lanes run over the partners q.
 constexpr int kPairLdsBlock = 1024;
-template <int PT, int NORD, int BS>
-__device__ __forceinline__ void pair_lds_block(const Rep* __restrict__ reps, int n_reps, int bpr) {
-    extern __shared__ double lds[];
-    // bpr workgroups per replica (when replicas are few): part takes every bpr-th
-    // group of BS/64 links
-    const int idx = xcd_remap(blockIdx.x, gridDim.x), r = idx / bpr, part = idx % bpr;
-    if (r >= n_reps) return;
-    const Rep R = reps[r];  // by value: the fields used stay in registers across the loops
-    const Prop pr = *R.prop;
-    if (pr.status != 1 || pr.generic) return;
-    for (int t = threadIdx.x; t < R.lds_doubles; t += BS) lds[t] = (t < R.bt_a.ncoef * R.bt_a.ncomp) ? R.bt_a.c[t] : R.bt_b.c[t - R.bt_a.ncoef * R.bt_a.ncomp];
-    __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+// The window links j = j0, j0 + step, ... of a proposal, one wavefront: lanes
+// run over the partners q; the OLD and NEW sums of link j go to pair_part[j].
+template <int PT, int NORD>
+__device__ __forceinline__ void pair_links_lds(const Rep& R, const double* lds, const Prop& pr, int j0, int step,
+                                               int lane) {
     const int M = R.M, N = R.N, p = pr.p;
-    if (part * (BS / kWave) >= pr.n_links) return;
-    for (int j = part * (BS / kWave) + w; j < pr.n_links; j += bpr * (BS / kWave)) {
+    for (int j = j0; j < pr.n_links; j += step) {
         const int s0 = (pr.b0 + j) % M, s1 = (s0 + 1) % M;
         double po0[3], po1[3], pn0[3], pn1[3];
         for (int d = 0; d < 3; ++d) {
@@ -622,6 +627,24 @@ __device__ __forceinline__ void pair_lds_block(const Rep* __restrict__ reps, int
         }
     }
 }
+
+template <int PT, int NORD, int BS>
+__device__ __forceinline__ void pair_lds_block(const Rep* __restrict__ reps, int n_reps, int bpr) {
+    extern __shared__ double lds[];
+    // bpr workgroups per replica (when replicas are few): part takes every bpr-th
+    // group of BS/64 links
+    const int idx = xcd_remap(blockIdx.x, gridDim.x), r = idx / bpr, part = idx % bpr;
+    if (r >= n_reps) return;
+    const Rep R = reps[r];  // by value: the fields used stay in registers across the loops
+    const Prop pr = *R.prop;
+    if (pr.status != 1 || pr.generic) return;
+    for (int t = threadIdx.x; t < R.lds_doubles; t += BS) lds[t] = (t < R.bt_a.ncoef * R.bt_a.ncomp) ? R.bt_a.c[t] : R.bt_b.c[t - R.bt_a.ncoef * R.bt_a.ncomp];
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (part * (BS / kWave) >= pr.n_links) return;
+    pair_links_lds<PT, NORD>(R, lds, pr, part * (BS / kWave) + w, bpr * (BS / kWave), lane);
+}
+
 
 // PT: 1 all replicas Bare, 2 all David, 0 mixed (type read per replica);
 // NORD: the David expansion order when all replicas share it, else 0
@@ -801,13 +824,8 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_loop(const RhoItem* __restric
 // K5: level-0 Metropolis decision and commit (bisect_class.h:100-123,24-36,127-139;
 // displace_particle_class.h:68-86), one wavefront per replica.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void decide_body(const Rep* __restrict__ reps, int n_reps,
-                                            unsigned long long* __restrict__ kt) {
-    if (kt && blockIdx.x == 0) kt_close(kt);
-    const int r = blockIdx.x;
-    if (r >= n_reps) return;
-    const Rep& R = reps[r];
-    const int lane = threadIdx.x;
+template <class Sync>
+__device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
     Prop* prp = R.prop;
     const Prop pr = *prp;
     if (pr.status == 0) return;
@@ -892,7 +910,7 @@ __device__ __forceinline__ void decide_body(const Rep* __restrict__ reps, int n_
         }
         prp->status = 0;
     }
-    __syncthreads();
+    sync();
     const int M = R.M, N = R.N;
     if (s_acc && pr.move == 3) {
         // Bisect::Accept of every chain, then PermBisect::AssignParticleLabels
@@ -907,7 +925,7 @@ __device__ __forceinline__ void decide_body(const Rep* __restrict__ reps, int n_
         }
         if (R.use_long_range)
             for (int m = lane; m < nm; m += kWave) R.sel[(pr.first + m) % M] ^= 1;
-        __syncthreads();
+        sync();
         if (n > 1) {
             const bool wrapped = pr.b0 + nb - 1 >= M;
             const int e = pr.e_slice;
@@ -950,12 +968,20 @@ __device__ __forceinline__ void decide_body(const Rep* __restrict__ reps, int n_
         }
     }
     if (pr.move == 3 && pr.status == 1) {
-        __syncthreads();
+        sync();
         for (int i = lane; i < pr.n_chain; i += kWave) {
             R.mv_pre[R.ch_lbl[i]] = -1;
             R.mv_post[R.ch_lbl2[i]] = -1;
         }
     }
+}
+
+__device__ __forceinline__ void decide_body(const Rep* __restrict__ reps, int n_reps,
+                                            unsigned long long* __restrict__ kt) {
+    if (kt && blockIdx.x == 0) kt_close(kt);
+    const int r = blockIdx.x;
+    if (r >= n_reps) return;
+    decide_impl(reps[r], (int)threadIdx.x, BlockSync{});
 }
 
 __global__ __launch_bounds__(64) void k_propose_bisect(const Rep* __restrict__ reps, int n_reps) {

@@ -84,8 +84,13 @@ def compare(got, ref_full, label, absolute=True):
     return int(ok.sum())
 
 
-@pytest.mark.parametrize("name", sorted(CASES))
-def test_hot_path_action_terms_match_oracle(name):
+RUNS = [(name, 0) for name in sorted(CASES)] + [("bare_c1", 256), ("david_small", 256), ("ilkka_c1", 512)]
+
+
+# persistent: 0 stages as rounds of launches, else the persistent stage kernel's workgroup size
+@pytest.mark.parametrize("name,persistent", RUNS)
+def test_hot_path_action_terms_match_oracle(name, persistent, monkeypatch):
+    monkeypatch.setenv("SPIMC_PERSISTENT", str(persistent))
     make, n_b, n_d, stages = CASES[name]
     s = make()
     pos = system.random_path(s, seed=3, spread=0.05)

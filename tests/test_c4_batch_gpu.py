@@ -1,7 +1,8 @@
 """The uneven batch at its stated size: 64 replicas with N in {16, 32, 64, 128},
 M in {16, 32, 64} and r_s in [1, 4], each on its own box and k-set (the
 reference's MPI clones, framework_class.h:44-53), advanced together by whole
-sweeps in one context, eagerly and from captured hipGraphs.
+sweeps in one context, eagerly and from captured hipGraphs, with the stages
+run as rounds of launches or as one persistent launch per stage.
 
 Every replica replays its own oracle chain: a sweep of N M / 2^l bisections
 (bisect_class.h:39-124) and N displacements (displace_particle_class.h:28-72),
@@ -47,13 +48,15 @@ def rel_err(a, b):
     return float(np.max(np.abs(a - b))) / max(1.0, float(np.max(np.abs(b)))) if b.size else 0.0
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_c4_batch_replays_every_replica(graph):
+# stages as rounds of launches, or each in one persistent launch (workgroup size)
+@pytest.mark.parametrize("graph,persistent", [(False, 0), (True, 0), (False, 256), (True, 256), (False, 1024)])
+def test_c4_batch_replays_every_replica(graph, persistent, monkeypatch):
+    monkeypatch.setenv("SPIMC_PERSISTENT", str(persistent))
     systems, ref = oracle_chains()
     assert len(systems) == 64
     assert {s.n_part for s in systems} == {16, 32, 64, 128} and {s.n_bead for s in systems} == {16, 32, 64}
     assert all(s.n_k == 257 for s in systems)
-    print(f"C4: {sum(s.n_part * s.n_bead for s in systems)} beads, graph={graph}", flush=True)
+    print(f"C4: {sum(s.n_part * s.n_bead for s in systems)} beads, graph={graph}, persistent={persistent}", flush=True)
     with abi.Engine(systems, seed=1) as e:
         for i, r in enumerate(ref):
             e.upload(r["pos"], r=i)

@@ -128,6 +128,60 @@ def test_replay_accept_reject_bit_exact(sysname):
         assert np.array_equal(e.counters(), o.counters())
 
 
+@pytest.mark.parametrize("block", [256, 1024])
+def test_persistent_stage_replay_bit_exact(sysname, block, monkeypatch):
+    """Whole stages in one launch of the persistent stage kernel (one workgroup
+    per replica looping over its attempts) replay the oracle's chain bit for
+    bit, three single replicas and the three batched together."""
+    monkeypatch.setenv("SPIMC_PERSISTENT", str(block))
+    s, pos = make(sysname, spread=0.05)
+    o = Oracle(s, seed=11)
+    o.upload(pos)
+    o.record(True)
+    n_b = s.n_part * s.n_bead >> s.bisect_n_level
+    ref = []
+    for _ in range(2):
+        ref.append(o.bisect(n_b))
+        ref.append(o.displace(s.n_part))
+    tape = o.recorded()
+    with abi.Engine(s, seed=99) as e:
+        e.upload(pos)
+        e.tape_set(tape)
+        for i in range(2):
+            assert np.array_equal(e.bisect(n_b)[0], ref[2 * i])
+            assert np.array_equal(e.displace(s.n_part)[0], ref[2 * i + 1])
+        assert e.tape_cursor() == len(tape)
+        assert np.array_equal(e.download(), o.download())
+        assert rel_err(e.rho_k(), o.rho_k()) < 1e-11 * s.n_part
+        assert np.array_equal(e.counters(), o.counters())
+
+
+def test_persistent_stage_mixed_batch(monkeypatch):
+    """Bare, David and Ilkka replicas in one persistent launch per stage (the
+    global-table arithmetic, pair type read per replica) replay their chains."""
+    monkeypatch.setenv("SPIMC_PERSISTENT", "512")
+    systems = mixed_systems()
+    tapes, refs, finals = [], [], []
+    for i, s in enumerate(systems):
+        pos = system.random_path(s, seed=60 + i, spread=0.05)
+        o = Oracle(s, seed=200 + i)
+        o.upload(pos)
+        o.record(True)
+        refs.append((o.bisect(40 + 8 * i), o.displace(8)))
+        tapes.append((pos, o.recorded()))
+        finals.append(o.download())
+    with abi.Engine(systems, seed=1) as e:
+        for i, (pos, tape) in enumerate(tapes):
+            e.upload(pos, r=i)
+            e.tape_set(tape, r=i)
+        ab = e.bisect([40 + 8 * i for i in range(len(systems))])
+        ad = e.displace(8)
+        for i in range(len(systems)):
+            assert np.array_equal(ab[i, :40 + 8 * i], refs[i][0]), i
+            assert np.array_equal(ad[i], refs[i][1]), i
+            assert np.array_equal(e.download(r=i), finals[i]), i
+
+
 def test_gradient_compat(sysname):
     s, pos = make(sysname)
     o = Oracle(s)
