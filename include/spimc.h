@@ -147,7 +147,7 @@ typedef struct spimc_system_desc {
     const double* david_du_kj;
     /* long range: per stored k-vector v_long_k (Bare: as read; David: u_k/volume),
      * and the raw k=0 and r=0 constants before the homologous scaling of
-     * bare_pair_action_class.h:89-95 / david_pair_action_class.h:533-543. */
+     * bare_pair_action_class.h:89-95 / david_pair_action_class.h:340-361. */
     const double* pair_v_long_k; /* [n_k] */
     double pair_v_long_k_0;
     double pair_v_long_r_0;

@@ -1151,7 +1151,9 @@ struct Oracle {
             double t_Q = 0.;
             const uint32_t from = p;
             bool found = false;
+            int last = -1;  // the last candidate with a nonzero weight
             for (uint32_t i = 0; i < N; ++i) {
+                if (t_c[i] > 0.) last = (int)i;
                 t_Q += t_c[i] / Q_p_c;
                 if (t_Q > x) {
                     p = i;
@@ -1160,10 +1162,17 @@ struct Oracle {
                 }
             }
             // a row whose running sum never passes x selects no particle; the
-            // reference would then revisit the same particle
+            // reference would then revisit the same particle.  Free-running, this
+            // engine's rule: the running sum fell an ulp short (the last candidate
+            // is the selection), or no candidate is left (the walk stops).  On a
+            // replay tape it is an error.
             if (!found) {
-                g_err = "permutation walk selected no particle";
-                return false;
+                if (use_tape) {
+                    g_err = "permutation walk selected no particle";
+                    return false;
+                }
+                if (last < 0) return false;
+                p = (uint32_t)last;
             }
             // the weight's factors t(ps[i], ps[i+1]) / t(ps[i], ps[i]), in order
             w_num.push_back(t_row[p]);

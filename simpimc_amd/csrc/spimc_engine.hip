@@ -630,7 +630,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
         double kv[3] = {n0 * box, n1 * box, n2 * box};
         h.kmag[k] = std::sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
         // Bare: CalcULong multiplies the k sum by level_tau (bare_pair_action_class.h:170);
-        // David: u_long_k = v_long_k * tau (david_pair_action_class.h:519).
+        // David: u_long_k = v_long_k * tau (david_pair_action_class.h:334).
         // Ilkka: u_long_k, du_long_k and v_long_k as read (ilkka_pair_action_class.h:63-80, 109-124, 152-168).
         h.coef_s[k] = (s.pair_type == SPIMC_PAIR_DAVID) ? s.pair_v_long_k[k] * tau : s.pair_v_long_k[k];
         h.coef_b[k] = s.pair_v_long_k[k];
@@ -836,7 +836,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     R.kin_dbeta_const = N * M * 3 / (2. * tau);
     R.long_scale = (s.pair_type == SPIMC_PAIR_BARE) ? tau : 1.;
     const double nn = (double)N;
-    // homologous constants (bare_pair_action_class.h:89-95, david_pair_action_class.h:533-543)
+    // homologous constants (bare_pair_action_class.h:89-95, david_pair_action_class.h:340-361)
     R.const_v = lr ? s.pair_v_long_k_0 * 0.5 * nn * nn * M + s.pair_v_long_r_0 * (-0.5) * nn * M : 0.;
     R.const_du = R.const_v;
     // Ilkka: du/diag constants for dU/dbeta (ilkka_pair_action_class.h:418-423)

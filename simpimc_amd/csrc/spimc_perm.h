@@ -239,24 +239,31 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
                     if (lane >= o) incl += v;
                 }
                 double t_q = incl - part;
-                int mine = 1 << 30;
+                int mine = 1 << 30, last = -1;
                 for (int i = lo; i < hi; ++i) {
                     const double tc = (i == p0) ? t_row[i] : (vis[i] >= 0 ? 0. : t_row[i]);
+                    if (tc > 0.) last = i;
                     t_q += tc / q_c;
-                    if (t_q > x) {
-                        mine = i;
-                        break;
-                    }
+                    if (t_q > x && mine == (1 << 30)) mine = i;
                 }
-                for (int o = kWave / 2; o > 0; o >>= 1) mine = min(mine, __shfl_xor(mine, o, kWave));
+                for (int o = kWave / 2; o > 0; o >>= 1) {
+                    mine = min(mine, __shfl_xor(mine, o, kWave));
+                    last = max(last, __shfl_xor(last, o, kWave));
+                }
                 found = mine < N ? mine : -1;
+                // Without a replay tape a scan that selects nothing is rounding (its
+                // total an ulp short of x: the last candidate is the selection) or a
+                // row without candidates (every weight below epsilon: the walk
+                // cannot continue, the attempt is rejected).  With a tape it stays
+                // an error, as in the oracle.
+                if (found < 0 && !tape) found = last >= 0 ? last : -2;
             }
             if (lane == 0) {
                 int st = 0, nx = p;
-                if (!go) {
+                if (!go || found == -2) {
                     st = 2;
                 } else if (found < 0) {
-                    st = 3;  // no particle selected (a degenerate row)
+                    st = 3;  // no particle selected (a degenerate row on a replay tape)
                 } else {
                     nx = found;
                     // the cycle weight's factors t(ps[i], ps[i+1]) / t(ps[i], ps[i]), in order

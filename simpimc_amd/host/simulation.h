@@ -275,7 +275,7 @@ struct BarePairAction : PairActionBase {
     }
 };
 
-// DavidPairAction loader (david_pair_action_class.h:377-545)
+// DavidPairAction loader (david_pair_action_class.h:192-360)
 struct DavidPairAction : PairActionBase {
     std::vector<double> grid, u_kj, du_kj;
     double v_long_k_0 = 0., v_long_r_0 = 0.;
@@ -292,7 +292,7 @@ struct DavidPairAction : PairActionBase {
         } else {
             throw std::runtime_error("grid type " + gtype + " needs grid_points in this engine");
         }
-        // tau must be one of the table's taus within 1e-6 (david_pair_action_class.h:419-430)
+        // tau must be one of the table's taus within 1e-6 (david_pair_action_class.h:236-245)
         const auto& taus = table.Read(pre + "/taus");
         const uint32_t n_tau = max_level + 1;
         int tau_i = -1;
@@ -589,7 +589,7 @@ struct Observable : Event {
     }
 };
 
-// Energy (energy_class.h:286-336, Write :339-432 without sectors)
+// Energy (energy_class.h:179-230, Write :232-325 without sectors)
 struct Energy : Observable {
     std::vector<std::string> action_names;
     int estimator;

@@ -7,7 +7,7 @@
 //    `<name> <f64|i64|str> <count>` followed by the values.
 // Multi-dimensional data are flat in storage order; Cube() returns a David
 // u_kj cube with the first index (the k,j component) fastest, as the reference
-// holds it in memory (david_pair_action_class.h:441-443).
+// holds it in memory (david_pair_action_class.h:257-258).
 #pragma once
 
 #include <fstream>

@@ -194,7 +194,7 @@ def bare_coulomb_ewald(sys: System, alpha: float, n_r: int = 1000, n_r_long: int
 
 
 def david_n_val(n_order: int) -> int:
-    """Number of u_kj components (david_pair_action_class.h:437-439)."""
+    """Number of u_kj components (david_pair_action_class.h:250-253)."""
     return 1 + sum(1 + i for i in range(1, n_order + 1))
 
 
@@ -228,7 +228,7 @@ def david_file_arrays(L: float, alpha: float, tau: float, n_grid: int, n_order: 
 
 
 def david_assemble(grid: np.ndarray, V: np.ndarray, u: np.ndarray, du: np.ndarray):
-    """The spline knots DavidPairAction's loader builds (david_pair_action_class.h:449-494):
+    """The spline knots DavidPairAction's loader builds (david_pair_action_class.h:262-309):
     component 0 = potential, components 1.. = file data; the loader's fill loop
     stops at n_grid - 1, so the last knot of every component stays 0."""
     n_val, n_grid = u.shape
@@ -463,7 +463,7 @@ def ilkka_table_datasets(sys: System) -> dict:
 def david_table_datasets(L: float, alpha: float, tau: float, k_n: np.ndarray, n_grid: int = 2000, n_order: int = 2,
                          seed: int = 7, amp: float = 1e-3, r_min: float = 0.01) -> dict:
     """u_kj_<n>/*, du_kj_dbeta_<n>/*, potential/data, long_range/*, squarer/v_image of
-    DavidPairAction (david_pair_action_class.h:395-530).  data arrays are the
+    DavidPairAction (david_pair_action_class.h:192-360).  data arrays are the
     (n_val, n_grid, n_tau) cubes flattened with the first index fastest."""
     grid, V, u, du = david_file_arrays(L, alpha, tau, n_grid, n_order, seed, amp, r_min)
     vol = L ** 3

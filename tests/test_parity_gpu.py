@@ -523,6 +523,34 @@ def test_adjoint_on_many_configurations():
     assert worst < 1e-6, worst
 
 
+def test_adjoint_on_1000_configurations():
+    """The differentiation configuration at its stated size: 1000 seeded paths
+    of the N=32, M=32 Bare/Ewald system (batched as 1000 replicas of one
+    context), the analytic dS/dr of every bead computed on each, and 8 random
+    beads per path checked against the oracle's central difference (h = 1e-5)
+    of the kinetic, short-range and Ewald action, to 1e-6 relative."""
+    s = system.config_c1()
+    n_cfg, n_beads = 1000, 8
+    rng = np.random.default_rng(12)
+    paths = [system.random_path(s, seed=20000 + i, spread=0.03 + 0.1 * rng.random()) for i in range(n_cfg)]
+    beads = [np.stack([rng.integers(0, s.n_bead, n_beads), rng.integers(0, s.n_part, n_beads)], axis=1)
+             for _ in range(n_cfg)]
+    o = Oracle(s)
+    ref = []
+    for pos, b in zip(paths, beads):
+        o.upload(pos)
+        ref.append(o.gradient_beads(b, abi.GRAD_ALL, 1e-5))
+    worst = 0.0
+    with abi.Engine([s] * n_cfg, seed=1) as e:
+        for i, pos in enumerate(paths):
+            e.upload(pos, r=i)
+        for i, b in enumerate(beads):
+            g = e.gradient(abi.GRAD_ALL, r=i)
+            worst = max(worst, rel_err(g[b[:, 0], b[:, 1]], ref[i]))
+    print(f"worst relative error over {n_cfg} configurations: {worst:.3g}", flush=True)
+    assert worst < 1e-6, worst
+
+
 def test_adjoint_at_full_size_on_sampled_beads():
     """At N=256, M=128 (David pair action, 1979 k-vectors) the analytic dS/dr of
     96 randomly chosen beads against the oracle's central difference of those

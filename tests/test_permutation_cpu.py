@@ -167,3 +167,37 @@ def test_cycle_lengths_of_a_permutation():
     o.upload(system.random_path(s, seed=1))
     o.set_perm([1, 2, 0, 4, 3, 5])
     assert list(o.cycle_lengths()) == [3, 2, 1]
+
+
+def degenerate_rows_system():
+    """Free bosons whose beads sit independently at random, with a table cutoff
+    epsilon so large that most rows t(p, .) of the permutation table have no
+    entry above it: the walk there has no candidate to select."""
+    s = free_system()
+    rng = np.random.default_rng(9)
+    pos = rng.uniform(-L / 2, L / 2, size=(3, M, N))
+    return s, pos
+
+
+def test_walk_without_candidates_rejects_free_running_and_fails_on_a_tape():
+    """A cycle walk whose row has no candidate (every weight below epsilon):
+    free-running, the attempt is rejected and the chain goes on; replaying the
+    same draws from a tape, it is an error (the reference would revisit the
+    same particle)."""
+    s, pos = degenerate_rows_system()
+    o = oracle_ffi.Oracle(s, seed=5)
+    o.upload(pos)
+    o.perm_configure(3, epsilon=0.5)
+    o.record(True)
+    acc = o.perm_bisect(200)
+    tape = o.recorded()
+    c = o.perm_counters()
+    # attempts ran without error; most walks stopped at a row without candidates
+    assert c[0] == 200 and acc.sum() == c[1]
+    assert int(np.sum(c[2])) < 200
+    r = oracle_ffi.Oracle(s, seed=6)
+    r.upload(pos)
+    r.perm_configure(3, epsilon=0.5)
+    r.tape_set(tape)
+    with pytest.raises(RuntimeError, match="selected no particle"):
+        r.perm_bisect(200)

@@ -318,3 +318,32 @@ def test_permutation_edge_cases(case):
         assert abs(a - b) <= 1e-10 * max(1.0, abs(b)), (a, b)
     if case == "two_particles":
         assert og[3][1] > 0  # swaps were accepted
+
+
+def test_walk_without_candidates_is_rejected_free_running():
+    """Free-running (device RNG), a cycle walk whose row has no candidate above
+    epsilon rejects the attempt instead of failing the stage; replaying the
+    oracle's draws, the same walk is an error on both sides."""
+    from test_permutation_cpu import degenerate_rows_system
+
+    s, pos = degenerate_rows_system()
+    with abi.Engine([s, s], seed=21) as e:
+        for r in range(2):
+            e.upload(pos, r=r)
+            e.perm_configure(3, epsilon=0.5, r=r)
+        acc = e.perm_bisect(200)
+        for r in range(2):
+            att, n_acc, _, _ = e.perm_counters(r)
+            assert att == 200 and n_acc == int(acc[r].sum())
+    o = Oracle(s, seed=5)
+    o.upload(pos)
+    o.perm_configure(3, epsilon=0.5)
+    o.record(True)
+    o.perm_bisect(200)
+    tape = o.recorded()
+    with abi.Engine(s, seed=1) as e:
+        e.upload(pos)
+        e.perm_configure(3, epsilon=0.5)
+        e.tape_set(tape)
+        with pytest.raises(abi.SpimcError):
+            e.perm_bisect(200)
