@@ -67,6 +67,8 @@ int noalloc_scratch_ptr(int device, void** out) {
 constexpr int kRhoKpt = 8;        // k-vectors per thread in k_rho (default; SPIMC_RHO_KPT 2, 4, 8, 16)
 constexpr int kRhoKptMin = 2;     // smallest chunking: bounds the number of k chunks
 constexpr int kNumCUs = 256;      // MI355X compute units (for workgroups-per-replica choices)
+// fewest waves for which the pair-once adjoint beats the per-particle one
+constexpr long long kAdjPairsMinWaves = 4096;
 constexpr int kBuildKpt = 2;
 constexpr int kRoundsPerGraph = 16;
 constexpr int kAdjMaxIdx = 40;
@@ -398,6 +400,11 @@ struct spimc_ctx {
     int pair_block = kPairLdsBlock;
     int pair_variant = 3;  // 0 Bare, 1 David order 2, 2 David other order, 3 mixed
     void (*adj_lds_fn)(const Rep*, unsigned int, int) = nullptr;
+    // the pair-once adjoint (two launches), used when its link ranges give the
+    // GPU enough waves; SPIMC_ADJ_PAIRS = 0 / 1 forces the choice
+    void (*adj_pairs_fn)(const Rep*, unsigned int, int, int) = nullptr;
+    int adj_pairs_env = -1;
+    bool adj_pairs_ok = false;  // its tables and slots fit in LDS
     int groups_wanted = 1;
     bool fork = false;  // rho_k beside the pair kernel on a second stream per group
     int perm_stage = kPermStage;  // largest permuting proposal (chain beads) bisected on LDS
@@ -1584,8 +1591,25 @@ int est_gradient(spimc_ctx* c, Rep* base, int nrep, uint32_t terms, int compat, 
         const size_t shmem = (size_t)c->lds_adj_max * sizeof(double);
         // workgroups per replica: enough to fill the GPU when the replicas are few
         const int bpr = std::max(1, std::min(c->max_M, (2 * kNumCUs + nrep - 1) / nrep));
-        rc = launch(c, SPIMC_KERNEL_ADJOINT, c->adj_lds_fn, dim3(nrep, bpr), kAdjLdsBlock, shmem, c->stream, base, terms,
-                    out_est);
+        // pair-once kernel: at most max_M / 2 link ranges of 16 waves' worth per
+        // replica; chosen when they give at least kAdjPairsMinWaves waves
+        const int rng_max = std::max(1, c->max_M / 2);
+        const int bpr2 = std::max(1, std::min((rng_max + kAdjPairWaves - 1) / kAdjPairWaves,
+                                              (int)((kAdjPairsMinWaves + kAdjPairWaves * nrep - 1) / (kAdjPairWaves * nrep))));
+        const long long waves2 = (long long)nrep * std::min(kAdjPairWaves * bpr2, rng_max);
+        const bool pairs = c->adj_pairs_ok &&
+                           (c->adj_pairs_env >= 0 ? c->adj_pairs_env != 0 : waves2 >= kAdjPairsMinWaves);
+        if (pairs) {
+            const size_t shmem2 = shmem + (size_t)kAdjPairWaves * kAdjSlotDoubles * sizeof(double);
+            rc = launch(c, SPIMC_KERNEL_ADJOINT, c->adj_pairs_fn, dim3(nrep, bpr2), kAdjLdsBlock, shmem2, c->stream,
+                        base, terms, out_est, 0);
+            if (!rc)
+                rc = launch(c, SPIMC_KERNEL_ADJOINT, c->adj_pairs_fn, dim3(nrep, bpr2), kAdjLdsBlock, shmem2, c->stream,
+                            base, terms, out_est, 1);
+        } else {
+            rc = launch(c, SPIMC_KERNEL_ADJOINT, c->adj_lds_fn, dim3(nrep, bpr), kAdjLdsBlock, shmem, c->stream, base,
+                        terms, out_est);
+        }
     } else {
         rc = launch(c, SPIMC_KERNEL_ADJOINT, k_adjoint_local, g, 64, 0, c->stream, base, terms, out_est);
     }
@@ -1803,10 +1827,20 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         c->lds_adj_max = c->lds_doubles_max;
         c->adj_lds_fn = vi == 0 ? k_adjoint_local_lds<1, 0> : vi == 1 ? k_adjoint_local_lds<2, 2>
                                                         : vi == 2 ? k_adjoint_local_lds<2, 0> : k_adjoint_local_lds<0, 0>;
+        c->adj_pairs_fn = vi == 0 ? k_adjoint_pairs_lds<1, 0> : vi == 1 ? k_adjoint_pairs_lds<2, 2>
+                                                            : vi == 2 ? k_adjoint_pairs_lds<2, 0> : k_adjoint_pairs_lds<0, 0>;
+        if (const char* fa = std::getenv("SPIMC_ADJ_PAIRS")) c->adj_pairs_env = std::atoi(fa) != 0;
         if (c->lds_adj_max > 0 &&
             hipFuncSetAttribute((const void*)c->adj_lds_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 c->lds_adj_max * (int)sizeof(double)) != hipSuccess)
             c->lds_adj_max = 0;
+        {
+            // the pair-once kernel adds its waves' partner slots to the tables
+            const size_t b2 = ((size_t)c->lds_adj_max + (size_t)kAdjPairWaves * kAdjSlotDoubles) * sizeof(double);
+            c->adj_pairs_ok = c->lds_adj_max > 0 && b2 <= kPairLdsMaxBytes &&
+                              hipFuncSetAttribute((const void*)c->adj_pairs_fn,
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)b2) == hipSuccess;
+        }
         bool whole = c->lds_doubles_max > 0;
         for (auto& h : c->reps) {
             const int need = h.dev.pair_type == SPIMC_PAIR_DAVID ? h.dev.lds_doubles_du : h.dev.lds_doubles;

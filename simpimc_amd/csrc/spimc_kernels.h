@@ -1518,6 +1518,152 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_local_lds(const Rep* _
     }
 }
 
+// dK/dr_i(b) of the two kinetic links at bead (b, i); the links follow the path
+// across the wrap.
+__device__ __forceinline__ void adj_kinetic(const Rep& R, int b, int i, double* g) {
+    const int M = R.M, bn = (b + 1) % M, bp = (b + M - 1) % M;
+    const int kn = next_label(R, b, i), kp = prev_label(R, b, i);
+    for (int d = 0; d < 3; ++d) {
+        const double ri = pos_at(R, d, b, i);
+        const double df = minimg(ri - pos_at(R, d, bn, kn), R.L, R.iL), db = minimg(pos_at(R, d, bp, kp) - ri, R.L, R.iL);
+        double kf = 2. * df * R.kin_i4lt, kb = 2. * db * R.kin_i4lt;
+        if (R.kin_images > 0) {
+            int ia = spl_locate(R.kin[0], df), ib = spl_locate(R.kin[0], db);
+            kf += spl_der(R.kin[0], ia, 0, df - R.kin[0].x[ia]);
+            kb += spl_der(R.kin[0], ib, 0, db - R.kin[0].x[ib]);
+        }
+        g[d] = kf - kb;
+    }
+}
+
+// Adds the short-range pair terms of link (b, b+1) to dS/dr at beads b and b+1,
+// one wave, each pair-link evaluated once for both of its particles.  The
+// particles are taken in tiles of 64.  For a tile pair (I, J >= I), lane l holds
+// particle p = 64I + l and at step s evaluates its link with q = 64J + (l + s)
+// mod 64 (I = J: (l + s) mod n for s = 1 .. n/2, the last step on half the lanes
+// when n is even).  p's share stays in registers; q's, the same vector with the
+// opposite sign, is added to q's slot in the wave's LDS slots (6 x 64 doubles,
+// component-major): at one step the lanes touch distinct slots, and one wave's
+// LDS operations complete in order, so every sum is formed in a fixed order.
+template <int PT, int NORD>
+__device__ void adj_link_pairs(const Rep& R, const double* lds, double* slot, double* __restrict__ o, int b, int lane) {
+    const int M = R.M, N = R.N, b1 = (b + 1) % M, T = (N + kWave - 1) / kWave;
+    for (int I = 0; I < T; ++I) {
+        const int nI = min(kWave, N - I * kWave), p = I * kWave + lane;
+        const bool pv = lane < nI;
+        double a0[3], a1[3], acc[6] = {0., 0., 0., 0., 0., 0.};
+        for (int d = 0; d < 3; ++d) {
+            a0[d] = pv ? pos_at(R, d, b, p) : 0.;
+            a1[d] = pv ? pos_at(R, d, b1, p) : 0.;
+        }
+        for (int J = I; J < T; ++J) {
+            const bool diag = J == I;
+            const int nJ = min(kWave, N - J * kWave);
+            const int s0 = diag ? 1 : 0, s1 = diag ? nI / 2 : kWave - 1;
+            for (int k = 0; k < 6; ++k) slot[k * kWave + lane] = 0.;
+            for (int s = s0; s <= s1; ++s) {
+                const int ql = diag ? (pv ? (lane + s) % nI : lane) : ((lane + s) & (kWave - 1));
+                const bool act = pv && (diag ? !((nI % 2 == 0) && s == nI / 2 && lane >= nI / 2) : ql < nJ);
+                if (!act) continue;
+                const int q = J * kWave + ql;
+                double c0[3], c1[3];
+                for (int d = 0; d < 3; ++d) {
+                    c0[d] = pos_at(R, d, b, q);
+                    c1[d] = pos_at(R, d, b1, q);
+                }
+                // the geometry of link_geom_fast, s = r - r' formed again where it is used
+                double rv[3], rpv[3], rr = 0., pp = 0., ss = 0.;
+                for (int d = 0; d < 3; ++d) {
+                    rv[d] = c0[d] - a0[d];
+                    rpv[d] = c1[d] - a1[d];
+                    rv[d] -= rint(rv[d] * R.iL) * R.L;
+                    rpv[d] += rint((rv[d] - rpv[d]) * R.iL) * R.L;
+                    const double sv = rv[d] - rpv[d];
+                    rr += rv[d] * rv[d];
+                    pp += rpv[d] * rpv[d];
+                    ss += sv * sv;
+                }
+                const double r = sqrt_pos(rr), rp = sqrt_pos(pp), sl = sqrt_pos(ss);
+                double ur, urp, us;
+                // r = r_q(b) - r_p(b): dU/dr_p(b) = -(U_r rhat + U_s shat), dU/dr_p(b+1) = -(U_r' rphat - U_s shat)
+                pair_u_grad_lds<PT, NORD>(R, lds, r, rp, sl, ur, urp, us);
+                const double fr = ur / r, frp = urp / rp, so = (sl > 0.) ? us / sl : 0.;
+                for (int d = 0; d < 3; ++d) {
+                    const double sv = rv[d] - rpv[d];
+                    const double e0 = fr * rv[d] + so * sv, e1 = frp * rpv[d] - so * sv;
+                    acc[d] -= e0;
+                    acc[3 + d] -= e1;
+                    slot[d * kWave + ql] += e0;
+                    slot[(3 + d) * kWave + ql] += e1;
+                }
+            }
+            if (diag) {
+                for (int k = 0; k < 6; ++k) acc[k] += slot[k * kWave + lane];
+            } else if (lane < nJ) {
+                double* o0 = o + 3 * ((size_t)b * N + J * kWave + lane);
+                double* o1 = o + 3 * ((size_t)b1 * N + J * kWave + lane);
+                for (int d = 0; d < 3; ++d) {
+                    o0[d] += slot[d * kWave + lane];
+                    o1[d] += slot[(3 + d) * kWave + lane];
+                }
+            }
+        }
+        if (pv) {
+            double* o0 = o + 3 * ((size_t)b * N + p);
+            double* o1 = o + 3 * ((size_t)b1 * N + p);
+            for (int d = 0; d < 3; ++d) {
+                o0[d] += acc[d];
+                o1[d] += acc[3 + d];
+            }
+        }
+    }
+}
+
+// Analytic adjoint, local terms, each pair-link evaluated once: one 1024-thread
+// workgroup per replica (gridDim.y per replica when replicas are few); wave w
+// owns links [b_lo, b_hi) of n_rng contiguous ranges of at least two links.
+// Phase 0 writes the kinetic term of beads b_lo+1 .. b_hi and adds the pair
+// terms of links b_lo+1 .. b_hi-1, all of whose beads it owns; phase 1, the
+// next launch, adds those of link b_lo, whose bead b_lo the previous range
+// wrote.  Every bead's sum is formed in a fixed order.  Dynamic LDS: the
+// replica's tables, then kAdjSlotDoubles per wave.
+constexpr int kAdjPairWaves = kAdjLdsBlock / kWave;
+constexpr int kAdjSlotDoubles = 6 * kWave;  // one wave's partner slots
+template <int PT, int NORD>
+__global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_pairs_lds(const Rep* __restrict__ reps, unsigned int terms,
+                                                                    int out_est, int phase) {
+    extern __shared__ double lds[];
+    const Rep R = reps[blockIdx.x];
+    const bool pairs = (terms & 2u) && R.pair_type != 0 && R.N > 1;
+    if (phase == 1 && !pairs) return;
+    if (pairs) stage_tables(R, lds, false);
+    const int M = R.M, N = R.N;
+    const int n_rng = max(1, min(kAdjPairWaves * (int)gridDim.y, M / 2));
+    const int w = blockIdx.y * kAdjPairWaves + threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    if (w >= n_rng) return;
+    const int b_lo = (int)((long long)M * w / n_rng), b_hi = (int)((long long)M * (w + 1) / n_rng);
+    double* o = out_est ? R.grad_est : R.grad;
+    // the wave's partner slots follow the replica's tables
+    double* slot = lds + R.lds_doubles + (threadIdx.x / kWave) * kAdjSlotDoubles;
+    if (phase == 1) {
+        adj_link_pairs<PT, NORD>(R, lds, slot, o, b_lo, lane);
+        return;
+    }
+    for (int b = b_lo + 1; b <= b_hi; ++b) {
+        const int bb = b % M;
+        for (int p = lane; p < N; p += kWave) {
+            double g[3] = {0., 0., 0.};
+            if (terms & 1u) adj_kinetic(R, bb, p, g);
+            double* op = o + 3 * ((size_t)bb * N + p);
+            op[0] = g[0];
+            op[1] = g[1];
+            op[2] = g[2];
+        }
+    }
+    if (pairs)
+        for (int b = b_lo + 1; b < b_hi; ++b) adj_link_pairs<PT, NORD>(R, lds, slot, o, b, lane);
+}
+
 // Whole-path pair sums with LDS tables (which 0: CalcdUdBeta, 1: CalcV on plain
 // minimum images): one 1024-thread workgroup per replica, per-slice partials.
 __global__ __launch_bounds__(kAdjLdsBlock) void k_pair_whole_lds(const Rep* __restrict__ reps, int which, int slot) {
