@@ -34,7 +34,10 @@ def parse():
     ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs of 16 rounds (about 2 %% "
                     "faster here; rocprofv3 kernel traces of graph replays are not taken, so the default is eager)")
     ap.add_argument("--plain-attempts", type=int, default=1024)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-sweeps", type=int, default=1,
+                    help="whole sweeps of the CPU oracle's moves on all host cores (the estimators run once)")
+    ap.add_argument("--cpu-one-thread-seconds", type=float, default=4.0)
+    ap.add_argument("--cpu-only", action="store_true", help="only the CPU baseline (no GPU work)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the engine's default launch shape instead of autotuning it before the warm-up")
     ap.add_argument("--no-cpu", action="store_true")
@@ -54,6 +57,9 @@ def parse():
     ap.add_argument("--no-extra", dest="extra", action="store_false",
                     help="skip the uneven-batch (64 replicas) and large-case (8 x N=1024, M=256) sub-lines of the "
                          "default run")
+    ap.add_argument("--c5-rounds", type=int, default=0,
+                    help="with --c5: time this many bisect rounds (after as many untimed ones) instead of whole "
+                         "sweeps, for hardware-counter passes over the rho_k kernel")
     ap.add_argument("--c5", action="store_true",
                     help="time only the large case (8 replicas of N=1024, M=256, 4012 k-vectors) and print that line")
     ap.add_argument("--copy-mib", type=int, default=4096,
@@ -93,36 +99,89 @@ def copy_ceiling(mib: int) -> float:
     return gbps
 
 
-def cpu_baseline(target_s: float):
-    """Oracle (CPU restatement) on a bounded sample of the same workload: one C2
-    replica, bisection and displace attempts in the sweep's 8:1 ratio."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sweeps: int, one_thread_s: float):
+    """The CPU oracle built with the reference's release flags (-O3 -ffast-math
+    -fopenmp, reference CMakeLists.txt:64-66) on one replica of the same
+    workload: `sweeps` whole sweeps of moves (2048 bisect + 256 displace
+    attempts each) on every host core, then the sweep's estimators timed apart
+    (virial energy with the reference's finite-difference GetActionGradient,
+    S(k), g(r) with 200 bins), and a bounded sample of the moves on one thread.
+    `value` is a whole sweep (moves and estimators) per its time, as on the GPU."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi
-    from simpimc_amd import system
+    from simpimc_amd import abi, system
 
-    threads = len(os.sched_getaffinity(0))
-    oracle_ffi.set_threads(threads)
+    oracle_ffi.use_timing_build()
+    cores = len(os.sched_getaffinity(0))
     s = system.config_c2()
-    o = oracle_ffi.Oracle(s, seed=17)
-    o.upload(system.random_path(s, seed=17, spread=0.05))
-    o.bisect(64)  # warm caches and OpenMP pools
-    o.displace(8)
-    u0 = o.bead_updates()
+    kc = 0.95 * float(system.kmag(s.L, s.k_n).max())
+
+    def fresh(threads):
+        oracle_ffi.set_threads(threads)
+        o = oracle_ffi.Oracle(s, seed=17)
+        o.upload(system.random_path(s, seed=17, spread=0.05))
+        o.obs_configure(energy=abi.ENERGY_VIRIAL_COMPAT, sk_k_cut=kc, gr_n_r=200)
+        o.bisect(64)  # warm caches and OpenMP pools
+        o.displace(8)
+        return o
+
+    o = fresh(cores)
+    per_sweep = 0
+    t_moves = 0.0
+    for k in range(sweeps):
+        u0 = o.bead_updates()
+        t0 = time.perf_counter()
+        o.sweep(1, 0)
+        t_moves += time.perf_counter() - t0
+        per_sweep = o.bead_updates() - u0
+        print(f"cpu oracle: sweep {k + 1}/{sweeps} moves done ({t_moves:.1f} s)", file=sys.stderr, flush=True)
+    est = {}
+    for which, name in ((abi.OBS_ENERGY, "virial_energy_fd_gradient"), (abi.OBS_SK, "sk"), (abi.OBS_GR, "gr")):
+        t0 = time.perf_counter()
+        o.obs_accumulate(which)
+        est[name] = time.perf_counter() - t0
+    print(f"cpu oracle: estimators {est}", file=sys.stderr, flush=True)
+    sweep_moves_s = t_moves / sweeps
+    sweep_s = sweep_moves_s + sum(est.values())
+
+    o1 = fresh(1)
+    u0 = o1.bead_updates()
     t0 = time.perf_counter()
     n_b = n_d = 0
-    while time.perf_counter() - t0 < target_s:
-        o.bisect(64)
-        o.displace(8)
+    while time.perf_counter() - t0 < one_thread_s:
+        o1.bisect(64)
+        o1.displace(8)
         n_b += 64
         n_d += 8
-    dt = time.perf_counter() - t0
-    du = o.bead_updates() - u0
+    dt1 = time.perf_counter() - t0
+    du1 = o1.bead_updates() - u0
+    print(f"cpu oracle, one thread: {du1 / dt1:.4g} bead updates/s over the moves", file=sys.stderr, flush=True)
     return {
-        "value": du / dt,
+        "value": per_sweep / sweep_s,
         "unit": "bead updates/s",
-        "cores": threads,
+        "cores": cores,
         "kind": "port",
-        "sample": f"1 replica of the same workload, {n_b} bisect + {n_d} displace attempts ({du} bead updates) in {dt:.1f} s",
+        "build": "-O3 -ffast-math -fopenmp (the oracle's timing build)",
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "sample": (f"1 replica of the same workload: {sweeps} whole sweep(s) of moves on {cores} threads "
+                   f"({per_sweep} bead updates each), then one sweep's estimators"),
+        "moves_only": per_sweep / sweep_moves_s,
+        "sweep_moves_s": sweep_moves_s,
+        "estimators_s": est,
+        "one_thread": {"moves_only": du1 / dt1,
+                       "sample": f"{n_b} bisect + {n_d} displace attempts ({du1} bead updates) in {dt1:.1f} s"},
     }
 
 
@@ -140,7 +199,11 @@ def main():
         print(json.dumps(uneven_bench(a, abi, system)))
         return
     if a.c5:
-        print(json.dumps(c5_bench(a, abi, system, None)))
+        print(json.dumps(c5_bench(a, abi, system, copy_ceiling(a.copy_mib) if a.copy_mib else None,
+                                  rounds=a.c5_rounds)))
+        return
+    if a.cpu_only:
+        print(json.dumps({"cpu_baseline": cpu_baseline(a.cpu_sweeps, a.cpu_one_thread_seconds)}))
         return
     copy_gbps = copy_ceiling(a.copy_mib) if a.copy_mib else None
     base = system.config_c2()
@@ -242,7 +305,7 @@ def main():
     if a.extra:
         extra_uneven = uneven_bench(a, abi, system, steps=5)
         extra_c5 = c5_bench(a, abi, system, copy_gbps, steps=2)
-    cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
+    cpu = None if a.no_cpu else cpu_baseline(a.cpu_sweeps, a.cpu_one_thread_seconds)
     out = {
         "metric": METRIC,
         "value": value,
@@ -372,7 +435,7 @@ def uneven_bench(a, abi, system, steps=None):
     }
 
 
-def c5_bench(a, abi, system, copy_gbps, steps=None):
+def c5_bench(a, abi, system, copy_gbps, steps=None, rounds=0):
     """The combined large case: 8 replicas of HEG N=1024, M=256, r_s=1, 4012
     stored k-vectors, David n_order=2, batched with the adjoint, the virial
     energy from it, S(k) on all k-vectors and g(r) on 200 bins every sweep.
@@ -388,15 +451,27 @@ def c5_bench(a, abi, system, copy_gbps, steps=None):
     eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, sk_k_cut=1e300, gr_n_r=200)
     flags = abi.SWEEP_ADJOINT | abi.SWEEP_ENERGY | abi.SWEEP_SK | abi.SWEEP_GR
     eng.timing(abi.TIMING_CLOCK)
-    for i in range(max(1, a.warmup)):
-        eng.sweep(1, flags)
-        print(f"c5 warmup sweep {i + 1} done", file=sys.stderr, flush=True)
-    eng.timing_reset()
-    u0 = eng.bead_updates()
-    t0 = time.perf_counter()
-    for i in range(steps):
-        eng.sweep(1, flags)
-        print(f"c5 timed sweep {i + 1}/{steps} done at {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    if rounds:
+        # bisect rounds only: every rho_k launch updates the same number of slices
+        eng.bisect(rounds)
+        print(f"c5 {rounds} untimed bisect rounds done", file=sys.stderr, flush=True)
+        eng.timing_reset()
+        u0 = eng.bead_updates()
+        t0 = time.perf_counter()
+        eng.bisect(rounds)
+        steps = 1
+        print(f"c5 {rounds} timed bisect rounds done", file=sys.stderr, flush=True)
+    else:
+        for i in range(max(1, a.warmup)):
+            eng.sweep(1, flags)
+            print(f"c5 warmup sweep {i + 1} done", file=sys.stderr, flush=True)
+        eng.timing_reset()
+        u0 = eng.bead_updates()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            eng.sweep(1, flags)
+            print(f"c5 timed sweep {i + 1}/{steps} done at {time.perf_counter() - t0:.2f} s", file=sys.stderr,
+                  flush=True)
     dt = time.perf_counter() - t0
     value = (eng.bead_updates() - u0) / dt
     rho_ms, rho_n, rho_bytes = eng.kernel_timing(abi.KERNEL_RHO)
@@ -409,7 +484,9 @@ def c5_bench(a, abi, system, copy_gbps, steps=None):
         "ms_per_step": 1e3 * dt / steps, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"HEG N=1024 M=256 r_s=1 tau=0.02, David n_order=2, Ewald {base.n_k} stored k-vectors, "
                                f"bisect level 4 + displace; {R} replicas, adjoint, virial energy, S(k) on all "
-                               f"k-vectors and g(r) on 200 bins every sweep", "replicas": R, "launch_shape": shape},
+                               f"k-vectors and g(r) on 200 bins every sweep"
+                               + (f" (timed: {rounds} bisect rounds, not whole sweeps)" if rounds else ""),
+                   "replicas": R, "launch_shape": shape},
         "rho_k_update_GBps": ach,
         "roofline": ({"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                       "frac": ach / HBM_PEAK_GBPS, "kernel": "k_rho (incremental rho_k update + long-range action)",

@@ -14,6 +14,8 @@ from simpimc_amd import abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "oracle", "libspimc_oracle.so")
+# the same source built with -O3 -ffast-math: for timing the CPU baseline only
+LIB_FAST = os.path.join(ROOT, "oracle", "libspimc_oracle_fast.so")
 
 _dp, _ip = abi._dp, abi._ip
 _c, _u32, _i32, _u64, _d = C.c_void_p, C.c_uint32, C.c_int, C.c_uint64, C.c_double
@@ -82,6 +84,15 @@ def lib():
             f.restype = r
         _lib = L
     return _lib
+
+
+def use_timing_build():
+    """Loads the -O3 -ffast-math build instead of the checker build; must come
+    before the first call into the oracle in this process."""
+    global LIB
+    if _lib is not None and LIB != LIB_FAST:
+        raise RuntimeError("the checker build of the oracle is already loaded")
+    LIB = LIB_FAST
 
 
 def check(rc):
