@@ -341,6 +341,15 @@ struct Oracle {
     std::vector<uint32_t> cyc;
     // PermBisect (perm_bisect_class.h:20-29, perm_bisect_iterative_class.h)
     double perm_log_epsilon = std::log(1.e-100);
+    double perm_epsilon = 1.e-100;
+    // PermBisectTable's n_perm_part (perm_bisect_table_class.h:235); 0 = not configured
+    uint32_t table_k = 0;
+    // the final table check: 0 the reference's perm_tot_0 / perm_tot_1
+    // (perm_bisect_table_class.h:223); 1 perm_tot_0 / (perm_tot_1 w), w the
+    // selected entry's weight, which detailed balance asks for when n_perm_part
+    // < N (the reverse move selects the inverse entry, of weight 1 / w, from the
+    // new table)
+    int table_balance = 0;
     uint32_t perm_n_level = 0;  // the permuting move's own level (its Bisect base, bisect_class.h:168)
     std::vector<uint64_t> perm_attempt, perm_accept;  // per cycle length - 1
     uint64_t perm_counters[2] = {0, 0};               // Move::n_attempt, n_accept
@@ -1246,6 +1255,152 @@ struct Oracle {
         }
         att_on = false;
     }
+    // ---- PermBisectTable (perm_bisect_table_class.h:10-228) ------------------
+    // The cycle list (BuildCycles, :54-102) is every permutation type of k =
+    // n_perm_part particles (scaffold's GenPerm) crossed with every k-subset of
+    // the particles (GenCombPermK without repetition or order), type-major.  The
+    // scaffold submodule is not in the reference tree, so the order within both
+    // lists is this code's: permutations of 0..k-1 and subsets (sorted members)
+    // in lexicographic order.  The order decides which entry a draw selects, not
+    // which distribution is sampled.
+    static bool next_subset(std::vector<uint32_t>& s, uint32_t n) {
+        const int k = (int)s.size();
+        int i = k - 1;
+        while (i >= 0 && s[i] == n - (uint32_t)k + (uint32_t)i) --i;
+        if (i < 0) return false;
+        ++s[i];
+        for (int j = i + 1; j < k; ++j) s[j] = s[j - 1] + 1;
+        return true;
+    }
+    // UpdatePermTable (:31-51) in the current mode: t(i, j) from the bead
+    // (i, bead0) to the bead n_bisect_beads links after (j, bead0)
+    void table_update(uint32_t bead0, std::vector<double>& t) {
+        t.assign((size_t)N * N, 0.);
+        const double i4ltn = 1. / (4. * lambda * tau) / n_bisect_beads;
+        std::vector<Bd> b1(N);
+        for (uint32_t p = 0; p < N; ++p) b1[p] = next(Bd{p, bead0}, n_bisect_beads);
+        for (uint32_t i = 0; i < N; ++i) {
+            const double* a = R(i, bead0);
+            double d[3];
+            dr(a, R(b1[i]), d);
+            const double s_ii = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            for (uint32_t j = 0; j < N; ++j) {
+                dr(a, R(b1[j]), d);
+                const double s_ij = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                const double exponent = (-s_ij + s_ii) * i4ltn;
+                t[(size_t)i * N + j] = exponent > perm_log_epsilon ? std::exp(exponent) : 0.;
+            }
+        }
+    }
+    // ConstructPermTable (:10-28): the running total of the cycle weights above
+    // epsilon, in list order.  With select, the entry whose running total first
+    // exceeds x (SelectCycle, :105-119, a bisection over the same running
+    // totals): its subset and permutation type; false if none does.
+    double table_total(const std::vector<double>& t, bool select, double x, std::vector<uint32_t>& part,
+                       std::vector<uint32_t>& type, double& weight, bool& found) {
+        const uint32_t k = table_k;
+        std::vector<uint32_t> pt(k), sub(k);
+        for (uint32_t i = 0; i < k; ++i) pt[i] = i;
+        double total = 0.;
+        found = false;
+        std::vector<uint32_t> last_part, last_type;
+        double last_w = 0.;
+        do {
+            for (uint32_t i = 0; i < k; ++i) sub[i] = i;
+            do {
+                double w = 1.;
+                for (uint32_t i = 0; i < k; ++i) w *= t[(size_t)sub[i] * N + sub[pt[i]]];
+                if (w > perm_epsilon) {
+                    total += w;
+                    if (select) {
+                        last_part = sub;
+                        last_type = pt;
+                        last_w = w;
+                        if (total > x) {
+                            part = sub;
+                            type = pt;
+                            weight = w;
+                            found = true;
+                            return total;
+                        }
+                    }
+                }
+            } while (next_subset(sub, N));
+        } while (std::next_permutation(pt.begin(), pt.end()));
+        if (select && !found && !last_part.empty()) {
+            // x at or past the total: the reference's bisection would index past
+            // its list.  This engine takes the last entry (free-running rounding);
+            // a replay tape reports an error.
+            part = last_part;
+            type = last_type;
+            weight = last_w;
+        }
+        return total;
+    }
+    // PermBisectTable::Attempt (:122-228).  The particle list given to the
+    // actions is that of PermBisectIterative (perm_bisect_iterative_class.h:139-146):
+    // on a window that crosses the wrap it also lists the labels the chains carry
+    // past it.  The reference's table move lists the cycle's own labels only
+    // (:133-141), which leaves the moved beads under the other labels out of the
+    // pair action and of Species::UpdateRhoK (species_class.h:406-424).
+    bool perm_table_attempt(uint32_t& n_chain_out) {
+        uint32_t bead0 = draw_index(M);
+        last_bead0 = bead0;
+        uint32_t bead1 = bead0 + n_bisect_beads;
+        bool roll_over = bead1 > M - 1;
+        const uint32_t k = table_k;
+        n_chain_out = k;
+        mode = SPIMC_MODE_NEW;
+        std::vector<double> t;
+        table_update(bead0, t);
+        std::vector<uint32_t> part, type;
+        double weight = 0.;
+        bool found = false;
+        const double tot0 = table_total(t, false, 0., part, type, weight, found);
+        const double x = tot0 * draw();  // UnifRand(0, perm_tot_0)
+        table_total(t, true, x, part, type, weight, found);
+        if (!found) {
+            if (use_tape || part.empty()) {
+                g_err = "permutation table selected no cycle";
+                return false;
+            }
+        }
+        std::vector<Bd> start(k), fm1(k);
+        std::vector<uint32_t> parts;
+        for (uint32_t i = 0; i < k; ++i) {
+            start[i] = Bd{part[i], bead0};
+            fm1[i] = next(start[i], n_bisect_beads - 1);
+            parts.push_back(part[i]);
+        }
+        if (roll_over) {
+            for (uint32_t i = 0; i < k; ++i) parts.push_back(next1(fm1[i]).p);
+            std::sort(parts.begin(), parts.end());
+            parts.erase(std::unique(parts.begin(), parts.end()), parts.end());
+        }
+        // PermuteBeads: bead fm1(i) is followed by the old successor of fm1(type(i))
+        cyc.resize(N);
+        for (uint32_t q = 0; q < N; ++q) cyc[q] = q;
+        for (uint32_t i = 0; i < k; ++i) cyc[fm1[i].p] = fm1[type[i]].p;
+        att_e = fm1[0].b;
+        att_on = true;
+        affected.clear();
+        for (uint32_t i = 0; i < k; ++i) {
+            Bd a = next1(start[i]);
+            for (uint32_t j = 1; j < n_bisect_beads; ++j, a = next1(a)) affected.push_back(a);
+        }
+        const double w_sel = weight;
+        if (!bisect_levels(start, bead0, bead1, parts, -std::log(w_sel))) return false;
+        // the neighbourhood changed: the table of the new path (:216-225)
+        if (k < N) {
+            mode = SPIMC_MODE_NEW;
+            table_update(bead0, t);
+            const double tot1 = table_total(t, false, 0., part, type, weight, found);
+            const double ratio = table_balance ? tot0 / (tot1 * w_sel) : tot0 / tot1;
+            if (ratio < draw()) return false;
+        }
+        return true;
+    }
+
     void perm_bisect_reject(uint32_t bead0, uint32_t n) {  // PermBisect::Reject (perm_bisect_class.h:76-90)
         att_on = false;
         if (n > 0) restore_affected();
@@ -1880,6 +2035,53 @@ int spimc_oracle_perm_bisect_stage(void* h, uint32_t n_attempts, uint8_t* accept
     return g_err.empty() ? SPIMC_OK : SPIMC_ERR_STATE;
 }
 
+// PermBisectTable's stage: the same counters and commits as the iterative move;
+// every attempt reaches the bisection with all k chains (n_perm_part =
+// c->part.size(), perm_bisect_table_class.h:134), so perm_attempt counts at k - 1.
+int spimc_oracle_perm_table_stage(void* h, uint32_t n_attempts, uint8_t* accepted) {
+    Oracle* o = (Oracle*)h;
+    g_err.clear();
+    if (o->table_k == 0) return fail(SPIMC_ERR_STATE, "permutation table not configured");
+    const uint32_t saved_level = o->n_level;
+    o->n_level = o->perm_n_level;
+    o->n_bisect_beads = 1u << o->n_level;
+    for (uint32_t i = 0; i < n_attempts; ++i) {
+        uint32_t n = 0;
+        o->perm_counters[0]++;
+        o->trace_begin();
+        bool ok = o->perm_table_attempt(n);
+        o->trace_end();
+        if (ok) {
+            o->perm_counters[1]++;
+            o->perm_attempt[n - 1]++;
+            o->perm_accept[n - 1]++;
+            o->perm_bisect_accept(o->last_bead0, n);
+        } else {
+            o->perm_attempt[n - 1]++;
+            o->perm_bisect_reject(o->last_bead0, n);
+        }
+        o->mode = SPIMC_MODE_NEW;
+        o->bead_updates += (uint64_t)n * (o->n_bisect_beads - 1);
+        if (accepted) accepted[i] = ok ? 1 : 0;
+        if (!g_err.empty()) break;
+    }
+    o->n_level = saved_level;
+    o->n_bisect_beads = 1u << o->n_level;
+    return g_err.empty() ? SPIMC_OK : SPIMC_ERR_STATE;
+}
+
+// PermBisectTable's n_perm_part (perm_bisect_table_class.h:235) and the rule of
+// its final table check (0 the reference's, 1 detailed balance)
+int spimc_oracle_perm_table_configure(void* h, uint32_t n_perm_part, uint32_t balance) {
+    Oracle* o = (Oracle*)h;
+    if (n_perm_part < 1 || n_perm_part > o->N || n_perm_part > 6)
+        return fail(SPIMC_ERR_ARG, "n_perm_part must be in 1 .. min(n_part, 6)");
+    if (balance > 1) return fail(SPIMC_ERR_ARG, "balance must be 0 or 1");
+    o->table_k = n_perm_part;
+    o->table_balance = (int)balance;
+    return SPIMC_OK;
+}
+
 // out[0] attempts, out[1] accepts, out[2 + c] perm_attempt and out[2 + N + c]
 // perm_accept of cycle length c + 1 (perm_bisect_class.h:25-26,66-87)
 int spimc_oracle_perm_counters(void* h, uint64_t* out) {
@@ -1899,6 +2101,7 @@ int spimc_oracle_perm_configure(void* h, uint32_t n_level, double epsilon) {
     if (n_level < 1 || n_level > (uint32_t)std::floor(std::log2((double)o->M)))
         return fail(SPIMC_ERR_ARG, "permuting bisection n_level out of range");
     o->perm_log_epsilon = std::log(epsilon);
+    o->perm_epsilon = epsilon;
     o->perm_n_level = n_level;
     return SPIMC_OK;
 }

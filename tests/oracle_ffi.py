@@ -64,6 +64,8 @@ SIG = {
     "spimc_oracle_perm_bisect_stage": ([_c, _u32, _pu8], _i32),
     "spimc_oracle_perm_counters": ([_c, _pu64], _i32),
     "spimc_oracle_perm_configure": ([_c, _u32, _d], _i32),
+    "spimc_oracle_perm_table_configure": ([_c, _u32, _u32], _i32),
+    "spimc_oracle_perm_table_stage": ([_c, _u32, _pu8], _i32),
     "spimc_oracle_perm_get": ([_c, _pu32], _i32),
     "spimc_oracle_perm_set": ([_c, _pu32], _i32),
     "spimc_oracle_cycle_lengths": ([_c, _pu32, _pu32], _i32),
@@ -278,6 +280,14 @@ class Oracle:
     def perm_bisect(self, n):
         acc = np.zeros(max(n, 1), dtype=np.uint8)
         check(self.L.spimc_oracle_perm_bisect_stage(self.h, n, acc.ctypes.data_as(_pu8)))
+        return acc[:n]
+
+    def perm_table_configure(self, n_perm_part, balance=0):
+        check(self.L.spimc_oracle_perm_table_configure(self.h, n_perm_part, balance))
+
+    def perm_table(self, n):
+        acc = np.zeros(max(n, 1), dtype=np.uint8)
+        check(self.L.spimc_oracle_perm_table_stage(self.h, n, acc.ctypes.data_as(_pu8)))
         return acc[:n]
 
     def perm_counters(self):
