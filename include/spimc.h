@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SPIMC_ABI_VERSION 3
+#define SPIMC_ABI_VERSION 4
 
 /* status codes */
 #define SPIMC_OK 0
@@ -346,6 +346,27 @@ int spimc_perm_configure(spimc_ctx* ctx, uint32_t replica, uint32_t n_level, dou
  * normals chain by chain (3 per midpoint in path order) and one uniform.
  * Replicas hold at most 4096 particles. */
 int spimc_perm_bisect_stage(spimc_ctx* ctx, const uint32_t* n_attempts, uint32_t n_uniform, uint8_t* accepted);
+/* PermBisectTable's attribute n_perm_part (perm_bisect_table_class.h:235) for
+ * one replica, 1 <= n_perm_part <= min(n_part, 6), at most 1024 particles and
+ * 2^28 cycle-list entries (n_part! / (n_part - n_perm_part)!).  balance selects
+ * the final table check (perm_bisect_table_class.h:216-225): 0 the reference's,
+ * accept if perm_tot_0 / perm_tot_1 >= u; 1 perm_tot_0 / (perm_tot_1 w) >= u,
+ * w the selected entry's weight, which detailed balance asks for when
+ * n_perm_part < n_part.  Uses spimc_perm_configure's n_level and epsilon.
+ * Synchronises the device. */
+int spimc_perm_table_configure(spimc_ctx* ctx, uint32_t replica, uint32_t n_perm_part, uint32_t balance);
+/* n_attempts[r] consecutive Move::DoEvent calls of PermBisectTable
+ * (perm_bisect_table_class.h:122-228, perm_bisect_class.h:59-90) on every
+ * replica; arguments as spimc_bisect_stage.  The cycle list is every
+ * permutation of n_perm_part particles (lexicographic) crossed with every
+ * n_perm_part-subset of the particles (lexicographic), permutation-major.
+ * Replay draws per attempt: bead0 (an integer as a double), the selection
+ * uniform u (the entry whose running total of weights first exceeds u times
+ * their total), per level the chains' normals chain by chain (3 per midpoint
+ * in path order) and one uniform, then, after an accepted level 0 and when
+ * n_perm_part < n_part, the table check's uniform.  Counters as
+ * spimc_perm_counters, at cycle length n_perm_part. */
+int spimc_perm_table_stage(spimc_ctx* ctx, const uint32_t* n_attempts, uint32_t n_uniform, uint8_t* accepted);
 /* sigma[p] for p < n_part: PathDump's permutation row 1 (path_dump_class.h:43-47);
  * upload is the Restart path of species_class.h:366-374.  Upload checks that
  * sigma is a permutation. */

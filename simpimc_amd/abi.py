@@ -128,6 +128,8 @@ SIGNATURES = {
     "spimc_action_trace_read": ([_c, _u32, _u32, _dp], _i32),
     "spimc_perm_configure": ([_c, _u32, _u32, _d], _i32),
     "spimc_perm_bisect_stage": ([_c, _pu32, _u32, _pu8], _i32),
+    "spimc_perm_table_configure": ([_c, _u32, _u32, _u32], _i32),
+    "spimc_perm_table_stage": ([_c, _pu32, _u32, _pu8], _i32),
     "spimc_perm_download": ([_c, _u32, _pu32], _i32),
     "spimc_perm_upload": ([_c, _u32, _pu32], _i32),
     "spimc_perm_counters": ([_c, _u32, _pu64], _i32),
@@ -331,6 +333,12 @@ class Engine:
 
     def perm_bisect(self, n):
         return self._stage(self.lib.spimc_perm_bisect_stage, n)
+
+    def perm_table_configure(self, n_perm_part: int, balance: int = 0, r: int = 0):
+        check(self.lib.spimc_perm_table_configure(self.h, r, n_perm_part, balance))
+
+    def perm_table(self, n):
+        return self._stage(self.lib.spimc_perm_table_stage, n)
 
     def perm(self, r: int = 0) -> np.ndarray:
         out = np.zeros(self.systems[r].n_part, dtype=np.uint32)

@@ -78,6 +78,11 @@ struct Prop {
     int e_slice;  // slice after which a permuting bisection relinks the chains
     int n_level;  // bisection level of a permuting bisection
     int rho_multi;  // 1: k_rho_multi updates rho_k (several beads per slice); 0: the k_rho items
+    // PermBisectTable's final table check, after a level-0 accept (tab_on): reject
+    // if tab_ratio < u, u the next tape value or u_tab (device RNG)
+    int tab_on;
+    double tab_ratio;
+    double u_tab;
 };
 
 struct Counters {
@@ -186,11 +191,17 @@ struct Rep {
     // permuting bisection (nullptr until the first permuting stage)
     int* ch_lbl;         // [N] chain labels before the wrap, in cycle order
     int* ch_lbl2;        // [N] chain labels after the wrap
+    int* ch_nxt;         // [N] NEW mode: chain i ends at the old end of chain ch_nxt[i]
+    int* ch_lead;        // [N] 1: chain i is the first of its cycle under ch_nxt
     int* mv_pre;         // [N] chain of a label before the wrap, -1 if none
     int* mv_post;        // [N] chain of a label after the wrap, -1 if none
     double* ch_new;      // [N][M - 1][3] NEW positions of the chains' moved beads
     unsigned long long* perm_cnt;  // [2 + 2N] attempts, accepts, perm_attempt[N], perm_accept[N]
     double perm_log_eps;            // log of the table cutoff epsilon
+    double perm_eps;                // the cutoff epsilon
+    double* perm_t;                 // [2][N][N] PermBisectTable's table (old path, new path)
+    int tab_k;                      // PermBisectTable's n_perm_part (0: not configured)
+    int tab_balance;                // its final check: 0 the reference's, 1 detailed balance
     int perm_n_level;               // bisection level of the permuting move
     int multi_cap;                  // replicas of the group: the second list's offset
     // per group: the proposals of a permuting round that the label-general pair

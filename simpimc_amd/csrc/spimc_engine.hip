@@ -69,6 +69,9 @@ constexpr int kRhoKptMin = 2;     // smallest chunking: bounds the number of k c
 constexpr int kNumCUs = 256;      // MI355X compute units (for workgroups-per-replica choices)
 // fewest waves for which the pair-once adjoint beats the per-particle one
 constexpr long long kAdjPairsMinWaves = 4096;
+// PermBisectTable: particles of a replica (two N x N tables) and cycle-list entries
+constexpr int kTabMaxN = 1024;
+constexpr double kTabMaxEntries = 268435456.;
 constexpr int kBuildKpt = 2;
 constexpr int kRoundsPerGraph = 16;
 constexpr int kAdjMaxIdx = 40;
@@ -266,6 +269,7 @@ struct RepHost {
     int n_chunks = 1;
     char* perm_arena = nullptr;    // perm, iperm
     char* perm_scratch = nullptr;  // the permuting move's chain arrays and counters
+    double* perm_t = nullptr;      // PermBisectTable's two N x N tables
 };
 
 // Replicas with identical constant tables (k-set, Ewald coefficients, pair and
@@ -404,6 +408,7 @@ struct spimc_ctx {
     // GPU enough waves; SPIMC_ADJ_PAIRS = 0 / 1 forces the choice
     void (*adj_pairs_fn)(const Rep*, unsigned int, int, int) = nullptr;
     int adj_pairs_env = -1;
+    bool perm_table = false;  // the running permuting stage is PermBisectTable's
     bool adj_pairs_ok = false;  // its tables and slots fit in LDS
     int groups_wanted = 1;
     bool fork = false;  // rho_k beside the pair kernel on a second stream per group
@@ -820,6 +825,7 @@ int build_replica(spimc_ctx* c, uint32_t r, const spimc_system_desc& s, uint64_t
     R.n_level = s.bisect_n_level;
     R.perm_n_level = s.bisect_n_level;
     R.perm_log_eps = std::log(1.e-100);  // PermBisect epsilon default (perm_bisect_class.h:104)
+    R.perm_eps = 1.e-100;
     R.pair_type = s.pair_type;
     R.use_long_range = lr ? 1 : 0;
     R.is_coulomb = (s.pair_type == SPIMC_PAIR_BARE && s.bare_is_coulomb) ? 1 : 0;
@@ -1221,7 +1227,8 @@ int launch_perm_round(spimc_ctx* c, const Group& G) {
     int rc;
     const bool tuned = c->lds_doubles_max > 0;
     const size_t shmem = (size_t)c->max_N * (sizeof(double) + sizeof(int));
-    if ((rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_perm, R, kPermBlock, shmem, st, reps, R, (int)tuned,
+    if ((rc = launch(c, SPIMC_KERNEL_PROPOSE, c->perm_table ? k_propose_perm<true> : k_propose_perm<false>, R,
+                     kPermBlock, shmem, st, reps, R, (int)tuned,
                      c->perm_stage)))
         return rc;
     const bool fork = c->any_long && G.side;
@@ -1420,15 +1427,17 @@ int ensure_perm(spimc_ctx* c, bool scratch) {
             auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
             const size_t b_int = up(N * sizeof(int)), b_cnt = up((2 + 2 * N) * sizeof(unsigned long long));
             const size_t b_new = up(N * (M - 1) * 3 * sizeof(double));
-            HIPCHK(hipMalloc((void**)&h.perm_scratch, 4 * b_int + b_cnt + b_new));
+            HIPCHK(hipMalloc((void**)&h.perm_scratch, 6 * b_int + b_cnt + b_new));
             char* q = h.perm_scratch;
             R.ch_lbl = reinterpret_cast<int*>(q);
             R.ch_lbl2 = reinterpret_cast<int*>(q + b_int);
             R.mv_pre = reinterpret_cast<int*>(q + 2 * b_int);
             R.mv_post = reinterpret_cast<int*>(q + 3 * b_int);
-            R.perm_cnt = reinterpret_cast<unsigned long long*>(q + 4 * b_int);
-            R.ch_new = reinterpret_cast<double*>(q + 4 * b_int + b_cnt);
-            HIPCHK(hipMemset(q, 0, 4 * b_int + b_cnt));
+            R.ch_nxt = reinterpret_cast<int*>(q + 4 * b_int);
+            R.ch_lead = reinterpret_cast<int*>(q + 5 * b_int);
+            R.perm_cnt = reinterpret_cast<unsigned long long*>(q + 6 * b_int);
+            R.ch_new = reinterpret_cast<double*>(q + 6 * b_int + b_cnt);
+            HIPCHK(hipMemset(q, 0, 6 * b_int + b_cnt));
             HIPCHK(hipMemset(R.mv_pre, 0xff, 2 * b_int));  // -1: no chain
             changed = true;
         }
@@ -1913,6 +1922,7 @@ int spimc_ctx_destroy(spimc_ctx* c) {
         if (h.gr_cnt) (void)hipFree(h.gr_cnt);
         if (h.perm_arena) (void)hipFree(h.perm_arena);
         if (h.perm_scratch) (void)hipFree(h.perm_scratch);
+        if (h.perm_t) (void)hipFree(h.perm_t);
     }
     for (void* p : {(void*)c->d_reps, (void*)c->d_counts, (void*)c->d_work, (void*)c->d_gr_rmax, (void*)c->d_error,
                     (void*)c->d_tmp, (void*)c->d_items_tmp})
@@ -2315,6 +2325,7 @@ int spimc_perm_configure(spimc_ctx* c, uint32_t r, uint32_t n_level, double epsi
     HIPCHK(hipDeviceSynchronize());
     h.dev.perm_n_level = (int)n_level;
     h.dev.perm_log_eps = std::log(epsilon);
+    h.dev.perm_eps = epsilon;
     return upload_reps(c);
 }
 
@@ -2322,7 +2333,43 @@ int spimc_perm_bisect_stage(spimc_ctx* c, const uint32_t* n_attempts, uint32_t n
     if (!c) return fail(SPIMC_ERR_ARG, "null context");
     int rc = ensure_perm(c, true);
     if (rc) return rc;
+    c->perm_table = false;
     return run_stage(c, 3, n_attempts, n_uniform, accepted, false);
+}
+
+int spimc_perm_table_configure(spimc_ctx* c, uint32_t r, uint32_t n_perm_part, uint32_t balance) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    RepHost& h = c->reps[r];
+    const int N = h.dev.N;
+    if (n_perm_part < 1 || (int)n_perm_part > std::min(N, kTabMaxK))
+        return fail(SPIMC_ERR_UNSUPPORTED, "n_perm_part must be in [1, min(n_part, 6)]");
+    if (balance > 1) return fail(SPIMC_ERR_ARG, "balance must be 0 or 1");
+    if (N > kTabMaxN) return fail(SPIMC_ERR_UNSUPPORTED, "the permutation table supports at most 1024 particles");
+    // the cycle list: k! types x C(N, k) subsets
+    double entries = 1.;
+    for (uint32_t i = 0; i < n_perm_part; ++i) entries *= (double)(N - (int)i);
+    if (entries > (double)kTabMaxEntries)
+        return fail(SPIMC_ERR_UNSUPPORTED, "the permutation table's cycle list exceeds 2^28 entries");
+    HIPCHK(hipDeviceSynchronize());
+    if (!h.perm_t) HIPCHK(hipMalloc((void**)&h.perm_t, 2 * (size_t)N * N * sizeof(double)));
+    h.dev.perm_t = h.perm_t;
+    h.dev.tab_k = (int)n_perm_part;
+    h.dev.tab_balance = (int)balance;
+    return upload_reps(c);
+}
+
+int spimc_perm_table_stage(spimc_ctx* c, const uint32_t* n_attempts, uint32_t n_uniform, uint8_t* accepted) {
+    if (!c) return fail(SPIMC_ERR_ARG, "null context");
+    for (size_t r = 0; r < c->reps.size(); ++r)
+        if ((n_attempts ? n_attempts[r] : n_uniform) > 0 && c->reps[r].dev.tab_k == 0)
+            return fail(SPIMC_ERR_STATE, "spimc_perm_table_configure was not called for a replica with attempts");
+    int rc = ensure_perm(c, true);
+    if (rc) return rc;
+    c->perm_table = true;
+    rc = run_stage(c, 3, n_attempts, n_uniform, accepted, false);
+    c->perm_table = false;
+    return rc;
 }
 
 int spimc_perm_download(spimc_ctx* c, uint32_t r, uint32_t* sigma) {

@@ -860,6 +860,15 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
             const double la = pr.lsr - change + pr.prev;
             acc = !(la < log(pr.u0));
         }
+        long long tape_next = pr.tape_next;
+        int status = pr.status;
+        if (acc && pr.move == 3 && pr.tab_on) {
+            // PermBisectTable: the table of the new path (perm_bisect_table_class.h:216-225)
+            int bad = 0;
+            const double u = R.tape ? tape_get(R, tape_next++, &bad) : pr.u_tab;
+            if (bad) status = 3;
+            acc = !(pr.tab_ratio < u);
+        }
         s_acc = acc;
         Counters* c = R.cnt;
         if (pr.status == 1 && pr.move != 3) {  // the round's pair and rho_k work, for the roofline figures
@@ -903,8 +912,8 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
             t[7] = l0 ? pr.prev : nan;
         }
         c->stage_i = si + 1;
-        if (R.tape) *R.tape_cursor = pr.tape_next;
-        if (pr.status == 3) {  // replay tape overrun / bad draw
+        if (R.tape) *R.tape_cursor = tape_next;
+        if (status == 3) {  // replay tape overrun / bad draw
             c->error = 1;
             atomicOr(R.ctx_error, 1u);
         }
@@ -916,7 +925,8 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
         // Bisect::Accept of every chain, then PermBisect::AssignParticleLabels
         // (perm_bisect_class.h:47-56): the slices after the relinked slice e take
         // the labels of the path they now continue, so chain label a_i there
-        // holds what a_{i+1} held, and the wrap permutation follows.
+        // holds what a_{ch_nxt[i]} held (each cycle of ch_nxt followed from its
+        // first chain), and the wrap permutation follows.
         const int n = pr.n_chain, nm = pr.count, nb = nm + 1;
         for (int idx = lane; idx < n * nm; idx += kWave) {
             const int i = idx / nm, t = idx % nm + 1;
@@ -929,29 +939,32 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
         if (n > 1) {
             const bool wrapped = pr.b0 + nb - 1 >= M;
             const int e = pr.e_slice;
+            const int* lbl = wrapped ? R.ch_lbl2 : R.ch_lbl;
             for (int idx = lane; idx < (M - 1 - e) * 3; idx += kWave) {
                 const int s = e + 1 + idx / 3, d = idx % 3;
                 double* col = R.pos + ((size_t)d * M + s) * N;
-                const int a0 = wrapped ? R.ch_lbl2[0] : R.ch_lbl[0];
-                const double keep = col[a0];
-                int ai = a0;
-                for (int i = 0; i + 1 < n; ++i) {
-                    const int an = wrapped ? R.ch_lbl2[i + 1] : R.ch_lbl[i + 1];
-                    col[ai] = col[an];
-                    ai = an;
+                for (int i0 = 0; i0 < n; ++i0) {
+                    if (!R.ch_lead[i0]) continue;
+                    const double keep = col[lbl[i0]];
+                    int ai = lbl[i0];
+                    for (int i = R.ch_nxt[i0]; i != i0; i = R.ch_nxt[i]) {
+                        col[ai] = col[lbl[i]];
+                        ai = lbl[i];
+                    }
+                    col[ai] = keep;
                 }
-                col[ai] = keep;
             }
             if (lane == 0) {
-                const int a0 = wrapped ? R.ch_lbl2[0] : R.ch_lbl[0];
-                const int keep = R.perm[a0];
-                int ai = a0;
-                for (int i = 0; i + 1 < n; ++i) {
-                    const int an = wrapped ? R.ch_lbl2[i + 1] : R.ch_lbl[i + 1];
-                    R.perm[ai] = R.perm[an];
-                    ai = an;
+                for (int i0 = 0; i0 < n; ++i0) {
+                    if (!R.ch_lead[i0]) continue;
+                    const int keep = R.perm[lbl[i0]];
+                    int ai = lbl[i0];
+                    for (int i = R.ch_nxt[i0]; i != i0; i = R.ch_nxt[i]) {
+                        R.perm[ai] = R.perm[lbl[i]];
+                        ai = lbl[i];
+                    }
+                    R.perm[ai] = keep;
                 }
-                R.perm[ai] = keep;
                 for (int i = 0; i < n; ++i) {
                     const int a = wrapped ? R.ch_lbl2[i] : R.ch_lbl[i];
                     R.iperm[R.perm[a]] = a;

@@ -29,11 +29,11 @@ __device__ __forceinline__ double chain_old(const Rep& R, int b0, int t, int i, 
     return pos_at(R, d, (b0 + t) % R.M, chain_label(R, b0, t, i));
 }
 // NEW mode: the start is shared; the end of chain i is the old end of chain
-// i + 1 (PermBisect::PermuteBeads, perm_bisect_class.h:32-44); the beads between
-// are the proposal's.
+// ch_nxt[i] (i + 1 of a cycle walk; PermBisect::PermuteBeads,
+// perm_bisect_class.h:32-44); the beads between are the proposal's.
 __device__ __forceinline__ double chain_new(const Rep& R, int b0, int nb, int n, int t, int i, int d) {
     if (t == 0) return chain_old(R, b0, 0, i, d);
-    if (t == nb) return chain_old(R, b0, nb, (i + 1) % n, d);
+    if (t == nb) return chain_old(R, b0, nb, R.ch_nxt[i], d);
     return R.ch_new[((size_t)i * (nb - 1) + t - 1) * 3 + d];
 }
 
@@ -100,6 +100,198 @@ __device__ __forceinline__ int perm_level_step(const Rep& R, Prop* pr, int level
 }
 
 // ---------------------------------------------------------------------------
+// PermBisectTable (perm_bisect_table_class.h:10-228): the cycle list is every
+// permutation type of k = n_perm_part particles crossed with every k-subset of
+// the particles, type-major (BuildCycles, :54-102).  scaffold's GenPerm and
+// GenCombPermK are not in the reference tree; both lists are taken here in
+// lexicographic order (permutations of 0..k-1; subsets by their sorted
+// members), as in the oracle.  An entry is never stored: thread t of the
+// proposal's workgroup walks entries [E t / 256, E (t + 1) / 256) of the list.
+// ---------------------------------------------------------------------------
+constexpr int kTabMaxK = 6;
+
+__device__ __forceinline__ long long tab_binom(int n, int r) {
+    if (r < 0 || r > n) return 0;
+    long long v = 1;
+    for (int i = 1; i <= r; ++i) v = v * (n - r + i) / i;
+    return v;
+}
+// element j of a k-array held in registers (no dynamic register indexing)
+__device__ __forceinline__ int tab_pick(const int (&a)[kTabMaxK], int j) {
+    int v = a[0];
+#pragma unroll
+    for (int i = 1; i < kTabMaxK; ++i) v = j == i ? a[i] : v;
+    return v;
+}
+// the rank-th k-subset of 0..n-1 in lexicographic order
+__device__ __forceinline__ void tab_subset(long long rank, int n, int k, int (&s)[kTabMaxK]) {
+    int x = 0;
+#pragma unroll
+    for (int i = 0; i < kTabMaxK; ++i) {
+        if (i < k) {
+            for (;;) {
+                const long long c = tab_binom(n - x - 1, k - i - 1);
+                if (rank < c) break;
+                rank -= c;
+                ++x;
+            }
+            s[i] = x++;
+        } else {
+            s[i] = 0;
+        }
+    }
+}
+// the next k-subset in lexicographic order; false after the last
+__device__ __forceinline__ bool tab_next_subset(int n, int k, int (&s)[kTabMaxK]) {
+    int top = -1;  // the last position below its maximum n - k + i
+#pragma unroll
+    for (int i = 0; i < kTabMaxK; ++i)
+        if (i < k && s[i] != n - k + i) top = i;
+    if (top < 0) return false;
+#pragma unroll
+    for (int i = 0; i < kTabMaxK; ++i) {
+        if (i == top) s[i] += 1;
+        if (i > top && i < k) s[i] = s[i - 1] + 1;
+    }
+    return true;
+}
+// the rank-th permutation of 0..k-1 in lexicographic order
+__device__ __forceinline__ void tab_perm(int rank, int k, int (&p)[kTabMaxK]) {
+    int fact = 1;
+    for (int i = 2; i < k; ++i) fact *= i;  // (k - 1)!
+    unsigned used = 0;
+#pragma unroll
+    for (int i = 0; i < kTabMaxK; ++i) {
+        if (i < k) {
+            const int d = rank / fact;
+            rank -= d * fact;
+            if (k - 1 - i > 0) fact /= (k - 1 - i);
+            int c = 0, seen = -1;
+            for (int v = 0; v < k; ++v)
+                if (!((used >> v) & 1u) && ++seen == d) {
+                    c = v;
+                    break;
+                }
+            p[i] = c;
+            used |= 1u << c;
+        } else {
+            p[i] = i;
+        }
+    }
+}
+// ConstructPermTable's weight of entry (subset s, type p): the product of
+// t(part(i), part(perm(i))) in order (perm_bisect_table_class.h:18-20)
+__device__ __forceinline__ double tab_weight(const double* __restrict__ t, int N, int k, const int (&s)[kTabMaxK],
+                                             const int (&p)[kTabMaxK]) {
+    double w = 1.;
+#pragma unroll
+    for (int i = 0; i < kTabMaxK; ++i)
+        if (i < k) w *= t[(size_t)s[i] * N + tab_pick(s, p[i])];
+    return w;
+}
+// UpdatePermTable (perm_bisect_table_class.h:31-51) into t: the path ends n links
+// after slice b0; pi (LDS, or null for the identity) maps particle j to the
+// particle whose old end j's chain now reaches.
+__device__ __forceinline__ void tab_build(const Rep& R, int b0, int nb, const int* pi, double* __restrict__ t,
+                                          int tid) {
+#pragma clang fp contract(off)
+    const int N = R.N, e_end = (b0 + nb) % R.M;
+    const bool end_wrapped = b0 + nb >= R.M;
+    const double i4ltn = 1. / (4. * R.lambda * R.tau) / nb;
+    for (int idx = tid; idx < N * N; idx += kPermBlock) {
+        const int i = idx / N, j = idx % N;
+        const int ei = pi ? pi[i] : i, ej = pi ? pi[j] : j;
+        const int li = end_wrapped ? label_after_wrap(R, ei) : ei, lj = end_wrapped ? label_after_wrap(R, ej) : ej;
+        double di[3], dj[3];
+        for (int d = 0; d < 3; ++d) {
+            const double a = pos_at(R, d, b0, i);
+            di[d] = minimg(a - pos_at(R, d, e_end, li), R.L, R.iL);
+            dj[d] = minimg(a - pos_at(R, d, e_end, lj), R.L, R.iL);
+        }
+        const double s_ii = di[0] * di[0] + di[1] * di[1] + di[2] * di[2];
+        const double s_ij = dj[0] * dj[0] + dj[1] * dj[1] + dj[2] * dj[2];
+        const double e = (-s_ij + s_ii) * i4ltn;
+        t[idx] = e > R.perm_log_eps ? exp(e) : 0.;
+    }
+}
+// One pass of the workgroup over the cycle list of table t: returns the
+// running total of the weights above epsilon (each thread sums its entries in
+// order, the thread sums are added in thread order: the association differs
+// from the reference's single running sum in the last bits).  With select, also
+// the first entry whose running total exceeds x and the last entry above
+// epsilon (-1 if none) in *sel, *last.  sh: kPermBlock doubles of LDS.
+__device__ __forceinline__ double tab_pass(const Rep& R, const double* __restrict__ t, int tid, bool select, double x,
+                                           double* sh, long long* shl, long long* sel, long long* last) {
+    const int N = R.N, k = R.tab_k;
+    const long long n_comb = tab_binom(N, k);
+    long long fk = 1;
+    for (int i = 2; i <= k; ++i) fk *= i;
+    const long long E = n_comb * fk;
+    const long long lo = E * tid / kPermBlock, hi = E * (tid + 1) / kPermBlock;
+    const double eps = R.perm_eps;
+    int s[kTabMaxK], p[kTabMaxK];
+    auto walk = [&](double run, bool pick, long long* found, long long* lastp) {
+        if (lo >= hi) return run;
+        int type = (int)(lo / n_comb);
+        tab_perm(type, k, p);
+        tab_subset(lo % n_comb, N, k, s);
+        for (long long e = lo; e < hi; ++e) {
+            const double w = tab_weight(t, N, k, s, p);
+            if (w > eps) {
+                run += w;
+                if (pick) {
+                    *lastp = e;
+                    if (*found < 0 && run > x) *found = e;
+                }
+            }
+            if (!tab_next_subset(N, k, s)) {
+                ++type;
+                if (type < (int)fk) {
+                    tab_perm(type, k, p);
+                    tab_subset(0, N, k, s);
+                }
+            }
+        }
+        return run;
+    };
+    sh[tid] = walk(0., false, nullptr, nullptr);
+    __syncthreads();
+    if (tid == 0) {
+        double acc = 0.;
+        for (int i = 0; i < kPermBlock; ++i) {
+            const double v = sh[i];
+            sh[i] = acc;  // exclusive
+            acc += v;
+        }
+        shl[0] = __double_as_longlong(acc);
+    }
+    __syncthreads();
+    const double total = __longlong_as_double(shl[0]);
+    if (select) {
+        long long f = -1, l = -1;
+        walk(sh[tid], true, &f, &l);
+        __syncthreads();
+        shl[1 + tid] = f;
+        shl[1 + kPermBlock + tid] = l;
+        __syncthreads();
+        if (tid == 0) {
+            long long fb = -1, lb = -1;
+            for (int i = 0; i < kPermBlock; ++i) {
+                if (fb < 0 && shl[1 + i] >= 0) fb = shl[1 + i];
+                if (shl[1 + kPermBlock + i] >= 0) lb = shl[1 + kPermBlock + i];
+            }
+            shl[1] = fb;
+            shl[2] = lb;
+        }
+        __syncthreads();
+        *sel = shl[1];
+        *last = shl[2];
+        __syncthreads();
+    }
+    return total;
+}
+
+// ---------------------------------------------------------------------------
 // PermBisectIterative::Attempt (perm_bisect_iterative_class.h:113-223) down to
 // level 0.  Draws in the reference's order: bead0, the first particle, per
 // walk step the continue and select uniforms (SelectCycleIterative, :32-110),
@@ -114,6 +306,10 @@ __device__ __forceinline__ int perm_level_step(const Rep& R, Prop* pr, int level
 // proposal whose label does not change inside the window
 // stage_max: the largest proposal, in chain beads, bisected on LDS (at most
 // kPermStage)
+// TABLE: PermBisectTable::Attempt (perm_bisect_table_class.h:122-228) instead:
+// bead0, the table's selection uniform, then the same bisection; the final
+// table check after level 0 is k_decide's (Prop::tab_*).
+template <bool TABLE>
 __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restrict__ reps, int n_reps, int tuned,
                                                              int stage_max) {
     const int r = blockIdx.x;
@@ -159,8 +355,8 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
         if (tape) {
             cur = *R.tape_cursor;
             b0 = (int)tape_get(R, cur, &bad);
-            p0 = (int)tape_get(R, cur + 1, &bad);
-            cur += 2;
+            p0 = TABLE ? 0 : (int)tape_get(R, cur + 1, &bad);
+            cur += TABLE ? 1 : 2;
             if (b0 < 0 || b0 >= M || p0 < 0 || p0 >= N) {
                 bad = 1;
                 b0 = 0;
@@ -168,8 +364,9 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
             }
         } else {
             b0 = (int)rng.index(0, 0, M);
-            p0 = (int)rng.index(0, 1, N);
+            p0 = TABLE ? 0 : (int)rng.index(0, 1, N);
         }
+        pr->tab_on = 0;
         s_i[1] = b0;
         s_i[2] = p0;
         s_i[3] = bad;
@@ -183,7 +380,71 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
     const double i4ltn = 1. / (4. * R.lambda * R.tau) / nb;
     // ---- the cycle walk ----
     int p = p0, n = 0, stop = 0;  // stop: 1 closed, 2 walk stopped, 3 error
-    for (int step = 0; !stop; ++step) {
+    __shared__ int s_tp[kTabMaxK];  // the table entry's permutation type
+    if constexpr (TABLE) {
+        // ---- the table of the old path, its running total, the selection
+        // (ConstructPermTable, SelectCycle: perm_bisect_table_class.h:10-28,105-119)
+        // and the table of the new path (:216-225) ----
+        __shared__ double sh[kPermBlock];
+        __shared__ long long shl[1 + 2 * kPermBlock];
+        const int k = R.tab_k;
+        double* t0 = R.perm_t;
+        double* t1 = R.perm_t + (size_t)N * N;
+        tab_build(R, b0, nb, nullptr, t0, tid);
+        __syncthreads();
+        long long sel = -1, last = -1;
+        const double tot0 = tab_pass(R, t0, tid, false, 0., sh, shl, &sel, &last);
+        if (tid == 0) {
+            int bad = s_i[3];
+            long long cur = s_cur;
+            const double u = tape ? tape_get(R, cur++, &bad) : rng.unif(1u, 0);
+            s_w[1] = tot0 * u;  // UnifRand(0, perm_tot_0)
+            s_i[3] = bad;
+            s_cur = cur;
+        }
+        __syncthreads();
+        tab_pass(R, t0, tid, true, s_w[1], sh, shl, &sel, &last);
+        // x at or past the total (rounding): the last entry free-running, an error on a tape
+        if (sel < 0 && !tape) sel = last;
+        if (sel < 0) {
+            stop = 3;
+        } else {
+            stop = 1;
+            n = k;
+            const long long n_comb = tab_binom(N, k);
+            int sb[kTabMaxK], tp[kTabMaxK];
+            tab_perm((int)(sel / n_comb), k, tp);
+            tab_subset(sel % n_comb, N, k, sb);
+            if (tid == 0) {
+                for (int i = 0; i < k; ++i) {
+                    R.ch_lbl[i] = tab_pick(sb, i);
+                    s_tp[i] = tab_pick(tp, i);
+                }
+                s_w[0] = tab_weight(t0, N, k, sb, tp);
+            }
+            // the new path's ends: chain part(i) reaches the old end of part(type(i))
+            for (int j = tid; j < N; j += kPermBlock) vis[j] = j;
+            __syncthreads();
+            if (tid == 0)
+                for (int i = 0; i < k; ++i) vis[tab_pick(sb, i)] = tab_pick(sb, tab_pick(tp, i));
+            __syncthreads();
+            double tot1 = tot0;
+            if (k < N) {
+                tab_build(R, b0, nb, vis, t1, tid);
+                __syncthreads();
+                tot1 = tab_pass(R, t1, tid, false, 0., sh, shl, &sel, &last);
+            }
+            if (tid == 0) {
+                pr->tab_on = k < N;
+                pr->tab_ratio = R.tab_balance ? tot0 / (tot1 * s_w[0]) : tot0 / tot1;
+                pr->u_tab = tape ? 0. : rng.unif(2u, 0);
+            }
+            __syncthreads();
+            for (int j = tid; j < N; j += kPermBlock) vis[j] = -1;
+        }
+        __syncthreads();
+    }
+    for (int step = 0; !TABLE && !stop; ++step) {
         if (tid == 0) {
             R.ch_lbl[n] = p;
             vis[p] = n;
@@ -285,13 +546,23 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
     }
     long long cur = s_cur;
     int bad = s_i[3];
-    // the visited map becomes the chain map before the wrap; chain labels after it
+    // the visited map becomes the chain map before the wrap; chain labels after
+    // it; the chain-end map of NEW mode and the first chain of each of its cycles
     for (int i = tid; i < n; i += kPermBlock) {
         const int l = R.ch_lbl[i];
         R.mv_pre[l] = i;
         const int l2 = label_after_wrap(R, l);
         R.ch_lbl2[i] = l2;
         R.mv_post[l2] = i;
+        if (TABLE) {
+            R.ch_nxt[i] = s_tp[i];
+            int lead = 1;
+            for (int j = s_tp[i]; j != i; j = s_tp[j]) lead &= j > i;
+            R.ch_lead[i] = lead;
+        } else {
+            R.ch_nxt[i] = (i + 1) % n;
+            R.ch_lead[i] = i == 0;
+        }
     }
     __syncthreads();  // chain_old reads the labels of every chain across the wrap
     if (stop != 1) {
@@ -328,7 +599,7 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
     if (tid == 0) {
         int k = 0;
         for (int i = 0; i < n; ++i)
-            if (nb != M || R.ch_lbl2[(i + 1) % n] != R.ch_lbl[i]) vis[k++] = i;
+            if (nb != M || R.ch_lbl2[R.ch_nxt[i]] != R.ch_lbl[i]) vis[k++] = i;
         s_open = k;
     }
     // A proposal of at most kPermStage beads (a few chains) is bisected by the
@@ -343,8 +614,8 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
             const int i = idx / per, t = (idx % per) / 3, d = idx % 3;
             const double x = chain_old(R, b0, t, i, d);
             st_old[idx] = x;
-            // NEW mode: the end of chain i is the old end of chain i + 1
-            st_new[idx] = t == nb ? chain_old(R, b0, nb, (i + 1) % n, d) : x;
+            // NEW mode: the end of chain i is the old end of chain ch_nxt[i]
+            st_new[idx] = t == nb ? chain_old(R, b0, nb, R.ch_nxt[i], d) : x;
         }
     } else {
         // NEW positions start as the old ones
@@ -414,7 +685,7 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
                         for (int d = 0; d < 3; ++d) dd[d] = minimg(so[3 * a + d] - so[3 * b + d], R.L, R.iL);
                         ko += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
                     }
-                    if (!ring || R.ch_lbl2[(i + 1) % n] != R.ch_lbl[i]) {
+                    if (!ring || R.ch_lbl2[R.ch_nxt[i]] != R.ch_lbl[i]) {
                         for (int d = 0; d < 3; ++d) dd[d] = minimg(sn[3 * a + d] - sn[3 * b + d], R.L, R.iL);
                         kn += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
                     }
@@ -507,7 +778,7 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
                         dd[d] = minimg(chain_old(R, b0, a, i, d) - chain_old(R, b0, b, i, d), R.L, R.iL);
                     ko += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
                 }
-                if (!ring || R.ch_lbl2[(i + 1) % n] != R.ch_lbl[i]) {
+                if (!ring || R.ch_lbl2[R.ch_nxt[i]] != R.ch_lbl[i]) {
                     for (int d = 0; d < 3; ++d)
                         dd[d] = minimg(chain_new(R, b0, nb, n, a, i, d) - chain_new(R, b0, nb, n, b, i, d), R.L, R.iL);
                     kn += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
