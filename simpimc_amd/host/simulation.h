@@ -426,6 +426,7 @@ struct IlkkaPairAction : PairActionBase {
 struct Engine {
     spimc_ctx* ctx = nullptr;
     uint32_t n_replicas = 1;
+    const void* perm_owner = nullptr;  // the permuting move whose level and epsilon the engine holds
     ~Engine() {
         if (ctx) spimc_ctx_destroy(ctx);
     }
@@ -523,46 +524,96 @@ struct Bisect : Move {
 
 // PermBisectIterative (perm_bisect_iterative_class.h:226-230, perm_bisect_class.h:100-116):
 // the bisection attributes plus epsilon; its own level per replica, and the
-// per-cycle-length counters of each block (perm_bisect_class.h:119-132)
+// per-cycle-length counters of each block (perm_bisect_class.h:119-132).  The
+// engine keeps one set of permutation counters per replica; each move adds to
+// its own copy what its stages changed, so two permuting moves keep apart.
 struct PermBisectIterative : Bisect {
     double epsilon;
     uint32_t n_part;
-    std::vector<uint64_t> perm_last;                    // [replica][2 n_part] at the last write
+    std::vector<uint64_t> own;                          // [replica][2 + 2 n_part] this move's counts
+    std::vector<uint64_t> perm_last;                    // [replica][2 + 2 n_part] at the last write
     std::vector<std::vector<uint64_t>> perm_att, perm_acc;  // the block's counts per replica
     bool h5_started = false;
     PermBisectIterative(const Input& in, Engine& e, const Path& path) : Bisect(in, e, path) {
         epsilon = in.GetAttribute<double>("epsilon", 1.e-100);
         n_part = path.n_part;
     }
-    int Stage(uint32_t n) override { return spimc_perm_bisect_stage(eng.ctx, nullptr, n, nullptr); }
+    virtual int PermStage(uint32_t n) { return spimc_perm_bisect_stage(eng.ctx, nullptr, n, nullptr); }
+    // PermBisect::n_perm_type: N for the iterative move (perm_bisect_iterative_class.h:229)
+    virtual int NPermType() const { return (int)n_part; }
+    std::vector<uint64_t> ReadAll() const {
+        const size_t w = 2 + 2 * (size_t)n_part;
+        std::vector<uint64_t> v(eng.n_replicas * w);
+        for (uint32_t r = 0; r < eng.n_replicas; ++r) check(spimc_perm_counters(eng.ctx, r, &v[r * w]), "counters");
+        return v;
+    }
+    // the engine holds one permuting level and epsilon per replica: a move takes
+    // them over (its own, adapted level per replica) before its stage
+    void Activate() {
+        for (uint32_t r = 0; r < eng.n_replicas; ++r)
+            check(spimc_perm_configure(eng.ctx, r, n_level.empty() ? n_level0 : n_level[r], epsilon), name.c_str());
+        eng.perm_owner = this;
+    }
+    int Stage(uint32_t n) override {
+        if (eng.perm_owner != this) Activate();
+        const auto before = ReadAll();
+        const int rc = PermStage(n);
+        if (rc) return rc;
+        const auto after = ReadAll();
+        own.resize(after.size(), 0);
+        for (size_t i = 0; i < after.size(); ++i) own[i] += after[i] - before[i];
+        return rc;
+    }
     void Counters(uint32_t r, uint64_t* c) const override {
-        std::vector<uint64_t> v(2 + 2 * (size_t)n_part);
-        check(spimc_perm_counters(eng.ctx, r, v.data()), "counters");
-        c[0] = v[0];
-        c[1] = v[1];
+        const size_t w = 2 + 2 * (size_t)n_part;
+        c[0] = own.empty() ? 0 : own[r * w];
+        c[1] = own.empty() ? 0 : own[r * w + 1];
     }
     void SetLevel(uint32_t r, uint32_t lv) override {
-        check(spimc_perm_configure(eng.ctx, r, lv, epsilon), "adaptive bisection level");
+        if (eng.perm_owner == this) check(spimc_perm_configure(eng.ctx, r, lv, epsilon), "adaptive bisection level");
     }
-    void Configure() {
-        for (uint32_t r = 0; r < eng.n_replicas; ++r) check(spimc_perm_configure(eng.ctx, r, n_level0, epsilon), name.c_str());
-    }
+    virtual void Configure() { Activate(); }
     // perm_attempt / perm_accept since the last write (PermBisect::Reset zeroes them)
     void PermBlock() {
         const size_t w = 2 + 2 * (size_t)n_part;
         perm_last.resize(eng.n_replicas * w, 0);
+        own.resize(eng.n_replicas * w, 0);
         perm_att.assign(eng.n_replicas, std::vector<uint64_t>(n_part));
         perm_acc.assign(eng.n_replicas, std::vector<uint64_t>(n_part));
-        std::vector<uint64_t> v(w);
         for (uint32_t r = 0; r < eng.n_replicas; ++r) {
-            check(spimc_perm_counters(eng.ctx, r, v.data()), "counters");
+            const uint64_t* v = &own[r * w];
             uint64_t* l = &perm_last[r * w];
             for (uint32_t c = 0; c < n_part; ++c) {
                 perm_att[r][c] = v[2 + c] - l[2 + c];
                 perm_acc[r][c] = v[2 + n_part + c] - l[2 + n_part + c];
             }
-            std::copy(v.begin(), v.end(), l);
+            std::copy(v, v + w, l);
         }
+    }
+};
+
+// PermBisectTable (perm_bisect_table_class.h:231-239): PermBisect's attributes
+// plus n_perm_part.  detailed_balance="1" (no reference counterpart) selects the
+// final table check perm_tot_0 / (perm_tot_1 w) in place of the reference's
+// perm_tot_0 / perm_tot_1, which is biased when n_perm_part < n_part.
+struct PermBisectTable : PermBisectIterative {
+    uint32_t n_perm_part;
+    uint32_t balance;
+    PermBisectTable(const Input& in, Engine& e, const Path& path) : PermBisectIterative(in, e, path) {
+        n_perm_part = in.GetAttribute<uint32_t>("n_perm_part");
+        balance = in.GetAttribute<bool>("detailed_balance", false) ? 1u : 0u;
+    }
+    int PermStage(uint32_t n) override { return spimc_perm_table_stage(eng.ctx, nullptr, n, nullptr); }
+    // the permutation types of n_perm_part particles (BuildCycles, perm_bisect_table_class.h:61)
+    int NPermType() const override {
+        int f = 1;
+        for (uint32_t i = 2; i <= n_perm_part; ++i) f *= (int)i;
+        return f;
+    }
+    void Configure() override {
+        PermBisectIterative::Configure();
+        for (uint32_t r = 0; r < eng.n_replicas; ++r)
+            check(spimc_perm_table_configure(eng.ctx, r, n_perm_part, balance), name.c_str());
     }
 };
 

@@ -66,6 +66,8 @@ struct Simulation {
                 mv = std::make_shared<Bisect>(m, eng, path);
             else if (type == "PermBisectIterative")
                 mv = std::make_shared<PermBisectIterative>(m, eng, path);
+            else if (type == "PermBisectTable")
+                mv = std::make_shared<PermBisectTable>(m, eng, path);
             else if (type == "DisplaceParticle")
                 mv = std::make_shared<DisplaceParticle>(m, eng, desc, path);
             else
@@ -268,7 +270,7 @@ struct Simulation {
         if (pb)
             for (uint32_t r = 0; r < h5.size(); ++r) {
                 const std::string g = "Moves/" + m.name + "/";
-                if (!pb->h5_started) h5[r]->Int(g + "n_perm_type", pb->n_part);
+                if (!pb->h5_started) h5[r]->Int(g + "n_perm_type", pb->NPermType());
                 h5[r]->AppendInts(g + "perm_attempt", std::vector<long long>(pb->perm_att[r].begin(), pb->perm_att[r].end()));
                 h5[r]->AppendInts(g + "perm_accept", std::vector<long long>(pb->perm_acc[r].begin(), pb->perm_acc[r].end()));
             }

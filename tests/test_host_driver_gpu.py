@@ -487,3 +487,71 @@ def test_permuting_driver_matches_binding_and_restarts(tmp_path):
     last_pos = pdump.reshape(dims)[-1].transpose(2, 1, 0)
     blocks2 = run("Q", 'init_type="Restart" prefix="P"')
     binding(last_pos, sig, blocks2, str(tmp_path / "Q.0.h5"))
+
+
+@pytest.mark.skipif(h5dump.H5DUMP is None, reason="h5dump not installed")
+def test_table_driver_matches_binding(tmp_path):
+    """<Move type="PermBisectTable" n_perm_part="2"> beside PermBisectIterative:
+    the driver's chain equals the binding's with the same seed (energies, the
+    final path and permutation), each move reports its own attempts and
+    per-cycle-length counts, and the table move's n_perm_type is 2! = 2."""
+    assert os.path.exists(DRIVER), "build the host driver first"
+    s = system.config_c1(n_part=8, n_bead=16, r_s=1.0, tau=0.05, n_level=3)
+    write_init(tmp_path, s)
+    system.write_table(str(tmp_path / "bare.table"), system.bare_table_datasets(s))
+    kc = (5.0 + 1e-9) * 2 * np.pi / s.L
+    N, M = s.n_part, s.n_bead
+    nb = (N * M) >> s.bisect_n_level
+    xml = PERM_XML.replace(
+        '<Move name="Bisect" type="Bisect" species="e" n_level="3" />',
+        '<Move name="Table" type="PermBisectTable" species="e" n_level="2" epsilon="1e-100" n_perm_part="2" '
+        'detailed_balance="1" />\n    <Move name="Bisect" type="Bisect" species="e" n_level="3" />').replace(
+        '<Loop n_step="{nb}"><Move name="Bisect" /></Loop>',
+        '<Loop n_step="{nt}"><Move name="Table" /></Loop>\n        <Loop n_step="{nb}"><Move name="Bisect" /></Loop>')
+    nt = 20
+    (tmp_path / "T.xml").write_text(xml.format(seed=99, M=M, beta=s.beta, L=s.L, kc=kc, N=N, nb=nb, nt=nt, prefix="T",
+                                               init=INIT_FILE, action=BARE.format(file="bare.table")))
+    res = subprocess.run([DRIVER, "T.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    blocks = [json.loads(line) for line in open(tmp_path / "T.spimc.jsonl")]
+    with abi.Engine(s, seed=99) as e:
+        e.upload(system.random_path(s, seed=8))
+        e.obs_configure(energy=abi.ENERGY_THERMAL, sk_k_cut=kc, gr_n_r=100)
+        e.perm_configure(3)
+        e.perm_table_configure(2, 1)
+        for blk in blocks:
+            own = {"Perm": np.zeros(2 + 2 * N, dtype=np.int64), "Table": np.zeros(2 + 2 * N, dtype=np.int64)}
+
+            def stage(name, fn, n):
+                c0 = np.concatenate([np.atleast_1d(x) for x in e.perm_counters()]).astype(np.int64)
+                fn(n)
+                c1 = np.concatenate([np.atleast_1d(x) for x in e.perm_counters()]).astype(np.int64)
+                own[name] += c1 - c0
+
+            for _ in range(3):
+                stage("Perm", e.perm_bisect, nb)
+                e.perm_configure(2)  # the table move bisects at its own level
+                stage("Table", e.perm_table, nt)
+                e.perm_configure(3)
+                e.bisect(nb)
+                e.displace(N)
+                e.obs_accumulate(abi.OBS_ENERGY)
+            en, nm = e.obs_read(abi.OBS_ENERGY)
+            e.obs_reset(abi.OBS_ENERGY)
+            got = blk["Observables"]["Energy"][0]
+            assert got["Kinetic"] == pytest.approx(en[0] / (M * nm), rel=1e-14)
+            for name, n_per in (("Perm", nb), ("Table", nt)):
+                mv = blk["Moves"][name][0]
+                assert mv["n_attempt"] == 3 * n_per == own[name][0]
+                assert mv["n_accept"] == own[name][1]
+                assert mv["perm_attempt"] == own[name][2:2 + N].tolist()
+                assert mv["perm_accept"] == own[name][2 + N:].tolist()
+            assert blk["Moves"]["Table"][0]["perm_attempt"][1] == 3 * nt
+        final = e.download()
+        sig = e.perm()
+    f = str(tmp_path / "T.0.h5")
+    assert h5dump.dataset(f, "/Moves/Table/n_perm_type")[1][0] == 2
+    dims, pdump = h5dump.dataset(f, "/Observables/path_dump/e/positions")
+    assert np.array_equal(final, pdump.reshape(dims)[-1].transpose(2, 1, 0))
+    pdims, perm = h5dump.dataset(f, "/Observables/path_dump/e/permutation")
+    assert np.array_equal(perm.reshape(pdims)[-1][:, 1], sig)
