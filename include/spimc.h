@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SPIMC_ABI_VERSION 4
+#define SPIMC_ABI_VERSION 5
 
 /* status codes */
 #define SPIMC_OK 0
@@ -241,6 +241,17 @@ int spimc_reject_proposal(spimc_ctx* ctx, uint32_t replica);
  * particles holds at most one index (Boltzmann moves move one particle). */
 int spimc_get_action(spimc_ctx* ctx, uint32_t replica, int action, int mode, uint32_t b0, uint32_t b1,
                      const uint32_t* particles, uint32_t n_particles, uint32_t level, double* out);
+/* Action::GetActionLaplacian(b0, b1, particles, level) (action_class.h:49) in
+ * OLD or NEW mode, arguments as spimc_get_action.  Kinetic
+ * (kinetic_class.h:147-165): n_d 4 / (4 lambda tau 2^level) per step of the
+ * particle's walk.  Pair action (pair_action_class.h:336-363): for the window
+ * beads b, the central second difference (eps 1e-4, CalcLaplacianU :167-199)
+ * of the links (b, b + 2^level) and (b, b - 2^level) with every other
+ * particle in each coordinate of bead (p, b); 0 above level 0 (max_level); the
+ * long-range part is left out, as in the reference.  Its only reference caller
+ * is the ContactDensity observable. */
+int spimc_get_action_laplacian(spimc_ctx* ctx, uint32_t replica, int action, int mode, uint32_t b0, uint32_t b1,
+                               const uint32_t* particles, uint32_t n_particles, uint32_t level, double* out);
 
 /* spimc_get_action without allocation.  No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under HIP graph

@@ -651,6 +651,58 @@ struct Oracle {
         return tot;
     }
 
+    // PairAction::CalcLaplacianU (pair_action_class.h:167-199): the central second
+    // difference (eps 1e-4) of the link action (b_i, b_j) of (p_i, p_j) in each
+    // coordinate of bead (p_i, b_i)
+    double calc_laplacian_u(uint32_t b_i, uint32_t b_j, uint32_t p_i, uint32_t p_j, uint32_t level) {
+        double* b = R(p_i, b_i);
+        const double r0[3] = {b[0], b[1], b[2]};
+        const double eps = 1.e-4;
+        double rm, rpm, rrpm, tot = 0.;
+        dr_drp_drrp(b_i, b_j, p_i, p_j, rm, rpm, rrpm);
+        const double f0 = calc_u(rm, rpm, rrpm, level);
+        for (int d = 0; d < 3; ++d) {
+            b[d] = r0[d] + eps;
+            dr_drp_drrp(b_i, b_j, p_i, p_j, rm, rpm, rrpm);
+            const double fp1 = calc_u(rm, rpm, rrpm, level);
+            b[d] = r0[d] - eps;
+            dr_drp_drrp(b_i, b_j, p_i, p_j, rm, rpm, rrpm);
+            const double fm1 = calc_u(rm, rpm, rrpm, level);
+            tot += (fp1 + fm1 - 2 * f0) / (eps * eps);
+            b[d] = r0[d];
+        }
+        return tot;
+    }
+    // PairAction::GetActionLaplacian (pair_action_class.h:336-363): both links of
+    // every window bead b_i, (b_i, b_i + skip) and (b_i, b_i - skip + M), of every
+    // pair of GenerateParticlePairs; the long-range part is left out, as there.
+    // The link slices are taken mod M.  The reference's bead_loop subtracts M
+    // once (species_class.h:101), so for a window bead b_i >= M + skip its index
+    // b_i - skip + M reads past the last slice; mod M is the same slice wherever
+    // the reference's index is in range.
+    double pair_get_action_laplacian(uint32_t b0, uint32_t b1, const std::vector<uint32_t>& parts, uint32_t level) {
+        if (level > max_level || pair_type == SPIMC_PAIR_NONE) return 0.;
+        if (N == 1 || lambda == 0.) return 0.;
+        std::vector<std::pair<uint32_t, uint32_t>> pairs;
+        std::vector<uint32_t> others;
+        for (uint32_t p = 0; p < N; ++p)
+            if (std::find(parts.begin(), parts.end(), p) == parts.end()) others.push_back(p);
+        for (uint32_t p : parts)
+            for (uint32_t q : others) pairs.emplace_back(p, q);
+        for (size_t i = 0; i + 1 < parts.size(); ++i)
+            for (size_t j = i + 1; j < parts.size(); ++j) pairs.emplace_back(parts[i], parts[j]);
+        if (pairs.empty()) return 0.;
+        const uint32_t skip = 1u << level;
+        double tot = 0.;
+        for (uint32_t b_i = b0; b_i < b1; b_i += skip) {
+            const uint32_t b_j = (b_i + skip) % M, b_k = (b_i - skip + M) % M;
+            for (auto& pq : pairs)
+                tot += calc_laplacian_u(b_i, b_j, pq.first, pq.second, level) +
+                       calc_laplacian_u(b_i, b_k, pq.first, pq.second, level);
+        }
+        return tot;
+    }
+
     std::vector<std::pair<uint32_t, uint32_t>> all_pairs() const {  // pair_action_class.h:45-59
         std::vector<std::pair<uint32_t, uint32_t>> pairs;
         for (uint32_t p = 0; p + 1 < N; ++p)
@@ -801,6 +853,24 @@ struct Oracle {
                 dr(R(a), R(nx), d);
                 tot -= kin_splines[skip - 1].log_rho_free(d);
                 a = nx;
+            }
+        }
+        return tot;
+    }
+    // Kinetic::GetActionLaplacian (kinetic_class.h:147-165): n_d 4 / (4 lambda tau
+    // skip) per step of the walk from (p, b0) to the bead b1 - b0 links later
+    // (i_4_lambda_tau, single_action_class.h:18)
+    double kinetic_get_action_laplacian(uint32_t b0, uint32_t b1, const std::vector<uint32_t>& parts, uint32_t level) {
+        const uint32_t skip = 1u << level;
+        const double i4lt_l = (1. / (4. * lambda * tau)) / skip;
+        double tot = 0.;
+        for (uint32_t p : parts) {
+            Bd a{p, bead_loop(b0)};
+            const Bd end = next(a, b1 - b0);
+            uint32_t guard = 0;
+            while (a != end && guard++ <= N * M) {
+                tot += 3 * 4. * i4lt_l;
+                a = next(a, skip);
             }
         }
         return tot;
@@ -1834,6 +1904,20 @@ int spimc_oracle_get_action(void* h, int action, int mode, uint32_t b0, uint32_t
         *out = o->kinetic_get_action(b0, b1, pv, level);
     else
         *out = o->pair_get_action(b0, b1, pv, level);
+    o->mode = SPIMC_MODE_NEW;
+    return SPIMC_OK;
+}
+
+// Action::GetActionLaplacian (action_class.h:49) of the Kinetic or pair action
+int spimc_oracle_get_action_laplacian(void* h, int action, int mode, uint32_t b0, uint32_t b1, const uint32_t* parts,
+                                      uint32_t n_parts, uint32_t level, double* out) {
+    Oracle* o = (Oracle*)h;
+    std::vector<uint32_t> pv(parts, parts + n_parts);
+    o->mode = mode;
+    if (action == SPIMC_ACTION_KINETIC)
+        *out = o->kinetic_get_action_laplacian(b0, b1, pv, level);
+    else
+        *out = o->pair_get_action_laplacian(b0, b1, pv, level);
     o->mode = SPIMC_MODE_NEW;
     return SPIMC_OK;
 }

@@ -129,6 +129,7 @@ SIGNATURES = {
     "spimc_perm_configure": ([_c, _u32, _u32, _d], _i32),
     "spimc_perm_bisect_stage": ([_c, _pu32, _u32, _pu8], _i32),
     "spimc_perm_table_configure": ([_c, _u32, _u32, _u32], _i32),
+    "spimc_get_action_laplacian": ([_c, _u32, _i32, _i32, _u32, _u32, _pu32, _u32, _u32, _dp], _i32),
     "spimc_perm_table_stage": ([_c, _pu32, _u32, _pu8], _i32),
     "spimc_perm_download": ([_c, _u32, _pu32], _i32),
     "spimc_perm_upload": ([_c, _u32, _pu32], _i32),
@@ -257,6 +258,13 @@ class Engine:
         arr = (C.c_uint32 * max(1, len(parts)))(*parts)
         out = C.c_double()
         check(self.lib.spimc_get_action(self.h, r, action, mode, b0, b1, arr, len(parts), level, C.byref(out)))
+        return out.value
+
+    def get_action_laplacian(self, action: int, mode: int, b0: int, b1: int, parts, level: int, r: int = 0) -> float:
+        arr = (C.c_uint32 * max(1, len(parts)))(*parts)
+        out = C.c_double()
+        check(self.lib.spimc_get_action_laplacian(self.h, r, action, mode, b0, b1, arr, len(parts), level,
+                                                  C.byref(out)))
         return out.value
 
     def d_action_d_beta(self, action: int, r: int = 0) -> float:

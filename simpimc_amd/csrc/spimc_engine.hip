@@ -2156,6 +2156,51 @@ int spimc_get_action_noalloc(spimc_ctx* c, uint32_t r, int action, int mode, uin
     return get_action_impl(c, r, action, mode, b0, b1, parts, n_parts, level, out, &sc);
 }
 
+int spimc_get_action_laplacian(spimc_ctx* c, uint32_t r, int action, int mode, uint32_t b0, uint32_t b1,
+                               const uint32_t* parts, uint32_t n_parts, uint32_t level, double* out) {
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!out || (n_parts && !parts)) return fail(SPIMC_ERR_ARG, "null argument");
+    if (n_parts > 1) return fail(SPIMC_ERR_UNSUPPORTED, "at most one particle per query");
+    if (b1 < b0) return fail(SPIMC_ERR_ARG, "b1 < b0");
+    if (level > (uint32_t)kMaxLevel) return fail(SPIMC_ERR_ARG, "level out of range");
+    const Rep& R = c->reps[r].dev;
+    if (n_parts == 1 && parts[0] >= (uint32_t)R.N) return fail(SPIMC_ERR_ARG, "particle index out of range");
+    if (b0 >= (uint32_t)R.M || b1 - b0 > (uint32_t)R.M) return fail(SPIMC_ERR_ARG, "window out of range");
+    *out = 0.;
+    if (n_parts == 0) return SPIMC_OK;
+    const int p = (int)parts[0];
+    const uint32_t skip = 1u << level;
+    if (action == SPIMC_ACTION_KINETIC) {
+        // Kinetic::GetActionLaplacian (kinetic_class.h:147-165): n_d 4 / (4 lambda
+        // tau skip) per step of the walk from (p, b0) to the bead b1 - b0 links
+        // later, a closed form of the walk's length added in the reference's order
+        uint32_t span = (b1 - b0) % (uint32_t)R.M;
+        if (span == 0 && b1 > b0 && R.perm) {
+            int nxt = p;
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipMemcpy(&nxt, R.perm + p, sizeof(int), hipMemcpyDeviceToHost));
+            if (nxt != p) span = (uint32_t)R.M;
+        }
+        if (span % skip) return fail(SPIMC_ERR_ARG, "window not a multiple of the level step");
+        const double i4lt_l = (1. / (4. * R.lambda * R.tau)) / skip;
+        double tot = 0.;
+        for (uint32_t i = 0; i < span / skip; ++i) tot += 3 * 4. * i4lt_l;
+        *out = tot;
+        return SPIMC_OK;
+    }
+    // the pair action: 0 above max_level (= 0) and for is_constant
+    if (level > 0 || R.pair_type == SPIMC_PAIR_NONE || R.N == 1 || b1 == b0) return SPIMC_OK;
+    if ((rc = write_iface_prop(c, r, p, (int)b0, 0))) return rc;
+    if ((rc = ensure_tmp(c, 1, 0))) return rc;
+    rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_pair_laplacian, 1, kLapBlock, 0, c->stream, c->d_reps, (int)r, mode, (int)b0,
+                (int)b1, p, c->d_tmp);
+    if (rc) return rc;
+    if ((rc = sync_and_resolve(c))) return rc;
+    HIPCHK(hipMemcpy(out, c->d_tmp, sizeof(double), hipMemcpyDeviceToHost));
+    return SPIMC_OK;
+}
+
 int spimc_d_action_d_beta(spimc_ctx* c, uint32_t r, int action, double* out) {
     int rc = check_rep(c, r);
     if (rc) return rc;

@@ -1112,6 +1112,65 @@ __global__ __launch_bounds__(256) void k_iface_items(uint2* __restrict__ items, 
 }
 
 // Sums a pair action query: links (mode column) + long-range window partials.
+// PairAction::GetActionLaplacian (pair_action_class.h:336-363) of particle p at
+// level 0: for the window beads b0 <= b < b1, the links (b, b + 1) and
+// (b, b - 1 + M) with every other particle, each through CalcLaplacianU
+// (:167-199), the central second difference (eps 1e-4) of the link action in
+// each coordinate of bead (p, b).  The long-range part is left out, as in the
+// reference.  NEW mode reads the pending interface proposal's beads.  One
+// workgroup; the thread sums are added in thread order.
+constexpr int kLapBlock = 256;
+__global__ __launch_bounds__(kLapBlock) void k_pair_laplacian(const Rep* __restrict__ reps, int r, int mode, int b0,
+                                                              int b1, int p, double* __restrict__ out) {
+    const Rep& R = reps[r];
+    const Prop pr = *R.prop;
+    const int M = R.M, N = R.N, nb = b1 - b0;
+    const double eps = 1.e-4;
+    __shared__ double red[kLapBlock / kWave];
+    auto at = [&](int d, int s, int lbl) { return (mode && pr.p == lbl) ? pos_new(R, pr, d, s) : pos_at(R, d, s, lbl); };
+    double tot = 0.;
+    for (int it = threadIdx.x; it < nb * (N - 1); it += kLapBlock) {
+        const int bi = b0 + it / (N - 1), qi = it % (N - 1), q = qi < p ? qi : qi + 1;
+        const int si = bi % M;
+        double lap = 0.;
+        for (int side = 0; side < 2; ++side) {
+            const int sj = side == 0 ? (bi + 1) % M : (bi - 1 + M) % M;
+            double a0[3], c0[3], a1[3], c1[3];
+            for (int d = 0; d < 3; ++d) {
+                a0[d] = at(d, si, p);
+                c0[d] = at(d, si, q);
+                a1[d] = at(d, sj, p);
+                c1[d] = at(d, sj, q);
+            }
+            LinkGeom g;
+            link_geom(a0, c0, a1, c1, R.L, R.iL, g);
+            const double f0 = pair_u(R, g.r, g.rp, g.s);
+            double part = 0.;
+            for (int d = 0; d < 3; ++d) {
+                const double x0 = a0[d];
+                a0[d] = x0 + eps;
+                link_geom(a0, c0, a1, c1, R.L, R.iL, g);
+                const double fp1 = pair_u(R, g.r, g.rp, g.s);
+                a0[d] = x0 - eps;
+                link_geom(a0, c0, a1, c1, R.L, R.iL, g);
+                const double fm1 = pair_u(R, g.r, g.rp, g.s);
+                part += (fp1 + fm1 - 2 * f0) / (eps * eps);
+                a0[d] = x0;
+            }
+            lap += part;
+        }
+        tot += lap;
+    }
+    tot = wave_sum(tot);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.;
+        for (int w = 0; w < kLapBlock / kWave; ++w) t += red[w];
+        *out = t;
+    }
+}
+
 __global__ void k_action_sum(const Rep* __restrict__ reps, int r, int mode, int n_links, int n_long,
                              const double* __restrict__ long_part, double* __restrict__ out) {
     const Rep& R = reps[r];

@@ -65,6 +65,7 @@ SIG = {
     "spimc_oracle_perm_counters": ([_c, _pu64], _i32),
     "spimc_oracle_perm_configure": ([_c, _u32, _d], _i32),
     "spimc_oracle_perm_table_configure": ([_c, _u32, _u32], _i32),
+    "spimc_oracle_get_action_laplacian": ([_c, _i32, _i32, _u32, _u32, _pu32, _u32, _u32, _dp], _i32),
     "spimc_oracle_perm_table_stage": ([_c, _u32, _pu8], _i32),
     "spimc_oracle_perm_get": ([_c, _pu32], _i32),
     "spimc_oracle_perm_set": ([_c, _pu32], _i32),
@@ -190,6 +191,13 @@ class Oracle:
         out = C.c_double()
         check(self.L.spimc_oracle_get_action(self.h, action, mode, b0, b1, arr, len(parts), level, C.byref(out)))
         return out.value
+
+    def get_action_laplacian(self, action, mode, b0, b1, parts, level):
+        pv = np.asarray(parts, dtype=np.uint32)
+        out = np.zeros(1)
+        check(self.L.spimc_oracle_get_action_laplacian(self.h, action, mode, b0, b1, pv.ctypes.data_as(_pu32), len(pv),
+                                                       level, dp(out)))
+        return float(out[0])
 
     def set_bisect_level(self, n_level):
         check(self.L.spimc_oracle_set_bisect_level(self.h, n_level))

@@ -56,9 +56,9 @@ def test_david_tau_not_in_table(tmp_path):
 def test_unknown_move_and_observable(tmp_path):
     system.write_table(str(tmp_path / "bare.table"), system.bare_table_datasets(system.config_c1()))
     res = run(tmp_path, BARE.format(file="bare.table"),
-              tweak=lambda x: x.replace('type="DisplaceParticle"', 'type="PermBisectTable"'))
+              tweak=lambda x: x.replace('type="DisplaceParticle"', 'type="VaryAction"'))
     assert res.returncode == 1
-    assert "Unrecognized Move, PermBisectTable" in res.stderr
+    assert "Unrecognized Move, VaryAction" in res.stderr
     res = run(tmp_path, BARE.format(file="bare.table"), extra='\n    <Observable name="S" type="Sign" />')
     assert res.returncode == 1
     assert "Unrecognized observable, Sign" in res.stderr

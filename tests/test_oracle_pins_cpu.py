@@ -355,3 +355,97 @@ def test_free_particle_bisection_always_accepts():
             tape += rng.standard_normal(3 * (8 >> (lvl + 1))).tolist() + [1 - 1e-9]
     o.tape_set(tape)
     assert o.bisect(40).all()
+
+
+# ---------------------------------------------------------------------------
+# GetActionLaplacian (action_class.h:49)
+# ---------------------------------------------------------------------------
+def _laplacian_numpy(s, pos, link_u, b0, b1, p, eps=1e-4):
+    """Second central differences (eps) in the coordinates of bead (p, b) of the
+    link actions (b, b+1) and (b, b-1) summed over the partners q, for the window
+    beads b0 <= b < b1 (level 0)."""
+    M, N = s.n_bead, s.n_part
+    tot = 0.0
+    for b in range(b0, b1):
+        for q in range(N):
+            if q == p:
+                continue
+            for b1_ in ((b + 1) % M, (b - 1) % M):
+                f0 = link_u(pos, b % M, b1_, p, q)
+                for d in range(3):
+                    x = pos.copy()
+                    x[d, b % M, p] += eps
+                    fp = link_u(x, b % M, b1_, p, q)
+                    x[d, b % M, p] -= 2 * eps
+                    fm = link_u(x, b % M, b1_, p, q)
+                    tot += (fp + fm - 2 * f0) / (eps * eps)
+    return tot
+
+
+def test_laplacian_of_the_bare_action_is_the_ewald_short_range_laplacian():
+    """Bare with the Ewald split: V = 1/r - erf(alpha r)/r, and away from r = 0
+    laplacian(erfc(alpha r)/r) = (4 alpha^3 / sqrt(pi)) exp(-alpha^2 r^2); the two
+    links of a bead each carry tau V(r) / 2 of its slice, so the pair Laplacian of
+    particle p over a window is tau (4 alpha^3 / sqrt(pi)) sum_b sum_q
+    exp(-alpha^2 r_pq(b)^2).  Without the long-range split (V = 1/r) it vanishes."""
+    s = system.config_c1(n_part=8, n_bead=8, n_level=2)
+    alpha = 5.0 / s.L
+    pos = system.random_path(s, seed=21, spread=0.2)
+    o = Oracle(s)
+    o.upload(pos)
+    p, b0, b1 = 3, 2, 6
+    want = 0.0
+    for b in range(b0, b1):
+        d = minimg(pos[:, b, :] - pos[:, b, p:p + 1], s.L)
+        r2 = np.sum(d * d, axis=0)
+        r2 = np.delete(r2, p)
+        assert np.all(np.sqrt(r2) > 0.2)
+        want += float(np.sum(np.exp(-alpha * alpha * r2)))
+    want *= s.tau * 4 * alpha ** 3 / math.sqrt(math.pi)
+    got = o.get_action_laplacian(abi.ACTION_PAIR, abi.MODE_NEW, b0, b1, [p], 0)
+    assert got == pytest.approx(want, rel=2e-6), (got, want)
+    s.use_long_range = False
+    o2 = Oracle(s)
+    o2.upload(pos)
+    assert abs(o2.get_action_laplacian(abi.ACTION_PAIR, abi.MODE_NEW, b0, b1, [p], 0)) < 1e-5 * want
+    # above max_level (0) the pair action has no Laplacian
+    assert o.get_action_laplacian(abi.ACTION_PAIR, abi.MODE_NEW, b0, b1, [p], 1) == 0.0
+
+
+def test_laplacian_of_the_david_action_is_a_second_difference_of_the_link_action():
+    s = system.config_c2(n_part=3, n_bead=4, n_level=1, k_shell=2.5, n_grid=300, seed=5)
+    pos = system.random_path(s, seed=6, spread=0.3, cofactor=0.4)
+    o = Oracle(s)
+    o.upload(pos)
+
+    def link_u(x, b, b1, p, q):
+        r, rp, sd = _ref_link_geometry(x, s.L, b, b1, p, q)
+        return o.pair_link(0, r, rp, sd)
+
+    for p, b0, b1 in ((0, 0, 4), (2, 1, 3), (1, 3, 5)):
+        got = o.get_action_laplacian(abi.ACTION_PAIR, abi.MODE_NEW, b0, b1, [p], 0)
+        want = _laplacian_numpy(s, pos, link_u, b0, b1, p)
+        assert got == pytest.approx(want, rel=1e-7, abs=1e-9), (p, b0, b1, got, want)
+
+
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_kinetic_laplacian_counts_the_walk_steps(level):
+    """Kinetic::GetActionLaplacian: n_d * 4 / (4 lambda tau 2^level) per step of the
+    walk; a whole-path window of a particle whose path closes on itself has no step."""
+    s = system.System(n_part=4, n_bead=8, beta=2.0, L=4.0, lam=0.5, bisect_n_level=2)
+    pos = system.random_path(s, seed=5, spread=0.4)
+    o = Oracle(s)
+    o.upload(pos)
+    skip = 1 << level
+    step = 3 * 4.0 * (1.0 / (4.0 * s.lam * s.tau)) / skip
+    for b0, span in ((1, 4), (6, 4), (0, 8)):
+        want = 0.0
+        for _ in range(span // skip):
+            want += step
+        got = o.get_action_laplacian(abi.ACTION_KINETIC, abi.MODE_NEW, b0, b0 + span, [2], level)
+        if span == s.n_bead:
+            want = 0.0  # the walk returns to its first bead at once
+        assert got == want, (b0, span, got, want)
+    o.set_perm(np.array([1, 0, 2, 3], dtype=np.uint32))
+    got = o.get_action_laplacian(abi.ACTION_KINETIC, abi.MODE_NEW, 0, 8, [0], level)
+    assert got == pytest.approx((8 // skip) * step, rel=1e-15)

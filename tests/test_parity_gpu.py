@@ -222,6 +222,42 @@ def test_adjoint_matches_central_difference(sysname, terms):
     assert rel_err(got, ref) < 1e-6
 
 
+def test_action_laplacian(sysname):
+    """Action::GetActionLaplacian (action_class.h:49): the kinetic closed form
+    bit for bit; the pair action's second differences (eps 1e-4) against the
+    oracle's.  A second difference divides the rounding of the link action by
+    eps^2 = 1e-8, so the two sides agree to n_terms * 8 * 2.2e-16 * max|U| / eps^2,
+    with max|U| the largest link action, OLD and NEW (an interface proposal)."""
+    s, pos = make(sysname)
+    o = Oracle(s)
+    o.upload(pos)
+    rng = np.random.default_rng(9)
+    with abi.Engine(s) as e:
+        e.upload(pos)
+        p, b0 = 5, 3
+        nb = 1 << s.bisect_n_level
+        new = pos[:, (b0 + 1 + np.arange(nb - 1)) % s.n_bead, p].T + rng.normal(0, 0.2, size=(nb - 1, 3))
+        for eng in (e, o):
+            eng.set_proposal(p, b0 + 1, new)
+        for mode in (abi.MODE_OLD, abi.MODE_NEW):
+            for level in range(s.bisect_n_level):
+                a = e.get_action_laplacian(abi.ACTION_KINETIC, mode, b0, b0 + nb, [p], level)
+                b = o.get_action_laplacian(abi.ACTION_KINETIC, mode, b0, b0 + nb, [p], level)
+                assert a == b, (mode, level, a, b)
+            for w0, w1 in ((b0, b0 + nb), (0, s.n_bead), (s.n_bead - 2, s.n_bead + 2)):
+                a = e.get_action_laplacian(abi.ACTION_PAIR, mode, w0, w1, [p], 0)
+                b = o.get_action_laplacian(abi.ACTION_PAIR, mode, w0, w1, [p], 0)
+                u_max = max(abs(o.get_action(abi.ACTION_PAIR, mode, w, w + 1, [p], 0)) for w in range(w0, w1))
+                n_terms = 2 * 3 * 3 * (w1 - w0) * (s.n_part - 1)
+                tol = n_terms * 8 * 2.2e-16 * max(u_max, 1e-3) / 1e-8
+                print(f"{sysname} mode {mode} window {w0}..{w1}: {a:.12g} vs {b:.12g}, tol {tol:.3g}", flush=True)
+                assert abs(a - b) <= tol, (mode, w0, w1, a, b, tol)
+                assert abs(b) > 10 * tol or s.pair_type == abi.PAIR_NONE
+            assert e.get_action_laplacian(abi.ACTION_PAIR, mode, b0, b0 + nb, [p], 1) == 0.0
+        e.reject()
+        o.reject()
+
+
 def test_adjoint_long_range_on_a_ragged_k_set():
     """The long-range adjoint walks the k-set in runs of consecutive n2; a k-set
     with holes (every third vector dropped, shuffled order) splits the runs at
