@@ -113,3 +113,43 @@ def test_free_running_chains_agree_with_oracle():
     limit = scipy.stats.chi2.ppf(0.9545, dof)
     print(f"S(k), g(r): chi2 = {chi2:.2f} over {dof} (two-sigma limit {limit:.2f})")
     assert chi2 < limit, (chi2, limit, z)
+
+
+def test_long_chain_thermal_and_virial_energies_agree():
+    """The plain-method configuration at its stated length: 32 replicas of the
+    N=32, M=32 Bare/Ewald gas run 10^4 sweeps on the device RNG (after 200 to
+    equilibrate), twice from the same seed: once accumulating the thermal energy
+    estimator, once the virial one (from the analytic adjoint).  Estimators draw
+    nothing, so both runs are the same chains (final paths equal bit for bit);
+    the two estimators of the energy must agree on them within 3 standard
+    errors of their per-replica difference."""
+    s = system.config_c1()
+    n_rep, n_eq, n_meas = 32, 200, 10000
+    out = {}
+    for est in (abi.ENERGY_THERMAL, abi.ENERGY_VIRIAL_ADJOINT):
+        with abi.Engine([s] * n_rep, seed=31337) as e:
+            for r in range(n_rep):
+                e.upload(system.random_path(s, seed=700 + r, spread=0.3), r=r)
+            e.obs_configure(energy=est)
+            e.sweep(n_eq, abi.SWEEP_GRAPH)
+            e.obs_reset(abi.OBS_ENERGY)
+            flags = abi.SWEEP_GRAPH | abi.SWEEP_ENERGY | (abi.SWEEP_ADJOINT if est == abi.ENERGY_VIRIAL_ADJOINT else 0)
+            for blk in range(10):
+                e.sweep(n_meas // 10, flags)
+                print(f"estimator {est}: {(blk + 1) * n_meas // 10} of {n_meas} sweeps", flush=True)
+            en = []
+            for r in range(n_rep):
+                v, nm = e.obs_read(abi.OBS_ENERGY, r=r)
+                assert nm == n_meas
+                en.append((v[0] + v[1]) / (s.n_bead * nm))
+            out[est] = (np.array(en), [e.download(r=r) for r in range(n_rep)])
+    (e_t, p_t), (e_v, p_v) = out[abi.ENERGY_THERMAL], out[abi.ENERGY_VIRIAL_ADJOINT]
+    for r in range(n_rep):
+        assert np.array_equal(p_t[r], p_v[r]), r
+    d = e_t - e_v
+    dm, dse = d.mean(), d.std(ddof=1) / np.sqrt(n_rep)
+    print(f"thermal {e_t.mean():.6f} +- {e_t.std(ddof=1) / np.sqrt(n_rep):.2g}, virial {e_v.mean():.6f} +- "
+          f"{e_v.std(ddof=1) / np.sqrt(n_rep):.2g}, difference {dm:.3g} +- {dse:.2g}", flush=True)
+    assert abs(dm) < 3.0 * dse, (dm, dse)
+    # the virial estimator's variance is the smaller (its purpose)
+    assert e_v.std() < e_t.std()
