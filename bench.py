@@ -435,6 +435,9 @@ def uneven_bench(a, abi, system, steps=None):
     }
 
 
+C5_TRAFFIC_RATIO = 1.048  # 16.14 MB of HBM traffic per 15.41 MB (algorithmic) rho_k launch
+
+
 def c5_bench(a, abi, system, copy_gbps, steps=None, rounds=0):
     """The combined large case: 8 replicas of HEG N=1024, M=256, r_s=1, 4012
     stored k-vectors, David n_order=2, batched with the adjoint, the virial
@@ -491,6 +494,9 @@ def c5_bench(a, abi, system, copy_gbps, steps=None, rounds=0):
         "roofline": ({"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                       "frac": ach / HBM_PEAK_GBPS, "kernel": "k_rho (incremental rho_k update + long-range action)",
                       "bytes_per_launch": rho_bytes / rho_n, "avg_launch_us": 1e3 * rho_ms / rho_n,
+                      # FETCH_SIZE / WRITE_SIZE passes over bisect rounds of this case, scaled by a copy kernel
+                      "traffic_over_algorithmic": C5_TRAFFIC_RATIO,
+                      "traffic_from": "profiles/k_rho_loop_heg1024_m256_8rep_counters.md",
                       "hbm_copy_GBps": copy_gbps, "frac_of_copy": (ach / copy_gbps) if copy_gbps else None}
                      if rho_n else None),
         "pair": ({"avg_us": 1e3 * pair_ms / pair_n, "link_evals_per_s": links / (pair_ms * 1e-3)} if pair_n else None),
