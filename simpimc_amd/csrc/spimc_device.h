@@ -886,6 +886,16 @@ struct Rng {
         sincospi(2. * u2, &sn, &cs);
         return which ? r * sn : r * cs;
     }
+    // normal(i, 0) and normal(i, 1) from one call
+    __device__ void normal2(uint32_t i, double* z0, double* z1) const {
+        uint4 w = call(i);
+        double u1 = u53(w.x, w.y), u2 = u53(w.z, w.w);
+        double r = sqrt(-2. * log(u1));
+        double sn, cs;
+        sincospi(2. * u2, &sn, &cs);
+        *z0 = r * cs;
+        *z1 = r * sn;
+    }
 };
 
 // ---------------------------------------------------------------------------

@@ -186,6 +186,10 @@ template <class Sync>
 __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync sync, bool items) {
     __shared__ double sh_old[(1 << kMaxLevel) + 1][3];
     __shared__ double sh_new[(1 << kMaxLevel) + 1][3];
+    // the attempt's normals, 3 per midpoint in level order, and level uniforms
+    // (device RNG), drawn before the level loop across the lanes
+    __shared__ double sh_z[3 * ((1 << kMaxLevel) - 1) + 1];
+    __shared__ double sh_u[kMaxLevel];
     __shared__ int s_info[4];
     __shared__ long long s_cur;
     __shared__ unsigned long long s_seq;
@@ -249,6 +253,17 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
             sh_new[j][d] = x;
         }
     }
+    if (!tape) {
+        // normal g is normal(1 + n + g / 2, g & 1): one call per pair of normals
+        const int n_norm = 3 * (nb - 1);
+        for (int q = lane; 2 * q < n_norm; q += kWave) {
+            double z0, z1;
+            rng.normal2(1 + n + q, &z0, &z1);
+            sh_z[2 * q] = z0;
+            sh_z[2 * q + 1] = z1;
+        }
+        for (int l = lane; l < n; l += kWave) sh_u[l] = rng.unif(1 + l, 0);
+    }
     sync();
     long long cur = s_cur;
     double prev = 0.;
@@ -273,10 +288,8 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
                 double z;
                 if (tape)
                     z = tape_get(R, cur + 3 * lane + d, &bad);
-                else {
-                    int g = g_base + 3 * lane + d;
-                    z = rng.normal(1 + n + g / 2, g & 1);
-                }
+                else
+                    z = sh_z[g_base + 3 * lane + d];
                 dnew[d] = sigma * z;
                 dnew[d] = minimg(dnew[d], R.L, R.iL);
                 t[d] = minimg(sh_new[c][d] - sh_new[a][d], R.L, R.iL);
@@ -309,7 +322,7 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
         if (tape)
             u = tape_get(R, cur, &bad);
         else
-            u = rng.unif(1 + level, 0);
+            u = sh_u[level];
         cur += 1;
         if (level > 0) {
             const double old_action = 0. + ko, new_action = 0. + kn;
