@@ -228,8 +228,10 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
                 b0 = 0;
             }
         } else {
-            p = (int)rng.index(0, 0, N);
-            b0 = (int)rng.index(0, 1, M);
+            // rng.index(0, 0, N) and rng.index(0, 1, M) from one call
+            const uint4 w = rng.call(0);
+            p = (int)(((uint64_t)w.x * (uint32_t)N) >> 32);
+            b0 = (int)(((uint64_t)w.y * (uint32_t)M) >> 32);
         }
         s_info[1] = p;
         s_info[2] = b0;
@@ -244,14 +246,15 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
     // its first one, so the reference's bead walks visit no midpoint and no
     // kinetic link (bisect_class.h:51-58,78)
     const bool wraps = (nb % M) == 0 && p2 == p;
-    for (int j = lane; j <= nb; j += kWave) {
-        int s = (b0 + j) % M;
+    // the window's beads (nb + 1 <= 65: two per lane at most) are loaded, the
+    // draws made while the loads are in flight, then the beads stored
+    double xw[2][3];
+    for (int k = 0; k < 2; ++k) {
+        const int j = lane + k * kWave;
+        if (j > nb) break;
+        const int s = (b0 + j) % M;
         const int lbl = (b0 + j >= M) ? p2 : p;
-        for (int d = 0; d < 3; ++d) {
-            double x = pos_at(R, d, s, lbl);
-            sh_old[j][d] = x;
-            sh_new[j][d] = x;
-        }
+        for (int d = 0; d < 3; ++d) xw[k][d] = pos_at(R, d, s, lbl);
     }
     if (!tape) {
         // normal g is normal(1 + n + g / 2, g & 1): one call per pair of normals
@@ -263,6 +266,14 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
             sh_z[2 * q + 1] = z1;
         }
         for (int l = lane; l < n; l += kWave) sh_u[l] = rng.unif(1 + l, 0);
+    }
+    for (int k = 0; k < 2; ++k) {
+        const int j = lane + k * kWave;
+        if (j > nb) break;
+        for (int d = 0; d < 3; ++d) {
+            sh_old[j][d] = xw[k][d];
+            sh_new[j][d] = xw[k][d];
+        }
     }
     sync();
     long long cur = s_cur;
@@ -838,11 +849,22 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_loop(const RhoItem* __restric
 // displace_particle_class.h:68-86), one wavefront per replica.
 // ---------------------------------------------------------------------------
 template <class Sync>
+__device__ __forceinline__ void commit_impl(const Rep& R, const Prop& pr, int lane, int acc, const double* pre,
+                                            unsigned char pre_sel, Sync sync);
+template <class Sync>
 __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
     Prop* prp = R.prop;
     const Prop pr = *prp;
     if (pr.status == 0) return;
     double po = 0., pn = 0., lo = 0., ln = 0.;
+    // the commit's loads of a one-particle move, issued before the sums: the
+    // lane's first new bead and the selector of its slice
+    double pre[3] = {0., 0., 0.};
+    unsigned char pre_sel = 0;
+    if (pr.status == 1 && pr.move != 3 && lane < pr.count) {
+        for (int d = 0; d < 3; ++d) pre[d] = R.prop_new[3 * lane + d];
+        if (R.use_long_range) pre_sel = R.sel[(pr.first + lane) % R.M];
+    }
     if (pr.status == 1) {
         for (int j = lane; j < pr.n_links; j += kWave) {
             double2 v = R.pair_part[j];
@@ -863,8 +885,9 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
         ln = wave_sum(ln);
     }
     __shared__ int s_acc;
+    int acc = 0, status = pr.status;
+    long long tape_next = pr.tape_next;
     if (lane == 0) {
-        int acc = 0;
         if (pr.status == 1) {
             const double pair_old = po + R.long_scale * lo, pair_new = pn + R.long_scale * ln;
             const double old_action = (0. + pr.k_old) + pair_old;
@@ -873,8 +896,6 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
             const double la = pr.lsr - change + pr.prev;
             acc = !(la < log(pr.u0));
         }
-        long long tape_next = pr.tape_next;
-        int status = pr.status;
         if (acc && pr.move == 3 && pr.tab_on) {
             // PermBisectTable: the table of the new path (perm_bisect_table_class.h:216-225)
             int bad = 0;
@@ -883,6 +904,11 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
             acc = !(pr.tab_ratio < u);
         }
         s_acc = acc;
+    }
+    sync();
+    commit_impl(R, pr, lane, s_acc, pre, pre_sel, sync);
+    // bookkeeping after the commit's stores are issued
+    if (lane == 0) {
         Counters* c = R.cnt;
         if (pr.status == 1 && pr.move != 3) {  // the round's pair and rho_k work, for the roofline figures
             c->rho_slicek += (unsigned long long)pr.count * (unsigned long long)R.nk;
@@ -932,8 +958,17 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
         }
         prp->status = 0;
     }
-    sync();
+}
+
+// The commit of a decided proposal (part of decide_impl): Bisect::Accept /
+// DisplaceParticle::Accept of its beads, or of every chain of a permuting
+// proposal and its relabelling.  pre / pre_sel: the lane's first new bead and
+// selector, loaded before the decision.
+template <class Sync>
+__device__ __forceinline__ void commit_impl(const Rep& R, const Prop& pr, int lane, int acc, const double* pre,
+                                            unsigned char pre_sel, Sync sync) {
     const int M = R.M, N = R.N;
+    const int s_acc = acc;
     if (s_acc && pr.move == 3) {
         // Bisect::Accept of every chain, then PermBisect::AssignParticleLabels
         // (perm_bisect_class.h:47-56): the slices after the relinked slice e take
@@ -989,8 +1024,9 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
             const int s = (pr.first + m) % M;
             // a bisection chain carries label p2 past the wrap
             const int l = (pr.move == 0 && pr.b0 + m + 1 >= M) ? pr.p2 : pr.p;
-            for (int d = 0; d < 3; ++d) R.pos[((size_t)d * M + s) * N + l] = R.prop_new[3 * m + d];
-            if (R.use_long_range) R.sel[s] ^= 1;
+            const bool first = m == lane;
+            for (int d = 0; d < 3; ++d) R.pos[((size_t)d * M + s) * N + l] = first ? pre[d] : R.prop_new[3 * m + d];
+            if (R.use_long_range) R.sel[s] = (unsigned char)((first ? pre_sel : R.sel[s]) ^ 1);
         }
     }
     if (pr.move == 3 && pr.status == 1) {
