@@ -608,6 +608,10 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
     // arithmetic in the same order per bead, with the draws in the same order.
     const bool fast = n * (nb + 1) <= min(stage_max, kPermStage);
     __shared__ double st_old[kPermStage * 3], st_new[kPermStage * 3];
+    // the LDS form's normals (normal g is normal(c_norm + g / 2, g & 1), 3 per
+    // midpoint of the open chains in level order) and level uniforms, drawn by
+    // the whole workgroup before the levels (device RNG)
+    __shared__ double st_z[kPermStage * 3 + 1], st_u[kMaxLevel];
     const int per = (nb + 1) * 3;
     if (fast) {
         for (int idx = tid; idx < n * per; idx += kPermBlock) {
@@ -616,6 +620,16 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
             st_old[idx] = x;
             // NEW mode: the end of chain i is the old end of chain ch_nxt[i]
             st_new[idx] = t == nb ? chain_old(R, b0, nb, R.ch_nxt[i], d) : x;
+        }
+        if (!tape) {
+            const int n_norm = 3 * n * (nb - 1);  // at least the open chains' count
+            for (int q = tid; 2 * q < n_norm; q += kPermBlock) {
+                double z0, z1;
+                rng.normal2(c_norm + (uint32_t)q, &z0, &z1);
+                st_z[2 * q] = z0;
+                st_z[2 * q + 1] = z1;
+            }
+            for (int l = tid; l < n_lev; l += kPermBlock) st_u[l] = rng.unif(c_lev + (uint32_t)l, 0);
         }
     } else {
         // NEW positions start as the old ones
@@ -654,12 +668,10 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
                     }
                     for (int d = 0; d < 3; ++d) {
                         double z;
-                        if (tape) {
+                        if (tape)
                             z = tape_get(R, cur + 3 * it + d, &bad);
-                        } else {
-                            const uint32_t g = g_base + 3u * (uint32_t)it + (uint32_t)d;
-                            z = rng.normal(c_norm + g / 2u, (int)(g & 1u));
-                        }
+                        else
+                            z = st_z[g_base + 3u * (uint32_t)it + (uint32_t)d];
                         dnew[d] = sigma * z;
                         dnew[d] = minimg(dnew[d], R.L, R.iL);
                         const double xa = sn[3 * a + d];
@@ -699,7 +711,7 @@ __global__ __launch_bounds__(kPermBlock) void k_propose_perm(const Rep* __restri
                 g_base += 3u * (uint32_t)(n_open * nmid);
                 double change = 0.;
                 if (lane == 0) {
-                    const double u = tape ? tape_get(R, cur, &bad) : rng.unif(c_lev + (uint32_t)level, 0);
+                    const double u = tape ? tape_get(R, cur, &bad) : st_u[level];
                     rej = perm_level_step(R, pr, level, lspo, lspn, ko, kn, prev, u, bad, b0, nb, n, n_lev, cur, &s_i[7],
                                           &change);
                 }

@@ -347,3 +347,36 @@ def test_walk_without_candidates_is_rejected_free_running():
         e.tape_set(tape)
         with pytest.raises(abi.SpimcError):
             e.perm_bisect(200)
+
+
+@pytest.mark.parametrize("table", [False, True])
+def test_lds_and_global_forms_agree_on_the_device_rng(table, monkeypatch):
+    """The permuting proposal's two forms (a few chains bisected by one
+    wavefront on LDS-staged beads, with the attempt's normals and level
+    uniforms drawn up front; the workgroup-wide form on global memory, which
+    draws each where it is used, SPIMC_PERM_STAGE=0) take the same draws in
+    the same order: free-running on the device RNG they give the same chains
+    bit for bit, for PermBisectIterative and for PermBisectTable."""
+    s = dense_bare()
+    out = []
+    for stage in ("256", "0"):
+        monkeypatch.setenv("SPIMC_PERM_STAGE", stage)
+        with abi.Engine([s] * 4, seed=606) as e:
+            for r in range(4):
+                e.upload(system.random_path(s, seed=60 + r, spread=0.3), r=r)
+                if table:
+                    e.perm_table_configure(3, 1, r=r)
+            for _ in range(6):
+                (e.perm_table if table else e.perm_bisect)(20)
+                e.bisect(16)
+            out.append(([e.download(r=r) for r in range(4)], [e.perm(r=r) for r in range(4)],
+                        [e.perm_counters(r=r) for r in range(4)]))
+    (p0, s0, c0), (p1, s1, c1) = out
+    moved = 0
+    for r in range(4):
+        assert np.array_equal(p0[r], p1[r]), r
+        assert np.array_equal(s0[r], s1[r]), r
+        assert c0[r][0] == c1[r][0] and c0[r][1] == c1[r][1], r
+        assert np.array_equal(c0[r][3], c1[r][3]), r
+        moved += int(c0[r][3][1:].sum())
+    assert moved > 0  # multi-particle cycles were accepted
