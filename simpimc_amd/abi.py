@@ -389,10 +389,11 @@ class Engine:
     def autotune(self, n_rounds: int = 32) -> dict:
         """Try the launch shapes on n_rounds bisection attempts each (real moves of
         the chains) and keep the fastest; returns the shape chosen."""
-        out = (C.c_int32 * 6)()
+        out = (C.c_int32 * 7)()
         check(self.lib.spimc_autotune(self.h, n_rounds, out))
         return {"fork": bool(out[0]), "rho_workgroups_per_cu": int(out[1]), "pair_workgroup": int(out[2]),
-                "shapes_tried": int(out[3]), "displace_rounds": int(out[4]), "persistent_workgroup": int(out[5])}
+                "shapes_tried": int(out[3]), "displace_rounds": int(out[4]), "persistent_workgroup": int(out[5]),
+                "one_launch": bool(out[6])}
 
     def bead_updates(self) -> int:
         v = C.c_uint64()

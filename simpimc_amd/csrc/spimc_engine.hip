@@ -371,6 +371,10 @@ struct Group {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // permuting rounds: the proposals for the label-general kernels (2 counters, 2 lists of n)
     unsigned int* d_multi = nullptr;
+    // one-launch bisection rounds: each rho_k item's replica (group-local index)
+    // and the replicas' arrival counters (zero between rounds)
+    int* d_rrep_bis = nullptr;
+    unsigned int* d_arrive = nullptr;
 };
 
 struct spimc_ctx {
@@ -449,6 +453,10 @@ struct spimc_ctx {
     int pers_block = 0;
     void (*pers_fn)(const Rep*, int, int, int, int) = nullptr;
     hipGraphExec_t pers_graph[2] = {nullptr, nullptr};  // stage begin + persistent stage, per move
+    // a round in one launch (k_round: pair links, rho_k items, then the replica's
+    // decision and next proposal by its last workgroup), for small batches
+    bool one_launch = false;
+    void (*round_fn)(const Rep*, int, int, const RhoItem*, const int*, unsigned int*) = nullptr;
     std::string noalloc_error;  // why the address lookup failed, if it did
 };
 
@@ -1007,6 +1015,7 @@ int build_work_lists(spimc_ctx* c, Group& G) {
         HIPCHK(hipMemset(d, 0, n * sizeof(RhoItem)));
     }
     int ob = 0, od = 0;
+    std::vector<int> mb;
     for (int r = G.r0; r < G.r0 + G.n; ++r) {
         Rep& R = c->reps[r].dev;
         if (!R.use_long_range) continue;
@@ -1015,8 +1024,17 @@ int build_work_lists(spimc_ctx* c, Group& G) {
         R.ritems_dis = G.d_ritem_dis + od;
         R.n_ritems_bis = nb;
         R.n_ritems_dis = nd;
+        mb.insert(mb.end(), nb, r - G.r0);
         ob += nb;
         od += nd;
+    }
+    if (!mb.empty()) {
+        HIPCHK(hipMalloc((void**)&G.d_rrep_bis, mb.size() * sizeof(int)));
+        HIPCHK(hipMemcpy(G.d_rrep_bis, mb.data(), mb.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (!G.d_arrive) {
+        HIPCHK(hipMalloc((void**)&G.d_arrive, G.n * sizeof(unsigned int)));
+        HIPCHK(hipMemset(G.d_arrive, 0, G.n * sizeof(unsigned int)));
     }
     return SPIMC_OK;
 }
@@ -1122,6 +1140,12 @@ bool pers_eligible(const spimc_ctx* c) {
             stage_draws(0, h.dev.n_level) > kStageMaxDraws)
             return false;
     return pers_shmem(c) <= kPersLdsBytes;
+}
+
+// One-launch rounds: the LDS pair kernel, the default rho_k chunking, rounds
+// decided in the next proposal's launch, and unpermuted paths.
+bool one_launch_eligible(const spimc_ctx* c) {
+    return c->round_fn && c->lds_doubles_max > 0 && c->fuse_rounds && !c->perm_on && c->rho_kpt == kRhoKpt;
 }
 
 
@@ -1273,8 +1297,22 @@ int launch_perm_round(spimc_ctx* c, const Group& G) {
     return launch(c, SPIMC_KERNEL_DECIDE, k_decide_perm, R, 64, 0, st, reps, R, G.d_multi);
 }
 
+// A bisection round of a small batch in one launch (k_round); the group's
+// first proposals come from a k_decide_propose_bisect launch at the stage's
+// start.  (Displace rounds stay three launches: their many rho_k items exceed
+// what the GPU holds at once, and the workgroups that decide then delay the
+// rest; measured slower.)
+int launch_one_round(spimc_ctx* c, const Group& G) {
+    const int R = G.n, waves = kRoundBlock / 64;
+    const int bpr = std::max(1, std::min((G.max_bis_links + waves - 1) / waves, (2 * kNumCUs + R - 1) / R));
+    const size_t shmem = (size_t)c->lds_doubles_max * sizeof(double);
+    return launch(c, SPIMC_KERNEL_PAIR, c->round_fn, R * bpr + G.n_ritem_bis, kRoundBlock, shmem, G.stream,
+                  c->d_reps + G.r0, R, bpr, (const RhoItem*)G.d_ritem_bis, (const int*)G.d_rrep_bis, G.d_arrive);
+}
+
 int launch_round(spimc_ctx* c, const Group& G, int move) {
     if (move == 3) return launch_perm_round(c, G);
+    if (c->one_launch && move == 0 && one_launch_eligible(c)) return launch_one_round(c, G);
     const int R = G.n;
     unsigned long long* kt = c->clock ? G.d_kt : nullptr;
     Rep* reps = c->d_reps + G.r0;  // kernels indexed by block: replicas of this group
@@ -1338,7 +1376,8 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
     return launch(c, SPIMC_KERNEL_DECIDE, k_decide, R, 64, 0, st, reps, R, kt);
 }
 
-// The decision of a stage's last round when rounds are fused.
+// The decision of a stage's last round when rounds are fused (one-launch rounds
+// decide their own).
 int launch_final_decide(spimc_ctx* c, const Group& G) {
     if (!c->fuse_rounds) return SPIMC_OK;
     unsigned long long* kt = c->clock ? G.d_kt : nullptr;
@@ -1501,6 +1540,17 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
         HIPCHK(hipEventRecord(c->stage_ready, c->stream));
         for (int g = 1; g < n_groups; ++g) HIPCHK(hipStreamWaitEvent(c->groups[g].stream, c->stage_ready, 0));
     }
+    // one-launch rounds: each group's first proposals, then rounds that decide
+    // and propose on their own (the last round's proposals find no attempt left)
+    const bool one = c->one_launch && move == 0 && one_launch_eligible(c);
+    if (one)
+        for (int g = 0; g < n_groups; ++g) {
+            const Group& G = c->groups[g];
+            if (!gmx[g]) continue;
+            rc = launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_bisect, G.n, 64, 0, G.stream, c->d_reps + G.r0, G.n,
+                        (unsigned long long*)nullptr);
+            if (rc) return rc;
+        }
     // rounds issued group by group in turn, so the streams advance together
     const uint32_t step = graph ? kRoundsPerGraph : 1;
     for (uint32_t t = 0; t < mx; t += step)
@@ -1512,7 +1562,7 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
             else if ((rc = launch_round(c, G, move)))
                 return rc;
         }
-    if (move != 3)
+    if (move != 3 && !one)
         for (int g = 0; g < n_groups; ++g)
             if (gmx[g] && (rc = launch_final_decide(c, c->groups[g]))) return rc;
     for (int g = 1; g < n_groups; ++g) {
@@ -1827,6 +1877,21 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         const int vi = pt == SPIMC_PAIR_BARE ? 0 : pt == SPIMC_PAIR_DAVID ? (nord == 2 ? 1 : 2) : 3;
         c->pair_variant = vi;
         if (set_pair_block(c, bs) != SPIMC_OK) c->lds_doubles_max = 0;
+        {
+            // one-launch rounds (SPIMC_ONE_LAUNCH or spimc_autotune): the pair type's
+            // instantiation with the default rho_k chunking
+            using RoundFn = void (*)(const Rep*, int, int, const RhoItem*, const int*, unsigned int*);
+            const RoundFn rf[2][4] = {{k_round<1, 0, kRhoKpt, false>, k_round<2, 2, kRhoKpt, false>,
+                                       k_round<2, 0, kRhoKpt, false>, k_round<0, 0, kRhoKpt, false>},
+                                      {k_round<1, 0, kRhoKpt, true>, k_round<2, 2, kRhoKpt, true>,
+                                       k_round<2, 0, kRhoKpt, true>, k_round<0, 0, kRhoKpt, true>}};
+            c->round_fn = rf[c->rho_nt ? 1 : 0][vi];
+            if (c->lds_doubles_max > 0 &&
+                hipFuncSetAttribute((const void*)c->round_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    c->lds_doubles_max * (int)sizeof(double)) != hipSuccess)
+                c->round_fn = nullptr;
+            if (const char* fo = std::getenv("SPIMC_ONE_LAUNCH")) c->one_launch = std::atoi(fo) != 0;
+        }
         // stages of small replicas in one persistent launch: off by default, set by
         // spimc_autotune or SPIMC_PERSISTENT (workgroup size 256, 512 or 1024)
         if (const char* fp = std::getenv("SPIMC_PERSISTENT")) {
@@ -1902,7 +1967,7 @@ int spimc_ctx_destroy(spimc_ctx* c) {
         for (auto& g : G.graph)
             if (g) (void)hipGraphExecDestroy(g);
         for (void* p : {(void*)G.d_pair_bis, (void*)G.d_pair_dis, (void*)G.d_ritem_bis, (void*)G.d_ritem_dis, (void*)G.d_kt,
-                        (void*)G.d_multi})
+                        (void*)G.d_multi, (void*)G.d_rrep_bis, (void*)G.d_arrive})
             if (p) (void)hipFree(p);
         if (G.done) (void)hipEventDestroy(G.done);
         if (G.side) {
@@ -2301,7 +2366,8 @@ int spimc_set_bisect_level(spimc_ctx* c, uint32_t r, uint32_t n_level) {
     h.d.bisect_n_level = n_level;
     for (auto& G : c->groups) {
         if ((int)r < G.r0 || (int)r >= G.r0 + G.n) continue;
-        for (void** p : {(void**)&G.d_pair_bis, (void**)&G.d_pair_dis, (void**)&G.d_ritem_bis, (void**)&G.d_ritem_dis})
+        for (void** p : {(void**)&G.d_pair_bis, (void**)&G.d_pair_dis, (void**)&G.d_ritem_bis, (void**)&G.d_ritem_dis,
+                         (void**)&G.d_rrep_bis})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;
@@ -2630,17 +2696,20 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
     if (n_rounds == 0) return fail(SPIMC_ERR_ARG, "n_rounds must be positive");
     if (c->prop_set) return fail(SPIMC_ERR_STATE, "an interface proposal is pending; accept or reject it first");
     struct Shape {
-        int fork, bpc, bs, pers;
+        int fork, bpc, bs, pers, one;
     };
     // launch shapes tried: rho_k beside the pair kernel with 1-3 looping workgroups
     // per CU and 256/512-thread pair workgroups, or the two kernels one after the
     // other; for small replicas also the persistent stage kernel with 256-1024 threads
-    std::vector<Shape> shapes = {{1, 2, 512, 0}, {1, 3, 512, 0}, {1, 1, 512, 0}, {1, 2, 256, 0},
-                                 {1, 3, 256, 0}, {0, 0, 1024, 0}, {0, 2, 512, 0}};
-    if (!c->any_long || c->lds_doubles_max == 0) shapes = {{0, 0, 1024, 0}, {0, 0, 512, 0}, {0, 0, 256, 0}};
+    std::vector<Shape> shapes = {{1, 2, 512, 0, 0}, {1, 3, 512, 0, 0}, {1, 1, 512, 0, 0}, {1, 2, 256, 0, 0},
+                                 {1, 3, 256, 0, 0}, {0, 0, 1024, 0, 0}, {0, 2, 512, 0, 0}};
+    if (!c->any_long || c->lds_doubles_max == 0)
+        shapes = {{0, 0, 1024, 0, 0}, {0, 0, 512, 0, 0}, {0, 0, 256, 0, 0}};
     if (pers_eligible(c))
-        for (int pb : {256, 512, 1024}) shapes.push_back({c->fork ? 1 : 0, c->rho_bpc, c->pair_block, pb});
-    const Shape saved{c->fork ? 1 : 0, c->rho_bpc, c->pair_block, c->pers_block};
+        for (int pb : {256, 512, 1024}) shapes.push_back({c->fork ? 1 : 0, c->rho_bpc, c->pair_block, pb, 0});
+    // batches of at most one replica per CU: also the one-launch rounds
+    if (one_launch_eligible(c) && (int)c->reps.size() <= kNumCUs) shapes.push_back({0, 0, c->pair_block, 0, 1});
+    const Shape saved{c->fork ? 1 : 0, c->rho_bpc, c->pair_block, c->pers_block, c->one_launch ? 1 : 0};
     // the sweep's mix: N M / 2^l bisection attempts per N displace attempts
     const Rep& R0 = c->reps[0].dev;
     const uint32_t n_dis = std::max<uint32_t>(1, (uint32_t)((uint64_t)n_rounds * (1u << R0.n_level) / R0.M));
@@ -2650,6 +2719,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
     for (const Shape& sh : shapes) {
         c->fork = sh.fork != 0;
         c->rho_bpc = sh.bpc;
+        c->one_launch = sh.one != 0;
         int rc = c->lds_doubles_max > 0 ? set_pair_block(c, sh.bs) : SPIMC_OK;
         if (rc == SPIMC_OK) rc = set_pers_block(c, sh.pers);
         if (rc) continue;
@@ -2663,6 +2733,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
         if (rc) {
             c->fork = saved.fork != 0;
             c->rho_bpc = saved.bpc;
+            c->one_launch = saved.one != 0;
             if (c->lds_doubles_max > 0) (void)set_pair_block(c, saved.bs);
             (void)set_pers_block(c, saved.pers);
             return rc;
@@ -2675,6 +2746,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
     }
     c->fork = best.fork != 0;
     c->rho_bpc = best.bpc;
+    c->one_launch = best.one != 0;
     if (c->lds_doubles_max > 0) {
         int rc = set_pair_block(c, best.bs);
         if (rc) return rc;
@@ -2688,6 +2760,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
         chosen[3] = tried;
         chosen[4] = (int32_t)n_dis;
         chosen[5] = c->pers_block;
+        chosen[6] = c->one_launch ? 1 : 0;
     }
     return SPIMC_OK;
 }

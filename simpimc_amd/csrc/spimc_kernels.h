@@ -50,6 +50,20 @@ __device__ __forceinline__ T gld(const T* p) {
     return *(const __attribute__((address_space(1))) T*)p;
 }
 
+// Partial sums passed between the workgroups of one launch (k_round): 64-bit
+// atomic stores and loads at agent scope, which the memory model performs at
+// the device's coherence point, so a workgroup on another XCD reads them
+// without any L2 write-back or invalidation of the caches.
+__device__ __forceinline__ void coh_store2(double2* p, double2 v) {
+    __hip_atomic_store(&p->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double2 coh_load2(const double2* p) {
+    double2* q = const_cast<double2*>(p);
+    return make_double2(__hip_atomic_load(&q->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                        __hip_atomic_load(&q->y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
 // 16-byte global loads and stores of one rho_k value; NT marks them
 // non-temporal (streamed through, not kept in the caches)
 typedef double d2v __attribute__((ext_vector_type(2)));
@@ -605,8 +619,9 @@ __device__ __forceinline__ void kt_close(unsigned long long* kt) {
 // j = w, w + 16, ...; lanes run over the partners q.
 constexpr int kPairLdsBlock = 1024;
 // The window links j = j0, j0 + step, ... of a proposal, one wavefront: lanes
-// run over the partners q; the OLD and NEW sums of link j go to pair_part[j].
-template <int PT, int NORD>
+// run over the partners q; the OLD and NEW sums of link j go to pair_part[j]
+// (COH: as agent-scope atomic stores).
+template <int PT, int NORD, bool COH = false>
 __device__ __forceinline__ void pair_links_lds(const Rep& R, const double* lds, const Prop& pr, int j0, int step,
                                                int lane) {
     const int M = R.M, N = R.N, p = pr.p;
@@ -647,7 +662,10 @@ __device__ __forceinline__ void pair_links_lds(const Rep& R, const double* lds, 
         uo = wave_sum(uo);
         un = wave_sum(un);
         if (lane == 0) {
-            R.pair_part[j] = make_double2(uo, un);
+            if (COH)
+                coh_store2(R.pair_part + j, make_double2(uo, un));
+            else
+                R.pair_part[j] = make_double2(uo, un);
         }
     }
 }
@@ -770,19 +788,76 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho(const Rep* __restrict__ reps,
     kt_end(kt, KT_RHO);
 }
 
+// One rho_k work item on a whole kRhoBlock-thread workgroup: its rho_k, k-table
+// and phase-chain loads are all issued before the first wait; the item's
+// (OLD, NEW) sums go to *d.part.  The caller separates items that reuse tab and
+// red with a workgroup barrier.  COH: the sums go out as agent-scope atomic
+// stores.
+template <int KPT, bool NT, bool COH = false>
+__device__ __forceinline__ void rho_item(const RhoItem& d, double2* tab, double (*red)[kRhoBlock / kWave]) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int k0 = d.k0 + threadIdx.x;
+    double2 rv[KPT];
+    int kp[KPT];
+    double cf[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const int k = k0 + i * kRhoBlock;
+        const bool ok = k < d.k1;
+        rv[i] = ok ? ld2<NT>(d.rc + k) : make_double2(0., 0.);
+        kp[i] = ok ? gld(d.kidx + k) : (128 | (128 << 8) | (128 << 16));
+        cf[i] = ok ? gld(d.coef + k) : 0.;
+    }
+    const int nt = 6 * d.stride, tx = (int)threadIdx.x;
+    const double2 t0 = tx < nt ? ld2<false>(d.ph + tx) : make_double2(0., 0.);
+    const double2 t1 = tx + kRhoBlock < nt ? ld2<false>(d.ph + tx + kRhoBlock) : make_double2(0., 0.);
+    if (tx < nt) tab[tx] = t0;
+    if (tx + kRhoBlock < nt) tab[tx + kRhoBlock] = t1;
+    __syncthreads();
+    double ao = 0., an = 0.;
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const int k = k0 + i * kRhoBlock;
+        const double2 en = phase_k(&tab[3 * d.stride], d.stride, kp[i]);
+        const double2 eo = phase_k(tab, d.stride, kp[i]);
+        double2 nv;
+        nv.x = rv[i].x + (en.x - eo.x);
+        nv.y = rv[i].y + (en.y - eo.y);
+        if (k < d.k1) st2<NT>(d.rn + k, nv);
+        ao += cf[i] * (rv[i].x * rv[i].x + rv[i].y * rv[i].y);
+        an += cf[i] * (nv.x * nv.x + nv.y * nv.y);
+    }
+    ao = wave_sum(ao);
+    an = wave_sum(an);
+    if (lane == 0) {
+        red[0][w] = ao;
+        red[1][w] = an;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double to = 0., tn = 0.;
+        for (int i = 0; i < kRhoBlock / kWave; ++i) {
+            to += red[0][i];
+            tn += red[1][i];
+        }
+        if (COH)
+            coh_store2(d.part, make_double2(to, tn));
+        else
+            *d.part = make_double2(to, tn);
+    }
+}
+
 // K2 with a fixed number of workgroups that loop over the work items: the
 // launch then holds at most that many workgroups per CU, so a concurrent pair
 // kernel on another stream always finds room on every CU.
 // Each item is one descriptor (scalar loads, prefetched while the previous
-// item computes); its rho_k, k-table and phase-chain loads are all issued
-// before the first wait.
+// item computes).
 template <int KPT, bool NT>
 __global__ __launch_bounds__(kRhoBlock) void k_rho_loop(const RhoItem* __restrict__ items, int n_items,
                                                         unsigned long long* __restrict__ kt) {
     kt_begin(kt, KT_RHO);
     __shared__ double2 tab[6 * (kMaxIdx + 1)];
     __shared__ double red[2][kRhoBlock / kWave];
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     int it = blockIdx.x;
     RhoItem d;
     if (it < n_items) d = items[it];
@@ -791,52 +866,7 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_loop(const RhoItem* __restric
         const int nx = it + gridDim.x;
         if (nx < n_items) nd = items[nx];
         if (d.valid) {
-            const int k0 = d.k0 + threadIdx.x;
-            double2 rv[KPT];
-            int kp[KPT];
-            double cf[KPT];
-#pragma unroll
-            for (int i = 0; i < KPT; ++i) {
-                const int k = k0 + i * kRhoBlock;
-                const bool ok = k < d.k1;
-                rv[i] = ok ? ld2<NT>(d.rc + k) : make_double2(0., 0.);
-                kp[i] = ok ? gld(d.kidx + k) : (128 | (128 << 8) | (128 << 16));
-                cf[i] = ok ? gld(d.coef + k) : 0.;
-            }
-            const int nt = 6 * d.stride, tx = (int)threadIdx.x;
-            const double2 t0 = tx < nt ? ld2<false>(d.ph + tx) : make_double2(0., 0.);
-            const double2 t1 = tx + kRhoBlock < nt ? ld2<false>(d.ph + tx + kRhoBlock) : make_double2(0., 0.);
-            if (tx < nt) tab[tx] = t0;
-            if (tx + kRhoBlock < nt) tab[tx + kRhoBlock] = t1;
-            __syncthreads();
-            double ao = 0., an = 0.;
-#pragma unroll
-            for (int i = 0; i < KPT; ++i) {
-                const int k = k0 + i * kRhoBlock;
-                const double2 en = phase_k(&tab[3 * d.stride], d.stride, kp[i]);
-                const double2 eo = phase_k(tab, d.stride, kp[i]);
-                double2 nv;
-                nv.x = rv[i].x + (en.x - eo.x);
-                nv.y = rv[i].y + (en.y - eo.y);
-                if (k < d.k1) st2<NT>(d.rn + k, nv);
-                ao += cf[i] * (rv[i].x * rv[i].x + rv[i].y * rv[i].y);
-                an += cf[i] * (nv.x * nv.x + nv.y * nv.y);
-            }
-            ao = wave_sum(ao);
-            an = wave_sum(an);
-            if (lane == 0) {
-                red[0][w] = ao;
-                red[1][w] = an;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double to = 0., tn = 0.;
-                for (int i = 0; i < kRhoBlock / kWave; ++i) {
-                    to += red[0][i];
-                    tn += red[1][i];
-                }
-                *d.part = make_double2(to, tn);
-            }
+            rho_item<KPT, NT>(d, tab, red);
             __syncthreads();  // tab and red are reused by the next item
         }
         d = nd;
@@ -851,7 +881,9 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_loop(const RhoItem* __restric
 template <class Sync>
 __device__ __forceinline__ void commit_impl(const Rep& R, const Prop& pr, int lane, int acc, const double* pre,
                                             unsigned char pre_sel, Sync sync);
-template <class Sync>
+// COH: the pair and rho_k sums are read with agent-scope atomic loads (they were
+// written in the same launch, by workgroups on any XCD).
+template <bool COH = false, class Sync>
 __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
     Prop* prp = R.prop;
     const Prop pr = *prp;
@@ -867,14 +899,14 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
     }
     if (pr.status == 1) {
         for (int j = lane; j < pr.n_links; j += kWave) {
-            double2 v = R.pair_part[j];
+            const double2 v = COH ? coh_load2(R.pair_part + j) : R.pair_part[j];
             po += v.x;
             pn += v.y;
         }
         if (R.use_long_range) {
             const int np = pr.count * R.n_chunks;
             for (int j = lane; j < np; j += kWave) {
-                double2 v = R.rho_part[j];
+                const double2 v = COH ? coh_load2(R.rho_part + j) : R.rho_part[j];
                 lo += v.x;
                 ln += v.y;
             }
@@ -1071,6 +1103,80 @@ __global__ __launch_bounds__(64) void k_decide_propose_displace(const Rep* __res
     decide_body(reps, n_reps, kt);
     __syncthreads();
     propose_displace_body(reps, n_reps);
+}
+
+// Ordering point of the one-wavefront bodies when they run in the first
+// wavefront of a larger workgroup whose other wavefronts have left: the
+// wavefront's memory operations complete, and no code moves across it.
+struct WaveSync {
+    __device__ void operator()() const {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+};
+
+// A whole bisection round in one launch, for batches too small to fill the
+// GPU (a round is then bound by latency, not by throughput).  Workgroups [0, n_reps * bpr)
+// (after the XCD remap) run the window links of the LDS pair kernel, the
+// others one rho_k item each (item_rep: the item's replica).  Every workgroup
+// then counts itself in on its replica's arrival counter; the last of the
+// replica's bpr + n_items workgroups decides and commits the replica's
+// proposal (K5) and makes its next one (K4), with its rho_k items, in its
+// first wavefront.  A replica's round thus ends with its own last workgroup:
+// no launch boundary separates the sums from the decision, and nothing waits
+// (the counter is only read by the atomic that increments it).  The sums pass
+// between workgroups as agent-scope atomic stores and loads, each waited for
+// before its workgroup arrives; agent-scope fences would instead write back a
+// whole L2 per workgroup, which measured slower than three launches per round.
+// The decision and proposal are the same code as in k_decide_propose_*, so the
+// chains are the same bit for bit.  The last workgroup resets the counter for
+// the next round.
+constexpr int kRoundBlock = kRhoBlock;
+template <int PT, int NORD, int KPT, bool NT>
+__global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ reps, int n_reps, int bpr,
+                                                       const RhoItem* __restrict__ items,
+                                                       const int* __restrict__ item_rep,
+                                                       unsigned int* __restrict__ arrive) {
+    extern __shared__ double lds[];
+    __shared__ double2 tab[6 * (kMaxIdx + 1)];
+    __shared__ double red[2][kRhoBlock / kWave];
+    __shared__ int s_last;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int n_pair = n_reps * bpr, idx = xcd_remap(blockIdx.x, gridDim.x);
+    int r;
+    if (idx < n_pair) {
+        r = idx / bpr;
+        const int part = idx % bpr;
+        const Rep R = reps[r];
+        const Prop pr = *R.prop;
+        if (pr.status == 1 && !pr.generic && part * (kRoundBlock / kWave) < pr.n_links) {
+            const int na = R.bt_a.ncoef * R.bt_a.ncomp;
+            for (int t = threadIdx.x; t < R.lds_doubles; t += kRoundBlock)
+                lds[t] = t < na ? R.bt_a.c[t] : R.bt_b.c[t - na];
+            __syncthreads();
+            pair_links_lds<PT, NORD, true>(R, lds, pr, part * (kRoundBlock / kWave) + w, bpr * (kRoundBlock / kWave),
+                                           lane);
+        }
+    } else {
+        const int it = idx - n_pair;
+        r = item_rep[it];
+        const RhoItem d = items[it];
+        if (d.valid) rho_item<KPT, NT, true>(d, tab, red);
+    }
+    // every wavefront's sums are performed before the workgroup arrives
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const Rep& R = reps[r];
+        const unsigned int expect = (unsigned int)bpr + (unsigned int)R.n_ritems_bis;
+        s_last = atomicAdd(arrive + r, 1u) + 1u == expect;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x >= kWave) return;
+    if (lane == 0) __hip_atomic_store(arrive + r, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    decide_impl<true>(reps[r], lane, WaveSync{});
+    WaveSync{}();
+    propose_bisect_impl(reps[r], lane, WaveSync{}, true);
 }
 
 // Sums the per-replica work counters into work[WORK_PAIR_LINKS], work[WORK_RHO_SLICEK].

@@ -2,7 +2,8 @@
 M in {16, 32, 64} and r_s in [1, 4], each on its own box and k-set (the
 reference's MPI clones, framework_class.h:44-53), advanced together by whole
 sweeps in one context, eagerly and from captured hipGraphs, with the stages
-run as rounds of launches or as one persistent launch per stage.
+run as rounds of launches (three per round, or one whose last workgroup per
+replica decides and proposes) or as one persistent launch per stage.
 
 Every replica replays its own oracle chain: a sweep of N M / 2^l bisections
 (bisect_class.h:39-124) and N displacements (displace_particle_class.h:28-72),
@@ -48,15 +49,19 @@ def rel_err(a, b):
     return float(np.max(np.abs(a - b))) / max(1.0, float(np.max(np.abs(b)))) if b.size else 0.0
 
 
-# stages as rounds of launches, or each in one persistent launch (workgroup size)
-@pytest.mark.parametrize("graph,persistent", [(False, 0), (True, 0), (False, 256), (True, 256), (False, 1024)])
-def test_c4_batch_replays_every_replica(graph, persistent, monkeypatch):
+# stages as rounds of launches (one: each round in one launch), or each stage in
+# one persistent launch (workgroup size)
+@pytest.mark.parametrize("graph,persistent,one", [(False, 0, 0), (True, 0, 0), (False, 256, 0), (True, 256, 0),
+                                                  (False, 1024, 0), (False, 0, 1), (True, 0, 1)])
+def test_c4_batch_replays_every_replica(graph, persistent, one, monkeypatch):
     monkeypatch.setenv("SPIMC_PERSISTENT", str(persistent))
+    monkeypatch.setenv("SPIMC_ONE_LAUNCH", str(one))
     systems, ref = oracle_chains()
     assert len(systems) == 64
     assert {s.n_part for s in systems} == {16, 32, 64, 128} and {s.n_bead for s in systems} == {16, 32, 64}
     assert all(s.n_k == 257 for s in systems)
-    print(f"C4: {sum(s.n_part * s.n_bead for s in systems)} beads, graph={graph}, persistent={persistent}", flush=True)
+    print(f"C4: {sum(s.n_part * s.n_bead for s in systems)} beads, graph={graph}, persistent={persistent}, one={one}",
+          flush=True)
     with abi.Engine(systems, seed=1) as e:
         for i, r in enumerate(ref):
             e.upload(r["pos"], r=i)

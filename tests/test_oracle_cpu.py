@@ -305,7 +305,7 @@ def test_library_exports_every_declared_symbol():
     assert names == set(abi.SIGNATURES)
     for n in names:
         assert hasattr(lib, n)
-    assert lib.spimc_abi_version() == 5
+    assert lib.spimc_abi_version() == 6
 
 
 def test_context_creation_errors_are_reported():

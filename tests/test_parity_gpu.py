@@ -102,7 +102,12 @@ def test_whole_path_estimators(sysname):
             assert abs(a - b) <= RTOL * max(1.0, abs(b)), (w, a, b)
 
 
-def test_replay_accept_reject_bit_exact(sysname):
+@pytest.mark.parametrize("one", ["0", "1"])
+def test_replay_accept_reject_bit_exact(sysname, one, monkeypatch):
+    """Bisection and displace stages replay the oracle's tape bit for bit, with
+    rounds of three launches and with rounds of one launch (SPIMC_ONE_LAUNCH;
+    the global-table replica keeps three)."""
+    monkeypatch.setenv("SPIMC_ONE_LAUNCH", one)
     s, pos = make(sysname, spread=0.05)
     o = Oracle(s, seed=11)
     o.upload(pos)
@@ -321,13 +326,16 @@ def test_virial_adjoint_estimator(sysname):
     assert rel_err(ge, re) < 1e-6
 
 
-@pytest.mark.parametrize("streams,fork", [("1", "0"), ("3", "0"), ("2", "1")])
-def test_batched_replicas_match_single_chains(streams, fork, monkeypatch):
+@pytest.mark.parametrize("streams,fork,one", [("1", "0", "0"), ("3", "0", "0"), ("2", "1", "0"), ("1", "0", "1"),
+                                              ("3", "0", "1")])
+def test_batched_replicas_match_single_chains(streams, fork, one, monkeypatch):
     """Replicas advanced together (on one stream, split into groups on three
     streams, or two groups whose rho_k kernels run on a second stream beside the
-    pair kernels) replay each single chain's tape bit for bit."""
+    pair kernels; rounds of three launches or of one) replay each single chain's
+    tape bit for bit."""
     monkeypatch.setenv("SPIMC_STREAMS", streams)
     monkeypatch.setenv("SPIMC_FORK", fork)
+    monkeypatch.setenv("SPIMC_ONE_LAUNCH", one)
     monkeypatch.setenv("SPIMC_RHO_BPC", "3" if fork == "1" else "0")
     systems = [system.config_c1(n_part=n, n_bead=m, r_s=rs) for n, m, rs in [(16, 16, 1.5), (32, 32, 2.0), (8, 16, 3.0), (24, 8, 1.2)]]
     tapes, refs, finals = [], [], []
@@ -406,12 +414,13 @@ def test_mixed_pair_actions_graph_sweep():
             assert np.array_equal(e.obs_read(abi.OBS_ENERGY, r=i)[0], g.obs_read(abi.OBS_ENERGY, r=i)[0]), i
 
 
-@pytest.mark.parametrize("streams,fork", [("1", "0"), ("2", "0"), ("2", "1")])
-def test_uneven_counts_and_graph_sweep(streams, fork, monkeypatch):
+@pytest.mark.parametrize("streams,fork,one", [("1", "0", "0"), ("2", "0", "0"), ("2", "1", "0"), ("2", "0", "1")])
+def test_uneven_counts_and_graph_sweep(streams, fork, one, monkeypatch):
     systems = system.config_c4(seed=3, n_replicas=6)
     with abi.Engine(systems, seed=5) as e:
         monkeypatch.setenv("SPIMC_STREAMS", streams)
         monkeypatch.setenv("SPIMC_FORK", fork)
+        monkeypatch.setenv("SPIMC_ONE_LAUNCH", one)
         g = abi.Engine(systems, seed=5)
         for i, s in enumerate(systems):
             pos = system.random_path(s, seed=i, spread=0.05)
