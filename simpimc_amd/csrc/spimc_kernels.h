@@ -194,74 +194,39 @@ struct BlockSync {
     __device__ void operator()() const { __syncthreads(); }
 };
 
-// Bisect::Attempt up to the level-0 decision, one wavefront (lane 0..63) on
-// replica R; items: also write the round's rho_k work items.
-template <class Sync>
-__device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync sync, bool items) {
-    __shared__ double sh_old[(1 << kMaxLevel) + 1][3];
-    __shared__ double sh_new[(1 << kMaxLevel) + 1][3];
-    // the attempt's normals, 3 per midpoint in level order, and level uniforms
-    // (device RNG), drawn before the level loop across the lanes
-    __shared__ double sh_z[3 * ((1 << kMaxLevel) - 1) + 1];
-    __shared__ double sh_u[kMaxLevel];
-    __shared__ int s_info[4];
-    __shared__ long long s_cur;
-    __shared__ unsigned long long s_seq;
-    Prop* pr = R.prop;
-    if (lane == 0) {
-        Counters* c = R.cnt;
-        unsigned int rem = c->remaining;
-        s_info[0] = rem > 0;
-        if (rem > 0) {
-            c->remaining = rem - 1;
-            s_seq = c->seq;
-            c->seq = s_seq + 1;
-        }
+// LDS of one bisection proposal (one wavefront): the window's beads, OLD and
+// NEW; the attempt's normals, 3 per midpoint in level order, and level
+// uniforms (device RNG), drawn before the level loop across the lanes; the
+// attempt's counter step and first draws.
+struct BisectLds {
+    double x_old[(1 << kMaxLevel) + 1][3];
+    double x_new[(1 << kMaxLevel) + 1][3];
+    double z[3 * ((1 << kMaxLevel) - 1) + 1];
+    double u[kMaxLevel];
+    int info[4];  // an attempt is left, p, b0, bad draw
+    long long cur;
+    unsigned long long seq;
+};
+
+// The counter step of an attempt (one lane): whether one is left, and its
+// sequence number (the device RNG's counter).
+__device__ __forceinline__ void bisect_take(const Rep& R, BisectLds& S) {
+    Counters* c = R.cnt;
+    const unsigned int rem = c->remaining;
+    S.info[0] = rem > 0;
+    if (rem > 0) {
+        c->remaining = rem - 1;
+        S.seq = c->seq;
+        c->seq = S.seq + 1;
     }
-    sync();
-    if (!s_info[0]) {
-        if (lane == 0) pr->status = 0;
-        if (items) rho_items_write(R, 0, 0, 0, 0, lane);
-        return;
-    }
-    const int n = R.n_level, nb = 1 << n, N = R.N, M = R.M;
-    Rng rng{make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32)), 1u, s_seq};
-    const bool tape = R.tape != nullptr;
-    int bad = 0;
-    if (lane == 0) {
-        int p, b0;
-        long long cur = 0;
-        if (tape) {
-            cur = *R.tape_cursor;
-            p = (int)tape_get(R, cur, &bad);
-            b0 = (int)tape_get(R, cur + 1, &bad);
-            cur += 2;
-            if (p < 0 || p >= N || b0 < 0 || b0 >= M) {
-                bad = 1;
-                p = 0;
-                b0 = 0;
-            }
-        } else {
-            // rng.index(0, 0, N) and rng.index(0, 1, M) from one call
-            const uint4 w = rng.call(0);
-            p = (int)(((uint64_t)w.x * (uint32_t)N) >> 32);
-            b0 = (int)(((uint64_t)w.y * (uint32_t)M) >> 32);
-        }
-        s_info[1] = p;
-        s_info[2] = b0;
-        s_info[3] = bad;
-        s_cur = cur;
-    }
-    sync();
-    const int p = s_info[1], b0 = s_info[2];
-    // the chain walked from (p, b0) carries label p2 after the wrap
-    const int p2 = label_after_wrap(R, p);
-    // 2^n_level == M and the chain closes on itself: the window's last bead is
-    // its first one, so the reference's bead walks visit no midpoint and no
-    // kinetic link (bisect_class.h:51-58,78)
-    const bool wraps = (nb % M) == 0 && p2 == p;
-    // the window's beads (nb + 1 <= 65: two per lane at most) are loaded, the
-    // draws made while the loads are in flight, then the beads stored
+}
+
+// The window's beads of (p, b0) into S, and on the device RNG (draws) the
+// attempt's normals and level uniforms, made while the loads are in flight
+// (the window holds nb + 1 <= 65 beads: two per lane at most).
+__device__ __forceinline__ void bisect_window(const Rep& R, int lane, BisectLds& S, int p, int p2, int b0,
+                                              bool draws) {
+    const int n = R.n_level, nb = 1 << n, M = R.M;
     double xw[2][3];
     for (int k = 0; k < 2; ++k) {
         const int j = lane + k * kWave;
@@ -270,27 +235,109 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
         const int lbl = (b0 + j >= M) ? p2 : p;
         for (int d = 0; d < 3; ++d) xw[k][d] = pos_at(R, d, s, lbl);
     }
-    if (!tape) {
+    if (draws) {
+        Rng rng{make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32)), 1u, S.seq};
         // normal g is normal(1 + n + g / 2, g & 1): one call per pair of normals
         const int n_norm = 3 * (nb - 1);
         for (int q = lane; 2 * q < n_norm; q += kWave) {
             double z0, z1;
             rng.normal2(1 + n + q, &z0, &z1);
-            sh_z[2 * q] = z0;
-            sh_z[2 * q + 1] = z1;
+            S.z[2 * q] = z0;
+            S.z[2 * q + 1] = z1;
         }
-        for (int l = lane; l < n; l += kWave) sh_u[l] = rng.unif(1 + l, 0);
+        for (int l = lane; l < n; l += kWave) S.u[l] = rng.unif(1 + l, 0);
     }
     for (int k = 0; k < 2; ++k) {
         const int j = lane + k * kWave;
         if (j > nb) break;
         for (int d = 0; d < 3; ++d) {
-            sh_old[j][d] = xw[k][d];
-            sh_new[j][d] = xw[k][d];
+            S.x_old[j][d] = xw[k][d];
+            S.x_new[j][d] = xw[k][d];
         }
     }
+}
+
+// The start of an attempt on the device RNG, ahead of its proposal and by
+// another wavefront than the one that finishes it (k_round): the counter
+// step, particle and first bead, window, normals and level uniforms into S.
+// The finishing wavefront reloads the window if the decision before it
+// committed moves of the same particle.
+template <class Sync>
+__device__ __forceinline__ void bisect_start_ahead(const Rep& R, int lane, Sync sync, BisectLds& S) {
+    if (lane == 0) bisect_take(R, S);
     sync();
-    long long cur = s_cur;
+    if (!S.info[0]) return;
+    if (lane == 0) {
+        // rng.index(0, 0, N) and rng.index(0, 1, M) from one call
+        Rng rng{make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32)), 1u, S.seq};
+        const uint4 w = rng.call(0);
+        S.info[1] = (int)(((uint64_t)w.x * (uint32_t)R.N) >> 32);
+        S.info[2] = (int)(((uint64_t)w.y * (uint32_t)R.M) >> 32);
+        S.info[3] = 0;
+        S.cur = 0;
+    }
+    sync();
+    const int p = S.info[1];
+    bisect_window(R, lane, S, p, label_after_wrap(R, p), S.info[2], true);
+}
+
+// Bisect::Attempt up to the level-0 decision, one wavefront (lane 0..63) on
+// replica R; items: also write the round's rho_k work items.  ahead: the
+// attempt was started by bisect_start_ahead (device RNG only).
+template <class Sync>
+__device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync sync, bool items, BisectLds& S,
+                                                    bool ahead) {
+    Prop* pr = R.prop;
+    if (!ahead) {
+        if (lane == 0) bisect_take(R, S);
+        sync();
+    }
+    if (!S.info[0]) {
+        if (lane == 0) pr->status = 0;
+        if (items) rho_items_write(R, 0, 0, 0, 0, lane);
+        return;
+    }
+    const int n = R.n_level, nb = 1 << n, N = R.N, M = R.M;
+    const bool tape = R.tape != nullptr;
+    int bad = 0;
+    if (!ahead) {
+        if (lane == 0) {
+            int p, b0;
+            long long cur = 0;
+            if (tape) {
+                cur = *R.tape_cursor;
+                p = (int)tape_get(R, cur, &bad);
+                b0 = (int)tape_get(R, cur + 1, &bad);
+                cur += 2;
+                if (p < 0 || p >= N || b0 < 0 || b0 >= M) {
+                    bad = 1;
+                    p = 0;
+                    b0 = 0;
+                }
+            } else {
+                // rng.index(0, 0, N) and rng.index(0, 1, M) from one call
+                Rng rng{make_uint2((uint32_t)R.key, (uint32_t)(R.key >> 32)), 1u, S.seq};
+                const uint4 w = rng.call(0);
+                p = (int)(((uint64_t)w.x * (uint32_t)N) >> 32);
+                b0 = (int)(((uint64_t)w.y * (uint32_t)M) >> 32);
+            }
+            S.info[1] = p;
+            S.info[2] = b0;
+            S.info[3] = bad;
+            S.cur = cur;
+        }
+        sync();
+    }
+    const int p = S.info[1], b0 = S.info[2];
+    // the chain walked from (p, b0) carries label p2 after the wrap
+    const int p2 = label_after_wrap(R, p);
+    // 2^n_level == M and the chain closes on itself: the window's last bead is
+    // its first one, so the reference's bead walks visit no midpoint and no
+    // kinetic link (bisect_class.h:51-58,78)
+    const bool wraps = (nb % M) == 0 && p2 == p;
+    if (!ahead) bisect_window(R, lane, S, p, p2, b0, !tape);
+    sync();
+    long long cur = S.cur;
     double prev = 0.;
     int g_base = 0;  // normals drawn so far (device RNG indexing)
     for (int level = n - 1; level >= 0; --level) {
@@ -305,21 +352,21 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
             const int a = 2 * lane * skip, b = a + skip, c = a + 2 * skip;
             double t[3], rbar[3], dold[3], dnew[3];
             for (int d = 0; d < 3; ++d) {
-                t[d] = minimg(sh_old[c][d] - sh_old[a][d], R.L, R.iL);
-                rbar[d] = sh_old[a][d] + 0.5 * t[d];
-                dold[d] = minimg(sh_old[b][d] - rbar[d], R.L, R.iL);
+                t[d] = minimg(S.x_old[c][d] - S.x_old[a][d], R.L, R.iL);
+                rbar[d] = S.x_old[a][d] + 0.5 * t[d];
+                dold[d] = minimg(S.x_old[b][d] - rbar[d], R.L, R.iL);
             }
             for (int d = 0; d < 3; ++d) {
                 double z;
                 if (tape)
                     z = tape_get(R, cur + 3 * lane + d, &bad);
                 else
-                    z = sh_z[g_base + 3 * lane + d];
+                    z = S.z[g_base + 3 * lane + d];
                 dnew[d] = sigma * z;
                 dnew[d] = minimg(dnew[d], R.L, R.iL);
-                t[d] = minimg(sh_new[c][d] - sh_new[a][d], R.L, R.iL);
-                rbar[d] = sh_new[a][d] + 0.5 * t[d];
-                sh_new[b][d] = rbar[d] + dnew[d];
+                t[d] = minimg(S.x_new[c][d] - S.x_new[a][d], R.L, R.iL);
+                rbar[d] = S.x_new[a][d] + 0.5 * t[d];
+                S.x_new[b][d] = rbar[d] + dnew[d];
             }
             lspo = -free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dold);
             lspn = -free_neg_log(&R.bis[level], R.bis_images, i4lt_b, dnew);
@@ -331,9 +378,9 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
         for (int i = lane; i < nlinks; i += kWave) {
             const int a = i * skip, b = a + skip;
             double dd[3];
-            for (int d = 0; d < 3; ++d) dd[d] = minimg(sh_old[a][d] - sh_old[b][d], R.L, R.iL);
+            for (int d = 0; d < 3; ++d) dd[d] = minimg(S.x_old[a][d] - S.x_old[b][d], R.L, R.iL);
             ko += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
-            for (int d = 0; d < 3; ++d) dd[d] = minimg(sh_new[a][d] - sh_new[b][d], R.L, R.iL);
+            for (int d = 0; d < 3; ++d) dd[d] = minimg(S.x_new[a][d] - S.x_new[b][d], R.L, R.iL);
             kn += free_neg_log(&R.kin[level], R.kin_images, i4lt_k, dd);
         }
         lspo = wave_sum(lspo);
@@ -347,7 +394,7 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
         if (tape)
             u = tape_get(R, cur, &bad);
         else
-            u = sh_u[level];
+            u = S.u[level];
         cur += 1;
         if (level > 0) {
             const double old_action = 0. + ko, new_action = 0. + kn;
@@ -370,7 +417,7 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
             prev = change;
         } else {
             const int any_bad = __any(bad);
-            const int status = (any_bad || s_info[3]) ? 3 : 1;
+            const int status = (any_bad || S.info[3]) ? 3 : 1;
             if (lane == 0) {
                 pr->status = status;
                 pr->move = 0;
@@ -391,10 +438,10 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
                 pr->tape_next = cur;
             }
             for (int j = lane; j < (wraps ? 0 : nb - 1); j += kWave)
-                for (int d = 0; d < 3; ++d) R.prop_new[3 * j + d] = sh_new[j + 1][d];
+                for (int d = 0; d < 3; ++d) R.prop_new[3 * j + d] = S.x_new[j + 1][d];
             if (R.use_long_range) {
                 for (int t = lane; t < (wraps ? 0 : 6 * (nb - 1)); t += kWave)
-                    moved_phase_chain(R, t / 6, t % 6, sh_old[t / 6 + 1], sh_new[t / 6 + 1]);
+                    moved_phase_chain(R, t / 6, t % 6, S.x_old[t / 6 + 1], S.x_new[t / 6 + 1]);
                 if (items) rho_items_write(R, 0, status, wraps ? 0 : nb - 1, (b0 + 1) % M, lane);
             }
         }
@@ -404,7 +451,8 @@ __device__ __forceinline__ void propose_bisect_impl(const Rep& R, int lane, Sync
 __device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps, int n_reps) {
     const int r = blockIdx.x;
     if (r >= n_reps) return;
-    propose_bisect_impl(reps[r], (int)threadIdx.x, BlockSync{}, true);
+    __shared__ BisectLds S;
+    propose_bisect_impl(reps[r], (int)threadIdx.x, BlockSync{}, true, S, false);
 }
 
 // DisplaceParticle::Attempt (displace_particle_class.h:28-72): NEW beads only.
@@ -882,12 +930,13 @@ template <class Sync>
 __device__ __forceinline__ void commit_impl(const Rep& R, const Prop& pr, int lane, int acc, const double* pre,
                                             unsigned char pre_sel, Sync sync);
 // COH: the pair and rho_k sums are read with agent-scope atomic loads (they were
-// written in the same launch, by workgroups on any XCD).
+// written in the same launch, by workgroups on any XCD).  Returns the particle
+// whose beads an accepted one-particle move committed, else -1.
 template <bool COH = false, class Sync>
-__device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
+__device__ __forceinline__ int decide_impl(const Rep& R, int lane, Sync sync) {
     Prop* prp = R.prop;
     const Prop pr = *prp;
-    if (pr.status == 0) return;
+    if (pr.status == 0) return -1;
     double po = 0., pn = 0., lo = 0., ln = 0.;
     // the commit's loads of a one-particle move, issued before the sums: the
     // lane's first new bead and the selector of its slice
@@ -990,6 +1039,7 @@ __device__ __forceinline__ void decide_impl(const Rep& R, int lane, Sync sync) {
         }
         prp->status = 0;
     }
+    return (pr.status == 1 && s_acc && pr.move != 3) ? pr.p : -1;
 }
 
 // The commit of a decided proposal (part of decide_impl): Bisect::Accept /
@@ -1172,11 +1222,30 @@ __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ r
         s_last = atomicAdd(arrive + r, 1u) + 1u == expect;
     }
     __syncthreads();
-    if (!s_last || threadIdx.x >= kWave) return;
-    if (lane == 0) __hip_atomic_store(arrive + r, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    decide_impl<true>(reps[r], lane, WaveSync{});
-    WaveSync{}();
-    propose_bisect_impl(reps[r], lane, WaveSync{}, true);
+    if (!s_last) return;
+    // the replica's last workgroup: its first wavefront decides while, on the
+    // device RNG, its second starts the next attempt (the draws depend on the
+    // attempt's sequence number only, not on the decision)
+    __shared__ BisectLds S;
+    __shared__ int s_moved;
+    const Rep& R = reps[r];
+    const bool ahead = R.tape == nullptr;
+    if (w == 0) {
+        if (lane == 0) __hip_atomic_store(arrive + r, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int moved = decide_impl<true>(R, lane, WaveSync{});
+        if (lane == 0) s_moved = moved;
+    } else if (w == 1 && ahead) {
+        bisect_start_ahead(R, lane, WaveSync{}, S);
+    }
+    __syncthreads();
+    if (w != 0) return;
+    if (ahead && S.info[0] && S.info[1] == s_moved) {
+        // the decision just committed beads of the attempt's particle: its window again
+        const int p = S.info[1];
+        bisect_window(R, lane, S, p, label_after_wrap(R, p), S.info[2], false);
+        WaveSync{}();
+    }
+    propose_bisect_impl(R, lane, WaveSync{}, true, S, ahead);
 }
 
 // Sums the per-replica work counters into work[WORK_PAIR_LINKS], work[WORK_RHO_SLICEK].
