@@ -929,15 +929,91 @@ __global__ __launch_bounds__(kRhoBlock) void k_rho_loop(const RhoItem* __restric
 template <class Sync>
 __device__ __forceinline__ void commit_impl(const Rep& R, const Prop& pr, int lane, int acc, const double* pre,
                                             unsigned char pre_sel, Sync sync);
+// The counters, accept flag and action trace of one decided attempt (one lane):
+// status as decided (3: replay error), po/pn/lo/ln the summed level-0 terms.
+__device__ __forceinline__ void decide_book(const Rep& R, const Prop& pr, int acc, int status, double po, double pn,
+                                            double lo, double ln) {
+    Counters* c = R.cnt;
+    if (pr.status == 1 && pr.move != 3) {  // the round's pair and rho_k work, for the roofline figures
+        c->rho_slicek += (unsigned long long)pr.count * (unsigned long long)R.nk;
+        if (R.pair_type) c->pair_links += 2ull * (unsigned long long)pr.n_links * (unsigned long long)(R.N - 1);
+    }
+    if (pr.move == 0) {
+        c->bis_att += 1;
+        c->bis_acc += acc;
+        c->bead_updates += (1ull << R.n_level) - 1;
+    } else if (pr.move == 3) {
+        // Move::DoEvent counts every attempt; PermBisect counts per cycle length
+        // the attempts that reached the bisection (perm_bisect_class.h:66-87)
+        unsigned long long* pc = R.perm_cnt;
+        pc[0] += 1;
+        pc[1] += acc;
+        if (pr.n_chain > 0) {
+            pc[2 + pr.n_chain - 1] += 1;
+            pc[2 + R.N + pr.n_chain - 1] += acc;
+        }
+        c->bead_updates += (unsigned long long)pr.n_chain * ((1ull << pr.n_level) - 1);
+    } else {
+        c->dis_att += 1;
+        c->dis_acc += acc;
+        c->bead_updates += R.M;
+    }
+    unsigned int si = c->stage_i;
+    if (si < R.accepted_cap) R.accepted[si] = (unsigned char)acc;
+    if (R.act_trace && si < R.accepted_cap) {
+        // the terms of the level-0 decision as summed (NaN: no level-0 decision)
+        double* t = R.act_trace + (size_t)si * kTraceWords;
+        const bool l0 = pr.status == 1;
+        const double nan = __builtin_nan("");
+        t[0] = l0 ? pr.k_old : nan;
+        t[1] = l0 ? pr.k_new : nan;
+        t[2] = l0 ? po : nan;
+        t[3] = l0 ? pn : nan;
+        t[4] = l0 ? R.long_scale * lo : nan;
+        t[5] = l0 ? R.long_scale * ln : nan;
+        t[6] = l0 ? pr.lsr : nan;
+        t[7] = l0 ? pr.prev : nan;
+    }
+    c->stage_i = si + 1;
+    if (status == 3) {  // replay tape overrun / bad draw
+        c->error = 1;
+        atomicOr(R.ctx_error, 1u);
+    }
+}
+
+// What decide_impl hands to a later decide_book when the caller defers it.
+struct DecideOut {
+    Prop pr;
+    double po, pn, lo, ln;
+    int acc, status;  // status 0: nothing was decided
+};
+
 // COH: the pair and rho_k sums are read with agent-scope atomic loads (they were
-// written in the same launch, by workgroups on any XCD).  Returns the particle
-// whose beads an accepted one-particle move committed, else -1.
+// written in the same launch, by workgroups on any XCD).  hint_links /
+// hint_rho >= 0: the pair and rho_k sums the proposal is expected to hold,
+// loaded before the proposal itself (summed again if it holds others).  out:
+// the bookkeeping is left to the caller (decide_book), which receives what it
+// needs there.  Returns the particle whose beads an accepted one-particle move
+// committed, else -1.
 template <bool COH = false, class Sync>
-__device__ __forceinline__ int decide_impl(const Rep& R, int lane, Sync sync) {
+__device__ __forceinline__ int decide_impl(const Rep& R, int lane, Sync sync, int hint_links = -1, int hint_rho = -1,
+                                           DecideOut* out = nullptr) {
     Prop* prp = R.prop;
+    double po = 0., pn = 0., lo = 0., ln = 0.;
+    if (hint_links >= 0) {
+        for (int j = lane; j < hint_links; j += kWave) {
+            const double2 v = COH ? coh_load2(R.pair_part + j) : R.pair_part[j];
+            po += v.x;
+            pn += v.y;
+        }
+        for (int j = lane; j < hint_rho; j += kWave) {
+            const double2 v = COH ? coh_load2(R.rho_part + j) : R.rho_part[j];
+            lo += v.x;
+            ln += v.y;
+        }
+    }
     const Prop pr = *prp;
     if (pr.status == 0) return -1;
-    double po = 0., pn = 0., lo = 0., ln = 0.;
     // the commit's loads of a one-particle move, issued before the sums: the
     // lane's first new bead and the selector of its slice
     double pre[3] = {0., 0., 0.};
@@ -947,13 +1023,14 @@ __device__ __forceinline__ int decide_impl(const Rep& R, int lane, Sync sync) {
         if (R.use_long_range) pre_sel = R.sel[(pr.first + lane) % R.M];
     }
     if (pr.status == 1) {
-        for (int j = lane; j < pr.n_links; j += kWave) {
-            const double2 v = COH ? coh_load2(R.pair_part + j) : R.pair_part[j];
-            po += v.x;
-            pn += v.y;
-        }
-        if (R.use_long_range) {
-            const int np = pr.count * R.n_chunks;
+        const int np = R.use_long_range ? pr.count * R.n_chunks : 0;
+        if (pr.n_links != hint_links || np != hint_rho) {
+            po = pn = lo = ln = 0.;
+            for (int j = lane; j < pr.n_links; j += kWave) {
+                const double2 v = COH ? coh_load2(R.pair_part + j) : R.pair_part[j];
+                po += v.x;
+                pn += v.y;
+            }
             for (int j = lane; j < np; j += kWave) {
                 const double2 v = COH ? coh_load2(R.rho_part + j) : R.rho_part[j];
                 lo += v.x;
@@ -990,52 +1067,17 @@ __device__ __forceinline__ int decide_impl(const Rep& R, int lane, Sync sync) {
     commit_impl(R, pr, lane, s_acc, pre, pre_sel, sync);
     // bookkeeping after the commit's stores are issued
     if (lane == 0) {
-        Counters* c = R.cnt;
-        if (pr.status == 1 && pr.move != 3) {  // the round's pair and rho_k work, for the roofline figures
-            c->rho_slicek += (unsigned long long)pr.count * (unsigned long long)R.nk;
-            if (R.pair_type) c->pair_links += 2ull * (unsigned long long)pr.n_links * (unsigned long long)(R.N - 1);
-        }
-        if (pr.move == 0) {
-            c->bis_att += 1;
-            c->bis_acc += acc;
-            c->bead_updates += (1ull << R.n_level) - 1;
-        } else if (pr.move == 3) {
-            // Move::DoEvent counts every attempt; PermBisect counts per cycle length
-            // the attempts that reached the bisection (perm_bisect_class.h:66-87)
-            unsigned long long* pc = R.perm_cnt;
-            pc[0] += 1;
-            pc[1] += acc;
-            if (pr.n_chain > 0) {
-                pc[2 + pr.n_chain - 1] += 1;
-                pc[2 + R.N + pr.n_chain - 1] += acc;
-            }
-            c->bead_updates += (unsigned long long)pr.n_chain * ((1ull << pr.n_level) - 1);
-        } else {
-            c->dis_att += 1;
-            c->dis_acc += acc;
-            c->bead_updates += R.M;
-        }
-        unsigned int si = c->stage_i;
-        if (si < R.accepted_cap) R.accepted[si] = (unsigned char)acc;
-        if (R.act_trace && si < R.accepted_cap) {
-            // the terms of the level-0 decision as summed above (NaN: no level-0 decision)
-            double* t = R.act_trace + (size_t)si * kTraceWords;
-            const bool l0 = pr.status == 1;
-            const double nan = __builtin_nan("");
-            t[0] = l0 ? pr.k_old : nan;
-            t[1] = l0 ? pr.k_new : nan;
-            t[2] = l0 ? po : nan;
-            t[3] = l0 ? pn : nan;
-            t[4] = l0 ? R.long_scale * lo : nan;
-            t[5] = l0 ? R.long_scale * ln : nan;
-            t[6] = l0 ? pr.lsr : nan;
-            t[7] = l0 ? pr.prev : nan;
-        }
-        c->stage_i = si + 1;
         if (R.tape) *R.tape_cursor = tape_next;
-        if (status == 3) {  // replay tape overrun / bad draw
-            c->error = 1;
-            atomicOr(R.ctx_error, 1u);
+        if (out) {
+            out->pr = pr;
+            out->acc = acc;
+            out->status = status;
+            out->po = po;
+            out->pn = pn;
+            out->lo = lo;
+            out->ln = ln;
+        } else {
+            decide_book(R, pr, acc, status, po, pn, lo, ln);
         }
         prp->status = 0;
     }
@@ -1227,17 +1269,31 @@ __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ r
     // device RNG, its second starts the next attempt (the draws depend on the
     // attempt's sequence number only, not on the decision)
     __shared__ BisectLds S;
+    __shared__ DecideOut s_out;
     __shared__ int s_moved;
     const Rep& R = reps[r];
     const bool ahead = R.tape == nullptr;
     if (w == 0) {
-        if (lane == 0) __hip_atomic_store(arrive + r, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int moved = decide_impl<true>(R, lane, WaveSync{});
+        if (lane == 0) {
+            __hip_atomic_store(arrive + r, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_out.status = 0;
+        }
+        // a bisection proposal that reached level 0 holds 2^l pair sums and
+        // 2^l - 1 moved slices of rho_k sums: loaded before the proposal itself
+        const int nb = 1 << R.n_level;
+        const int moved = decide_impl<true>(R, lane, WaveSync{}, R.pair_type ? nb : 0,
+                                            R.use_long_range ? (nb - 1) * R.n_chunks : 0, &s_out);
         if (lane == 0) s_moved = moved;
     } else if (w == 1 && ahead) {
         bisect_start_ahead(R, lane, WaveSync{}, S);
     }
     __syncthreads();
+    if (w == 2) {
+        // the decision's counters and flags, off the proposal's path
+        if (lane == 0 && s_out.status != 0)
+            decide_book(R, s_out.pr, s_out.acc, s_out.status, s_out.po, s_out.pn, s_out.lo, s_out.ln);
+        return;
+    }
     if (w != 0) return;
     if (ahead && S.info[0] && S.info[1] == s_moved) {
         // the decision just committed beads of the attempt's particle: its window again

@@ -84,13 +84,16 @@ def compare(got, ref_full, label, absolute=True):
     return int(ok.sum())
 
 
-RUNS = [(name, 0) for name in sorted(CASES)] + [("bare_c1", 256), ("david_small", 256), ("ilkka_c1", 512)]
+RUNS = ([(name, 0, 0) for name in sorted(CASES)] + [("bare_c1", 256, 0), ("david_small", 256, 0), ("ilkka_c1", 512, 0)]
+        + [("bare_c1", 0, 1), ("david_small", 0, 1)])
 
 
-# persistent: 0 stages as rounds of launches, else the persistent stage kernel's workgroup size
-@pytest.mark.parametrize("name,persistent", RUNS)
-def test_hot_path_action_terms_match_oracle(name, persistent, monkeypatch):
+# persistent: 0 stages as rounds of launches, else the persistent stage kernel's
+# workgroup size; one: bisection rounds of one launch each
+@pytest.mark.parametrize("name,persistent,one", RUNS)
+def test_hot_path_action_terms_match_oracle(name, persistent, one, monkeypatch):
     monkeypatch.setenv("SPIMC_PERSISTENT", str(persistent))
+    monkeypatch.setenv("SPIMC_ONE_LAUNCH", str(one))
     make, n_b, n_d, stages = CASES[name]
     s = make()
     pos = system.random_path(s, seed=3, spread=0.05)
