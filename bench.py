@@ -436,14 +436,16 @@ def uneven_bench(a, abi, system, steps=None):
 
 
 C5_TRAFFIC_RATIO = 1.048  # 16.14 MB of HBM traffic per 15.41 MB (algorithmic) rho_k launch
+C5_ROOFLINE_ROUNDS = 1024  # bisect rounds that time the rho_k kernel of the combined case
 
 
 def c5_bench(a, abi, system, copy_gbps, steps=None, rounds=0):
     """The combined large case: 8 replicas of HEG N=1024, M=256, r_s=1, 4012
     stored k-vectors, David n_order=2, batched with the adjoint, the virial
     energy from it, S(k) on all k-vectors and g(r) on 200 bins every sweep.
-    Reported: bead updates/s and the rho_k kernel's algorithmic GB/s from the
-    device clocks (32 B per (slice, k) update)."""
+    Reported: bead updates/s of the timed sweeps, and the rho_k kernel's
+    algorithmic GB/s from the device clocks (32 B per (slice, k) update) over
+    bisect rounds run after them."""
     steps = steps or a.steps
     base = system.config_c5()
     R = 8
@@ -453,9 +455,9 @@ def c5_bench(a, abi, system, copy_gbps, steps=None, rounds=0):
     shape = eng.autotune(16) if a.autotune else None
     eng.obs_configure(energy=abi.ENERGY_VIRIAL_ADJOINT, sk_k_cut=1e300, gr_n_r=200)
     flags = abi.SWEEP_ADJOINT | abi.SWEEP_ENERGY | abi.SWEEP_SK | abi.SWEEP_GR
-    eng.timing(abi.TIMING_CLOCK)
     if rounds:
         # bisect rounds only: every rho_k launch updates the same number of slices
+        eng.timing(abi.TIMING_CLOCK)
         eng.bisect(rounds)
         print(f"c5 {rounds} untimed bisect rounds done", file=sys.stderr, flush=True)
         eng.timing_reset()
@@ -468,7 +470,6 @@ def c5_bench(a, abi, system, copy_gbps, steps=None, rounds=0):
         for i in range(max(1, a.warmup)):
             eng.sweep(1, flags)
             print(f"c5 warmup sweep {i + 1} done", file=sys.stderr, flush=True)
-        eng.timing_reset()
         u0 = eng.bead_updates()
         t0 = time.perf_counter()
         for i in range(steps):
@@ -477,6 +478,15 @@ def c5_bench(a, abi, system, copy_gbps, steps=None, rounds=0):
                   flush=True)
     dt = time.perf_counter() - t0
     value = (eng.bead_updates() - u0) / dt
+    if not rounds:
+        # the rho_k kernel's own launches, timed on the device clocks over bisect
+        # rounds after the sweeps (with the clocks on, bisection rounds run as
+        # three launches, so every rho_k update is one timed k_rho_loop item)
+        eng.timing(abi.TIMING_CLOCK)
+        eng.bisect(C5_ROOFLINE_ROUNDS)
+        eng.timing_reset()
+        eng.bisect(C5_ROOFLINE_ROUNDS)
+        print(f"c5 {C5_ROOFLINE_ROUNDS} bisect rounds on the device clocks done", file=sys.stderr, flush=True)
     rho_ms, rho_n, rho_bytes = eng.kernel_timing(abi.KERNEL_RHO)
     pair_ms, pair_n, links = eng.kernel_timing(abi.KERNEL_PAIR)
     eng.close()
@@ -494,6 +504,9 @@ def c5_bench(a, abi, system, copy_gbps, steps=None, rounds=0):
         "roofline": ({"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                       "frac": ach / HBM_PEAK_GBPS, "kernel": "k_rho (incremental rho_k update + long-range action)",
                       "bytes_per_launch": rho_bytes / rho_n, "avg_launch_us": 1e3 * rho_ms / rho_n,
+                      "duration_from": "device wall clock, first block start to last block end, over "
+                                       + (f"the {rounds} timed bisect rounds" if rounds else
+                                          f"{C5_ROOFLINE_ROUNDS} bisect rounds after the timed sweeps"),
                       # FETCH_SIZE / WRITE_SIZE passes over bisect rounds of this case, scaled by a copy kernel
                       "traffic_over_algorithmic": C5_TRAFFIC_RATIO,
                       "traffic_from": "profiles/k_rho_loop_heg1024_m256_8rep_counters.md",

@@ -1147,6 +1147,12 @@ bool pers_eligible(const spimc_ctx* c) {
 bool one_launch_eligible(const spimc_ctx* c) {
     return c->round_fn && c->lds_doubles_max > 0 && c->fuse_rounds && !c->perm_on && c->rho_kpt == kRhoKpt;
 }
+// Whether a stage of this move runs one-launch rounds: bisection only, and not
+// while the device clocks time the pair and rho_k kernels (k_round has no
+// separate rho_k kernel to time, so those rounds run as three launches).
+bool one_launch_now(const spimc_ctx* c, int move) {
+    return c->one_launch && move == 0 && !c->clock && one_launch_eligible(c);
+}
 
 
 
@@ -1312,7 +1318,7 @@ int launch_one_round(spimc_ctx* c, const Group& G) {
 
 int launch_round(spimc_ctx* c, const Group& G, int move) {
     if (move == 3) return launch_perm_round(c, G);
-    if (c->one_launch && move == 0 && one_launch_eligible(c)) return launch_one_round(c, G);
+    if (one_launch_now(c, move)) return launch_one_round(c, G);
     const int R = G.n;
     unsigned long long* kt = c->clock ? G.d_kt : nullptr;
     Rep* reps = c->d_reps + G.r0;  // kernels indexed by block: replicas of this group
@@ -1542,7 +1548,7 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
     }
     // one-launch rounds: each group's first proposals, then rounds that decide
     // and propose on their own (the last round's proposals find no attempt left)
-    const bool one = c->one_launch && move == 0 && one_launch_eligible(c);
+    const bool one = one_launch_now(c, move);
     if (one)
         for (int g = 0; g < n_groups; ++g) {
             const Group& G = c->groups[g];
