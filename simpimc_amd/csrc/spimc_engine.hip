@@ -1628,7 +1628,11 @@ int est_pair_whole(spimc_ctx* c, Rep* base, int nrep, int which) {
     int rc;
     if (c->lds_whole_max > 0) {
         const size_t shmem = (size_t)c->lds_whole_max * sizeof(double);
-        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_pair_whole_lds, nrep, kAdjLdsBlock, shmem, c->stream, base, which, kSlotPair);
+        // workgroups per replica, each over every spb-th slice: enough to fill the
+        // GPU when the replicas are few
+        const int spb = std::max(1, std::min(c->max_M, (2 * kNumCUs + nrep - 1) / nrep));
+        rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_pair_whole_lds, dim3(nrep, spb), kAdjLdsBlock, shmem, c->stream, base,
+                    which, kSlotPair);
     } else {
         dim3 g(c->max_M, nrep);
         rc = launch(c, SPIMC_KERNEL_ESTIMATORS, k_pair_whole, g, 256, 0, c->stream, base, which, kSlotPair);

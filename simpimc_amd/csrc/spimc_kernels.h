@@ -2017,7 +2017,8 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_adjoint_pairs_lds(const Rep* _
 }
 
 // Whole-path pair sums with LDS tables (which 0: CalcdUdBeta, 1: CalcV on plain
-// minimum images): one 1024-thread workgroup per replica, per-slice partials.
+// minimum images): gridDim.y 1024-thread workgroups per replica (blockIdx.x),
+// each over every gridDim.y-th slice, per-slice partials.
 __global__ __launch_bounds__(kAdjLdsBlock) void k_pair_whole_lds(const Rep* __restrict__ reps, int which, int slot) {
     extern __shared__ double lds[];
     const Rep& R = reps[blockIdx.x];
@@ -2026,7 +2027,9 @@ __global__ __launch_bounds__(kAdjLdsBlock) void k_pair_whole_lds(const Rep* __re
     const double* ldu = lds;
     __shared__ double red[kAdjLdsBlock / kWave];
     const int N = R.N, M = R.M, half = N / 2;
-    for (int b = 0; b < M; ++b) {
+    // slices b = blockIdx.y, + gridDim.y, ...: each slice's sum is made by one
+    // workgroup in the same order whatever the number of workgroups
+    for (int b = blockIdx.y; b < M; b += gridDim.y) {
         const int b1 = (b + 1) % M;
         double t = 0.;
         // pairs (i, i + d mod N), d = 1..N/2: thread handles i = tid % N and a slice of d
