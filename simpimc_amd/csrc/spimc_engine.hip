@@ -1310,7 +1310,12 @@ int launch_perm_round(spimc_ctx* c, const Group& G) {
 // rest; measured slower.)
 int launch_one_round(spimc_ctx* c, const Group& G) {
     const int R = G.n, waves = kRoundBlock / 64;
-    const int bpr = std::max(1, std::min((G.max_bis_links + waves - 1) / waves, (2 * kNumCUs + R - 1) / R));
+    // links per workgroup: fewer when a replica of more than 256 particles splits
+    // each link's partners over wavefronts (pair_split)
+    int max_N = 1;
+    for (int r = G.r0; r < G.r0 + G.n; ++r) max_N = std::max(max_N, c->reps[r].dev.N);
+    const int lpb = waves / std::min(waves, (max_N + 255) / 256);
+    const int bpr = std::max(1, std::min((G.max_bis_links + lpb - 1) / lpb, (2 * kNumCUs + R - 1) / R));
     const size_t shmem = (size_t)c->lds_doubles_max * sizeof(double);
     return launch(c, SPIMC_KERNEL_PAIR, c->round_fn, R * bpr + G.n_ritem_bis, kRoundBlock, shmem, G.stream,
                   c->d_reps + G.r0, R, bpr, (const RhoItem*)G.d_ritem_bis, (const int*)G.d_rrep_bis, G.d_arrive);

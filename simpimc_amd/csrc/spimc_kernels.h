@@ -669,22 +669,26 @@ constexpr int kPairLdsBlock = 1024;
 // The window links j = j0, j0 + step, ... of a proposal, one wavefront: lanes
 // run over the partners q; the OLD and NEW sums of link j go to pair_part[j]
 // (COH: as agent-scope atomic stores).
-template <int PT, int NORD, bool COH = false>
-__device__ __forceinline__ void pair_links_lds(const Rep& R, const double* lds, const Prop& pr, int j0, int step,
-                                               int lane) {
-    const int M = R.M, N = R.N, p = pr.p;
-    for (int j = j0; j < pr.n_links; j += step) {
-        const int s0 = (pr.b0 + j) % M, s1 = (s0 + 1) % M;
-        double po0[3], po1[3], pn0[3], pn1[3];
-        for (int d = 0; d < 3; ++d) {
-            po0[d] = pos_at(R, d, s0, p);
-            po1[d] = pos_at(R, d, s1, p);
-            pn0[d] = pos_new(R, pr, d, s0);
-            pn1[d] = pos_new(R, pr, d, s1);
-        }
-        double uo = 0., un = 0.;
+// One lane's OLD and NEW sums of window link j over the partners q_lo + lane,
+// q_lo + lane + 64, ... below q_hi (q_lo a multiple of 128).
+template <int PT, int NORD>
+__device__ __forceinline__ void pair_link_range(const Rep& R, const double* lds, const Prop& pr, int j, int q_lo,
+                                                int q_hi, int lane, double& uo, double& un) {
+    const int M = R.M, p = pr.p;
+    const int s0 = (pr.b0 + j) % M, s1 = (s0 + 1) % M;
+    double po0[3], po1[3], pn0[3], pn1[3];
+    for (int d = 0; d < 3; ++d) {
+        po0[d] = pos_at(R, d, s0, p);
+        po1[d] = pos_at(R, d, s1, p);
+        pn0[d] = pos_new(R, pr, d, s0);
+        pn1[d] = pos_new(R, pr, d, s1);
+    }
+    uo = 0.;
+    un = 0.;
+    {
+        const int N = q_hi;
         // two partners per lane per step: independent evaluation chains for the scheduler
-        for (int q = lane; q < N; q += 2 * kWave) {
+        for (int q = q_lo + lane; q < N; q += 2 * kWave) {
             const int q2 = q + kWave;
             const bool ok1 = q != p, ok2 = (q2 < N) && (q2 != p);
             const int qq = (q2 < N) ? q2 : q;
@@ -707,6 +711,15 @@ __device__ __forceinline__ void pair_links_lds(const Rep& R, const double* lds, 
             uo += (ok1 ? uo1 : 0.) + (ok2 ? uo2 : 0.);
             un += (ok1 ? un1 : 0.) + (ok2 ? un2 : 0.);
         }
+    }
+}
+
+template <int PT, int NORD, bool COH = false>
+__device__ __forceinline__ void pair_links_lds(const Rep& R, const double* lds, const Prop& pr, int j0, int step,
+                                               int lane) {
+    for (int j = j0; j < pr.n_links; j += step) {
+        double uo, un;
+        pair_link_range<PT, NORD>(R, lds, pr, j, 0, R.N, lane, uo, un);
         uo = wave_sum(uo);
         un = wave_sum(un);
         if (lane == 0) {
@@ -715,6 +728,49 @@ __device__ __forceinline__ void pair_links_lds(const Rep& R, const double* lds, 
             else
                 R.pair_part[j] = make_double2(uo, un);
         }
+    }
+}
+
+// Wavefronts per window link of a W-wavefront workgroup: a replica of more
+// than 256 particles splits each link's partners over up to W wavefronts.
+template <int W>
+__device__ __forceinline__ int pair_split(int N) {
+    return min(W, (N + 4 * kWave - 1) / (4 * kWave));
+}
+
+// The window links of a proposal on a W-wavefront workgroup (part of bpr for
+// the replica), ns = pair_split<W>(N) wavefronts per link: wavefront w takes
+// link jb + w / ns and partner range w % ns, ranges of q_len partners (a
+// multiple of 128).  A link's sum adds its ranges' wavefront sums in range
+// order; with ns = 1 it is pair_links_lds's.  Every wavefront of the
+// workgroup calls it (it holds workgroup barriers); wpart: W entries of LDS.
+template <int PT, int NORD, bool COH, int W>
+__device__ __forceinline__ void pair_links_split(const Rep& R, const double* lds, const Prop& pr, int part, int bpr,
+                                                 int w, int lane, double2* wpart) {
+    const int ns = pair_split<W>(R.N), lpb = W / ns;
+    const int q_len = ((R.N + ns - 1) / ns + 2 * kWave - 1) / (2 * kWave) * (2 * kWave);
+    for (int jb = part * lpb; jb < pr.n_links; jb += bpr * lpb) {
+        const int j = jb + w / ns, k = w % ns;
+        double uo = 0., un = 0.;
+        if (w < lpb * ns && j < pr.n_links) {
+            pair_link_range<PT, NORD>(R, lds, pr, j, k * q_len, min(R.N, (k + 1) * q_len), lane, uo, un);
+            uo = wave_sum(uo);
+            un = wave_sum(un);
+        }
+        if (lane == 0) wpart[w] = make_double2(uo, un);
+        __syncthreads();
+        if ((int)threadIdx.x < lpb && jb + (int)threadIdx.x < pr.n_links) {
+            double so = wpart[threadIdx.x * ns].x, sn = wpart[threadIdx.x * ns].y;
+            for (int kk = 1; kk < ns; ++kk) {
+                so += wpart[threadIdx.x * ns + kk].x;
+                sn += wpart[threadIdx.x * ns + kk].y;
+            }
+            if (COH)
+                coh_store2(R.pair_part + jb + threadIdx.x, make_double2(so, sn));
+            else
+                R.pair_part[jb + threadIdx.x] = make_double2(so, sn);
+        }
+        __syncthreads();
     }
 }
 
@@ -1232,6 +1288,7 @@ __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ r
     extern __shared__ double lds[];
     __shared__ double2 tab[6 * (kMaxIdx + 1)];
     __shared__ double red[2][kRhoBlock / kWave];
+    __shared__ double2 wpart[kRoundBlock / kWave];
     __shared__ int s_last;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int n_pair = n_reps * bpr, idx = xcd_remap(blockIdx.x, gridDim.x);
@@ -1241,13 +1298,13 @@ __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ r
         const int part = idx % bpr;
         const Rep R = reps[r];
         const Prop pr = *R.prop;
-        if (pr.status == 1 && !pr.generic && part * (kRoundBlock / kWave) < pr.n_links) {
+        constexpr int W = kRoundBlock / kWave;
+        if (pr.status == 1 && !pr.generic && part * (W / pair_split<W>(R.N)) < pr.n_links) {
             const int na = R.bt_a.ncoef * R.bt_a.ncomp;
             for (int t = threadIdx.x; t < R.lds_doubles; t += kRoundBlock)
                 lds[t] = t < na ? R.bt_a.c[t] : R.bt_b.c[t - na];
             __syncthreads();
-            pair_links_lds<PT, NORD, true>(R, lds, pr, part * (kRoundBlock / kWave) + w, bpr * (kRoundBlock / kWave),
-                                           lane);
+            pair_links_split<PT, NORD, true, W>(R, lds, pr, part, bpr, w, lane, wpart);
         }
     } else {
         const int it = idx - n_pair;

@@ -85,7 +85,7 @@ def compare(got, ref_full, label, absolute=True):
 
 
 RUNS = ([(name, 0, 0) for name in sorted(CASES)] + [("bare_c1", 256, 0), ("david_small", 256, 0), ("ilkka_c1", 512, 0)]
-        + [("bare_c1", 0, 1), ("david_small", 0, 1)])
+        + [("bare_c1", 0, 1), ("david_small", 0, 1), ("c5_replica", 0, 1)])
 
 
 # persistent: 0 stages as rounds of launches, else the persistent stage kernel's
