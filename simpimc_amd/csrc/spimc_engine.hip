@@ -371,9 +371,11 @@ struct Group {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // permuting rounds: the proposals for the label-general kernels (2 counters, 2 lists of n)
     unsigned int* d_multi = nullptr;
-    // one-launch bisection rounds: each rho_k item's replica (group-local index)
+    // one-launch bisection rounds: pair workgroups per replica, rho_k items per
+    // workgroup, the rho_k workgroups' items (first, count, group-local replica),
     // and the replicas' arrival counters (zero between rounds)
-    int* d_rrep_bis = nullptr;
+    int round_bpr = 1, round_ipb = 1, n_rgrp = 0;
+    int4* d_rgrp = nullptr;
     unsigned int* d_arrive = nullptr;
 };
 
@@ -456,7 +458,7 @@ struct spimc_ctx {
     // a round in one launch (k_round: pair links, rho_k items, then the replica's
     // decision and next proposal by its last workgroup), for small batches
     bool one_launch = false;
-    void (*round_fn)(const Rep*, int, int, const RhoItem*, const int*, unsigned int*) = nullptr;
+    void (*round_fn)(const Rep*, int, int, const RhoItem*, const int4*, int, unsigned int*) = nullptr;
     std::string noalloc_error;  // why the address lookup failed, if it did
 };
 
@@ -1015,7 +1017,6 @@ int build_work_lists(spimc_ctx* c, Group& G) {
         HIPCHK(hipMemset(d, 0, n * sizeof(RhoItem)));
     }
     int ob = 0, od = 0;
-    std::vector<int> mb;
     for (int r = G.r0; r < G.r0 + G.n; ++r) {
         Rep& R = c->reps[r].dev;
         if (!R.use_long_range) continue;
@@ -1024,13 +1025,41 @@ int build_work_lists(spimc_ctx* c, Group& G) {
         R.ritems_dis = G.d_ritem_dis + od;
         R.n_ritems_bis = nb;
         R.n_ritems_dis = nd;
-        mb.insert(mb.end(), nb, r - G.r0);
         ob += nb;
         od += nd;
     }
-    if (!mb.empty()) {
-        HIPCHK(hipMalloc((void**)&G.d_rrep_bis, mb.size() * sizeof(int)));
-        HIPCHK(hipMemcpy(G.d_rrep_bis, mb.data(), mb.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (c->round_fn && c->lds_doubles_max > 0) {
+        // one-launch rounds: pair workgroups per replica (fewer links each when a
+        // replica of more than 256 particles splits its links' partners), and as
+        // many rho_k items per workgroup as keep the launch within what the GPU
+        // holds at once (a deciding workgroup would otherwise hold a slot that
+        // later workgroups wait for)
+        const int waves = kRoundBlock / 64;
+        int max_N = 1;
+        for (int r = G.r0; r < G.r0 + G.n; ++r) max_N = std::max(max_N, c->reps[r].dev.N);
+        const int lpb = waves / std::min(waves, (max_N + 255) / 256);
+        G.round_bpr = std::max(1, std::min((G.max_bis_links + lpb - 1) / lpb, (2 * kNumCUs + G.n - 1) / G.n));
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)c->round_fn, kRoundBlock,
+                                                         (size_t)c->lds_doubles_max * sizeof(double)) != hipSuccess)
+            occ = 1;
+        const int room = std::max(1, std::max(1, occ) * kNumCUs - G.n * G.round_bpr);
+        G.round_ipb = std::max(1, (ob + room - 1) / room);
+        if (const char* f = std::getenv("SPIMC_ROUND_IPB")) G.round_ipb = std::max(1, std::atoi(f));
+        std::vector<int4> grp;
+        int o = 0;
+        for (int r = G.r0; r < G.r0 + G.n; ++r) {
+            const Rep& R = c->reps[r].dev;
+            if (!R.use_long_range) continue;
+            for (int f = 0; f < R.n_ritems_bis; f += G.round_ipb)
+                grp.push_back(make_int4(o + f, std::min(G.round_ipb, R.n_ritems_bis - f), r - G.r0, 0));
+            o += R.n_ritems_bis;
+        }
+        G.n_rgrp = (int)grp.size();
+        if (!grp.empty()) {
+            HIPCHK(hipMalloc((void**)&G.d_rgrp, grp.size() * sizeof(int4)));
+            HIPCHK(hipMemcpy(G.d_rgrp, grp.data(), grp.size() * sizeof(int4), hipMemcpyHostToDevice));
+        }
     }
     if (!G.d_arrive) {
         HIPCHK(hipMalloc((void**)&G.d_arrive, G.n * sizeof(unsigned int)));
@@ -1309,16 +1338,11 @@ int launch_perm_round(spimc_ctx* c, const Group& G) {
 // what the GPU holds at once, and the workgroups that decide then delay the
 // rest; measured slower.)
 int launch_one_round(spimc_ctx* c, const Group& G) {
-    const int R = G.n, waves = kRoundBlock / 64;
-    // links per workgroup: fewer when a replica of more than 256 particles splits
-    // each link's partners over wavefronts (pair_split)
-    int max_N = 1;
-    for (int r = G.r0; r < G.r0 + G.n; ++r) max_N = std::max(max_N, c->reps[r].dev.N);
-    const int lpb = waves / std::min(waves, (max_N + 255) / 256);
-    const int bpr = std::max(1, std::min((G.max_bis_links + lpb - 1) / lpb, (2 * kNumCUs + R - 1) / R));
+    const int R = G.n;
     const size_t shmem = (size_t)c->lds_doubles_max * sizeof(double);
-    return launch(c, SPIMC_KERNEL_PAIR, c->round_fn, R * bpr + G.n_ritem_bis, kRoundBlock, shmem, G.stream,
-                  c->d_reps + G.r0, R, bpr, (const RhoItem*)G.d_ritem_bis, (const int*)G.d_rrep_bis, G.d_arrive);
+    return launch(c, SPIMC_KERNEL_PAIR, c->round_fn, R * G.round_bpr + G.n_rgrp, kRoundBlock, shmem, G.stream,
+                  c->d_reps + G.r0, R, G.round_bpr, (const RhoItem*)G.d_ritem_bis, (const int4*)G.d_rgrp,
+                  G.round_ipb, G.d_arrive);
 }
 
 int launch_round(spimc_ctx* c, const Group& G, int move) {
@@ -1895,7 +1919,7 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
         {
             // one-launch rounds (SPIMC_ONE_LAUNCH or spimc_autotune): the pair type's
             // instantiation with the default rho_k chunking
-            using RoundFn = void (*)(const Rep*, int, int, const RhoItem*, const int*, unsigned int*);
+            using RoundFn = void (*)(const Rep*, int, int, const RhoItem*, const int4*, int, unsigned int*);
             const RoundFn rf[2][4] = {{k_round<1, 0, kRhoKpt, false>, k_round<2, 2, kRhoKpt, false>,
                                        k_round<2, 0, kRhoKpt, false>, k_round<0, 0, kRhoKpt, false>},
                                       {k_round<1, 0, kRhoKpt, true>, k_round<2, 2, kRhoKpt, true>,
@@ -1982,7 +2006,7 @@ int spimc_ctx_destroy(spimc_ctx* c) {
         for (auto& g : G.graph)
             if (g) (void)hipGraphExecDestroy(g);
         for (void* p : {(void*)G.d_pair_bis, (void*)G.d_pair_dis, (void*)G.d_ritem_bis, (void*)G.d_ritem_dis, (void*)G.d_kt,
-                        (void*)G.d_multi, (void*)G.d_rrep_bis, (void*)G.d_arrive})
+                        (void*)G.d_multi, (void*)G.d_rgrp, (void*)G.d_arrive})
             if (p) (void)hipFree(p);
         if (G.done) (void)hipEventDestroy(G.done);
         if (G.side) {
@@ -2382,7 +2406,7 @@ int spimc_set_bisect_level(spimc_ctx* c, uint32_t r, uint32_t n_level) {
     for (auto& G : c->groups) {
         if ((int)r < G.r0 || (int)r >= G.r0 + G.n) continue;
         for (void** p : {(void**)&G.d_pair_bis, (void**)&G.d_pair_dis, (void**)&G.d_ritem_bis, (void**)&G.d_ritem_dis,
-                         (void**)&G.d_rrep_bis})
+                         (void**)&G.d_rgrp})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;

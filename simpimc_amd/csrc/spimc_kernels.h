@@ -1264,11 +1264,13 @@ struct WaveSync {
 };
 
 // A whole bisection round in one launch, for batches too small to fill the
-// GPU (a round is then bound by latency, not by throughput).  Workgroups [0, n_reps * bpr)
-// (after the XCD remap) run the window links of the LDS pair kernel, the
-// others one rho_k item each (item_rep: the item's replica).  Every workgroup
-// then counts itself in on its replica's arrival counter; the last of the
-// replica's bpr + n_items workgroups decides and commits the replica's
+// GPU (a round is then bound by latency, not by throughput).  Workgroups
+// [0, n_reps * bpr) (after the XCD remap) run the window links of the LDS pair
+// kernel, the others a group of up to ipb rho_k items of one replica each
+// (rgrp: first item, items, replica; ipb > 1 only where one item per
+// workgroup would exceed what the GPU holds at once).  Every workgroup then
+// counts itself in on its replica's arrival counter; the last of the
+// replica's bpr + ceil(n_items / ipb) workgroups decides and commits the replica's
 // proposal (K5) and makes its next one (K4), with its rho_k items, in its
 // first wavefront.  A replica's round thus ends with its own last workgroup:
 // no launch boundary separates the sums from the decision, and nothing waits
@@ -1283,7 +1285,7 @@ constexpr int kRoundBlock = kRhoBlock;
 template <int PT, int NORD, int KPT, bool NT>
 __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ reps, int n_reps, int bpr,
                                                        const RhoItem* __restrict__ items,
-                                                       const int* __restrict__ item_rep,
+                                                       const int4* __restrict__ rgrp, int ipb,
                                                        unsigned int* __restrict__ arrive) {
     extern __shared__ double lds[];
     __shared__ double2 tab[6 * (kMaxIdx + 1)];
@@ -1307,17 +1309,26 @@ __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ r
             pair_links_split<PT, NORD, true, W>(R, lds, pr, part, bpr, w, lane, wpart);
         }
     } else {
-        const int it = idx - n_pair;
-        r = item_rep[it];
-        const RhoItem d = items[it];
-        if (d.valid) rho_item<KPT, NT, true>(d, tab, red);
+        // a group of up to ipb rho_k items of one replica, one after the other
+        const int4 g = rgrp[idx - n_pair];  // first item, items, replica
+        r = g.z;
+        RhoItem d = items[g.x];
+        for (int i = 0; i < g.y; ++i) {
+            RhoItem nd;
+            if (i + 1 < g.y) nd = items[g.x + i + 1];
+            if (d.valid) {
+                rho_item<KPT, NT, true>(d, tab, red);
+                __syncthreads();  // tab and red are reused by the next item
+            }
+            d = nd;
+        }
     }
     // every wavefront's sums are performed before the workgroup arrives
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         const Rep& R = reps[r];
-        const unsigned int expect = (unsigned int)bpr + (unsigned int)R.n_ritems_bis;
+        const unsigned int expect = (unsigned int)bpr + (unsigned int)((R.n_ritems_bis + ipb - 1) / ipb);
         s_last = atomicAdd(arrive + r, 1u) + 1u == expect;
     }
     __syncthreads();

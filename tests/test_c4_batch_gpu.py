@@ -51,11 +51,14 @@ def rel_err(a, b):
 
 # stages as rounds of launches (one: each round in one launch), or each stage in
 # one persistent launch (workgroup size)
+# (one = 3: one-launch rounds whose rho_k workgroups take 3 items each)
 @pytest.mark.parametrize("graph,persistent,one", [(False, 0, 0), (True, 0, 0), (False, 256, 0), (True, 256, 0),
-                                                  (False, 1024, 0), (False, 0, 1), (True, 0, 1)])
+                                                  (False, 1024, 0), (False, 0, 1), (True, 0, 1), (False, 0, 3)])
 def test_c4_batch_replays_every_replica(graph, persistent, one, monkeypatch):
     monkeypatch.setenv("SPIMC_PERSISTENT", str(persistent))
-    monkeypatch.setenv("SPIMC_ONE_LAUNCH", str(one))
+    monkeypatch.setenv("SPIMC_ONE_LAUNCH", str(min(one, 1)))
+    if one > 1:
+        monkeypatch.setenv("SPIMC_ROUND_IPB", str(one))
     systems, ref = oracle_chains()
     assert len(systems) == 64
     assert {s.n_part for s in systems} == {16, 32, 64, 128} and {s.n_bead for s in systems} == {16, 32, 64}
