@@ -2746,8 +2746,12 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
         shapes = {{0, 0, 1024, 0, 0}, {0, 0, 512, 0, 0}, {0, 0, 256, 0, 0}};
     if (pers_eligible(c))
         for (int pb : {256, 512, 1024}) shapes.push_back({c->fork ? 1 : 0, c->rho_bpc, c->pair_block, pb, 0});
-    // batches of at most one replica per CU: also the one-launch rounds
-    if (one_launch_eligible(c) && (int)c->reps.size() <= kNumCUs) shapes.push_back({0, 0, c->pair_block, 0, 1});
+    // batches of at most one replica per CU: also one-launch bisection rounds,
+    // with the displace rounds' rho_k after or beside their pair kernel
+    if (one_launch_eligible(c) && (int)c->reps.size() <= kNumCUs) {
+        shapes.push_back({0, 0, c->pair_block, 0, 1});
+        if (c->any_long) shapes.push_back({1, 2, 512, 0, 1});
+    }
     const Shape saved{c->fork ? 1 : 0, c->rho_bpc, c->pair_block, c->pers_block, c->one_launch ? 1 : 0};
     // the sweep's mix: N M / 2^l bisection attempts per N displace attempts
     const Rep& R0 = c->reps[0].dev;
