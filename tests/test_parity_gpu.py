@@ -648,10 +648,13 @@ def _edge(name):
 
 @pytest.mark.parametrize("name", ["one_particle", "two_slices", "window_is_path", "no_long_range", "images",
                                   "three_k_vectors"])
-def test_edge_configurations(name):
+@pytest.mark.parametrize("one", ["0", "1"])
+def test_edge_configurations(name, one, monkeypatch):
     """Replay, estimators and gradients on edge inputs: one particle (no pairs),
     two slices, a bisection window as long as the path, no long range, image
-    sums in the kinetic action and the sampling, and a three-vector k-set."""
+    sums in the kinetic action and the sampling, and a three-vector k-set;
+    bisection rounds of three launches or of one (SPIMC_ONE_LAUNCH)."""
+    monkeypatch.setenv("SPIMC_ONE_LAUNCH", one)
     s = _edge(name)
     if name == "three_k_vectors":
         assert s.n_k == 3
