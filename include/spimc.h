@@ -419,9 +419,10 @@ int spimc_bead_updates(spimc_ctx* ctx, uint64_t* out);
  * pair kernel or after it, rho_k workgroups per CU, pair workgroup size; and,
  * when every replica has N M <= 16384 on unpermuted paths, the persistent
  * stage kernel that runs a replica's whole stage in one workgroup, with 256,
- * 512 or 1024 threads; and, for at most 256 replicas, bisection rounds of one
- * launch each, whose last workgroup per replica decides the replica's proposal
- * and makes its next one) it runs one bisection and one displace attempt per
+ * 512 or 1024 threads; and, for at most 256 replicas, rounds of one launch
+ * each, for bisection or for both moves, whose last workgroup per replica
+ * decides the replica's proposal and makes its next one) it runs one
+ * bisection and one displace attempt per
  * replica untimed, then times n_rounds bisection attempts followed by
  * n_displace displace attempts (the sweep's ratio, N M / 2^l : N, for replica
  * 0); it keeps the fastest shape for later stages and drops captured graphs.
@@ -429,7 +430,7 @@ int spimc_bead_updates(spimc_ctx* ctx, uint64_t* out);
  * the shape never changes results beyond rounding.  chosen (optional, 7 ints):
  * fork, rho_k workgroups per CU, pair workgroup size, number of shapes tried,
  * n_displace, persistent stage workgroup size (0: rounds of launches),
- * one-launch bisection rounds (1) or not (0).  No
+ * one-launch rounds for bisection (1), for both moves (2), or none (0).  No
  * counterpart in the reference. */
 int spimc_autotune(spimc_ctx* ctx, uint32_t n_rounds, int32_t* chosen);
 

@@ -393,7 +393,7 @@ class Engine:
         check(self.lib.spimc_autotune(self.h, n_rounds, out))
         return {"fork": bool(out[0]), "rho_workgroups_per_cu": int(out[1]), "pair_workgroup": int(out[2]),
                 "shapes_tried": int(out[3]), "displace_rounds": int(out[4]), "persistent_workgroup": int(out[5]),
-                "one_launch": bool(out[6])}
+                "one_launch": int(out[6])}
 
     def bead_updates(self) -> int:
         v = C.c_uint64()

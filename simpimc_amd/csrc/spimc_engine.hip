@@ -374,8 +374,8 @@ struct Group {
     // one-launch bisection rounds: pair workgroups per replica, rho_k items per
     // workgroup, the rho_k workgroups' items (first, count, group-local replica),
     // and the replicas' arrival counters (zero between rounds)
-    int round_bpr = 1, round_ipb = 1, n_rgrp = 0;
-    int4* d_rgrp = nullptr;
+    int round_bpr[2] = {1, 1}, round_ipb[2] = {1, 1}, n_rgrp[2] = {0, 0};  // per move
+    int4* d_rgrp[2] = {nullptr, nullptr};
     unsigned int* d_arrive = nullptr;
 };
 
@@ -458,7 +458,8 @@ struct spimc_ctx {
     // a round in one launch (k_round: pair links, rho_k items, then the replica's
     // decision and next proposal by its last workgroup), for small batches
     bool one_launch = false;
-    void (*round_fn)(const Rep*, int, int, const RhoItem*, const int4*, int, unsigned int*) = nullptr;
+    void (*round_fn[2])(const Rep*, int, int, const RhoItem*, const int4*, int, unsigned int*) = {nullptr, nullptr};
+    int one_moves = 1;  // one-launch rounds for bisection (1), or for both moves (2)
     std::string noalloc_error;  // why the address lookup failed, if it did
 };
 
@@ -1028,37 +1029,41 @@ int build_work_lists(spimc_ctx* c, Group& G) {
         ob += nb;
         od += nd;
     }
-    if (c->round_fn && c->lds_doubles_max > 0) {
-        // one-launch rounds: pair workgroups per replica (fewer links each when a
-        // replica of more than 256 particles splits its links' partners), and as
-        // many rho_k items per workgroup as keep the launch within what the GPU
-        // holds at once (a deciding workgroup would otherwise hold a slot that
-        // later workgroups wait for)
+    if (c->round_fn[0] && c->lds_doubles_max > 0) {
+        // one-launch rounds, per move: pair workgroups per replica (fewer links
+        // each when a replica of more than 256 particles splits its links'
+        // partners), and as many rho_k items per workgroup as keep the launch
+        // within what the GPU holds at once (a deciding workgroup would otherwise
+        // hold a slot that later workgroups wait for)
         const int waves = kRoundBlock / 64;
         int max_N = 1;
         for (int r = G.r0; r < G.r0 + G.n; ++r) max_N = std::max(max_N, c->reps[r].dev.N);
         const int lpb = waves / std::min(waves, (max_N + 255) / 256);
-        G.round_bpr = std::max(1, std::min((G.max_bis_links + lpb - 1) / lpb, (2 * kNumCUs + G.n - 1) / G.n));
-        int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)c->round_fn, kRoundBlock,
-                                                         (size_t)c->lds_doubles_max * sizeof(double)) != hipSuccess)
-            occ = 1;
-        const int room = std::max(1, std::max(1, occ) * kNumCUs - G.n * G.round_bpr);
-        G.round_ipb = std::max(1, (ob + room - 1) / room);
-        if (const char* f = std::getenv("SPIMC_ROUND_IPB")) G.round_ipb = std::max(1, std::atoi(f));
-        std::vector<int4> grp;
-        int o = 0;
-        for (int r = G.r0; r < G.r0 + G.n; ++r) {
-            const Rep& R = c->reps[r].dev;
-            if (!R.use_long_range) continue;
-            for (int f = 0; f < R.n_ritems_bis; f += G.round_ipb)
-                grp.push_back(make_int4(o + f, std::min(G.round_ipb, R.n_ritems_bis - f), r - G.r0, 0));
-            o += R.n_ritems_bis;
-        }
-        G.n_rgrp = (int)grp.size();
-        if (!grp.empty()) {
-            HIPCHK(hipMalloc((void**)&G.d_rgrp, grp.size() * sizeof(int4)));
-            HIPCHK(hipMemcpy(G.d_rgrp, grp.data(), grp.size() * sizeof(int4), hipMemcpyHostToDevice));
+        for (int mv = 0; mv < 2; ++mv) {
+            const int links = mv == 0 ? G.max_bis_links : G.max_M, n_items = mv == 0 ? ob : od;
+            G.round_bpr[mv] = std::max(1, std::min((links + lpb - 1) / lpb, (2 * kNumCUs + G.n - 1) / G.n));
+            int occ = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)c->round_fn[mv], kRoundBlock,
+                                                             (size_t)c->lds_doubles_max * sizeof(double)) != hipSuccess)
+                occ = 1;
+            const int room = std::max(1, std::max(1, occ) * kNumCUs - G.n * G.round_bpr[mv]);
+            G.round_ipb[mv] = std::max(1, (n_items + room - 1) / room);
+            if (const char* f = std::getenv("SPIMC_ROUND_IPB")) G.round_ipb[mv] = std::max(1, std::atoi(f));
+            std::vector<int4> grp;
+            int o = 0;
+            for (int r = G.r0; r < G.r0 + G.n; ++r) {
+                const Rep& R = c->reps[r].dev;
+                if (!R.use_long_range) continue;
+                const int n = mv == 0 ? R.n_ritems_bis : R.n_ritems_dis;
+                for (int f = 0; f < n; f += G.round_ipb[mv])
+                    grp.push_back(make_int4(o + f, std::min(G.round_ipb[mv], n - f), r - G.r0, 0));
+                o += n;
+            }
+            G.n_rgrp[mv] = (int)grp.size();
+            if (!grp.empty()) {
+                HIPCHK(hipMalloc((void**)&G.d_rgrp[mv], grp.size() * sizeof(int4)));
+                HIPCHK(hipMemcpy(G.d_rgrp[mv], grp.data(), grp.size() * sizeof(int4), hipMemcpyHostToDevice));
+            }
         }
     }
     if (!G.d_arrive) {
@@ -1174,13 +1179,15 @@ bool pers_eligible(const spimc_ctx* c) {
 // One-launch rounds: the LDS pair kernel, the default rho_k chunking, rounds
 // decided in the next proposal's launch, and unpermuted paths.
 bool one_launch_eligible(const spimc_ctx* c) {
-    return c->round_fn && c->lds_doubles_max > 0 && c->fuse_rounds && !c->perm_on && c->rho_kpt == kRhoKpt;
+    return c->round_fn[0] && c->round_fn[1] && c->lds_doubles_max > 0 && c->fuse_rounds && !c->perm_on &&
+           c->rho_kpt == kRhoKpt;
 }
-// Whether a stage of this move runs one-launch rounds: bisection only, and not
-// while the device clocks time the pair and rho_k kernels (k_round has no
-// separate rho_k kernel to time, so those rounds run as three launches).
+// Whether a stage of this move runs one-launch rounds: bisection, and with
+// one_moves 2 displacement too; not while the device clocks time the pair and
+// rho_k kernels (k_round has no separate rho_k kernel to time, so those rounds
+// run as three launches).
 bool one_launch_now(const spimc_ctx* c, int move) {
-    return c->one_launch && move == 0 && !c->clock && one_launch_eligible(c);
+    return c->one_launch && move < c->one_moves && !c->clock && one_launch_eligible(c);
 }
 
 
@@ -1337,17 +1344,18 @@ int launch_perm_round(spimc_ctx* c, const Group& G) {
 // start.  (Displace rounds stay three launches: their many rho_k items exceed
 // what the GPU holds at once, and the workgroups that decide then delay the
 // rest; measured slower.)
-int launch_one_round(spimc_ctx* c, const Group& G) {
+int launch_one_round(spimc_ctx* c, const Group& G, int move) {
     const int R = G.n;
     const size_t shmem = (size_t)c->lds_doubles_max * sizeof(double);
-    return launch(c, SPIMC_KERNEL_PAIR, c->round_fn, R * G.round_bpr + G.n_rgrp, kRoundBlock, shmem, G.stream,
-                  c->d_reps + G.r0, R, G.round_bpr, (const RhoItem*)G.d_ritem_bis, (const int4*)G.d_rgrp,
-                  G.round_ipb, G.d_arrive);
+    return launch(c, SPIMC_KERNEL_PAIR, c->round_fn[move], R * G.round_bpr[move] + G.n_rgrp[move], kRoundBlock, shmem,
+                  G.stream, c->d_reps + G.r0, R, G.round_bpr[move],
+                  (const RhoItem*)(move == 0 ? G.d_ritem_bis : G.d_ritem_dis), (const int4*)G.d_rgrp[move],
+                  G.round_ipb[move], G.d_arrive);
 }
 
 int launch_round(spimc_ctx* c, const Group& G, int move) {
     if (move == 3) return launch_perm_round(c, G);
-    if (one_launch_now(c, move)) return launch_one_round(c, G);
+    if (one_launch_now(c, move)) return launch_one_round(c, G, move);
     const int R = G.n;
     unsigned long long* kt = c->clock ? G.d_kt : nullptr;
     Rep* reps = c->d_reps + G.r0;  // kernels indexed by block: replicas of this group
@@ -1582,8 +1590,10 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
         for (int g = 0; g < n_groups; ++g) {
             const Group& G = c->groups[g];
             if (!gmx[g]) continue;
-            rc = launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_bisect, G.n, 64, 0, G.stream, c->d_reps + G.r0, G.n,
-                        (unsigned long long*)nullptr);
+            rc = move == 0 ? launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_bisect, G.n, 64, 0, G.stream,
+                                    c->d_reps + G.r0, G.n, (unsigned long long*)nullptr)
+                           : launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_displace, G.n, 64, 0, G.stream,
+                                    c->d_reps + G.r0, G.n, (unsigned long long*)nullptr);
             if (rc) return rc;
         }
     // rounds issued group by group in turn, so the streams advance together
@@ -1920,16 +1930,25 @@ int spimc_ctx_create(const spimc_system_desc* descs, uint32_t n_replicas, uint64
             // one-launch rounds (SPIMC_ONE_LAUNCH or spimc_autotune): the pair type's
             // instantiation with the default rho_k chunking
             using RoundFn = void (*)(const Rep*, int, int, const RhoItem*, const int4*, int, unsigned int*);
-            const RoundFn rf[2][4] = {{k_round<1, 0, kRhoKpt, false>, k_round<2, 2, kRhoKpt, false>,
-                                       k_round<2, 0, kRhoKpt, false>, k_round<0, 0, kRhoKpt, false>},
-                                      {k_round<1, 0, kRhoKpt, true>, k_round<2, 2, kRhoKpt, true>,
-                                       k_round<2, 0, kRhoKpt, true>, k_round<0, 0, kRhoKpt, true>}};
-            c->round_fn = rf[c->rho_nt ? 1 : 0][vi];
-            if (c->lds_doubles_max > 0 &&
-                hipFuncSetAttribute((const void*)c->round_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    c->lds_doubles_max * (int)sizeof(double)) != hipSuccess)
-                c->round_fn = nullptr;
-            if (const char* fo = std::getenv("SPIMC_ONE_LAUNCH")) c->one_launch = std::atoi(fo) != 0;
+            const RoundFn rf[2][2][4] = {
+                {{k_round<1, 0, kRhoKpt, false, 0>, k_round<2, 2, kRhoKpt, false, 0>, k_round<2, 0, kRhoKpt, false, 0>,
+                  k_round<0, 0, kRhoKpt, false, 0>},
+                 {k_round<1, 0, kRhoKpt, true, 0>, k_round<2, 2, kRhoKpt, true, 0>, k_round<2, 0, kRhoKpt, true, 0>,
+                  k_round<0, 0, kRhoKpt, true, 0>}},
+                {{k_round<1, 0, kRhoKpt, false, 1>, k_round<2, 2, kRhoKpt, false, 1>, k_round<2, 0, kRhoKpt, false, 1>,
+                  k_round<0, 0, kRhoKpt, false, 1>},
+                 {k_round<1, 0, kRhoKpt, true, 1>, k_round<2, 2, kRhoKpt, true, 1>, k_round<2, 0, kRhoKpt, true, 1>,
+                  k_round<0, 0, kRhoKpt, true, 1>}}};
+            for (int mv = 0; mv < 2; ++mv) c->round_fn[mv] = rf[mv][c->rho_nt ? 1 : 0][vi];
+            if (c->lds_doubles_max > 0)
+                for (int mv = 0; mv < 2; ++mv)
+                    if (hipFuncSetAttribute((const void*)c->round_fn[mv], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            c->lds_doubles_max * (int)sizeof(double)) != hipSuccess)
+                        c->round_fn[0] = c->round_fn[1] = nullptr;
+            if (const char* fo = std::getenv("SPIMC_ONE_LAUNCH")) {
+                c->one_launch = std::atoi(fo) != 0;
+                c->one_moves = std::atoi(fo) >= 2 ? 2 : 1;
+            }
         }
         // stages of small replicas in one persistent launch: off by default, set by
         // spimc_autotune or SPIMC_PERSISTENT (workgroup size 256, 512 or 1024)
@@ -2006,7 +2025,8 @@ int spimc_ctx_destroy(spimc_ctx* c) {
         for (auto& g : G.graph)
             if (g) (void)hipGraphExecDestroy(g);
         for (void* p : {(void*)G.d_pair_bis, (void*)G.d_pair_dis, (void*)G.d_ritem_bis, (void*)G.d_ritem_dis, (void*)G.d_kt,
-                        (void*)G.d_multi, (void*)G.d_rgrp, (void*)G.d_arrive})
+                        (void*)G.d_multi, (void*)G.d_rgrp[0], (void*)G.d_rgrp[1],
+                        (void*)G.d_arrive})
             if (p) (void)hipFree(p);
         if (G.done) (void)hipEventDestroy(G.done);
         if (G.side) {
@@ -2406,7 +2426,7 @@ int spimc_set_bisect_level(spimc_ctx* c, uint32_t r, uint32_t n_level) {
     for (auto& G : c->groups) {
         if ((int)r < G.r0 || (int)r >= G.r0 + G.n) continue;
         for (void** p : {(void**)&G.d_pair_bis, (void**)&G.d_pair_dis, (void**)&G.d_ritem_bis, (void**)&G.d_ritem_dis,
-                         (void**)&G.d_rgrp})
+                         (void**)&G.d_rgrp[0], (void**)&G.d_rgrp[1]})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;
@@ -2751,8 +2771,9 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
     if (one_launch_eligible(c) && (int)c->reps.size() <= kNumCUs) {
         shapes.push_back({0, 0, c->pair_block, 0, 1});
         if (c->any_long) shapes.push_back({1, 2, 512, 0, 1});
+        shapes.push_back({0, 0, c->pair_block, 0, 2});
     }
-    const Shape saved{c->fork ? 1 : 0, c->rho_bpc, c->pair_block, c->pers_block, c->one_launch ? 1 : 0};
+    const Shape saved{c->fork ? 1 : 0, c->rho_bpc, c->pair_block, c->pers_block, c->one_launch ? c->one_moves : 0};
     // the sweep's mix: N M / 2^l bisection attempts per N displace attempts
     const Rep& R0 = c->reps[0].dev;
     const uint32_t n_dis = std::max<uint32_t>(1, (uint32_t)((uint64_t)n_rounds * (1u << R0.n_level) / R0.M));
@@ -2763,6 +2784,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
         c->fork = sh.fork != 0;
         c->rho_bpc = sh.bpc;
         c->one_launch = sh.one != 0;
+        c->one_moves = sh.one >= 2 ? 2 : 1;
         int rc = c->lds_doubles_max > 0 ? set_pair_block(c, sh.bs) : SPIMC_OK;
         if (rc == SPIMC_OK) rc = set_pers_block(c, sh.pers);
         if (rc) continue;
@@ -2777,6 +2799,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
             c->fork = saved.fork != 0;
             c->rho_bpc = saved.bpc;
             c->one_launch = saved.one != 0;
+            c->one_moves = saved.one >= 2 ? 2 : 1;
             if (c->lds_doubles_max > 0) (void)set_pair_block(c, saved.bs);
             (void)set_pers_block(c, saved.pers);
             return rc;
@@ -2790,6 +2813,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
     c->fork = best.fork != 0;
     c->rho_bpc = best.bpc;
     c->one_launch = best.one != 0;
+    c->one_moves = best.one >= 2 ? 2 : 1;
     if (c->lds_doubles_max > 0) {
         int rc = set_pair_block(c, best.bs);
         if (rc) return rc;
@@ -2803,7 +2827,7 @@ int spimc_autotune(spimc_ctx* c, uint32_t n_rounds, int32_t* chosen) {
         chosen[3] = tried;
         chosen[4] = (int32_t)n_dis;
         chosen[5] = c->pers_block;
-        chosen[6] = c->one_launch ? 1 : 0;
+        chosen[6] = c->one_launch ? c->one_moves : 0;
     }
     return SPIMC_OK;
 }

@@ -1282,7 +1282,7 @@ struct WaveSync {
 // chains are the same bit for bit.  The last workgroup resets the counter for
 // the next round.
 constexpr int kRoundBlock = kRhoBlock;
-template <int PT, int NORD, int KPT, bool NT>
+template <int PT, int NORD, int KPT, bool NT, int MOVE>
 __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ reps, int n_reps, int bpr,
                                                        const RhoItem* __restrict__ items,
                                                        const int4* __restrict__ rgrp, int ipb,
@@ -1328,29 +1328,31 @@ __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ r
     __syncthreads();
     if (threadIdx.x == 0) {
         const Rep& R = reps[r];
-        const unsigned int expect = (unsigned int)bpr + (unsigned int)((R.n_ritems_bis + ipb - 1) / ipb);
+        const int n_items = MOVE == 0 ? R.n_ritems_bis : R.n_ritems_dis;
+        const unsigned int expect = (unsigned int)bpr + (unsigned int)((n_items + ipb - 1) / ipb);
         s_last = atomicAdd(arrive + r, 1u) + 1u == expect;
     }
     __syncthreads();
     if (!s_last) return;
-    // the replica's last workgroup: its first wavefront decides while, on the
-    // device RNG, its second starts the next attempt (the draws depend on the
-    // attempt's sequence number only, not on the decision)
+    // the replica's last workgroup: its first wavefront decides while, for a
+    // bisection on the device RNG, its second starts the next attempt (the draws
+    // depend on the attempt's sequence number only, not on the decision)
     __shared__ BisectLds S;
     __shared__ DecideOut s_out;
     __shared__ int s_moved;
     const Rep& R = reps[r];
-    const bool ahead = R.tape == nullptr;
+    const bool ahead = MOVE == 0 && R.tape == nullptr;
     if (w == 0) {
         if (lane == 0) {
             __hip_atomic_store(arrive + r, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_out.status = 0;
         }
-        // a bisection proposal that reached level 0 holds 2^l pair sums and
-        // 2^l - 1 moved slices of rho_k sums: loaded before the proposal itself
-        const int nb = 1 << R.n_level;
-        const int moved = decide_impl<true>(R, lane, WaveSync{}, R.pair_type ? nb : 0,
-                                            R.use_long_range ? (nb - 1) * R.n_chunks : 0, &s_out);
+        // the pair and rho_k sums of a proposal that reached level 0 (bisection:
+        // 2^l links and 2^l - 1 moved slices; displacement: M of each), loaded
+        // before the proposal itself
+        const int nl = MOVE == 0 ? 1 << R.n_level : R.M, nm = MOVE == 0 ? nl - 1 : R.M;
+        const int moved = decide_impl<true>(R, lane, WaveSync{}, R.pair_type ? nl : 0,
+                                            R.use_long_range ? nm * R.n_chunks : 0, &s_out);
         if (lane == 0) s_moved = moved;
     } else if (w == 1 && ahead) {
         bisect_start_ahead(R, lane, WaveSync{}, S);
@@ -1363,6 +1365,10 @@ __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ r
         return;
     }
     if (w != 0) return;
+    if (MOVE == 1) {
+        propose_displace_impl(R, lane, WaveSync{}, true);
+        return;
+    }
     if (ahead && S.info[0] && S.info[1] == s_moved) {
         // the decision just committed beads of the attempt's particle: its window again
         const int p = S.info[1];

@@ -85,11 +85,11 @@ def compare(got, ref_full, label, absolute=True):
 
 
 RUNS = ([(name, 0, 0) for name in sorted(CASES)] + [("bare_c1", 256, 0), ("david_small", 256, 0), ("ilkka_c1", 512, 0)]
-        + [("bare_c1", 0, 1), ("david_small", 0, 1), ("c5_replica", 0, 1)])
+        + [("bare_c1", 0, 1), ("david_small", 0, 1), ("c5_replica", 0, 1), ("david_small", 0, 2), ("c5_replica", 0, 2)])
 
 
 # persistent: 0 stages as rounds of launches, else the persistent stage kernel's
-# workgroup size; one: bisection rounds of one launch each
+# workgroup size; one: rounds of one launch each (1 bisection, 2 both moves)
 @pytest.mark.parametrize("name,persistent,one", RUNS)
 def test_hot_path_action_terms_match_oracle(name, persistent, one, monkeypatch):
     monkeypatch.setenv("SPIMC_PERSISTENT", str(persistent))

@@ -51,20 +51,23 @@ def rel_err(a, b):
 
 # stages as rounds of launches (one: each round in one launch), or each stage in
 # one persistent launch (workgroup size)
-# (one = 3: one-launch rounds whose rho_k workgroups take 3 items each)
-@pytest.mark.parametrize("graph,persistent,one", [(False, 0, 0), (True, 0, 0), (False, 256, 0), (True, 256, 0),
-                                                  (False, 1024, 0), (False, 0, 1), (True, 0, 1), (False, 0, 3)])
-def test_c4_batch_replays_every_replica(graph, persistent, one, monkeypatch):
+# (one: 1 bisection rounds in one launch, 2 displace rounds too; ipb: rho_k
+# items per workgroup of those launches, 0 as sized for the GPU)
+@pytest.mark.parametrize("graph,persistent,one,ipb", [(False, 0, 0, 0), (True, 0, 0, 0), (False, 256, 0, 0),
+                                                      (True, 256, 0, 0), (False, 1024, 0, 0), (False, 0, 1, 0),
+                                                      (True, 0, 1, 0), (False, 0, 1, 3), (False, 0, 2, 0),
+                                                      (True, 0, 2, 5)])
+def test_c4_batch_replays_every_replica(graph, persistent, one, ipb, monkeypatch):
     monkeypatch.setenv("SPIMC_PERSISTENT", str(persistent))
-    monkeypatch.setenv("SPIMC_ONE_LAUNCH", str(min(one, 1)))
-    if one > 1:
-        monkeypatch.setenv("SPIMC_ROUND_IPB", str(one))
+    monkeypatch.setenv("SPIMC_ONE_LAUNCH", str(one))
+    if ipb:
+        monkeypatch.setenv("SPIMC_ROUND_IPB", str(ipb))
     systems, ref = oracle_chains()
     assert len(systems) == 64
     assert {s.n_part for s in systems} == {16, 32, 64, 128} and {s.n_bead for s in systems} == {16, 32, 64}
     assert all(s.n_k == 257 for s in systems)
-    print(f"C4: {sum(s.n_part * s.n_bead for s in systems)} beads, graph={graph}, persistent={persistent}, one={one}",
-          flush=True)
+    print(f"C4: {sum(s.n_part * s.n_bead for s in systems)} beads, graph={graph}, persistent={persistent}, one={one}, "
+          f"ipb={ipb}", flush=True)
     with abi.Engine(systems, seed=1) as e:
         for i, r in enumerate(ref):
             e.upload(r["pos"], r=i)

@@ -102,11 +102,11 @@ def test_whole_path_estimators(sysname):
             assert abs(a - b) <= RTOL * max(1.0, abs(b)), (w, a, b)
 
 
-@pytest.mark.parametrize("one", ["0", "1"])
+@pytest.mark.parametrize("one", ["0", "1", "2"])
 def test_replay_accept_reject_bit_exact(sysname, one, monkeypatch):
     """Bisection and displace stages replay the oracle's tape bit for bit, with
-    rounds of three launches and with rounds of one launch (SPIMC_ONE_LAUNCH;
-    the global-table replica keeps three)."""
+    rounds of three launches and with rounds of one launch (SPIMC_ONE_LAUNCH 1:
+    bisection, 2: both moves; the global-table replica keeps three)."""
     monkeypatch.setenv("SPIMC_ONE_LAUNCH", one)
     s, pos = make(sysname, spread=0.05)
     o = Oracle(s, seed=11)
@@ -648,12 +648,12 @@ def _edge(name):
 
 @pytest.mark.parametrize("name", ["one_particle", "two_slices", "window_is_path", "no_long_range", "images",
                                   "three_k_vectors"])
-@pytest.mark.parametrize("one", ["0", "1"])
+@pytest.mark.parametrize("one", ["0", "1", "2"])
 def test_edge_configurations(name, one, monkeypatch):
     """Replay, estimators and gradients on edge inputs: one particle (no pairs),
     two slices, a bisection window as long as the path, no long range, image
     sums in the kinetic action and the sampling, and a three-vector k-set;
-    bisection rounds of three launches or of one (SPIMC_ONE_LAUNCH)."""
+    rounds of three launches or of one (SPIMC_ONE_LAUNCH)."""
     monkeypatch.setenv("SPIMC_ONE_LAUNCH", one)
     s = _edge(name)
     if name == "three_k_vectors":
