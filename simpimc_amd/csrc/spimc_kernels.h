@@ -1263,8 +1263,9 @@ struct WaveSync {
     }
 };
 
-// A whole bisection round in one launch, for batches too small to fill the
-// GPU (a round is then bound by latency, not by throughput).  Workgroups
+// A whole round (MOVE 0 bisection, 1 displacement) in one launch, for batches
+// too small to fill the GPU (a round is then bound by latency, not by
+// throughput).  Workgroups
 // [0, n_reps * bpr) (after the XCD remap) run the window links of the LDS pair
 // kernel, the others a group of up to ipb rho_k items of one replica each
 // (rgrp: first item, items, replica; ipb > 1 only where one item per
