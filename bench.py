@@ -480,8 +480,14 @@ def c5_bench(a, abi, system, copy_gbps, steps=None, rounds=0):
     value = (eng.bead_updates() - u0) / dt
     if not rounds:
         # the rho_k kernel's own launches, timed on the device clocks over bisect
-        # rounds after the sweeps (with the clocks on, bisection rounds run as
-        # three launches, so every rho_k update is one timed k_rho_loop item)
+        # rounds on a second context of the same replicas with the default launch
+        # shape (rho_k after the pair kernel, a workgroup per item: the shape of the
+        # counter passes); with the clocks on, bisection rounds run as three
+        # launches, so every rho_k update is one timed k_rho_loop item
+        eng.close()
+        eng = abi.Engine([base] * R, seed=2030)
+        for r in range(R):
+            eng.upload(system.random_path(base, seed=5000 + r, spread=0.05), r=r)
         eng.timing(abi.TIMING_CLOCK)
         eng.bisect(C5_ROOFLINE_ROUNDS)
         eng.timing_reset()
@@ -506,7 +512,9 @@ def c5_bench(a, abi, system, copy_gbps, steps=None, rounds=0):
                       "bytes_per_launch": rho_bytes / rho_n, "avg_launch_us": 1e3 * rho_ms / rho_n,
                       "duration_from": "device wall clock, first block start to last block end, over "
                                        + (f"the {rounds} timed bisect rounds" if rounds else
-                                          f"{C5_ROOFLINE_ROUNDS} bisect rounds after the timed sweeps"),
+                                          f"{C5_ROOFLINE_ROUNDS} bisect rounds of the same replicas with the "
+                                          "default launch shape (rho_k after the pair kernel, a workgroup per "
+                                          "item), after the timed sweeps"),
                       # FETCH_SIZE / WRITE_SIZE passes over bisect rounds of this case, scaled by a copy kernel
                       "traffic_over_algorithmic": C5_TRAFFIC_RATIO,
                       "traffic_from": "profiles/k_rho_loop_heg1024_m256_8rep_counters.md",
