@@ -117,12 +117,13 @@ __device__ __forceinline__ void moved_row(const Rep& R, int m, int s) { R.rho_ro
 // valid when the proposal goes to level 0 and moves that bead (bead m sits at
 // slice first + m).  Idle items (no attempt left, rejected at a coarse level,
 // fewer moved beads) are marked invalid.
-__device__ __forceinline__ void rho_items_write(const Rep& R, int move, int status, int count, int first, int lane) {
+__device__ __forceinline__ void rho_items_write(const Rep& R, int move, int status, int count, int first, int lane,
+                                                int nthr = kWave) {
     RhoItem* items = move == 0 ? R.ritems_bis : R.ritems_dis;
     const int n = move == 0 ? R.n_ritems_bis : R.n_ritems_dis;
     if (!items) return;
     const int stride = R.max_idx + 1, nch = R.n_chunks, ck = kRhoBlock * R.rho_kpt;
-    for (int t = lane; t < n; t += kWave) {
+    for (int t = lane; t < n; t += nthr) {
         const int m = t / nch, ch = t % nch;
         RhoItem d{};
         if (status == 1 && m < count) {
@@ -192,6 +193,15 @@ __global__ __launch_bounds__(256) void k_rho_build(const Rep* __restrict__ reps,
 // workgroup of the round kernels).
 struct BlockSync {
     __device__ void operator()() const { __syncthreads(); }
+};
+// Ordering point of the one-wavefront bodies when they run in one wavefront of
+// a larger workgroup: the wavefront's memory operations complete, and no code
+// moves across it.
+struct WaveSync {
+    __device__ void operator()() const {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
 };
 
 // LDS of one bisection proposal (one wavefront): the window's beads, OLD and
@@ -461,8 +471,12 @@ __device__ __forceinline__ void propose_bisect_body(const Rep* __restrict__ reps
 // the wrap link (p, M-1) -> (perm[p], 0), the one link of the walk a rigid shift
 // changes.  The link into (p, 0) from (iperm[p], M-1), which the shift also
 // changes, is outside the walk and not counted, as in the reference.
+// lane: the thread's index among the nthr that run the proposal (its draws and
+// record on thread 0; the moved slices' positions, phase chains and rho_k
+// items over all nthr).
 template <class Sync>
-__device__ __forceinline__ void propose_displace_impl(const Rep& R, int lane, Sync sync, bool items) {
+__device__ __forceinline__ void propose_displace_impl(const Rep& R, int lane, Sync sync, bool items,
+                                                      int nthr = kWave) {
     __shared__ int s_act, s_p, s_status;
     __shared__ double s_dr[3];
     Prop* pr = R.prop;
@@ -529,14 +543,14 @@ __device__ __forceinline__ void propose_displace_impl(const Rep& R, int lane, Sy
     }
     sync();
     if (!s_act) {
-        if (items) rho_items_write(R, 1, 0, 0, 0, lane);
+        if (items) rho_items_write(R, 1, 0, 0, 0, lane, nthr);
         return;
     }
     const int p = s_p;
-    for (int s = lane; s < R.M; s += kWave)
+    for (int s = lane; s < R.M; s += nthr)
         for (int d = 0; d < 3; ++d) R.prop_new[3 * s + d] = pos_at(R, d, s, p) + s_dr[d];
     if (R.use_long_range)
-        for (int t = lane; t < 6 * R.M; t += kWave) {
+        for (int t = lane; t < 6 * R.M; t += nthr) {
             const int s = t / 6;
             double xo[3], xn[3];
             for (int d = 0; d < 3; ++d) {
@@ -545,7 +559,7 @@ __device__ __forceinline__ void propose_displace_impl(const Rep& R, int lane, Sy
             }
             moved_phase_chain(R, s, t % 6, xo, xn);
         }
-    if (R.use_long_range) if (items) rho_items_write(R, 1, s_status, R.M, 0, lane);
+    if (R.use_long_range) if (items) rho_items_write(R, 1, s_status, R.M, 0, lane, nthr);
 }
 
 __device__ __forceinline__ void propose_displace_body(const Rep* __restrict__ reps, int n_reps) {
@@ -1246,22 +1260,22 @@ __global__ __launch_bounds__(64) void k_decide_propose_bisect(const Rep* __restr
     __syncthreads();
     propose_bisect_body(reps, n_reps);
 }
-__global__ __launch_bounds__(64) void k_decide_propose_displace(const Rep* __restrict__ reps, int n_reps,
-                                                                unsigned long long* __restrict__ kt) {
-    decide_body(reps, n_reps, kt);
+// Displacement: the decision on the first wavefront, then the proposal, whose
+// M moved slices (positions, phase chains, rho_k items) take all kDisBlock
+// threads.
+constexpr int kDisBlock = 256;
+__global__ __launch_bounds__(kDisBlock) void k_decide_propose_displace(const Rep* __restrict__ reps, int n_reps,
+                                                                       unsigned long long* __restrict__ kt) {
+    const int r = blockIdx.x;
+    if (threadIdx.x < kWave) {
+        if (kt && blockIdx.x == 0) kt_close(kt);
+        if (r < n_reps) decide_impl(reps[r], (int)threadIdx.x, WaveSync{});
+    }
     __syncthreads();
-    propose_displace_body(reps, n_reps);
+    if (r >= n_reps) return;
+    propose_displace_impl(reps[r], (int)threadIdx.x, BlockSync{}, true, kDisBlock);
 }
 
-// Ordering point of the one-wavefront bodies when they run in the first
-// wavefront of a larger workgroup whose other wavefronts have left: the
-// wavefront's memory operations complete, and no code moves across it.
-struct WaveSync {
-    __device__ void operator()() const {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-    }
-};
 
 // A whole round (MOVE 0 bisection, 1 displacement) in one launch, for batches
 // too small to fill the GPU (a round is then bound by latency, not by
