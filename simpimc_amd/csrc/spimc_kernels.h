@@ -1314,11 +1314,23 @@ __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ r
         r = idx / bpr;
         const int part = idx % bpr;
         const Rep R = reps[r];
+        // the tables' first kRoundBlock * 4 values are loaded while the proposal is
+        const int na = R.bt_a.ncoef * R.bt_a.ncomp, nd = R.lds_doubles;
+        double tv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int t = threadIdx.x + k * kRoundBlock;
+            tv[k] = t < nd ? (t < na ? R.bt_a.c[t] : R.bt_b.c[t - na]) : 0.;
+        }
         const Prop pr = *R.prop;
         constexpr int W = kRoundBlock / kWave;
         if (pr.status == 1 && !pr.generic && part * (W / pair_split<W>(R.N)) < pr.n_links) {
-            const int na = R.bt_a.ncoef * R.bt_a.ncomp;
-            for (int t = threadIdx.x; t < R.lds_doubles; t += kRoundBlock)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int t = threadIdx.x + k * kRoundBlock;
+                if (t < nd) lds[t] = tv[k];
+            }
+            for (int t = threadIdx.x + 4 * kRoundBlock; t < nd; t += kRoundBlock)
                 lds[t] = t < na ? R.bt_a.c[t] : R.bt_b.c[t - na];
             __syncthreads();
             pair_links_split<PT, NORD, true, W>(R, lds, pr, part, bpr, w, lane, wpart);
