@@ -689,11 +689,14 @@ def test_edge_configurations(name, one, monkeypatch):
         assert rel_err(e.gradient(abi.GRAD_ALL), o.gradient(abi.GRAD_ALL, h=1e-5)) < 1e-6
 
 
-@pytest.mark.parametrize("images", [0, 1])
-def test_bisect_level_changes(images):
+@pytest.mark.parametrize("images,one", [(0, "0"), (1, "0"), (0, "1")])
+def test_bisect_level_changes(images, one, monkeypatch):
     """Adaptive bisection (bisect_class.h:143-151): stages at levels 3, 2, 4, 1
     and 3 of two replicas, each replaying its oracle's tape, with the sampling
-    image splines of every level; the sweep's attempt count follows the level."""
+    image splines of every level; the sweep's attempt count follows the level.
+    With one-launch rounds (one = "1") every level change rebuilds their
+    workgroup tables."""
+    monkeypatch.setenv("SPIMC_ONE_LAUNCH", one)
     systems = [system.config_c1(n_part=16, n_bead=32), system.config_c1(n_part=8, n_bead=16)]
     for s in systems:
         s.bisect_n_images = images
