@@ -1330,8 +1330,21 @@ __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ r
                 const int t = threadIdx.x + k * kRoundBlock;
                 if (t < nd) lds[t] = tv[k];
             }
-            for (int t = threadIdx.x + 4 * kRoundBlock; t < nd; t += kRoundBlock)
-                lds[t] = t < na ? R.bt_a.c[t] : R.bt_b.c[t - na];
+            // the rest (large tables: 96 KB for David's) 16 values per thread at a
+            // time, all loads in flight before their stores
+            for (int t0 = threadIdx.x + 4 * kRoundBlock; t0 < nd; t0 += 16 * kRoundBlock) {
+                double v[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const int t = t0 + k * kRoundBlock;
+                    v[k] = t < nd ? (t < na ? R.bt_a.c[t] : R.bt_b.c[t - na]) : 0.;
+                }
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const int t = t0 + k * kRoundBlock;
+                    if (t < nd) lds[t] = v[k];
+                }
+            }
             __syncthreads();
             pair_links_split<PT, NORD, true, W>(R, lds, pr, part, bpr, w, lane, wpart);
         }
