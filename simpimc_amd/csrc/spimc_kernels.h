@@ -788,6 +788,27 @@ __device__ __forceinline__ void pair_links_split(const Rep& R, const double* lds
     }
 }
 
+// Copies values [t_lo, n) of the tables a | b (a holds the first na) into lds
+// with nthr threads, 16 values per thread at a time, every load issued before
+// the stores that wait on it (a large table then costs a few load latencies
+// instead of one per value).
+__device__ __forceinline__ void stage_batched(double* lds, const double* a, const double* b, int na, int n, int t_lo,
+                                              int nthr) {
+    for (int t0 = t_lo + (int)threadIdx.x; t0 < n; t0 += 16 * nthr) {
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int t = t0 + k * nthr;
+            v[k] = t < n ? (t < na ? a[t] : b[t - na]) : 0.;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int t = t0 + k * nthr;
+            if (t < n) lds[t] = v[k];
+        }
+    }
+}
+
 template <int PT, int NORD, int BS>
 __device__ __forceinline__ void pair_lds_block(const Rep* __restrict__ reps, int n_reps, int bpr) {
     extern __shared__ double lds[];
@@ -798,7 +819,7 @@ __device__ __forceinline__ void pair_lds_block(const Rep* __restrict__ reps, int
     const Rep R = reps[r];  // by value: the fields used stay in registers across the loops
     const Prop pr = *R.prop;
     if (pr.status != 1 || pr.generic) return;
-    for (int t = threadIdx.x; t < R.lds_doubles; t += BS) lds[t] = (t < R.bt_a.ncoef * R.bt_a.ncomp) ? R.bt_a.c[t] : R.bt_b.c[t - R.bt_a.ncoef * R.bt_a.ncomp];
+    stage_batched(lds, R.bt_a.c, R.bt_b.c, R.bt_a.ncoef * R.bt_a.ncomp, R.lds_doubles, 0, BS);
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     if (part * (BS / kWave) >= pr.n_links) return;
@@ -1330,21 +1351,8 @@ __global__ __launch_bounds__(kRoundBlock) void k_round(const Rep* __restrict__ r
                 const int t = threadIdx.x + k * kRoundBlock;
                 if (t < nd) lds[t] = tv[k];
             }
-            // the rest (large tables: 96 KB for David's) 16 values per thread at a
-            // time, all loads in flight before their stores
-            for (int t0 = threadIdx.x + 4 * kRoundBlock; t0 < nd; t0 += 16 * kRoundBlock) {
-                double v[16];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const int t = t0 + k * kRoundBlock;
-                    v[k] = t < nd ? (t < na ? R.bt_a.c[t] : R.bt_b.c[t - na]) : 0.;
-                }
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const int t = t0 + k * kRoundBlock;
-                    if (t < nd) lds[t] = v[k];
-                }
-            }
+            // the rest (large tables: 96 KB for David's)
+            stage_batched(lds, R.bt_a.c, R.bt_b.c, na, nd, 4 * kRoundBlock, kRoundBlock);
             __syncthreads();
             pair_links_split<PT, NORD, true, W>(R, lds, pr, part, bpr, w, lane, wpart);
         }
@@ -1874,12 +1882,10 @@ __global__ __launch_bounds__(64) void k_adjoint_local(const Rep* __restrict__ re
 // Loads a replica's link-action tables (du_only: the David du_kj_dbeta table
 // instead) into LDS.
 __device__ __forceinline__ void stage_tables(const Rep& R, double* lds, bool du_only) {
-    if (du_only) {
-        for (int t = threadIdx.x; t < R.lds_doubles_du; t += blockDim.x) lds[t] = R.bt_du.c[t];
-    } else {
-        const int na = R.bt_a.ncoef * R.bt_a.ncomp;
-        for (int t = threadIdx.x; t < R.lds_doubles; t += blockDim.x) lds[t] = (t < na) ? R.bt_a.c[t] : R.bt_b.c[t - na];
-    }
+    if (du_only)
+        stage_batched(lds, R.bt_du.c, R.bt_du.c, R.lds_doubles_du, R.lds_doubles_du, 0, (int)blockDim.x);
+    else
+        stage_batched(lds, R.bt_a.c, R.bt_b.c, R.bt_a.ncoef * R.bt_a.ncomp, R.lds_doubles, 0, (int)blockDim.x);
     __syncthreads();
 }
 
