@@ -1046,7 +1046,10 @@ int build_work_lists(spimc_ctx* c, Group& G) {
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)c->round_fn[mv], kRoundBlock,
                                                              (size_t)c->lds_doubles_max * sizeof(double)) != hipSuccess)
                 occ = 1;
-            const int room = std::max(1, std::max(1, occ) * kNumCUs - G.n * G.round_bpr[mv]);
+            // at least half of what the GPU holds goes to rho_k workgroups, also
+            // when the pair workgroups alone would fill it
+            const int cap = std::max(1, occ) * kNumCUs;
+            const int room = std::max(cap / 2, cap - G.n * G.round_bpr[mv]);
             G.round_ipb[mv] = std::max(1, (n_items + room - 1) / room);
             if (const char* f = std::getenv("SPIMC_ROUND_IPB")) G.round_ipb[mv] = std::max(1, std::atoi(f));
             std::vector<int4> grp;
