@@ -1367,7 +1367,7 @@ int launch_round(spimc_ctx* c, const Group& G, int move) {
     int rc;
     if (c->fuse_rounds) {
         // the previous round's decision and this round's proposal in one launch
-        rc = move == 0 ? launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_bisect, R, 64, 0, st, reps, R, kt)
+        rc = move == 0 ? launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_bisect, R, kBisBlock, 0, st, reps, R, kt)
                        : launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_displace, R, kDisBlock, 0, st, reps, R, kt);
     } else if (move == 0) {
         rc = launch(c, SPIMC_KERNEL_PROPOSE, k_propose_bisect, R, 64, 0, st, reps, R);
@@ -1593,7 +1593,7 @@ int run_stage(spimc_ctx* c, int move, const uint32_t* counts, uint32_t n_uniform
         for (int g = 0; g < n_groups; ++g) {
             const Group& G = c->groups[g];
             if (!gmx[g]) continue;
-            rc = move == 0 ? launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_bisect, G.n, 64, 0, G.stream,
+            rc = move == 0 ? launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_bisect, G.n, kBisBlock, 0, G.stream,
                                     c->d_reps + G.r0, G.n, (unsigned long long*)nullptr)
                            : launch(c, SPIMC_KERNEL_PROPOSE, k_decide_propose_displace, G.n, kDisBlock, 0, G.stream,
                                     c->d_reps + G.r0, G.n, (unsigned long long*)nullptr);

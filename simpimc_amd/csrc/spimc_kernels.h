@@ -1274,12 +1274,45 @@ __global__ __launch_bounds__(64) void k_decide(const Rep* __restrict__ reps, int
 // K5 of one round and K4 of the next in one launch: a replica's pending
 // proposal is decided and committed (nothing is pending at the first round of
 // a stage), then its next proposal is made.  Saves one dependent launch per
-// round; the stage's last decision is a k_decide of its own.
-__global__ __launch_bounds__(64) void k_decide_propose_bisect(const Rep* __restrict__ reps, int n_reps,
-                                                              unsigned long long* __restrict__ kt) {
-    decide_body(reps, n_reps, kt);
+// round; the stage's last decision is a k_decide of its own.  Two wavefronts
+// per replica: the first decides while, on the device RNG of an unpermuted
+// path, the second starts the next attempt (counter step, draws, window) and
+// afterwards does the decision's bookkeeping, as in k_round.
+constexpr int kBisBlock = 2 * kWave;
+__global__ __launch_bounds__(kBisBlock) void k_decide_propose_bisect(const Rep* __restrict__ reps, int n_reps,
+                                                                     unsigned long long* __restrict__ kt) {
+    __shared__ BisectLds S;
+    __shared__ DecideOut s_out;
+    __shared__ int s_moved;
+    const int r = blockIdx.x, w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    if (kt && blockIdx.x == 0 && w == 0) kt_close(kt);
+    if (r >= n_reps) return;
+    const Rep& R = reps[r];
+    // a window that crosses the wrap of a permuted path holds a second label,
+    // which the window check after the decision does not follow
+    const bool ahead = R.tape == nullptr && R.perm == nullptr;
+    if (w == 0) {
+        if (lane == 0) s_out.status = 0;
+        const int nb = 1 << R.n_level;
+        const int moved = decide_impl(R, lane, WaveSync{}, R.pair_type ? nb : 0,
+                                      R.use_long_range ? (nb - 1) * R.n_chunks : 0, &s_out);
+        if (lane == 0) s_moved = moved;
+    } else if (ahead) {
+        bisect_start_ahead(R, lane, WaveSync{}, S);
+    }
     __syncthreads();
-    propose_bisect_body(reps, n_reps);
+    if (w == 1) {
+        if (lane == 0 && s_out.status != 0)
+            decide_book(R, s_out.pr, s_out.acc, s_out.status, s_out.po, s_out.pn, s_out.lo, s_out.ln);
+        return;
+    }
+    if (ahead && S.info[0] && S.info[1] == s_moved) {
+        // the decision just committed beads of the attempt's particle: its window again
+        const int p = S.info[1];
+        bisect_window(R, lane, S, p, p, S.info[2], false);
+        WaveSync{}();
+    }
+    propose_bisect_impl(R, lane, WaveSync{}, true, S, ahead);
 }
 // Displacement: the decision on the first wavefront, then the proposal, whose
 // M moved slices (positions, phase chains, rho_k items) take all kDisBlock
